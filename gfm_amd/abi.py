@@ -1,0 +1,89 @@
+"""ctypes mirror of the C-ABI in include/gfm_hip.h.
+
+The structures here must match the header field for field; the test
+``tests/test_abi.py`` checks their sizes against ``sizeof`` reported by the
+compiled libraries.
+"""
+import ctypes as ct
+
+import numpy as np
+
+D = ct.c_double
+I32 = ct.c_int32
+
+
+class GfmCtrl(ct.Structure):
+    _fields_ = [
+        ("mp", I32), ("np", I32), ("lp", I32), ("nr0", I32),
+        ("ncells", I32), ("i_ref", I32), ("j_ref", I32), ("k_ref", I32),
+        ("sweep_order", I32), ("precision", I32), ("want_pre_residuals", I32), ("want_irc", I32),
+        ("i1st_rad_done", I32), ("isothermal_walls", I32), ("gray_radiation", I32), ("pad0", I32),
+        ("itr_gas", I32), ("itr_gend", I32), ("maxgi", I32), ("pad1", I32),
+        ("bgcon", D), ("factor", D), ("fadj", D),
+        ("t0", D), ("trdg", D), ("h_0", D), ("r0", D), ("area0", D), ("vol0", D),
+        ("reyg", D), ("reyg2", D), ("eun", D), ("epsr", D), ("taur", D), ("ug0", D), ("dnst0", D),
+        ("qr0", D), ("q0", D), ("gmfr", D), ("tk0", D), ("epsi0", D),
+        ("gra", D * 3),
+        ("p_ref", D),
+        ("t_mx", D), ("t_mn", D), ("u_mx", D), ("u_mn", D), ("p_mx", D), ("p_mn", D), ("f_mx", D), ("f_mn", D),
+        ("rf", D * 21), ("rfl", D * 21), ("rf_hsource", D),
+        ("ntimesg", I32 * 20),
+        ("gfin0", D),
+        ("t_wall", D), ("h_g", D), ("kappa_gray", D), ("sigma_sb", D),
+        ("res_mass", D), ("res_mass_max", D),
+        ("imx", I32), ("jmx", I32), ("kmx", I32), ("pad2", I32),
+        ("p_resid_pre", D), ("p_resid", D), ("pcorr_resid", D), ("h_resid_pre", D), ("h_resid", D),
+        ("u_resid_pre", D * 3), ("u_resid", D * 3),
+        ("gf_resid_pre", D * 10), ("gf_resid", D * 10),
+        ("resg", D * 40),
+        ("gfex", D), ("gfin", D), ("fuel_out", D), ("tmean_ex", D), ("q_react", D), ("q_exhaust", D), ("q_wall", D),
+        ("tave", D), ("tave_vol", D), ("tmax_c", D),
+        ("t_av_exit", D), ("t_x_exit", D),
+    ]
+
+    def copy(self):
+        c = GfmCtrl()
+        ct.memmove(ct.byref(c), ct.byref(self), ct.sizeof(GfmCtrl))
+        return c
+
+
+DP = ct.POINTER(ct.c_double)
+
+
+class GfmHostArrays(ct.Structure):
+    _fields_ = [
+        ("x", DP), ("dx", DP), ("r", DP), ("dr", DP), ("rs", DP), ("z", DP), ("dz", DP),
+        ("ibcell", ct.POINTER(ct.c_int16)),
+        ("p", DP), ("t", DP), ("dnst", DP), ("theta", DP), ("tmu", DP), ("gcp", DP), ("glam", DP),
+        ("gama", DP), ("gte", DP), ("crate", DP), ("hrate", DP), ("bs_hprev", DP),
+        ("ug", DP), ("gf", DP),
+        ("rtim", DP), ("delrt", DP), ("hrt", DP), ("delr", DP), ("timed", DP), ("hrd", DP),
+    ]
+
+
+FIELD3 = ("p", "t", "dnst", "theta", "tmu", "gcp", "glam", "gama", "gte", "crate", "hrate", "bs_hprev")
+GRID1 = ("x", "dx", "r", "dr", "rs", "z", "dz")
+KIN = ("rtim", "delrt", "hrt", "delr", "timed", "hrd")
+
+
+def _ptr(a, ctype=ct.c_double):
+    assert a.flags["F_CONTIGUOUS"] or a.ndim == 1, "arrays must be Fortran-ordered"
+    return a.ctypes.data_as(ct.POINTER(ctype))
+
+
+def host_arrays(arrs):
+    """Build a GfmHostArrays pointing at the numpy arrays in ``arrs``.
+
+    The caller must keep ``arrs`` alive while the structure is used."""
+    h = GfmHostArrays()
+    for k in GRID1 + FIELD3 + KIN + ("ug", "gf"):
+        a = arrs[k]
+        assert a.dtype == np.float64, k
+        setattr(h, k, _ptr(a))
+    assert arrs["ibcell"].dtype == np.int16
+    h.ibcell = _ptr(arrs["ibcell"], ct.c_int16)
+    return h
+
+
+def copy_arrays(arrs):
+    return {k: (np.array(v, order="F", copy=True) if isinstance(v, np.ndarray) else v) for k, v in arrs.items()}

@@ -1,0 +1,178 @@
+/*
+ * gfm_hip.h -- C-ABI of the MI355X combustion-space SIMPLER solver.
+ *
+ * The reference solver (GFM v4, combustion/Code) keeps all state in the
+ * Fortran module GBL_VAR and runs the combustion-space iteration through
+ * argument-less subroutines:
+ *
+ *   SUBROUTINE SPHASE                       C/Sphase.f90:10   (called C/A0main.f90:204)
+ *   SUBROUTINE GFLOW                        C/Gflow.f90:6     (called C/Sphase.f90:51)
+ *   SUBROUTINE ADLBL3(XT,IB,IE,JB,JE,KB,KE) C/Adlbl3.f90:10   (called C/Gslv3v.f90:93,133,163, C/Gslv3g.f90:96)
+ *   subroutine calc_residual(xt,resid,...)  C/Adlbl3.f90:239
+ *
+ * (C/ = combustion/Code/ of the reference package.)
+ *
+ * This header is the drop-in boundary for those subroutines.  The host
+ * (Fortran through ISO_C_BINDING, or any C-callable language) owns every
+ * array in the reference layout: 1-based, column-major, i fastest, on the
+ * doubled-index lattice (MP,NP,LP) with MP = 2*nx+4 (cell centres at even
+ * indices, faces at odd indices).  The library copies them into
+ * device-resident compact arrays at gfm_hip_init / gfm_hip_upload and back
+ * at gfm_hip_download.  All pointers are plain host pointers; no
+ * language-runtime types cross the boundary.
+ *
+ * Error convention: every entry point returns 0 on success and a non-zero
+ * GFM_E_* code on failure; gfm_hip_last_error() returns a message.  The
+ * reference has no return codes (fatal paths call stop_run, C/A0main.f90:415);
+ * the Fortran wrapper maps a non-zero return to that call.
+ */
+#ifndef GFM_HIP_H
+#define GFM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GFM_ABI_VERSION 1
+
+enum {
+  GFM_OK = 0,
+  GFM_E_ARG = 1,      /* bad argument / inconsistent dimensions            */
+  GFM_E_DEVICE = 2,   /* HIP runtime error (allocation, launch, copy)      */
+  GFM_E_STATE = 3,    /* call out of order (e.g. sphase before init)       */
+  GFM_E_NUMERIC = 4   /* non-finite value detected in a reduction          */
+};
+
+/* Line-relaxation plane order used by ADLBL3.
+ *   GFM_ORDER_REFERENCE: x-planes visited forward then backward each sweep,
+ *     each plane using the latest x-neighbours (C/Adlbl3.f90:39-51).
+ *   GFM_ORDER_ZEBRA: each sweep updates even-numbered planes, then odd,
+ *     then even, then odd (every plane still visited twice per sweep) with
+ *     the reference's per-plane operator ADLZ unchanged.  Planes of one
+ *     colour are independent, so they are solved concurrently. */
+enum { GFM_ORDER_REFERENCE = 0, GFM_ORDER_ZEBRA = 1 };
+
+/* Arithmetic of the assembly and line sweeps.  The p' defect, the global
+ * balances/reductions and the enthalpy->temperature inversion are always
+ * FP64. */
+enum { GFM_PREC_FP64 = 0, GFM_PREC_MIXED = 1 };
+
+/* Host arrays of one case, reference layout (see top of file).  Arrays of
+ * rank 3 have MP*NP*LP entries; UG holds 3 and GF holds 9 consecutive
+ * such blocks (GF(:,:,:,1..9) = fuel, O2, h, k, eps, CO2, N2, -, H2O;
+ * C/Var_mod.f90:116-125). */
+typedef struct gfm_host_arrays {
+  double *x, *dx;        /* (MP)  C/Grid3.f90:64  */
+  double *r, *dr, *rs;   /* (NP)                  */
+  double *z, *dz;        /* (LP)                  */
+  int16_t *ibcell;       /* (MP,NP,LP) cell codes after 26-neighbour propagation, C/Grid3.f90:259-275 */
+  double *p, *t, *dnst, *theta, *tmu, *gcp, *glam, *gama, *gte, *crate, *hrate;
+  double *bs_hprev;      /* enthalpy-source history, C/Source.f90:511-520 */
+  double *ug;            /* (MP,NP,LP,3) */
+  double *gf;            /* (MP,NP,LP,9) */
+  double *rtim, *delrt, *hrt, *delr, *timed, *hrd;   /* (NR0) kinetic tables, C/Setup.f90:858-888 */
+} gfm_host_arrays;
+
+/* Scalars of GBL_VAR read or written by the path (C/Var_mod.f90).  All
+ * nondimensional as after SETUP (C/Setup.f90:542-731). */
+typedef struct gfm_ctrl {
+  /* dimensions */
+  int32_t mp, np, lp, nr0;
+  int32_t ncells;                 /* open cells, C/Grid3.f90:259-267 */
+  int32_t i_ref, j_ref, k_ref;    /* pressure reference (inlet) cell */
+  /* options */
+  int32_t sweep_order;            /* GFM_ORDER_* */
+  int32_t precision;              /* GFM_PREC_* */
+  int32_t want_pre_residuals;     /* compute calc_residual before each solve (log only) */
+  int32_t want_irc;               /* IRC residual block of ADLBL3 (RESG) */
+  int32_t i1st_rad_done;          /* 0: adiabatic walls, C/Extr.f90:194 */
+  int32_t isothermal_walls;       /* 1: wall heat sink of C/Source.f90:473-509 with fixed wall T */
+  int32_t gray_radiation;         /* 1: gray emission/absorption source term */
+  int32_t pad0;
+  /* iteration state */
+  int32_t itr_gas, itr_gend, maxgi, pad1;
+  double bgcon;
+  double factor, fadj;
+  /* reference scales and nondimensional groups */
+  double t0, trdg, h_0, r0, area0, vol0;
+  double reyg, reyg2, eun, epsr, taur, ug0, dnst0, qr0, q0, gmfr, tk0, epsi0;
+  double gra[3];
+  double p_ref;
+  double t_mx, t_mn, u_mx, u_mn, p_mx, p_mn, f_mx, f_mn;
+  double rf[21], rfl[21], rf_hsource;
+  int32_t ntimesg[20];
+  double gfin0;                 /* inlet mass flow / GMFR, C/Sbc.f90:375, C/Extr.f90:62 */
+  /* wall and radiation model (reference extension; used when enabled) */
+  double t_wall, h_g, kappa_gray, sigma_sb;
+  /* outputs of the last iteration */
+  double res_mass, res_mass_max;
+  int32_t imx, jmx, kmx, pad2;
+  double p_resid_pre, p_resid, pcorr_resid, h_resid_pre, h_resid;
+  double u_resid_pre[3], u_resid[3];
+  double gf_resid_pre[10], gf_resid[10];
+  double resg[40];                /* RESG(20,2) column-major */
+  double gfex, gfin, fuel_out, tmean_ex, q_react, q_exhaust, q_wall;
+  double tave, tave_vol, tmax_c;
+  double t_av_exit, t_x_exit;     /* C/Sphase.f90:74-89 */
+} gfm_ctrl;
+
+/* Opaque device-side context. */
+typedef struct gfm_ctx gfm_ctx;
+
+/* Create a device context for one case on `device` and upload the host
+ * state.  Replaces nothing in the reference: it is the allocation that
+ * allocate_large_gas_phase_arrays (C/Setup.f90:740) does for the host. */
+int gfm_hip_init(const gfm_ctrl *ctrl, const gfm_host_arrays *h, int device, gfm_ctx **out);
+
+/* Copy host state to the device (after a host-side change such as a
+ * restart read, C/Initsv.f90:225) / device state back to the host (before
+ * any host consumer: sav3f C/A0main.f90:210, sprint :257). */
+int gfm_hip_upload(gfm_ctx *ctx, const gfm_ctrl *ctrl, const gfm_host_arrays *h);
+int gfm_hip_download(gfm_ctx *ctx, gfm_ctrl *ctrl, gfm_host_arrays *h);
+
+/* The tail of INITSV (C/Initsv.f90:63-84): INTPV, EXTR(1), INTP on the
+ * device state. */
+int gfm_hip_initsv_tail(gfm_ctx *ctx, gfm_ctrl *ctrl);
+
+/* One GFLOW call (C/Gflow.f90:6-408): one SIMPLER outer iteration.  Does
+ * not advance itr_gas; ctrl carries itr_gas in and the iteration's
+ * residuals and balances out. */
+int gfm_hip_gflow(gfm_ctx *ctx, gfm_ctrl *ctrl);
+
+/* SPHASE (C/Sphase.f90:37-235): iterate GFLOW while itr_gas < itr_gend and
+ * itr_gas < maxgi, stopping after `max_iters` iterations (the host's
+ * polling cadence, C/Sphase.f90:106-162) or when res_mass < bgcon
+ * (C/Sphase.f90:193).  If res_hist is non-null it receives res_mass of
+ * every iteration run (max_iters entries).  Returns the iterations run in
+ * *n_done. */
+int gfm_hip_sphase(gfm_ctx *ctx, gfm_ctrl *ctrl, int max_iters, double *res_hist, int *n_done);
+
+/* ADLBL3 on a host-provided system (C/Adlbl3.f90:10-97): AP, AS(…,3,2),
+ * BS and XT on the doubled lattice, NTIMESG(nelg) sweeps over the index
+ * box [ib..ie]x[jb..je]x[kb..ke] (stride 2, 1-based).  XT is updated in
+ * place; RESG(nelg,1:2) is returned in resg[0..1] when want_irc is set. */
+int gfm_hip_adlbl3(gfm_ctx *ctx, const gfm_ctrl *ctrl, int nelg,
+                   const double *ap, const double *as, const double *bs, double *xt,
+                   int ib, int ie, int jb, int je, int kb, int ke, double *resg);
+
+/* calc_residual (C/Adlbl3.f90:239-271) on a host-provided system. */
+int gfm_hip_calc_residual(gfm_ctx *ctx, const double *ap, const double *as, const double *bs,
+                          const double *xt, int ib, int ie, int jb, int je, int kb, int ke,
+                          double *resid);
+
+/* Per-kernel timing of the last gfm_hip_sphase call: fills up to `cap`
+ * (name, total ms, launches) records; returns the count in *n. */
+int gfm_hip_kernel_times(gfm_ctx *ctx, int enable, char *names, int name_len,
+                         double *ms, int *launches, int cap, int *n);
+
+int gfm_hip_finalize(gfm_ctx *ctx);
+
+const char *gfm_hip_last_error(void);
+int gfm_hip_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GFM_HIP_H */

@@ -1,0 +1,1493 @@
+/*
+ * gfm_oracle.c -- CPU restatement of the GFM v4 combustion-space SIMPLER
+ * iteration, used ONLY as the test oracle and as the CPU baseline.
+ *
+ * TEST INFRASTRUCTURE.  Nothing in the product (gfm_amd/, the HIP library)
+ * links, loads or calls this file; tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py use it to check and to time the GPU path.
+ *
+ * It works on the reference's own data model: every field on the
+ * doubled-index lattice (MP,NP,LP), 1-based, column-major, FP64, with the
+ * same loop orders as the Fortran so that floating-point results follow it
+ * closely.  Each function cites the reference unit it restates
+ * (C/ = combustion/Code/ of the reference package).
+ *
+ * Deliberate restrictions (checked at entry, documented in the design
+ * notes): single phase (NDNP=0), Cartesian (CYL=.false.), turbulent,
+ * reacting, steady, SIMPLER; no half-blocked cells (IBCELL>=0); no
+ * radiation transport (only the optional gray source term).  Unused
+ * diagnostic sums (FLX/FLY/FLUX_Z plane fluxes of FLX1D, the Info/gresid
+ * file writes) are not formed.
+ *
+ * Parity pinning: see the header of tests/test_oracle.py for which closed
+ * forms, invariants and reference-held data pin each function here.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+#include "../include/gfm_hip.h"
+
+/* ---- constants of GBL_VAR (C/Var_mod.f90:8-200) ---- */
+static const double small20 = 1.0e-20, small16 = 1.0e-16, big = 1.0e20;
+static const double wf = 16.0426, wo2 = 31.9988, wco2 = 44.0098, wh2o = 18.0152, wn2 = 28.0134;
+static const double cmu = 9.0e-2, ct1 = 1.44, ct2 = 1.92;
+static const double cpf1 = 2.156e3 / 1.0057e3;
+static const double sigma_eq[12] = {0.7, 0.7, 0.7, 1.0, 1.0, 0.7, 0.7, 0.7, 0.7, 0.7, 0.7, 0.7};
+static double wt0, stoich;
+static const int iyf = 1, iyo2 = 2, ih = 3, ik = 4, ieps = 5, iyco2 = 6, iyn2 = 7, iyh2o = 9;
+static const int lstar = 1, lend = 7;
+/* NIST Shomate coefficients (C/Setup.f90:103-141), species order
+ * fuel, O2, CO2, N2, H2O; [n][0]=low set, [n][1]=high set. */
+static const double a_nist[5][2] = {{-0.703029, 85.81217}, {29.659, 29.659}, {24.99735, 58.16639}, {26.092, 26.092}, {30.092, 41.96426}};
+static const double b_nist[5][2] = {{108.4773, 11.26467}, {6.137261, 6.137261}, {55.18696, 2.720074}, {8.218801, 8.218801}, {6.832514, 8.622053}};
+static const double c_nist[5][2] = {{-42.52157, -2.114146}, {-1.186521, -1.186521}, {-33.69137, -0.492289}, {-1.976141, -1.976141}, {6.793435, -1.49978}};
+static const double d_nist[5][2] = {{5.862788, 0.13819}, {0.09578, 0.09578}, {7.948387, 0.038844}, {0.159274, 0.159274}, {-2.53448, 0.098119}};
+static const double e_nist[5][2] = {{0.678565, -26.42221}, {-0.219663, -0.219663}, {-0.136638, -6.447293}, {0.044434, 0.044434}, {0.082139, -11.15764}};
+static const double f_nist[5][2] = {{-76.84376, -153.5327}, {-9.861391, -9.861391}, {-403.6075, -425.9186}, {-7.98923, -7.98923}, {-250.881, -272.1797}};
+static const double h_nist[5][2] = {{-74.8731, -74.8731}, {0.0, 0.0}, {-393.5224, -393.5224}, {0.0, 0.0}, {-241.8264, -241.8264}};
+static const double cutT[5] = {1300.0, 6000.0, 1200.0, 6000.0, 1700.0};
+static const int gf_chem_index[5] = {1, 2, 6, 7, 9};
+static const int nmax_newton = 10;
+static const double tolerance_enth = 1.0e-11;
+
+/* ---- module state (pointers into the caller's arrays) ---- */
+static int MP, NP, LP, MPM1, MPM2, NPM1, NPM2, LPM1, LPM2, NR0;
+static size_t NN;
+static int NJY1, NJY2, LKZ1, LKZ2, NJY1V[4], NJY2V[4], LKZ1V[4], LKZ2V[4];
+static double *X_, *DX_, *R_, *DR_, *RS_, *Z_, *DZ_;
+static int16_t *IBC;
+static double *P_, *T_, *DNST_, *THETA_, *TMU_, *GCP_, *GLAM_, *GAMA_, *GTE_, *CRATE_, *HRATE_, *BSH_;
+static double *UG_, *GF_;
+static double *RTIM_, *DELRT_, *HRT_, *DELR_, *TIMED_, *HRD_;
+static double *FZ_, *BS_, *SFP_, *AP_, *AS_;
+static gfm_ctrl *C;
+static double SGM = 1.0;
+static int NELG = 0;
+static double T_new_state = 0.0; /* module variable T_new of enth_to_T */
+
+#define IX3(i, j, k) ((((size_t)(k)-1) * (size_t)NP + ((size_t)(j)-1)) * (size_t)MP + ((size_t)(i)-1))
+#define X(i) X_[(i)-1]
+#define DX(i) DX_[(i)-1]
+#define R(j) R_[(j)-1]
+#define DR(j) DR_[(j)-1]
+#define RS(j) RS_[(j)-1]
+#define Z(k) Z_[(k)-1]
+#define DZ(k) DZ_[(k)-1]
+#define IBCELL(i, j, k) ((int)IBC[IX3(i, j, k)])
+#define P(i, j, k) P_[IX3(i, j, k)]
+#define T(i, j, k) T_[IX3(i, j, k)]
+#define DNST(i, j, k) DNST_[IX3(i, j, k)]
+#define THETA(i, j, k) THETA_[IX3(i, j, k)]
+#define TMU(i, j, k) TMU_[IX3(i, j, k)]
+#define GCP(i, j, k) GCP_[IX3(i, j, k)]
+#define GLAM(i, j, k) GLAM_[IX3(i, j, k)]
+#define GAMA(i, j, k) GAMA_[IX3(i, j, k)]
+#define GTE(i, j, k) GTE_[IX3(i, j, k)]
+#define CRATE(i, j, k) CRATE_[IX3(i, j, k)]
+#define HRATE(i, j, k) HRATE_[IX3(i, j, k)]
+#define BSHPREV(i, j, k) BSH_[IX3(i, j, k)]
+#define UG(i, j, k, l) UG_[IX3(i, j, k) + NN * (size_t)((l)-1)]
+#define GF(i, j, k, l) GF_[IX3(i, j, k) + NN * (size_t)((l)-1)]
+#define FZ(i, j, k) FZ_[IX3(i, j, k)]
+#define BS(i, j, k) BS_[IX3(i, j, k)]
+#define SFP(i, j, k) SFP_[IX3(i, j, k)]
+#define AP(i, j, k) AP_[IX3(i, j, k)]
+#define AS(i, j, k, d, m) AS_[IX3(i, j, k) + NN * (size_t)(((d)-1) + 3 * ((m)-1))]
+
+static inline double dmax(double a, double b) { return a > b ? a : b; }
+static inline double dmin(double a, double b) { return a < b ? a : b; }
+static inline int imin(int a, int b) { return a < b ? a : b; }
+static inline int imax(int a, int b) { return a > b ? a : b; }
+static inline int imod10(int a) { return a % 10; } /* Fortran MOD keeps the sign of a, as C % */
+
+/* ======================================================================
+ * Geometry, C/Grid3.f90:310-471 (no half-blocked cells)
+ * ====================================================================== */
+static double AREA_C(int i, int j, int k, int ix) {
+  if (IBCELL(i, j, k) == 1) return 0.0;
+  if (ix == 1) return DR(j) * RS(j) * DZ(k);
+  if (ix == 2) return DX(i) * DZ(k);
+  return DX(i) * DR(j) * RS(j);
+}
+static double VOL_C0(int i, int j, int k) { return DX(i) * DR(j) * RS(j) * DZ(k); }
+static double VOL_C(int i, int j, int k) {
+  double vm, vp;
+  if (i & 1) {
+    vm = VOL_C0(i - 1, j, k); if (IBCELL(i - 2, j, k) <= 0) vm *= 0.5;
+    vp = VOL_C0(i + 1, j, k); if (IBCELL(i + 2, j, k) <= 0) vp *= 0.5;
+    return vp + vm;
+  } else if (j & 1) {
+    vm = VOL_C0(i, j - 1, k); if (IBCELL(i, j - 2, k) <= 0) vm *= 0.5;
+    vp = VOL_C0(i, j + 1, k); if (IBCELL(i, j + 2, k) <= 0) vp *= 0.5;
+    return vp + vm;
+  } else if (k & 1) {
+    vm = VOL_C0(i, j, k - 1); if (IBCELL(i, j, k - 2) <= 0) vm *= 0.5;
+    vp = VOL_C0(i, j, k + 1); if (IBCELL(i, j, k + 2) <= 0) vp *= 0.5;
+    return vp + vm;
+  }
+  return VOL_C0(i, j, k);
+}
+static double DXAV(int i, int j, int k) {
+  if (IBCELL(i - 2, j, k) >= 1) return X(i + 1) - X(i - 2);
+  if (IBCELL(i + 2, j, k) >= 1) return X(i + 2) - X(i - 1);
+  return DX(i);
+}
+static double DRAV(int i, int j, int k) {
+  if (IBCELL(i, j - 2, k) >= 1) return (RS(j + 1) + RS(j - 2)) * 0.5 / RS(j) * (R(j + 1) - R(j - 2));
+  if (IBCELL(i, j + 2, k) >= 1) return (RS(j + 2) + RS(j - 1)) * 0.5 / RS(j) * (R(j + 2) - R(j - 1));
+  return DR(j);
+}
+static double DZAV(int i, int j, int k) {
+  if (IBCELL(i, j, k - 2) >= 1) return Z(k + 1) - Z(k - 2);
+  if (IBCELL(i, j, k + 2) >= 1) return Z(k + 2) - Z(k - 1);
+  return DZ(k);
+}
+
+/* ======================================================================
+ * Properties, C/Prop.f90
+ * ====================================================================== */
+/* DA: exponential-scheme Peclet function, C/Prop.f90:97-117 */
+double oracle_da(double d, double f) {
+  if (d <= 1.0e-25) return 0.0;
+  double fa = fabs(f), pe = fa / d;
+  if (pe <= 1.0e-2) return d / (1.0 + pe * (0.5 + pe * (1.0 / 6.0 + pe / 24.0)));
+  if (pe <= 20.0) return fa / (exp(pe) - 1.0);
+  return 0.0;
+}
+#define DA oracle_da
+/* TGAMF: wall-function factor, C/Gflow.f90:417-426 */
+double oracle_tgamf(double b) {
+  if (b < 11.5) return 1.0;
+  double g0 = b / (log(9.0 * b) * 2.5);
+  return dmax(g0, 1.0);
+}
+/* CONC, C/Prop.f90:28-59 */
+static void CONC(int i, int j, int k) {
+  double g0 = GF(i, j, k, iyf) + GF(i, j, k, iyo2) + GF(i, j, k, iyco2) + GF(i, j, k, iyn2);
+  if (g0 > 1.0) {
+    g0 = g0 + GF(i, j, k, iyh2o);
+    GF(i, j, k, iyf) /= g0; GF(i, j, k, iyo2) /= g0; GF(i, j, k, iyco2) /= g0;
+    GF(i, j, k, iyn2) /= g0; GF(i, j, k, iyh2o) /= g0;
+  } else {
+    GF(i, j, k, iyh2o) = 1.0 - g0;
+  }
+}
+/* dnst_calc, C/Prop.f90:187-198 */
+static void DNST_CALC(int i, int j, int k) {
+  CONC(i, j, k);
+  double wt = 1.0 / (GF(i, j, k, iyf) / wf + GF(i, j, k, iyco2) / wco2 + GF(i, j, k, iyh2o) / wh2o +
+                     GF(i, j, k, iyo2) / wo2 + GF(i, j, k, iyn2) / wn2);
+  double temp = (1.0 + P(i, j, k)) / T(i, j, k) * wt / wt0;
+  DNST(i, j, k) = dmax(temp, small20);
+}
+/* THER, C/Prop.f90:130-154 (TURB, AMU=0) */
+static void THER(int i, int j, int k) {
+  GF(i, j, k, ieps) = dmax(GF(i, j, k, ieps), 1.0e-12);
+  double g0 = cmu * C->reyg * C->epsr / (2.0 * C->taur) * GF(i, j, k, ik) * GF(i, j, k, ik) / GF(i, j, k, ieps) * DNST(i, j, k);
+  TMU(i, j, k) = g0;
+  GLAM(i, j, k) = 1.0;
+  GCP(i, j, k) = cpf1;
+  GAMA(i, j, k) = 2.0 / C->reyg * (TMU(i, j, k) + 1.0);
+}
+/* BOUND, C/Prop.f90:243-296 */
+static void BOUND(double *f, int il, int ipp) {
+  if (ipp == 1) {
+    if (il == 0) { *f = dmin(*f, C->p_mx); *f = dmax(*f, C->p_mn); }
+    else if (il == 4) { *f = dmin(*f, C->t_mx); *f = dmax(*f, C->t_mn); }
+    else if (il == 5) { *f = dmax(*f, 1.0e-6); }
+    else { *f = dmin(*f, C->u_mx); *f = dmax(*f, C->u_mn); if (fabs(*f) < 1.0e-12) *f = 0.0; }
+  } else if (ipp == 2) {
+    if (il == ik) *f = dmax(*f, 1.0e-6);
+    else if (il == ieps) *f = dmax(*f, 1.0e-12);
+    else if (il == ih) return;
+    else if (il == 20) *f = dmax(*f, 1.0e-1);
+    else if (il == 8) *f = dmax(*f, 1.0e-14);
+    else { *f = dmin(*f, 1.0); *f = dmax(*f, 0.0); }
+  }
+}
+/* Mixture Shomate coefficients with the reference's f_mix/h_mix
+ * construction (f_mix = d_mix + Y*F, h_mix = e_mix + Y*H, not
+ * accumulated; C/Prop.f90:398-418). */
+typedef struct { double a, b, c, d, e, f, h; int lohi[5]; } mixco;
+static void mix_add(mixco *m, double y, int n, int s, double sgn) {
+  m->a += sgn * y * a_nist[n][s]; m->b += sgn * y * b_nist[n][s]; m->c += sgn * y * c_nist[n][s];
+  m->d += sgn * y * d_nist[n][s]; m->e += sgn * y * e_nist[n][s];
+  m->f = m->d + sgn * y * f_nist[n][s]; m->h = m->e + sgn * y * h_nist[n][s];
+}
+static void mix_build(mixco *m, const double y[5], double realT) {
+  memset(m, 0, sizeof(*m));
+  for (int n = 0; n < 5; n++) {
+    int s = (realT < cutT[n]) ? 0 : 1;
+    mix_add(m, y[n], n, s, 1.0);
+    m->lohi[n] = s + 1;
+  }
+}
+/* enth_to_T: Newton on the Shomate mixture enthalpy, C/Prop.f90:369-505.
+ * y = mass fractions (fuel,O2,CO2,N2,H2O), h = normalised enthalpy,
+ * tn = normalised temperature (initial guess in, result out). */
+double oracle_enth_to_t(const double y[5], double h, double tn, double h_0, double t0) {
+  double w_mix = 1.0 / (y[0] / wf + y[2] / wco2 + y[4] / wh2o + y[1] / wo2 + y[3] / wn2);
+  double h_s = h * h_0;
+  double h_molar = h_s * w_mix / 1.0e6;
+  double realT = tn * t0;
+  mixco m;
+  mix_build(&m, y, realT);
+  double T_old = realT / 1000.0, T_new = T_new_state;
+  for (int n = 1; n <= nmax_newton; n++) {
+    double gp = m.a + T_old * (m.b + T_old * (m.c + T_old * m.d)) + m.e / (T_old * T_old);
+    if (gp == 0.0) break;
+    double g = T_old * (m.a + T_old * ((m.b / 2.0) + T_old * ((m.c / 3.0) + T_old * (m.d / 4.0)))) - (m.e / T_old) + m.f - m.h;
+    double changeT = (g - h_molar) / gp;
+    T_new = T_old - changeT;
+    if (fabs(changeT) < tolerance_enth) break;
+    T_old = T_new;
+    for (int nn = 0; nn < 5; nn++) {
+      if (nn == 1 || nn == 3) continue;
+      if (T_new * 1000.0 < cutT[nn] && m.lohi[nn] != 1) {
+        mix_add(&m, y[nn], nn, 1, -1.0);
+        m.lohi[nn] = 1;
+        mix_add(&m, y[nn], nn, 0, 1.0);
+      }
+      if (T_new * 1000.0 >= cutT[nn] && m.lohi[nn] != 2) {
+        mix_add(&m, y[nn], nn, 0, -1.0);
+        m.lohi[nn] = 2;
+        mix_add(&m, y[nn], nn, 1, 1.0);
+      }
+    }
+  }
+  T_new_state = T_new;
+  return T_new * 1000.0 / t0;
+}
+/* T_to_enth, C/Prop.f90:525-565 */
+double oracle_t_to_enth(const double y[5], double tn, double h_0, double t0) {
+  double w_mix = 1.0 / (y[0] / wf + y[2] / wco2 + y[4] / wh2o + y[1] / wo2 + y[3] / wn2);
+  double realT = tn * t0;
+  mixco m;
+  mix_build(&m, y, realT);
+  double t = realT / 1000.0;
+  double ef = t * (m.a + t * ((m.b / 2.0) + t * ((m.c / 3.0) + t * (m.d / 4.0)))) - (m.e / t) + m.f - m.h;
+  return ef * 1.0e6 / (w_mix * h_0);
+}
+static void ygather(int i, int j, int k, double y[5]) {
+  for (int n = 0; n < 5; n++) y[n] = GF(i, j, k, gf_chem_index[n]);
+}
+static void ENTH_TO_T(int i, int j, int k) {
+  double y[5]; ygather(i, j, k, y);
+  T(i, j, k) = oracle_enth_to_t(y, GF(i, j, k, ih), T(i, j, k), C->h_0, C->t0);
+}
+static void T_TO_ENTH(int i, int j, int k) {
+  double y[5]; ygather(i, j, k, y);
+  GF(i, j, k, ih) = oracle_t_to_enth(y, T(i, j, k), C->h_0, C->t0);
+}
+/* ZY1: table interpolation, C/Source.f90:814-831 */
+double oracle_zy1(const double *x, const double *y, double x1, int n) {
+  if (x1 < x[0]) return y[0] * x1 / x[0];
+  for (int i = 2; i <= n; i++)
+    if (x1 < x[i - 1]) return y[i - 2] + (y[i - 1] - y[i - 2]) * ((x1 - x[i - 2]) / (x[i - 1] - x[i - 2]));
+  return y[n - 1];
+}
+
+/* ======================================================================
+ * Momentum assembly helpers, C/Gslv3x.f90 and VINDX C/Dflow.f90:189-219
+ * ====================================================================== */
+static void VINDX(int iu, int m, int i, int j, int k, int *i1, int *j1, int *k1, int *i2, int *j2, int *k2) {
+  int nx[3] = {0, 0, 0};
+  nx[iu - 1] = m;
+  *i1 = i + nx[0]; *i2 = *i1 + nx[0];
+  *j1 = j + nx[1]; *j2 = *j1 + nx[1];
+  *k1 = k + nx[2]; *k2 = *k1 + nx[2];
+}
+static double FLW(int i, int j, int k, int ix, int m) {
+  double f = THETA(i, j, k) * DNST(i, j, k) * UG(i, j, k, ix) * AREA_C(i, j, k, ix);
+  return (m == 1) ? dmax(0.0, f) : dmax(0.0, -f);
+}
+static double FLWD(int i, int j, int k, int ix, int m) {
+  double area = AREA_C(i, j, k, ix);
+  double ddl = (ix == 1) ? DX(i) : (ix == 2) ? DR(j) : DZ(k);
+  double f = THETA(i, j, k) * DNST(i, j, k) * UG(i, j, k, ix) * area;
+  double diff = THETA(i, j, k) * GAMA(i, j, k) / SGM * area / ddl;
+  return (m == 1) ? DA(diff, f) + dmax(0.0, f) : DA(diff, f) + dmax(0.0, -f);
+}
+static double FDDL(int i, int j, int k, int iu) {
+  int b = IBCELL(i, j, k);
+  if (iu == 1) {
+    if (b <= 0) return DX(i);
+    if (IBCELL(i + 1, j, k) == 0) return X(i + 1) - X(i);
+    if (IBCELL(i - 1, j, k) == 0) return X(i) - X(i - 1);
+    return big;
+  } else if (iu == 2) {
+    if (b <= 0) return DR(j) * RS(j);
+    if (IBCELL(i, j + 1, k) == 0) return (R(j + 1) - R(j)) * RS(j);
+    if (IBCELL(i, j - 1, k) == 0) return (R(j) - R(j - 1)) * RS(j);
+    return big;
+  } else {
+    if (b <= 0) return DZ(k);
+    if (IBCELL(i, j, k + 1) == 0) return Z(k + 1) - Z(k);
+    if (IBCELL(i, j, k - 1) == 0) return Z(k) - Z(k - 1);
+    return big;
+  }
+}
+/* ECELL, C/Gslv3x.f90:148-200 (LP>=6) ; FFL[d][m], d,m 0-based */
+static void ECELL(int i, int j, int k, int iu, double ffl[3][2]) {
+  if (iu == 1) {
+    for (int m = 0; m < 2; m++) {
+      int i1 = m ? i + 1 : i - 1, i2 = m ? i + 2 : i - 2;
+      ffl[0][m] = (IBCELL(i2, j, k) >= 1) ? DX(i1) : fabs(X(i) - X(i1));
+      ffl[1][m] = ffl[0][m] * DZ(k);
+      ffl[2][m] = ffl[0][m] * RS(j) * DR(j);
+    }
+  } else if (iu == 2) {
+    for (int m = 0; m < 2; m++) {
+      int j1 = m ? j + 1 : j - 1, j2 = m ? j + 2 : j - 2;
+      ffl[1][m] = (IBCELL(i, j2, k) >= 1) ? RS(j1) * DR(j1) : RS(j1) * fabs(R(j) - R(j1));
+      ffl[0][m] = ffl[1][m] * DZ(k);
+      ffl[2][m] = ffl[1][m] * DX(i);
+    }
+  } else {
+    for (int m = 0; m < 2; m++) {
+      int k1 = m ? k + 1 : k - 1, k2 = m ? k + 2 : k - 2;
+      ffl[2][m] = (IBCELL(i, j, k2) >= 1) ? DZ(k1) : fabs(Z(k) - Z(k1));
+      ffl[0][m] = ffl[2][m] * RS(j) * DR(j);
+      ffl[1][m] = ffl[2][m] * DX(i);
+    }
+  }
+}
+static double VOL1(int i, int j, int k, int ix, double ffl[3][2]) {
+  return (ffl[ix - 1][0] + ffl[ix - 1][1]) * AREA_C(i, j, k, ix);
+}
+/* GSLV3X1, C/Gslv3x.f90:119-139 */
+static double GSLV3X1(int i, int j, int k, int iu) {
+  double g0 = 0.0;
+  for (int m = -1; m <= 1; m += 2) {
+    int m1 = (m + 3) / 2, i1, j1, k1, i2, j2, k2;
+    VINDX(iu, m, i, j, k, &i1, &j1, &k1, &i2, &j2, &k2);
+    int ibc1 = IBCELL(i2, j2, k2);
+    if (ibc1 == 3) AS(i, j, k, iu, m1) = 0.0;
+    else if (ibc1 == 2) AS(i, j, k, iu, m1) = FLW(i2, j2, k2, iu, m1);
+    else AS(i, j, k, iu, m1) = FLWD(i1, j1, k1, iu, m1);
+    AP(i, j, k) += AS(i, j, k, iu, m1);
+    g0 += AS(i, j, k, iu, m1) * FZ(i2, j2, k2);
+  }
+  return g0;
+}
+/* SRC1: viscous and buoyancy source of momentum, C/Source.f90:664-809 (NDNP=0) */
+static void SRC1(int i, int j, int k, int iu1) {
+  if (IBCELL(i, j, k) >= 1) { BS(i, j, k) = 0.0; SFP(i, j, k) = 0.0; return; }
+  double vis[3][2] = {{0}}, dxn[3], dx1[3] = {0}, dx0[3] = {0};
+  dxn[0] = DX(i); dxn[1] = DR(j) * RS(j); dxn[2] = DZ(k);
+  for (int m = -1; m <= 1; m += 2) {
+    int i1, j1, k1, i2, j2, k2;
+    VINDX(iu1, m, i, j, k, &i1, &j1, &k1, &i2, &j2, &k2);
+    if (iu1 == 1) { dx0[0] = DX(i); dx1[0] = DX(i1); }
+    else if (iu1 == 2) { dx0[1] = DR(j) * RS(j); dx1[1] = DR(j1) * RS(j1); }
+    else { dx0[2] = DZ(k); dx1[2] = DZ(k1); }
+    double thgm;
+    if (IBCELL(i2, j2, k2) == 2) {
+      thgm = THETA(i2, j2, k2) * GAMA(i2, j2, k2);
+      if (iu1 == 1) dxn[0] += fabs(X(i2) - X(i1));
+      if (iu1 == 2) dxn[1] += fabs(R(j2) - R(j1)) * RS(j);
+      if (iu1 == 3) dxn[2] += fabs(Z(k2) - Z(k1));
+    } else {
+      thgm = THETA(i1, j1, k1) * GAMA(i1, j1, k1);
+    }
+    if (m == -1) vis[iu1 - 1][0] = thgm * (FZ(i, j, k) - FZ(i2, j2, k2)) / dx1[iu1 - 1];
+    else vis[iu1 - 1][1] = thgm * (FZ(i2, j2, k2) - FZ(i, j, k)) / dx1[iu1 - 1];
+  }
+  double vms = (vis[iu1 - 1][1] - vis[iu1 - 1][0]) / dxn[iu1 - 1];
+  for (int iu = 1; iu <= 3; iu++) {
+    if (iu == iu1) continue;
+    int i1 = i, j1 = j, k1 = k;
+    for (int m = -1; m <= 1; m += 2) {
+      if (iu == 1) i1 = i + m; else if (iu == 2) j1 = j + m; else k1 = k + m;
+      double du0;
+      if (iu1 == 1) du0 = FZ(i + 1, j1, k1) - FZ(i - 1, j1, k1);
+      else if (iu1 == 2) du0 = FZ(i1, j + 1, k1) - FZ(i1, j - 1, k1);
+      else du0 = FZ(i1, j1, k + 1) - FZ(i1, j1, k - 1);
+      int m1 = (m + 3) / 2;
+      vis[iu - 1][m1 - 1] = THETA(i1, j1, k1) * GAMA(i1, j1, k1) * du0 / dx0[iu1 - 1];
+      if (iu == 2) vis[iu - 1][m1 - 1] *= RS(j1);
+    }
+    vms += (vis[iu - 1][1] - vis[iu - 1][0]) / dxn[iu - 1];
+  }
+  BS(i, j, k) = vms + 0.0 + 0.0;
+  BS(i, j, k) = BS(i, j, k) + THETA(i, j, k) * DNST(i, j, k) * C->gra[iu1 - 1];
+  SFP(i, j, k) = -0.0;
+}
+
+/* ======================================================================
+ * Source terms, C/Source.f90:6-658
+ * ====================================================================== */
+static void SOURCE(int mi1, int mi3, int nj1, int nj3, int lk1, int lk3) {
+  int nelg = NELG;
+  if (nelg >= 1 && nelg <= 3) {
+    for (int i = mi1; i <= mi3; i += 2)
+      for (int j = nj1; j <= nj3; j += 2)
+        for (int k = lk1; k <= lk3; k += 2) SRC1(i, j, k, nelg);
+  } else if (nelg == 4 || nelg == 5) {
+    if (nelg == 5) { C->res_mass = 0.0; C->res_mass_max = 0.0; C->imx = 2; C->jmx = 2; C->kmx = 2; }
+    for (int i = mi1; i <= mi3; i += 2)
+      for (int j = nj1; j <= nj3; j += 2)
+        for (int k = lk1; k <= lk3; k += 2) {
+          if (IBCELL(i, j, k) >= 1) continue;
+          double ax = AREA_C(i, j, k, 1);
+          double frx1 = (THETA(i - 1, j, k) * DNST(i - 1, j, k) * UG(i - 1, j, k, 1)) * ax;
+          double frx2 = (THETA(i + 1, j, k) * DNST(i + 1, j, k) * UG(i + 1, j, k, 1)) * ax;
+          double ay = AREA_C(i, j, k, 2);
+          double fry1 = (THETA(i, j - 1, k) * DNST(i, j - 1, k) * UG(i, j - 1, k, 2)) * ay;
+          double fry2 = (THETA(i, j + 1, k) * DNST(i, j + 1, k) * UG(i, j + 1, k, 2)) * ay;
+          double az = AREA_C(i, j, k, 3);
+          double frz1 = (THETA(i, j, k - 1) * DNST(i, j, k - 1) * UG(i, j, k - 1, 3)) * az;
+          double frz2 = (THETA(i, j, k + 1) * DNST(i, j, k + 1) * UG(i, j, k + 1, 3)) * az;
+          double rx = frx1 - frx2, ry = fry1 - fry2, rz = frz1 - frz2, evms = 0.0;
+          BS(i, j, k) = (rx + ry + rz + evms) / C->eun;
+          SFP(i, j, k) = 0.0;
+          if (nelg == 5) {
+            double r_mass;
+            if (IBCELL(i - 1, j, k) == 3 || IBCELL(i + 1, j, k) == 3 || IBCELL(i, j - 1, k) == 3 ||
+                IBCELL(i, j + 1, k) == 3 || IBCELL(i, j, k - 1) == 3 || IBCELL(i, j, k + 1) == 3)
+              continue;
+            double fmax = dmax(dmax(dmax(fabs(frx1), fabs(frx2)), dmax(fabs(fry1), fabs(fry2))),
+                               dmax(dmax(fabs(frz1), fabs(frz2)), evms));
+            r_mass = (fmax == 0.0) ? 0.0 : fabs(BS(i, j, k)) * C->eun / fmax;
+            if (C->res_mass_max < r_mass) { C->imx = i; C->jmx = j; C->kmx = k; C->res_mass_max = r_mass; }
+            C->res_mass += r_mass;
+          }
+        }
+    if (nelg == 5) C->res_mass /= C->ncells;
+  } else if (nelg == 6) {
+    /* fuel: integral reaction submodel, C/Source.f90:104-242 */
+    for (int i = mi1; i <= mi3; i += 2)
+      for (int j = nj1; j <= nj3; j += 2)
+        for (int k = lk1; k <= lk3; k += 2) {
+          if (IBCELL(i, j, k) >= 1) { BS(i, j, k) = FZ(i, j, k); SFP(i, j, k) = 0.0; CRATE(i, j, k) = 0.0; HRATE(i, j, k) = 0.0; continue; }
+          double sc = 0.0;
+          double d11 = fabs(UG(i + 1, j, k, 1) / DX(i)), d12 = fabs(UG(i - 1, j, k, 1) / DX(i));
+          double d21 = fabs(UG(i, j + 1, k, 2) / DR(j)), d22 = fabs(UG(i, j - 1, k, 2) / DR(j));
+          double d31 = fabs(UG(i, j, k + 1, 3) / DZ(k)), d32 = fabs(UG(i, j, k - 1, 3) / DZ(k));
+          double dtime = dmax(dmax(dmax(d11, d12), dmax(d21, d22)), dmax(dmax(d31, d32), 1.0e-14));
+          dtime = 1.0 / dtime;
+          double drtime = dtime * C->r0 / C->ug0;
+          double ayf = dmin(GF(i, j, k, iyf), GF(i, j, k, iyo2) / stoich);
+          if (ayf < 1.0e-5) { CRATE(i, j, k) = 0.0; HRATE(i, j, k) = 0.0; SFP(i, j, k) = 0.0; BS(i, j, k) = 0.0; continue; }
+          double zp1 = GF(i, j, k, iyco2) / wco2, delr1;
+          if (zp1 <= small20) delr1 = 0.0;
+          else { double zf = GF(i, j, k, iyf) / wf; delr1 = zp1 / (zf + zp1); }
+          double time1 = oracle_zy1(DELR_, TIMED_, delr1, NR0);
+          double time2 = time1 + drtime;
+          double delr2 = oracle_zy1(RTIM_, DELRT_, time2, NR0);
+          double ayf0 = ayf / (1.0 - delr1) * DNST(i, j, k) * THETA(i, j, k);
+          double sr = ayf0 * (delr2 - delr1) / dtime;
+          if (sr <= 0.0) { CRATE(i, j, k) = 0.0; HRATE(i, j, k) = 0.0; SFP(i, j, k) = 0.0; BS(i, j, k) = sc; continue; }
+          CRATE(i, j, k) = sr;
+          SFP(i, j, k) = -CRATE(i, j, k) / (GF(i, j, k, iyf) + small20);
+          BS(i, j, k) = sc;
+          HRATE(i, j, k) = CRATE(i, j, k) * C->q0;
+        }
+  } else if (nelg == 7 || nelg == 13) {
+    double g0 = (nelg == 7) ? wco2 / wf : wh2o * 2.0 / wf;
+    for (int i = mi1; i <= mi3; i += 2)
+      for (int j = nj1; j <= nj3; j += 2)
+        for (int k = lk1; k <= lk3; k += 2) {
+          SFP(i, j, k) = 0.0;
+          if (IBCELL(i, j, k) >= 1) { BS(i, j, k) = FZ(i, j, k); continue; }
+          BS(i, j, k) = CRATE(i, j, k) * g0;
+        }
+  } else if (nelg == 8 || nelg == 9) {
+    if (nelg == 8) {
+      C->q_react = 0.0;
+      for (int i = mi1; i <= mi3; i += 2)
+        for (int j = nj1; j <= nj3; j += 2)
+          for (int k = lk1; k <= lk3; k += 2) {
+            if (IBCELL(i, j, k) >= 1) continue;
+            double dudx, dvdx, dwdx, dudr, dvdr, dwdr, dudz, dvdz, dwdz;
+            if (IBCELL(i + 1, j, k) == 3 || IBCELL(i - 1, j, k) == 3) { dudx = dvdx = dwdx = 0.0; }
+            else {
+              dudx = (UG(i + 1, j, k, 1) - UG(i - 1, j, k, 1)) / DX(i);
+              dvdx = (UG(i + 1, j, k, 2) - UG(i - 1, j, k, 2)) / DX(i);
+              dwdx = (UG(i + 1, j, k, 3) - UG(i - 1, j, k, 3)) / DX(i);
+            }
+            if (nj3 < 6 || IBCELL(i, j + 1, k) == 3 || IBCELL(i, j - 1, k) == 3) { dudr = dvdr = dwdr = 0.0; }
+            else {
+              dudr = (UG(i, j + 1, k, 1) - UG(i, j - 1, k, 1)) / DR(j);
+              dvdr = (UG(i, j + 1, k, 2) - UG(i, j - 1, k, 2)) / DR(j);
+              dwdr = (UG(i, j + 1, k, 3) - UG(i, j - 1, k, 3)) / DR(j);
+            }
+            if (lk3 < 6 || IBCELL(i, j, k + 1) == 3 || IBCELL(i, j, k - 1) == 3) { dudz = dvdz = dwdz = 0.0; }
+            else {
+              dudz = (UG(i, j, k + 1, 1) - UG(i, j, k - 1, 1)) / DZ(k);
+              dvdz = (UG(i, j, k + 1, 2) - UG(i, j, k - 1, 2)) / DZ(k);
+              dwdz = (UG(i, j, k + 1, 3) - UG(i, j, k - 1, 3)) / DZ(k);
+            }
+            double vr = 0.0;
+            GTE(i, j, k) = 2.0 * (dudx * dudx + dvdr * dvdr + vr * vr + dwdz * dwdz) + (dudr + dvdx) * (dudr + dvdx) +
+                           (dwdr + dvdz) * (dwdr + dvdz) + (dudz + dwdx) * (dudz + dwdx) -
+                           (dudx + dvdr + dwdz) * (dudx + dvdr + dwdz) / 1.5;
+          }
+      for (int i = mi1; i <= mi3; i += 2)
+        for (int j = nj1; j <= nj3; j += 2)
+          for (int k = lk1; k <= lk3; k += 2) {
+            if (IBCELL(i, j, k) >= 1) { BS(i, j, k) = FZ(i, j, k); SFP(i, j, k) = 0.0; continue; }
+            double cpmix = cpf1 * GF(i, j, k, iyf) + (2.0e3 / 1.0057e3) * GF(i, j, k, iyco2) +
+                           (2.0e3 / 1.0057e3) * GF(i, j, k, iyh2o) + (0.9136e3 / 1.0057e3) * GF(i, j, k, iyo2) +
+                           (0.526e3 / 1.0057e3) * GF(i, j, k, iyn2);
+            BS(i, j, k) = 0.0 + HRATE(i, j, k);
+            SFP(i, j, k) = -0.0 / cpmix;
+            double vol = VOL_C(i, j, k) * C->vol0;
+            C->q_react += HRATE(i, j, k) * C->qr0 * vol;
+          }
+      if (C->itr_gas > 1) {
+        for (int i = mi1; i <= mi3; i += 2)
+          for (int j = nj1; j <= nj3; j += 2)
+            for (int k = lk1; k <= lk3; k += 2) {
+              if (IBCELL(i, j, k) >= 1) continue;
+              BS(i, j, k) = C->rf_hsource * BS(i, j, k) + (1.0 - C->rf_hsource) * BSHPREV(i, j, k);
+            }
+      }
+      /* bs_hprev=bs: only cell-centre entries are ever read back */
+      for (int i = 2; i <= MP; i += 2)
+        for (int j = 2; j <= NP; j += 2)
+          for (int k = 2; k <= LP; k += 2) BSHPREV(i, j, k) = BS(i, j, k);
+      return;
+    }
+    /* k: C/Source.f90:524-572 (GTE from the enthalpy call, LSTAR<=3) */
+    for (int i = mi1; i <= mi3; i += 2)
+      for (int j = nj1; j <= nj3; j += 2)
+        for (int k = lk1; k <= lk3; k += 2) {
+          if (IBCELL(i, j, k) >= 1) { BS(i, j, k) = FZ(i, j, k); SFP(i, j, k) = 0.0; continue; }
+          BS(i, j, k) = 2.0 / (C->reyg * C->epsr) * THETA(i, j, k) * TMU(i, j, k) * GTE(i, j, k) - 0.0;
+          SFP(i, j, k) = -C->taur * THETA(i, j, k) * DNST(i, j, k) * GF(i, j, k, ieps) / (GF(i, j, k, ik) + small20);
+        }
+  } else if (nelg == 10) {
+    double cbl0 = 2.5 * sqrt(C->epsr) / C->taur * pow(cmu, 0.75);
+    for (int i = mi1; i <= mi3; i += 2)
+      for (int j = nj1; j <= nj3; j += 2)
+        for (int k = lk1; k <= lk3; k += 2) {
+          SFP(i, j, k) = 0.0;
+          if (IBCELL(i, j, k) >= 1) { BS(i, j, k) = FZ(i, j, k); continue; }
+          double cbl = cbl0 * pow(GF(i, j, k, ik), 1.5);
+          BS(i, j, k) = 0.0;
+          if (MP > 8 && IBCELL(i + 1, j, k) == 1) { BS(i, j, k) += big * cbl / (X(i + 1) - X(i)); SFP(i, j, k) = -big; }
+          if (MP > 8 && IBCELL(i - 1, j, k) == 1) { BS(i, j, k) += big * cbl / (X(i) - X(i - 1)); SFP(i, j, k) = -big; }
+          if (NP > 8 && IBCELL(i, j + 1, k) == 1) { BS(i, j, k) += big * cbl / (R(j + 1) - R(j)); SFP(i, j, k) = -big; }
+          if (NP > 8 && IBCELL(i, j - 1, k) == 1) { BS(i, j, k) += big * cbl / (R(j) - R(j - 1)); SFP(i, j, k) = -big; }
+          if (LP > 8 && IBCELL(i, j, k + 1) == 1) { BS(i, j, k) += big * cbl / (Z(k + 1) - Z(k)); SFP(i, j, k) = -big; }
+          if (LP > 8 && IBCELL(i, j, k - 1) == 1) { BS(i, j, k) += big * cbl / (Z(k) - Z(k - 1)); SFP(i, j, k) = -big; }
+          if (SFP(i, j, k) <= -big) continue;
+          double bt = THETA(i, j, k);
+          BS(i, j, k) = ct1 * cmu * bt / C->taur * DNST(i, j, k) * GTE(i, j, k) * GF(i, j, k, ik);
+          SFP(i, j, k) = -ct2 * C->taur * bt * DNST(i, j, k) * GF(i, j, k, ieps) / (GF(i, j, k, ik) + small20);
+        }
+  } else if (nelg == 11) {
+    for (int i = mi1; i <= mi3; i += 2)
+      for (int j = nj1; j <= nj3; j += 2)
+        for (int k = lk1; k <= lk3; k += 2) {
+          if (IBCELL(i, j, k) >= 1) { BS(i, j, k) = FZ(i, j, k); SFP(i, j, k) = 0.0; continue; }
+          BS(i, j, k) = 0.0; SFP(i, j, k) = 0.0;
+        }
+  } else if (nelg == 12) {
+    for (int i = mi1; i <= mi3; i += 2)
+      for (int j = nj1; j <= nj3; j += 2)
+        for (int k = lk1; k <= lk3; k += 2) {
+          SFP(i, j, k) = 0.0;
+          if (IBCELL(i, j, k) >= 1) { BS(i, j, k) = FZ(i, j, k); continue; }
+          BS(i, j, k) = -CRATE(i, j, k) * stoich;
+        }
+  }
+}
+
+/* ======================================================================
+ * Line relaxation, C/Adlbl3.f90
+ * ====================================================================== */
+static double *XT2_, *XT3_; /* (NP,LP) plane buffers */
+#define XT2(j, k) XT2_[((size_t)(k)-1) * NP + ((j)-1)]
+#define XT3(j, k) XT3_[((size_t)(k)-1) * NP + ((j)-1)]
+#define XTA(i, j, k) xt[IX3(i, j, k)]
+
+/* ADLZY: TDMA along y at plane i, line k, C/Adlbl3.f90:147-181 */
+static void ADLZY(double *xt, int i, int jb, int je, int k, int ipl, int iml, int jbml) {
+  static double *pt = NULL, *qt = NULL;
+  static int cap = 0;
+  if (cap < NP + 2) { free(pt); free(qt); cap = NP + 2; pt = malloc(sizeof(double) * cap); qt = malloc(sizeof(double) * cap); }
+  const int ld = 2;
+  int kpl = k + ld, kml = k - ld;
+  pt[jbml] = 0.0; qt[jbml] = XTA(i, jbml, k);
+  for (int j = jb; j <= je; j += ld) {
+    int jml = j - ld;
+    double alp = AP(i, j, k) - pt[jml] * AS(i, j, k, 2, 1);
+    double bsy = BS(i, j, k);
+    if (fabs(alp) <= small20 || IBCELL(i, j, k) >= 1) { pt[j] = 0.0; qt[j] = XTA(i, j, k); }
+    else {
+      pt[j] = AS(i, j, k, 2, 2) / alp;
+      bsy = bsy + AS(i, j, k, 1, 2) * XTA(ipl, j, k) + AS(i, j, k, 1, 1) * XTA(iml, j, k);
+      if (LP >= 6) bsy = bsy + AS(i, j, k, 3, 2) * XTA(i, j, kpl) + AS(i, j, k, 3, 1) * XTA(i, j, kml);
+      qt[j] = (bsy + AS(i, j, k, 2, 1) * qt[jml]) / alp;
+    }
+  }
+  for (int j = je; j >= jb; j -= ld) XT2(j, k) = XT2(j + ld, k) * pt[j] + qt[j];
+}
+/* ADLYZ: TDMA along z at plane i, line j, C/Adlbl3.f90:187-221 */
+static void ADLYZ(double *xt, int i, int j, int kb, int ke, int ipl, int iml) {
+  static double *pt = NULL, *qt = NULL;
+  static int cap = 0;
+  if (cap < LP + 2) { free(pt); free(qt); cap = LP + 2; pt = malloc(sizeof(double) * cap); qt = malloc(sizeof(double) * cap); }
+  const int ld = 2;
+  int jpl = j + ld, jml = j - ld, kbml = kb - ld;
+  pt[kbml] = 0.0; qt[kbml] = XTA(i, j, kbml);
+  for (int k = kb; k <= ke; k += ld) {
+    int kml = k - ld;
+    double alp = AP(i, j, k) - pt[kml] * AS(i, j, k, 3, 1);
+    double bsy = BS(i, j, k);
+    if (fabs(alp) <= small20 || IBCELL(i, j, k) >= 1) { pt[k] = 0.0; qt[k] = XTA(i, j, k); }
+    else {
+      pt[k] = AS(i, j, k, 3, 2) / alp;
+      bsy = bsy + AS(i, j, k, 1, 2) * XTA(ipl, j, k) + AS(i, j, k, 1, 1) * XTA(iml, j, k);
+      if (NP >= 6) bsy = bsy + AS(i, j, k, 2, 2) * XTA(i, jpl, k) + AS(i, j, k, 2, 1) * XTA(i, jml, k);
+      qt[k] = (bsy + AS(i, j, k, 3, 1) * qt[kml]) / alp;
+    }
+  }
+  for (int k = ke; k >= kb; k -= ld) XT3(j, k) = XT3(j, k + ld) * pt[k] + qt[k];
+}
+/* ADLZ: one plane visit, C/Adlbl3.f90:103-141 */
+static void ADLZ(double *xt, int i, int jb, int je, int kb, int ke) {
+  const int ld = 2;
+  int kepl = ke + ld, jbml = jb - ld, jepl = je + ld, ipl = i + ld, iml = i - ld;
+  for (int k = kb; k <= kepl; k += ld)
+    for (int j = jb; j <= jepl; j += ld) { XT2(j, k) = XTA(i, j, k); XT3(j, k) = XTA(i, j, k); }
+  for (int l = 1; l <= 2; l++) {
+    for (int k = kb; k <= ke; k += ld) ADLZY(xt, i, jb, je, k, ipl, iml, jbml);
+    for (int j = jb; j <= je; j += ld) ADLYZ(xt, i, j, kb, ke, ipl, iml);
+    for (int k = kb; k <= ke; k += ld)
+      for (int j = jb; j <= je; j += ld) XTA(i, j, k) = (XT2(j, k) + XT3(j, k)) / 2.0;
+  }
+}
+/* ADLBL3, C/Adlbl3.f90:10-97.  order selects the x-plane visiting order. */
+static void ADLBL3(double *xt, int ib, int ie, int jb, int je, int kb, int ke) {
+  const int ld = 2;
+  int nim = C->ntimesg[NELG - 1];
+  for (int ns = 1; ns <= nim; ns++) {
+    if (C->sweep_order == GFM_ORDER_REFERENCE) {
+      for (int i = ib; i <= ie; i += ld) ADLZ(xt, i, jb, je, kb, ke);
+      for (int i = ie; i >= ib; i -= ld) ADLZ(xt, i, jb, je, kb, ke);
+    } else {
+      for (int half = 0; half < 4; half++) {
+        int colour = half & 1;
+        for (int i = ib + colour * ld; i <= ie; i += 2 * ld) ADLZ(xt, i, jb, je, kb, ke);
+      }
+    }
+  }
+  if (C->want_irc) {
+    double res0 = 0.0, resx = 0.0;
+    for (int i = ib; i <= ie; i += ld)
+      for (int j = jb; j <= je; j += ld)
+        for (int k = kb; k <= ke; k += ld) {
+          if (AP(i, j, k) == 1.0 || IBCELL(i, j, k) > 0) continue;
+          double resc = AS(i, j, k, 1, 2) * XTA(i + ld, j, k) + AS(i, j, k, 1, 1) * XTA(i - ld, j, k) +
+                        AS(i, j, k, 2, 2) * XTA(i, j + ld, k) + AS(i, j, k, 2, 1) * XTA(i, j - ld, k) +
+                        AS(i, j, k, 3, 2) * XTA(i, j, k + ld) + AS(i, j, k, 3, 1) * XTA(i, j, k - ld) +
+                        BS(i, j, k) - AP(i, j, k) * XTA(i, j, k);
+          resc = fabs(resc);
+          resx = dmax(resx, resc);
+          res0 += resc;
+        }
+    res0 /= C->ncells;
+    C->resg[NELG - 1] = res0;
+    C->resg[20 + NELG - 1] = resx;
+  }
+}
+/* calc_residual, C/Adlbl3.f90:239-271 */
+static double CALC_RESIDUAL(const double *xt, int ib, int ie, int jb, int je, int kb, int ke) {
+  double resid = 0.0;
+  for (int k = kb; k <= ke; k += 2)
+    for (int j = jb; j <= je; j += 2)
+      for (int i = ib; i <= ie; i += 2) {
+        if (IBCELL(i, j, k) > 0) continue;
+        double t1 = AS(i, j, k, 1, 1) * XTA(i - 2, j, k), t2 = AS(i, j, k, 1, 2) * XTA(i + 2, j, k);
+        double t3 = AS(i, j, k, 2, 1) * XTA(i, j - 2, k), t4 = AS(i, j, k, 2, 2) * XTA(i, j + 2, k);
+        double t5 = AS(i, j, k, 3, 1) * XTA(i, j, k - 2), t6 = AS(i, j, k, 3, 2) * XTA(i, j, k + 2);
+        double t7 = AP(i, j, k) * XTA(i, j, k), t8 = BS(i, j, k);
+        double eq_max = dmax(dmax(dmax(fabs(t1), fabs(t2)), dmax(fabs(t3), fabs(t4))),
+                             dmax(dmax(fabs(t5), fabs(t6)), dmax(fabs(t7), fabs(t8))));
+        resid += fabs(t1 + t2 + t3 + t4 + t5 + t6 + t8 - t7) / eq_max;
+      }
+  resid /= C->ncells;
+  return dmax(resid, 1.0e-16);
+}
+
+/* ======================================================================
+ * GSLVX1: momentum coefficients and SIMPLER pseudo-velocity,
+ * C/Gslv3x.f90:9-113
+ * ====================================================================== */
+static void GSLVX1(int iu1) {
+  int nx[3] = {0, 0, 0};
+  nx[iu1 - 1] = 1;
+  int mi1 = 4 + nx[0], mi3 = MPM2 - nx[0];
+  int nj1 = NJY1V[iu1], nj3 = NJY2V[iu1], lk1 = LKZ1V[iu1], lk3 = LKZ2V[iu1];
+  SOURCE(mi1, mi3, nj1, nj3, lk1, lk3);
+  for (int i = mi1; i <= mi3; i += 2)
+    for (int j = nj1; j <= nj3; j += 2)
+      for (int k = lk1; k <= lk3; k += 2) {
+        if (IBCELL(i, j, k) >= 1) continue;
+        AP(i, j, k) = 0.0;
+        double g0 = GSLV3X1(i, j, k, iu1);
+        double ffl[3][2];
+        ECELL(i, j, k, iu1, ffl);
+        for (int iu = 1; iu <= 3; iu++) {
+          if (iu == iu1) continue;
+          for (int m = -1; m <= 1; m += 2) {
+            int m1 = (m + 3) / 2, i1, j1, k1, i2, j2, k2;
+            VINDX(iu, m, i, j, k, &i1, &j1, &k1, &i2, &j2, &k2);
+            if (IBCELL(i1, j1, k1) == 3) { AS(i, j, k, iu, m1) = 0.0; continue; }
+            double f = 0.0, diff = 0.0;
+            int ii1 = 0, jj1 = 0, kk1 = 0;
+            for (int n = -1; n <= 1; n += 2) {
+              int n1 = (n + 3) / 2;
+              ii1 = i1 + nx[0] * n; jj1 = j1 + nx[1] * n; kk1 = k1 + nx[2] * n;
+              int ibc1 = IBCELL(ii1, jj1, kk1);
+              if (ibc1 == 3) continue;
+              double area = ffl[iu - 1][n1 - 1];
+              f += THETA(ii1, jj1, kk1) * DNST(ii1, jj1, kk1) * FZ(ii1, jj1, kk1) * area;
+              if (ibc1 == 2) continue;
+              double ddl = FDDL(ii1, jj1, kk1, iu);
+              diff += THETA(ii1, jj1, kk1) * GAMA(ii1, jj1, kk1) / SGM * area / ddl;
+            }
+            double fld = (m1 == 1) ? DA(diff, f) + dmax(0.0, f) : DA(diff, f) + dmax(0.0, -f);
+            if (IBCELL(ii1, jj1, kk1) == 2 && IBCELL(i2, j2, k2) == 1) {
+              double bs0 = fld * (UG(ii1, j2, k2, iu1) - UG(i, j, k, iu1));
+              BS(i, j, k) += bs0;
+              AS(i, j, k, iu, m1) = 0.0;
+              continue;
+            }
+            AS(i, j, k, iu, m1) = fld;
+            AP(i, j, k) += fld;
+            g0 += fld * FZ(i2, j2, k2);
+          }
+        }
+        double vol = VOL1(i, j, k, iu1, ffl);
+        BS(i, j, k) *= vol;
+        AP(i, j, k) += (0.0 - 0.0 - SFP(i, j, k)) * vol;
+        if (fabs(AP(i, j, k)) > small20) UG(i, j, k, iu1) = (g0 + BS(i, j, k)) / AP(i, j, k);
+      }
+}
+
+/* ======================================================================
+ * Boundary extrapolation, C/Extr.f90
+ * ====================================================================== */
+static void EXTR1(double *f1, double *f2, int i, int j, int k, int i1, int j1, int k1, int i2, int j2, int k2, int ix) {
+  *f2 = 0.0;
+  int ibc0 = imod10(IBCELL(i, j, k));
+  if (ix == 3) {
+    double dzk = Z(k2) - Z(k1);
+    if (dzk != 0.0) { int k0 = (ibc0 == 2) ? (k + k1) / 2 : k; *f2 = (Z(k0) - Z(k1)) / dzk; }
+  } else if (ix == 2) {
+    double drj = R(j2) - R(j1);
+    if (drj != 0.0) { int j0 = (ibc0 == 2) ? (j + j1) / 2 : j; *f2 = (R(j0) - R(j1)) / drj; }
+  } else if (ix == 1) {
+    double dxi = X(i2) - X(i1);
+    if (dxi != 0.0) { int i0 = (ibc0 == 2) ? (i + i1) / 2 : i; *f2 = (X(i0) - X(i1)) / dxi; }
+  }
+  *f1 = 1.0 - *f2;
+}
+/* EXT0 / EXT1: extrapolation stencil of a boundary cell, C/Extr.f90:575-661.
+ * mode 0 looks for an open neighbour, mode 1 for a non-wall neighbour.
+ * Unset outputs keep the caller's previous values, as the Fortran does. */
+static void EXTIDX(int mode, int i, int j, int k, int *i1, int *j1, int *k1, int *i2, int *j2, int *k2, int *ix) {
+  for (int d = 1; d <= 3; d++) {
+    int nx[3] = {0, 0, 0};
+    nx[d - 1] = 1;
+    for (int m = -2; m <= 2; m += 4) {
+      *i1 = imax(2, imin(MP, i + m * nx[0]));
+      *j1 = imax(2, imin(NP, j + m * nx[1]));
+      *k1 = imax(2, imin(LP, k + m * nx[2]));
+      int ok = mode == 0 ? (IBCELL(*i1, *j1, *k1) <= 0) : (imod10(IBCELL(*i1, *j1, *k1)) != 1);
+      if (ok) {
+        *i2 = imax(2, imin(MP, *i1 + m * nx[0]));
+        *j2 = imax(2, imin(NP, *j1 + m * nx[1]));
+        *k2 = imax(2, imin(LP, *k1 + m * nx[2]));
+        if (imod10(IBCELL(*i2, *j2, *k2)) == 1) { *i2 = *i1; *j2 = *j1; *k2 = *k1; }
+        *ix = d;
+        return;
+      }
+    }
+  }
+  *ix = 4;
+}
+/* SBC4: copy a cell's state to a neighbour position, C/Sbc.f90:506-522 */
+static void SBC4(int i1, int j1, int k1, int i, int j, int k) {
+  P(i, j, k) = P(i1, j1, k1); T(i, j, k) = T(i1, j1, k1);
+  for (int l = 1; l <= 9; l++) GF(i, j, k, l) = GF(i1, j1, k1, l);
+  DNST(i, j, k) = DNST(i1, j1, k1); THETA(i, j, k) = THETA(i1, j1, k1); TMU(i, j, k) = TMU(i1, j1, k1);
+  GLAM(i, j, k) = GLAM(i1, j1, k1); GCP(i, j, k) = GCP(i1, j1, k1); GAMA(i, j, k) = GAMA(i1, j1, k1);
+}
+/* EXTR(1), C/Extr.f90:31-250 */
+static int ex_i1 = 2, ex_j1 = 2, ex_k1 = 2, ex_i2 = 2, ex_j2 = 2, ex_k2 = 2, ex_ix = 1;
+static void EXTR_GAS(void) {
+  C->gfin0 = 0.0;
+  C->q_exhaust = 0.0;
+  for (int i = 2; i <= MP; i += 2)
+    for (int j = 2; j <= NP; j += 2)
+      for (int k = 2; k <= LP; k += 2) {
+        int ibc0 = imod10(IBCELL(i, j, k));
+        if (ibc0 != 2 && ibc0 != 3) continue;
+        EXTIDX(0, i, j, k, &ex_i1, &ex_j1, &ex_k1, &ex_i2, &ex_j2, &ex_k2, &ex_ix);
+        int i1 = ex_i1, j1 = ex_j1, k1 = ex_k1, i2 = ex_i2, j2 = ex_j2, k2 = ex_k2, ix = ex_ix;
+        double f1, f2;
+        EXTR1(&f1, &f2, i, j, k, i1, j1, k1, i2, j2, k2, ix);
+        GF(i, j, k, ieps) = GF(i1, j1, k1, ieps);
+        P(i, j, k) = P(i1, j1, k1) * f1 + P(i2, j2, k2) * f2;
+        if (ibc0 == 2) {
+          double g0 = TMU(i, j, k);
+          THER(i, j, k);
+          TMU(i, j, k) = g0;
+          GAMA(i, j, k) = GAMA(i1, j1, k1);
+          double area = AREA_C(i, j, k, ix);
+          g0 = THETA(i, j, k) * DNST(i, j, k) * UG(i, j, k, ix) * area;
+          C->gfin0 += fabs(g0);
+          continue;
+        }
+        GCP(i, j, k) = GCP(i1, j1, k1) * f1 + GCP(i2, j2, k2) * f2;
+        GLAM(i, j, k) = GLAM(i1, j1, k1) * f1 + GLAM(i2, j2, k2) * f2;
+        TMU(i, j, k) = TMU(i1, j1, k1);
+        GAMA(i, j, k) = GAMA(i1, j1, k1);
+        for (int l = lstar; l <= lend; l++) {
+          if (l == ieps) continue;
+          double g0 = GF(i1, j1, k1, l) * f1 + GF(i2, j2, k2, l) * f2;
+          if (l == ik) GF(i, j, k, ik) = dmax(g0, 1.0e-6);
+          else if (l == ih) GF(i, j, k, l) = g0;
+          else GF(i, j, k, l) = dmax(g0, 0.0);
+        }
+        ENTH_TO_T(i, j, k);
+        DNST_CALC(i, j, k);
+        double area = AREA_C(i, j, k, ix) * C->area0;
+        double g0 = fabs(UG(i, j, k, ix)) * C->ug0 * area;
+        g0 = g0 * THETA(i, j, k) * DNST(i, j, k) * C->dnst0;
+        C->q_exhaust += GF(i, j, k, ih) * C->h_0 * g0;
+        if (ibc0 == 3) SBC4(i, j, k, (i + i1) / 2, (j + j1) / 2, (k + k1) / 2);
+      }
+  for (int i = 2; i <= MP; i += 2)
+    for (int j = 2; j <= NP; j += 2)
+      for (int k = 2; k <= LP; k += 2) {
+        if (imod10(IBCELL(i, j, k)) != 1) continue;
+        EXTIDX(1, i, j, k, &ex_i1, &ex_j1, &ex_k1, &ex_i2, &ex_j2, &ex_k2, &ex_ix);
+        int i1 = ex_i1, j1 = ex_j1, k1 = ex_k1, i2 = ex_i2, j2 = ex_j2, k2 = ex_k2, ix = ex_ix;
+        double f1, f2;
+        EXTR1(&f1, &f2, i, j, k, i1, j1, k1, i2, j2, k2, ix);
+        P(i, j, k) = P(i1, j1, k1) * f1 + P(i2, j2, k2) * f2;
+        GCP(i, j, k) = GCP(i1, j1, k1) * f1 + GCP(i2, j2, k2) * f2;
+        GLAM(i, j, k) = GLAM(i1, j1, k1) * f1 + GLAM(i2, j2, k2) * f2;
+        TMU(i, j, k) = TMU(i2, j2, k2);
+        GAMA(i, j, k) = GAMA(i2, j2, k2);
+        for (int l = lstar; l <= lend; l++)
+          if (l != ih) GF(i, j, k, l) = GF(i1, j1, k1, l);
+        if (C->i1st_rad_done == 0 && !C->isothermal_walls) {
+          GF(i, j, k, ih) = GF(i1, j1, k1, ih);
+          T(i, j, k) = T(i1, j1, k1);
+        }
+        DNST_CALC(i, j, k);
+      }
+}
+/* EXTRV: exit-face velocities and exit mass flow, C/Extr.f90:358-567 */
+static void EXTRV(int mex) {
+  double tarea_mean = 0.0, area_ex = 0.0;
+  if (mex == 0) { C->gfex = 0.0; C->fuel_out = 0.0; C->tmean_ex = 0.0; }
+  /* x-direction */
+  for (int i = 3; i <= MPM1; i += 2)
+    for (int j = 2; j <= NP; j += 2)
+      for (int k = 2; k <= LP; k += 2) {
+        if (imod10(IBCELL(i, j, k)) != 3) continue;
+        int done = 0;
+        for (int m = -1; m <= 1 && !done; m += 2) {
+          if (IBCELL(i + m, j, k) >= 1) continue;
+          int i0 = i - m, i1 = i + 2 * m;
+          if (UG(i1, j, k, 1) * m > 0.0) UG(i1, j, k, 1) = 0.0;
+          double area = AREA_C(i, j, k, 1);
+          area_ex += area;
+          double g0 = THETA(i1, j, k) * DNST(i1, j, k) * UG(i1, j, k, 1) * area;
+          if (mex == 0) {
+            C->gfex += fabs(g0);
+            C->fuel_out += GF(i, j, k, iyf) * fabs(g0) * C->dnst0 * C->ug0 * C->area0;
+            C->tmean_ex += T(i, j, k) * fabs(g0);
+            tarea_mean += T(i, j, k) * area;
+            done = 1; break;
+          }
+          UG(i1, j, k, 1) *= C->factor;
+          g0 *= C->factor;
+          UG(i, j, k, 1) = g0 / (THETA(i, j, k) * DNST(i, j, k) * area);
+          BOUND(&UG(i, j, k, 1), 1, 1);
+          UG(i0, j, k, 1) = UG(i, j, k, 1);
+          done = 1;
+        }
+        if (done) continue;
+        if (j > 2 && IBCELL(i, j - 2, k) <= 0) UG(i, j, k, 1) = UG(i, j - 2, k, 1);
+        else if (j < NP && IBCELL(i, j + 2, k) <= 0) UG(i, j, k, 1) = UG(i, j + 2, k, 1);
+        else if (k > 2 && IBCELL(i, j, k - 2) <= 0) UG(i, j, k, 1) = UG(i, j, k - 2, 1);
+        else if (k < LP && IBCELL(i, j, k + 2) <= 0) UG(i, j, k, 1) = UG(i, j, k + 2, 1);
+      }
+  /* y-direction (loop order J,K,I as in the reference) */
+  for (int j = 3; j <= NPM1; j += 2)
+    for (int k = 2; k <= LP; k += 2)
+      for (int i = 2; i <= MP; i += 2) {
+        if (imod10(IBCELL(i, j, k)) != 3) continue;
+        int done = 0;
+        for (int m = -1; m <= 1 && !done; m += 2) {
+          if (IBCELL(i, j + m, k) >= 1) continue;
+          int j0 = j - m, j1 = j + 2 * m;
+          if (UG(i, j1, k, 2) * m > 0.0) UG(i, j1, k, 2) = 0.0;
+          double area = AREA_C(i, j, k, 2);
+          area_ex += area;
+          double g0 = THETA(i, j1, k) * DNST(i, j1, k) * UG(i, j1, k, 2) * area;
+          if (mex == 0) {
+            C->gfex += fabs(g0);
+            C->fuel_out += GF(i, j, k, iyf) * fabs(g0) * C->dnst0 * C->ug0 * C->area0;
+            C->tmean_ex += T(i, j, k) * fabs(g0);
+            tarea_mean += T(i, j, k) * area;
+            done = 1; break;
+          }
+          UG(i, j1, k, 2) *= C->factor;
+          g0 *= C->factor;
+          UG(i, j, k, 2) = g0 / (THETA(i, j, k) * DNST(i, j, k) * area);
+          BOUND(&UG(i, j, k, 2), 2, 1);
+          UG(i, j0, k, 2) = UG(i, j, k, 2);
+          done = 1;
+        }
+        if (done) continue;
+        if (k > 2 && IBCELL(i, j, k - 2) <= 0) UG(i, j, k, 2) = UG(i, j, k - 2, 2);
+        else if (k < LP && IBCELL(i, j, k + 2) <= 0) UG(i, j, k, 2) = UG(i, j, k + 2, 2);
+        else if (i > 2 && IBCELL(i - 2, j, k) <= 0) UG(i, j, k, 2) = UG(i - 2, j, k, 2);
+        else if (i < MP && IBCELL(i + 2, j, k) <= 0) UG(i, j, k, 2) = UG(i + 2, j, k, 2);
+      }
+  /* z-direction (loop order K,I,J); the reference skips the exit-face
+   * velocity update for z (GOTO 160, C/Extr.f90:529) */
+  for (int k = 3; k <= LPM1; k += 2)
+    for (int i = 2; i <= MP; i += 2)
+      for (int j = 2; j <= NP; j += 2) {
+        if (imod10(IBCELL(i, j, k)) != 3) continue;
+        int done = 0;
+        for (int m = -1; m <= 1 && !done; m += 2) {
+          if (IBCELL(i, j, k + m) >= 1) continue;
+          int k1 = k + 2 * m;
+          if (UG(i, j, k1, 3) * m > 0.0) UG(i, j, k1, 3) = 0.0;
+          double area = AREA_C(i, j, k, 3);
+          area_ex += area;
+          double g0 = THETA(i, j, k1) * DNST(i, j, k1) * UG(i, j, k1, 3) * area;
+          if (mex == 0) {
+            C->gfex += fabs(g0);
+            C->fuel_out += GF(i, j, k, iyf) * fabs(g0) * C->dnst0 * C->ug0 * C->area0;
+            C->tmean_ex += T(i, j, k) * fabs(g0);
+            tarea_mean += T(i, j, k) * area;
+            done = 1; break;
+          }
+          UG(i, j, k1, 3) *= C->factor;
+          done = 1;
+        }
+        if (done) continue;
+        if (i > 2 && IBCELL(i - 2, j, k) <= 0) UG(i, j, k, 3) = UG(i - 2, j, k, 3);
+        else if (i < MP && IBCELL(i + 2, j, k) <= 0) UG(i, j, k, 3) = UG(i + 2, j, k, 3);
+        else if (j > 2 && IBCELL(i, j - 2, k) <= 0) UG(i, j, k, 3) = UG(i, j - 2, k, 3);
+        else if (j < NP && IBCELL(i, j + 2, k) <= 0) UG(i, j, k, 3) = UG(i, j + 2, k, 3);
+      }
+  if (mex == 0) {
+    C->gfex *= C->gmfr;
+    if (C->gfex < small16) C->tmean_ex = tarea_mean * C->t0 / area_ex;
+    else C->tmean_ex = C->tmean_ex * C->t0 * C->gmfr / C->gfex;
+  }
+}
+
+/* ======================================================================
+ * Interpolation to non-centre lattice positions, C/Intp.f90
+ * ====================================================================== */
+static double INTP1(int i, int j, int k, int ix) {
+  if (ix == 3) return (DZ(k) <= 0.0 || k > LP) ? 0.0 : (Z(k) - Z(k - 1)) / DZ(k);
+  if (ix == 2) return (DR(j) <= 0.0 || j > NP) ? 0.0 : (R(j) - R(j - 1)) / DR(j);
+  return (DX(i) <= 0.0 || i > MP) ? 0.0 : (X(i) - X(i - 1)) / DX(i);
+}
+/* INTP0, C/Intp.f90:145-175.  f1 is the caller's weight and is modified
+ * in place at inlet faces, exactly as the Fortran dummy argument. */
+static void INTP0(double *f1, int i, int j, int k, int i1, int j1, int k1, int i2, int j2, int k2) {
+  if (IBCELL(i, j, k) == 3) return;
+  if (IBCELL(i, j, k) == 2) {
+    if (IBCELL(i1, j1, k1) == 2 && IBCELL(i2, j2, k2) == 0) *f1 = 1.0;
+    else if (IBCELL(i1, j1, k1) == 0 && IBCELL(i2, j2, k2) == 2) *f1 = 0.0;
+  }
+  double a = *f1, b = 1.0 - a;
+  P(i, j, k) = a * P(i1, j1, k1) + b * P(i2, j2, k2);
+  T(i, j, k) = a * T(i1, j1, k1) + b * T(i2, j2, k2);
+  for (int l = lstar; l <= lend; l++) GF(i, j, k, l) = a * GF(i1, j1, k1, l) + b * GF(i2, j2, k2, l);
+  CONC(i, j, k);
+  TMU(i, j, k) = a * TMU(i1, j1, k1) + b * TMU(i2, j2, k2);
+  GCP(i, j, k) = a * GCP(i1, j1, k1) + b * GCP(i2, j2, k2);
+  GLAM(i, j, k) = a * GLAM(i1, j1, k1) + b * GLAM(i2, j2, k2);
+  GAMA(i, j, k) = a * GAMA(i1, j1, k1) + b * GAMA(i2, j2, k2);
+  THETA(i, j, k) = a * THETA(i1, j1, k1) + b * THETA(i2, j2, k2);
+  DNST_CALC(i, j, k);
+}
+/* INTP, C/Intp.f90:32-72 */
+static void INTP(void) {
+  for (int i = 2; i <= MP; i += 2) {
+    for (int k = 3; k <= LPM1; k += 2) {
+      double f1 = INTP1(i, 0, k, 3);
+      for (int j = 2; j <= NP; j += 2) {
+        if (IBCELL(i, j, k) == 1) continue;
+        INTP0(&f1, i, j, k, i, j, k + 1, i, j, k - 1);
+      }
+    }
+    for (int j = 3; j <= NPM1; j += 2) {
+      double f1 = INTP1(i, j, 0, 2);
+      for (int k = 2; k <= LP; k++) {
+        if (IBCELL(i, j, k) == 1) continue;
+        INTP0(&f1, i, j, k, i, j + 1, k, i, j - 1, k);
+      }
+    }
+  }
+  for (int i = 3; i <= MPM1; i += 2) {
+    double f1 = INTP1(i, 0, 0, 1);
+    for (int k = 2; k <= LP; k++)
+      for (int j = 2; j <= NP; j++) {
+        if (IBCELL(i, j, k) == 1) continue;
+        INTP0(&f1, i, j, k, i + 1, j, k, i - 1, j, k);
+      }
+  }
+}
+/* INTPV1, C/Intp.f90:230-272 */
+static void INTPV1(int m, int n, int l, int iu, int ix) {
+  int i, j, k;
+  if (iu == 1) { i = m; j = n; k = l; }
+  else if (iu == 2) { j = m; k = n; i = l; }
+  else { k = m; i = n; j = l; }
+  int b = IBCELL(i, j, k);
+  if (b >= 1 && b != 4) return;
+  double f1 = INTP1(i, j, k, ix), f2 = 1.0 - f1;
+  if (ix == 1) {
+    UG(i, j, k, iu) = UG(i + 1, j, k, iu) * f1 + UG(i - 1, j, k, iu) * f2;
+  } else if (ix == 2) {
+    if (IBCELL(i, j + 1, k) == 2 && b == 0) { f1 = 0.0; f2 = 1.0; }
+    else if (IBCELL(i, j - 1, k) == 2 && b == 0) { f1 = 1.0; f2 = 0.0; }
+    UG(i, j, k, iu) = UG(i, j + 1, k, iu) * f1 + UG(i, j - 1, k, iu) * f2;
+  } else {
+    if (IBCELL(i, j, k + 1) == 2 && b == 0) { f1 = 0.0; f2 = 1.0; }
+    else if (IBCELL(i, j, k - 1) == 2 && b == 0) { f1 = 1.0; f2 = 0.0; }
+    UG(i, j, k, iu) = UG(i, j, k + 1, iu) * f1 + UG(i, j, k - 1, iu) * f2;
+  }
+  BOUND(&UG(i, j, k, iu), iu, 1);
+}
+/* INTPV, C/Intp.f90:84-139 */
+static void INTPV(void) {
+  for (int iu = 1; iu <= 3; iu++) {
+    int m3, n3, l3;
+    if (iu == 1) { m3 = MP; n3 = NP; l3 = LP; }
+    else if (iu == 2) { m3 = NP; n3 = LP; l3 = MP; }
+    else { m3 = LP; n3 = MP; l3 = NP; }
+    int ixa = (iu == 1) ? 2 : (iu == 2) ? 3 : 1;
+    int ixb = (iu == 1) ? 3 : (iu == 2) ? 1 : 2;
+    for (int m = 3; m <= m3 - 1; m += 2) {
+      if (n3 > 3)
+        for (int l = 2; l <= l3; l += 2)
+          for (int n = 3; n <= n3 - 1; n += 2) INTPV1(m, n, l, iu, ixa);
+      if (l3 <= 3) continue;
+      for (int l = 3; l <= l3 - 1; l += 2)
+        for (int n = 2; n <= n3; n++) INTPV1(m, n, l, iu, ixb);
+    }
+    for (int m = 4; m <= m3; m += 2)
+      for (int n = 2; n <= n3; n++)
+        for (int l = 2; l <= l3; l++) INTPV1(m, n, l, iu, iu);
+  }
+}
+
+/* ======================================================================
+ * GSLV3V, C/Gslv3v.f90:7-236
+ * ====================================================================== */
+static void GSLV3V(void) {
+  SGM = 1.0;
+  for (size_t q = 0; q < NN; q++) AP_[q] = 1.0;
+  memset(AS_, 0, sizeof(double) * NN * 6);
+  for (int iu1 = 1; iu1 <= 3; iu1++) {
+    int nx[3] = {0, 0, 0};
+    nx[iu1 - 1] = 1;
+    for (int i = 2 + nx[0]; i <= MP - nx[0]; i += 2)
+      for (int j = 2 + nx[1]; j <= NP - nx[1]; j += 2)
+        for (int k = 2 + nx[2]; k <= LP - nx[2]; k += 2) FZ(i, j, k) = UG(i, j, k, iu1);
+  }
+  for (int iu1 = 1; iu1 <= 3; iu1++) { NELG = iu1; GSLVX1(iu1); }
+
+  /* pressure equation */
+  NELG = 4;
+  int mi1 = 4, mi3 = MPM2, nj1 = NJY1, nj3 = NJY2, lk1 = LKZ1, lk3 = LKZ2;
+  SOURCE(mi1, mi3, nj1, nj3, lk1, lk3);
+  for (int i = mi1; i <= mi3; i += 2)
+    for (int j = nj1; j <= nj3; j += 2)
+      for (int k = lk1; k <= lk3; k += 2) {
+        if (IBCELL(i, j, k) >= 1) { BS(i, j, k) = P(i, j, k); continue; }
+        AP(i, j, k) = -SFP(i, j, k);
+        for (int m = 1; m <= 2; m++) {
+          int i1 = (m == 1) ? i - 1 : i + 1;
+          if (IBCELL(i1, j, k) <= 0 && fabs(AP(i1, j, k)) > small20) {
+            double area = AREA_C(i, j, k, 1);
+            AS(i, j, k, 1, m) = THETA(i1, j, k) * THETA(i1, j, k) * DNST(i1, j, k) * area * area / AP(i1, j, k) * DXAV(i1, j, k) / DX(i1);
+          }
+          AP(i, j, k) += AS(i, j, k, 1, m);
+        }
+        for (int m = 1; m <= 2; m++) {
+          int j1 = (m == 1) ? j - 1 : j + 1;
+          if (IBCELL(i, j1, k) <= 0 && fabs(AP(i, j1, k)) > small20) {
+            double area = AREA_C(i, j, k, 2);
+            AS(i, j, k, 2, m) = THETA(i, j1, k) * THETA(i, j1, k) * DNST(i, j1, k) * area * area / AP(i, j1, k) * DRAV(i, j1, k) / DR(j1);
+          }
+          AP(i, j, k) += AS(i, j, k, 2, m);
+        }
+        for (int m = 1; m <= 2; m++) {
+          int k1 = (m == 1) ? k - 1 : k + 1;
+          if (IBCELL(i, j, k1) <= 0 && fabs(AP(i, j, k1)) > small20) {
+            double area = AREA_C(i, j, k, 3);
+            AS(i, j, k, 3, m) = THETA(i, j, k1) * THETA(i, j, k1) * DNST(i, j, k1) * area * area / AP(i, j, k1) * DZAV(i, j, k1) / DZ(k1);
+          }
+          AP(i, j, k) += AS(i, j, k, 3, m);
+        }
+      }
+  if (C->want_pre_residuals) C->p_resid_pre = CALC_RESIDUAL(P_, mi1, mi3, nj1, nj3, lk1, lk3);
+  ADLBL3(P_, mi1, mi3, nj1, nj3, lk1, lk3);
+  C->p_resid = CALC_RESIDUAL(P_, mi1, mi3, nj1, nj3, lk1, lk3);
+  for (int i = mi1; i <= mi3; i += 2)
+    for (int j = nj1; j <= nj3; j += 2)
+      for (int k = lk1; k <= lk3; k += 2)
+        if (IBCELL(i, j, k) < 1) BOUND(&P(i, j, k), 0, 1);
+
+  /* starred velocities */
+  for (int iu1 = 1; iu1 <= 3; iu1++) {
+    NELG = iu1;
+    int nx[3] = {0, 0, 0};
+    nx[iu1 - 1] = 1;
+    mi1 = 4 + nx[0]; mi3 = MPM2 - nx[0];
+    nj1 = NJY1V[iu1]; nj3 = NJY2V[iu1]; lk1 = LKZ1V[iu1]; lk3 = LKZ2V[iu1];
+    for (int i = mi1; i <= mi3; i += 2)
+      for (int j = nj1; j <= nj3; j += 2)
+        for (int k = lk1; k <= lk3; k += 2) {
+          if (IBCELL(i, j, k) >= 1) continue;
+          AP(i, j, k) = AP(i, j, k) / C->rf[NELG - 1];
+          double vol = VOL_C(i, j, k), dpdx;
+          if (iu1 == 1) dpdx = (P(i - 1, j, k) - P(i + 1, j, k)) / DX(i);
+          else if (iu1 == 2) dpdx = (P(i, j - 1, k) - P(i, j + 1, k)) / DR(j) / RS(j);
+          else dpdx = (P(i, j, k - 1) - P(i, j, k + 1)) / DZ(k);
+          BS(i, j, k) = BS(i, j, k) + C->eun * THETA(i, j, k) * vol * dpdx + (1.0 - C->rf[NELG - 1]) * AP(i, j, k) * FZ(i, j, k);
+        }
+    if (C->want_pre_residuals) C->u_resid_pre[iu1 - 1] = CALC_RESIDUAL(FZ_, mi1, mi3, nj1, nj3, lk1, lk3);
+    ADLBL3(FZ_, mi1, mi3, nj1, nj3, lk1, lk3);
+    C->u_resid[iu1 - 1] = CALC_RESIDUAL(FZ_, mi1, mi3, nj1, nj3, lk1, lk3);
+    for (int i = mi1; i <= mi3; i += 2)
+      for (int j = nj1; j <= nj3; j += 2)
+        for (int k = lk1; k <= lk3; k += 2) {
+          if (IBCELL(i, j, k) >= 1) continue;
+          BOUND(&FZ(i, j, k), iu1, 1);
+          UG(i, j, k, iu1) = FZ(i, j, k);
+        }
+  }
+
+  /* pressure correction */
+  NELG = 5;
+  mi1 = 4; mi3 = MPM2; nj1 = NJY1; nj3 = NJY2; lk1 = LKZ1; lk3 = LKZ2;
+  SOURCE(mi1, mi3, nj1, nj3, lk1, lk3);
+  for (int i = 2; i <= MP; i += 2)
+    for (int j = 2; j <= NP; j += 2)
+      for (int k = 2; k <= LP; k += 2) FZ(i, j, k) = 0.0;
+  ADLBL3(FZ_, mi1, mi3, nj1, nj3, lk1, lk3);
+  C->pcorr_resid = CALC_RESIDUAL(FZ_, mi1, mi3, nj1, nj3, lk1, lk3);
+
+  /* velocity correction */
+  for (int iu1 = 1; iu1 <= 3; iu1++) {
+    int nx[3] = {0, 0, 0};
+    nx[iu1 - 1] = 1;
+    mi1 = 4 + nx[0]; mi3 = MPM2 - nx[0];
+    nj1 = NJY1V[iu1]; nj3 = NJY2V[iu1]; lk1 = LKZ1V[iu1]; lk3 = LKZ2V[iu1];
+    for (int i = mi1; i <= mi3; i += 2)
+      for (int j = nj1; j <= nj3; j += 2)
+        for (int k = lk1; k <= lk3; k += 2) {
+          if (IBCELL(i, j, k) >= 1 || fabs(AP(i, j, k)) < small20) continue;
+          double vol = VOL_C(i, j, k), dpdx;
+          if (iu1 == 1) dpdx = (FZ(i - 1, j, k) - FZ(i + 1, j, k)) / DX(i);
+          else if (iu1 == 2) dpdx = (FZ(i, j - 1, k) - FZ(i, j + 1, k)) / DR(j) / RS(j);
+          else dpdx = (FZ(i, j, k - 1) - FZ(i, j, k + 1)) / DZ(k);
+          double g0 = C->eun * THETA(i, j, k) * dpdx * vol / (AP(i, j, k) * C->rf[iu1 - 1]);
+          UG(i, j, k, iu1) += g0;
+          BOUND(&UG(i, j, k, iu1), iu1, 1);
+        }
+  }
+  EXTRV(0);
+  C->gfin = C->gfin0 * C->gmfr; /* FLX1D, C/Flx1d.f90:75-77 */
+  if (C->itr_gas > 10) {
+    if (C->gfex < small20) { C->factor = 1.0; C->fadj = 0.0; }
+    else {
+      C->factor = C->gfin / C->gfex;
+      if (C->factor > C->f_mx) { C->factor = C->f_mx; C->fadj = 0.0; }
+      else if (C->factor < C->f_mn) { C->factor = C->f_mn; C->fadj = 0.0; }
+    }
+  }
+  EXTRV(1);
+  INTPV();
+}
+
+/* ======================================================================
+ * GSLV3G / GSLV3G0, C/Gslv3g.f90
+ * ====================================================================== */
+static void GSLV3G0(int mi1, int mi3, int nj1, int nj3, int lk1, int lk3, int igs) {
+  for (int i = mi1; i <= mi3; i += 2)
+    for (int j = nj1; j <= nj3; j += 2)
+      for (int k = lk1; k <= lk3; k += 2) {
+        if (IBCELL(i, j, k) >= 1) continue;
+        AP(i, j, k) = 0.0;
+        for (int iu = 1; iu <= 3; iu++)
+          for (int m = -1; m <= 1; m += 2) {
+            int m1 = (m + 3) / 2;
+            if (igs != 1) {
+              int i1, j1, k1, i2, j2, k2;
+              VINDX(iu, m, i, j, k, &i1, &j1, &k1, &i2, &j2, &k2);
+              int ibc0 = IBCELL(i2, j2, k2);
+              if (ibc0 <= 0) AS(i, j, k, iu, m1) = FLWD(i1, j1, k1, iu, m1);
+              else if (ibc0 == 2) AS(i, j, k, iu, m1) = FLW(i2, j2, k2, iu, m1);
+              else AS(i, j, k, iu, m1) = 0.0;
+            }
+            AP(i, j, k) += AS(i, j, k, iu, m1);
+          }
+      }
+}
+static void GSLV3G(void) {
+  int mi1 = 4, mi3 = MPM2, nj1 = NJY1, nj3 = NJY2, lk1 = LKZ1, lk3 = LKZ2;
+  for (int l = lstar; l <= lend; l++) {
+    if (l == iyf) NELG = 6;
+    else if (l == iyco2) NELG = 7;
+    else if (l == ih) NELG = 8;
+    else if (l == ik) NELG = 9;
+    else if (l == ieps) NELG = 10;
+    else if (l == iyn2) NELG = 11;
+    else if (l == iyo2) NELG = 12;
+    else continue;
+    for (int i = 2; i <= MP; i += 2)
+      for (int j = 2; j <= NP; j += 2)
+        for (int k = 2; k <= LP; k += 2) FZ(i, j, k) = GF(i, j, k, l);
+    SOURCE(mi1, mi3, nj1, nj3, lk1, lk3);
+    if (l == lstar || SGM != sigma_eq[l - 1]) { SGM = sigma_eq[l - 1]; GSLV3G0(mi1, mi3, nj1, nj3, lk1, lk3, 0); }
+    else GSLV3G0(mi1, mi3, nj1, nj3, lk1, lk3, 1);
+    for (int i = mi1; i <= mi3; i += 2)
+      for (int j = nj1; j <= nj3; j += 2)
+        for (int k = lk1; k <= lk3; k += 2) {
+          if (IBCELL(i, j, k) >= 1) continue;
+          if (fabs(AP(i, j, k)) <= small20) { AP(i, j, k) = 1.0; BS(i, j, k) = FZ(i, j, k); continue; }
+          double vol = VOL_C(i, j, k);
+          BS(i, j, k) *= vol;
+          AP(i, j, k) = AP(i, j, k) - (SFP(i, j, k) * vol);
+          AP(i, j, k) = AP(i, j, k) / C->rfl[l - 1];
+          BS(i, j, k) = BS(i, j, k) + (1.0 - C->rfl[l - 1]) * FZ(i, j, k) * AP(i, j, k);
+        }
+    if (C->want_pre_residuals) {
+      C->gf_resid_pre[l - 1] = CALC_RESIDUAL(FZ_, mi1, mi3, nj1, nj3, lk1, lk3);
+      if (NELG == 8) C->h_resid_pre = C->gf_resid_pre[l - 1];
+    }
+    ADLBL3(FZ_, mi1, mi3, nj1, nj3, lk1, lk3);
+    C->gf_resid[l - 1] = CALC_RESIDUAL(FZ_, mi1, mi3, nj1, nj3, lk1, lk3);
+    for (int i = mi1; i <= mi3; i += 2)
+      for (int j = nj1; j <= nj3; j += 2)
+        for (int k = lk1; k <= lk3; k += 2) {
+          if (IBCELL(i, j, k) >= 1) continue;
+          BOUND(&FZ(i, j, k), l, 2);
+          GF(i, j, k, l) = FZ(i, j, k);
+        }
+  }
+  C->h_resid = C->gf_resid[ih - 1];
+}
+
+/* ======================================================================
+ * GFLOW, C/Gflow.f90:6-408
+ * ====================================================================== */
+static void T_AVE_CALC(void) {
+  double tmax = 0.0, tave = 0.0, tvol = 0.0, rvt = 0.0, vt = 0.0;
+  for (int k = 4; k <= LP - 2; k += 2)
+    for (int j = 4; j <= NP - 2; j += 2)
+      for (int i = 4; i <= MP - 2; i += 2) {
+        if (IBCELL(i, j, k) != 0) continue;
+        double vol = VOL_C(i, j, k);
+        tave += T(i, j, k) * DNST(i, j, k) * vol;
+        rvt += DNST(i, j, k) * vol;
+        tvol += T(i, j, k) * vol;
+        vt += vol;
+        tmax = dmax(tmax, T(i, j, k));
+      }
+  C->tave = tave * C->t0 / rvt;
+  C->tave_vol = tvol * C->t0 / vt;
+  C->tmax_c = tmax * C->t0;
+}
+static void GFLOW(void) {
+  /* theta next to walls */
+  for (int i = 2; i <= MP; i++)
+    for (int j = 2; j <= NP; j++)
+      for (int k = 2; k <= LP; k++) {
+        if (IBCELL(i, j, k) >= 1) continue;
+        double g1, g2;
+        if (IBCELL(i - 1, j, k) == 1) { g1 = (X(i) - X(i - 1)) * (X(i) - X(i - 1)); g2 = (X(i + 2) - X(i - 1)) * (X(i + 2) - X(i - 1)); THETA(i - 1, j, k) = THETA(i, j, k) * (g2 / (g2 - g1)) + THETA(i + 2, j, k) * (g1 / (g1 - g2)); }
+        if (IBCELL(i + 1, j, k) == 1) { g1 = (X(i) - X(i + 1)) * (X(i) - X(i + 1)); g2 = (X(i - 2) - X(i + 1)) * (X(i - 2) - X(i + 1)); THETA(i + 1, j, k) = THETA(i, j, k) * (g2 / (g2 - g1)) + THETA(i - 2, j, k) * (g1 / (g1 - g2)); }
+        if (IBCELL(i, j - 1, k) == 1) { g1 = (R(j) - R(j - 1)) * (R(j) - R(j - 1)); g2 = (R(j + 2) - R(j - 1)) * (R(j + 2) - R(j - 1)); THETA(i, j - 1, k) = THETA(i, j, k) * (g2 / (g2 - g1)) + THETA(i, j + 2, k) * (g1 / (g1 - g2)); }
+        if (IBCELL(i, j + 1, k) == 1) { g1 = (R(j) - R(j + 1)) * (R(j) - R(j + 1)); g2 = (R(j - 2) - R(j + 1)) * (R(j - 2) - R(j + 1)); THETA(i, j + 1, k) = THETA(i, j, k) * (g2 / (g2 - g1)) + THETA(i, j - 2, k) * (g1 / (g1 - g2)); }
+        if (IBCELL(i, j, k - 1) == 1) { g1 = (Z(k) - Z(k - 1)) * (Z(k) - Z(k - 1)); g2 = (Z(k + 2) - Z(k - 1)) * (Z(k + 2) - Z(k - 1)); THETA(i, j, k - 1) = THETA(i, j, k) * (g2 / (g2 - g1)) + THETA(i, j, k + 2) * (g1 / (g1 - g2)); }
+        if (IBCELL(i, j, k + 1) == 1) { g1 = (Z(k) - Z(k + 1)) * (Z(k) - Z(k + 1)); g2 = (Z(k - 2) - Z(k + 1)) * (Z(k - 2) - Z(k + 1)); THETA(i, j, k + 1) = THETA(i, j, k) * (g2 / (g2 - g1)) + THETA(i, j, k - 2) * (g1 / (g1 - g2)); }
+      }
+  for (size_t q = 0; q < NN; q++)
+    if (IBC[q] == 1) GAMA_[q] = 0.0;
+  double rt20 = C->reyg2 * sqrt(C->epsr) * pow(cmu, 0.25);
+  for (int i = 2; i <= MP; i++)
+    for (int j = 2; j <= NP; j++)
+      for (int k = 2; k <= LP; k++) {
+        if (IBCELL(i, j, k) >= 1) continue;
+        double c = 1.0;
+        double blvm = rt20 * DNST(i, j, k) * sqrt(GF(i, j, k, ik)) / c;
+        double vgam = c / C->reyg2;
+        if (IBCELL(i - 1, j, k) == 1) { double g = vgam * oracle_tgamf(blvm * (X(i) - X(i - 1))); GAMA(i - 1, j, k) = dmax(g, GAMA(i - 1, j, k)); TMU(i - 1, j, k) = C->reyg2 * GAMA(i - 1, j, k); TMU(i - 2, j, k) = TMU(i - 1, j, k); }
+        if (IBCELL(i + 1, j, k) == 1) { double g = vgam * oracle_tgamf(blvm * (X(i + 1) - X(i))); GAMA(i + 1, j, k) = dmax(g, GAMA(i + 1, j, k)); TMU(i + 1, j, k) = C->reyg2 * GAMA(i + 1, j, k); TMU(i + 2, j, k) = TMU(i + 1, j, k); }
+        if (IBCELL(i, j - 1, k) == 1) { double g = vgam * oracle_tgamf(blvm * (R(j) - R(j - 1))); GAMA(i, j - 1, k) = dmax(g, GAMA(i, j - 1, k)); TMU(i, j - 1, k) = C->reyg2 * GAMA(i, j - 1, k); TMU(i, j - 2, k) = TMU(i, j - 1, k); }
+        if (IBCELL(i, j + 1, k) == 1) { double g = vgam * oracle_tgamf(blvm * (R(j + 1) - R(j))); GAMA(i, j + 1, k) = dmax(g, GAMA(i, j + 1, k)); TMU(i, j + 1, k) = C->reyg2 * GAMA(i, j + 1, k); TMU(i, j + 2, k) = TMU(i, j + 1, k); }
+        if (IBCELL(i, j, k - 1) == 1) { double g = vgam * oracle_tgamf(blvm * (Z(k) - Z(k - 1))); GAMA(i, j, k - 1) = dmax(g, GAMA(i, j, k - 1)); TMU(i, j, k - 1) = C->reyg2 * GAMA(i, j, k - 1); TMU(i, j, k - 2) = TMU(i, j, k - 1); }
+        if (IBCELL(i, j, k + 1) == 1) { double g = vgam * oracle_tgamf(blvm * (Z(k + 1) - Z(k))); GAMA(i, j, k + 1) = dmax(g, GAMA(i, j, k + 1)); TMU(i, j, k + 1) = C->reyg2 * GAMA(i, j, k + 1); TMU(i, j, k + 2) = TMU(i, j, k + 1); }
+      }
+
+  GSLV3V();
+
+  /* pressure referencing */
+  int i = C->i_ref, j = C->j_ref, k = C->k_ref;
+  int i1 = i, j1 = j, k1 = k, i2 = i, j2 = j, k2 = k, nx = 0;
+  if (i < MPM2 && IBCELL(i + 2, j, k) == 0) { nx = 1; i1 = i + 2; i2 = (IBCELL(i + 4, j, k) == 0) ? i + 4 : i1; }
+  else if (i > 2 && IBCELL(i - 2, j, k) == 0) { nx = 1; i1 = i - 2; i2 = (IBCELL(i - 4, j, k) == 0) ? i - 4 : i1; }
+  else if (j < NPM2 && IBCELL(i, j + 2, k) == 0) { nx = 2; j1 = j + 2; j2 = (j < NPM2 && IBCELL(i, j + 4, k) <= 0) ? j + 4 : j1; }
+  else if (j > 2 && IBCELL(i, j - 2, k) <= 0) { nx = 2; j1 = j - 2; j2 = (j > 4 && IBCELL(i, j - 4, k) <= 0) ? j - 4 : j1; }
+  else if (k < LPM2 && IBCELL(i, j, k + 2) <= 0) { nx = 3; k1 = k + 2; k2 = (k < LPM2 && IBCELL(i, j, k + 4) <= 0) ? k + 4 : k1; }
+  else if (k > 2 && IBCELL(i, j, k - 2) <= 0) { nx = 3; k1 = k - 2; k2 = (k > 4 && IBCELL(i, j, k - 4) <= 0) ? k - 4 : k1; }
+  double f1, f2;
+  EXTR1(&f1, &f2, i, j, k, i1, j1, k1, i2, j2, k2, nx);
+  P(i, j, k) = P(i1, j1, k1) * f1 + P(i2, j2, k2) * f2;
+  double g0 = P(i1, j1, k1) - C->p_ref;
+
+  GSLV3G();
+
+  for (int ii = 2; ii <= MP; ii += 2)
+    for (int jj = 2; jj <= NP; jj += 2)
+      for (int kk = 2; kk <= LP; kk += 2) {
+        if (IBCELL(ii, jj, kk) >= 1) continue;
+        CONC(ii, jj, kk);
+        ENTH_TO_T(ii, jj, kk);
+        if (T(ii, jj, kk) > C->t_mx) { T(ii, jj, kk) = C->t_mx; T_TO_ENTH(ii, jj, kk); }
+        else if (T(ii, jj, kk) < C->t_mn) { T(ii, jj, kk) = C->t_mn; T_TO_ENTH(ii, jj, kk); }
+        P(ii, jj, kk) -= g0;
+        BOUND(&P(ii, jj, kk), 0, 1);
+        DNST_CALC(ii, jj, kk);
+        THER(ii, jj, kk);
+      }
+  T_AVE_CALC();
+  EXTR_GAS();
+  INTP();
+}
+
+/* exit temperature reductions of SPHASE, C/Sphase.f90:74-89 */
+static void EXIT_T(void) {
+  double g1 = 0.0, tav = 0.0, tx = 0.0;
+  for (int i = 2; i <= MP; i += 2)
+    for (int j = 2; j <= NP; j += 2)
+      for (int k = 2; k <= LP; k += 2) {
+        if (IBCELL(i, j, k) != 3) continue;
+        double g0 = AREA_C(i, j, k, 3);
+        if (g0 < small20) g0 = AREA_C(i, j, k, 2);
+        if (g0 < small20) g0 = AREA_C(i, j, k, 1);
+        g1 += g0;
+        tav += T(i, j, k) * g0;
+        if (T(i, j, k) > tx) tx = T(i, j, k);
+      }
+  C->t_av_exit = (g1 > 0.0) ? tav * C->t0 / g1 : 0.0;
+  C->t_x_exit = tx * C->t0;
+}
+
+/* ======================================================================
+ * Binding and entry points
+ * ====================================================================== */
+static int bind(gfm_ctrl *c, const gfm_host_arrays *h) {
+  C = c;
+  MP = c->mp; NP = c->np; LP = c->lp; NR0 = c->nr0;
+  if (MP < 8 || NP < 8 || LP < 8 || (MP & 1) || (NP & 1) || (LP & 1)) return GFM_E_ARG;
+  MPM1 = MP - 1; MPM2 = MP - 2; NPM1 = NP - 1; NPM2 = NP - 2; LPM1 = LP - 1; LPM2 = LP - 2;
+  NN = (size_t)MP * NP * LP;
+  NJY1 = 4; NJY2 = NPM2; NJY1V[1] = 4; NJY2V[1] = NPM2; NJY1V[2] = 5; NJY2V[2] = NP - 3; NJY1V[3] = 4; NJY2V[3] = NPM2;
+  LKZ1 = 4; LKZ2 = LPM2; LKZ1V[1] = 4; LKZ2V[1] = LPM2; LKZ1V[2] = 4; LKZ2V[2] = LPM2; LKZ1V[3] = 5; LKZ2V[3] = LP - 3;
+  X_ = h->x; DX_ = h->dx; R_ = h->r; DR_ = h->dr; RS_ = h->rs; Z_ = h->z; DZ_ = h->dz;
+  IBC = h->ibcell;
+  P_ = h->p; T_ = h->t; DNST_ = h->dnst; THETA_ = h->theta; TMU_ = h->tmu; GCP_ = h->gcp; GLAM_ = h->glam;
+  GAMA_ = h->gama; GTE_ = h->gte; CRATE_ = h->crate; HRATE_ = h->hrate; BSH_ = h->bs_hprev;
+  UG_ = h->ug; GF_ = h->gf;
+  RTIM_ = h->rtim; DELRT_ = h->delrt; HRT_ = h->hrt; DELR_ = h->delr; TIMED_ = h->timed; HRD_ = h->hrd;
+  wt0 = 1.0 / (0.5 / wf + 0.0 / wco2 + 0.0 / wh2o + 0.49 / wo2 + 0.01 / wn2);
+  stoich = 2.0 * wo2 / wf;
+  for (size_t q = 0; q < NN; q++)
+    if (IBC[q] < 0) return GFM_E_ARG;
+  static size_t cap = 0;
+  if (cap != NN) {
+    free(FZ_); free(BS_); free(SFP_); free(AP_); free(AS_); free(XT2_); free(XT3_);
+    FZ_ = calloc(NN, sizeof(double)); BS_ = calloc(NN, sizeof(double)); SFP_ = calloc(NN, sizeof(double));
+    AP_ = calloc(NN, sizeof(double)); AS_ = calloc(NN * 6, sizeof(double));
+    XT2_ = calloc((size_t)(NP + 4) * (LP + 4), sizeof(double)); XT3_ = calloc((size_t)(NP + 4) * (LP + 4), sizeof(double));
+    if (!FZ_ || !BS_ || !SFP_ || !AP_ || !AS_ || !XT2_ || !XT3_) return GFM_E_DEVICE;
+    cap = NN;
+  }
+  return GFM_OK;
+}
+
+/* tail of INITSV, C/Initsv.f90:63-84 */
+int oracle_initsv_tail(gfm_ctrl *c, const gfm_host_arrays *h) {
+  int rc = bind(c, h);
+  if (rc) return rc;
+  INTPV();
+  EXTR_GAS();
+  INTP();
+  return GFM_OK;
+}
+
+/* n_iters GFLOW calls with the SPHASE loop control (C/Sphase.f90:37-235).
+ * res_hist (may be NULL) receives res_mass per iteration.  Returns the
+ * number of iterations run (negative on error). */
+int oracle_sphase(gfm_ctrl *c, const gfm_host_arrays *h, int n_iters, double *res_hist) {
+  int rc = bind(c, h);
+  if (rc) return -rc;
+  int n = 0;
+  while (n < n_iters && c->itr_gas < c->itr_gend && c->itr_gas < c->maxgi) {
+    c->itr_gas += 1;
+    GFLOW();
+    EXIT_T();
+    if (res_hist) res_hist[n] = c->res_mass;
+    n++;
+    if (c->res_mass < c->bgcon) { c->itr_gend = c->itr_gas; break; }
+  }
+  return n;
+}
+
+/* ADLBL3 on caller arrays (ap, as(…,3,2), bs, xt on the lattice). */
+int oracle_adlbl3(gfm_ctrl *c, const gfm_host_arrays *h, int nelg, const double *ap, const double *as,
+                  const double *bs, double *xt, int ib, int ie, int jb, int je, int kb, int ke) {
+  int rc = bind(c, h);
+  if (rc) return rc;
+  memcpy(AP_, ap, sizeof(double) * NN);
+  memcpy(AS_, as, sizeof(double) * NN * 6);
+  memcpy(BS_, bs, sizeof(double) * NN);
+  NELG = nelg;
+  ADLBL3(xt, ib, ie, jb, je, kb, ke);
+  return GFM_OK;
+}
+double oracle_calc_residual(gfm_ctrl *c, const gfm_host_arrays *h, const double *ap, const double *as,
+                            const double *bs, const double *xt, int ib, int ie, int jb, int je, int kb, int ke) {
+  if (bind(c, h)) return -1.0;
+  memcpy(AP_, ap, sizeof(double) * NN);
+  memcpy(AS_, as, sizeof(double) * NN * 6);
+  memcpy(BS_, bs, sizeof(double) * NN);
+  return CALC_RESIDUAL(xt, ib, ie, jb, je, kb, ke);
+}
+/* Expose the last assembled system of the named stage for the staged
+ * parity tests: copies AP/AS/BS (and FZ) out. */
+void oracle_scratch(double *ap, double *as, double *bs, double *fz) {
+  if (ap) memcpy(ap, AP_, sizeof(double) * NN);
+  if (as) memcpy(as, AS_, sizeof(double) * NN * 6);
+  if (bs) memcpy(bs, BS_, sizeof(double) * NN);
+  if (fz) memcpy(fz, FZ_, sizeof(double) * NN);
+}
+int oracle_abi_version(void) { return GFM_ABI_VERSION; }
+size_t oracle_ctrl_size(void) { return sizeof(gfm_ctrl); }
