@@ -1,0 +1,110 @@
+#!/usr/bin/env python3
+"""Benchmark of the combustion-space SIMPLER path on one MI355X.
+
+A step is one SIMPLER outer iteration (one GFLOW: momentum, pressure, p',
+velocity correction, 7 scalar equations, properties, boundary updates)
+over the 192x96x64 stretched furnace (1,179,648 cells), all state resident
+in HBM.  Prints one JSON line with Mcell-updates/s (cells x iterations /
+s), the roofline figure of the line-relaxation kernel measured with HIP
+events during the timed region, and the CPU baseline (the oracle, one core,
+on a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(spec_args, iters=2):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from gfm_amd import case as cs
+    c = cs.furnace_large(**spec_args)
+    c.ctrl.sweep_order = 0
+    c.ctrl.want_pre_residuals = 0
+    oracle.initsv_tail(c)
+    t = time.perf_counter()
+    n = len(oracle.sphase(c, iters))
+    dt = time.perf_counter() - t
+    return {"value": c.ncells * n / dt / 1e6, "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{n} SIMPLER iterations of the same 192x96x64 case, reference x-plane order, "
+                      f"oracle/gfm_oracle.c (gcc -O2) on 1 host core, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nx", type=int, default=192)
+    ap.add_argument("--ny", type=int, default=96)
+    ap.add_argument("--nz", type=int, default=64)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "a case shards as x-slabs across GPUs only in the design notes", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from gfm_amd import case as cs
+    from gfm_amd.solver import CombustionSpace
+    try:
+        import torch
+        sync = torch.cuda.synchronize
+    except Exception:  # the library synchronises its own stream per iteration
+        sync = lambda: None
+
+    spec_args = dict(nx=args.nx, ny=args.ny, nz=args.nz, seed=2, maxgi=10 ** 7, bgcon=0.0)
+    case = cs.furnace_large(**spec_args)
+    case.ctrl.sweep_order = 1
+    case.ctrl.want_pre_residuals = 0
+    n = case.ncells
+    cspace = CombustionSpace(case, device=0)
+    for _ in range(args.warmup):
+        cspace.gflow()
+    sync()
+    cspace.kernel_times(1)
+    t0 = time.perf_counter()
+    hist = cspace.sphase(args.steps)
+    sync()
+    dt = time.perf_counter() - t0
+    kt = cspace.kernel_times(0)
+    steps = len(hist)
+    res_final = float(hist[-1]) if steps else float("nan")
+    value = n * steps / dt / 1e6
+
+    ms, by, la = kt.get("k_line_pcr", (0.0, 0.0, 0))
+    avg_s = ms / 1e3 / max(la, 1)
+    per_launch = by / max(la, 1)
+    achieved = per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_line_pcr", "avg_launch_us": round(avg_s * 1e6, 2), "bytes_per_launch": per_launch,
+            "launches": la}
+    line = {
+        "metric": "time to residual 1e-5 & Mcell-updates/s: learned+FP32 vs plain FP64, 1 GPU",
+        "value": round(value, 3), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": round(dt / max(steps, 1) * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"plain FP64 SIMPLER, stretched furnace {args.nx}x{args.ny}x{args.nz} "
+                               f"({n} cells), zebra x-plane order", "cells": n, "res_mass_last": res_final},
+        "roofline": roof,
+    }
+    kt_sum = {k: {"ms": round(v[0], 3), "launches": v[2]} for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0])}
+    print(json.dumps({"kernel_times_ms": kt_sum}), file=sys.stderr)
+    cspace.close()
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(spec_args, args.cpu_iters)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

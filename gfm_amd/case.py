@@ -591,8 +591,9 @@ def furnace_small(nx=48, ny=24, nz=16, seed=1, noise=0.01, maxgi=300, bgcon=1.0e
     dy, dz = Ly / ny, Lz / nz
 
     def span(centre, width, d, n):
-        a = int(round((centre - width / 2) / d))
-        b = int(round((centre + width / 2) / d)) - 1
+        a = int(math.floor((centre - width / 2) / d + 0.5))
+        b = int(math.floor((centre + width / 2) / d + 0.5)) - 1
+        b = max(b, a)
         return max(a, 0), min(b, n - 1)
 
     fy = span(0.6, 0.1, dy, ny); fz = span(0.4, 0.1, dz, nz)

@@ -1,0 +1,76 @@
+// gfm_dev.h -- device data model of the MI355X SIMPLER solver.
+//
+// Every field of the reference lives on the doubled-index lattice
+// (MP,NP,LP) (centres at even indices, faces/edges/corners at odd ones,
+// C/Var_mod.f90).  On the device a lattice array is stored as its 8 parity
+// sub-lattices ("classes"), each a dense x-fastest block of
+// NXC*NYC*NZC entries with NXC = MP/2+1:
+//
+//   offset(I,J,K) = cls*S + ((K>>1)*NYC + (J>>1))*NXC + (I>>1),
+//   cls = (I&1) | (J&1)<<1 | (K&1)<<2,  S = NXC*NYC*NZC.
+//
+// Kernels keep the reference's lattice indices (I,J,K); neighbours at +-2
+// stay in the same class and sit +-1 apart, so a wavefront walking I of one
+// class reads contiguous memory.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+struct DevGeom {
+  int MP, NP, LP;
+  int NXC, NYC, NZC;
+  long long S;  // entries per class
+};
+
+// Everything the kernels read: pointers to device arrays and the scalar
+// constants of the case.  Held in __constant__ memory (one bound case at a
+// time per device).
+struct DevCase {
+  DevGeom g;
+  // 1-D grid (1-based index i -> x[i-1])
+  const double *x, *dx, *r, *dr, *rs, *z, *dz;
+  const int16_t *ib;
+  const int8_t *intp_code;  // effective INTP weight override per lattice position
+  double *p, *t, *dnst, *theta, *tmu, *gcp, *glam, *gama, *gte, *crate, *hrate, *bsh;
+  double *ug[3];
+  double *gf[9];
+  double *fz, *bs, *sfp, *ap;
+  double *as[6];  // AS(.,d,m) -> as[(d-1)+3*(m-1)]
+  const double *rtim, *delrt, *delr, *timed;
+  double *scal;  // device scalar block (see SC_* indices)
+  long long pstride;  // stride between partial-sum slots
+  int nr0, ncells;
+  int i_ref, j_ref, k_ref;
+  int njy1v[4], njy2v[4], lkz1v[4], lkz2v[4];
+  double t0, trdg, h_0, r0, area0, vol0;
+  double reyg, reyg2, eun, epsr, taur, ug0, dnst0, qr0, q0, gmfr, tk0, epsi0;
+  double gra[3], p_ref;
+  double t_mx, t_mn, u_mx, u_mn, p_mx, p_mn, f_mx, f_mn;
+  double rf[21], rfl[21], rf_hsource;
+  double t_wall;
+  int i1st_rad_done, isothermal_walls;
+  double wt0, stoich;
+};
+
+// device scalar block indices
+enum {
+  SC_RES_MASS = 0, SC_RES_MAX, SC_IMX, SC_JMX, SC_KMX,
+  SC_GFEX, SC_FUEL_OUT, SC_TMEAN_EX, SC_TAREA, SC_AREA_EX,
+  SC_GFIN0, SC_GFIN, SC_FACTOR, SC_FADJ, SC_QEXH, SC_QREACT, SC_G0REF,
+  SC_TAVE_NUM, SC_TAVE_DEN, SC_TVOL_NUM, SC_TVOL_DEN, SC_TMAX,
+  SC_RESID0 = 32,  // 64 residual slots follow
+  SC_COUNT = 128
+};
+
+struct Box {  // lattice index box with a uniform step
+  int i0, ni, j0, nj, k0, nk, step;
+};
+
+static inline Box make_box(int i0, int i1, int j0, int j1, int k0, int k1, int step) {
+  Box b;
+  b.i0 = i0; b.j0 = j0; b.k0 = k0; b.step = step;
+  b.ni = (i1 >= i0) ? (i1 - i0) / step + 1 : 0;
+  b.nj = (j1 >= j0) ? (j1 - j0) / step + 1 : 0;
+  b.nk = (k1 >= k0) ? (k1 - k0) / step + 1 : 0;
+  return b;
+}

@@ -1,0 +1,728 @@
+// gfm_hip.hip -- host side of the MI355X SIMPLER solver and its C-ABI
+// (include/gfm_hip.h).  One translation unit: the kernels are included so
+// that the __constant__ case block is shared without relocatable device
+// code.
+#include "gfm_kernels.hip"
+#include "../../include/gfm_hip.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <string>
+#include <vector>
+
+static thread_local std::string g_err;
+static void set_err(const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+#define HCHECK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess) {                                                        \
+      set_err("%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return GFM_E_DEVICE;                                                         \
+    }                                                                              \
+  } while (0)
+
+// residual slots in the device scalar block
+enum { RS_P_PRE = 0, RS_P = 1, RS_U_PRE = 2, RS_U = 5, RS_PC = 8, RS_GF_PRE = 10, RS_GF = 20, RS_IRC = 30 };
+
+struct ProfRec {
+  std::string name;
+  double ms = 0.0, bytes = 0.0;
+  long launches = 0;
+};
+struct Prof {
+  bool on = false;
+  std::vector<ProfRec> recs;
+  std::vector<hipEvent_t> ev;
+  std::vector<std::pair<int, double>> pending;  // record index, bytes
+  int used = 0;
+};
+
+struct gfm_ctx {
+  int device = 0;
+  Prof prof;
+  gfm_ctrl ctrl;
+  DevCase dc;
+  std::vector<void *> allocs;
+  double *xt2 = nullptr, *xt3 = nullptr, *part = nullptr, *stage = nullptr;
+  int16_t *stage16 = nullptr;
+  long long nlat = 0, ncls = 0;
+  hipStream_t stream = nullptr;
+  double sgm = 1.0;
+};
+static gfm_ctx *g_bound = nullptr;
+
+static int prof_rec(gfm_ctx *c, const char *name) {
+  for (size_t q = 0; q < c->prof.recs.size(); q++)
+    if (c->prof.recs[q].name == name) return (int)q;
+  ProfRec r;
+  r.name = name;
+  c->prof.recs.push_back(r);
+  return (int)c->prof.recs.size() - 1;
+}
+static void prof_flush(gfm_ctx *c) {
+  Prof &p = c->prof;
+  if (p.used == 0) return;
+  (void)hipStreamSynchronize(c->stream);
+  for (int u = 0; u < p.used; u++) {
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, p.ev[2 * u], p.ev[2 * u + 1]);
+    ProfRec &r = p.recs[p.pending[u].first];
+    r.ms += ms;
+    r.bytes += p.pending[u].second;
+    r.launches += 1;
+  }
+  p.used = 0;
+  p.pending.clear();
+}
+static void prof_begin(gfm_ctx *c) {
+  Prof &p = c->prof;
+  if ((size_t)(2 * p.used + 2) > p.ev.size()) prof_flush(c);
+  if (p.ev.empty()) {
+    p.ev.resize(8192);
+    for (auto &e : p.ev) (void)hipEventCreate(&e);
+  }
+  (void)hipEventRecord(p.ev[2 * p.used], c->stream);
+}
+static void prof_end(gfm_ctx *c, const char *name, double bytes) {
+  Prof &p = c->prof;
+  (void)hipEventRecord(p.ev[2 * p.used + 1], c->stream);
+  p.pending.push_back({prof_rec(c, name), bytes});
+  p.used++;
+}
+
+static int bind(gfm_ctx *c) {
+  if (g_bound == c) return GFM_OK;
+  HCHECK(hipSetDevice(c->device));
+  HCHECK(dev_bind_case(c->dc));
+  g_bound = c;
+  return GFM_OK;
+}
+
+template <typename T>
+static int dalloc(gfm_ctx *c, T **p, size_t n) {
+  void *q = nullptr;
+  HCHECK(hipMalloc(&q, n * sizeof(T)));
+  HCHECK(hipMemset(q, 0, n * sizeof(T)));
+  c->allocs.push_back(q);
+  *p = (T *)q;
+  return GFM_OK;
+}
+
+static dim3 box_grid(const Box &b) { return dim3((b.ni + 63) / 64, b.nj, b.nk); }
+static long long box_blocks(const Box &b) { return (long long)((b.ni + 63) / 64) * b.nj * b.nk; }
+#define LAUNCH(kern, b, ...)                                                              \
+  do {                                                                                    \
+    if ((b).ni > 0 && (b).nj > 0 && (b).nk > 0) {                                         \
+      if (c->prof.on) prof_begin(c);                                                      \
+      hipLaunchKernelGGL(kern, box_grid(b), dim3(64), 0, c->stream, b, ##__VA_ARGS__);   \
+      if (c->prof.on) prof_end(c, #kern, 0.0);                                            \
+    }                                                                                     \
+  } while (0)
+
+static void fill_devcase_scalars(gfm_ctx *c, const gfm_ctrl *k) {
+  DevCase &d = c->dc;
+  d.nr0 = k->nr0; d.ncells = k->ncells;
+  d.i_ref = k->i_ref; d.j_ref = k->j_ref; d.k_ref = k->k_ref;
+  int NP = k->np, LP = k->lp;
+  int njy1v[4] = {0, 4, 5, 4}, njy2v[4] = {0, NP - 2, NP - 3, NP - 2};
+  int lkz1v[4] = {0, 4, 4, 5}, lkz2v[4] = {0, LP - 2, LP - 2, LP - 3};
+  memcpy(d.njy1v, njy1v, sizeof njy1v); memcpy(d.njy2v, njy2v, sizeof njy2v);
+  memcpy(d.lkz1v, lkz1v, sizeof lkz1v); memcpy(d.lkz2v, lkz2v, sizeof lkz2v);
+  d.t0 = k->t0; d.trdg = k->trdg; d.h_0 = k->h_0; d.r0 = k->r0; d.area0 = k->area0; d.vol0 = k->vol0;
+  d.reyg = k->reyg; d.reyg2 = k->reyg2; d.eun = k->eun; d.epsr = k->epsr; d.taur = k->taur; d.ug0 = k->ug0;
+  d.dnst0 = k->dnst0; d.qr0 = k->qr0; d.q0 = k->q0; d.gmfr = k->gmfr; d.tk0 = k->tk0; d.epsi0 = k->epsi0;
+  for (int q = 0; q < 3; q++) d.gra[q] = k->gra[q];
+  d.p_ref = k->p_ref;
+  d.t_mx = k->t_mx; d.t_mn = k->t_mn; d.u_mx = k->u_mx; d.u_mn = k->u_mn; d.p_mx = k->p_mx; d.p_mn = k->p_mn;
+  d.f_mx = k->f_mx; d.f_mn = k->f_mn;
+  for (int q = 0; q < 21; q++) { d.rf[q] = k->rf[q]; d.rfl[q] = k->rfl[q]; }
+  d.rf_hsource = k->rf_hsource;
+  d.t_wall = k->t_wall;
+  d.i1st_rad_done = k->i1st_rad_done;
+  d.isothermal_walls = k->isothermal_walls;
+  d.wt0 = 1.0 / (0.5 / WF + 0.0 / WCO2 + 0.0 / WH2O + 0.49 / WO2 + 0.01 / WN2);
+  d.stoich = 2.0 * WO2 / WF;
+}
+
+// sticky INTP weights: replay the reference's INTP loop nest on the host
+// (C/Intp.f90:38-70 with INTP0's in-place F1 update, :149-157) and record
+// for every position whether F1 is the loop's INTP1 weight (0), forced to
+// one (1) or forced to zero (2).
+static void intp_codes(const gfm_ctrl *k, const int16_t *ibf, std::vector<int8_t> &code, const DevGeom &gg) {
+  const int MPh = k->mp, NPh = k->np, LPh = k->lp;
+  auto ib = [&](int i, int j, int kk) -> int { return ibf[(((size_t)kk - 1) * NPh + (j - 1)) * MPh + (i - 1)]; };
+  auto li = [&](int i, int j, int kk) -> long long {
+    int c = (i & 1) | ((j & 1) << 1) | ((kk & 1) << 2);
+    return (long long)c * gg.S + ((long long)(kk >> 1) * gg.NYC + (j >> 1)) * gg.NXC + (i >> 1);
+  };
+  code.assign((size_t)gg.S * 8, 0);
+  auto visit = [&](int &st, int i, int j, int kk, int i1, int j1, int k1, int i2, int j2, int k2) {
+    if (ib(i, j, kk) == 3) return;
+    if (ib(i, j, kk) == 2) {
+      if (ib(i1, j1, k1) == 2 && ib(i2, j2, k2) == 0) st = 1;
+      else if (ib(i1, j1, k1) == 0 && ib(i2, j2, k2) == 2) st = 2;
+    }
+    code[li(i, j, kk)] = (int8_t)st;
+  };
+  for (int i = 2; i <= MPh; i += 2) {
+    for (int kk = 3; kk <= LPh - 1; kk += 2) {
+      int st = 0;
+      for (int j = 2; j <= NPh; j += 2) {
+        if (ib(i, j, kk) == 1) continue;
+        visit(st, i, j, kk, i, j, kk + 1, i, j, kk - 1);
+      }
+    }
+    for (int j = 3; j <= NPh - 1; j += 2) {
+      int st = 0;
+      for (int kk = 2; kk <= LPh; kk++) {
+        if (ib(i, j, kk) == 1) continue;
+        visit(st, i, j, kk, i, j + 1, kk, i, j - 1, kk);
+      }
+    }
+  }
+  for (int i = 3; i <= MPh - 1; i += 2) {
+    int st = 0;
+    for (int kk = 2; kk <= LPh; kk++)
+      for (int j = 2; j <= NPh; j++) {
+        if (ib(i, j, kk) == 1) continue;
+        visit(st, i, j, kk, i + 1, j, kk, i - 1, j, kk);
+      }
+  }
+}
+
+// ---------------------------------------------------------------------
+// host <-> device state
+// ---------------------------------------------------------------------
+static int scatter(gfm_ctx *c, const double *h, double *d) {
+  HCHECK(hipMemcpyAsync(c->stage, h, c->nlat * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_scatter, dim3((unsigned)((c->nlat + 255) / 256)), dim3(256), 0, c->stream, c->stage, d, c->nlat);
+  return GFM_OK;
+}
+static int gather(gfm_ctx *c, const double *d, double *h) {
+  hipLaunchKernelGGL(k_gather, dim3((unsigned)((c->nlat + 255) / 256)), dim3(256), 0, c->stream, d, c->stage, c->nlat);
+  HCHECK(hipMemcpyAsync(h, c->stage, c->nlat * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  return GFM_OK;
+}
+
+#define FIELD_LIST(X_) X_(p) X_(t) X_(dnst) X_(theta) X_(tmu) X_(gcp) X_(glam) X_(gama) X_(gte) X_(crate) X_(hrate)
+
+static int upload_state(gfm_ctx *c, const gfm_ctrl *k, const gfm_host_arrays *h) {
+  int rc = bind(c);
+  if (rc) return rc;
+  DevCase &d = c->dc;
+#define UP(f) if ((rc = scatter(c, h->f, d.f))) return rc;
+  FIELD_LIST(UP)
+#undef UP
+  if ((rc = scatter(c, h->bs_hprev, d.bsh))) return rc;
+  for (int q = 0; q < 3; q++)
+    if ((rc = scatter(c, h->ug + (size_t)q * c->nlat, d.ug[q]))) return rc;
+  for (int q = 0; q < 9; q++)
+    if ((rc = scatter(c, h->gf + (size_t)q * c->nlat, d.gf[q]))) return rc;
+  double sc[SC_COUNT] = {0};
+  sc[SC_FACTOR] = k->factor;
+  sc[SC_FADJ] = k->fadj;
+  sc[SC_GFIN0] = k->gfin0;
+  HCHECK(hipMemcpyAsync(d.scal, sc, sizeof sc, hipMemcpyHostToDevice, c->stream));
+  HCHECK(hipStreamSynchronize(c->stream));
+  return GFM_OK;
+}
+
+static void pull_scalars(gfm_ctx *c, gfm_ctrl *k, const double *sc) {
+  k->res_mass = sc[SC_RES_MASS];
+  k->res_mass_max = sc[SC_RES_MAX];
+  k->imx = (int)sc[SC_IMX]; k->jmx = (int)sc[SC_JMX]; k->kmx = (int)sc[SC_KMX];
+  k->gfex = sc[SC_GFEX]; k->fuel_out = sc[SC_FUEL_OUT]; k->tmean_ex = sc[SC_TMEAN_EX];
+  k->gfin0 = sc[SC_GFIN0]; k->gfin = sc[SC_GFIN]; k->factor = sc[SC_FACTOR]; k->fadj = sc[SC_FADJ];
+  k->q_exhaust = sc[SC_QEXH]; k->q_react = sc[SC_QREACT];
+  if (sc[SC_TAVE_DEN] > 0.0) {
+    k->tave = sc[SC_TAVE_NUM] * k->t0 / sc[SC_TAVE_DEN];
+    k->tave_vol = sc[SC_TVOL_NUM] * k->t0 / sc[SC_TVOL_DEN];
+    k->tmax_c = sc[SC_TMAX] * k->t0;
+  }
+  const double *r = sc + SC_RESID0;
+  if (k->want_pre_residuals) {
+    k->p_resid_pre = r[RS_P_PRE];
+    for (int q = 0; q < 3; q++) k->u_resid_pre[q] = r[RS_U_PRE + q];
+    for (int q = 0; q < 7; q++) k->gf_resid_pre[q] = r[RS_GF_PRE + q];
+    k->h_resid_pre = r[RS_GF_PRE + 2];
+  }
+  k->p_resid = r[RS_P];
+  for (int q = 0; q < 3; q++) k->u_resid[q] = r[RS_U + q];
+  k->pcorr_resid = r[RS_PC];
+  for (int q = 0; q < 7; q++) k->gf_resid[q] = r[RS_GF + q];
+  k->h_resid = r[RS_GF + 2];
+  if (k->want_irc)
+    for (int q = 0; q < 12; q++) { k->resg[q] = r[RS_IRC + 2 * q]; k->resg[20 + q] = r[RS_IRC + 2 * q + 1]; }
+}
+
+// ---------------------------------------------------------------------
+// ADLBL3 (C/Adlbl3.f90:10-97) and calc_residual
+// ---------------------------------------------------------------------
+static void plane_op(gfm_ctx *c, double *xt, int pi0, int pdi, int npl, int jb, int je, int kb, int ke) {
+  int nyl = (ke - kb) / 2 + 1, nzl = (je - jb) / 2 + 1;
+  const double cells = (double)npl * nyl * nzl;
+  const double line_bytes = cells * 10.0 * sizeof(double);
+  for (int l = 0; l < 2; l++) {
+    if (c->prof.on) prof_begin(c);
+    hipLaunchKernelGGL(k_line_pcr, dim3(npl * nyl), dim3(64), 0, c->stream, xt, c->xt2, 2, pi0, pdi, npl, jb, je, kb, ke);
+    if (c->prof.on) { prof_end(c, "k_line_pcr", line_bytes); prof_begin(c); }
+    hipLaunchKernelGGL(k_line_pcr, dim3(npl * nzl), dim3(64), 0, c->stream, xt, c->xt3, 3, pi0, pdi, npl, jb, je, kb, ke);
+    if (c->prof.on) { prof_end(c, "k_line_pcr", line_bytes); prof_begin(c); }
+    long long tot = (long long)npl * nzl * nyl;
+    hipLaunchKernelGGL(k_line_avg, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, xt, c->xt2, c->xt3, pi0, pdi,
+                       npl, jb, nzl, kb, nyl);
+    if (c->prof.on) prof_end(c, "k_line_avg", cells * 3.0 * sizeof(double));
+  }
+}
+static void residual(gfm_ctx *c, const double *xt, int ib, int ie, int jb, int je, int kb, int ke, int slot, int irc) {
+  Box b = make_box(ib, ie, jb, je, kb, ke, 2);
+  LAUNCH(k_residual, b, xt, c->part, irc);
+  hipLaunchKernelGGL(k_residual_finalize, dim3(1), dim3(256), 0, c->stream, c->part, (int)box_blocks(b), SC_RESID0 + slot, irc);
+}
+static void adlbl3(gfm_ctx *c, double *xt, int nelg, int ib, int ie, int jb, int je, int kb, int ke) {
+  int nim = c->ctrl.ntimesg[nelg - 1];
+  int npl = (ie - ib) / 2 + 1;
+  for (int s = 0; s < nim; s++) {
+    if (c->ctrl.sweep_order == GFM_ORDER_REFERENCE) {
+      for (int i = ib; i <= ie; i += 2) plane_op(c, xt, i, 0, 1, jb, je, kb, ke);
+      for (int i = ie; i >= ib; i -= 2) plane_op(c, xt, i, 0, 1, jb, je, kb, ke);
+    } else {
+      for (int half = 0; half < 4; half++) {
+        int col = half & 1;
+        int n = (npl - col + 1) / 2;
+        if (n > 0) plane_op(c, xt, ib + 2 * col, 4, n, jb, je, kb, ke);
+      }
+    }
+  }
+  if (c->ctrl.want_irc) residual(c, xt, ib, ie, jb, je, kb, ke, RS_IRC + 2 * (nelg - 1), 1);
+}
+
+// ---------------------------------------------------------------------
+// GFLOW, C/Gflow.f90:6-408
+// ---------------------------------------------------------------------
+static int gflow(gfm_ctx *c) {
+  const gfm_ctrl &k = c->ctrl;
+  DevCase &d = c->dc;
+  const int MPh = k.mp, NPh = k.np, LPh = k.lp;
+  const int itr = k.itr_gas;
+  Box all1 = make_box(2, MPh, 2, NPh, 2, LPh, 1);
+  Box cen = make_box(4, MPh - 2, 4, NPh - 2, 4, LPh - 2, 2);
+  Box cen_all = make_box(2, MPh, 2, NPh, 2, LPh, 2);
+
+  LAUNCH(k_theta_wall, all1);
+  LAUNCH(k_gama_wall_zero, all1);
+  double rt20 = k.reyg2 * sqrt(k.epsr) * pow(CMU, 0.25);
+  LAUNCH(k_wall_function, all1, rt20);
+
+  // ---- GSLV3V ----
+  for (int iu1 = 1; iu1 <= 3; iu1++) {
+    int nx0 = iu1 == 1, nx1 = iu1 == 2, nx2 = iu1 == 3;
+    Box fb = make_box(2 + nx0, MPh - nx0, 2 + nx1, NPh - nx1, 2 + nx2, LPh - nx2, 2);
+    LAUNCH(k_copy_ug_fz, fb, iu1);
+  }
+  Box mom[4];
+  for (int iu1 = 1; iu1 <= 3; iu1++) {
+    int nx0 = iu1 == 1;
+    mom[iu1] = make_box(4 + nx0, MPh - 2 - nx0, d.njy1v[iu1], d.njy2v[iu1], d.lkz1v[iu1], d.lkz2v[iu1], 2);
+  }
+  for (int iu1 = 1; iu1 <= 3; iu1++) LAUNCH(k_momentum_assemble, mom[iu1], iu1);
+  LAUNCH(k_pressure_assemble, cen);
+  if (k.want_pre_residuals) residual(c, d.p, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_P_PRE, 0);
+  adlbl3(c, d.p, 4, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2);
+  residual(c, d.p, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_P, 0);
+  LAUNCH(k_bound_p, cen);
+  for (int iu1 = 1; iu1 <= 3; iu1++) {
+    const Box &b = mom[iu1];
+    int ie = b.i0 + 2 * (b.ni - 1), je = b.j0 + 2 * (b.nj - 1), ke = b.k0 + 2 * (b.nk - 1);
+    LAUNCH(k_starred, b, iu1);
+    if (k.want_pre_residuals) residual(c, d.fz, b.i0, ie, b.j0, je, b.k0, ke, RS_U_PRE + iu1 - 1, 0);
+    adlbl3(c, d.fz, iu1, b.i0, ie, b.j0, je, b.k0, ke);
+    residual(c, d.fz, b.i0, ie, b.j0, je, b.k0, ke, RS_U + iu1 - 1, 0);
+    LAUNCH(k_bound_store_ug, b, iu1);
+  }
+  // p'
+  LAUNCH(k_pcorr_source, cen, c->part);
+  hipLaunchKernelGGL(k_pcorr_finalize, dim3(1), dim3(256), 0, c->stream, c->part, box_blocks(cen));
+  LAUNCH(k_zero_fz, cen_all);
+  adlbl3(c, d.fz, 5, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2);
+  residual(c, d.fz, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_PC, 0);
+  for (int iu1 = 1; iu1 <= 3; iu1++) LAUNCH(k_vel_correct, mom[iu1], iu1);
+  // EXTRV(0), FLX1D, factor, EXTRV(1)
+  Box ex[3] = {make_box(3, MPh - 1, 2, NPh, 2, LPh, 2), make_box(2, MPh, 3, NPh - 1, 2, LPh, 2), make_box(2, MPh, 2, NPh, 3, LPh - 1, 2)};
+  {
+    // three directions accumulate into one set of partials: run them in
+    // order and sum each into the scalar block
+    for (int q = 0; q < 3; q++) {
+      LAUNCH(k_extrv, ex[q], q + 1, 0, c->part);
+      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->part, (int)box_blocks(ex[q]), 5, SC_GFEX, q > 0);
+    }
+  }
+  hipLaunchKernelGGL(k_factor, dim3(1), dim3(1), 0, c->stream, itr);
+  for (int q = 0; q < 3; q++) LAUNCH(k_extrv, ex[q], q + 1, 1, c->part);
+  // INTPV
+  {
+    // IU=1
+    Box b;
+    b = make_box(3, MPh - 1, 3, NPh - 1, 2, LPh, 2); LAUNCH(k_intpv, b, 1, 2, 1, 1, 0);
+    b = make_box(3, MPh - 1, 2, NPh, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 1, 3, 1, 0, 1);
+    b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 1, 3, 1, 1, 1);
+    for (int pj = 0; pj < 2; pj++)
+      for (int pk = 0; pk < 2; pk++) {
+        b = make_box(4, MPh, 2 + pj, pj ? NPh - 1 : NPh, 2 + pk, pk ? LPh - 1 : LPh, 2);
+        LAUNCH(k_intpv, b, 1, 1, 0, pj, pk);
+      }
+    // IU=2
+    b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 2, 3, 0, 1, 1);
+    b = make_box(3, MPh - 1, 3, NPh - 1, 2, LPh, 2); LAUNCH(k_intpv, b, 2, 1, 1, 1, 0);
+    b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 2, 1, 1, 1, 1);
+    for (int pi = 0; pi < 2; pi++)
+      for (int pk = 0; pk < 2; pk++) {
+        b = make_box(2 + pi, pi ? MPh - 1 : MPh, 4, NPh, 2 + pk, pk ? LPh - 1 : LPh, 2);
+        LAUNCH(k_intpv, b, 2, 2, pi, 0, pk);
+      }
+    // IU=3
+    b = make_box(3, MPh - 1, 2, NPh, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 3, 1, 1, 0, 1);
+    b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 3, 2, 0, 1, 1);
+    b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 3, 2, 1, 1, 1);
+    for (int pi = 0; pi < 2; pi++)
+      for (int pj = 0; pj < 2; pj++) {
+        b = make_box(2 + pi, pi ? MPh - 1 : MPh, 2 + pj, pj ? NPh - 1 : NPh, 4, LPh, 2);
+        LAUNCH(k_intpv, b, 3, 3, pi, pj, 0);
+      }
+  }
+  // pressure reference
+  hipLaunchKernelGGL(k_pref, dim3(1), dim3(1), 0, c->stream);
+
+  // ---- GSLV3G ----
+  const int Ls[7] = {1, 2, 3, 4, 5, 6, 7};
+  const int nelgs[7] = {6, 12, 8, 9, 10, 7, 11};
+  const double sigma[7] = {0.7, 0.7, 0.7, 1.0, 1.0, 0.7, 0.7};
+  for (int e = 0; e < 7; e++) {
+    int l = Ls[e], nelg = nelgs[e];
+    LAUNCH(k_copy_gf_fz, cen_all, l);
+    int igs = 1;
+    if (e == 0 || c->sgm != sigma[e]) { c->sgm = sigma[e]; igs = 0; }
+    LAUNCH(k_scalar_assemble, cen, l, nelg, igs, c->sgm, itr, c->part);
+    if (nelg == 8)
+      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->part, (int)box_blocks(cen), 1, SC_QREACT, 0);
+    if (k.want_pre_residuals) residual(c, d.fz, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_GF_PRE + l - 1, 0);
+    adlbl3(c, d.fz, nelg, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2);
+    residual(c, d.fz, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_GF + l - 1, 0);
+    LAUNCH(k_bound_store_gf, cen, l);
+  }
+  c->sgm = 1.0;  // GSLV3V resets SGM at the next iteration
+  // properties, T_ave, EXTR(1), INTP
+  LAUNCH(k_props, cen_all);
+  LAUNCH(k_tave, cen, c->part);
+  hipLaunchKernelGGL(k_tave_finalize, dim3(1), dim3(256), 0, c->stream, c->part, (int)box_blocks(cen));
+  LAUNCH(k_extr_inout, cen_all, c->part);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->part, (int)box_blocks(cen_all), 1, SC_GFIN0, 0);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->part + c->dc.pstride, (int)box_blocks(cen_all), 1, SC_QEXH, 0);
+  LAUNCH(k_extr_wall, cen_all);
+
+  {
+    Box b = make_box(2, MPh, 2, NPh, 3, LPh - 1, 2); LAUNCH(k_intp, b, 3);
+    b = make_box(2, MPh, 3, NPh - 1, 2, LPh, 2); LAUNCH(k_intp, b, 2);
+    b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intp, b, 2);
+    for (int pj = 0; pj < 2; pj++)
+      for (int pk = 0; pk < 2; pk++) {
+        b = make_box(3, MPh - 1, 2 + pj, pj ? NPh - 1 : NPh, 2 + pk, pk ? LPh - 1 : LPh, 2);
+        LAUNCH(k_intp, b, 1);
+      }
+  }
+  HCHECK(hipGetLastError());
+  return GFM_OK;
+}
+
+static int initsv_tail(gfm_ctx *c) {
+  const gfm_ctrl &k = c->ctrl;
+  const int MPh = k.mp, NPh = k.np, LPh = k.lp;
+  // INTPV (same launches as in gflow)
+  Box b;
+  b = make_box(3, MPh - 1, 3, NPh - 1, 2, LPh, 2); LAUNCH(k_intpv, b, 1, 2, 1, 1, 0);
+  b = make_box(3, MPh - 1, 2, NPh, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 1, 3, 1, 0, 1);
+  b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 1, 3, 1, 1, 1);
+  for (int pj = 0; pj < 2; pj++)
+    for (int pk = 0; pk < 2; pk++) { b = make_box(4, MPh, 2 + pj, pj ? NPh - 1 : NPh, 2 + pk, pk ? LPh - 1 : LPh, 2); LAUNCH(k_intpv, b, 1, 1, 0, pj, pk); }
+  b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 2, 3, 0, 1, 1);
+  b = make_box(3, MPh - 1, 3, NPh - 1, 2, LPh, 2); LAUNCH(k_intpv, b, 2, 1, 1, 1, 0);
+  b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 2, 1, 1, 1, 1);
+  for (int pi = 0; pi < 2; pi++)
+    for (int pk = 0; pk < 2; pk++) { b = make_box(2 + pi, pi ? MPh - 1 : MPh, 4, NPh, 2 + pk, pk ? LPh - 1 : LPh, 2); LAUNCH(k_intpv, b, 2, 2, pi, 0, pk); }
+  b = make_box(3, MPh - 1, 2, NPh, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 3, 1, 1, 0, 1);
+  b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 3, 2, 0, 1, 1);
+  b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 3, 2, 1, 1, 1);
+  for (int pi = 0; pi < 2; pi++)
+    for (int pj = 0; pj < 2; pj++) { b = make_box(2 + pi, pi ? MPh - 1 : MPh, 2 + pj, pj ? NPh - 1 : NPh, 4, LPh, 2); LAUNCH(k_intpv, b, 3, 3, pi, pj, 0); }
+  Box cen_all = make_box(2, MPh, 2, NPh, 2, LPh, 2);
+  LAUNCH(k_extr_inout, cen_all, c->part);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->part, (int)box_blocks(cen_all), 1, SC_GFIN0, 0);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->part + c->dc.pstride, (int)box_blocks(cen_all), 1, SC_QEXH, 0);
+  LAUNCH(k_extr_wall, cen_all);
+  b = make_box(2, MPh, 2, NPh, 3, LPh - 1, 2); LAUNCH(k_intp, b, 3);
+  b = make_box(2, MPh, 3, NPh - 1, 2, LPh, 2); LAUNCH(k_intp, b, 2);
+  b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intp, b, 2);
+  for (int pj = 0; pj < 2; pj++)
+    for (int pk = 0; pk < 2; pk++) { b = make_box(3, MPh - 1, 2 + pj, pj ? NPh - 1 : NPh, 2 + pk, pk ? LPh - 1 : LPh, 2); LAUNCH(k_intp, b, 1); }
+  HCHECK(hipGetLastError());
+  return GFM_OK;
+}
+
+// ---------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------
+extern "C" {
+
+const char *gfm_hip_last_error(void) { return g_err.c_str(); }
+int gfm_hip_abi_version(void) { return GFM_ABI_VERSION; }
+size_t gfm_hip_ctrl_size(void) { return sizeof(gfm_ctrl); }
+
+int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ctx **out) {
+  if (!k || !h || !out) { set_err("null argument"); return GFM_E_ARG; }
+  *out = nullptr;
+  if (k->mp < 8 || k->np < 8 || k->lp < 8 || (k->mp & 1) || (k->np & 1) || (k->lp & 1)) {
+    set_err("bad lattice dimensions %d %d %d", k->mp, k->np, k->lp);
+    return GFM_E_ARG;
+  }
+  if ((k->np - 6) / 2 + 1 > PCR_MAXN || (k->lp - 6) / 2 + 1 > PCR_MAXN) {
+    set_err("line length above %d cells is not supported", PCR_MAXN);
+    return GFM_E_ARG;
+  }
+  if (k->sweep_order != GFM_ORDER_REFERENCE && k->sweep_order != GFM_ORDER_ZEBRA) { set_err("bad sweep_order"); return GFM_E_ARG; }
+  if (k->precision != GFM_PREC_FP64) { set_err("precision mode %d not available", k->precision); return GFM_E_ARG; }
+  const long long nlat = (long long)k->mp * k->np * k->lp;
+  for (long long q = 0; q < nlat; q++)
+    if (h->ibcell[q] < 0) { set_err("half-blocked cells (IBCELL<0) are not supported"); return GFM_E_ARG; }
+  int ndev = 0;
+  HCHECK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) { set_err("device %d not present (%d devices)", device, ndev); return GFM_E_DEVICE; }
+  HCHECK(hipSetDevice(device));
+  gfm_ctx *c = new gfm_ctx();
+  c->device = device;
+  c->ctrl = *k;
+  c->nlat = nlat;
+  DevGeom &gg = c->dc.g;
+  gg.MP = k->mp; gg.NP = k->np; gg.LP = k->lp;
+  gg.NXC = k->mp / 2 + 1; gg.NYC = k->np / 2 + 1; gg.NZC = k->lp / 2 + 1;
+  gg.S = (long long)gg.NXC * gg.NYC * gg.NZC;
+  c->ncls = gg.S * 8;
+  int rc = 0;
+  auto fail = [&](int r) { gfm_hip_finalize(c); return r; };
+  HCHECK(hipStreamCreate(&c->stream));
+  DevCase &d = c->dc;
+  const long long N = c->ncls;
+#define AL(f) if ((rc = dalloc(c, &d.f, N))) return fail(rc);
+  FIELD_LIST(AL)
+  AL(bsh) AL(fz) AL(bs) AL(sfp) AL(ap)
+#undef AL
+  for (int q = 0; q < 3; q++) if ((rc = dalloc(c, &d.ug[q], N))) return fail(rc);
+  for (int q = 0; q < 9; q++) if ((rc = dalloc(c, &d.gf[q], N))) return fail(rc);
+  for (int q = 0; q < 6; q++) if ((rc = dalloc(c, &d.as[q], N))) return fail(rc);
+  if ((rc = dalloc(c, &c->xt2, N))) return fail(rc);
+  if ((rc = dalloc(c, &c->xt3, N))) return fail(rc);
+  if ((rc = dalloc(c, &c->stage, N))) return fail(rc);
+  if ((rc = dalloc(c, &c->stage16, N))) return fail(rc);
+  // partial sums: 5 slots of the largest reduction grid
+  long long maxb = (long long)((gg.NXC + 63) / 64) * gg.NYC * gg.NZC + 64;
+  d.pstride = maxb;
+  if ((rc = dalloc(c, &c->part, 5 * maxb))) return fail(rc);
+  double *scal;
+  if ((rc = dalloc(c, &scal, SC_COUNT))) return fail(rc);
+  d.scal = scal;
+  int16_t *ibd;
+  if ((rc = dalloc(c, &ibd, N))) return fail(rc);
+  d.ib = ibd;
+  int8_t *icode;
+  if ((rc = dalloc(c, &icode, N))) return fail(rc);
+  d.intp_code = icode;
+  double *g1[7];
+  const double *h1[7] = {h->x, h->dx, h->r, h->dr, h->rs, h->z, h->dz};
+  int n1[7] = {k->mp, k->mp, k->np, k->np, k->np, k->lp, k->lp};
+  for (int q = 0; q < 7; q++) {
+    if ((rc = dalloc(c, &g1[q], n1[q]))) return fail(rc);
+    HCHECK(hipMemcpy(g1[q], h1[q], n1[q] * sizeof(double), hipMemcpyHostToDevice));
+  }
+  d.x = g1[0]; d.dx = g1[1]; d.r = g1[2]; d.dr = g1[3]; d.rs = g1[4]; d.z = g1[5]; d.dz = g1[6];
+  double *kt[4];
+  const double *hk[4] = {h->rtim, h->delrt, h->delr, h->timed};
+  for (int q = 0; q < 4; q++) {
+    if ((rc = dalloc(c, &kt[q], k->nr0))) return fail(rc);
+    HCHECK(hipMemcpy(kt[q], hk[q], k->nr0 * sizeof(double), hipMemcpyHostToDevice));
+  }
+  d.rtim = kt[0]; d.delrt = kt[1]; d.delr = kt[2]; d.timed = kt[3];
+  fill_devcase_scalars(c, k);
+  if ((rc = bind(c))) return fail(rc);
+  // cell codes and INTP weight codes
+  HCHECK(hipMemcpy(c->stage16, h->ibcell, nlat * sizeof(int16_t), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_scatter_i16, dim3((unsigned)((nlat + 255) / 256)), dim3(256), 0, c->stream, c->stage16, ibd, nlat);
+  std::vector<int8_t> code;
+  intp_codes(k, h->ibcell, code, gg);
+  HCHECK(hipMemcpy(icode, code.data(), N, hipMemcpyHostToDevice));
+  if ((rc = upload_state(c, k, h))) return fail(rc);
+  *out = c;
+  return GFM_OK;
+}
+
+int gfm_hip_upload(gfm_ctx *c, const gfm_ctrl *k, const gfm_host_arrays *h) {
+  if (!c || !k || !h) { set_err("null argument"); return GFM_E_ARG; }
+  c->ctrl = *k;
+  fill_devcase_scalars(c, k);
+  g_bound = nullptr;
+  return upload_state(c, k, h);
+}
+
+int gfm_hip_download(gfm_ctx *c, gfm_ctrl *k, gfm_host_arrays *h) {
+  if (!c || !h) { set_err("null argument"); return GFM_E_ARG; }
+  int rc = bind(c);
+  if (rc) return rc;
+  DevCase &d = c->dc;
+#define DN_(f) if ((rc = gather(c, d.f, h->f))) return rc;
+  FIELD_LIST(DN_)
+#undef DN_
+  if ((rc = gather(c, d.bsh, h->bs_hprev))) return rc;
+  for (int q = 0; q < 3; q++) if ((rc = gather(c, d.ug[q], h->ug + (size_t)q * c->nlat))) return rc;
+  for (int q = 0; q < 9; q++) if ((rc = gather(c, d.gf[q], h->gf + (size_t)q * c->nlat))) return rc;
+  HCHECK(hipStreamSynchronize(c->stream));
+  if (k) *k = c->ctrl;
+  return GFM_OK;
+}
+
+int gfm_hip_initsv_tail(gfm_ctx *c, gfm_ctrl *k) {
+  if (!c) { set_err("null context"); return GFM_E_ARG; }
+  int rc = bind(c);
+  if (rc) return rc;
+  if ((rc = initsv_tail(c))) return rc;
+  double sc[SC_COUNT];
+  HCHECK(hipMemcpyAsync(sc, c->dc.scal, sizeof sc, hipMemcpyDeviceToHost, c->stream));
+  HCHECK(hipStreamSynchronize(c->stream));
+  c->ctrl.gfin0 = sc[SC_GFIN0];
+  c->ctrl.q_exhaust = sc[SC_QEXH];
+  if (k) { k->gfin0 = c->ctrl.gfin0; k->q_exhaust = c->ctrl.q_exhaust; }
+  return GFM_OK;
+}
+
+int gfm_hip_gflow(gfm_ctx *c, gfm_ctrl *k) {
+  if (!c || !k) { set_err("null argument"); return GFM_E_ARG; }
+  int rc = bind(c);
+  if (rc) return rc;
+  c->ctrl.itr_gas = k->itr_gas;
+  c->ctrl.sweep_order = k->sweep_order;
+  c->ctrl.want_pre_residuals = k->want_pre_residuals;
+  c->ctrl.want_irc = k->want_irc;
+  if ((rc = gflow(c))) return rc;
+  double sc[SC_COUNT];
+  HCHECK(hipMemcpyAsync(sc, c->dc.scal, sizeof sc, hipMemcpyDeviceToHost, c->stream));
+  HCHECK(hipStreamSynchronize(c->stream));
+  pull_scalars(c, &c->ctrl, sc);
+  pull_scalars(c, k, sc);
+  if (c->prof.on) prof_flush(c);
+  return GFM_OK;
+}
+
+int gfm_hip_sphase(gfm_ctx *c, gfm_ctrl *k, int max_iters, double *res_hist, int *n_done) {
+  if (!c || !k) { set_err("null argument"); return GFM_E_ARG; }
+  int n = 0, rc;
+  while (n < max_iters && k->itr_gas < k->itr_gend && k->itr_gas < k->maxgi) {
+    k->itr_gas += 1;
+    if ((rc = gfm_hip_gflow(c, k))) { if (n_done) *n_done = n; return rc; }
+    if (!isfinite(k->res_mass)) { set_err("non-finite res_mass at iteration %d", k->itr_gas); if (n_done) *n_done = n + 1; return GFM_E_NUMERIC; }
+    if (res_hist) res_hist[n] = k->res_mass;
+    n++;
+    if (k->res_mass < k->bgcon) { k->itr_gend = k->itr_gas; break; }
+  }
+  c->ctrl.itr_gas = k->itr_gas;
+  c->ctrl.itr_gend = k->itr_gend;
+  if (n_done) *n_done = n;
+  return GFM_OK;
+}
+
+int gfm_hip_adlbl3(gfm_ctx *c, const gfm_ctrl *k, int nelg, const double *ap, const double *as, const double *bs, double *xt,
+                   int ib, int ie, int jb, int je, int kb, int ke, double *resg) {
+  if (!c || !k || !ap || !as || !bs || !xt) { set_err("null argument"); return GFM_E_ARG; }
+  if (nelg < 1 || nelg > 20) { set_err("bad nelg"); return GFM_E_ARG; }
+  int rc = bind(c);
+  if (rc) return rc;
+  DevCase &d = c->dc;
+  c->ctrl.sweep_order = k->sweep_order;
+  c->ctrl.want_irc = k->want_irc;
+  for (int q = 0; q < 20; q++) c->ctrl.ntimesg[q] = k->ntimesg[q];
+  if ((rc = scatter(c, ap, d.ap))) return rc;
+  for (int q = 0; q < 6; q++) if ((rc = scatter(c, as + (size_t)q * c->nlat, d.as[q]))) return rc;
+  if ((rc = scatter(c, bs, d.bs))) return rc;
+  if ((rc = scatter(c, xt, d.fz))) return rc;
+  adlbl3(c, d.fz, nelg, ib, ie, jb, je, kb, ke);
+  if ((rc = gather(c, d.fz, xt))) return rc;
+  double sc[SC_COUNT];
+  HCHECK(hipMemcpyAsync(sc, d.scal, sizeof sc, hipMemcpyDeviceToHost, c->stream));
+  HCHECK(hipStreamSynchronize(c->stream));
+  if (resg && k->want_irc) { resg[0] = sc[SC_RESID0 + RS_IRC + 2 * (nelg - 1)]; resg[1] = sc[SC_RESID0 + RS_IRC + 2 * (nelg - 1) + 1]; }
+  return GFM_OK;
+}
+
+int gfm_hip_calc_residual(gfm_ctx *c, const double *ap, const double *as, const double *bs, const double *xt, int ib, int ie, int jb,
+                          int je, int kb, int ke, double *resid) {
+  if (!c || !ap || !as || !bs || !xt || !resid) { set_err("null argument"); return GFM_E_ARG; }
+  int rc = bind(c);
+  if (rc) return rc;
+  DevCase &d = c->dc;
+  if ((rc = scatter(c, ap, d.ap))) return rc;
+  for (int q = 0; q < 6; q++) if ((rc = scatter(c, as + (size_t)q * c->nlat, d.as[q]))) return rc;
+  if ((rc = scatter(c, bs, d.bs))) return rc;
+  if ((rc = scatter(c, xt, c->xt2))) return rc;
+  residual(c, c->xt2, ib, ie, jb, je, kb, ke, 63, 0);
+  double sc[SC_COUNT];
+  HCHECK(hipMemcpyAsync(sc, d.scal, sizeof sc, hipMemcpyDeviceToHost, c->stream));
+  HCHECK(hipStreamSynchronize(c->stream));
+  *resid = sc[SC_RESID0 + 63];
+  return GFM_OK;
+}
+
+int gfm_hip_kernel_times(gfm_ctx *c, int enable, char *names, int name_len, double *ms, double *bytes, int *launches, int cap,
+                         int *n) {
+  if (!c) { set_err("null context"); return GFM_E_ARG; }
+  prof_flush(c);
+  if (enable == 1) {
+    c->prof.recs.clear();
+    c->prof.on = true;
+  } else if (enable == 0) {
+    c->prof.on = false;
+  }
+  int cnt = 0;
+  for (const ProfRec &r : c->prof.recs) {
+    if (cnt >= cap) break;
+    if (names && name_len > 0) {
+      strncpy(names + (size_t)cnt * name_len, r.name.c_str(), name_len - 1);
+      names[(size_t)cnt * name_len + name_len - 1] = 0;
+    }
+    if (ms) ms[cnt] = r.ms;
+    if (bytes) bytes[cnt] = r.bytes;
+    if (launches) launches[cnt] = (int)r.launches;
+    cnt++;
+  }
+  if (n) *n = cnt;
+  return GFM_OK;
+}
+
+int gfm_hip_finalize(gfm_ctx *c) {
+  if (!c) return GFM_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void *p : c->allocs) (void)hipFree(p);
+  for (auto &e : c->prof.ev) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (g_bound == c) g_bound = nullptr;
+  delete c;
+  return GFM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,1350 @@
+// gfm_kernels.hip -- CDNA4 kernels of the combustion-space SIMPLER
+// iteration.  Each kernel restates one loop nest of the reference
+// (C/ = combustion/Code/ of the GFM v4 package) over the parity-class
+// layout of gfm_dev.h; per-point arithmetic follows the reference order of
+// operations so that FP64 results track it to rounding.
+#include "gfm_dev.h"
+#include <string.h>
+
+__constant__ DevCase g;
+__constant__ double c_nist[7][5][2];  // a,b,c,d,e,f,h x species x {low,high}
+__constant__ double c_cutT[5];
+
+static const double h_nist_tab[7][5][2] = {
+    {{-0.703029, 85.81217}, {29.659, 29.659}, {24.99735, 58.16639}, {26.092, 26.092}, {30.092, 41.96426}},
+    {{108.4773, 11.26467}, {6.137261, 6.137261}, {55.18696, 2.720074}, {8.218801, 8.218801}, {6.832514, 8.622053}},
+    {{-42.52157, -2.114146}, {-1.186521, -1.186521}, {-33.69137, -0.492289}, {-1.976141, -1.976141}, {6.793435, -1.49978}},
+    {{5.862788, 0.13819}, {0.09578, 0.09578}, {7.948387, 0.038844}, {0.159274, 0.159274}, {-2.53448, 0.098119}},
+    {{0.678565, -26.42221}, {-0.219663, -0.219663}, {-0.136638, -6.447293}, {0.044434, 0.044434}, {0.082139, -11.15764}},
+    {{-76.84376, -153.5327}, {-9.861391, -9.861391}, {-403.6075, -425.9186}, {-7.98923, -7.98923}, {-250.881, -272.1797}},
+    {{-74.8731, -74.8731}, {0.0, 0.0}, {-393.5224, -393.5224}, {0.0, 0.0}, {-241.8264, -241.8264}}};
+static const double h_cutT[5] = {1300.0, 6000.0, 1200.0, 6000.0, 1700.0};
+
+#define SMALL20 1.0e-20
+#define SMALL16 1.0e-16
+#define BIGV 1.0e20
+#define WF 16.0426
+#define WO2 31.9988
+#define WCO2 44.0098
+#define WH2O 18.0152
+#define WN2 28.0134
+#define CMU 9.0e-2
+#define CT1 1.44
+#define CT2 1.92
+#define CPF1 (2.156e3 / 1.0057e3)
+
+__device__ __forceinline__ long long lidx(int i, int j, int k) {
+  int c = (i & 1) | ((j & 1) << 1) | ((k & 1) << 2);
+  return (long long)c * g.g.S + ((long long)(k >> 1) * g.g.NYC + (j >> 1)) * g.g.NXC + (i >> 1);
+}
+#define MP g.g.MP
+#define NP g.g.NP
+#define LP g.g.LP
+#define X(i) g.x[(i)-1]
+#define DX(i) g.dx[(i)-1]
+#define R(j) g.r[(j)-1]
+#define DR(j) g.dr[(j)-1]
+#define RS(j) g.rs[(j)-1]
+#define Z(k) g.z[(k)-1]
+#define DZ(k) g.dz[(k)-1]
+#define IBC(i, j, k) ((int)g.ib[lidx(i, j, k)])
+#define P_(i, j, k) g.p[lidx(i, j, k)]
+#define T_(i, j, k) g.t[lidx(i, j, k)]
+#define DN(i, j, k) g.dnst[lidx(i, j, k)]
+#define TH(i, j, k) g.theta[lidx(i, j, k)]
+#define TMU(i, j, k) g.tmu[lidx(i, j, k)]
+#define GCP(i, j, k) g.gcp[lidx(i, j, k)]
+#define GLAM(i, j, k) g.glam[lidx(i, j, k)]
+#define GAMA(i, j, k) g.gama[lidx(i, j, k)]
+#define GTE(i, j, k) g.gte[lidx(i, j, k)]
+#define CRATE(i, j, k) g.crate[lidx(i, j, k)]
+#define HRATE(i, j, k) g.hrate[lidx(i, j, k)]
+#define UG(i, j, k, l) g.ug[(l)-1][lidx(i, j, k)]
+#define GF(i, j, k, l) g.gf[(l)-1][lidx(i, j, k)]
+#define FZ(i, j, k) g.fz[lidx(i, j, k)]
+#define BS(i, j, k) g.bs[lidx(i, j, k)]
+#define SFP(i, j, k) g.sfp[lidx(i, j, k)]
+#define AP(i, j, k) g.ap[lidx(i, j, k)]
+#define AS(i, j, k, d, m) g.as[((d)-1) + 3 * ((m)-1)][lidx(i, j, k)]
+
+enum { IYF = 1, IYO2 = 2, IH = 3, IK = 4, IEPS = 5, IYCO2 = 6, IYN2 = 7, IYH2O = 9 };
+
+__device__ __forceinline__ bool box_pt(const Box &b, int &i, int &j, int &k) {
+  int tx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tx >= b.ni) return false;
+  i = b.i0 + b.step * tx;
+  j = b.j0 + b.step * (int)blockIdx.y;
+  k = b.k0 + b.step * (int)blockIdx.z;
+  return true;
+}
+
+// ---------------- wave-level reductions (blockDim == 64) ----------------
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ long long block_lin() {
+  return blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+}
+
+// ======================================================================
+// geometry, C/Grid3.f90:310-471 (no half-blocked cells)
+// ======================================================================
+__device__ __forceinline__ double AREA_C(int i, int j, int k, int ix) {
+  if (IBC(i, j, k) == 1) return 0.0;
+  if (ix == 1) return DR(j) * RS(j) * DZ(k);
+  if (ix == 2) return DX(i) * DZ(k);
+  return DX(i) * DR(j) * RS(j);
+}
+__device__ __forceinline__ double VOL_C0(int i, int j, int k) { return DX(i) * DR(j) * RS(j) * DZ(k); }
+__device__ double VOL_C(int i, int j, int k) {
+  double vm, vp;
+  if (i & 1) {
+    vm = VOL_C0(i - 1, j, k); if (IBC(i - 2, j, k) <= 0) vm *= 0.5;
+    vp = VOL_C0(i + 1, j, k); if (IBC(i + 2, j, k) <= 0) vp *= 0.5;
+    return vp + vm;
+  } else if (j & 1) {
+    vm = VOL_C0(i, j - 1, k); if (IBC(i, j - 2, k) <= 0) vm *= 0.5;
+    vp = VOL_C0(i, j + 1, k); if (IBC(i, j + 2, k) <= 0) vp *= 0.5;
+    return vp + vm;
+  } else if (k & 1) {
+    vm = VOL_C0(i, j, k - 1); if (IBC(i, j, k - 2) <= 0) vm *= 0.5;
+    vp = VOL_C0(i, j, k + 1); if (IBC(i, j, k + 2) <= 0) vp *= 0.5;
+    return vp + vm;
+  }
+  return VOL_C0(i, j, k);
+}
+__device__ double DXAV(int i, int j, int k) {
+  if (IBC(i - 2, j, k) >= 1) return X(i + 1) - X(i - 2);
+  if (IBC(i + 2, j, k) >= 1) return X(i + 2) - X(i - 1);
+  return DX(i);
+}
+__device__ double DRAV(int i, int j, int k) {
+  if (IBC(i, j - 2, k) >= 1) return (RS(j + 1) + RS(j - 2)) * 0.5 / RS(j) * (R(j + 1) - R(j - 2));
+  if (IBC(i, j + 2, k) >= 1) return (RS(j + 2) + RS(j - 1)) * 0.5 / RS(j) * (R(j + 2) - R(j - 1));
+  return DR(j);
+}
+__device__ double DZAV(int i, int j, int k) {
+  if (IBC(i, j, k - 2) >= 1) return Z(k + 1) - Z(k - 2);
+  if (IBC(i, j, k + 2) >= 1) return Z(k + 2) - Z(k - 1);
+  return DZ(k);
+}
+
+// ======================================================================
+// properties, C/Prop.f90
+// ======================================================================
+__device__ __forceinline__ double DA(double d, double f) {  // C/Prop.f90:97-117
+  if (d <= 1.0e-25) return 0.0;
+  double fa = fabs(f), pe = fa / d;
+  if (pe <= 1.0e-2) return d / (1.0 + pe * (0.5 + pe * (1.0 / 6.0 + pe / 24.0)));
+  if (pe <= 20.0) return fa / (exp(pe) - 1.0);
+  return 0.0;
+}
+__device__ __forceinline__ double TGAMF(double b) {  // C/Gflow.f90:417-426
+  if (b < 11.5) return 1.0;
+  return fmax(b / (log(9.0 * b) * 2.5), 1.0);
+}
+__device__ void CONC(long long q) {  // C/Prop.f90:28-59
+  double yf = g.gf[0][q], yo = g.gf[1][q], yc = g.gf[5][q], yn = g.gf[6][q];
+  double g0 = yf + yo + yc + yn;
+  if (g0 > 1.0) {
+    g0 = g0 + g.gf[8][q];
+    g.gf[0][q] = yf / g0; g.gf[1][q] = yo / g0; g.gf[5][q] = yc / g0; g.gf[6][q] = yn / g0;
+    g.gf[8][q] = g.gf[8][q] / g0;
+  } else {
+    g.gf[8][q] = 1.0 - g0;
+  }
+}
+__device__ void DNST_CALC(long long q) {  // C/Prop.f90:187-198
+  CONC(q);
+  double wt = 1.0 / (g.gf[0][q] / WF + g.gf[5][q] / WCO2 + g.gf[8][q] / WH2O + g.gf[1][q] / WO2 + g.gf[6][q] / WN2);
+  double temp = (1.0 + g.p[q]) / g.t[q] * wt / g.wt0;
+  g.dnst[q] = fmax(temp, SMALL20);
+}
+__device__ void THER(long long q) {  // C/Prop.f90:130-154
+  double e = fmax(g.gf[4][q], 1.0e-12);
+  g.gf[4][q] = e;
+  double kk = g.gf[3][q];
+  double t = CMU * g.reyg * g.epsr / (2.0 * g.taur) * kk * kk / e * g.dnst[q];
+  g.tmu[q] = t;
+  g.glam[q] = 1.0;
+  g.gcp[q] = CPF1;
+  g.gama[q] = 2.0 / g.reyg * (t + 1.0);
+}
+__device__ __forceinline__ double BOUND_U(double f) {  // C/Prop.f90:257-260
+  f = fmin(f, g.u_mx); f = fmax(f, g.u_mn);
+  if (fabs(f) < 1.0e-12) f = 0.0;
+  return f;
+}
+__device__ __forceinline__ double BOUND_P(double f) { return fmax(fmin(f, g.p_mx), g.p_mn); }
+__device__ __forceinline__ double BOUND_GF(double f, int l) {  // C/Prop.f90:264-279
+  if (l == IK) return fmax(f, 1.0e-6);
+  if (l == IEPS) return fmax(f, 1.0e-12);
+  if (l == IH) return f;
+  return fmax(fmin(f, 1.0), 0.0);
+}
+
+struct Mix { double a, b, c, d, e, f, h; int lohi[5]; };
+__device__ __forceinline__ void mix_add(Mix &m, double y, int n, int s, double sg) {
+  m.a += sg * y * c_nist[0][n][s]; m.b += sg * y * c_nist[1][n][s]; m.c += sg * y * c_nist[2][n][s];
+  m.d += sg * y * c_nist[3][n][s]; m.e += sg * y * c_nist[4][n][s];
+  m.f = m.d + sg * y * c_nist[5][n][s]; m.h = m.e + sg * y * c_nist[6][n][s];
+}
+__device__ void mix_build(Mix &m, const double y[5], double realT) {
+  m.a = m.b = m.c = m.d = m.e = m.f = m.h = 0.0;
+  for (int n = 0; n < 5; n++) {
+    int s = (realT < c_cutT[n]) ? 0 : 1;
+    mix_add(m, y[n], n, s, 1.0);
+    m.lohi[n] = s + 1;
+  }
+}
+__device__ __forceinline__ void ygather(long long q, double y[5]) {
+  y[0] = g.gf[0][q]; y[1] = g.gf[1][q]; y[2] = g.gf[5][q]; y[3] = g.gf[6][q]; y[4] = g.gf[8][q];
+}
+// enth_to_T, C/Prop.f90:369-505 (FP64 Newton with the reference's
+// coefficient construction and lo/hi switching)
+__device__ void ENTH_TO_T(long long q) {
+  double y[5];
+  ygather(q, y);
+  double w_mix = 1.0 / (y[0] / WF + y[2] / WCO2 + y[4] / WH2O + y[1] / WO2 + y[3] / WN2);
+  double h_molar = g.gf[2][q] * g.h_0 * w_mix / 1.0e6;
+  double realT = g.t[q] * g.t0;
+  Mix m;
+  mix_build(m, y, realT);
+  double T_old = realT / 1000.0, T_new = T_old;
+  for (int n = 1; n <= 10; n++) {
+    double gp = m.a + T_old * (m.b + T_old * (m.c + T_old * m.d)) + m.e / (T_old * T_old);
+    if (gp == 0.0) break;
+    double gg = T_old * (m.a + T_old * ((m.b / 2.0) + T_old * ((m.c / 3.0) + T_old * (m.d / 4.0)))) - (m.e / T_old) + m.f - m.h;
+    double ch = (gg - h_molar) / gp;
+    T_new = T_old - ch;
+    if (fabs(ch) < 1.0e-11) break;
+    T_old = T_new;
+    for (int nn = 0; nn < 5; nn++) {
+      if (nn == 1 || nn == 3) continue;
+      if (T_new * 1000.0 < c_cutT[nn] && m.lohi[nn] != 1) { mix_add(m, y[nn], nn, 1, -1.0); m.lohi[nn] = 1; mix_add(m, y[nn], nn, 0, 1.0); }
+      if (T_new * 1000.0 >= c_cutT[nn] && m.lohi[nn] != 2) { mix_add(m, y[nn], nn, 0, -1.0); m.lohi[nn] = 2; mix_add(m, y[nn], nn, 1, 1.0); }
+    }
+  }
+  g.t[q] = T_new * 1000.0 / g.t0;
+}
+__device__ void T_TO_ENTH(long long q) {  // C/Prop.f90:525-565
+  double y[5];
+  ygather(q, y);
+  double w_mix = 1.0 / (y[0] / WF + y[2] / WCO2 + y[4] / WH2O + y[1] / WO2 + y[3] / WN2);
+  double realT = g.t[q] * g.t0;
+  Mix m;
+  mix_build(m, y, realT);
+  double t = realT / 1000.0;
+  double ef = t * (m.a + t * ((m.b / 2.0) + t * ((m.c / 3.0) + t * (m.d / 4.0)))) - (m.e / t) + m.f - m.h;
+  g.gf[2][q] = ef * 1.0e6 / (w_mix * g.h_0);
+}
+__device__ double ZY1(const double *x, const double *y, double x1, int n) {  // C/Source.f90:814-831
+  if (x1 < x[0]) return y[0] * x1 / x[0];
+  for (int i = 1; i < n; i++)
+    if (x1 < x[i]) return y[i - 1] + (y[i] - y[i - 1]) * ((x1 - x[i - 1]) / (x[i] - x[i - 1]));
+  return y[n - 1];
+}
+
+// ======================================================================
+// GFLOW wall treatment, C/Gflow.f90:16-130
+// ======================================================================
+__global__ void k_theta_wall(Box b) {
+  int i, j, k;
+  if (!box_pt(b, i, j, k)) return;
+  if (IBC(i, j, k) >= 1) return;
+  double g1, g2;
+  if (IBC(i - 1, j, k) == 1) { g1 = (X(i) - X(i - 1)) * (X(i) - X(i - 1)); g2 = (X(i + 2) - X(i - 1)) * (X(i + 2) - X(i - 1)); TH(i - 1, j, k) = TH(i, j, k) * (g2 / (g2 - g1)) + TH(i + 2, j, k) * (g1 / (g1 - g2)); }
+  if (IBC(i + 1, j, k) == 1) { g1 = (X(i) - X(i + 1)) * (X(i) - X(i + 1)); g2 = (X(i - 2) - X(i + 1)) * (X(i - 2) - X(i + 1)); TH(i + 1, j, k) = TH(i, j, k) * (g2 / (g2 - g1)) + TH(i - 2, j, k) * (g1 / (g1 - g2)); }
+  if (IBC(i, j - 1, k) == 1) { g1 = (R(j) - R(j - 1)) * (R(j) - R(j - 1)); g2 = (R(j + 2) - R(j - 1)) * (R(j + 2) - R(j - 1)); TH(i, j - 1, k) = TH(i, j, k) * (g2 / (g2 - g1)) + TH(i, j + 2, k) * (g1 / (g1 - g2)); }
+  if (IBC(i, j + 1, k) == 1) { g1 = (R(j) - R(j + 1)) * (R(j) - R(j + 1)); g2 = (R(j - 2) - R(j + 1)) * (R(j - 2) - R(j + 1)); TH(i, j + 1, k) = TH(i, j, k) * (g2 / (g2 - g1)) + TH(i, j - 2, k) * (g1 / (g1 - g2)); }
+  if (IBC(i, j, k - 1) == 1) { g1 = (Z(k) - Z(k - 1)) * (Z(k) - Z(k - 1)); g2 = (Z(k + 2) - Z(k - 1)) * (Z(k + 2) - Z(k - 1)); TH(i, j, k - 1) = TH(i, j, k) * (g2 / (g2 - g1)) + TH(i, j, k + 2) * (g1 / (g1 - g2)); }
+  if (IBC(i, j, k + 1) == 1) { g1 = (Z(k) - Z(k + 1)) * (Z(k) - Z(k + 1)); g2 = (Z(k - 2) - Z(k + 1)) * (Z(k - 2) - Z(k + 1)); TH(i, j, k + 1) = TH(i, j, k) * (g2 / (g2 - g1)) + TH(i, j, k - 2) * (g1 / (g1 - g2)); }
+}
+__global__ void k_gama_wall_zero(Box b) {
+  int i, j, k;
+  if (!box_pt(b, i, j, k)) return;
+  if (IBC(i, j, k) == 1) GAMA(i, j, k) = 0.0;
+}
+// Wall functions.  Each wall lattice position next to open fluid has one
+// open neighbour in a rectangular furnace, so the updates do not collide.
+__global__ void k_wall_function(Box b, double rt20) {
+  int i, j, k;
+  if (!box_pt(b, i, j, k)) return;
+  if (IBC(i, j, k) >= 1) return;
+  double blvm = rt20 * DN(i, j, k) * sqrt(GF(i, j, k, IK));
+  double vgam = 1.0 / g.reyg2;
+#define WF_UPD(cond, ia, ja, ka, ib_, jb_, kb_, dist)                    \
+  if (cond) {                                                             \
+    double gg = vgam * TGAMF(blvm * (dist));                              \
+    GAMA(ia, ja, ka) = fmax(gg, GAMA(ia, ja, ka));                        \
+    TMU(ia, ja, ka) = g.reyg2 * GAMA(ia, ja, ka);                         \
+    TMU(ib_, jb_, kb_) = TMU(ia, ja, ka);                                 \
+  }
+  WF_UPD(IBC(i - 1, j, k) == 1, i - 1, j, k, i - 2, j, k, X(i) - X(i - 1))
+  WF_UPD(IBC(i + 1, j, k) == 1, i + 1, j, k, i + 2, j, k, X(i + 1) - X(i))
+  WF_UPD(IBC(i, j - 1, k) == 1, i, j - 1, k, i, j - 2, k, R(j) - R(j - 1))
+  WF_UPD(IBC(i, j + 1, k) == 1, i, j + 1, k, i, j + 2, k, R(j + 1) - R(j))
+  WF_UPD(IBC(i, j, k - 1) == 1, i, j, k - 1, i, j, k - 2, Z(k) - Z(k - 1))
+  WF_UPD(IBC(i, j, k + 1) == 1, i, j, k + 1, i, j, k + 2, Z(k + 1) - Z(k))
+#undef WF_UPD
+}
+
+// ======================================================================
+// momentum, C/Gslv3x.f90 + SRC1 C/Source.f90:664-809
+// ======================================================================
+__device__ __forceinline__ void VINDX(int iu, int m, int i, int j, int k, int &i1, int &j1, int &k1, int &i2, int &j2, int &k2) {
+  int n0 = (iu == 1) ? m : 0, n1 = (iu == 2) ? m : 0, n2 = (iu == 3) ? m : 0;
+  i1 = i + n0; i2 = i1 + n0; j1 = j + n1; j2 = j1 + n1; k1 = k + n2; k2 = k1 + n2;
+}
+__device__ double FLW(int i, int j, int k, int ix, int m) {
+  double f = TH(i, j, k) * DN(i, j, k) * UG(i, j, k, ix) * AREA_C(i, j, k, ix);
+  return (m == 1) ? fmax(0.0, f) : fmax(0.0, -f);
+}
+__device__ double FLWD(int i, int j, int k, int ix, int m, double sgm) {
+  double area = AREA_C(i, j, k, ix);
+  double ddl = (ix == 1) ? DX(i) : (ix == 2) ? DR(j) : DZ(k);
+  double f = TH(i, j, k) * DN(i, j, k) * UG(i, j, k, ix) * area;
+  double diff = TH(i, j, k) * GAMA(i, j, k) / sgm * area / ddl;
+  return (m == 1) ? DA(diff, f) + fmax(0.0, f) : DA(diff, f) + fmax(0.0, -f);
+}
+__device__ double FDDL(int i, int j, int k, int iu) {
+  int b = IBC(i, j, k);
+  if (iu == 1) {
+    if (b <= 0) return DX(i);
+    if (IBC(i + 1, j, k) == 0) return X(i + 1) - X(i);
+    if (IBC(i - 1, j, k) == 0) return X(i) - X(i - 1);
+    return BIGV;
+  } else if (iu == 2) {
+    if (b <= 0) return DR(j) * RS(j);
+    if (IBC(i, j + 1, k) == 0) return (R(j + 1) - R(j)) * RS(j);
+    if (IBC(i, j - 1, k) == 0) return (R(j) - R(j - 1)) * RS(j);
+    return BIGV;
+  }
+  if (b <= 0) return DZ(k);
+  if (IBC(i, j, k + 1) == 0) return Z(k + 1) - Z(k);
+  if (IBC(i, j, k - 1) == 0) return Z(k) - Z(k - 1);
+  return BIGV;
+}
+__device__ void ECELL(int i, int j, int k, int iu, double ffl[3][2]) {  // C/Gslv3x.f90:148-200
+  if (iu == 1) {
+    for (int m = 0; m < 2; m++) {
+      int i1 = m ? i + 1 : i - 1, i2 = m ? i + 2 : i - 2;
+      ffl[0][m] = (IBC(i2, j, k) >= 1) ? DX(i1) : fabs(X(i) - X(i1));
+      ffl[1][m] = ffl[0][m] * DZ(k);
+      ffl[2][m] = ffl[0][m] * RS(j) * DR(j);
+    }
+  } else if (iu == 2) {
+    for (int m = 0; m < 2; m++) {
+      int j1 = m ? j + 1 : j - 1, j2 = m ? j + 2 : j - 2;
+      ffl[1][m] = (IBC(i, j2, k) >= 1) ? RS(j1) * DR(j1) : RS(j1) * fabs(R(j) - R(j1));
+      ffl[0][m] = ffl[1][m] * DZ(k);
+      ffl[2][m] = ffl[1][m] * DX(i);
+    }
+  } else {
+    for (int m = 0; m < 2; m++) {
+      int k1 = m ? k + 1 : k - 1, k2 = m ? k + 2 : k - 2;
+      ffl[2][m] = (IBC(i, j, k2) >= 1) ? DZ(k1) : fabs(Z(k) - Z(k1));
+      ffl[0][m] = ffl[2][m] * RS(j) * DR(j);
+      ffl[1][m] = ffl[2][m] * DX(i);
+    }
+  }
+}
+
+// FZ = UG on a face class, C/Gslv3v.f90:26-34
+__global__ void k_copy_ug_fz(Box b, int iu1) {
+  int i, j, k;
+  if (!box_pt(b, i, j, k)) return;
+  long long q = lidx(i, j, k);
+  g.fz[q] = g.ug[iu1 - 1][q];
+}
+
+// SOURCE(IU1) -> SRC1 fused with GSLVX1 for one face (C/Gslv3x.f90:26-111)
+__global__ void k_momentum_assemble(Box b, int iu1) {
+  int i, j, k;
+  if (!box_pt(b, i, j, k)) return;
+  long long q = lidx(i, j, k);
+  if (IBC(i, j, k) >= 1) { g.bs[q] = 0.0; g.sfp[q] = 0.0; return; }
+  // ---- SRC1 ----
+  double vis[3][2] = {{0, 0}, {0, 0}, {0, 0}}, dxn[3], dx1[3] = {0, 0, 0}, dx0[3] = {0, 0, 0};
+  dxn[0] = DX(i); dxn[1] = DR(j) * RS(j); dxn[2] = DZ(k);
+  const double fzc = FZ(i, j, k);
+  for (int m = -1; m <= 1; m += 2) {
+    int i1, j1, k1, i2, j2, k2;
+    VINDX(iu1, m, i, j, k, i1, j1, k1, i2, j2, k2);
+    if (iu1 == 1) { dx0[0] = DX(i); dx1[0] = DX(i1); }
+    else if (iu1 == 2) { dx0[1] = DR(j) * RS(j); dx1[1] = DR(j1) * RS(j1); }
+    else { dx0[2] = DZ(k); dx1[2] = DZ(k1); }
+    double thgm;
+    if (IBC(i2, j2, k2) == 2) {
+      thgm = TH(i2, j2, k2) * GAMA(i2, j2, k2);
+      if (iu1 == 1) dxn[0] += fabs(X(i2) - X(i1));
+      if (iu1 == 2) dxn[1] += fabs(R(j2) - R(j1)) * RS(j);
+      if (iu1 == 3) dxn[2] += fabs(Z(k2) - Z(k1));
+    } else {
+      thgm = TH(i1, j1, k1) * GAMA(i1, j1, k1);
+    }
+    if (m == -1) vis[iu1 - 1][0] = thgm * (fzc - FZ(i2, j2, k2)) / dx1[iu1 - 1];
+    else vis[iu1 - 1][1] = thgm * (FZ(i2, j2, k2) - fzc) / dx1[iu1 - 1];
+  }
+  double vms = (vis[iu1 - 1][1] - vis[iu1 - 1][0]) / dxn[iu1 - 1];
+  for (int iu = 1; iu <= 3; iu++) {
+    if (iu == iu1) continue;
+    int i1 = i, j1 = j, k1 = k;
+    for (int m = -1; m <= 1; m += 2) {
+      if (iu == 1) i1 = i + m; else if (iu == 2) j1 = j + m; else k1 = k + m;
+      double du0;
+      if (iu1 == 1) du0 = FZ(i + 1, j1, k1) - FZ(i - 1, j1, k1);
+      else if (iu1 == 2) du0 = FZ(i1, j + 1, k1) - FZ(i1, j - 1, k1);
+      else du0 = FZ(i1, j1, k + 1) - FZ(i1, j1, k - 1);
+      int m1 = (m + 3) / 2;
+      double v = TH(i1, j1, k1) * GAMA(i1, j1, k1) * du0 / dx0[iu1 - 1];
+      if (iu == 2) v *= RS(j1);
+      vis[iu - 1][m1 - 1] = v;
+    }
+    vms += (vis[iu - 1][1] - vis[iu - 1][0]) / dxn[iu - 1];
+  }
+  double bs = vms;
+  bs = bs + TH(i, j, k) * DN(i, j, k) * g.gra[iu1 - 1];
+  const double sfp = -0.0;
+  g.sfp[q] = sfp;
+  // ---- GSLVX1 ----
+  double ap = 0.0, g0 = 0.0;
+  for (int m = -1; m <= 1; m += 2) {  // GSLV3X1
+    int m1 = (m + 3) / 2, i1, j1, k1, i2, j2, k2;
+    VINDX(iu1, m, i, j, k, i1, j1, k1, i2, j2, k2);
+    int ibc1 = IBC(i2, j2, k2);
+    double a;
+    if (ibc1 == 3) a = 0.0;
+    else if (ibc1 == 2) a = FLW(i2, j2, k2, iu1, m1);
+    else a = FLWD(i1, j1, k1, iu1, m1, 1.0);
+    g.as[(iu1 - 1) + 3 * (m1 - 1)][q] = a;
+    ap += a;
+    g0 += a * FZ(i2, j2, k2);
+  }
+  double ffl[3][2];
+  ECELL(i, j, k, iu1, ffl);
+  const int nx0 = (iu1 == 1), nx1 = (iu1 == 2), nx2 = (iu1 == 3);
+  for (int iu = 1; iu <= 3; iu++) {
+    if (iu == iu1) continue;
+    for (int m = -1; m <= 1; m += 2) {
+      int m1 = (m + 3) / 2, i1, j1, k1, i2, j2, k2;
+      VINDX(iu, m, i, j, k, i1, j1, k1, i2, j2, k2);
+      double *asp = &g.as[(iu - 1) + 3 * (m1 - 1)][q];
+      if (IBC(i1, j1, k1) == 3) { *asp = 0.0; continue; }
+      double f = 0.0, diff = 0.0;
+      int ii1 = 0, jj1 = 0, kk1 = 0;
+      for (int n = -1; n <= 1; n += 2) {
+        int n1 = (n + 3) / 2;
+        ii1 = i1 + nx0 * n; jj1 = j1 + nx1 * n; kk1 = k1 + nx2 * n;
+        int ibc1 = IBC(ii1, jj1, kk1);
+        if (ibc1 == 3) continue;
+        double area = ffl[iu - 1][n1 - 1];
+        f += TH(ii1, jj1, kk1) * DN(ii1, jj1, kk1) * FZ(ii1, jj1, kk1) * area;
+        if (ibc1 == 2) continue;
+        double ddl = FDDL(ii1, jj1, kk1, iu);
+        diff += TH(ii1, jj1, kk1) * GAMA(ii1, jj1, kk1) / 1.0 * area / ddl;
+      }
+      double fld = (m1 == 1) ? DA(diff, f) + fmax(0.0, f) : DA(diff, f) + fmax(0.0, -f);
+      if (IBC(ii1, jj1, kk1) == 2 && IBC(i2, j2, k2) == 1) {
+        bs += fld * (UG(ii1, j2, k2, iu1) - UG(i, j, k, iu1));
+        *asp = 0.0;
+        continue;
+      }
+      *asp = fld;
+      ap += fld;
+      g0 += fld * FZ(i2, j2, k2);
+    }
+  }
+  double vol = (ffl[iu1 - 1][0] + ffl[iu1 - 1][1]) * AREA_C(i, j, k, iu1);  // VOL1
+  bs *= vol;
+  ap += (0.0 - 0.0 - sfp) * vol;
+  g.bs[q] = bs;
+  g.ap[q] = ap;
+  if (fabs(ap) > SMALL20) UG(i, j, k, iu1) = (g0 + bs) / ap;
+}
+
+// SOURCE(4) + pressure-equation coefficients, C/Gslv3v.f90:43-89 and
+// C/Source.f90:35-74
+__global__ void k_pressure_assemble(Box b) {
+  int i, j, k;
+  if (!box_pt(b, i, j, k)) return;
+  long long q = lidx(i, j, k);
+  if (IBC(i, j, k) >= 1) { g.bs[q] = g.p[q]; return; }
+  double ax = AREA_C(i, j, k, 1), ay = AREA_C(i, j, k, 2), az = AREA_C(i, j, k, 3);
+  double frx1 = (TH(i - 1, j, k) * DN(i - 1, j, k) * UG(i - 1, j, k, 1)) * ax;
+  double frx2 = (TH(i + 1, j, k) * DN(i + 1, j, k) * UG(i + 1, j, k, 1)) * ax;
+  double fry1 = (TH(i, j - 1, k) * DN(i, j - 1, k) * UG(i, j - 1, k, 2)) * ay;
+  double fry2 = (TH(i, j + 1, k) * DN(i, j + 1, k) * UG(i, j + 1, k, 2)) * ay;
+  double frz1 = (TH(i, j, k - 1) * DN(i, j, k - 1) * UG(i, j, k - 1, 3)) * az;
+  double frz2 = (TH(i, j, k + 1) * DN(i, j, k + 1) * UG(i, j, k + 1, 3)) * az;
+  g.bs[q] = ((frx1 - frx2) + (fry1 - fry2) + (frz1 - frz2) + 0.0) / g.eun;
+  g.sfp[q] = 0.0;
+  double ap = -0.0;
+  for (int d = 1; d <= 3; d++) {
+    double area = (d == 1) ? ax : (d == 2) ? ay : az;
+    for (int m = 1; m <= 2; m++) {
+      int i1 = i, j1 = j, k1 = k;
+      int o = (m == 1) ? -1 : 1;
+      if (d == 1) i1 += o; else if (d == 2) j1 += o; else k1 += o;
+      double a = 0.0;
+      double apf = AP(i1, j1, k1);
+      if (IBC(i1, j1, k1) <= 0 && fabs(apf) > SMALL20) {
+        double th = TH(i1, j1, k1);
+        double av = (d == 1) ? DXAV(i1, j1, k1) / DX(i1) : (d == 2) ? DRAV(i1, j1, k1) / DR(j1) : DZAV(i1, j1, k1) / DZ(k1);
+        a = th * th * DN(i1, j1, k1) * area * area / apf * av;
+      }
+      g.as[(d - 1) + 3 * (m - 1)][q] = a;
+      ap += a;
+    }
+  }
+  g.ap[q] = ap;
+}
+__global__ void k_bound_p(Box b) {
+  int i, j, k;
+  if (!box_pt(b, i, j, k)) return;
+  if (IBC(i, j, k) >= 1) return;
+  long long q = lidx(i, j, k);
+  g.p[q] = BOUND_P(g.p[q]);
+}
+// starred-momentum right-hand side, C/Gslv3v.f90:116-130
+__global__ void k_starred(Box b, int iu1) {
+  int i, j, k;
+  if (!box_pt(b, i, j, k)) return;
+  if (IBC(i, j, k) >= 1) return;
+  long long q = lidx(i, j, k);
+  double ap = g.ap[q] / g.rf[iu1 - 1];
+  g.ap[q] = ap;
+  double vol = VOL_C(i, j, k), dpdx;
+  if (iu1 == 1) dpdx = (P_(i - 1, j, k) - P_(i + 1, j, k)) / DX(i);
+  else if (iu1 == 2) dpdx = (P_(i, j - 1, k) - P_(i, j + 1, k)) / DR(j) / RS(j);
+  else dpdx = (P_(i, j, k - 1) - P_(i, j, k + 1)) / DZ(k);
+  g.bs[q] = g.bs[q] + g.eun * g.theta[q] * vol * dpdx + (1.0 - g.rf[iu1 - 1]) * ap * g.fz[q];
+}
+__global__ void k_bound_store_ug(Box b, int iu1) {
+  int i, j, k;
+  if (!box_pt(b, i, j, k)) return;
+  if (IBC(i, j, k) >= 1) return;
+  long long q = lidx(i, j, k);
+  double f = BOUND_U(g.fz[q]);
+  g.fz[q] = f;
+  g.ug[iu1 - 1][q] = f;
+}
+
+// SOURCE(5): mass source of the starred velocities and res_mass
+// (C/Source.f90:41-101).  Per-block partial sum / max / argmax.
+__global__ void k_pcorr_source(Box b, double *part) {
+  int i, j, k;
+  double rm = 0.0, rmax = 0.0;
+  long long key = 0x7fffffffffffffffLL;
+  if (box_pt(b, i, j, k) && IBC(i, j, k) < 1) {
+    long long q = lidx(i, j, k);
+    double ax = AREA_C(i, j, k, 1), ay = AREA_C(i, j, k, 2), az = AREA_C(i, j, k, 3);
+    double frx1 = (TH(i - 1, j, k) * DN(i - 1, j, k) * UG(i - 1, j, k, 1)) * ax;
+    double frx2 = (TH(i + 1, j, k) * DN(i + 1, j, k) * UG(i + 1, j, k, 1)) * ax;
+    double fry1 = (TH(i, j - 1, k) * DN(i, j - 1, k) * UG(i, j - 1, k, 2)) * ay;
+    double fry2 = (TH(i, j + 1, k) * DN(i, j + 1, k) * UG(i, j + 1, k, 2)) * ay;
+    double frz1 = (TH(i, j, k - 1) * DN(i, j, k - 1) * UG(i, j, k - 1, 3)) * az;
+    double frz2 = (TH(i, j, k + 1) * DN(i, j, k + 1) * UG(i, j, k + 1, 3)) * az;
+    double bs = ((frx1 - frx2) + (fry1 - fry2) + (frz1 - frz2) + 0.0) / g.eun;
+    g.bs[q] = bs;
+    g.sfp[q] = 0.0;
+    bool exitadj = IBC(i - 1, j, k) == 3 || IBC(i + 1, j, k) == 3 || IBC(i, j - 1, k) == 3 ||
+                   IBC(i, j + 1, k) == 3 || IBC(i, j, k - 1) == 3 || IBC(i, j, k + 1) == 3;
+    if (!exitadj) {
+      double fmx = fmax(fmax(fmax(fabs(frx1), fabs(frx2)), fmax(fabs(fry1), fabs(fry2))), fmax(fmax(fabs(frz1), fabs(frz2)), 0.0));
+      double r = (fmx == 0.0) ? 0.0 : fabs(bs) * g.eun / fmx;
+      rm = r;
+      rmax = r;
+      key = ((long long)i * NP + j) * LP + k;
+    }
+  }
+  double s = wave_sum(rm);
+  // argmax: largest r, smallest reference loop-order key on ties
+  for (int o = 32; o > 0; o >>= 1) {
+    double r2 = __shfl_down(rmax, o, 64);
+    long long k2 = __shfl_down(key, o, 64);
+    if (r2 > rmax || (r2 == rmax && k2 < key)) { rmax = r2; key = k2; }
+  }
+  if (threadIdx.x == 0) {
+    long long bl = block_lin();
+    long long nb = (long long)gridDim.x * gridDim.y * gridDim.z;
+    part[bl] = s;
+    part[nb + bl] = rmax;
+    ((long long *)part)[2 * nb + bl] = key;
+  }
+}
+__global__ void k_pcorr_finalize(const double *part, long long nb) {
+  __shared__ double ss[256];
+  __shared__ double sm[256];
+  __shared__ long long sk[256];
+  double s = 0.0, m = 0.0;
+  long long key = 0x7fffffffffffffffLL;
+  for (long long b = threadIdx.x; b < nb; b += 256) {
+    s += part[b];
+    double r = part[nb + b];
+    long long k2 = ((const long long *)part)[2 * nb + b];
+    if (r > m || (r == m && k2 < key)) { m = r; key = k2; }
+  }
+  ss[threadIdx.x] = s; sm[threadIdx.x] = m; sk[threadIdx.x] = key;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      ss[threadIdx.x] += ss[threadIdx.x + o];
+      double r2 = sm[threadIdx.x + o];
+      long long k2 = sk[threadIdx.x + o];
+      if (r2 > sm[threadIdx.x] || (r2 == sm[threadIdx.x] && k2 < sk[threadIdx.x])) { sm[threadIdx.x] = r2; sk[threadIdx.x] = k2; }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    g.scal[SC_RES_MASS] = ss[0] / g.ncells;
+    g.scal[SC_RES_MAX] = sm[0];
+    long long kk = sk[0];
+    if (sm[0] > 0.0) {
+      g.scal[SC_KMX] = (double)(kk % LP);
+      g.scal[SC_JMX] = (double)((kk / LP) % NP);
+      g.scal[SC_IMX] = (double)(kk / ((long long)LP * NP));
+    } else {
+      g.scal[SC_IMX] = g.scal[SC_JMX] = g.scal[SC_KMX] = 2.0;
+    }
+  }
+}
+__global__ void k_zero_fz(Box b) {
+  int i, j, k;
+  if (!box_pt(b, i, j, k)) return;
+  FZ(i, j, k) = 0.0;
+}
+// velocity correction, C/Gslv3v.f90:179-204
+__global__ void k_vel_correct(Box b, int iu1) {
+  int i, j, k;
+  if (!box_pt(b, i, j, k)) return;
+  long long q = lidx(i, j, k);
+  if (IBC(i, j, k) >= 1 || fabs(g.ap[q]) < SMALL20) return;
+  double vol = VOL_C(i, j, k), dpdx;
+  if (iu1 == 1) dpdx = (FZ(i - 1, j, k) - FZ(i + 1, j, k)) / DX(i);
+  else if (iu1 == 2) dpdx = (FZ(i, j - 1, k) - FZ(i, j + 1, k)) / DR(j) / RS(j);
+  else dpdx = (FZ(i, j, k - 1) - FZ(i, j, k + 1)) / DZ(k);
+  double g0 = g.eun * g.theta[q] * dpdx * vol / (g.ap[q] * g.rf[iu1 - 1]);
+  g.ug[iu1 - 1][q] = BOUND_U(g.ug[iu1 - 1][q] + g0);
+}
+
+// ======================================================================
+// EXTRV, C/Extr.f90:358-567: exit faces of direction iu.  Partials:
+// gfex, fuel_out, Tmean_ex, Tarea, area_ex (5 per block).
+// ======================================================================
+__global__ void k_extrv(Box b, int iu, int mex, double *part) {
+  int i, j, k;
+  double s[5] = {0, 0, 0, 0, 0};
+  if (box_pt(b, i, j, k) && (IBC(i, j, k) % 10) == 3) {
+    int done = 0;
+    for (int m = -1; m <= 1 && !done; m += 2) {
+      int ia = i, ja = j, ka = k;
+      if (iu == 1) ia += m; else if (iu == 2) ja += m; else ka += m;
+      if (IBC(ia, ja, ka) >= 1) continue;
+      int i1 = i, j1 = j, k1 = k, i0 = i, j0 = j, k0 = k;
+      if (iu == 1) { i1 = i + 2 * m; i0 = i - m; } else if (iu == 2) { j1 = j + 2 * m; j0 = j - m; } else { k1 = k + 2 * m; k0 = k - m; }
+      double *u1 = &UG(i1, j1, k1, iu);
+      if (*u1 * m > 0.0) *u1 = 0.0;
+      double area = AREA_C(i, j, k, iu);
+      s[4] += area;
+      double g0 = TH(i1, j1, k1) * DN(i1, j1, k1) * (*u1) * area;
+      if (mex == 0) {
+        s[0] += fabs(g0);
+        s[1] += GF(i, j, k, IYF) * fabs(g0) * g.dnst0 * g.ug0 * g.area0;
+        s[2] += T_(i, j, k) * fabs(g0);
+        s[3] += T_(i, j, k) * area;
+        done = 1;
+        break;
+      }
+      double fac = g.scal[SC_FACTOR];
+      *u1 = *u1 * fac;
+      g0 = g0 * fac;
+      if (iu != 3) {
+        double u = BOUND_U(g0 / (TH(i, j, k) * DN(i, j, k) * area));
+        UG(i, j, k, iu) = u;
+        UG(i0, j0, k0, iu) = u;
+      }
+      done = 1;
+    }
+    if (!done) {
+      // copy from an open neighbour face of the same class, in the
+      // reference's order for this direction
+      int c[4][3];
+      bool inr[4];
+      if (iu == 1) {
+        c[0][0] = i; c[0][1] = j - 2; c[0][2] = k; inr[0] = j > 2;
+        c[1][0] = i; c[1][1] = j + 2; c[1][2] = k; inr[1] = j < NP;
+        c[2][0] = i; c[2][1] = j; c[2][2] = k - 2; inr[2] = k > 2;
+        c[3][0] = i; c[3][1] = j; c[3][2] = k + 2; inr[3] = k < LP;
+      } else if (iu == 2) {
+        c[0][0] = i; c[0][1] = j; c[0][2] = k - 2; inr[0] = k > 2;
+        c[1][0] = i; c[1][1] = j; c[1][2] = k + 2; inr[1] = k < LP;
+        c[2][0] = i - 2; c[2][1] = j; c[2][2] = k; inr[2] = i > 2;
+        c[3][0] = i + 2; c[3][1] = j; c[3][2] = k; inr[3] = i < MP;
+      } else {
+        c[0][0] = i - 2; c[0][1] = j; c[0][2] = k; inr[0] = i > 2;
+        c[1][0] = i + 2; c[1][1] = j; c[1][2] = k; inr[1] = i < MP;
+        c[2][0] = i; c[2][1] = j - 2; c[2][2] = k; inr[2] = j > 2;
+        c[3][0] = i; c[3][1] = j + 2; c[3][2] = k; inr[3] = j < NP;
+      }
+      for (int n = 0; n < 4; n++) {
+        if (inr[n] && IBC(c[n][0], c[n][1], c[n][2]) <= 0) { UG(i, j, k, iu) = UG(c[n][0], c[n][1], c[n][2], iu); break; }
+      }
+    }
+  }
+  if (mex != 0) return;
+  for (int q = 0; q < 5; q++) {
+    double v = wave_sum(s[q]);
+    if (threadIdx.x == 0) part[(long long)q * g.pstride + block_lin()] = v;
+  }
+}
+
+// generic finalize: sum nb partials of slot q (stride 4096) into scal[dst]
+__global__ void k_sum_partials(const double *part, int nb, int nq, int dst0, int accumulate) {
+  __shared__ double ss[256];
+  for (int q = 0; q < nq; q++) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) s += part[(long long)q * g.pstride + b];
+    ss[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) ss[threadIdx.x] += ss[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) g.scal[dst0 + q] = accumulate ? g.scal[dst0 + q] + ss[0] : ss[0];
+    __syncthreads();
+  }
+}
+// FLX1D's GFIN and the outlet flow factor, C/Flx1d.f90:75-77, C/Gslv3v.f90:213-231
+__global__ void k_factor(int itr_gas) {
+  double *s = g.scal;
+  double gfex = s[SC_GFEX] * g.gmfr;
+  s[SC_GFEX] = gfex;
+  if (gfex < SMALL16) s[SC_TMEAN_EX] = s[SC_TAREA] * g.t0 / s[SC_AREA_EX];
+  else s[SC_TMEAN_EX] = s[SC_TMEAN_EX] * g.t0 * g.gmfr / gfex;
+  s[SC_GFIN] = s[SC_GFIN0] * g.gmfr;
+  if (itr_gas > 10) {
+    if (gfex < SMALL20) { s[SC_FACTOR] = 1.0; s[SC_FADJ] = 0.0; }
+    else {
+      double f = s[SC_GFIN] / gfex;
+      if (f > g.f_mx) { f = g.f_mx; s[SC_FADJ] = 0.0; }
+      else if (f < g.f_mn) { f = g.f_mn; s[SC_FADJ] = 0.0; }
+      s[SC_FACTOR] = f;
+    }
+  }
+}
+
+// ======================================================================
+// INTPV, C/Intp.f90:84-139,230-272: velocity component iu interpolated in
+// direction ix onto the box positions.
+// ======================================================================
+__device__ __forceinline__ double INTP1(int i, int j, int k, int ix) {
+  if (ix == 3) return (DZ(k) <= 0.0 || k > LP) ? 0.0 : (Z(k) - Z(k - 1)) / DZ(k);
+  if (ix == 2) return (DR(j) <= 0.0 || j > NP) ? 0.0 : (R(j) - R(j - 1)) / DR(j);
+  return (DX(i) <= 0.0 || i > MP) ? 0.0 : (X(i) - X(i - 1)) / DX(i);
+}
+__global__ void k_intpv(Box b, int iu, int ix, int pi, int pj, int pk) {
+  int i, j, k;
+  if (!box_pt(b, i, j, k)) return;
+  // box covers a class; parities fixed per launch
+  (void)pi; (void)pj; (void)pk;
+  int bc = IBC(i, j, k);
+  if (bc >= 1 && bc != 4) return;
+  double f1 = INTP1(i, j, k, ix), f2 = 1.0 - f1;
+  double v;
+  if (ix == 1) {
+    v = UG(i + 1, j, k, iu) * f1 + UG(i - 1, j, k, iu) * f2;
+  } else if (ix == 2) {
+    if (IBC(i, j + 1, k) == 2 && bc == 0) { f1 = 0.0; f2 = 1.0; }
+    else if (IBC(i, j - 1, k) == 2 && bc == 0) { f1 = 1.0; f2 = 0.0; }
+    v = UG(i, j + 1, k, iu) * f1 + UG(i, j - 1, k, iu) * f2;
+  } else {
+    if (IBC(i, j, k + 1) == 2 && bc == 0) { f1 = 0.0; f2 = 1.0; }
+    else if (IBC(i, j, k - 1) == 2 && bc == 0) { f1 = 1.0; f2 = 0.0; }
+    v = UG(i, j, k + 1, iu) * f1 + UG(i, j, k - 1, iu) * f2;
+  }
+  UG(i, j, k, iu) = BOUND_U(v);
+}
+
+// pressure referencing at the reference inlet cell, C/Gflow.f90:182-242
+__global__ void k_pref() {
+  int i = g.i_ref, j = g.j_ref, k = g.k_ref;
+  int i1 = i, j1 = j, k1 = k, i2 = i, j2 = j, k2 = k, nx = 0;
+  int MPM2 = MP - 2, NPM2 = NP - 2, LPM2 = LP - 2;
+  if (i < MPM2 && IBC(i + 2, j, k) == 0) { nx = 1; i1 = i + 2; i2 = (IBC(i + 4, j, k) == 0) ? i + 4 : i1; }
+  else if (i > 2 && IBC(i - 2, j, k) == 0) { nx = 1; i1 = i - 2; i2 = (IBC(i - 4, j, k) == 0) ? i - 4 : i1; }
+  else if (j < NPM2 && IBC(i, j + 2, k) == 0) { nx = 2; j1 = j + 2; j2 = (j < NPM2 && IBC(i, j + 4, k) <= 0) ? j + 4 : j1; }
+  else if (j > 2 && IBC(i, j - 2, k) <= 0) { nx = 2; j1 = j - 2; j2 = (j > 4 && IBC(i, j - 4, k) <= 0) ? j - 4 : j1; }
+  else if (k < LPM2 && IBC(i, j, k + 2) <= 0) { nx = 3; k1 = k + 2; k2 = (k < LPM2 && IBC(i, j, k + 4) <= 0) ? k + 4 : k1; }
+  else if (k > 2 && IBC(i, j, k - 2) <= 0) { nx = 3; k1 = k - 2; k2 = (k > 4 && IBC(i, j, k - 4) <= 0) ? k - 4 : k1; }
+  double f2 = 0.0;
+  int ibc0 = IBC(i, j, k) % 10;
+  if (nx == 3) { double d = Z(k2) - Z(k1); if (d != 0.0) { int k0 = (ibc0 == 2) ? (k + k1) / 2 : k; f2 = (Z(k0) - Z(k1)) / d; } }
+  else if (nx == 2) { double d = R(j2) - R(j1); if (d != 0.0) { int j0 = (ibc0 == 2) ? (j + j1) / 2 : j; f2 = (R(j0) - R(j1)) / d; } }
+  else if (nx == 1) { double d = X(i2) - X(i1); if (d != 0.0) { int i0 = (ibc0 == 2) ? (i + i1) / 2 : i; f2 = (X(i0) - X(i1)) / d; } }
+  double f1 = 1.0 - f2;
+  P_(i, j, k) = P_(i1, j1, k1) * f1 + P_(i2, j2, k2) * f2;
+  g.scal[SC_G0REF] = P_(i1, j1, k1) - g.p_ref;
+}
+
+// ======================================================================
+// scalar transport, C/Gslv3g.f90 + SOURCE(6..12) C/Source.f90:104-654
+// ======================================================================
+__global__ void k_copy_gf_fz(Box b, int l) {
+  int i, j, k;
+  if (!box_pt(b, i, j, k)) return;
+  long long q = lidx(i, j, k);
+  g.fz[q] = g.gf[l - 1][q];
+}
+// source + flux coefficients (GSLV3G0) + volume and relaxation terms of one
+// scalar equation at one cell.  igs=0 recomputes AS.  part[] collects
+// q_react for the enthalpy equation.
+__global__ void k_scalar_assemble(Box b, int l, int nelg, int igs, double sgm, int itr_gas, double *part) {
+  int i, j, k;
+  double qr = 0.0;
+  if (box_pt(b, i, j, k)) {
+    long long q = lidx(i, j, k);
+    const double fz = g.fz[q];
+    if (IBC(i, j, k) >= 1) {
+      g.bs[q] = fz;
+      g.sfp[q] = 0.0;
+      if (nelg == 6) { g.crate[q] = 0.0; g.hrate[q] = 0.0; }
+    } else {
+      double bs = 0.0, sfp = 0.0;
+      if (nelg == 6) {  // fuel, integral reaction model C/Source.f90:127-242
+        double d11 = fabs(UG(i + 1, j, k, 1) / DX(i)), d12 = fabs(UG(i - 1, j, k, 1) / DX(i));
+        double d21 = fabs(UG(i, j + 1, k, 2) / DR(j)), d22 = fabs(UG(i, j - 1, k, 2) / DR(j));
+        double d31 = fabs(UG(i, j, k + 1, 3) / DZ(k)), d32 = fabs(UG(i, j, k - 1, 3) / DZ(k));
+        double dtime = fmax(fmax(fmax(d11, d12), fmax(d21, d22)), fmax(fmax(d31, d32), 1.0e-14));
+        dtime = 1.0 / dtime;
+        double drtime = dtime * g.r0 / g.ug0;
+        double yf = g.gf[0][q];
+        double ayf = fmin(yf, g.gf[1][q] / g.stoich);
+        double cr = 0.0, hr = 0.0;
+        if (ayf >= 1.0e-5) {
+          double zp1 = g.gf[5][q] / WCO2, delr1;
+          if (zp1 <= SMALL20) delr1 = 0.0;
+          else { double zf = yf / WF; delr1 = zp1 / (zf + zp1); }
+          double time1 = ZY1(g.delr, g.timed, delr1, g.nr0);
+          double delr2 = ZY1(g.rtim, g.delrt, time1 + drtime, g.nr0);
+          double ayf0 = ayf / (1.0 - delr1) * g.dnst[q] * g.theta[q];
+          double sr = ayf0 * (delr2 - delr1) / dtime;
+          if (sr > 0.0) { cr = sr; sfp = -cr / (yf + SMALL20); hr = cr * g.q0; }
+        }
+        g.crate[q] = cr;
+        g.hrate[q] = hr;
+        bs = 0.0;
+      } else if (nelg == 7) {
+        bs = g.crate[q] * (WCO2 / WF);
+      } else if (nelg == 12) {
+        bs = -g.crate[q] * g.stoich;
+      } else if (nelg == 11) {
+        bs = 0.0;
+      } else if (nelg == 8) {  // enthalpy: GTE, heat release, source history C/Source.f90:278-520
+        double dudx, dvdx, dwdx, dudr, dvdr, dwdr, dudz, dvdz, dwdz;
+        if (IBC(i + 1, j, k) == 3 || IBC(i - 1, j, k) == 3) { dudx = dvdx = dwdx = 0.0; }
+        else {
+          dudx = (UG(i + 1, j, k, 1) - UG(i - 1, j, k, 1)) / DX(i);
+          dvdx = (UG(i + 1, j, k, 2) - UG(i - 1, j, k, 2)) / DX(i);
+          dwdx = (UG(i + 1, j, k, 3) - UG(i - 1, j, k, 3)) / DX(i);
+        }
+        if (IBC(i, j + 1, k) == 3 || IBC(i, j - 1, k) == 3) { dudr = dvdr = dwdr = 0.0; }
+        else {
+          dudr = (UG(i, j + 1, k, 1) - UG(i, j - 1, k, 1)) / DR(j);
+          dvdr = (UG(i, j + 1, k, 2) - UG(i, j - 1, k, 2)) / DR(j);
+          dwdr = (UG(i, j + 1, k, 3) - UG(i, j - 1, k, 3)) / DR(j);
+        }
+        if (IBC(i, j, k + 1) == 3 || IBC(i, j, k - 1) == 3) { dudz = dvdz = dwdz = 0.0; }
+        else {
+          dudz = (UG(i, j, k + 1, 1) - UG(i, j, k - 1, 1)) / DZ(k);
+          dvdz = (UG(i, j, k + 1, 2) - UG(i, j, k - 1, 2)) / DZ(k);
+          dwdz = (UG(i, j, k + 1, 3) - UG(i, j, k - 1, 3)) / DZ(k);
+        }
+        g.gte[q] = 2.0 * (dudx * dudx + dvdr * dvdr + 0.0 * 0.0 + dwdz * dwdz) + (dudr + dvdx) * (dudr + dvdx) +
+                   (dwdr + dvdz) * (dwdr + dvdz) + (dudz + dwdx) * (dudz + dwdx) - (dudx + dvdr + dwdz) * (dudx + dvdr + dwdz) / 1.5;
+        double hr = g.hrate[q];
+        bs = 0.0 + hr;
+        sfp = -0.0;
+        qr = hr * g.qr0 * (VOL_C(i, j, k) * g.vol0);
+        if (itr_gas > 1) bs = g.rf_hsource * bs + (1.0 - g.rf_hsource) * g.bsh[q];
+        g.bsh[q] = bs;
+      } else if (nelg == 9) {  // k, C/Source.f90:529-572
+        bs = 2.0 / (g.reyg * g.epsr) * g.theta[q] * g.tmu[q] * g.gte[q] - 0.0;
+        sfp = -g.taur * g.theta[q] * g.dnst[q] * g.gf[4][q] / (g.gf[3][q] + SMALL20);
+      } else if (nelg == 10) {  // eps, C/Source.f90:579-618
+        double cbl0 = 2.5 * sqrt(g.epsr) / g.taur * pow(CMU, 0.75);
+        double kk = g.gf[3][q];
+        double cbl = cbl0 * pow(kk, 1.5);
+        bs = 0.0;
+        if (MP > 8 && IBC(i + 1, j, k) == 1) { bs += BIGV * cbl / (X(i + 1) - X(i)); sfp = -BIGV; }
+        if (MP > 8 && IBC(i - 1, j, k) == 1) { bs += BIGV * cbl / (X(i) - X(i - 1)); sfp = -BIGV; }
+        if (NP > 8 && IBC(i, j + 1, k) == 1) { bs += BIGV * cbl / (R(j + 1) - R(j)); sfp = -BIGV; }
+        if (NP > 8 && IBC(i, j - 1, k) == 1) { bs += BIGV * cbl / (R(j) - R(j - 1)); sfp = -BIGV; }
+        if (LP > 8 && IBC(i, j, k + 1) == 1) { bs += BIGV * cbl / (Z(k + 1) - Z(k)); sfp = -BIGV; }
+        if (LP > 8 && IBC(i, j, k - 1) == 1) { bs += BIGV * cbl / (Z(k) - Z(k - 1)); sfp = -BIGV; }
+        if (!(sfp <= -BIGV)) {
+          double bt = g.theta[q];
+          bs = CT1 * CMU * bt / g.taur * g.dnst[q] * g.gte[q] * kk;
+          sfp = -CT2 * g.taur * bt * g.dnst[q] * g.gf[4][q] / (kk + SMALL20);
+        }
+      }
+      // GSLV3G0 coefficients, C/Gslv3g.f90:123-152
+      double ap = 0.0;
+      for (int iu = 1; iu <= 3; iu++)
+        for (int m = -1; m <= 1; m += 2) {
+          int m1 = (m + 3) / 2;
+          double *asp = &g.as[(iu - 1) + 3 * (m1 - 1)][q];
+          if (igs != 1) {
+            int i1, j1, k1, i2, j2, k2;
+            VINDX(iu, m, i, j, k, i1, j1, k1, i2, j2, k2);
+            int ibc0 = IBC(i2, j2, k2);
+            double a;
+            if (ibc0 <= 0) a = FLWD(i1, j1, k1, iu, m1, sgm);
+            else if (ibc0 == 2) a = FLW(i2, j2, k2, iu, m1);
+            else a = 0.0;
+            *asp = a;
+          }
+          ap += *asp;
+        }
+      // volume and under-relaxation, C/Gslv3g.f90:64-87
+      if (fabs(ap) <= SMALL20) {
+        ap = 1.0;
+        bs = fz;
+      } else {
+        double vol = VOL_C(i, j, k);
+        bs *= vol;
+        ap = ap - (sfp * vol);
+        ap = ap / g.rfl[l - 1];
+        bs = bs + (1.0 - g.rfl[l - 1]) * fz * ap;
+      }
+      g.ap[q] = ap;
+      g.bs[q] = bs;
+      g.sfp[q] = sfp;
+    }
+  }
+  if (nelg == 8) {
+    double v = wave_sum(qr);
+    if (threadIdx.x == 0) part[block_lin()] = v;
+  }
+}
+__global__ void k_bound_store_gf(Box b, int l) {
+  int i, j, k;
+  if (!box_pt(b, i, j, k)) return;
+  if (IBC(i, j, k) >= 1) return;
+  long long q = lidx(i, j, k);
+  double f = BOUND_GF(g.fz[q], l);
+  g.fz[q] = f;
+  g.gf[l - 1][q] = f;
+}
+
+// property update of open cells, C/Gflow.f90:302-382
+__global__ void k_props(Box b) {
+  int i, j, k;
+  if (!box_pt(b, i, j, k)) return;
+  if (IBC(i, j, k) >= 1) return;
+  long long q = lidx(i, j, k);
+  CONC(q);
+  ENTH_TO_T(q);
+  double t = g.t[q];
+  if (t > g.t_mx) { g.t[q] = g.t_mx; T_TO_ENTH(q); }
+  else if (t < g.t_mn) { g.t[q] = g.t_mn; T_TO_ENTH(q); }
+  g.p[q] = BOUND_P(g.p[q] - g.scal[SC_G0REF]);
+  DNST_CALC(q);
+  THER(q);
+}
+// T_ave_calc, C/Utility.f90:1485-1513 (partials: T*rho*V, rho*V, T*V, V, max T)
+__global__ void k_tave(Box b, double *part) {
+  int i, j, k;
+  double s[5] = {0, 0, 0, 0, 0};
+  if (box_pt(b, i, j, k) && IBC(i, j, k) == 0) {
+    long long q = lidx(i, j, k);
+    double vol = VOL_C(i, j, k), t = g.t[q], d = g.dnst[q];
+    s[0] = t * d * vol; s[1] = d * vol; s[2] = t * vol; s[3] = vol; s[4] = t;
+  }
+  for (int n = 0; n < 4; n++) {
+    double v = wave_sum(s[n]);
+    if (threadIdx.x == 0) part[(long long)n * g.pstride + block_lin()] = v;
+  }
+  double m = wave_max(s[4]);
+  if (threadIdx.x == 0) part[4LL * g.pstride + block_lin()] = m;
+}
+__global__ void k_tave_finalize(const double *part, int nb) {
+  __shared__ double ss[256];
+  for (int n = 0; n < 5; n++) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) s = (n == 4) ? fmax(s, part[(long long)n * g.pstride + b]) : s + part[(long long)n * g.pstride + b];
+    ss[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) ss[threadIdx.x] = (n == 4) ? fmax(ss[threadIdx.x], ss[threadIdx.x + o]) : ss[threadIdx.x] + ss[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) g.scal[SC_TAVE_NUM + n] = ss[0];
+    __syncthreads();
+  }
+}
+
+// ======================================================================
+// EXTR(1), C/Extr.f90:31-250
+// ======================================================================
+// extrapolation stencil (EXT0 mode 0 / EXT1 mode 1), C/Extr.f90:575-661.
+// A cell with no qualifying neighbour (interior corner of the wall
+// shell) extrapolates from itself.
+__device__ int EXTIDX(int mode, int i, int j, int k, int &i1, int &j1, int &k1, int &i2, int &j2, int &k2) {
+  for (int d = 1; d <= 3; d++) {
+    int n0 = (d == 1), n1 = (d == 2), n2 = (d == 3);
+    for (int m = -2; m <= 2; m += 4) {
+      i1 = max(2, min(MP, i + m * n0)); j1 = max(2, min(NP, j + m * n1)); k1 = max(2, min(LP, k + m * n2));
+      int b1 = IBC(i1, j1, k1);
+      bool ok = (mode == 0) ? (b1 <= 0) : ((b1 % 10) != 1);
+      if (ok) {
+        i2 = max(2, min(MP, i1 + m * n0)); j2 = max(2, min(NP, j1 + m * n1)); k2 = max(2, min(LP, k1 + m * n2));
+        if ((IBC(i2, j2, k2) % 10) == 1) { i2 = i1; j2 = j1; k2 = k1; }
+        return d;
+      }
+    }
+  }
+  i1 = i; j1 = j; k1 = k; i2 = i; j2 = j; k2 = k;
+  return 4;
+}
+__device__ __forceinline__ void EXTR1F(double &f1, double &f2, int i, int j, int k, int i1, int j1, int k1, int i2, int j2, int k2, int ix) {
+  f2 = 0.0;
+  int ibc0 = IBC(i, j, k) % 10;
+  if (ix == 3) { double d = Z(k2) - Z(k1); if (d != 0.0) { int k0 = (ibc0 == 2) ? (k + k1) / 2 : k; f2 = (Z(k0) - Z(k1)) / d; } }
+  else if (ix == 2) { double d = R(j2) - R(j1); if (d != 0.0) { int j0 = (ibc0 == 2) ? (j + j1) / 2 : j; f2 = (R(j0) - R(j1)) / d; } }
+  else if (ix == 1) { double d = X(i2) - X(i1); if (d != 0.0) { int i0 = (ibc0 == 2) ? (i + i1) / 2 : i; f2 = (X(i0) - X(i1)) / d; } }
+  f1 = 1.0 - f2;
+}
+// inlets and exits (partials: GFIN0, q_exhaust)
+__global__ void k_extr_inout(Box b, double *part) {
+  int i, j, k;
+  double s0 = 0.0, s1 = 0.0;
+  if (box_pt(b, i, j, k)) {
+    int ibc0 = IBC(i, j, k) % 10;
+    if (ibc0 == 2 || ibc0 == 3) {
+      int i1, j1, k1, i2, j2, k2;
+      int ix = EXTIDX(0, i, j, k, i1, j1, k1, i2, j2, k2);
+      double f1, f2;
+      EXTR1F(f1, f2, i, j, k, i1, j1, k1, i2, j2, k2, ix);
+      long long q = lidx(i, j, k), q1 = lidx(i1, j1, k1), q2 = lidx(i2, j2, k2);
+      g.gf[4][q] = g.gf[4][q1];
+      g.p[q] = g.p[q1] * f1 + g.p[q2] * f2;
+      if (ibc0 == 2) {
+        double t0 = g.tmu[q];
+        THER(q);
+        g.tmu[q] = t0;
+        g.gama[q] = g.gama[q1];
+        double area = (ix <= 3) ? AREA_C(i, j, k, ix) : 0.0;
+        s0 = fabs(g.theta[q] * g.dnst[q] * ((ix <= 3) ? g.ug[ix - 1][q] : 0.0) * area);
+      } else {
+        g.gcp[q] = g.gcp[q1] * f1 + g.gcp[q2] * f2;
+        g.glam[q] = g.glam[q1] * f1 + g.glam[q2] * f2;
+        g.tmu[q] = g.tmu[q1];
+        g.gama[q] = g.gama[q1];
+        for (int l = 1; l <= 7; l++) {
+          if (l == IEPS) continue;
+          double v = g.gf[l - 1][q1] * f1 + g.gf[l - 1][q2] * f2;
+          if (l == IK) v = fmax(v, 1.0e-6);
+          else if (l != IH) v = fmax(v, 0.0);
+          g.gf[l - 1][q] = v;
+        }
+        ENTH_TO_T(q);
+        DNST_CALC(q);
+        double area = AREA_C(i, j, k, ix) * g.area0;
+        double g0 = fabs(g.ug[ix - 1][q]) * g.ug0 * area;
+        g0 = g0 * g.theta[q] * g.dnst[q] * g.dnst0;
+        s1 = g.gf[2][q] * g.h_0 * g0;
+        // SBC4 onto the exit face between the cell and its neighbour
+        long long d = lidx((i + i1) / 2, (j + j1) / 2, (k + k1) / 2);
+        g.p[d] = g.p[q]; g.t[d] = g.t[q];
+        for (int l = 0; l < 9; l++) g.gf[l][d] = g.gf[l][q];
+        g.dnst[d] = g.dnst[q]; g.theta[d] = g.theta[q]; g.tmu[d] = g.tmu[q];
+        g.glam[d] = g.glam[q]; g.gcp[d] = g.gcp[q]; g.gama[d] = g.gama[q];
+      }
+    }
+  }
+  double v0 = wave_sum(s0), v1 = wave_sum(s1);
+  if (threadIdx.x == 0) { part[block_lin()] = v0; part[g.pstride + block_lin()] = v1; }
+}
+__global__ void k_extr_wall(Box b) {
+  int i, j, k;
+  if (!box_pt(b, i, j, k)) return;
+  if ((IBC(i, j, k) % 10) != 1) return;
+  int i1, j1, k1, i2, j2, k2;
+  int ix = EXTIDX(1, i, j, k, i1, j1, k1, i2, j2, k2);
+  double f1, f2;
+  EXTR1F(f1, f2, i, j, k, i1, j1, k1, i2, j2, k2, ix);
+  long long q = lidx(i, j, k), q1 = lidx(i1, j1, k1), q2 = lidx(i2, j2, k2);
+  g.p[q] = g.p[q1] * f1 + g.p[q2] * f2;
+  g.gcp[q] = g.gcp[q1] * f1 + g.gcp[q2] * f2;
+  g.glam[q] = g.glam[q1] * f1 + g.glam[q2] * f2;
+  g.tmu[q] = g.tmu[q2];
+  g.gama[q] = g.gama[q2];
+  for (int l = 1; l <= 7; l++)
+    if (l != IH) g.gf[l - 1][q] = g.gf[l - 1][q1];
+  if (g.i1st_rad_done == 0 && !g.isothermal_walls) {
+    g.gf[2][q] = g.gf[2][q1];
+    g.t[q] = g.t[q1];
+  }
+  DNST_CALC(q);
+}
+
+// ======================================================================
+// INTP, C/Intp.f90:32-72.  The weight of each position is the reference's
+// loop-carried F1 (modified in place at inlet faces by INTP0) and is
+// precomputed per position in intp_code: 0 -> INTP1 weight, 1 -> 1.0,
+// 2 -> 0.0.  dir: 3 = from (k+1,k-1), 2 = (j+1,j-1), 1 = (i+1,i-1).
+// ======================================================================
+__global__ void k_intp(Box b, int dir) {
+  int i, j, k;
+  if (!box_pt(b, i, j, k)) return;
+  int bc = IBC(i, j, k);
+  if (bc == 1 || bc == 3) return;
+  long long q = lidx(i, j, k);
+  int code = g.intp_code[q];
+  double a = (code == 1) ? 1.0 : (code == 2) ? 0.0 : INTP1(i, j, k, dir);
+  double bb = 1.0 - a;
+  long long q1, q2;
+  if (dir == 3) { q1 = lidx(i, j, k + 1); q2 = lidx(i, j, k - 1); }
+  else if (dir == 2) { q1 = lidx(i, j + 1, k); q2 = lidx(i, j - 1, k); }
+  else { q1 = lidx(i + 1, j, k); q2 = lidx(i - 1, j, k); }
+  g.p[q] = a * g.p[q1] + bb * g.p[q2];
+  g.t[q] = a * g.t[q1] + bb * g.t[q2];
+  for (int l = 0; l < 7; l++) g.gf[l][q] = a * g.gf[l][q1] + bb * g.gf[l][q2];
+  CONC(q);
+  g.tmu[q] = a * g.tmu[q1] + bb * g.tmu[q2];
+  g.gcp[q] = a * g.gcp[q1] + bb * g.gcp[q2];
+  g.glam[q] = a * g.glam[q1] + bb * g.glam[q2];
+  g.gama[q] = a * g.gama[q1] + bb * g.gama[q2];
+  g.theta[q] = a * g.theta[q1] + bb * g.theta[q2];
+  DNST_CALC(q);
+}
+
+// exit temperature of SPHASE, C/Sphase.f90:74-89 (partials: area, T*area, max T)
+__global__ void k_exit_t(Box b, double *part) {
+  int i, j, k;
+  double s0 = 0.0, s1 = 0.0, mx = 0.0;
+  if (box_pt(b, i, j, k) && IBC(i, j, k) == 3) {
+    double g0 = AREA_C(i, j, k, 3);
+    if (g0 < SMALL20) g0 = AREA_C(i, j, k, 2);
+    if (g0 < SMALL20) g0 = AREA_C(i, j, k, 1);
+    double t = T_(i, j, k);
+    s0 = g0; s1 = t * g0; mx = t;
+  }
+  double v0 = wave_sum(s0), v1 = wave_sum(s1), v2 = wave_max(mx);
+  if (threadIdx.x == 0) { part[block_lin()] = v0; part[g.pstride + block_lin()] = v1; part[2 * g.pstride + block_lin()] = v2; }
+}
+
+// ======================================================================
+// residual of an assembled system, calc_residual C/Adlbl3.f90:239-271
+// (partials of the normalised sum) and the IRC block C/Adlbl3.f90:60-94
+// (partials of sum and max of |r|).
+// ======================================================================
+__global__ void k_residual(Box b, const double *xt, double *part, int irc) {
+  int i, j, k;
+  double s = 0.0, s2 = 0.0, mx = 0.0;
+  if (box_pt(b, i, j, k) && IBC(i, j, k) <= 0) {
+    long long q = lidx(i, j, k);
+    double t1 = g.as[0][q] * xt[lidx(i - 2, j, k)], t2 = g.as[3][q] * xt[lidx(i + 2, j, k)];
+    double t3 = g.as[1][q] * xt[lidx(i, j - 2, k)], t4 = g.as[4][q] * xt[lidx(i, j + 2, k)];
+    double t5 = g.as[2][q] * xt[lidx(i, j, k - 2)], t6 = g.as[5][q] * xt[lidx(i, j, k + 2)];
+    double t7 = g.ap[q] * xt[q], t8 = g.bs[q];
+    if (!irc) {
+      double eq_max = fmax(fmax(fmax(fabs(t1), fabs(t2)), fmax(fabs(t3), fabs(t4))), fmax(fmax(fabs(t5), fabs(t6)), fmax(fabs(t7), fabs(t8))));
+      s = fabs(t1 + t2 + t3 + t4 + t5 + t6 + t8 - t7) / eq_max;
+    } else if (g.ap[q] != 1.0) {
+      double r = fabs(t2 + t1 + t4 + t3 + t6 + t5 + t8 - t7);
+      s2 = r; mx = r;
+    }
+  }
+  if (!irc) {
+    double v = wave_sum(s);
+    if (threadIdx.x == 0) part[block_lin()] = v;
+  } else {
+    double v = wave_sum(s2), m = wave_max(mx);
+    if (threadIdx.x == 0) { part[block_lin()] = v; part[g.pstride + block_lin()] = m; }
+  }
+}
+__global__ void k_residual_finalize(const double *part, int nb, int slot, int irc) {
+  __shared__ double ss[256];
+  __shared__ double sm[256];
+  double s = 0.0, m = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) {
+    s += part[b];
+    if (irc) m = fmax(m, part[g.pstride + b]);
+  }
+  ss[threadIdx.x] = s; sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) { ss[threadIdx.x] += ss[threadIdx.x + o]; sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + o]); }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (!irc) g.scal[slot] = fmax(ss[0] / g.ncells, 1.0e-16);
+    else { g.scal[slot] = ss[0] / g.ncells; g.scal[slot + 1] = sm[0]; }
+  }
+}
+
+// ======================================================================
+// ADLBL3 line relaxation, C/Adlbl3.f90:10-221.  One wavefront solves one
+// y-line (dir 2) or z-line (dir 3) of one x-plane by parallel cyclic
+// reduction in LDS.  Rows that the reference's TDMA pins (blocked cells,
+// zero pivot) become identity rows carrying the current value.  The
+// neighbour values along x and along the other in-plane direction come
+// from the current XT, as in ADLZY/ADLYZ.  Result goes to xo.
+// ======================================================================
+#define PCR_MAXN 256
+__global__ __launch_bounds__(64) void k_line_pcr(const double *xt, double *xo, int dir, int pi0, int pdi, int npl,
+                                                  int jb, int je, int kb, int ke) {
+  __shared__ double A[PCR_MAXN], B[PCR_MAXN], Cc[PCR_MAXN], D[PCR_MAXN];
+  const int lane = threadIdx.x;
+  // line id -> (plane, other index)
+  int nother = (dir == 2) ? (ke - kb) / 2 + 1 : (je - jb) / 2 + 1;
+  int line = blockIdx.x;
+  int pl = line / nother, o = line % nother;
+  if (pl >= npl) return;
+  int i = pi0 + pdi * pl;
+  int n = (dir == 2) ? (je - jb) / 2 + 1 : (ke - kb) / 2 + 1;
+  // load rows
+  for (int e = lane; e < n; e += 64) {
+    int j = (dir == 2) ? jb + 2 * e : jb + 2 * o;
+    int k = (dir == 2) ? kb + 2 * o : kb + 2 * e;
+    long long q = lidx(i, j, k);
+    double ap = g.ap[q];
+    double xv = xt[q];
+    if (g.ib[q] >= 1 || fabs(ap) <= SMALL20) {
+      A[e] = 0.0; B[e] = 1.0; Cc[e] = 0.0; D[e] = xv;
+      continue;
+    }
+    double bsy = g.bs[q];
+    bsy = bsy + g.as[3][q] * xt[lidx(i + 2, j, k)] + g.as[0][q] * xt[lidx(i - 2, j, k)];
+    double am, apn;
+    if (dir == 2) {
+      bsy = bsy + g.as[5][q] * xt[lidx(i, j, k + 2)] + g.as[2][q] * xt[lidx(i, j, k - 2)];
+      am = g.as[1][q]; apn = g.as[4][q];
+      if (e == 0) { bsy += am * xt[lidx(i, j - 2, k)]; am = 0.0; }
+      if (e == n - 1) { bsy += apn * xt[lidx(i, j + 2, k)]; apn = 0.0; }
+    } else {
+      bsy = bsy + g.as[4][q] * xt[lidx(i, j + 2, k)] + g.as[1][q] * xt[lidx(i, j - 2, k)];
+      am = g.as[2][q]; apn = g.as[5][q];
+      if (e == 0) { bsy += am * xt[lidx(i, j, k - 2)]; am = 0.0; }
+      if (e == n - 1) { bsy += apn * xt[lidx(i, j, k + 2)]; apn = 0.0; }
+    }
+    A[e] = -am; B[e] = ap; Cc[e] = -apn; D[e] = bsy;
+  }
+  __syncthreads();
+  double na[PCR_MAXN / 64], nb_[PCR_MAXN / 64], nc[PCR_MAXN / 64], nd[PCR_MAXN / 64];
+  for (int s = 1; s < n; s <<= 1) {
+#pragma unroll
+    for (int t = 0; t < PCR_MAXN / 64; t++) {
+      int e = lane + 64 * t;
+      if (e >= n) continue;
+      double ae = A[e], be = B[e], ce = Cc[e], de = D[e];
+      double am = 0.0, bm = 1.0, cm = 0.0, dm = 0.0, ap2 = 0.0, bp = 1.0, cp = 0.0, dp = 0.0;
+      if (e - s >= 0) { am = A[e - s]; bm = B[e - s]; cm = Cc[e - s]; dm = D[e - s]; }
+      if (e + s < n) { ap2 = A[e + s]; bp = B[e + s]; cp = Cc[e + s]; dp = D[e + s]; }
+      double al = -ae / bm, ga = -ce / bp;
+      na[t] = al * am;
+      nc[t] = ga * cp;
+      nb_[t] = be + al * cm + ga * ap2;
+      nd[t] = de + al * dm + ga * dp;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < PCR_MAXN / 64; t++) {
+      int e = lane + 64 * t;
+      if (e >= n) continue;
+      A[e] = na[t]; B[e] = nb_[t]; Cc[e] = nc[t]; D[e] = nd[t];
+    }
+    __syncthreads();
+  }
+  for (int e = lane; e < n; e += 64) {
+    int j = (dir == 2) ? jb + 2 * e : jb + 2 * o;
+    int k = (dir == 2) ? kb + 2 * o : kb + 2 * e;
+    xo[lidx(i, j, k)] = D[e] / B[e];
+  }
+}
+// XT = (XT2 + XT3)/2 on the planes of one sweep step, C/Adlbl3.f90:134-138
+__global__ void k_line_avg(double *xt, const double *x2, const double *x3, int pi0, int pdi, int npl, int jb, int nj, int kb, int nk) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  long long tot = (long long)npl * nj * nk;
+  if (t >= tot) return;
+  int pl = t / (nj * nk), r = t % (nj * nk);
+  int kk = r / nj, jj = r % nj;
+  int i = pi0 + pdi * pl, j = jb + 2 * jj, k = kb + 2 * kk;
+  long long q = lidx(i, j, k);
+  xt[q] = (x2[q] + x3[q]) / 2.0;
+}
+
+// ======================================================================
+// layout conversion between the reference's Fortran-ordered lattice and
+// the class layout
+// ======================================================================
+__global__ void k_scatter(const double *src, double *dst, long long n) {
+  long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  int i = (int)(f % MP) + 1;
+  int j = (int)((f / MP) % NP) + 1;
+  int k = (int)(f / ((long long)MP * NP)) + 1;
+  dst[lidx(i, j, k)] = src[f];
+}
+__global__ void k_gather(const double *src, double *dst, long long n) {
+  long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  int i = (int)(f % MP) + 1;
+  int j = (int)((f / MP) % NP) + 1;
+  int k = (int)(f / ((long long)MP * NP)) + 1;
+  dst[f] = src[lidx(i, j, k)];
+}
+__global__ void k_scatter_i16(const int16_t *src, int16_t *dst, long long n) {
+  long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  int i = (int)(f % MP) + 1;
+  int j = (int)((f / MP) % NP) + 1;
+  int k = (int)(f / ((long long)MP * NP)) + 1;
+  dst[lidx(i, j, k)] = src[f];
+}
+
+#undef MP
+#undef NP
+#undef LP
+#undef X
+#undef DX
+#undef R
+#undef DR
+#undef RS
+#undef Z
+#undef DZ
+#undef IBC
+#undef P_
+#undef T_
+#undef DN
+#undef TH
+#undef TMU
+#undef GCP
+#undef GLAM
+#undef GAMA
+#undef GTE
+#undef CRATE
+#undef HRATE
+#undef UG
+#undef GF
+#undef FZ
+#undef BS
+#undef SFP
+#undef AP
+#undef AS
+
+// ---------------- host-side binding of the constants ----------------
+hipError_t dev_bind_case(const DevCase &c) {
+  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(g), &c, sizeof(DevCase));
+  if (e != hipSuccess) return e;
+  e = hipMemcpyToSymbol(HIP_SYMBOL(c_nist), h_nist_tab, sizeof(h_nist_tab));
+  if (e != hipSuccess) return e;
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_cutT), h_cutT, sizeof(h_cutT));
+}

@@ -1,0 +1,149 @@
+"""Host mirror of the reference's SPHASE/GFLOW/ADLBL3 interface over the
+HIP C-ABI (include/gfm_hip.h).
+
+The reference drives the combustion space through argument-less Fortran
+subroutines acting on module state (SPHASE C/Sphase.f90:10, GFLOW
+C/Gflow.f90:6, ADLBL3 C/Adlbl3.f90:10).  ``CombustionSpace`` keeps that
+shape: it owns one case's device state and exposes ``sphase``, ``gflow``,
+``adlbl3`` and ``calc_residual`` with the reference's argument meaning.
+Errors from the library raise ``GfmError`` (the reference's stop_run).
+There is no CPU fallback: without the built library every call raises.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+
+from .abi import GfmCtrl, host_arrays
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgfm_hip.so")
+_lib = None
+
+
+class GfmError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libgfm_hip.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GfmError(f"{LIB_PATH} is missing: build it with `make -C gfm_amd/csrc` "
+                           "(or __graft_entry__.build()); there is no CPU fallback")
+        L = ct.CDLL(LIB_PATH)
+        P = ct.c_void_p
+        D = ct.POINTER(ct.c_double)
+        L.gfm_hip_last_error.restype = ct.c_char_p
+        L.gfm_hip_ctrl_size.restype = ct.c_size_t
+        L.gfm_hip_init.argtypes = [ct.POINTER(GfmCtrl), P, ct.c_int, ct.POINTER(P)]
+        L.gfm_hip_upload.argtypes = [P, ct.POINTER(GfmCtrl), P]
+        L.gfm_hip_download.argtypes = [P, ct.POINTER(GfmCtrl), P]
+        L.gfm_hip_initsv_tail.argtypes = [P, ct.POINTER(GfmCtrl)]
+        L.gfm_hip_gflow.argtypes = [P, ct.POINTER(GfmCtrl)]
+        L.gfm_hip_sphase.argtypes = [P, ct.POINTER(GfmCtrl), ct.c_int, D, ct.POINTER(ct.c_int)]
+        L.gfm_hip_adlbl3.argtypes = [P, ct.POINTER(GfmCtrl), ct.c_int, D, D, D, D] + [ct.c_int] * 6 + [D]
+        L.gfm_hip_calc_residual.argtypes = [P, D, D, D, D] + [ct.c_int] * 6 + [D]
+        L.gfm_hip_finalize.argtypes = [P]
+        L.gfm_hip_kernel_times.argtypes = [P, ct.c_int, ct.c_char_p, ct.c_int, D, D, ct.POINTER(ct.c_int), ct.c_int,
+                                           ct.POINTER(ct.c_int)]
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = lib().gfm_hip_last_error().decode(errors="replace")
+        raise GfmError(f"{what} failed (code {rc}): {msg}")
+
+
+def _dp(a):
+    return a.ctypes.data_as(ct.POINTER(ct.c_double))
+
+
+class CombustionSpace:
+    """Device-resident state of one case (gfm_hip_init .. gfm_hip_finalize)."""
+
+    def __init__(self, case, device=0, initsv=True):
+        self.case = case
+        self._h = host_arrays(case.arrays)
+        self._ctx = ct.c_void_p()
+        _check(lib().gfm_hip_init(ct.byref(case.ctrl), ct.cast(ct.byref(self._h), ct.c_void_p), int(device),
+                                  ct.byref(self._ctx)), "gfm_hip_init")
+        if initsv:
+            self.initsv_tail()
+
+    # -- the reference's subroutine interface --
+    def initsv_tail(self):
+        _check(lib().gfm_hip_initsv_tail(self._ctx, ct.byref(self.case.ctrl)), "gfm_hip_initsv_tail")
+
+    def gflow(self):
+        """One GFLOW (one SIMPLER iteration) at itr_gas+1."""
+        self.case.ctrl.itr_gas += 1
+        _check(lib().gfm_hip_gflow(self._ctx, ct.byref(self.case.ctrl)), "gfm_hip_gflow")
+        return self.case.ctrl.res_mass
+
+    def sphase(self, max_iters):
+        """SPHASE: iterate until res_mass < bgcon, itr_gend/maxgi, or
+        max_iters; returns the res_mass history of the iterations run."""
+        hist = np.zeros(max(int(max_iters), 1))
+        n = ct.c_int(0)
+        rc = lib().gfm_hip_sphase(self._ctx, ct.byref(self.case.ctrl), int(max_iters), _dp(hist), ct.byref(n))
+        _check(rc, "gfm_hip_sphase")
+        return hist[: n.value]
+
+    def adlbl3(self, nelg, ap, as_, bs, xt, box):
+        """ADLBL3(XT,IB,IE,JB,JE,KB,KE) on host arrays (XT updated in place)."""
+        resg = np.zeros(2)
+        _check(lib().gfm_hip_adlbl3(self._ctx, ct.byref(self.case.ctrl), int(nelg), _dp(ap), _dp(as_), _dp(bs), _dp(xt),
+                                    *map(int, box), _dp(resg)), "gfm_hip_adlbl3")
+        return resg
+
+    def calc_residual(self, ap, as_, bs, xt, box):
+        r = np.zeros(1)
+        _check(lib().gfm_hip_calc_residual(self._ctx, _dp(ap), _dp(as_), _dp(bs), _dp(xt), *map(int, box), _dp(r)),
+               "gfm_hip_calc_residual")
+        return float(r[0])
+
+    def kernel_times(self, enable=-1, cap=128):
+        """Per-kernel HIP-event timings: enable=1 resets and starts, 0 stops,
+        -1 only reads.  Returns {name: (ms, algorithmic_bytes, launches)}."""
+        nl = 64
+        names = ct.create_string_buffer(nl * cap)
+        ms = np.zeros(cap)
+        by = np.zeros(cap)
+        la = (ct.c_int * cap)()
+        n = ct.c_int(0)
+        _check(lib().gfm_hip_kernel_times(self._ctx, int(enable), names, nl, _dp(ms), _dp(by), la, cap, ct.byref(n)),
+               "gfm_hip_kernel_times")
+        out = {}
+        for r in range(n.value):
+            nm = names.raw[r * nl:(r + 1) * nl].split(b"\0", 1)[0].decode()
+            out[nm] = (float(ms[r]), float(by[r]), int(la[r]))
+        return out
+
+    def download(self):
+        """Copy the device state back into case.arrays (sav3f/sprint point)."""
+        _check(lib().gfm_hip_download(self._ctx, None, ct.cast(ct.byref(self._h), ct.c_void_p)), "gfm_hip_download")
+        return self.case
+
+    def upload(self):
+        _check(lib().gfm_hip_upload(self._ctx, ct.byref(self.case.ctrl), ct.cast(ct.byref(self._h), ct.c_void_p)),
+               "gfm_hip_upload")
+
+    def close(self):
+        if self._ctx:
+            lib().gfm_hip_finalize(self._ctx)
+            self._ctx = ct.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

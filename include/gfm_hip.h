@@ -29,6 +29,7 @@
 #ifndef GFM_HIP_H
 #define GFM_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -162,15 +163,22 @@ int gfm_hip_calc_residual(gfm_ctx *ctx, const double *ap, const double *as, cons
                           const double *xt, int ib, int ie, int jb, int je, int kb, int ke,
                           double *resid);
 
-/* Per-kernel timing of the last gfm_hip_sphase call: fills up to `cap`
- * (name, total ms, launches) records; returns the count in *n. */
+/* Per-kernel timing with HIP events recorded on the library's stream around
+ * each launch.  enable=1 clears the records and starts recording, enable=0
+ * stops; every call returns up to `cap` records accumulated so far: name
+ * (names + r*name_len, NUL-terminated), total milliseconds, algorithmic
+ * bytes moved (the line-relaxation kernel counts 10 words per cell row:
+ * 8 coefficients, one read and one write of the unknown; other kernels 0)
+ * and launches.  *n receives the record count. */
 int gfm_hip_kernel_times(gfm_ctx *ctx, int enable, char *names, int name_len,
-                         double *ms, int *launches, int cap, int *n);
+                         double *ms, double *bytes, int *launches, int cap, int *n);
 
 int gfm_hip_finalize(gfm_ctx *ctx);
 
 const char *gfm_hip_last_error(void);
 int gfm_hip_abi_version(void);
+/* sizeof(gfm_ctrl) as compiled into the library, for binding checks. */
+size_t gfm_hip_ctrl_size(void);
 
 #ifdef __cplusplus
 }

@@ -1491,3 +1491,12 @@ void oracle_scratch(double *ap, double *as, double *bs, double *fz) {
 }
 int oracle_abi_version(void) { return GFM_ABI_VERSION; }
 size_t oracle_ctrl_size(void) { return sizeof(gfm_ctrl); }
+
+/* SOURCE(5) alone on the caller's state (C/Source.f90:41-101): returns
+ * res_mass, writes res_mass_max and its location into the ctrl block. */
+double oracle_mass_residual(gfm_ctrl *c, const gfm_host_arrays *h) {
+  if (bind(c, h)) return -1.0;
+  NELG = 5;
+  SOURCE(4, MPM2, NJY1, NJY2, LKZ1, LKZ2);
+  return c->res_mass;
+}

@@ -1,0 +1,48 @@
+"""CPU checks of the C-ABI: the library and the oracle load, export every
+symbol include/gfm_hip.h declares, and agree with the ctypes mirror on the
+size of the scalar block.  No compute calls (no GPU needed)."""
+import ctypes as ct
+import os
+import re
+import subprocess
+
+import pytest
+
+from gfm_amd import solver
+from gfm_amd.abi import GfmCtrl, GfmHostArrays
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    txt = open(os.path.join(ROOT, "include", "gfm_hip.h")).read()
+    return sorted(set(re.findall(r"\b(gfm_hip_\w+)\s*\(", txt)))
+
+
+def test_library_exports_header_symbols():
+    lib = solver.lib()
+    out = subprocess.run(["nm", "-D", "--defined-only", solver.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r" T (gfm_hip_\w+)", out))
+    missing = [s for s in declared() if s not in exported]
+    assert not missing, missing
+    for s in declared():
+        getattr(lib, s)
+
+
+def test_ctrl_layout():
+    lib = solver.lib()
+    assert lib.gfm_hip_ctrl_size() == ct.sizeof(GfmCtrl)
+    assert lib.gfm_hip_abi_version() == 1
+    assert ct.sizeof(GfmHostArrays) == 28 * 8
+
+
+def test_oracle_layout():
+    import oracle
+    assert oracle.lib().oracle_ctrl_size() == ct.sizeof(GfmCtrl)
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(solver, "_lib", None)
+    monkeypatch.setattr(solver, "LIB_PATH", str(tmp_path / "nope.so"))
+    with pytest.raises(solver.GfmError):
+        solver.lib()
