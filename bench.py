@@ -81,13 +81,13 @@ def main():
     res_final = float(hist[-1]) if steps else float("nan")
     value = n * steps / dt / 1e6
 
-    ms, by, la = kt.get("k_line_pcr", (0.0, 0.0, 0))
+    ms, by, la = kt.get("k_lines", (0.0, 0.0, 0))
     avg_s = ms / 1e3 / max(la, 1)
     per_launch = by / max(la, 1)
     achieved = per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_line_pcr", "avg_launch_us": round(avg_s * 1e6, 2), "bytes_per_launch": per_launch,
+            "kernel": "k_lines", "avg_launch_us": round(avg_s * 1e6, 2), "bytes_per_launch": per_launch,
             "launches": la}
     line = {
         "metric": "time to residual 1e-5 & Mcell-updates/s: learned+FP32 vs plain FP64, 1 GPU",

@@ -3,15 +3,18 @@
 // Every field of the reference lives on the doubled-index lattice
 // (MP,NP,LP) (centres at even indices, faces/edges/corners at odd ones,
 // C/Var_mod.f90).  On the device a lattice array is stored as its 8 parity
-// sub-lattices ("classes"), each a dense x-fastest block of
-// NXC*NYC*NZC entries with NXC = MP/2+1:
+// sub-lattices ("classes"), each a dense x-fastest block.  Within a row the
+// compact x index c = I>>1 is split by its parity (the zebra colour of the
+// x-plane in the line relaxation) so that planes of one colour are
+// adjacent in memory:
 //
-//   offset(I,J,K) = cls*S + ((K>>1)*NYC + (J>>1))*NXC + (I>>1),
-//   cls = (I&1) | (J&1)<<1 | (K&1)<<2,  S = NXC*NYC*NZC.
+//   xpos(c)      = (c&1)*HX + (c>>1),      HX = ceil(NXC/2), NXC = MP/2+1
+//   offset(I,J,K) = cls*S + ((K>>1)*NYC + (J>>1))*NXP + xpos(I>>1),
+//   cls = (I&1) | (J&1)<<1 | (K&1)<<2,  NXP = 2*HX,  S = NXP*NYC*NZC.
 //
-// Kernels keep the reference's lattice indices (I,J,K); neighbours at +-2
-// stay in the same class and sit +-1 apart, so a wavefront walking I of one
-// class reads contiguous memory.
+// Kernels keep the reference's lattice indices (I,J,K).  A wavefront
+// walking consecutive I of one class touches two contiguous segments; a
+// wavefront walking consecutive planes of one colour touches one.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -19,8 +22,16 @@
 struct DevGeom {
   int MP, NP, LP;
   int NXC, NYC, NZC;
+  int HX, NXP;  // half row and padded row length
   long long S;  // entries per class
 };
+
+__host__ __device__ __forceinline__ long long lat_index(const DevGeom &gg, int i, int j, int k) {
+  int cls = (i & 1) | ((j & 1) << 1) | ((k & 1) << 2);
+  int c = i >> 1;
+  int xp = (c & 1) * gg.HX + (c >> 1);
+  return (long long)cls * gg.S + ((long long)(k >> 1) * gg.NYC + (j >> 1)) * gg.NXP + xp;
+}
 
 // Everything the kernels read: pointers to device arrays and the scalar
 // constants of the case.  Held in __constant__ memory (one bound case at a

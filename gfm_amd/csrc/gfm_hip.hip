@@ -161,10 +161,7 @@ static void fill_devcase_scalars(gfm_ctx *c, const gfm_ctrl *k) {
 static void intp_codes(const gfm_ctrl *k, const int16_t *ibf, std::vector<int8_t> &code, const DevGeom &gg) {
   const int MPh = k->mp, NPh = k->np, LPh = k->lp;
   auto ib = [&](int i, int j, int kk) -> int { return ibf[(((size_t)kk - 1) * NPh + (j - 1)) * MPh + (i - 1)]; };
-  auto li = [&](int i, int j, int kk) -> long long {
-    int c = (i & 1) | ((j & 1) << 1) | ((kk & 1) << 2);
-    return (long long)c * gg.S + ((long long)(kk >> 1) * gg.NYC + (j >> 1)) * gg.NXC + (i >> 1);
-  };
+  auto li = [&](int i, int j, int kk) -> long long { return lat_index(gg, i, j, kk); };
   code.assign((size_t)gg.S * 8, 0);
   auto visit = [&](int &st, int i, int j, int kk, int i1, int j1, int k1, int i2, int j2, int k2) {
     if (ib(i, j, kk) == 3) return;
@@ -269,15 +266,21 @@ static void pull_scalars(gfm_ctx *c, gfm_ctrl *k, const double *sc) {
 // ADLBL3 (C/Adlbl3.f90:10-97) and calc_residual
 // ---------------------------------------------------------------------
 static void plane_op(gfm_ctx *c, double *xt, int pi0, int pdi, int npl, int jb, int je, int kb, int ke) {
-  int nyl = (ke - kb) / 2 + 1, nzl = (je - jb) / 2 + 1;
+  const int nyl = (ke - kb) / 2 + 1, nzl = (je - jb) / 2 + 1;  // y-lines per plane, z-lines per plane
+  const int ny = nzl, nz = nyl;                                  // rows of a y-line, of a z-line
   const double cells = (double)npl * nyl * nzl;
   const double line_bytes = cells * 10.0 * sizeof(double);
+  const int lw = (npl >= 16) ? 16 : 1;
+  const int ntile = (npl + lw - 1) / lw;
+  const int nmax = ny > nz ? ny : nz;
+  const size_t shm = 4 * (size_t)lw * (nmax | 1) * sizeof(double);
   for (int l = 0; l < 2; l++) {
     if (c->prof.on) prof_begin(c);
-    hipLaunchKernelGGL(k_line_pcr, dim3(npl * nyl), dim3(64), 0, c->stream, xt, c->xt2, 2, pi0, pdi, npl, jb, je, kb, ke);
-    if (c->prof.on) { prof_end(c, "k_line_pcr", line_bytes); prof_begin(c); }
-    hipLaunchKernelGGL(k_line_pcr, dim3(npl * nzl), dim3(64), 0, c->stream, xt, c->xt3, 3, pi0, pdi, npl, jb, je, kb, ke);
-    if (c->prof.on) { prof_end(c, "k_line_pcr", line_bytes); prof_begin(c); }
+    if (lw == 16)
+      hipLaunchKernelGGL(k_lines<16>, dim3(ntile * (nz + ny)), dim3(1024), shm, c->stream, xt, c->xt2, c->xt3, pi0, pdi, npl, jb, je, kb, ke);
+    else
+      hipLaunchKernelGGL(k_lines<1>, dim3(ntile * (nz + ny)), dim3(1024), shm, c->stream, xt, c->xt2, c->xt3, pi0, pdi, npl, jb, je, kb, ke);
+    if (c->prof.on) { prof_end(c, "k_lines", 2.0 * line_bytes); prof_begin(c); }
     long long tot = (long long)npl * nzl * nyl;
     hipLaunchKernelGGL(k_line_avg, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, xt, c->xt2, c->xt3, pi0, pdi,
                        npl, jb, nzl, kb, nyl);
@@ -514,7 +517,9 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
   DevGeom &gg = c->dc.g;
   gg.MP = k->mp; gg.NP = k->np; gg.LP = k->lp;
   gg.NXC = k->mp / 2 + 1; gg.NYC = k->np / 2 + 1; gg.NZC = k->lp / 2 + 1;
-  gg.S = (long long)gg.NXC * gg.NYC * gg.NZC;
+  gg.HX = (gg.NXC + 1) / 2;
+  gg.NXP = 2 * gg.HX;
+  gg.S = (long long)gg.NXP * gg.NYC * gg.NZC;
   c->ncls = gg.S * 8;
   int rc = 0;
   auto fail = [&](int r) { gfm_hip_finalize(c); return r; };
@@ -533,7 +538,7 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
   if ((rc = dalloc(c, &c->stage, N))) return fail(rc);
   if ((rc = dalloc(c, &c->stage16, N))) return fail(rc);
   // partial sums: 5 slots of the largest reduction grid
-  long long maxb = (long long)((gg.NXC + 63) / 64) * gg.NYC * gg.NZC + 64;
+  long long maxb = (long long)((gg.NXP + 63) / 64) * gg.NYC * gg.NZC + 64;
   d.pstride = maxb;
   if ((rc = dalloc(c, &c->part, 5 * maxb))) return fail(rc);
   double *scal;
@@ -562,6 +567,14 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
   d.rtim = kt[0]; d.delrt = kt[1]; d.delr = kt[2]; d.timed = kt[3];
   fill_devcase_scalars(c, k);
   if ((rc = bind(c))) return fail(rc);
+  {
+    int nmax = (k->np > k->lp ? k->np : k->lp) / 2;
+    int shm = 4 * 16 * (nmax | 1) * (int)sizeof(double);
+    if (shm > 64 * 1024) {
+      HCHECK(hipFuncSetAttribute((const void *)k_lines<16>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+      HCHECK(hipFuncSetAttribute((const void *)k_lines<1>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+    }
+  }
   // cell codes and INTP weight codes
   HCHECK(hipMemcpy(c->stage16, h->ibcell, nlat * sizeof(int16_t), hipMemcpyHostToDevice));
   hipLaunchKernelGGL(k_scatter_i16, dim3((unsigned)((nlat + 255) / 256)), dim3(256), 0, c->stream, c->stage16, ibd, nlat);

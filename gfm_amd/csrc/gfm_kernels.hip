@@ -33,10 +33,7 @@ static const double h_cutT[5] = {1300.0, 6000.0, 1200.0, 6000.0, 1700.0};
 #define CT2 1.92
 #define CPF1 (2.156e3 / 1.0057e3)
 
-__device__ __forceinline__ long long lidx(int i, int j, int k) {
-  int c = (i & 1) | ((j & 1) << 1) | ((k & 1) << 2);
-  return (long long)c * g.g.S + ((long long)(k >> 1) * g.g.NYC + (j >> 1)) * g.g.NXC + (i >> 1);
-}
+__device__ __forceinline__ long long lidx(int i, int j, int k) { return lat_index(g.g, i, j, k); }
 #define MP g.g.MP
 #define NP g.g.NP
 #define LP g.g.LP
@@ -584,8 +581,12 @@ __global__ void k_pcorr_finalize(const double *part, long long nb) {
   __shared__ long long sk[256];
   double s = 0.0, m = 0.0;
   long long key = 0x7fffffffffffffffLL;
+  double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  long long b0 = threadIdx.x;
+  for (; b0 + 768 < nb; b0 += 1024) { s += part[b0]; s1 += part[b0 + 256]; s2 += part[b0 + 512]; s3 += part[b0 + 768]; }
+  for (; b0 < nb; b0 += 256) s += part[b0];
+  s = (s + s1) + (s2 + s3);
   for (long long b = threadIdx.x; b < nb; b += 256) {
-    s += part[b];
     double r = part[nb + b];
     long long k2 = ((const long long *)part)[2 * nb + b];
     if (r > m || (r == m && k2 < key)) { m = r; key = k2; }
@@ -708,8 +709,12 @@ __global__ void k_extrv(Box b, int iu, int mex, double *part) {
 __global__ void k_sum_partials(const double *part, int nb, int nq, int dst0, int accumulate) {
   __shared__ double ss[256];
   for (int q = 0; q < nq; q++) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nb; b += 256) s += part[(long long)q * g.pstride + b];
+    double s = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    const double *pp = part + (long long)q * g.pstride;
+    int b = threadIdx.x;
+    for (; b + 768 < nb; b += 1024) { s += pp[b]; s1 += pp[b + 256]; s2 += pp[b + 512]; s3 += pp[b + 768]; }
+    for (; b < nb; b += 256) s += pp[b];
+    s = (s + s1) + (s2 + s3);
     ss[threadIdx.x] = s;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
@@ -1173,11 +1178,16 @@ __global__ void k_residual(Box b, const double *xt, double *part, int irc) {
 __global__ void k_residual_finalize(const double *part, int nb, int slot, int irc) {
   __shared__ double ss[256];
   __shared__ double sm[256];
-  double s = 0.0, m = 0.0;
-  for (int b = threadIdx.x; b < nb; b += 256) {
+  double s = 0.0, m = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int b = threadIdx.x;
+  if (!irc) {
+    for (; b + 768 < nb; b += 1024) { s += part[b]; s1 += part[b + 256]; s2 += part[b + 512]; s3 += part[b + 768]; }
+  }
+  for (; b < nb; b += 256) {
     s += part[b];
     if (irc) m = fmax(m, part[g.pstride + b]);
   }
+  s = (s + s1) + (s2 + s3);
   ss[threadIdx.x] = s; sm[threadIdx.x] = m;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
@@ -1191,94 +1201,125 @@ __global__ void k_residual_finalize(const double *part, int nb, int slot, int ir
 }
 
 // ======================================================================
-// ADLBL3 line relaxation, C/Adlbl3.f90:10-221.  One wavefront solves one
-// y-line (dir 2) or z-line (dir 3) of one x-plane by parallel cyclic
-// reduction in LDS.  Rows that the reference's TDMA pins (blocked cells,
-// zero pivot) become identity rows carrying the current value.  The
+// ADLBL3 line relaxation, C/Adlbl3.f90:10-221.  Each y-line (along J) or
+// z-line (along K) of an x-plane is a tridiagonal system solved by
+// parallel cyclic reduction.  Rows that the reference's TDMA pins (blocked
+// cells, zero pivot) become identity rows carrying the current value; the
 // neighbour values along x and along the other in-plane direction come
-// from the current XT, as in ADLZY/ADLYZ.  Result goes to xo.
+// from the current XT, as in ADLZY/ADLYZ.
 // ======================================================================
 #define PCR_MAXN 256
-__global__ __launch_bounds__(64) void k_line_pcr(const double *xt, double *xo, int dir, int pi0, int pdi, int npl,
-                                                  int jb, int je, int kb, int ke) {
-  __shared__ double A[PCR_MAXN], B[PCR_MAXN], Cc[PCR_MAXN], D[PCR_MAXN];
-  const int lane = threadIdx.x;
-  // line id -> (plane, other index)
-  int nother = (dir == 2) ? (ke - kb) / 2 + 1 : (je - jb) / 2 + 1;
-  int line = blockIdx.x;
-  int pl = line / nother, o = line % nother;
-  if (pl >= npl) return;
-  int i = pi0 + pdi * pl;
-  int n = (dir == 2) ? (je - jb) / 2 + 1 : (ke - kb) / 2 + 1;
-  // load rows
-  for (int e = lane; e < n; e += 64) {
+
+// XT = (XT2 + XT3)/2 on the planes of one sweep step, C/Adlbl3.f90:134-138.
+// Consecutive threads take consecutive planes of one colour (contiguous).
+__global__ void k_line_avg(double *xt, const double *x2, const double *x3, int pi0, int pdi, int npl, int jb, int nj, int kb, int nk) {
+  long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long tot = (long long)npl * nj * nk;
+  if (t >= tot) return;
+  int pl = (int)(t % npl);
+  long long r = t / npl;
+  int jj = (int)(r % nj), kk = (int)(r / nj);
+  int i = pi0 + pdi * pl, j = jb + 2 * jj, k = kb + 2 * kk;
+  long long q = lidx(i, j, k);
+  xt[q] = (x2[q] + x3[q]) / 2.0;
+}
+
+// Line relaxation, tiled.  A 1024-thread workgroup takes LW lines that sit
+// at the same (k) -- y-lines -- or the same (j) -- z-lines -- on LW
+// consecutive planes of one colour, which are adjacent in memory.  The
+// workgroup loads the rows of the tile cooperatively (16 consecutive
+// threads read 16 adjacent doubles), forms the tridiagonal rows of
+// ADLZY/ADLYZ (C/Adlbl3.f90:147-221) in LDS as [line][row], then each
+// wavefront solves one line by parallel cyclic reduction and the tile is
+// written back cooperatively.  Blocks [0, ntile*nz) do y-lines into xy,
+// the rest z-lines into xz.
+template <int LW>
+__global__ __launch_bounds__(1024) void k_lines(const double *__restrict__ xt, double *__restrict__ xy, double *__restrict__ xz,
+                                                int pi0, int pdi, int npl, int jb, int je, int kb, int ke) {
+  extern __shared__ double sm[];
+  const int ny = (je - jb) / 2 + 1, nz = (ke - kb) / 2 + 1;
+  const int ntile = (npl + LW - 1) / LW;
+  int b = blockIdx.x, dir, n, o, tile;
+  double *xo;
+  if (b < ntile * nz) { dir = 2; n = ny; tile = b % ntile; o = b / ntile; xo = xy; }
+  else { b -= ntile * nz; dir = 3; n = nz; tile = b % ntile; o = b / ntile; xo = xz; }
+  const int npad = n | 1;
+  double *A = sm, *B = A + LW * npad, *Cc = B + LW * npad, *D = Cc + LW * npad;
+  const int t = threadIdx.x;
+  const int w = t % LW, rg = t / LW, nrg = 1024 / LW;
+  const int pl = tile * LW + w;
+  const bool valid = pl < npl;
+  const int i = pi0 + pdi * (valid ? pl : 0);
+  for (int e = rg; e < n; e += nrg) {
+    if (!valid) break;
     int j = (dir == 2) ? jb + 2 * e : jb + 2 * o;
     int k = (dir == 2) ? kb + 2 * o : kb + 2 * e;
     long long q = lidx(i, j, k);
     double ap = g.ap[q];
-    double xv = xt[q];
+    double a, bb, c, d;
     if (g.ib[q] >= 1 || fabs(ap) <= SMALL20) {
-      A[e] = 0.0; B[e] = 1.0; Cc[e] = 0.0; D[e] = xv;
-      continue;
-    }
-    double bsy = g.bs[q];
-    bsy = bsy + g.as[3][q] * xt[lidx(i + 2, j, k)] + g.as[0][q] * xt[lidx(i - 2, j, k)];
-    double am, apn;
-    if (dir == 2) {
-      bsy = bsy + g.as[5][q] * xt[lidx(i, j, k + 2)] + g.as[2][q] * xt[lidx(i, j, k - 2)];
-      am = g.as[1][q]; apn = g.as[4][q];
-      if (e == 0) { bsy += am * xt[lidx(i, j - 2, k)]; am = 0.0; }
-      if (e == n - 1) { bsy += apn * xt[lidx(i, j + 2, k)]; apn = 0.0; }
+      a = 0.0; bb = 1.0; c = 0.0; d = xt[q];
     } else {
-      bsy = bsy + g.as[4][q] * xt[lidx(i, j + 2, k)] + g.as[1][q] * xt[lidx(i, j - 2, k)];
-      am = g.as[2][q]; apn = g.as[5][q];
-      if (e == 0) { bsy += am * xt[lidx(i, j, k - 2)]; am = 0.0; }
-      if (e == n - 1) { bsy += apn * xt[lidx(i, j, k + 2)]; apn = 0.0; }
+      double bsy = g.bs[q];
+      bsy = bsy + g.as[3][q] * xt[lidx(i + 2, j, k)] + g.as[0][q] * xt[lidx(i - 2, j, k)];
+      double am, apn;
+      if (dir == 2) {
+        bsy = bsy + g.as[5][q] * xt[lidx(i, j, k + 2)] + g.as[2][q] * xt[lidx(i, j, k - 2)];
+        am = g.as[1][q]; apn = g.as[4][q];
+        if (e == 0) { bsy += am * xt[lidx(i, j - 2, k)]; am = 0.0; }
+        if (e == n - 1) { bsy += apn * xt[lidx(i, j + 2, k)]; apn = 0.0; }
+      } else {
+        bsy = bsy + g.as[4][q] * xt[lidx(i, j + 2, k)] + g.as[1][q] * xt[lidx(i, j - 2, k)];
+        am = g.as[2][q]; apn = g.as[5][q];
+        if (e == 0) { bsy += am * xt[lidx(i, j, k - 2)]; am = 0.0; }
+        if (e == n - 1) { bsy += apn * xt[lidx(i, j, k + 2)]; apn = 0.0; }
+      }
+      a = -am; bb = ap; c = -apn; d = bsy;
     }
-    A[e] = -am; B[e] = ap; Cc[e] = -apn; D[e] = bsy;
+    A[w * npad + e] = a; B[w * npad + e] = bb; Cc[w * npad + e] = c; D[w * npad + e] = d;
   }
   __syncthreads();
+  const int wave = t >> 6, lane = t & 63;
+  const bool solve = (wave < LW) && (tile * LW + wave < npl);
+  double *a_ = A + wave * npad, *b_ = B + wave * npad, *c_ = Cc + wave * npad, *d_ = D + wave * npad;
   double na[PCR_MAXN / 64], nb_[PCR_MAXN / 64], nc[PCR_MAXN / 64], nd[PCR_MAXN / 64];
   for (int s = 1; s < n; s <<= 1) {
+    if (solve) {
 #pragma unroll
-    for (int t = 0; t < PCR_MAXN / 64; t++) {
-      int e = lane + 64 * t;
-      if (e >= n) continue;
-      double ae = A[e], be = B[e], ce = Cc[e], de = D[e];
-      double am = 0.0, bm = 1.0, cm = 0.0, dm = 0.0, ap2 = 0.0, bp = 1.0, cp = 0.0, dp = 0.0;
-      if (e - s >= 0) { am = A[e - s]; bm = B[e - s]; cm = Cc[e - s]; dm = D[e - s]; }
-      if (e + s < n) { ap2 = A[e + s]; bp = B[e + s]; cp = Cc[e + s]; dp = D[e + s]; }
-      double al = -ae / bm, ga = -ce / bp;
-      na[t] = al * am;
-      nc[t] = ga * cp;
-      nb_[t] = be + al * cm + ga * ap2;
-      nd[t] = de + al * dm + ga * dp;
+      for (int u = 0; u < PCR_MAXN / 64; u++) {
+        int e = lane + 64 * u;
+        if (e >= n) continue;
+        double ae = a_[e], be = b_[e], ce = c_[e], de = d_[e];
+        double am = 0.0, bm = 1.0, cm = 0.0, dm = 0.0, ap2 = 0.0, bp = 1.0, cp = 0.0, dp = 0.0;
+        if (e - s >= 0) { am = a_[e - s]; bm = b_[e - s]; cm = c_[e - s]; dm = d_[e - s]; }
+        if (e + s < n) { ap2 = a_[e + s]; bp = b_[e + s]; cp = c_[e + s]; dp = d_[e + s]; }
+        double al = -ae / bm, ga = -ce / bp;
+        na[u] = al * am;
+        nc[u] = ga * cp;
+        nb_[u] = be + al * cm + ga * ap2;
+        nd[u] = de + al * dm + ga * dp;
+      }
     }
     __syncthreads();
+    if (solve) {
 #pragma unroll
-    for (int t = 0; t < PCR_MAXN / 64; t++) {
-      int e = lane + 64 * t;
-      if (e >= n) continue;
-      A[e] = na[t]; B[e] = nb_[t]; Cc[e] = nc[t]; D[e] = nd[t];
+      for (int u = 0; u < PCR_MAXN / 64; u++) {
+        int e = lane + 64 * u;
+        if (e >= n) continue;
+        a_[e] = na[u]; b_[e] = nb_[u]; c_[e] = nc[u]; d_[e] = nd[u];
+      }
     }
     __syncthreads();
   }
-  for (int e = lane; e < n; e += 64) {
+  if (solve)
+    for (int e = lane; e < n; e += 64) d_[e] = d_[e] / b_[e];
+  __syncthreads();
+  for (int e = rg; e < n; e += nrg) {
+    if (!valid) break;
     int j = (dir == 2) ? jb + 2 * e : jb + 2 * o;
     int k = (dir == 2) ? kb + 2 * o : kb + 2 * e;
-    xo[lidx(i, j, k)] = D[e] / B[e];
+    xo[lidx(i, j, k)] = D[w * npad + e];
   }
-}
-// XT = (XT2 + XT3)/2 on the planes of one sweep step, C/Adlbl3.f90:134-138
-__global__ void k_line_avg(double *xt, const double *x2, const double *x3, int pi0, int pdi, int npl, int jb, int nj, int kb, int nk) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  long long tot = (long long)npl * nj * nk;
-  if (t >= tot) return;
-  int pl = t / (nj * nk), r = t % (nj * nk);
-  int kk = r / nj, jj = r % nj;
-  int i = pi0 + pdi * pl, j = jb + 2 * jj, k = kb + 2 * kk;
-  long long q = lidx(i, j, k);
-  xt[q] = (x2[q] + x3[q]) / 2.0;
 }
 
 // ======================================================================

@@ -1,0 +1,40 @@
+"""Writes gfm_types.f90: ISO_C_BINDING mirrors of the C structs in
+include/gfm_hip.h, generated from the ctypes mirror gfm_amd/abi.py so the
+three stay identical.  Run: python gfm_amd/fortran/gen_types.py"""
+import ctypes as ct
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+from gfm_amd.abi import GfmCtrl, GfmHostArrays  # noqa: E402
+
+
+def ftype(t):
+    if t is ct.c_int32:
+        return "integer(c_int32_t)", None
+    if t is ct.c_double:
+        return "real(c_double)", None
+    if hasattr(t, "_length_"):
+        base, _ = ftype(t._type_)
+        return base, t._length_
+    if issubclass(t, ct._Pointer):
+        return "type(c_ptr)", None
+    raise TypeError(t)
+
+
+def emit(cls, fname):
+    out = [f"  type, bind(C) :: {fname}"]
+    for name, t in cls._fields_:
+        ty, n = ftype(t)
+        out.append(f"    {ty} :: {name}" + (f"({n})" if n else ""))
+    out.append(f"  end type {fname}")
+    return out
+
+
+lines = ["! Generated by gen_types.py from gfm_amd/abi.py -- do not edit.",
+         "! ISO_C_BINDING mirrors of gfm_ctrl and gfm_host_arrays (include/gfm_hip.h).",
+         "module gfm_types", "  use iso_c_binding", "  implicit none"]
+lines += emit(GfmCtrl, "gfm_ctrl") + emit(GfmHostArrays, "gfm_host_arrays")
+lines += ["end module gfm_types", ""]
+open(os.path.join(HERE, "gfm_types.f90"), "w").write("\n".join(lines))

@@ -1,0 +1,62 @@
+! gfm_comb.f90 -- Fortran driver of the MI355X combustion-space loop.
+!
+! Reads a case state (reference layout) from a stream file, runs SPHASE on
+! the GPU for the requested iterations, writes the state back and prints
+! the res_mass history (the convg log line of C/Sphase.f90:53).
+!   usage: gfm_comb <state-in> <state-out> <iterations> [sweep_order]
+program gfm_comb
+  use iso_c_binding
+  use gfm_types
+  use gfm_state
+  use gfm_solver
+  implicit none
+  character(len=512) :: fin, fout, arg
+  integer :: niter, order, i
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+  call get_command_argument(3, arg)
+  read (arg, *) niter
+  order = -1
+  if (command_argument_count() >= 4) then
+    call get_command_argument(4, arg)
+    read (arg, *) order
+  end if
+  call read_state(trim(fin))
+  if (order >= 0) ctrl%sweep_order = order
+  ctrl%itr_gend = ctrl%itr_gas + niter
+  poll_every = max(1, min(100, niter))
+  call SPHASE
+  do i = 1, n_hist
+    write (*, '(i7,2e25.16)') ctrl%itr_gas - n_hist + i, res_hist(i)
+  end do
+  call write_state(trim(fout))
+  call gpu_close()
+contains
+  subroutine read_state(path)
+    character(len=*), intent(in) :: path
+    character(len=8) :: magic
+    integer(c_int32_t) :: ver, csize, m, n, l, nr
+    integer :: u
+    open (newunit=u, file=path, access='stream', form='unformatted', status='old', action='read')
+    read (u) magic, ver, csize
+    if (magic /= 'GFMSTATE' .or. csize /= storage_size(ctrl) / 8) call stop_run('bad state file '//path)
+    read (u) ctrl
+    m = ctrl%mp; n = ctrl%np; l = ctrl%lp; nr = ctrl%nr0
+    call allocate_state(m, n, l, nr)
+    read (u) X, DX, R, DR, RS, Z, DZ, IBCELL
+    read (u) P, T, DNST, THETA, TMU, GCP, GLAM, GAMA, GTE, CRATE, HRATE, BS_HPREV, UG, GF
+    read (u) RTIM, DELRT, HRT, DELR, TIMED, HRD
+    close (u)
+  end subroutine
+  subroutine write_state(path)
+    character(len=*), intent(in) :: path
+    integer :: u
+    open (newunit=u, file=path, access='stream', form='unformatted', status='replace', action='write')
+    write (u) 'GFMSTATE', int(1, c_int32_t), int(storage_size(ctrl) / 8, c_int32_t)
+    write (u) ctrl
+    write (u) X, DX, R, DR, RS, Z, DZ, IBCELL
+    write (u) P, T, DNST, THETA, TMU, GCP, GLAM, GAMA, GTE, CRATE, HRATE, BS_HPREV, UG, GF
+    write (u) RTIM, DELRT, HRT, DELR, TIMED, HRD
+    close (u)
+  end subroutine
+end program gfm_comb

@@ -1,0 +1,85 @@
+! gfm_hip_mod.f90 -- Fortran interfaces to the C-ABI of include/gfm_hip.h.
+module gfm_hip_mod
+  use iso_c_binding
+  use gfm_types
+  implicit none
+  integer(c_int), parameter :: GFM_ORDER_REFERENCE = 0, GFM_ORDER_ZEBRA = 1
+  interface
+    integer(c_int) function gfm_hip_init(ctrl, h, device, ctx) bind(C, name="gfm_hip_init")
+      import :: c_int, c_ptr, gfm_ctrl, gfm_host_arrays
+      type(gfm_ctrl), intent(in) :: ctrl
+      type(gfm_host_arrays), intent(in) :: h
+      integer(c_int), value :: device
+      type(c_ptr), intent(out) :: ctx
+    end function
+    integer(c_int) function gfm_hip_upload(ctx, ctrl, h) bind(C, name="gfm_hip_upload")
+      import :: c_int, c_ptr, gfm_ctrl, gfm_host_arrays
+      type(c_ptr), value :: ctx
+      type(gfm_ctrl), intent(in) :: ctrl
+      type(gfm_host_arrays), intent(in) :: h
+    end function
+    integer(c_int) function gfm_hip_download(ctx, ctrl, h) bind(C, name="gfm_hip_download")
+      import :: c_int, c_ptr, gfm_ctrl, gfm_host_arrays
+      type(c_ptr), value :: ctx
+      type(gfm_ctrl), intent(inout) :: ctrl
+      type(gfm_host_arrays), intent(in) :: h
+    end function
+    integer(c_int) function gfm_hip_initsv_tail(ctx, ctrl) bind(C, name="gfm_hip_initsv_tail")
+      import :: c_int, c_ptr, gfm_ctrl
+      type(c_ptr), value :: ctx
+      type(gfm_ctrl), intent(inout) :: ctrl
+    end function
+    integer(c_int) function gfm_hip_gflow(ctx, ctrl) bind(C, name="gfm_hip_gflow")
+      import :: c_int, c_ptr, gfm_ctrl
+      type(c_ptr), value :: ctx
+      type(gfm_ctrl), intent(inout) :: ctrl
+    end function
+    integer(c_int) function gfm_hip_sphase(ctx, ctrl, max_iters, res_hist, n_done) bind(C, name="gfm_hip_sphase")
+      import :: c_int, c_ptr, c_double, gfm_ctrl
+      type(c_ptr), value :: ctx
+      type(gfm_ctrl), intent(inout) :: ctrl
+      integer(c_int), value :: max_iters
+      real(c_double), intent(out) :: res_hist(*)
+      integer(c_int), intent(out) :: n_done
+    end function
+    integer(c_int) function gfm_hip_adlbl3(ctx, ctrl, nelg, ap, as, bs, xt, ib, ie, jb, je, kb, ke, resg) &
+        bind(C, name="gfm_hip_adlbl3")
+      import :: c_int, c_ptr, c_double, gfm_ctrl
+      type(c_ptr), value :: ctx
+      type(gfm_ctrl), intent(in) :: ctrl
+      integer(c_int), value :: nelg, ib, ie, jb, je, kb, ke
+      real(c_double), intent(in) :: ap(*), as(*), bs(*)
+      real(c_double), intent(inout) :: xt(*)
+      real(c_double), intent(out) :: resg(2)
+    end function
+    integer(c_int) function gfm_hip_calc_residual(ctx, ap, as, bs, xt, ib, ie, jb, je, kb, ke, resid) &
+        bind(C, name="gfm_hip_calc_residual")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: ap(*), as(*), bs(*), xt(*)
+      integer(c_int), value :: ib, ie, jb, je, kb, ke
+      real(c_double), intent(out) :: resid
+    end function
+    integer(c_int) function gfm_hip_finalize(ctx) bind(C, name="gfm_hip_finalize")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    type(c_ptr) function gfm_hip_last_error() bind(C, name="gfm_hip_last_error")
+      import :: c_ptr
+    end function
+  end interface
+contains
+  function gfm_error_text() result(msg)
+    character(len=:), allocatable :: msg
+    character(kind=c_char), pointer :: p(:)
+    integer :: n
+    call c_f_pointer(gfm_hip_last_error(), p, [512])
+    n = 0
+    do while (n < 512)
+      if (p(n + 1) == c_null_char) exit
+      n = n + 1
+    end do
+    allocate (character(len=n) :: msg)
+    msg = transfer(p(1:n), msg)
+  end function
+end module gfm_hip_mod

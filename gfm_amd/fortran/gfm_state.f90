@@ -1,0 +1,144 @@
+! gfm_state.f90 -- Fortran host layer over the MI355X SIMPLER library.
+!
+! Module gfm_state holds the subset of the reference's MODULE GBL_VAR
+! (C/Var_mod.f90:3-889) that the combustion-space loop reads and writes,
+! in the reference layout (1-based, column-major, doubled lattice).
+! Module gfm_solver keeps the reference's subroutine interface:
+!   SPHASE                      C/Sphase.f90:10   (called C/A0main.f90:204)
+!   GFLOW                       C/Gflow.f90:6     (called C/Sphase.f90:51)
+!   ADLBL3(XT,IB,IE,JB,JE,KB,KE) C/Adlbl3.f90:10
+! and runs them on the GPU through ISO_C_BINDING (gfm_hip_mod).  A non-zero
+! return from the library ends the run through stop_run, as the reference's
+! fatal paths do (C/A0main.f90:415-424).
+module gfm_state
+  use iso_c_binding
+  use gfm_types
+  implicit none
+  integer :: MP, NP, LP, NR0
+  real(c_double), allocatable, target :: X(:), DX(:), R(:), DR(:), RS(:), Z(:), DZ(:)
+  integer(c_int16_t), allocatable, target :: IBCELL(:, :, :)
+  real(c_double), allocatable, target :: P(:, :, :), T(:, :, :), DNST(:, :, :), THETA(:, :, :), TMU(:, :, :)
+  real(c_double), allocatable, target :: GCP(:, :, :), GLAM(:, :, :), GAMA(:, :, :), GTE(:, :, :)
+  real(c_double), allocatable, target :: CRATE(:, :, :), HRATE(:, :, :), BS_HPREV(:, :, :)
+  real(c_double), allocatable, target :: UG(:, :, :, :), GF(:, :, :, :)
+  real(c_double), allocatable, target :: RTIM(:), DELRT(:), HRT(:), DELR(:), TIMED(:), HRD(:)
+  ! ADLBL3's module inputs (C/Adlbl3.f90 reads AP, AS, BS from GBL_VAR)
+  real(c_double), allocatable, target :: AP(:, :, :), AS(:, :, :, :, :), BS(:, :, :)
+  type(gfm_ctrl), target :: ctrl
+  type(gfm_host_arrays), target :: harr
+  character(len=256) :: casedir = '.'
+contains
+  subroutine allocate_state(m, n, l, nr)
+    integer, intent(in) :: m, n, l, nr
+    MP = m; NP = n; LP = l; NR0 = nr
+    allocate (X(MP), DX(MP), R(NP), DR(NP), RS(NP), Z(LP), DZ(LP))
+    allocate (IBCELL(MP, NP, LP))
+    allocate (P(MP, NP, LP), T(MP, NP, LP), DNST(MP, NP, LP), THETA(MP, NP, LP), TMU(MP, NP, LP))
+    allocate (GCP(MP, NP, LP), GLAM(MP, NP, LP), GAMA(MP, NP, LP), GTE(MP, NP, LP))
+    allocate (CRATE(MP, NP, LP), HRATE(MP, NP, LP), BS_HPREV(MP, NP, LP))
+    allocate (UG(MP, NP, LP, 3), GF(MP, NP, LP, 9))
+    allocate (RTIM(NR0), DELRT(NR0), HRT(NR0), DELR(NR0), TIMED(NR0), HRD(NR0))
+  end subroutine
+
+  subroutine bind_host_arrays()
+    harr%x = c_loc(X); harr%dx = c_loc(DX); harr%r = c_loc(R); harr%dr = c_loc(DR); harr%rs = c_loc(RS)
+    harr%z = c_loc(Z); harr%dz = c_loc(DZ); harr%ibcell = c_loc(IBCELL)
+    harr%p = c_loc(P); harr%t = c_loc(T); harr%dnst = c_loc(DNST); harr%theta = c_loc(THETA)
+    harr%tmu = c_loc(TMU); harr%gcp = c_loc(GCP); harr%glam = c_loc(GLAM); harr%gama = c_loc(GAMA)
+    harr%gte = c_loc(GTE); harr%crate = c_loc(CRATE); harr%hrate = c_loc(HRATE); harr%bs_hprev = c_loc(BS_HPREV)
+    harr%ug = c_loc(UG); harr%gf = c_loc(GF)
+    harr%rtim = c_loc(RTIM); harr%delrt = c_loc(DELRT); harr%hrt = c_loc(HRT)
+    harr%delr = c_loc(DELR); harr%timed = c_loc(TIMED); harr%hrd = c_loc(HRD)
+  end subroutine
+
+  ! stop_run: record why the run ended and stop (C/A0main.f90:415-424)
+  subroutine stop_run(reason)
+    character(len=*), intent(in) :: reason
+    integer :: u
+    open (newunit=u, file=trim(casedir)//'/runend.txt', status='replace')
+    write (u, '(a)') trim(reason)
+    close (u)
+    write (*, '(a)') trim(reason)
+    stop 1
+  end subroutine
+end module gfm_state
+
+module gfm_solver
+  use iso_c_binding
+  use gfm_types
+  use gfm_hip_mod
+  use gfm_state
+  implicit none
+  type(c_ptr) :: gpu_ctx = c_null_ptr
+  integer :: poll_every = 100              ! iterations between host polls (runstop.dat)
+  real(c_double), allocatable :: res_hist(:)
+  integer :: n_hist = 0
+contains
+  subroutine check(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: what
+    if (rc /= 0) call stop_run('GPU: '//what//' failed: '//gfm_error_text())
+  end subroutine
+
+  ! device state for the module arrays; INITSV's tail runs on the device
+  subroutine gpu_open(device)
+    integer, intent(in) :: device
+    call bind_host_arrays()
+    call check(gfm_hip_init(ctrl, harr, int(device, c_int), gpu_ctx), 'gfm_hip_init')
+    call check(gfm_hip_initsv_tail(gpu_ctx, ctrl), 'gfm_hip_initsv_tail')
+  end subroutine
+
+  subroutine gpu_sync_to_host()
+    call check(gfm_hip_download(gpu_ctx, ctrl, harr), 'gfm_hip_download')
+  end subroutine
+
+  subroutine gpu_close()
+    integer(c_int) :: rc
+    if (c_associated(gpu_ctx)) rc = gfm_hip_finalize(gpu_ctx)
+    gpu_ctx = c_null_ptr
+  end subroutine
+
+  ! SPHASE: the combustion-space loop (C/Sphase.f90:37-235) on the device,
+  ! returning to the host every poll_every iterations to honour runstop.dat
+  ! (C/Sphase.f90:125-131).  The module arrays are current on return.
+  subroutine SPHASE
+    integer(c_int) :: n_done
+    real(c_double), allocatable :: chunk(:)
+    logical :: stop_req
+    if (.not. c_associated(gpu_ctx)) call gpu_open(0)
+    allocate (chunk(poll_every))
+    if (.not. allocated(res_hist)) allocate (res_hist(max(ctrl%maxgi, 1)))
+    do while (ctrl%itr_gas < ctrl%itr_gend .and. ctrl%itr_gas < ctrl%maxgi)
+      call check(gfm_hip_sphase(gpu_ctx, ctrl, int(poll_every, c_int), chunk, n_done), 'gfm_hip_sphase')
+      if (n_done > 0 .and. n_hist + n_done <= size(res_hist)) then
+        res_hist(n_hist + 1:n_hist + n_done) = chunk(1:n_done)
+        n_hist = n_hist + n_done
+      end if
+      if (n_done == 0) exit
+      inquire (file=trim(casedir)//'/runstop.dat', exist=stop_req)
+      if (stop_req) exit
+    end do
+    call gpu_sync_to_host()
+  end subroutine
+
+  ! GFLOW: one SIMPLER iteration (C/Gflow.f90:6-408) at itr_gas
+  subroutine GFLOW
+    if (.not. c_associated(gpu_ctx)) call gpu_open(0)
+    call check(gfm_hip_gflow(gpu_ctx, ctrl), 'gfm_hip_gflow')
+  end subroutine
+
+  ! ADLBL3 (C/Adlbl3.f90:10): line relaxation of XT with the module's AP,
+  ! AS and BS and NTIMESG(NELG) sweeps; RESG(NELG,1:2) back in ctrl%resg.
+  subroutine ADLBL3(XT, IB, IE, JB, JE, KB, KE, NELG)
+    real(c_double), intent(inout) :: XT(MP, NP, LP)
+    integer, intent(in) :: IB, IE, JB, JE, KB, KE, NELG
+    real(c_double) :: resg(2)
+    if (.not. c_associated(gpu_ctx)) call gpu_open(0)
+    call check(gfm_hip_adlbl3(gpu_ctx, ctrl, int(NELG, c_int), AP, AS, BS, XT, int(IB, c_int), int(IE, c_int), &
+                              int(JB, c_int), int(JE, c_int), int(KB, c_int), int(KE, c_int), resg), 'gfm_hip_adlbl3')
+    if (ctrl%want_irc /= 0) then
+      ctrl%resg(NELG) = resg(1)
+      ctrl%resg(20 + NELG) = resg(2)
+    end if
+  end subroutine
+end module gfm_solver
