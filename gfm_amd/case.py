@@ -116,6 +116,7 @@ class CaseSpec:
     q0: float = 5.0e7           # heat of combustion, J/kg fuel
     melt_bottom_T: object = None  # callable x->T (K) for an IBCELL=4 bottom, or None
     t_wall: float = None        # isothermal wall temperature (K), None -> adiabatic (reference)
+    gray_kappa: float = 0.0     # gray absorption coefficient (1/m) of the radiation source term; 0 -> off
     bgcon: float = 1.0e-5
     maxgi: int = 2000
     meta: dict = field(default_factory=dict)
@@ -312,7 +313,9 @@ def build(spec: CaseSpec) -> Case:
         c.ntimesg[q] = NTIMESG[q]
     c.sigma_sb = 5.668e-8
     c.h_g = 10.0
-    c.kappa_gray = 0.0
+    c.kappa_gray = spec.gray_kappa
+    c.gray_radiation = 1 if spec.gray_kappa > 0.0 else 0
+    c.isothermal_walls = 1 if spec.t_wall is not None else 0
     c.t_wall = (spec.t_wall if spec.t_wall is not None else spec.t_init) / T0
 
     arr = {"x": x, "dx": dx, "r": r, "dr": dr, "rs": np.ones(np_), "z": z, "dz": dz, "ibcell": ib}
@@ -412,7 +415,7 @@ def build(spec: CaseSpec) -> Case:
         GF[sl_c + (q - 1,)] = view
     wall = cen_ib == 1
     Tc = T[sl_c]
-    Tc[wall] = spec.t_init / T0
+    Tc[wall] = (spec.t_wall if spec.t_wall is not None else spec.t_init) / T0
     T[sl_c] = Tc
     y5 = [GF[sl_c + (q - 1,)] for q in CHEM]
     hc = GF[sl_c + (IH - 1,)]
@@ -581,12 +584,16 @@ def stretched_faces(length, n, ratio):
     return np.concatenate([left, right[1:]])
 
 
-def furnace_small(nx=48, ny=24, nz=16, seed=1, noise=0.01, maxgi=300, bgcon=1.0e-12):
+def furnace_small(nx=48, ny=24, nz=16, seed=1, noise=0.01, maxgi=300, bgcon=1.0e-12, walls="isothermal",
+                  gray_kappa=0.3):
     """3.0 x 1.2 x 0.8 m chamber, uniform cells, one CH4 port (0.1x0.1 m,
     30 m/s, 300 K) centred on the x=0 wall inside an air port (0.3x0.2 m,
     10 m/s, 1100 K), full-face exit at x=3 m; uniform initial field
-    u=0, T=1500 K, k=eps=1e-2 with +-1 % seeded noise.  For grids other
-    than 48x24x16 the ports keep their physical size."""
+    u=0, T=1500 K, k=eps=1e-2 with +-1 % seeded noise; walls isothermal at
+    1500 K and a gray radiation source with kappa=0.3 1/m (walls="adiabatic",
+    gray_kappa=0 give the reference's behaviour before its first radiation
+    pass).  For grids other than 48x24x16 the ports keep their physical
+    size."""
     Lx, Ly, Lz = 3.0, 1.2, 0.8
     dy, dz = Ly / ny, Lz / nz
 
@@ -604,11 +611,13 @@ def furnace_small(nx=48, ny=24, nz=16, seed=1, noise=0.01, maxgi=300, bgcon=1.0e
                 T=1100.0, yf=0.0, yo2=0.233, yn2=0.767, uvw=(10.0, 0.0, 0.0))
     spec = CaseSpec(name=f"furnace {nx}x{ny}x{nz}", faces_x=uniform_faces(Lx, nx), faces_y=uniform_faces(Ly, ny),
                     faces_z=uniform_faces(Lz, nz), inlets=[fuel, air], t_init=1500.0, k_init=1.0e-2,
-                    eps_init=1.0e-2, noise=noise, seed=seed, maxgi=maxgi, bgcon=bgcon, t_wall=1500.0)
+                    eps_init=1.0e-2, noise=noise, seed=seed, maxgi=maxgi, bgcon=bgcon,
+                    t_wall=1500.0 if walls == "isothermal" else None, gray_kappa=gray_kappa)
     return build(spec)
 
 
-def furnace_large(nx=192, ny=96, nz=64, seed=2, noise=0.01, maxgi=2000, bgcon=1.0e-5):
+def furnace_large(nx=192, ny=96, nz=64, seed=2, noise=0.01, maxgi=2000, bgcon=1.0e-5, walls="isothermal",
+                  gray_kappa=0.3):
     """7.0 x 3.0 x 1.6 m furnace, y-faces stretched toward the burner walls
     (ratio 1.05), two opposed CH4 burner pairs on the y walls at x=1.5 m and
     x=4.0 m (fuel 25 m/s 300 K, air 12 m/s 1200 K; the air port is sized
@@ -635,5 +644,5 @@ def furnace_large(nx=192, ny=96, nz=64, seed=2, noise=0.01, maxgi=2000, bgcon=1.
     spec = CaseSpec(name=f"furnace {nx}x{ny}x{nz} stretched", faces_x=fx, faces_y=fy, faces_z=fzf, inlets=inlets,
                     t_init=1500.0, k_init=1.0e-2, eps_init=1.0e-2, noise=noise, seed=seed, maxgi=maxgi, bgcon=bgcon,
                     melt_bottom_T=lambda xx: 1450.0 + 20.0 * math.sin(2.0 * math.pi * xx / Lx + phase),
-                    t_wall=1600.0)
+                    t_wall=1600.0 if walls == "isothermal" else None, gray_kappa=gray_kappa)
     return build(spec)

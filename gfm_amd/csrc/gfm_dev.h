@@ -58,8 +58,8 @@ struct DevCase {
   double gra[3], p_ref;
   double t_mx, t_mn, u_mx, u_mn, p_mx, p_mn, f_mx, f_mn;
   double rf[21], rfl[21], rf_hsource;
-  double t_wall;
-  int i1st_rad_done, isothermal_walls;
+  double t_wall, kappa_gray, sigma_sb, h_g;
+  int i1st_rad_done, isothermal_walls, gray_radiation;
   double wt0, stoich;
 };
 
@@ -69,6 +69,8 @@ enum {
   SC_GFEX, SC_FUEL_OUT, SC_TMEAN_EX, SC_TAREA, SC_AREA_EX,
   SC_GFIN0, SC_GFIN, SC_FACTOR, SC_FADJ, SC_QEXH, SC_QREACT, SC_G0REF,
   SC_TAVE_NUM, SC_TAVE_DEN, SC_TVOL_NUM, SC_TVOL_DEN, SC_TMAX,
+  SC_QWALL,  // total convective heat to walls and melt surface (W)
+  SC_ITR,  // itr_gas of the running iteration (read by kernels so a captured graph is reusable)
   SC_RESID0 = 32,  // 64 residual slots follow
   SC_COUNT = 128
 };

@@ -58,6 +58,11 @@ struct gfm_ctx {
   long long nlat = 0, ncls = 0;
   hipStream_t stream = nullptr;
   double sgm = 1.0;
+  // one SIMPLER iteration captured as a graph, replayed every iteration
+  bool use_graph = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_key = -1;
 };
 static gfm_ctx *g_bound = nullptr;
 
@@ -148,8 +153,12 @@ static void fill_devcase_scalars(gfm_ctx *c, const gfm_ctrl *k) {
   for (int q = 0; q < 21; q++) { d.rf[q] = k->rf[q]; d.rfl[q] = k->rfl[q]; }
   d.rf_hsource = k->rf_hsource;
   d.t_wall = k->t_wall;
+  d.kappa_gray = k->kappa_gray;
+  d.sigma_sb = k->sigma_sb;
+  d.h_g = k->h_g;
   d.i1st_rad_done = k->i1st_rad_done;
   d.isothermal_walls = k->isothermal_walls;
+  d.gray_radiation = k->gray_radiation;
   d.wt0 = 1.0 / (0.5 / WF + 0.0 / WCO2 + 0.0 / WH2O + 0.49 / WO2 + 0.01 / WN2);
   d.stoich = 2.0 * WO2 / WF;
 }
@@ -240,7 +249,7 @@ static void pull_scalars(gfm_ctx *c, gfm_ctrl *k, const double *sc) {
   k->imx = (int)sc[SC_IMX]; k->jmx = (int)sc[SC_JMX]; k->kmx = (int)sc[SC_KMX];
   k->gfex = sc[SC_GFEX]; k->fuel_out = sc[SC_FUEL_OUT]; k->tmean_ex = sc[SC_TMEAN_EX];
   k->gfin0 = sc[SC_GFIN0]; k->gfin = sc[SC_GFIN]; k->factor = sc[SC_FACTOR]; k->fadj = sc[SC_FADJ];
-  k->q_exhaust = sc[SC_QEXH]; k->q_react = sc[SC_QREACT];
+  k->q_exhaust = sc[SC_QEXH]; k->q_react = sc[SC_QREACT]; k->q_wall = sc[SC_QWALL];
   if (sc[SC_TAVE_DEN] > 0.0) {
     k->tave = sc[SC_TAVE_NUM] * k->t0 / sc[SC_TAVE_DEN];
     k->tave_vol = sc[SC_TVOL_NUM] * k->t0 / sc[SC_TVOL_DEN];
@@ -273,13 +282,14 @@ static void plane_op(gfm_ctx *c, double *xt, int pi0, int pdi, int npl, int jb, 
   const int lw = (npl >= 16) ? 16 : 1;
   const int ntile = (npl + lw - 1) / lw;
   const int nmax = ny > nz ? ny : nz;
-  const size_t shm = 4 * (size_t)lw * (nmax | 1) * sizeof(double);
+  const size_t shm = 5 * (size_t)lw * (nmax | 1) * sizeof(double);
   for (int l = 0; l < 2; l++) {
     if (c->prof.on) prof_begin(c);
+    const unsigned nblk = (unsigned)(ntile * (nz + ny));
     if (lw == 16)
-      hipLaunchKernelGGL(k_lines<16>, dim3(ntile * (nz + ny)), dim3(1024), shm, c->stream, xt, c->xt2, c->xt3, pi0, pdi, npl, jb, je, kb, ke);
+      hipLaunchKernelGGL(k_lines<16>, dim3(nblk), dim3(1024), shm, c->stream, xt, c->xt2, c->xt3, pi0, pdi, npl, jb, je, kb, ke);
     else
-      hipLaunchKernelGGL(k_lines<1>, dim3(ntile * (nz + ny)), dim3(1024), shm, c->stream, xt, c->xt2, c->xt3, pi0, pdi, npl, jb, je, kb, ke);
+      hipLaunchKernelGGL(k_lines<1>, dim3(nblk), dim3(1024), shm, c->stream, xt, c->xt2, c->xt3, pi0, pdi, npl, jb, je, kb, ke);
     if (c->prof.on) { prof_end(c, "k_lines", 2.0 * line_bytes); prof_begin(c); }
     long long tot = (long long)npl * nzl * nyl;
     hipLaunchKernelGGL(k_line_avg, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, xt, c->xt2, c->xt3, pi0, pdi,
@@ -317,7 +327,6 @@ static int gflow(gfm_ctx *c) {
   const gfm_ctrl &k = c->ctrl;
   DevCase &d = c->dc;
   const int MPh = k.mp, NPh = k.np, LPh = k.lp;
-  const int itr = k.itr_gas;
   Box all1 = make_box(2, MPh, 2, NPh, 2, LPh, 1);
   Box cen = make_box(4, MPh - 2, 4, NPh - 2, 4, LPh - 2, 2);
   Box cen_all = make_box(2, MPh, 2, NPh, 2, LPh, 2);
@@ -370,7 +379,7 @@ static int gflow(gfm_ctx *c) {
       hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->part, (int)box_blocks(ex[q]), 5, SC_GFEX, q > 0);
     }
   }
-  hipLaunchKernelGGL(k_factor, dim3(1), dim3(1), 0, c->stream, itr);
+  hipLaunchKernelGGL(k_factor, dim3(1), dim3(1), 0, c->stream);
   for (int q = 0; q < 3; q++) LAUNCH(k_extrv, ex[q], q + 1, 1, c->part);
   // INTPV
   {
@@ -415,9 +424,11 @@ static int gflow(gfm_ctx *c) {
     LAUNCH(k_copy_gf_fz, cen_all, l);
     int igs = 1;
     if (e == 0 || c->sgm != sigma[e]) { c->sgm = sigma[e]; igs = 0; }
-    LAUNCH(k_scalar_assemble, cen, l, nelg, igs, c->sgm, itr, c->part);
-    if (nelg == 8)
+    LAUNCH(k_scalar_assemble, cen, l, nelg, igs, c->sgm, c->part);
+    if (nelg == 8) {
       hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->part, (int)box_blocks(cen), 1, SC_QREACT, 0);
+      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->part + c->dc.pstride, (int)box_blocks(cen), 1, SC_QWALL, 0);
+    }
     if (k.want_pre_residuals) residual(c, d.fz, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_GF_PRE + l - 1, 0);
     adlbl3(c, d.fz, nelg, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2);
     residual(c, d.fz, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_GF + l - 1, 0);
@@ -497,8 +508,8 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
     set_err("bad lattice dimensions %d %d %d", k->mp, k->np, k->lp);
     return GFM_E_ARG;
   }
-  if ((k->np - 6) / 2 + 1 > PCR_MAXN || (k->lp - 6) / 2 + 1 > PCR_MAXN) {
-    set_err("line length above %d cells is not supported", PCR_MAXN);
+  if ((k->np - 6) / 2 + 1 > 254 || (k->lp - 6) / 2 + 1 > 254) {
+    set_err("line length above %d cells is not supported", 254);
     return GFM_E_ARG;
   }
   if (k->sweep_order != GFM_ORDER_REFERENCE && k->sweep_order != GFM_ORDER_ZEBRA) { set_err("bad sweep_order"); return GFM_E_ARG; }
@@ -513,6 +524,10 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
   gfm_ctx *c = new gfm_ctx();
   c->device = device;
   c->ctrl = *k;
+  {
+    const char *ev = getenv("GFM_HIP_GRAPH");
+    c->use_graph = !(ev && ev[0] == '0');
+  }
   c->nlat = nlat;
   DevGeom &gg = c->dc.g;
   gg.MP = k->mp; gg.NP = k->np; gg.LP = k->lp;
@@ -569,7 +584,7 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
   if ((rc = bind(c))) return fail(rc);
   {
     int nmax = (k->np > k->lp ? k->np : k->lp) / 2;
-    int shm = 4 * 16 * (nmax | 1) * (int)sizeof(double);
+    int shm = 5 * 16 * (nmax | 1) * (int)sizeof(double);
     if (shm > 64 * 1024) {
       HCHECK(hipFuncSetAttribute((const void *)k_lines<16>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
       HCHECK(hipFuncSetAttribute((const void *)k_lines<1>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
@@ -624,6 +639,37 @@ int gfm_hip_initsv_tail(gfm_ctx *c, gfm_ctrl *k) {
   return GFM_OK;
 }
 
+static void drop_graph(gfm_ctx *c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+  c->graph_key = -1;
+}
+
+// Launch one GFLOW.  The launch sequence depends only on the options in
+// the key, so it is captured once into a graph and replayed; itr_gas
+// reaches the kernels through the device scalar block.
+static int run_gflow(gfm_ctx *c) {
+  hipLaunchKernelGGL(k_set_itr, dim3(1), dim3(1), 0, c->stream, c->ctrl.itr_gas);
+  if (!c->use_graph || c->prof.on) return gflow(c);
+  int key = c->ctrl.sweep_order | (c->ctrl.want_pre_residuals << 2) | (c->ctrl.want_irc << 3);
+  if (!c->graph_exec || key != c->graph_key) {
+    drop_graph(c);
+    HCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = gflow(c);
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &gr);
+    if (rc) { if (gr) (void)hipGraphDestroy(gr); return rc; }
+    if (e != hipSuccess) { set_err("graph capture failed: %s", hipGetErrorString(e)); return GFM_E_DEVICE; }
+    c->graph = gr;
+    HCHECK(hipGraphInstantiateWithFlags(&c->graph_exec, c->graph, 0));
+    c->graph_key = key;
+  }
+  HCHECK(hipGraphLaunch(c->graph_exec, c->stream));
+  return GFM_OK;
+}
+
 int gfm_hip_gflow(gfm_ctx *c, gfm_ctrl *k) {
   if (!c || !k) { set_err("null argument"); return GFM_E_ARG; }
   int rc = bind(c);
@@ -632,7 +678,7 @@ int gfm_hip_gflow(gfm_ctx *c, gfm_ctrl *k) {
   c->ctrl.sweep_order = k->sweep_order;
   c->ctrl.want_pre_residuals = k->want_pre_residuals;
   c->ctrl.want_irc = k->want_irc;
-  if ((rc = gflow(c))) return rc;
+  if ((rc = run_gflow(c))) return rc;
   double sc[SC_COUNT];
   HCHECK(hipMemcpyAsync(sc, c->dc.scal, sizeof sc, hipMemcpyDeviceToHost, c->stream));
   HCHECK(hipStreamSynchronize(c->stream));
@@ -732,6 +778,7 @@ int gfm_hip_finalize(gfm_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void *p : c->allocs) (void)hipFree(p);
   for (auto &e : c->prof.ev) (void)hipEventDestroy(e);
+  drop_graph(c);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   if (g_bound == c) g_bound = nullptr;
   delete c;

@@ -726,7 +726,8 @@ __global__ void k_sum_partials(const double *part, int nb, int nq, int dst0, int
   }
 }
 // FLX1D's GFIN and the outlet flow factor, C/Flx1d.f90:75-77, C/Gslv3v.f90:213-231
-__global__ void k_factor(int itr_gas) {
+__global__ void k_factor() {
+  const int itr_gas = (int)g.scal[SC_ITR];
   double *s = g.scal;
   double gfex = s[SC_GFEX] * g.gmfr;
   s[SC_GFEX] = gfex;
@@ -777,6 +778,8 @@ __global__ void k_intpv(Box b, int iu, int ix, int pi, int pj, int pk) {
 }
 
 // pressure referencing at the reference inlet cell, C/Gflow.f90:182-242
+__global__ void k_set_itr(int itr) { g.scal[SC_ITR] = (double)itr; }
+
 __global__ void k_pref() {
   int i = g.i_ref, j = g.j_ref, k = g.k_ref;
   int i1 = i, j1 = j, k1 = k, i2 = i, j2 = j, k2 = k, nx = 0;
@@ -798,6 +801,68 @@ __global__ void k_pref() {
 }
 
 // ======================================================================
+// gray radiation source and wall convective flux (reference extension,
+// enabled by gray_radiation / isothermal_walls)
+// ======================================================================
+__device__ __forceinline__ double EMF(double xkl, double xl0) {  // C/Rad_fun.f90:97-116
+  const double x1 = 1.0e-3;
+  const double g1 = (1.0 - exp(-x1)) / x1;
+  double x0 = xkl * xl0;
+  return (x0 <= x1) ? x0 * g1 : 1.0 - exp(-x0);
+}
+// net gray radiation into an open cell (W): QA - QE with QE from
+// C/Rad0.f90:112-157 (one band) and QA from an isothermal black enclosure
+__device__ double GRAY_QNET(int i, int j, int k) {
+  double tk = T_(i, j, k) * g.t0;
+  double vol = VOL_C(i, j, k);
+  double sa = ((DX(i) * DR(j)) + (DR(j) * DZ(k)) + (DX(i) * DZ(k))) * 2.0;
+  if (sa <= SMALL16) return 0.0;
+  double opl = 6.0 * vol / sa;
+  if (opl <= 1.0e-10) return 0.0;
+  double e = EMF(g.kappa_gray, opl);
+  double tw = g.t_wall * g.t0;
+  double qe = (tk >= 300.0) ? sa * g.sigma_sb * tk * tk * tk * tk * e : 0.0;
+  double qa = sa * g.sigma_sb * tw * tw * tw * tw * e;
+  return qa - qe;
+}
+// convective flux (W/m2) to a wall or melt cell at twk (K), C/Qwall.f90:1419-1467
+__device__ double Q_CONV_FACE(long long q, double ds, double twk) {
+  const double kappa_cmu = 0.4 * pow(9.0e-2, 0.25), e_cmu = 9.0 * pow(9.0e-2, 0.25);
+  const double pi = 3.141592653589793;
+  const double c_htw = 0.4 * ((pi / 4) / sin(pi / 4)) * sqrt(26.0) * (0.7 - 1.0) / pow(0.7, 0.25);
+  double cpmix = 1.0057e3 * (CPF1 * g.gf[0][q] + (2.0e3 / 1.0057e3) * g.gf[5][q] + (2.0e3 / 1.0057e3) * g.gf[8][q] +
+                             (0.9136e3 / 1.0057e3) * g.gf[1][q] + (0.526e3 / 1.0057e3) * g.gf[6][q]);
+  double tgas = g.t[q] * g.t0;
+  double gmu_t = ((2.0e-15 * tgas + -1.1e-11) * tgas + 4.5e-8) * tgas + 7.0e-6;
+  double h_t;
+  if (g.tk0 * g.gf[3][q] < 0.1) h_t = 0.0;
+  else {
+    double sq = sqrt(g.tk0 * g.gf[3][q]);
+    double dtmp = log(e_cmu * ds * g.dnst0 * g.dnst[q] * sq / gmu_t) + (c_htw / sq);
+    h_t = (dtmp < 0.01) ? 0.0 : fmin(50.0, cpmix * kappa_cmu * g.dnst0 * g.dnst[q] * sq / dtmp);
+  }
+  return fmax(g.h_g, h_t) * (tgas - twk);
+}
+// wall sink of one open cell, C/Source.f90:473-509; qw gets sum q_conv*area
+__device__ double WALL_SINK(int i, int j, int k, double &qw) {
+  double s = 0.0;
+  long long q = lidx(i, j, k);
+  for (int d = 0; d < 6; d++) {
+    int di = (d == 0) ? -2 : (d == 1) ? 2 : 0, dj = (d == 2) ? -2 : (d == 3) ? 2 : 0, dk = (d == 4) ? -2 : (d == 5) ? 2 : 0;
+    int b = IBC(i + di, j + dj, k + dk);
+    if (b != 1 && b != 4) continue;
+    double dsw, dsc, area;
+    if (di) { dsw = fabs(X(i + di / 2) - X(i)); dsc = DX(i); area = DR(j) * DZ(k); }
+    else if (dj) { dsw = fabs(R(j + dj / 2) - R(j)); dsc = DR(j); area = DX(i) * DZ(k); }
+    else { dsw = fabs(Z(k + dk / 2) - Z(k)); dsc = DZ(k); area = DX(i) * DR(j); }
+    double qc = Q_CONV_FACE(q, dsw, T_(i + di, j + dj, k + dk) * g.t0);
+    qw += qc * area;
+    s += qc / (dsc * g.r0 * g.qr0);
+  }
+  return s;
+}
+
+// ======================================================================
 // scalar transport, C/Gslv3g.f90 + SOURCE(6..12) C/Source.f90:104-654
 // ======================================================================
 __global__ void k_copy_gf_fz(Box b, int l) {
@@ -809,9 +874,10 @@ __global__ void k_copy_gf_fz(Box b, int l) {
 // source + flux coefficients (GSLV3G0) + volume and relaxation terms of one
 // scalar equation at one cell.  igs=0 recomputes AS.  part[] collects
 // q_react for the enthalpy equation.
-__global__ void k_scalar_assemble(Box b, int l, int nelg, int igs, double sgm, int itr_gas, double *part) {
+__global__ void k_scalar_assemble(Box b, int l, int nelg, int igs, double sgm, double *part) {
+  const int itr_gas = (int)g.scal[SC_ITR];
   int i, j, k;
-  double qr = 0.0;
+  double qr = 0.0, qw = 0.0;
   if (box_pt(b, i, j, k)) {
     long long q = lidx(i, j, k);
     const double fz = g.fz[q];
@@ -875,7 +941,10 @@ __global__ void k_scalar_assemble(Box b, int l, int nelg, int igs, double sgm, i
         double hr = g.hrate[q];
         bs = 0.0 + hr;
         sfp = -0.0;
-        qr = hr * g.qr0 * (VOL_C(i, j, k) * g.vol0);
+        double vol0 = VOL_C(i, j, k) * g.vol0;
+        qr = hr * g.qr0 * vol0;
+        if (g.gray_radiation) bs += GRAY_QNET(i, j, k) / (vol0 * g.qr0);
+        if (g.isothermal_walls) bs -= WALL_SINK(i, j, k, qw);
         if (itr_gas > 1) bs = g.rf_hsource * bs + (1.0 - g.rf_hsource) * g.bsh[q];
         g.bsh[q] = bs;
       } else if (nelg == 9) {  // k, C/Source.f90:529-572
@@ -933,8 +1002,8 @@ __global__ void k_scalar_assemble(Box b, int l, int nelg, int igs, double sgm, i
     }
   }
   if (nelg == 8) {
-    double v = wave_sum(qr);
-    if (threadIdx.x == 0) part[block_lin()] = v;
+    double v = wave_sum(qr), w = wave_sum(qw);
+    if (threadIdx.x == 0) { part[block_lin()] = v; part[g.pstride + block_lin()] = w; }
   }
 }
 __global__ void k_bound_store_gf(Box b, int l) {
@@ -1244,7 +1313,7 @@ __global__ __launch_bounds__(1024) void k_lines(const double *__restrict__ xt, d
   if (b < ntile * nz) { dir = 2; n = ny; tile = b % ntile; o = b / ntile; xo = xy; }
   else { b -= ntile * nz; dir = 3; n = nz; tile = b % ntile; o = b / ntile; xo = xz; }
   const int npad = n | 1;
-  double *A = sm, *B = A + LW * npad, *Cc = B + LW * npad, *D = Cc + LW * npad;
+  double *A = sm, *B = A + LW * npad, *Cc = B + LW * npad, *D = Cc + LW * npad, *RB = D + LW * npad;
   const int t = threadIdx.x;
   const int w = t % LW, rg = t / LW, nrg = 1024 / LW;
   const int pl = tile * LW + w;
@@ -1282,7 +1351,12 @@ __global__ __launch_bounds__(1024) void k_lines(const double *__restrict__ xt, d
   const int wave = t >> 6, lane = t & 63;
   const bool solve = (wave < LW) && (tile * LW + wave < npl);
   double *a_ = A + wave * npad, *b_ = B + wave * npad, *c_ = Cc + wave * npad, *d_ = D + wave * npad;
-  double na[PCR_MAXN / 64], nb_[PCR_MAXN / 64], nc[PCR_MAXN / 64], nd[PCR_MAXN / 64];
+  double *rb_ = RB + wave * npad;
+  // rb_ holds 1/b: one division per element and reduction step
+  if (solve)
+    for (int e = lane; e < n; e += 64) rb_[e] = 1.0 / b_[e];
+  __syncthreads();
+  double na[PCR_MAXN / 64], nb_[PCR_MAXN / 64], nc[PCR_MAXN / 64], nd[PCR_MAXN / 64], nr[PCR_MAXN / 64];
   for (int s = 1; s < n; s <<= 1) {
     if (solve) {
 #pragma unroll
@@ -1290,13 +1364,14 @@ __global__ __launch_bounds__(1024) void k_lines(const double *__restrict__ xt, d
         int e = lane + 64 * u;
         if (e >= n) continue;
         double ae = a_[e], be = b_[e], ce = c_[e], de = d_[e];
-        double am = 0.0, bm = 1.0, cm = 0.0, dm = 0.0, ap2 = 0.0, bp = 1.0, cp = 0.0, dp = 0.0;
-        if (e - s >= 0) { am = a_[e - s]; bm = b_[e - s]; cm = c_[e - s]; dm = d_[e - s]; }
-        if (e + s < n) { ap2 = a_[e + s]; bp = b_[e + s]; cp = c_[e + s]; dp = d_[e + s]; }
-        double al = -ae / bm, ga = -ce / bp;
+        double am = 0.0, rbm = 1.0, cm = 0.0, dm = 0.0, ap2 = 0.0, rbp = 1.0, cp = 0.0, dp = 0.0;
+        if (e - s >= 0) { am = a_[e - s]; rbm = rb_[e - s]; cm = c_[e - s]; dm = d_[e - s]; }
+        if (e + s < n) { ap2 = a_[e + s]; rbp = rb_[e + s]; cp = c_[e + s]; dp = d_[e + s]; }
+        double al = -ae * rbm, ga = -ce * rbp;
         na[u] = al * am;
         nc[u] = ga * cp;
         nb_[u] = be + al * cm + ga * ap2;
+        nr[u] = 1.0 / nb_[u];
         nd[u] = de + al * dm + ga * dp;
       }
     }
@@ -1306,13 +1381,13 @@ __global__ __launch_bounds__(1024) void k_lines(const double *__restrict__ xt, d
       for (int u = 0; u < PCR_MAXN / 64; u++) {
         int e = lane + 64 * u;
         if (e >= n) continue;
-        a_[e] = na[u]; b_[e] = nb_[u]; c_[e] = nc[u]; d_[e] = nd[u];
+        a_[e] = na[u]; b_[e] = nb_[u]; c_[e] = nc[u]; d_[e] = nd[u]; rb_[e] = nr[u];
       }
     }
     __syncthreads();
   }
   if (solve)
-    for (int e = lane; e < n; e += 64) d_[e] = d_[e] / b_[e];
+    for (int e = lane; e < n; e += 64) d_[e] = d_[e] * rb_[e];
   __syncthreads();
   for (int e = rg; e < n; e += nrg) {
     if (!valid) break;

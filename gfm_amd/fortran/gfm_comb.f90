@@ -27,7 +27,7 @@ program gfm_comb
   poll_every = max(1, min(100, niter))
   call SPHASE
   do i = 1, n_hist
-    write (*, '(i7,2e25.16)') ctrl%itr_gas - n_hist + i, res_hist(i)
+    write (*, '(i7,es26.17e3)') ctrl%itr_gas - n_hist + i, res_hist(i)
   end do
   call write_state(trim(fout))
   call gpu_close()

@@ -417,6 +417,76 @@ static void SRC1(int i, int j, int k, int iu1) {
 }
 
 /* ======================================================================
+ * Gray radiation source and wall convective flux (reference extension
+ * used when ctrl.gray_radiation / ctrl.isothermal_walls are set)
+ * ====================================================================== */
+/* EMF: band emissivity of a path, C/Rad_fun.f90:97-116 */
+double oracle_emf(double xkl, double xl0) {
+  const double x1 = 1.0e-3;
+  const double g1 = (1.0 - exp(-x1)) / x1;
+  double x0 = xkl * xl0;
+  return (x0 <= x1) ? x0 * g1 : 1.0 - exp(-x0);
+}
+/* Net gray radiation into an open cell (W): emission of the gray gas
+ * QE = SA*sigma*T^4*EMF(kappa,OPL) with OPL = 6V/SA (C/Rad0.f90:112-157,
+ * one band, DEB=1) and absorption from an isothermal black enclosure at
+ * T_wall with the same emissivity (Kirchhoff), so QA-QE =
+ * SA*sigma*EMF*(Tw^4 - T^4).  Cells below 300 K do not emit
+ * (TG_MN, C/Rad0.f90:42). */
+static double gray_qnet(int i, int j, int k) {
+  double tk = T(i, j, k) * C->t0;
+  double vol = VOL_C(i, j, k);
+  double sa = ((DX(i) * DR(j)) + (DR(j) * DZ(k)) + (DX(i) * DZ(k))) * 2.0;
+  if (sa <= small16) return 0.0;
+  double opl = 6.0 * vol / sa;
+  if (opl <= 1.0e-10) return 0.0;
+  double e = oracle_emf(C->kappa_gray, opl);
+  double tw = C->t_wall * C->t0;
+  double qe = (tk >= 300.0) ? sa * C->sigma_sb * tk * tk * tk * tk * e : 0.0;
+  double qa = sa * C->sigma_sb * tw * tw * tw * tw * e;
+  return qa - qe;
+}
+/* Convective heat flux (W/m2) from the gas cell (i,j,k) to a wall or melt
+ * surface cell at temperature twk (K) across the face at distance ds
+ * (calc_q_conv_wall, C/Qwall.f90:1419-1467). */
+static double q_conv_face(int i, int j, int k, double ds, double twk) {
+  const double kappa_cmu = 0.4 * pow(9.0e-2, 0.25), e_cmu = 9.0 * pow(9.0e-2, 0.25);
+  const double pi = 3.141592653589793;
+  const double c_htw = 0.4 * ((pi / 4) / sin(pi / 4)) * sqrt(26.0) * (0.7 - 1.0) / pow(0.7, 0.25);
+  double cpmix = 1.0057e3 * (cpf1 * GF(i, j, k, iyf) + (2.0e3 / 1.0057e3) * GF(i, j, k, iyco2) +
+                             (2.0e3 / 1.0057e3) * GF(i, j, k, iyh2o) + (0.9136e3 / 1.0057e3) * GF(i, j, k, iyo2) +
+                             (0.526e3 / 1.0057e3) * GF(i, j, k, iyn2));
+  double tgas = T(i, j, k) * C->t0;
+  double gmu_t = ((2.0e-15 * tgas + -1.1e-11) * tgas + 4.5e-8) * tgas + 7.0e-6;
+  double h_t;
+  if (C->tk0 * GF(i, j, k, ik) < 0.1) h_t = 0.0;
+  else {
+    double sq = sqrt(C->tk0 * GF(i, j, k, ik));
+    double dtmp = log(e_cmu * ds * C->dnst0 * DNST(i, j, k) * sq / gmu_t) + (c_htw / sq);
+    h_t = (dtmp < 0.01) ? 0.0 : dmin(50.0, cpmix * kappa_cmu * C->dnst0 * DNST(i, j, k) * sq / dtmp);
+  }
+  return dmax(C->h_g, h_t) * (tgas - twk);
+}
+/* Wall sink of one open cell: sum over its wall/melt neighbours of
+ * q_conv/(ds*r0*qr0) (C/Source.f90:473-509); *qw accumulates q_conv*area. */
+static double wall_sink(int i, int j, int k, double *qw) {
+  double s = 0.0;
+  for (int d = 0; d < 6; d++) {
+    int di = (d == 0) ? -2 : (d == 1) ? 2 : 0, dj = (d == 2) ? -2 : (d == 3) ? 2 : 0, dk = (d == 4) ? -2 : (d == 5) ? 2 : 0;
+    int b = IBCELL(i + di, j + dj, k + dk);
+    if (b != 1 && b != 4) continue;
+    double dsw, dsc, area;
+    if (di) { dsw = fabs(X(i + di / 2) - X(i)); dsc = DX(i); area = DR(j) * DZ(k); }
+    else if (dj) { dsw = fabs(R(j + dj / 2) - R(j)); dsc = DR(j); area = DX(i) * DZ(k); }
+    else { dsw = fabs(Z(k + dk / 2) - Z(k)); dsc = DZ(k); area = DX(i) * DR(j); }
+    double q = q_conv_face(i, j, k, dsw, T(i + di, j + dj, k + dk) * C->t0);
+    *qw += q * area;
+    s += q / (dsc * C->r0 * C->qr0);
+  }
+  return s;
+}
+
+/* ======================================================================
  * Source terms, C/Source.f90:6-658
  * ====================================================================== */
 static void SOURCE(int mi1, int mi3, int nj1, int nj3, int lk1, int lk3) {
@@ -497,6 +567,7 @@ static void SOURCE(int mi1, int mi3, int nj1, int nj3, int lk1, int lk3) {
   } else if (nelg == 8 || nelg == 9) {
     if (nelg == 8) {
       C->q_react = 0.0;
+      C->q_wall = 0.0;
       for (int i = mi1; i <= mi3; i += 2)
         for (int j = nj1; j <= nj3; j += 2)
           for (int k = lk1; k <= lk3; k += 2) {
@@ -536,6 +607,8 @@ static void SOURCE(int mi1, int mi3, int nj1, int nj3, int lk1, int lk3) {
             SFP(i, j, k) = -0.0 / cpmix;
             double vol = VOL_C(i, j, k) * C->vol0;
             C->q_react += HRATE(i, j, k) * C->qr0 * vol;
+            if (C->gray_radiation) BS(i, j, k) += gray_qnet(i, j, k) / (vol * C->qr0);
+            if (C->isothermal_walls) BS(i, j, k) -= wall_sink(i, j, k, &C->q_wall);
           }
       if (C->itr_gas > 1) {
         for (int i = mi1; i <= mi3; i += 2)

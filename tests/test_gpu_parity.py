@@ -25,8 +25,8 @@ def small_case(order):
     return c
 
 
-def _run_pair(order, iters, nx=16, ny=12, nz=10):
-    base = cs.furnace_small(nx=nx, ny=ny, nz=nz, seed=3)
+def _run_pair(order, iters, nx=16, ny=12, nz=10, **kw):
+    base = cs.furnace_small(nx=nx, ny=ny, nz=nz, seed=3, **kw)
     base.ctrl.sweep_order = order
     o = base.clone()
     g = base.clone()
@@ -69,3 +69,41 @@ def test_five_iterations_zebra():
     np.testing.assert_allclose(h_g, h_o, rtol=1e-7)
     for k, v in errs.items():
         assert v < 1e-7, (k, v)
+
+
+def test_graph_replay_matches_eager(monkeypatch):
+    """The captured-graph iteration must reproduce the eager launches bit
+    for bit on every replay, with eager copies and syncs in between."""
+    base = small_case(1)
+    runs = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("GFM_HIP_GRAPH", mode)
+        c = base.clone()
+        states = []
+        with CombustionSpace(c) as s:
+            for it in range(6):
+                s.gflow()
+                s.download()
+                states.append((c.ctrl.res_mass, c.arrays["p"].copy(), c.arrays["ug"].copy(), c.arrays["gf"].copy()))
+        runs[mode] = states
+    for a, b in zip(runs["0"], runs["1"]):
+        assert a[0] == b[0]
+        for x, y in zip(a[1:], b[1:]):
+            np.testing.assert_array_equal(x, y)
+
+
+def test_adiabatic_reference_physics():
+    """Walls adiabatic and no radiation (the reference before its first
+    radiation pass)."""
+    o, g, h_o, h_g = _run_pair(1, 3, walls="adiabatic", gray_kappa=0.0)
+    errs = field_errors(o, g)
+    np.testing.assert_allclose(h_g, h_o, rtol=1e-9)
+    for k, v in errs.items():
+        assert v < 1e-9, (k, v)
+
+
+def test_wall_heat_flux_matches():
+    o, g, h_o, h_g = _run_pair(1, 3)
+    assert o.ctrl.q_wall != 0.0
+    assert g.ctrl.q_wall == pytest.approx(o.ctrl.q_wall, rel=1e-9)
+    assert g.ctrl.q_react == pytest.approx(o.ctrl.q_react, rel=1e-9)

@@ -283,3 +283,27 @@ def test_oracle_iterates_and_decreases_residual():
     h = oracle.sphase(c, 40)
     assert np.all(np.isfinite(h))
     assert h[-1] < 0.25 * h[0]
+
+
+def test_emf_closed_form():
+    lib = oracle.lib()
+    lib.oracle_emf.restype = ct.c_double
+    lib.oracle_emf.argtypes = [ct.c_double, ct.c_double]
+    for x in (0.5, 2.0, 0.002):
+        assert lib.oracle_emf(x, 1.0) == pytest.approx(-math.expm1(-x), rel=1e-14)
+    g1 = -math.expm1(-1e-3) / 1e-3
+    assert lib.oracle_emf(1e-4, 2.0) == pytest.approx(2e-4 * g1, rel=1e-14)
+    # optically thin limit of the gray emission: SA*EMF(kappa, 6V/SA) -> 6 kappa V
+    assert lib.oracle_emf(0.3, 1e-3) * 6.0 / 1e-3 == pytest.approx(6 * 0.3 * g1, rel=1e-14)
+
+
+@pytest.mark.parametrize("walls,kappa", [("isothermal", 0.3), ("adiabatic", 0.0)])
+def test_physics_variants_iterate(walls, kappa):
+    c = cs.furnace_small(nx=12, ny=10, nz=8, seed=8, walls=walls, gray_kappa=kappa)
+    oracle.initsv_tail(c)
+    h = oracle.sphase(c, 10)
+    assert np.all(np.isfinite(h))
+    if walls == "isothermal":
+        assert c.ctrl.q_wall != 0.0
+    else:
+        assert c.ctrl.q_wall == 0.0
