@@ -48,6 +48,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--eager", action="store_true", help="launch kernels one by one instead of replaying a graph")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run a torch device copy of this size (reference kernel for rocprof byte counters)")
     args = ap.parse_args()
@@ -78,6 +79,7 @@ def main():
     case = cs.furnace_large(**spec_args)
     case.ctrl.sweep_order = 1
     case.ctrl.want_pre_residuals = 0
+    case.ctrl.precision = 1 if args.precision == "mixed" else 0
     n = case.ncells
     cspace = CombustionSpace(case, device=0)
     for _ in range(args.warmup):
@@ -108,8 +110,9 @@ def main():
         "metric": "time to residual 1e-5 & Mcell-updates/s: learned+FP32 vs plain FP64, 1 GPU",
         "value": round(value, 3), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
         "ms_per_step": round(dt / max(steps, 1) * 1e3, 3), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"plain FP64 SIMPLER, stretched furnace {args.nx}x{args.ny}x{args.nz} "
+        "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": "synthetic",
+        "config": {"workload": f"{'plain FP64' if args.precision == 'fp64' else 'mixed FP32/FP64'} SIMPLER, "
+                               f"stretched furnace {args.nx}x{args.ny}x{args.nz} "
                                f"({n} cells), zebra x-plane order", "cells": n, "res_mass_last": res_final},
         "roofline": roof,
     }

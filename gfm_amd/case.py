@@ -108,6 +108,7 @@ class CaseSpec:
     faces_z: np.ndarray
     inlets: list
     exit_face: str = "x+"
+    exit_yz: tuple = None        # (y0, y1, z0, z1) port on the x+ end wall; None -> full face
     t_init: float = 1500.0
     k_init: float = None        # SI m2/s2; None -> TKE() from velocity
     eps_init: float = None      # SI m2/s3
@@ -220,10 +221,17 @@ def build(spec: CaseSpec) -> Case:
     cen[0, :, :] = 1; cen[-1, :, :] = 1
     cen[:, 0, :] = 1; cen[:, -1, :] = 1
     cen[:, :, 0] = 1; cen[:, :, -1] = 1
-    if spec.exit_face == "x+":
+    if spec.exit_face != "x+":
+        raise ValueError(spec.exit_face)
+    if spec.exit_yz is None:
         cen[-1, 1:-1, 1:-1] = 3
     else:
-        raise ValueError(spec.exit_face)
+        y0, y1, z0, z1 = spec.exit_yz
+        yc = 0.5 * (np.asarray(spec.faces_y)[:-1] + np.asarray(spec.faces_y)[1:])
+        zc = 0.5 * (np.asarray(spec.faces_z)[:-1] + np.asarray(spec.faces_z)[1:])
+        jy = np.nonzero((yc > y0) & (yc < y1))[0] + 1
+        kz = np.nonzero((zc > z0) & (zc < z1))[0] + 1
+        cen[-1, jy[0]:jy[-1] + 1, kz[0]:kz[-1] + 1] = 3
     if spec.melt_bottom_T is not None:
         cen[1:-1, 1:-1, 0] = 4
     for inl in spec.inlets:
@@ -617,7 +625,7 @@ def furnace_small(nx=48, ny=24, nz=16, seed=1, noise=0.01, maxgi=300, bgcon=1.0e
 
 
 def furnace_large(nx=192, ny=96, nz=64, seed=2, noise=0.01, maxgi=2000, bgcon=1.0e-5, walls="isothermal",
-                  gray_kappa=0.3):
+                  gray_kappa=0.3, exit_port=None):
     """7.0 x 3.0 x 1.6 m furnace, y-faces stretched toward the burner walls
     (ratio 1.05), two opposed CH4 burner pairs on the y walls at x=1.5 m and
     x=4.0 m (fuel 25 m/s 300 K, air 12 m/s 1200 K; the air port is sized
@@ -644,5 +652,5 @@ def furnace_large(nx=192, ny=96, nz=64, seed=2, noise=0.01, maxgi=2000, bgcon=1.
     spec = CaseSpec(name=f"furnace {nx}x{ny}x{nz} stretched", faces_x=fx, faces_y=fy, faces_z=fzf, inlets=inlets,
                     t_init=1500.0, k_init=1.0e-2, eps_init=1.0e-2, noise=noise, seed=seed, maxgi=maxgi, bgcon=bgcon,
                     melt_bottom_T=lambda xx: 1450.0 + 20.0 * math.sin(2.0 * math.pi * xx / Lx + phase),
-                    t_wall=1600.0 if walls == "isothermal" else None, gray_kappa=gray_kappa)
+                    t_wall=1600.0 if walls == "isothermal" else None, gray_kappa=gray_kappa, exit_yz=exit_port)
     return build(spec)

@@ -47,6 +47,10 @@ struct DevCase {
   double *gf[9];
   double *fz, *bs, *sfp, *ap;
   double *as[6];  // AS(.,d,m) -> as[(d-1)+3*(m-1)]
+  // FP32 coefficient storage of the mixed-precision mode
+  float *apf, *bsf;
+  float *asf[6];
+  int mixed;
   const double *rtim, *delrt, *delr, *timed;
   double *scal;  // device scalar block (see SC_* indices)
   long long pstride;  // stride between partial-sum slots

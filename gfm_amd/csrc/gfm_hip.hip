@@ -282,15 +282,23 @@ static void plane_op(gfm_ctx *c, double *xt, int pi0, int pdi, int npl, int jb, 
   const int lw = (npl >= 16) ? 16 : 1;
   const int ntile = (npl + lw - 1) / lw;
   const int nmax = ny > nz ? ny : nz;
-  const size_t shm = 5 * (size_t)lw * (nmax | 1) * sizeof(double);
+  const bool mixed = c->dc.mixed != 0;
+  const size_t shm = 5 * (size_t)lw * (nmax | 1) * (mixed ? sizeof(float) : sizeof(double));
   for (int l = 0; l < 2; l++) {
     if (c->prof.on) prof_begin(c);
     const unsigned nblk = (unsigned)(ntile * (nz + ny));
-    if (lw == 16)
-      hipLaunchKernelGGL(k_lines<16>, dim3(nblk), dim3(1024), shm, c->stream, xt, c->xt2, c->xt3, pi0, pdi, npl, jb, je, kb, ke);
-    else
-      hipLaunchKernelGGL(k_lines<1>, dim3(nblk), dim3(1024), shm, c->stream, xt, c->xt2, c->xt3, pi0, pdi, npl, jb, je, kb, ke);
-    if (c->prof.on) { prof_end(c, "k_lines", 2.0 * line_bytes); prof_begin(c); }
+    if (mixed) {
+      if (lw == 16)
+        hipLaunchKernelGGL((k_lines<16, float>), dim3(nblk), dim3(1024), shm, c->stream, xt, c->xt2, c->xt3, pi0, pdi, npl, jb, je, kb, ke);
+      else
+        hipLaunchKernelGGL((k_lines<1, float>), dim3(nblk), dim3(1024), shm, c->stream, xt, c->xt2, c->xt3, pi0, pdi, npl, jb, je, kb, ke);
+    } else {
+      if (lw == 16)
+        hipLaunchKernelGGL((k_lines<16, double>), dim3(nblk), dim3(1024), shm, c->stream, xt, c->xt2, c->xt3, pi0, pdi, npl, jb, je, kb, ke);
+      else
+        hipLaunchKernelGGL((k_lines<1, double>), dim3(nblk), dim3(1024), shm, c->stream, xt, c->xt2, c->xt3, pi0, pdi, npl, jb, je, kb, ke);
+    }
+    if (c->prof.on) { prof_end(c, "k_lines", 2.0 * line_bytes * (mixed ? 0.6 : 1.0)); prof_begin(c); }
     long long tot = (long long)npl * nzl * nyl;
     hipLaunchKernelGGL(k_line_avg, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, xt, c->xt2, c->xt3, pi0, pdi,
                        npl, jb, nzl, kb, nyl);
@@ -513,7 +521,7 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
     return GFM_E_ARG;
   }
   if (k->sweep_order != GFM_ORDER_REFERENCE && k->sweep_order != GFM_ORDER_ZEBRA) { set_err("bad sweep_order"); return GFM_E_ARG; }
-  if (k->precision != GFM_PREC_FP64) { set_err("precision mode %d not available", k->precision); return GFM_E_ARG; }
+  if (k->precision != GFM_PREC_FP64 && k->precision != GFM_PREC_MIXED) { set_err("bad precision mode %d", k->precision); return GFM_E_ARG; }
   const long long nlat = (long long)k->mp * k->np * k->lp;
   for (long long q = 0; q < nlat; q++)
     if (h->ibcell[q] < 0) { set_err("half-blocked cells (IBCELL<0) are not supported"); return GFM_E_ARG; }
@@ -548,6 +556,12 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
   for (int q = 0; q < 3; q++) if ((rc = dalloc(c, &d.ug[q], N))) return fail(rc);
   for (int q = 0; q < 9; q++) if ((rc = dalloc(c, &d.gf[q], N))) return fail(rc);
   for (int q = 0; q < 6; q++) if ((rc = dalloc(c, &d.as[q], N))) return fail(rc);
+  d.mixed = (k->precision == GFM_PREC_MIXED) ? 1 : 0;
+  if (d.mixed) {
+    if ((rc = dalloc(c, &d.apf, N))) return fail(rc);
+    if ((rc = dalloc(c, &d.bsf, N))) return fail(rc);
+    for (int q = 0; q < 6; q++) if ((rc = dalloc(c, &d.asf[q], N))) return fail(rc);
+  }
   if ((rc = dalloc(c, &c->xt2, N))) return fail(rc);
   if ((rc = dalloc(c, &c->xt3, N))) return fail(rc);
   if ((rc = dalloc(c, &c->stage, N))) return fail(rc);
@@ -586,8 +600,8 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
     int nmax = (k->np > k->lp ? k->np : k->lp) / 2;
     int shm = 5 * 16 * (nmax | 1) * (int)sizeof(double);
     if (shm > 64 * 1024) {
-      HCHECK(hipFuncSetAttribute((const void *)k_lines<16>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
-      HCHECK(hipFuncSetAttribute((const void *)k_lines<1>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+      HCHECK(hipFuncSetAttribute((const void *)k_lines<16, double>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+      HCHECK(hipFuncSetAttribute((const void *)k_lines<1, double>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
     }
   }
   // cell codes and INTP weight codes
@@ -709,6 +723,7 @@ int gfm_hip_adlbl3(gfm_ctx *c, const gfm_ctrl *k, int nelg, const double *ap, co
                    int ib, int ie, int jb, int je, int kb, int ke, double *resg) {
   if (!c || !k || !ap || !as || !bs || !xt) { set_err("null argument"); return GFM_E_ARG; }
   if (nelg < 1 || nelg > 20) { set_err("bad nelg"); return GFM_E_ARG; }
+  if (c->dc.mixed) { set_err("gfm_hip_adlbl3 on host arrays needs a GFM_PREC_FP64 context"); return GFM_E_STATE; }
   int rc = bind(c);
   if (rc) return rc;
   DevCase &d = c->dc;
@@ -731,6 +746,7 @@ int gfm_hip_adlbl3(gfm_ctx *c, const gfm_ctrl *k, int nelg, const double *ap, co
 int gfm_hip_calc_residual(gfm_ctx *c, const double *ap, const double *as, const double *bs, const double *xt, int ib, int ie, int jb,
                           int je, int kb, int ke, double *resid) {
   if (!c || !ap || !as || !bs || !xt || !resid) { set_err("null argument"); return GFM_E_ARG; }
+  if (c->dc.mixed) { set_err("gfm_hip_calc_residual on host arrays needs a GFM_PREC_FP64 context"); return GFM_E_STATE; }
   int rc = bind(c);
   if (rc) return rc;
   DevCase &d = c->dc;

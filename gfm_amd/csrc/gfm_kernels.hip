@@ -34,6 +34,16 @@ static const double h_cutT[5] = {1300.0, 6000.0, 1200.0, 6000.0, 1700.0};
 #define CPF1 (2.156e3 / 1.0057e3)
 
 __device__ __forceinline__ long long lidx(int i, int j, int k) { return lat_index(g.g, i, j, k); }
+
+// Coefficients AP, AS(.,d,m), BS of the equation being assembled/solved.
+// Plain mode stores them in FP64, mixed mode in FP32 (the assembly and the
+// line sweeps then move half the bytes); every reader converts to FP64.
+__device__ __forceinline__ double CAP(long long q) { return g.mixed ? (double)g.apf[q] : g.ap[q]; }
+__device__ __forceinline__ double CBS(long long q) { return g.mixed ? (double)g.bsf[q] : g.bs[q]; }
+__device__ __forceinline__ double CAS(int s, long long q) { return g.mixed ? (double)g.asf[s][q] : g.as[s][q]; }
+__device__ __forceinline__ void SAP(long long q, double v) { if (g.mixed) g.apf[q] = (float)v; else g.ap[q] = v; }
+__device__ __forceinline__ void SBS(long long q, double v) { if (g.mixed) g.bsf[q] = (float)v; else g.bs[q] = v; }
+__device__ __forceinline__ void SAS(int s, long long q, double v) { if (g.mixed) g.asf[s][q] = (float)v; else g.as[s][q] = v; }
 #define MP g.g.MP
 #define NP g.g.NP
 #define LP g.g.LP
@@ -61,7 +71,7 @@ __device__ __forceinline__ long long lidx(int i, int j, int k) { return lat_inde
 #define FZ(i, j, k) g.fz[lidx(i, j, k)]
 #define BS(i, j, k) g.bs[lidx(i, j, k)]
 #define SFP(i, j, k) g.sfp[lidx(i, j, k)]
-#define AP(i, j, k) g.ap[lidx(i, j, k)]
+#define AP(i, j, k) CAP(lidx(i, j, k))
 #define AS(i, j, k, d, m) g.as[((d)-1) + 3 * ((m)-1)][lidx(i, j, k)]
 
 enum { IYF = 1, IYO2 = 2, IH = 3, IK = 4, IEPS = 5, IYCO2 = 6, IYN2 = 7, IYH2O = 9 };
@@ -365,7 +375,7 @@ __global__ void k_momentum_assemble(Box b, int iu1) {
   int i, j, k;
   if (!box_pt(b, i, j, k)) return;
   long long q = lidx(i, j, k);
-  if (IBC(i, j, k) >= 1) { g.bs[q] = 0.0; g.sfp[q] = 0.0; return; }
+  if (IBC(i, j, k) >= 1) { SBS(q, 0.0); return; }
   // ---- SRC1 ----
   double vis[3][2] = {{0, 0}, {0, 0}, {0, 0}}, dxn[3], dx1[3] = {0, 0, 0}, dx0[3] = {0, 0, 0};
   dxn[0] = DX(i); dxn[1] = DR(j) * RS(j); dxn[2] = DZ(k);
@@ -408,7 +418,6 @@ __global__ void k_momentum_assemble(Box b, int iu1) {
   double bs = vms;
   bs = bs + TH(i, j, k) * DN(i, j, k) * g.gra[iu1 - 1];
   const double sfp = -0.0;
-  g.sfp[q] = sfp;
   // ---- GSLVX1 ----
   double ap = 0.0, g0 = 0.0;
   for (int m = -1; m <= 1; m += 2) {  // GSLV3X1
@@ -419,7 +428,7 @@ __global__ void k_momentum_assemble(Box b, int iu1) {
     if (ibc1 == 3) a = 0.0;
     else if (ibc1 == 2) a = FLW(i2, j2, k2, iu1, m1);
     else a = FLWD(i1, j1, k1, iu1, m1, 1.0);
-    g.as[(iu1 - 1) + 3 * (m1 - 1)][q] = a;
+    SAS((iu1 - 1) + 3 * (m1 - 1), q, a);
     ap += a;
     g0 += a * FZ(i2, j2, k2);
   }
@@ -431,8 +440,8 @@ __global__ void k_momentum_assemble(Box b, int iu1) {
     for (int m = -1; m <= 1; m += 2) {
       int m1 = (m + 3) / 2, i1, j1, k1, i2, j2, k2;
       VINDX(iu, m, i, j, k, i1, j1, k1, i2, j2, k2);
-      double *asp = &g.as[(iu - 1) + 3 * (m1 - 1)][q];
-      if (IBC(i1, j1, k1) == 3) { *asp = 0.0; continue; }
+      const int asl = (iu - 1) + 3 * (m1 - 1);
+      if (IBC(i1, j1, k1) == 3) { SAS(asl, q, 0.0); continue; }
       double f = 0.0, diff = 0.0;
       int ii1 = 0, jj1 = 0, kk1 = 0;
       for (int n = -1; n <= 1; n += 2) {
@@ -449,10 +458,10 @@ __global__ void k_momentum_assemble(Box b, int iu1) {
       double fld = (m1 == 1) ? DA(diff, f) + fmax(0.0, f) : DA(diff, f) + fmax(0.0, -f);
       if (IBC(ii1, jj1, kk1) == 2 && IBC(i2, j2, k2) == 1) {
         bs += fld * (UG(ii1, j2, k2, iu1) - UG(i, j, k, iu1));
-        *asp = 0.0;
+        SAS(asl, q, 0.0);
         continue;
       }
-      *asp = fld;
+      SAS(asl, q, fld);
       ap += fld;
       g0 += fld * FZ(i2, j2, k2);
     }
@@ -460,8 +469,8 @@ __global__ void k_momentum_assemble(Box b, int iu1) {
   double vol = (ffl[iu1 - 1][0] + ffl[iu1 - 1][1]) * AREA_C(i, j, k, iu1);  // VOL1
   bs *= vol;
   ap += (0.0 - 0.0 - sfp) * vol;
-  g.bs[q] = bs;
-  g.ap[q] = ap;
+  SBS(q, bs);
+  SAP(q, ap);
   if (fabs(ap) > SMALL20) UG(i, j, k, iu1) = (g0 + bs) / ap;
 }
 
@@ -471,7 +480,7 @@ __global__ void k_pressure_assemble(Box b) {
   int i, j, k;
   if (!box_pt(b, i, j, k)) return;
   long long q = lidx(i, j, k);
-  if (IBC(i, j, k) >= 1) { g.bs[q] = g.p[q]; return; }
+  if (IBC(i, j, k) >= 1) { SBS(q, g.p[q]); return; }
   double ax = AREA_C(i, j, k, 1), ay = AREA_C(i, j, k, 2), az = AREA_C(i, j, k, 3);
   double frx1 = (TH(i - 1, j, k) * DN(i - 1, j, k) * UG(i - 1, j, k, 1)) * ax;
   double frx2 = (TH(i + 1, j, k) * DN(i + 1, j, k) * UG(i + 1, j, k, 1)) * ax;
@@ -479,8 +488,7 @@ __global__ void k_pressure_assemble(Box b) {
   double fry2 = (TH(i, j + 1, k) * DN(i, j + 1, k) * UG(i, j + 1, k, 2)) * ay;
   double frz1 = (TH(i, j, k - 1) * DN(i, j, k - 1) * UG(i, j, k - 1, 3)) * az;
   double frz2 = (TH(i, j, k + 1) * DN(i, j, k + 1) * UG(i, j, k + 1, 3)) * az;
-  g.bs[q] = ((frx1 - frx2) + (fry1 - fry2) + (frz1 - frz2) + 0.0) / g.eun;
-  g.sfp[q] = 0.0;
+  SBS(q, ((frx1 - frx2) + (fry1 - fry2) + (frz1 - frz2) + 0.0) / g.eun);
   double ap = -0.0;
   for (int d = 1; d <= 3; d++) {
     double area = (d == 1) ? ax : (d == 2) ? ay : az;
@@ -495,11 +503,11 @@ __global__ void k_pressure_assemble(Box b) {
         double av = (d == 1) ? DXAV(i1, j1, k1) / DX(i1) : (d == 2) ? DRAV(i1, j1, k1) / DR(j1) : DZAV(i1, j1, k1) / DZ(k1);
         a = th * th * DN(i1, j1, k1) * area * area / apf * av;
       }
-      g.as[(d - 1) + 3 * (m - 1)][q] = a;
+      SAS((d - 1) + 3 * (m - 1), q, a);
       ap += a;
     }
   }
-  g.ap[q] = ap;
+  SAP(q, ap);
 }
 __global__ void k_bound_p(Box b) {
   int i, j, k;
@@ -514,13 +522,13 @@ __global__ void k_starred(Box b, int iu1) {
   if (!box_pt(b, i, j, k)) return;
   if (IBC(i, j, k) >= 1) return;
   long long q = lidx(i, j, k);
-  double ap = g.ap[q] / g.rf[iu1 - 1];
-  g.ap[q] = ap;
+  double ap = CAP(q) / g.rf[iu1 - 1];
+  SAP(q, ap);
   double vol = VOL_C(i, j, k), dpdx;
   if (iu1 == 1) dpdx = (P_(i - 1, j, k) - P_(i + 1, j, k)) / DX(i);
   else if (iu1 == 2) dpdx = (P_(i, j - 1, k) - P_(i, j + 1, k)) / DR(j) / RS(j);
   else dpdx = (P_(i, j, k - 1) - P_(i, j, k + 1)) / DZ(k);
-  g.bs[q] = g.bs[q] + g.eun * g.theta[q] * vol * dpdx + (1.0 - g.rf[iu1 - 1]) * ap * g.fz[q];
+  SBS(q, CBS(q) + g.eun * g.theta[q] * vol * dpdx + (1.0 - g.rf[iu1 - 1]) * ap * g.fz[q]);
 }
 __global__ void k_bound_store_ug(Box b, int iu1) {
   int i, j, k;
@@ -548,8 +556,7 @@ __global__ void k_pcorr_source(Box b, double *part) {
     double frz1 = (TH(i, j, k - 1) * DN(i, j, k - 1) * UG(i, j, k - 1, 3)) * az;
     double frz2 = (TH(i, j, k + 1) * DN(i, j, k + 1) * UG(i, j, k + 1, 3)) * az;
     double bs = ((frx1 - frx2) + (fry1 - fry2) + (frz1 - frz2) + 0.0) / g.eun;
-    g.bs[q] = bs;
-    g.sfp[q] = 0.0;
+    SBS(q, bs);
     bool exitadj = IBC(i - 1, j, k) == 3 || IBC(i + 1, j, k) == 3 || IBC(i, j - 1, k) == 3 ||
                    IBC(i, j + 1, k) == 3 || IBC(i, j, k - 1) == 3 || IBC(i, j, k + 1) == 3;
     if (!exitadj) {
@@ -625,12 +632,13 @@ __global__ void k_vel_correct(Box b, int iu1) {
   int i, j, k;
   if (!box_pt(b, i, j, k)) return;
   long long q = lidx(i, j, k);
-  if (IBC(i, j, k) >= 1 || fabs(g.ap[q]) < SMALL20) return;
+  const double apq = CAP(q);
+  if (IBC(i, j, k) >= 1 || fabs(apq) < SMALL20) return;
   double vol = VOL_C(i, j, k), dpdx;
   if (iu1 == 1) dpdx = (FZ(i - 1, j, k) - FZ(i + 1, j, k)) / DX(i);
   else if (iu1 == 2) dpdx = (FZ(i, j - 1, k) - FZ(i, j + 1, k)) / DR(j) / RS(j);
   else dpdx = (FZ(i, j, k - 1) - FZ(i, j, k + 1)) / DZ(k);
-  double g0 = g.eun * g.theta[q] * dpdx * vol / (g.ap[q] * g.rf[iu1 - 1]);
+  double g0 = g.eun * g.theta[q] * dpdx * vol / (apq * g.rf[iu1 - 1]);
   g.ug[iu1 - 1][q] = BOUND_U(g.ug[iu1 - 1][q] + g0);
 }
 
@@ -882,8 +890,7 @@ __global__ void k_scalar_assemble(Box b, int l, int nelg, int igs, double sgm, d
     long long q = lidx(i, j, k);
     const double fz = g.fz[q];
     if (IBC(i, j, k) >= 1) {
-      g.bs[q] = fz;
-      g.sfp[q] = 0.0;
+      SBS(q, fz);
       if (nelg == 6) { g.crate[q] = 0.0; g.hrate[q] = 0.0; }
     } else {
       double bs = 0.0, sfp = 0.0;
@@ -972,7 +979,8 @@ __global__ void k_scalar_assemble(Box b, int l, int nelg, int igs, double sgm, d
       for (int iu = 1; iu <= 3; iu++)
         for (int m = -1; m <= 1; m += 2) {
           int m1 = (m + 3) / 2;
-          double *asp = &g.as[(iu - 1) + 3 * (m1 - 1)][q];
+          const int asl = (iu - 1) + 3 * (m1 - 1);
+          double av;
           if (igs != 1) {
             int i1, j1, k1, i2, j2, k2;
             VINDX(iu, m, i, j, k, i1, j1, k1, i2, j2, k2);
@@ -981,9 +989,12 @@ __global__ void k_scalar_assemble(Box b, int l, int nelg, int igs, double sgm, d
             if (ibc0 <= 0) a = FLWD(i1, j1, k1, iu, m1, sgm);
             else if (ibc0 == 2) a = FLW(i2, j2, k2, iu, m1);
             else a = 0.0;
-            *asp = a;
+            SAS(asl, q, a);
+            av = g.mixed ? (double)(float)a : a;
+          } else {
+            av = CAS(asl, q);
           }
-          ap += *asp;
+          ap += av;
         }
       // volume and under-relaxation, C/Gslv3g.f90:64-87
       if (fabs(ap) <= SMALL20) {
@@ -996,9 +1007,8 @@ __global__ void k_scalar_assemble(Box b, int l, int nelg, int igs, double sgm, d
         ap = ap / g.rfl[l - 1];
         bs = bs + (1.0 - g.rfl[l - 1]) * fz * ap;
       }
-      g.ap[q] = ap;
-      g.bs[q] = bs;
-      g.sfp[q] = sfp;
+      SAP(q, ap);
+      SBS(q, bs);
     }
   }
   if (nelg == 8) {
@@ -1224,14 +1234,15 @@ __global__ void k_residual(Box b, const double *xt, double *part, int irc) {
   double s = 0.0, s2 = 0.0, mx = 0.0;
   if (box_pt(b, i, j, k) && IBC(i, j, k) <= 0) {
     long long q = lidx(i, j, k);
-    double t1 = g.as[0][q] * xt[lidx(i - 2, j, k)], t2 = g.as[3][q] * xt[lidx(i + 2, j, k)];
-    double t3 = g.as[1][q] * xt[lidx(i, j - 2, k)], t4 = g.as[4][q] * xt[lidx(i, j + 2, k)];
-    double t5 = g.as[2][q] * xt[lidx(i, j, k - 2)], t6 = g.as[5][q] * xt[lidx(i, j, k + 2)];
-    double t7 = g.ap[q] * xt[q], t8 = g.bs[q];
+    double t1 = CAS(0, q) * xt[lidx(i - 2, j, k)], t2 = CAS(3, q) * xt[lidx(i + 2, j, k)];
+    double t3 = CAS(1, q) * xt[lidx(i, j - 2, k)], t4 = CAS(4, q) * xt[lidx(i, j + 2, k)];
+    double t5 = CAS(2, q) * xt[lidx(i, j, k - 2)], t6 = CAS(5, q) * xt[lidx(i, j, k + 2)];
+    const double apq = CAP(q);
+    double t7 = apq * xt[q], t8 = CBS(q);
     if (!irc) {
       double eq_max = fmax(fmax(fmax(fabs(t1), fabs(t2)), fmax(fabs(t3), fabs(t4))), fmax(fmax(fabs(t5), fabs(t6)), fmax(fabs(t7), fabs(t8))));
       s = fabs(t1 + t2 + t3 + t4 + t5 + t6 + t8 - t7) / eq_max;
-    } else if (g.ap[q] != 1.0) {
+    } else if (apq != 1.0) {
       double r = fabs(t2 + t1 + t4 + t3 + t6 + t5 + t8 - t7);
       s2 = r; mx = r;
     }
@@ -1302,10 +1313,24 @@ __global__ void k_line_avg(double *xt, const double *x2, const double *x3, int p
 // wavefront solves one line by parallel cyclic reduction and the tile is
 // written back cooperatively.  Blocks [0, ntile*nz) do y-lines into xy,
 // the rest z-lines into xz.
-template <int LW>
+template <typename T>
+__device__ __forceinline__ T coef_ap(long long q) {
+  if constexpr (sizeof(T) == 4) return g.apf[q]; else return g.ap[q];
+}
+template <typename T>
+__device__ __forceinline__ T coef_bs(long long q) {
+  if constexpr (sizeof(T) == 4) return g.bsf[q]; else return g.bs[q];
+}
+template <typename T>
+__device__ __forceinline__ T coef_as(int s, long long q) {
+  if constexpr (sizeof(T) == 4) return g.asf[s][q]; else return g.as[s][q];
+}
+
+template <int LW, typename T>
 __global__ __launch_bounds__(1024) void k_lines(const double *__restrict__ xt, double *__restrict__ xy, double *__restrict__ xz,
                                                 int pi0, int pdi, int npl, int jb, int je, int kb, int ke) {
-  extern __shared__ double sm[];
+  extern __shared__ double smd[];
+  T *sm = reinterpret_cast<T *>(smd);
   const int ny = (je - jb) / 2 + 1, nz = (ke - kb) / 2 + 1;
   const int ntile = (npl + LW - 1) / LW;
   int b = blockIdx.x, dir, n, o, tile;
@@ -1313,7 +1338,7 @@ __global__ __launch_bounds__(1024) void k_lines(const double *__restrict__ xt, d
   if (b < ntile * nz) { dir = 2; n = ny; tile = b % ntile; o = b / ntile; xo = xy; }
   else { b -= ntile * nz; dir = 3; n = nz; tile = b % ntile; o = b / ntile; xo = xz; }
   const int npad = n | 1;
-  double *A = sm, *B = A + LW * npad, *Cc = B + LW * npad, *D = Cc + LW * npad, *RB = D + LW * npad;
+  T *A = sm, *B = A + LW * npad, *Cc = B + LW * npad, *D = Cc + LW * npad, *RB = D + LW * npad;
   const int t = threadIdx.x;
   const int w = t % LW, rg = t / LW, nrg = 1024 / LW;
   const int pl = tile * LW + w;
@@ -1324,24 +1349,24 @@ __global__ __launch_bounds__(1024) void k_lines(const double *__restrict__ xt, d
     int j = (dir == 2) ? jb + 2 * e : jb + 2 * o;
     int k = (dir == 2) ? kb + 2 * o : kb + 2 * e;
     long long q = lidx(i, j, k);
-    double ap = g.ap[q];
-    double a, bb, c, d;
-    if (g.ib[q] >= 1 || fabs(ap) <= SMALL20) {
-      a = 0.0; bb = 1.0; c = 0.0; d = xt[q];
+    T ap = coef_ap<T>(q);
+    T a, bb, c, d;
+    if (g.ib[q] >= 1 || fabs((double)ap) <= SMALL20) {
+      a = 0; bb = 1; c = 0; d = (T)xt[q];
     } else {
-      double bsy = g.bs[q];
-      bsy = bsy + g.as[3][q] * xt[lidx(i + 2, j, k)] + g.as[0][q] * xt[lidx(i - 2, j, k)];
-      double am, apn;
+      T bsy = coef_bs<T>(q);
+      bsy = bsy + coef_as<T>(3, q) * (T)xt[lidx(i + 2, j, k)] + coef_as<T>(0, q) * (T)xt[lidx(i - 2, j, k)];
+      T am, apn;
       if (dir == 2) {
-        bsy = bsy + g.as[5][q] * xt[lidx(i, j, k + 2)] + g.as[2][q] * xt[lidx(i, j, k - 2)];
-        am = g.as[1][q]; apn = g.as[4][q];
-        if (e == 0) { bsy += am * xt[lidx(i, j - 2, k)]; am = 0.0; }
-        if (e == n - 1) { bsy += apn * xt[lidx(i, j + 2, k)]; apn = 0.0; }
+        bsy = bsy + coef_as<T>(5, q) * (T)xt[lidx(i, j, k + 2)] + coef_as<T>(2, q) * (T)xt[lidx(i, j, k - 2)];
+        am = coef_as<T>(1, q); apn = coef_as<T>(4, q);
+        if (e == 0) { bsy += am * (T)xt[lidx(i, j - 2, k)]; am = 0; }
+        if (e == n - 1) { bsy += apn * (T)xt[lidx(i, j + 2, k)]; apn = 0; }
       } else {
-        bsy = bsy + g.as[4][q] * xt[lidx(i, j + 2, k)] + g.as[1][q] * xt[lidx(i, j - 2, k)];
-        am = g.as[2][q]; apn = g.as[5][q];
-        if (e == 0) { bsy += am * xt[lidx(i, j, k - 2)]; am = 0.0; }
-        if (e == n - 1) { bsy += apn * xt[lidx(i, j, k + 2)]; apn = 0.0; }
+        bsy = bsy + coef_as<T>(4, q) * (T)xt[lidx(i, j + 2, k)] + coef_as<T>(1, q) * (T)xt[lidx(i, j - 2, k)];
+        am = coef_as<T>(2, q); apn = coef_as<T>(5, q);
+        if (e == 0) { bsy += am * (T)xt[lidx(i, j, k - 2)]; am = 0; }
+        if (e == n - 1) { bsy += apn * (T)xt[lidx(i, j, k + 2)]; apn = 0; }
       }
       a = -am; bb = ap; c = -apn; d = bsy;
     }
@@ -1350,28 +1375,28 @@ __global__ __launch_bounds__(1024) void k_lines(const double *__restrict__ xt, d
   __syncthreads();
   const int wave = t >> 6, lane = t & 63;
   const bool solve = (wave < LW) && (tile * LW + wave < npl);
-  double *a_ = A + wave * npad, *b_ = B + wave * npad, *c_ = Cc + wave * npad, *d_ = D + wave * npad;
-  double *rb_ = RB + wave * npad;
+  T *a_ = A + wave * npad, *b_ = B + wave * npad, *c_ = Cc + wave * npad, *d_ = D + wave * npad;
+  T *rb_ = RB + wave * npad;
   // rb_ holds 1/b: one division per element and reduction step
   if (solve)
-    for (int e = lane; e < n; e += 64) rb_[e] = 1.0 / b_[e];
+    for (int e = lane; e < n; e += 64) rb_[e] = (T)1 / b_[e];
   __syncthreads();
-  double na[PCR_MAXN / 64], nb_[PCR_MAXN / 64], nc[PCR_MAXN / 64], nd[PCR_MAXN / 64], nr[PCR_MAXN / 64];
+  T na[PCR_MAXN / 64], nb_[PCR_MAXN / 64], nc[PCR_MAXN / 64], nd[PCR_MAXN / 64], nr[PCR_MAXN / 64];
   for (int s = 1; s < n; s <<= 1) {
     if (solve) {
 #pragma unroll
       for (int u = 0; u < PCR_MAXN / 64; u++) {
         int e = lane + 64 * u;
         if (e >= n) continue;
-        double ae = a_[e], be = b_[e], ce = c_[e], de = d_[e];
-        double am = 0.0, rbm = 1.0, cm = 0.0, dm = 0.0, ap2 = 0.0, rbp = 1.0, cp = 0.0, dp = 0.0;
+        T ae = a_[e], be = b_[e], ce = c_[e], de = d_[e];
+        T am = 0, rbm = 1, cm = 0, dm = 0, ap2 = 0, rbp = 1, cp = 0, dp = 0;
         if (e - s >= 0) { am = a_[e - s]; rbm = rb_[e - s]; cm = c_[e - s]; dm = d_[e - s]; }
         if (e + s < n) { ap2 = a_[e + s]; rbp = rb_[e + s]; cp = c_[e + s]; dp = d_[e + s]; }
-        double al = -ae * rbm, ga = -ce * rbp;
+        T al = -ae * rbm, ga = -ce * rbp;
         na[u] = al * am;
         nc[u] = ga * cp;
         nb_[u] = be + al * cm + ga * ap2;
-        nr[u] = 1.0 / nb_[u];
+        nr[u] = (T)1 / nb_[u];
         nd[u] = de + al * dm + ga * dp;
       }
     }
@@ -1393,7 +1418,7 @@ __global__ __launch_bounds__(1024) void k_lines(const double *__restrict__ xt, d
     if (!valid) break;
     int j = (dir == 2) ? jb + 2 * e : jb + 2 * o;
     int k = (dir == 2) ? kb + 2 * o : kb + 2 * e;
-    xo[lidx(i, j, k)] = D[w * npad + e];
+    xo[lidx(i, j, k)] = (double)D[w * npad + e];
   }
 }
 
