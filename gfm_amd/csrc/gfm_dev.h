@@ -8,13 +8,18 @@
 // x-plane in the line relaxation) so that planes of one colour are
 // adjacent in memory:
 //
-//   xpos(c)      = (c&1)*HX + (c>>1),      HX = ceil(NXC/2), NXC = MP/2+1
+//   xpos(c)      = (c&1)*HX + (c>>1) + 15,
 //   offset(I,J,K) = cls*S + ((K>>1)*NYC + (J>>1))*NXP + xpos(I>>1),
-//   cls = (I&1) | (J&1)<<1 | (K&1)<<2,  NXP = 2*HX,  S = NXP*NYC*NZC.
+//   cls = (I&1) | (J&1)<<1 | (K&1)<<2,  NXP = 2*HX,  S = NXP*NYC*NZC,
+//   HX = ceil((ceil(NXC/2) + 15)/16)*16,  NXC = MP/2+1.
 //
-// Kernels keep the reference's lattice indices (I,J,K).  A wavefront
-// walking consecutive I of one class touches two contiguous segments; a
-// wavefront walking consecutive planes of one colour touches one.
+// HX is a multiple of 16 doubles (128 bytes, one L2 line) and the offset 15
+// puts the first interior plane of either colour (c = 2 or 3, c>>1 = 1) at
+// the start of a line, so a tile of 16 consecutive planes of one colour is
+// exactly one line of each row.  Kernels keep the reference's lattice
+// indices (I,J,K).  A wavefront walking consecutive I of one class touches
+// two contiguous segments; a wavefront walking consecutive planes of one
+// colour touches one.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -22,14 +27,14 @@
 struct DevGeom {
   int MP, NP, LP;
   int NXC, NYC, NZC;
-  int HX, NXP;  // half row and padded row length
+  int HX, NXP;  // half-row stride and row length (multiples of 16)
   long long S;  // entries per class
 };
 
 __host__ __device__ __forceinline__ long long lat_index(const DevGeom &gg, int i, int j, int k) {
   int cls = (i & 1) | ((j & 1) << 1) | ((k & 1) << 2);
   int c = i >> 1;
-  int xp = (c & 1) * gg.HX + (c >> 1);
+  int xp = (c & 1) * gg.HX + (c >> 1) + 15;
   return (long long)cls * gg.S + ((long long)(k >> 1) * gg.NYC + (j >> 1)) * gg.NXP + xp;
 }
 

@@ -540,7 +540,7 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
   DevGeom &gg = c->dc.g;
   gg.MP = k->mp; gg.NP = k->np; gg.LP = k->lp;
   gg.NXC = k->mp / 2 + 1; gg.NYC = k->np / 2 + 1; gg.NZC = k->lp / 2 + 1;
-  gg.HX = (gg.NXC + 1) / 2;
+  gg.HX = ((gg.NXC + 1) / 2 + 15 + 15) / 16 * 16;
   gg.NXP = 2 * gg.HX;
   gg.S = (long long)gg.NXP * gg.NYC * gg.NZC;
   c->ncls = gg.S * 8;
