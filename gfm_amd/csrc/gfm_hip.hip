@@ -214,6 +214,14 @@ static int scatter(gfm_ctx *c, const double *h, double *d) {
   hipLaunchKernelGGL(k_scatter, dim3((unsigned)((c->nlat + 255) / 256)), dim3(256), 0, c->stream, c->stage, d, c->nlat);
   return GFM_OK;
 }
+// Coefficient array of a staged system: FP64 copy, and its FP32 copy in a
+// mixed-precision context (the line kernels and residuals read that one).
+static int scatter_coef(gfm_ctx *c, const double *h, double *d, float *d32) {
+  int rc = scatter(c, h, d);
+  if (rc || !c->dc.mixed) return rc;
+  hipLaunchKernelGGL(k_narrow, dim3((unsigned)((c->ncls + 255) / 256)), dim3(256), 0, c->stream, d, d32, c->ncls);
+  return GFM_OK;
+}
 static int gather(gfm_ctx *c, const double *d, double *h) {
   hipLaunchKernelGGL(k_gather, dim3((unsigned)((c->nlat + 255) / 256)), dim3(256), 0, c->stream, d, c->stage, c->nlat);
   HCHECK(hipMemcpyAsync(h, c->stage, c->nlat * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -279,17 +287,19 @@ static void plane_op(gfm_ctx *c, double *xt, int pi0, int pdi, int npl, int jb, 
   const int ny = nzl, nz = nyl;                                  // rows of a y-line, of a z-line
   const double cells = (double)npl * nyl * nzl;
   const double line_bytes = cells * 10.0 * sizeof(double);
-  const int lw = (npl >= 16) ? 16 : 1;
+  const bool mixed = c->dc.mixed != 0;
+  // one 128-byte line of each row per tile: 16 doubles or 32 floats
+  const int lwide = mixed ? 32 : 16;
+  const int lw = (npl >= lwide) ? lwide : 1;
   const int ntile = (npl + lw - 1) / lw;
   const int nmax = ny > nz ? ny : nz;
-  const bool mixed = c->dc.mixed != 0;
-  const size_t shm = 5 * (size_t)lw * (nmax | 1) * (mixed ? sizeof(float) : sizeof(double));
+  const size_t shm = 3 * (size_t)lw * (nmax | 1) * (mixed ? sizeof(float) : sizeof(double));
   for (int l = 0; l < 2; l++) {
     if (c->prof.on) prof_begin(c);
     const unsigned nblk = (unsigned)(ntile * (nz + ny));
     if (mixed) {
-      if (lw == 16)
-        hipLaunchKernelGGL((k_lines<16, float>), dim3(nblk), dim3(1024), shm, c->stream, xt, c->xt2, c->xt3, pi0, pdi, npl, jb, je, kb, ke);
+      if (lw == 32)
+        hipLaunchKernelGGL((k_lines<32, float>), dim3(nblk), dim3(1024), shm, c->stream, xt, c->xt2, c->xt3, pi0, pdi, npl, jb, je, kb, ke);
       else
         hipLaunchKernelGGL((k_lines<1, float>), dim3(nblk), dim3(1024), shm, c->stream, xt, c->xt2, c->xt3, pi0, pdi, npl, jb, je, kb, ke);
     } else {
@@ -597,11 +607,17 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
   fill_devcase_scalars(c, k);
   if ((rc = bind(c))) return fail(rc);
   {
+    // LDS of k_lines: 3 arrays of LW lines x (longest line | 1) words
     int nmax = (k->np > k->lp ? k->np : k->lp) / 2;
-    int shm = 5 * 16 * (nmax | 1) * (int)sizeof(double);
-    if (shm > 64 * 1024) {
-      HCHECK(hipFuncSetAttribute((const void *)k_lines<16, double>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
-      HCHECK(hipFuncSetAttribute((const void *)k_lines<1, double>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+    int shm64 = 3 * 16 * (nmax | 1) * (int)sizeof(double);
+    int shm32 = 3 * 32 * (nmax | 1) * (int)sizeof(float);
+    if (shm64 > 64 * 1024) {
+      HCHECK(hipFuncSetAttribute((const void *)k_lines<16, double>, hipFuncAttributeMaxDynamicSharedMemorySize, shm64));
+      HCHECK(hipFuncSetAttribute((const void *)k_lines<1, double>, hipFuncAttributeMaxDynamicSharedMemorySize, shm64));
+    }
+    if (shm32 > 64 * 1024) {
+      HCHECK(hipFuncSetAttribute((const void *)k_lines<32, float>, hipFuncAttributeMaxDynamicSharedMemorySize, shm32));
+      HCHECK(hipFuncSetAttribute((const void *)k_lines<1, float>, hipFuncAttributeMaxDynamicSharedMemorySize, shm32));
     }
   }
   // cell codes and INTP weight codes
@@ -719,20 +735,33 @@ int gfm_hip_sphase(gfm_ctx *c, gfm_ctrl *k, int max_iters, double *res_hist, int
   return GFM_OK;
 }
 
+// A box of ADLBL3/calc_residual reads XT two lattice steps outside itself
+// (C/Adlbl3.f90:147-221), so it must lie in [3, M-2] along each axis with
+// both ends on the same parity.
+static int check_box(const gfm_ctx *c, int ib, int ie, int jb, int je, int kb, int ke) {
+  const DevGeom &gg = c->dc.g;
+  auto ok = [](int b, int e, int m) { return b >= 3 && b <= e && e <= m - 2 && ((e - b) & 1) == 0; };
+  if (!ok(ib, ie, gg.MP) || !ok(jb, je, gg.NP) || !ok(kb, ke, gg.LP)) {
+    set_err("box must satisfy 3 <= begin <= end <= M-2 with end-begin even on each axis");
+    return GFM_E_ARG;
+  }
+  return GFM_OK;
+}
+
 int gfm_hip_adlbl3(gfm_ctx *c, const gfm_ctrl *k, int nelg, const double *ap, const double *as, const double *bs, double *xt,
                    int ib, int ie, int jb, int je, int kb, int ke, double *resg) {
   if (!c || !k || !ap || !as || !bs || !xt) { set_err("null argument"); return GFM_E_ARG; }
   if (nelg < 1 || nelg > 20) { set_err("bad nelg"); return GFM_E_ARG; }
-  if (c->dc.mixed) { set_err("gfm_hip_adlbl3 on host arrays needs a GFM_PREC_FP64 context"); return GFM_E_STATE; }
+  if (int rb = check_box(c, ib, ie, jb, je, kb, ke)) return rb;
   int rc = bind(c);
   if (rc) return rc;
   DevCase &d = c->dc;
   c->ctrl.sweep_order = k->sweep_order;
   c->ctrl.want_irc = k->want_irc;
   for (int q = 0; q < 20; q++) c->ctrl.ntimesg[q] = k->ntimesg[q];
-  if ((rc = scatter(c, ap, d.ap))) return rc;
-  for (int q = 0; q < 6; q++) if ((rc = scatter(c, as + (size_t)q * c->nlat, d.as[q]))) return rc;
-  if ((rc = scatter(c, bs, d.bs))) return rc;
+  if ((rc = scatter_coef(c, ap, d.ap, d.apf))) return rc;
+  for (int q = 0; q < 6; q++) if ((rc = scatter_coef(c, as + (size_t)q * c->nlat, d.as[q], d.asf[q]))) return rc;
+  if ((rc = scatter_coef(c, bs, d.bs, d.bsf))) return rc;
   if ((rc = scatter(c, xt, d.fz))) return rc;
   adlbl3(c, d.fz, nelg, ib, ie, jb, je, kb, ke);
   if ((rc = gather(c, d.fz, xt))) return rc;
@@ -746,13 +775,13 @@ int gfm_hip_adlbl3(gfm_ctx *c, const gfm_ctrl *k, int nelg, const double *ap, co
 int gfm_hip_calc_residual(gfm_ctx *c, const double *ap, const double *as, const double *bs, const double *xt, int ib, int ie, int jb,
                           int je, int kb, int ke, double *resid) {
   if (!c || !ap || !as || !bs || !xt || !resid) { set_err("null argument"); return GFM_E_ARG; }
-  if (c->dc.mixed) { set_err("gfm_hip_calc_residual on host arrays needs a GFM_PREC_FP64 context"); return GFM_E_STATE; }
+  if (int rb = check_box(c, ib, ie, jb, je, kb, ke)) return rb;
   int rc = bind(c);
   if (rc) return rc;
   DevCase &d = c->dc;
-  if ((rc = scatter(c, ap, d.ap))) return rc;
-  for (int q = 0; q < 6; q++) if ((rc = scatter(c, as + (size_t)q * c->nlat, d.as[q]))) return rc;
-  if ((rc = scatter(c, bs, d.bs))) return rc;
+  if ((rc = scatter_coef(c, ap, d.ap, d.apf))) return rc;
+  for (int q = 0; q < 6; q++) if ((rc = scatter_coef(c, as + (size_t)q * c->nlat, d.as[q], d.asf[q]))) return rc;
+  if ((rc = scatter_coef(c, bs, d.bs, d.bsf))) return rc;
   if ((rc = scatter(c, xt, c->xt2))) return rc;
   residual(c, c->xt2, ib, ie, jb, je, kb, ke, 63, 0);
   double sc[SC_COUNT];

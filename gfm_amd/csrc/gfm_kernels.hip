@@ -1306,13 +1306,15 @@ __global__ void k_line_avg(double *xt, const double *x2, const double *x3, int p
 
 // Line relaxation, tiled.  A 1024-thread workgroup takes LW lines that sit
 // at the same (k) -- y-lines -- or the same (j) -- z-lines -- on LW
-// consecutive planes of one colour, which are adjacent in memory.  The
-// workgroup loads the rows of the tile cooperatively (16 consecutive
-// threads read 16 adjacent doubles), forms the tridiagonal rows of
-// ADLZY/ADLYZ (C/Adlbl3.f90:147-221) in LDS as [line][row], then each
-// wavefront solves one line by parallel cyclic reduction and the tile is
-// written back cooperatively.  Blocks [0, ntile*nz) do y-lines into xy,
-// the rest z-lines into xz.
+// consecutive planes of one colour, which are adjacent in memory (16
+// doubles or 32 floats: one 128-byte line of each row).  The workgroup
+// loads the rows of the tile cooperatively, every load of a row issued
+// before any is used, forms the tridiagonal rows of ADLZY/ADLYZ
+// (C/Adlbl3.f90:147-221) and stores them in LDS normalised to a unit
+// diagonal, [line][row].  Each wavefront then solves whole lines by
+// parallel cyclic reduction with wavefront-scope synchronisation only,
+// and the tile is written back cooperatively.  Blocks [0, ntile*nz) do
+// y-lines into xy, the rest z-lines into xz.
 template <typename T>
 __device__ __forceinline__ T coef_ap(long long q) {
   if constexpr (sizeof(T) == 4) return g.apf[q]; else return g.ap[q];
@@ -1324,6 +1326,14 @@ __device__ __forceinline__ T coef_bs(long long q) {
 template <typename T>
 __device__ __forceinline__ T coef_as(int s, long long q) {
   if constexpr (sizeof(T) == 4) return g.asf[s][q]; else return g.as[s][q];
+}
+
+// Orders the LDS accesses of the lanes of one wavefront around the point
+// (all lanes must reach it together).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 template <int LW, typename T>
@@ -1338,86 +1348,88 @@ __global__ __launch_bounds__(1024) void k_lines(const double *__restrict__ xt, d
   if (b < ntile * nz) { dir = 2; n = ny; tile = b % ntile; o = b / ntile; xo = xy; }
   else { b -= ntile * nz; dir = 3; n = nz; tile = b % ntile; o = b / ntile; xo = xz; }
   const int npad = n | 1;
-  T *A = sm, *B = A + LW * npad, *Cc = B + LW * npad, *D = Cc + LW * npad, *RB = D + LW * npad;
+  T *A = sm, *Cc = A + LW * npad, *D = Cc + LW * npad;
   const int t = threadIdx.x;
-  const int w = t % LW, rg = t / LW, nrg = 1024 / LW;
+  constexpr int NRG = 1024 / LW;
+  const int w = t % LW, rg = t / LW;
   const int pl = tile * LW + w;
   const bool valid = pl < npl;
   const int i = pi0 + pdi * (valid ? pl : 0);
-  for (int e = rg; e < n; e += nrg) {
+  // AS slots along the line (minus, plus) and across it, AS(.,d,m) -> (d-1)+3*(m-1)
+  const int s_lm = (dir == 2) ? 1 : 2, s_lp = s_lm + 3;
+  const int s_cm = (dir == 2) ? 2 : 1, s_cp = s_cm + 3;
+  for (int e = rg; e < n; e += NRG) {
     if (!valid) break;
-    int j = (dir == 2) ? jb + 2 * e : jb + 2 * o;
-    int k = (dir == 2) ? kb + 2 * o : kb + 2 * e;
-    long long q = lidx(i, j, k);
-    T ap = coef_ap<T>(q);
-    T a, bb, c, d;
-    if (g.ib[q] >= 1 || fabs((double)ap) <= SMALL20) {
-      a = 0; bb = 1; c = 0; d = (T)xt[q];
+    const int j = (dir == 2) ? jb + 2 * e : jb + 2 * o;
+    const int k = (dir == 2) ? kb + 2 * o : kb + 2 * e;
+    const int dj = (dir == 2) ? 2 : 0, dk = 2 - dj;  // step along the line
+    const long long q = lidx(i, j, k);
+    const int code = g.ib[q];
+    const T ap = coef_ap<T>(q), bs = coef_bs<T>(q);
+    const T ax_p = coef_as<T>(3, q), ax_m = coef_as<T>(0, q);
+    const T ac_p = coef_as<T>(s_cp, q), ac_m = coef_as<T>(s_cm, q);
+    const T al_m = coef_as<T>(s_lm, q), al_p = coef_as<T>(s_lp, q);
+    const double x0 = xt[q];
+    const double xxp = xt[lidx(i + 2, j, k)], xxm = xt[lidx(i - 2, j, k)];
+    const double xcp = xt[lidx(i, j + 2 - dj, k + 2 - dk)], xcm = xt[lidx(i, j - 2 + dj, k - 2 + dk)];
+    const double xlm = (e == 0) ? xt[lidx(i, j - dj, k - dk)] : 0.0;
+    const double xlp = (e == n - 1) ? xt[lidx(i, j + dj, k + dk)] : 0.0;
+    T a, c, d;
+    if (code >= 1 || fabs((double)ap) <= SMALL20) {
+      a = 0; c = 0; d = (T)x0;
     } else {
-      T bsy = coef_bs<T>(q);
-      bsy = bsy + coef_as<T>(3, q) * (T)xt[lidx(i + 2, j, k)] + coef_as<T>(0, q) * (T)xt[lidx(i - 2, j, k)];
-      T am, apn;
-      if (dir == 2) {
-        bsy = bsy + coef_as<T>(5, q) * (T)xt[lidx(i, j, k + 2)] + coef_as<T>(2, q) * (T)xt[lidx(i, j, k - 2)];
-        am = coef_as<T>(1, q); apn = coef_as<T>(4, q);
-        if (e == 0) { bsy += am * (T)xt[lidx(i, j - 2, k)]; am = 0; }
-        if (e == n - 1) { bsy += apn * (T)xt[lidx(i, j + 2, k)]; apn = 0; }
-      } else {
-        bsy = bsy + coef_as<T>(4, q) * (T)xt[lidx(i, j + 2, k)] + coef_as<T>(1, q) * (T)xt[lidx(i, j - 2, k)];
-        am = coef_as<T>(2, q); apn = coef_as<T>(5, q);
-        if (e == 0) { bsy += am * (T)xt[lidx(i, j, k - 2)]; am = 0; }
-        if (e == n - 1) { bsy += apn * (T)xt[lidx(i, j, k + 2)]; apn = 0; }
-      }
-      a = -am; bb = ap; c = -apn; d = bsy;
+      T bsy = bs + ax_p * (T)xxp + ax_m * (T)xxm;
+      bsy = bsy + ac_p * (T)xcp + ac_m * (T)xcm;
+      T am = al_m, apn = al_p;
+      if (e == 0) { bsy += am * (T)xlm; am = 0; }
+      if (e == n - 1) { bsy += apn * (T)xlp; apn = 0; }
+      const T r = (T)1 / ap;
+      a = -am * r; c = -apn * r; d = bsy * r;
     }
-    A[w * npad + e] = a; B[w * npad + e] = bb; Cc[w * npad + e] = c; D[w * npad + e] = d;
+    A[w * npad + e] = a; Cc[w * npad + e] = c; D[w * npad + e] = d;
   }
   __syncthreads();
+  // PCR on unit-diagonal rows: row e (a, 1, c | d) combined with rows e-s
+  // and e+s gives (-a*a_m, 1 - a*c_m - c*a_p, -c*c_p | d - a*d_m - c*d_p),
+  // renormalised.  After ceil(log2 n) steps every row is (0, 1, 0 | x).
   const int wave = t >> 6, lane = t & 63;
-  const bool solve = (wave < LW) && (tile * LW + wave < npl);
-  T *a_ = A + wave * npad, *b_ = B + wave * npad, *c_ = Cc + wave * npad, *d_ = D + wave * npad;
-  T *rb_ = RB + wave * npad;
-  // rb_ holds 1/b: one division per element and reduction step
-  if (solve)
-    for (int e = lane; e < n; e += 64) rb_[e] = (T)1 / b_[e];
-  __syncthreads();
-  T na[PCR_MAXN / 64], nb_[PCR_MAXN / 64], nc[PCR_MAXN / 64], nd[PCR_MAXN / 64], nr[PCR_MAXN / 64];
-  for (int s = 1; s < n; s <<= 1) {
-    if (solve) {
+  constexpr int NU = PCR_MAXN / 64;
+  for (int ln = wave; ln < LW; ln += 16) {
+    if (tile * LW + ln >= npl) break;
+    T *a_ = A + ln * npad, *c_ = Cc + ln * npad, *d_ = D + ln * npad;
+    T ra[NU], rc[NU], rd[NU];
 #pragma unroll
-      for (int u = 0; u < PCR_MAXN / 64; u++) {
-        int e = lane + 64 * u;
-        if (e >= n) continue;
-        T ae = a_[e], be = b_[e], ce = c_[e], de = d_[e];
-        T am = 0, rbm = 1, cm = 0, dm = 0, ap2 = 0, rbp = 1, cp = 0, dp = 0;
-        if (e - s >= 0) { am = a_[e - s]; rbm = rb_[e - s]; cm = c_[e - s]; dm = d_[e - s]; }
-        if (e + s < n) { ap2 = a_[e + s]; rbp = rb_[e + s]; cp = c_[e + s]; dp = d_[e + s]; }
-        T al = -ae * rbm, ga = -ce * rbp;
-        na[u] = al * am;
-        nc[u] = ga * cp;
-        nb_[u] = be + al * cm + ga * ap2;
-        nr[u] = (T)1 / nb_[u];
-        nd[u] = de + al * dm + ga * dp;
-      }
+    for (int u = 0; u < NU; u++) {
+      const int e = lane + 64 * u;
+      ra[u] = 0; rc[u] = 0; rd[u] = 0;
+      if (e < n) { ra[u] = a_[e]; rc[u] = c_[e]; rd[u] = d_[e]; }
     }
-    __syncthreads();
-    if (solve) {
+    for (int s = 1; s < n; s <<= 1) {
 #pragma unroll
-      for (int u = 0; u < PCR_MAXN / 64; u++) {
-        int e = lane + 64 * u;
+      for (int u = 0; u < NU; u++) {
+        const int e = lane + 64 * u;
         if (e >= n) continue;
-        a_[e] = na[u]; b_[e] = nb_[u]; c_[e] = nc[u]; d_[e] = nd[u]; rb_[e] = nr[u];
+        T am = 0, cm = 0, dm = 0, apl = 0, cp = 0, dp = 0;
+        if (e - s >= 0) { am = a_[e - s]; cm = c_[e - s]; dm = d_[e - s]; }
+        if (e + s < n) { apl = a_[e + s]; cp = c_[e + s]; dp = d_[e + s]; }
+        const T r = (T)1 / ((T)1 - ra[u] * cm - rc[u] * apl);
+        const T na = -ra[u] * am * r, nc = -rc[u] * cp * r, nd = (rd[u] - ra[u] * dm - rc[u] * dp) * r;
+        ra[u] = na; rc[u] = nc; rd[u] = nd;
       }
+      wave_sync();
+#pragma unroll
+      for (int u = 0; u < NU; u++) {
+        const int e = lane + 64 * u;
+        if (e < n) { a_[e] = ra[u]; c_[e] = rc[u]; d_[e] = rd[u]; }
+      }
+      wave_sync();
     }
-    __syncthreads();
   }
-  if (solve)
-    for (int e = lane; e < n; e += 64) d_[e] = d_[e] * rb_[e];
   __syncthreads();
-  for (int e = rg; e < n; e += nrg) {
+  for (int e = rg; e < n; e += NRG) {
     if (!valid) break;
-    int j = (dir == 2) ? jb + 2 * e : jb + 2 * o;
-    int k = (dir == 2) ? kb + 2 * o : kb + 2 * e;
+    const int j = (dir == 2) ? jb + 2 * e : jb + 2 * o;
+    const int k = (dir == 2) ? kb + 2 * o : kb + 2 * e;
     xo[lidx(i, j, k)] = (double)D[w * npad + e];
   }
 }
@@ -1433,6 +1445,11 @@ __global__ void k_scatter(const double *src, double *dst, long long n) {
   int j = (int)((f / MP) % NP) + 1;
   int k = (int)(f / ((long long)MP * NP)) + 1;
   dst[lidx(i, j, k)] = src[f];
+}
+// FP32 copy of a coefficient array (mixed-precision contexts).
+__global__ void k_narrow(const double *src, float *dst, long long n) {
+  long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (f < n) dst[f] = (float)src[f];
 }
 __global__ void k_gather(const double *src, double *dst, long long n) {
   long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x;

@@ -152,13 +152,17 @@ int gfm_hip_sphase(gfm_ctx *ctx, gfm_ctrl *ctrl, int max_iters, double *res_hist
 
 /* ADLBL3 on a host-provided system (C/Adlbl3.f90:10-97): AP, AS(…,3,2),
  * BS and XT on the doubled lattice, NTIMESG(nelg) sweeps over the index
- * box [ib..ie]x[jb..je]x[kb..ke] (stride 2, 1-based).  XT is updated in
- * place; RESG(nelg,1:2) is returned in resg[0..1] when want_irc is set. */
+ * box [ib..ie]x[jb..je]x[kb..ke] (stride 2, 1-based; 3 <= begin <= end <=
+ * M-2 on each axis, else GFM_E_ARG).  XT is updated in place;
+ * RESG(nelg,1:2) is returned in resg[0..1] when want_irc is set.  In a
+ * GFM_PREC_MIXED context the coefficients are rounded to FP32 on the
+ * device and the sweeps run in FP32, as inside gfm_hip_gflow. */
 int gfm_hip_adlbl3(gfm_ctx *ctx, const gfm_ctrl *ctrl, int nelg,
                    const double *ap, const double *as, const double *bs, double *xt,
                    int ib, int ie, int jb, int je, int kb, int ke, double *resg);
 
-/* calc_residual (C/Adlbl3.f90:239-271) on a host-provided system. */
+/* calc_residual (C/Adlbl3.f90:239-271) on a host-provided system (box as
+ * for gfm_hip_adlbl3; FP32-rounded coefficients in a mixed context). */
 int gfm_hip_calc_residual(gfm_ctx *ctx, const double *ap, const double *as, const double *bs,
                           const double *xt, int ib, int ie, int jb, int je, int kb, int ke,
                           double *resid);
