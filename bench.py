@@ -19,6 +19,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PMC_SUMMARY = os.path.join("profiles", "pmc_k_lines_furnace192x96x64.md")
+
+
+def measured_traffic(precision, dims):
+    """HBM bytes per k_lines launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (counters cannot be read from inside this process).
+    Only for the workload the counters were taken on."""
+    if dims != (192, 96, 64):
+        return None
+    try:
+        with open(os.path.join(ROOT, PMC_SUMMARY)) as f:
+            for ln in f:
+                if ln.startswith(f"traffic_bytes_per_launch[{precision}] = "):
+                    return float(ln.split("=")[1])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(spec_args, iters=2):
@@ -102,10 +119,11 @@ def main():
     avg_s = ms / 1e3 / max(la, 1)
     per_launch = by / max(la, 1)
     achieved = per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    traffic = measured_traffic(args.precision, (args.nx, args.ny, args.nz))
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": "k_lines", "avg_launch_us": round(avg_s * 1e6, 2), "bytes_per_launch": per_launch,
-            "launches": la}
+            "launches": la, "traffic_source": PMC_SUMMARY if traffic else None}
     line = {
         "metric": "time to residual 1e-5 & Mcell-updates/s: learned+FP32 vs plain FP64, 1 GPU",
         "value": round(value, 3), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,

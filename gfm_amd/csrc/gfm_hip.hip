@@ -31,6 +31,8 @@ static void set_err(const char *fmt, ...) {
     }                                                                              \
   } while (0)
 
+#include "gfm_pnet.hip"
+
 // residual slots in the device scalar block
 enum { RS_P_PRE = 0, RS_P = 1, RS_U_PRE = 2, RS_U = 5, RS_PC = 8, RS_GF_PRE = 10, RS_GF = 20, RS_IRC = 30 };
 
@@ -54,6 +56,7 @@ struct gfm_ctx {
   DevCase dc;
   std::vector<void *> allocs;
   double *xt2 = nullptr, *xt3 = nullptr, *part = nullptr, *stage = nullptr;
+  float *xtf = nullptr;  // FP32 working copy of the unknown of ADLBL3 (mixed precision)
   int16_t *stage16 = nullptr;
   long long nlat = 0, ncls = 0;
   hipStream_t stream = nullptr;
@@ -282,37 +285,73 @@ static void pull_scalars(gfm_ctx *c, gfm_ctrl *k, const double *sc) {
 // ---------------------------------------------------------------------
 // ADLBL3 (C/Adlbl3.f90:10-97) and calc_residual
 // ---------------------------------------------------------------------
-static void plane_op(gfm_ctx *c, double *xt, int pi0, int pdi, int npl, int jb, int je, int kb, int ke) {
+// Line-kernel instance: tile width (16 planes, or 2 for fewer planes than
+// a tile) and rows per lane NU (lines of up to 32*NU rows).
+static int lines_lw(int npl) { return npl >= 16 ? 16 : 2; }
+static int lines_nu(int nmax) { return nmax <= 64 ? 2 : nmax <= 96 ? 3 : nmax <= 128 ? 4 : 8; }
+static size_t lines_shm(int lw, int nmax, size_t word) { return 3 * (size_t)lw * (nmax | 1) * word; }
+#define LINES_INSTANCES(X_) X_(16, 2) X_(16, 3) X_(16, 4) X_(16, 8) X_(2, 2) X_(2, 3) X_(2, 4) X_(2, 8)
+template <typename T>
+static void launch_lines(gfm_ctx *c, int lw, int nu, unsigned nblk, size_t shm, const T *xt, T *xy, T *xz, int pi0, int pdi,
+                         int npl, int jb, int je, int kb, int ke) {
+#define LINES_CASE(LW_, NU_)                                                                                        \
+  if (lw == LW_ && nu == NU_) {                                                                                     \
+    hipLaunchKernelGGL((k_lines<LW_, T, NU_>), dim3(nblk), dim3(LINES_NT), shm, c->stream, xt, xy, xz, pi0, pdi, npl, \
+                       jb, je, kb, ke);                                                                             \
+    return;                                                                                                         \
+  }
+  LINES_INSTANCES(LINES_CASE)
+#undef LINES_CASE
+}
+template <typename T>
+static int set_lines_attr(size_t shm) {
+#define LINES_ATTR(LW_, NU_) \
+  HCHECK(hipFuncSetAttribute((const void *)k_lines<LW_, T, NU_>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+  LINES_INSTANCES(LINES_ATTR)
+#undef LINES_ATTR
+  return GFM_OK;
+}
+
+// One zebra half-step or one reference-order plane: the two passes of ADLZ
+// on npl planes i = pi0 + pdi*p, each pass a y/z line solve from the same
+// XT followed by the average.
+template <typename T>
+static void plane_op(gfm_ctx *c, T *xt, T *x2, T *x3, int pi0, int pdi, int npl, int jb, int je, int kb, int ke) {
   const int nyl = (ke - kb) / 2 + 1, nzl = (je - jb) / 2 + 1;  // y-lines per plane, z-lines per plane
   const int ny = nzl, nz = nyl;                                  // rows of a y-line, of a z-line
   const double cells = (double)npl * nyl * nzl;
-  const double line_bytes = cells * 10.0 * sizeof(double);
-  const bool mixed = c->dc.mixed != 0;
-  // one 128-byte line of each row per tile: 16 doubles or 32 floats
-  const int lwide = mixed ? 32 : 16;
-  const int lw = (npl >= lwide) ? lwide : 1;
+  // algorithmic bytes: 8 coefficients, one read and one write of the unknown per row and direction
+  const double line_bytes = cells * 2.0 * (8.0 * sizeof(T) + 2.0 * sizeof(T));
+  const int lw = lines_lw(npl);
   const int ntile = (npl + lw - 1) / lw;
   const int nmax = ny > nz ? ny : nz;
-  const size_t shm = 3 * (size_t)lw * (nmax | 1) * (mixed ? sizeof(float) : sizeof(double));
+  const int nu = lines_nu(nmax);
+  const size_t shm = lines_shm(lw, nmax, sizeof(T));
+  const unsigned nblk = (unsigned)(ntile * (nz + ny));
+  const long long tot = (long long)npl * nzl * nyl;
   for (int l = 0; l < 2; l++) {
     if (c->prof.on) prof_begin(c);
-    const unsigned nblk = (unsigned)(ntile * (nz + ny));
-    if (mixed) {
-      if (lw == 32)
-        hipLaunchKernelGGL((k_lines<32, float>), dim3(nblk), dim3(1024), shm, c->stream, xt, c->xt2, c->xt3, pi0, pdi, npl, jb, je, kb, ke);
-      else
-        hipLaunchKernelGGL((k_lines<1, float>), dim3(nblk), dim3(1024), shm, c->stream, xt, c->xt2, c->xt3, pi0, pdi, npl, jb, je, kb, ke);
-    } else {
-      if (lw == 16)
-        hipLaunchKernelGGL((k_lines<16, double>), dim3(nblk), dim3(1024), shm, c->stream, xt, c->xt2, c->xt3, pi0, pdi, npl, jb, je, kb, ke);
-      else
-        hipLaunchKernelGGL((k_lines<1, double>), dim3(nblk), dim3(1024), shm, c->stream, xt, c->xt2, c->xt3, pi0, pdi, npl, jb, je, kb, ke);
-    }
-    if (c->prof.on) { prof_end(c, "k_lines", 2.0 * line_bytes * (mixed ? 0.6 : 1.0)); prof_begin(c); }
-    long long tot = (long long)npl * nzl * nyl;
-    hipLaunchKernelGGL(k_line_avg, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, xt, c->xt2, c->xt3, pi0, pdi,
+    launch_lines<T>(c, lw, nu, nblk, shm, xt, x2, x3, pi0, pdi, npl, jb, je, kb, ke);
+    if (c->prof.on) { prof_end(c, "k_lines", line_bytes); prof_begin(c); }
+    hipLaunchKernelGGL(k_line_avg<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, xt, x2, x3, pi0, pdi,
                        npl, jb, nzl, kb, nyl);
-    if (c->prof.on) prof_end(c, "k_line_avg", cells * 3.0 * sizeof(double));
+    if (c->prof.on) prof_end(c, "k_line_avg", cells * 3.0 * sizeof(T));
+  }
+}
+template <typename T>
+static void sweeps(gfm_ctx *c, T *xt, T *x2, T *x3, int nim, int ib, int ie, int jb, int je, int kb, int ke) {
+  const int npl = (ie - ib) / 2 + 1;
+  for (int s = 0; s < nim; s++) {
+    if (c->ctrl.sweep_order == GFM_ORDER_REFERENCE) {
+      for (int i = ib; i <= ie; i += 2) plane_op<T>(c, xt, x2, x3, i, 0, 1, jb, je, kb, ke);
+      for (int i = ie; i >= ib; i -= 2) plane_op<T>(c, xt, x2, x3, i, 0, 1, jb, je, kb, ke);
+    } else {
+      for (int half = 0; half < 4; half++) {
+        int col = half & 1;
+        int n = (npl - col + 1) / 2;
+        if (n > 0) plane_op<T>(c, xt, x2, x3, ib + 2 * col, 4, n, jb, je, kb, ke);
+      }
+    }
   }
 }
 static void residual(gfm_ctx *c, const double *xt, int ib, int ie, int jb, int je, int kb, int ke, int slot, int irc) {
@@ -321,19 +360,19 @@ static void residual(gfm_ctx *c, const double *xt, int ib, int ie, int jb, int j
   hipLaunchKernelGGL(k_residual_finalize, dim3(1), dim3(256), 0, c->stream, c->part, (int)box_blocks(b), SC_RESID0 + slot, irc);
 }
 static void adlbl3(gfm_ctx *c, double *xt, int nelg, int ib, int ie, int jb, int je, int kb, int ke) {
-  int nim = c->ctrl.ntimesg[nelg - 1];
-  int npl = (ie - ib) / 2 + 1;
-  for (int s = 0; s < nim; s++) {
-    if (c->ctrl.sweep_order == GFM_ORDER_REFERENCE) {
-      for (int i = ib; i <= ie; i += 2) plane_op(c, xt, i, 0, 1, jb, je, kb, ke);
-      for (int i = ie; i >= ib; i -= 2) plane_op(c, xt, i, 0, 1, jb, je, kb, ke);
-    } else {
-      for (int half = 0; half < 4; half++) {
-        int col = half & 1;
-        int n = (npl - col + 1) / 2;
-        if (n > 0) plane_op(c, xt, ib + 2 * col, 4, n, jb, je, kb, ke);
-      }
-    }
+  const int nim = c->ctrl.ntimesg[nelg - 1];
+  if (!c->dc.mixed) {
+    sweeps<double>(c, xt, c->xt2, c->xt3, nim, ib, ie, jb, je, kb, ke);
+  } else {
+    // FP32 sweeps on a working copy of the box and the neighbours it reads
+    // (one lattice step of the class outside the box); rows the sweeps pin
+    // keep their FP64 values.
+    Box ext = make_box(ib - 2, ie + 2, jb - 2, je + 2, kb - 2, ke + 2, 2);
+    Box box = make_box(ib, ie, jb, je, kb, ke, 2);
+    LAUNCH(k_box_narrow, ext, xt, c->xtf);
+    sweeps<float>(c, c->xtf, reinterpret_cast<float *>(c->xt2), reinterpret_cast<float *>(c->xt3), nim, ib, ie, jb, je, kb,
+                  ke);
+    LAUNCH(k_box_widen, box, c->xtf, xt);
   }
   if (c->ctrl.want_irc) residual(c, xt, ib, ie, jb, je, kb, ke, RS_IRC + 2 * (nelg - 1), 1);
 }
@@ -572,6 +611,11 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
     if ((rc = dalloc(c, &d.bsf, N))) return fail(rc);
     for (int q = 0; q < 6; q++) if ((rc = dalloc(c, &d.asf[q], N))) return fail(rc);
   }
+  if (d.mixed) {
+    float *xf;
+    if ((rc = dalloc(c, &xf, N))) return fail(rc);
+    c->xtf = xf;
+  }
   if ((rc = dalloc(c, &c->xt2, N))) return fail(rc);
   if ((rc = dalloc(c, &c->xt3, N))) return fail(rc);
   if ((rc = dalloc(c, &c->stage, N))) return fail(rc);
@@ -607,17 +651,11 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
   fill_devcase_scalars(c, k);
   if ((rc = bind(c))) return fail(rc);
   {
-    // LDS of k_lines: 3 arrays of LW lines x (longest line | 1) words
+    // LDS of k_lines beyond the default 64 KB for long lines
     int nmax = (k->np > k->lp ? k->np : k->lp) / 2;
-    int shm64 = 3 * 16 * (nmax | 1) * (int)sizeof(double);
-    int shm32 = 3 * 32 * (nmax | 1) * (int)sizeof(float);
-    if (shm64 > 64 * 1024) {
-      HCHECK(hipFuncSetAttribute((const void *)k_lines<16, double>, hipFuncAttributeMaxDynamicSharedMemorySize, shm64));
-      HCHECK(hipFuncSetAttribute((const void *)k_lines<1, double>, hipFuncAttributeMaxDynamicSharedMemorySize, shm64));
-    }
-    if (shm32 > 64 * 1024) {
-      HCHECK(hipFuncSetAttribute((const void *)k_lines<32, float>, hipFuncAttributeMaxDynamicSharedMemorySize, shm32));
-      HCHECK(hipFuncSetAttribute((const void *)k_lines<1, float>, hipFuncAttributeMaxDynamicSharedMemorySize, shm32));
+    size_t shm = lines_shm(16, nmax, d.mixed ? sizeof(float) : sizeof(double));
+    if (shm > 64 * 1024) {
+      if ((rc = d.mixed ? set_lines_attr<float>(shm) : set_lines_attr<double>(shm))) return fail(rc);
     }
   }
   // cell codes and INTP weight codes

@@ -1292,7 +1292,8 @@ __global__ void k_residual_finalize(const double *part, int nb, int slot, int ir
 
 // XT = (XT2 + XT3)/2 on the planes of one sweep step, C/Adlbl3.f90:134-138.
 // Consecutive threads take consecutive planes of one colour (contiguous).
-__global__ void k_line_avg(double *xt, const double *x2, const double *x3, int pi0, int pdi, int npl, int jb, int nj, int kb, int nk) {
+template <typename T>
+__global__ void k_line_avg(T *xt, const T *x2, const T *x3, int pi0, int pdi, int npl, int jb, int nj, int kb, int nk) {
   long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   long long tot = (long long)npl * nj * nk;
   if (t >= tot) return;
@@ -1301,20 +1302,36 @@ __global__ void k_line_avg(double *xt, const double *x2, const double *x3, int p
   int jj = (int)(r % nj), kk = (int)(r / nj);
   int i = pi0 + pdi * pl, j = jb + 2 * jj, k = kb + 2 * kk;
   long long q = lidx(i, j, k);
-  xt[q] = (x2[q] + x3[q]) / 2.0;
+  xt[q] = (x2[q] + x3[q]) / (T)2;
 }
 
-// Line relaxation, tiled.  A 1024-thread workgroup takes LW lines that sit
+// FP32 working copy of an unknown over a box (mixed precision) and back.
+__global__ void k_box_narrow(Box b, const double *src, float *dst) {
+  int i, j, k;
+  if (!box_pt(b, i, j, k)) return;
+  long long q = lidx(i, j, k);
+  dst[q] = (float)src[q];
+}
+__global__ void k_box_widen(Box b, const float *src, double *dst) {
+  int i, j, k;
+  if (!box_pt(b, i, j, k)) return;
+  long long q = lidx(i, j, k);
+  dst[q] = (double)src[q];
+}
+
+// Line relaxation, tiled.  A 512-thread workgroup takes LW lines that sit
 // at the same (k) -- y-lines -- or the same (j) -- z-lines -- on LW
 // consecutive planes of one colour, which are adjacent in memory (16
-// doubles or 32 floats: one 128-byte line of each row).  The workgroup
-// loads the rows of the tile cooperatively, every load of a row issued
-// before any is used, forms the tridiagonal rows of ADLZY/ADLYZ
-// (C/Adlbl3.f90:147-221) and stores them in LDS normalised to a unit
-// diagonal, [line][row].  Each wavefront then solves whole lines by
-// parallel cyclic reduction with wavefront-scope synchronisation only,
-// and the tile is written back cooperatively.  Blocks [0, ntile*nz) do
-// y-lines into xy, the rest z-lines into xz.
+// planes: one 128-byte line of each FP64 row).  The workgroup loads the
+// rows of the tile cooperatively, every load of a row issued before any is
+// used, forms the tridiagonal rows of ADLZY/ADLYZ (C/Adlbl3.f90:147-221)
+// and stores them in LDS normalised to a unit diagonal, [line][row] with an
+// odd row pitch.  Each wavefront then solves two lines, 32 lanes and up to
+// NU rows per lane each, by parallel cyclic reduction with
+// wavefront-scope synchronisation only, and the tile is written back
+// cooperatively.  Blocks [0, ntile*nz) do y-lines into xy, the rest
+// z-lines into xz.  T is the arithmetic and storage type of the sweep
+// (FP64, or FP32 in mixed precision, where xt is the FP32 working copy).
 template <typename T>
 __device__ __forceinline__ T coef_ap(long long q) {
   if constexpr (sizeof(T) == 4) return g.apf[q]; else return g.ap[q];
@@ -1328,6 +1345,18 @@ __device__ __forceinline__ T coef_as(int s, long long q) {
   if constexpr (sizeof(T) == 4) return g.asf[s][q]; else return g.as[s][q];
 }
 
+// 1/x: the hardware reciprocal, refined by two Newton-Raphson steps in
+// FP64 (error below 2 ulp); the hardware FP32 reciprocal (1 ulp) as is.
+template <typename T> __device__ __forceinline__ T line_rcp(T x);
+template <> __device__ __forceinline__ double line_rcp<double>(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+template <> __device__ __forceinline__ float line_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // Orders the LDS accesses of the lanes of one wavefront around the point
 // (all lanes must reach it together).
 __device__ __forceinline__ void wave_sync() {
@@ -1336,21 +1365,24 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int LW, typename T>
-__global__ __launch_bounds__(1024) void k_lines(const double *__restrict__ xt, double *__restrict__ xy, double *__restrict__ xz,
-                                                int pi0, int pdi, int npl, int jb, int je, int kb, int ke) {
+template <typename T> struct LinePair { T a, c; };
+
+#define LINES_NT 512
+template <int LW, typename T, int NU>
+__global__ __launch_bounds__(LINES_NT) void k_lines(const T *__restrict__ xt, T *__restrict__ xy, T *__restrict__ xz,
+                                                    int pi0, int pdi, int npl, int jb, int je, int kb, int ke) {
   extern __shared__ double smd[];
-  T *sm = reinterpret_cast<T *>(smd);
   const int ny = (je - jb) / 2 + 1, nz = (ke - kb) / 2 + 1;
   const int ntile = (npl + LW - 1) / LW;
   int b = blockIdx.x, dir, n, o, tile;
-  double *xo;
+  T *xo;
   if (b < ntile * nz) { dir = 2; n = ny; tile = b % ntile; o = b / ntile; xo = xy; }
   else { b -= ntile * nz; dir = 3; n = nz; tile = b % ntile; o = b / ntile; xo = xz; }
   const int npad = n | 1;
-  T *A = sm, *Cc = A + LW * npad, *D = Cc + LW * npad;
+  LinePair<T> *AC = reinterpret_cast<LinePair<T> *>(smd);
+  T *D = reinterpret_cast<T *>(AC + LW * npad);
   const int t = threadIdx.x;
-  constexpr int NRG = 1024 / LW;
+  constexpr int NRG = LINES_NT / LW;
   const int w = t % LW, rg = t / LW;
   const int pl = tile * LW + w;
   const bool valid = pl < npl;
@@ -1358,69 +1390,73 @@ __global__ __launch_bounds__(1024) void k_lines(const double *__restrict__ xt, d
   // AS slots along the line (minus, plus) and across it, AS(.,d,m) -> (d-1)+3*(m-1)
   const int s_lm = (dir == 2) ? 1 : 2, s_lp = s_lm + 3;
   const int s_cm = (dir == 2) ? 2 : 1, s_cp = s_cm + 3;
+  const int dj = (dir == 2) ? 2 : 0, dk = 2 - dj;  // lattice step along the line
+#pragma unroll 2
   for (int e = rg; e < n; e += NRG) {
     if (!valid) break;
     const int j = (dir == 2) ? jb + 2 * e : jb + 2 * o;
     const int k = (dir == 2) ? kb + 2 * o : kb + 2 * e;
-    const int dj = (dir == 2) ? 2 : 0, dk = 2 - dj;  // step along the line
     const long long q = lidx(i, j, k);
     const int code = g.ib[q];
     const T ap = coef_ap<T>(q), bs = coef_bs<T>(q);
     const T ax_p = coef_as<T>(3, q), ax_m = coef_as<T>(0, q);
     const T ac_p = coef_as<T>(s_cp, q), ac_m = coef_as<T>(s_cm, q);
     const T al_m = coef_as<T>(s_lm, q), al_p = coef_as<T>(s_lp, q);
-    const double x0 = xt[q];
-    const double xxp = xt[lidx(i + 2, j, k)], xxm = xt[lidx(i - 2, j, k)];
-    const double xcp = xt[lidx(i, j + 2 - dj, k + 2 - dk)], xcm = xt[lidx(i, j - 2 + dj, k - 2 + dk)];
-    const double xlm = (e == 0) ? xt[lidx(i, j - dj, k - dk)] : 0.0;
-    const double xlp = (e == n - 1) ? xt[lidx(i, j + dj, k + dk)] : 0.0;
+    const T x0 = xt[q];
+    const T xxp = xt[lidx(i + 2, j, k)], xxm = xt[lidx(i - 2, j, k)];
+    const T xcp = xt[lidx(i, j + 2 - dj, k + 2 - dk)], xcm = xt[lidx(i, j - 2 + dj, k - 2 + dk)];
+    const T xlm = (e == 0) ? xt[lidx(i, j - dj, k - dk)] : (T)0;
+    const T xlp = (e == n - 1) ? xt[lidx(i, j + dj, k + dk)] : (T)0;
     T a, c, d;
     if (code >= 1 || fabs((double)ap) <= SMALL20) {
-      a = 0; c = 0; d = (T)x0;
+      a = 0; c = 0; d = x0;
     } else {
-      T bsy = bs + ax_p * (T)xxp + ax_m * (T)xxm;
-      bsy = bsy + ac_p * (T)xcp + ac_m * (T)xcm;
+      T bsy = bs + ax_p * xxp + ax_m * xxm;
+      bsy = bsy + ac_p * xcp + ac_m * xcm;
       T am = al_m, apn = al_p;
-      if (e == 0) { bsy += am * (T)xlm; am = 0; }
-      if (e == n - 1) { bsy += apn * (T)xlp; apn = 0; }
-      const T r = (T)1 / ap;
+      if (e == 0) { bsy += am * xlm; am = 0; }
+      if (e == n - 1) { bsy += apn * xlp; apn = 0; }
+      const T r = line_rcp<T>(ap);
       a = -am * r; c = -apn * r; d = bsy * r;
     }
-    A[w * npad + e] = a; Cc[w * npad + e] = c; D[w * npad + e] = d;
+    AC[w * npad + e] = LinePair<T>{a, c};
+    D[w * npad + e] = d;
   }
   __syncthreads();
   // PCR on unit-diagonal rows: row e (a, 1, c | d) combined with rows e-s
   // and e+s gives (-a*a_m, 1 - a*c_m - c*a_p, -c*c_p | d - a*d_m - c*d_p),
   // renormalised.  After ceil(log2 n) steps every row is (0, 1, 0 | x).
-  const int wave = t >> 6, lane = t & 63;
-  constexpr int NU = PCR_MAXN / 64;
-  for (int ln = wave; ln < LW; ln += 16) {
-    if (tile * LW + ln >= npl) break;
-    T *a_ = A + ln * npad, *c_ = Cc + ln * npad, *d_ = D + ln * npad;
+  const int wave = t >> 6, lane = t & 63, half = lane >> 5, l32 = lane & 31;
+  for (int pr = wave; pr < LW / 2; pr += LINES_NT / 64) {
+    const int ln = 2 * pr + half;
+    const bool act = tile * LW + ln < npl;  // differs between the halves: every wave_sync stays outside it
+    LinePair<T> *ac_ = AC + ln * npad;
+    T *d_ = D + ln * npad;
     T ra[NU], rc[NU], rd[NU];
 #pragma unroll
     for (int u = 0; u < NU; u++) {
-      const int e = lane + 64 * u;
+      const int e = l32 + 32 * u;
       ra[u] = 0; rc[u] = 0; rd[u] = 0;
-      if (e < n) { ra[u] = a_[e]; rc[u] = c_[e]; rd[u] = d_[e]; }
+      if (act && e < n) { const LinePair<T> p = ac_[e]; ra[u] = p.a; rc[u] = p.c; rd[u] = d_[e]; }
     }
     for (int s = 1; s < n; s <<= 1) {
 #pragma unroll
       for (int u = 0; u < NU; u++) {
-        const int e = lane + 64 * u;
-        if (e >= n) continue;
-        T am = 0, cm = 0, dm = 0, apl = 0, cp = 0, dp = 0;
-        if (e - s >= 0) { am = a_[e - s]; cm = c_[e - s]; dm = d_[e - s]; }
-        if (e + s < n) { apl = a_[e + s]; cp = c_[e + s]; dp = d_[e + s]; }
-        const T r = (T)1 / ((T)1 - ra[u] * cm - rc[u] * apl);
-        const T na = -ra[u] * am * r, nc = -rc[u] * cp * r, nd = (rd[u] - ra[u] * dm - rc[u] * dp) * r;
+        const int e = l32 + 32 * u;
+        if (!act || e >= n) continue;
+        LinePair<T> pm{0, 0}, pp{0, 0};
+        T dm = 0, dp = 0;
+        if (e - s >= 0) { pm = ac_[e - s]; dm = d_[e - s]; }
+        if (e + s < n) { pp = ac_[e + s]; dp = d_[e + s]; }
+        const T r = line_rcp<T>((T)1 - ra[u] * pm.c - rc[u] * pp.a);
+        const T na = -ra[u] * pm.a * r, nc = -rc[u] * pp.c * r, nd = (rd[u] - ra[u] * dm - rc[u] * dp) * r;
         ra[u] = na; rc[u] = nc; rd[u] = nd;
       }
       wave_sync();
 #pragma unroll
       for (int u = 0; u < NU; u++) {
-        const int e = lane + 64 * u;
-        if (e < n) { a_[e] = ra[u]; c_[e] = rc[u]; d_[e] = rd[u]; }
+        const int e = l32 + 32 * u;
+        if (act && e < n) { ac_[e] = LinePair<T>{ra[u], rc[u]}; d_[e] = rd[u]; }
       }
       wave_sync();
     }
@@ -1430,7 +1466,7 @@ __global__ __launch_bounds__(1024) void k_lines(const double *__restrict__ xt, d
     if (!valid) break;
     const int j = (dir == 2) ? jb + 2 * e : jb + 2 * o;
     const int k = (dir == 2) ? kb + 2 * o : kb + 2 * e;
-    xo[lidx(i, j, k)] = (double)D[w * npad + e];
+    xo[lidx(i, j, k)] = D[w * npad + e];
   }
 }
 

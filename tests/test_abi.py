@@ -1,6 +1,6 @@
 """CPU checks of the C-ABI: the library and the oracle load, export every
-symbol include/gfm_hip.h declares, and agree with the ctypes mirror on the
-size of the scalar block.  No compute calls (no GPU needed)."""
+symbol the headers in include/ declare, and agree with the ctypes mirror on
+the size of the scalar block.  No compute calls (no GPU needed)."""
 import ctypes as ct
 import os
 import re
@@ -15,14 +15,18 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared():
-    txt = open(os.path.join(ROOT, "include", "gfm_hip.h")).read()
-    return sorted(set(re.findall(r"\b(gfm_hip_\w+)\s*\(", txt)))
+    names = set()
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if h.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", h)).read()
+            names |= set(re.findall(r"\b(gfm_(?:hip|pnet)_\w+)\s*\(", txt))
+    return sorted(names)
 
 
 def test_library_exports_header_symbols():
     lib = solver.lib()
     out = subprocess.run(["nm", "-D", "--defined-only", solver.LIB_PATH], capture_output=True, text=True, check=True).stdout
-    exported = set(re.findall(r" T (gfm_hip_\w+)", out))
+    exported = set(re.findall(r" T (gfm_(?:hip|pnet)_\w+)", out))
     missing = [s for s in declared() if s not in exported]
     assert not missing, missing
     for s in declared():

@@ -20,8 +20,8 @@ from helpers import manufactured
 pytestmark = pytest.mark.gpu
 
 
-def _setup(order, sweeps, nx=20, precision=0):
-    c = cs.furnace_small(nx=nx, ny=14, nz=12, seed=5)
+def _setup(order, sweeps, nx=20, precision=0, ny=14, nz=12):
+    c = cs.furnace_small(nx=nx, ny=ny, nz=nz, seed=5)
     c.ctrl.sweep_order = order
     c.ctrl.precision = precision
     c.ctrl.ntimesg[3] = sweeps
@@ -29,12 +29,13 @@ def _setup(order, sweeps, nx=20, precision=0):
     return c, box
 
 
-@pytest.mark.parametrize("order,nx", [(1, 20), (0, 20), (1, 72)])
-def test_adlbl3_matches_oracle(order, nx):
+@pytest.mark.parametrize("order,nx,ny", [(1, 20, 14), (0, 20, 14), (1, 72, 14), (1, 34, 100), (1, 34, 140)])
+def test_adlbl3_matches_oracle(order, nx, ny):
     """Three sweeps from xt = 0 in the box: every value within 1e-12 of the
     oracle's (relative to the largest).  nx=72 (36 planes per colour) runs
-    the tiled line kernel."""
-    c, box = _setup(order, 3, nx)
+    the tiled line kernel; ny=100 and 140 give y-lines longer than 96 and
+    128 rows (4 and 8 rows per lane)."""
+    c, box = _setup(order, 3, nx, ny=ny)
     ap, as_, bs, xt, xs, sl = manufactured(c, box, seed=11)
     xo = xt.copy(order="F")
     oracle.adlbl3(c, 4, ap, as_, bs, xo, box)
@@ -42,7 +43,7 @@ def test_adlbl3_matches_oracle(order, nx):
         xg = xt.copy(order="F")
         s.adlbl3(4, ap, as_, bs, xg, box)
     err = np.max(np.abs(xg - xo)) / np.max(np.abs(xo))
-    print("adlbl3 order", order, "nx", nx, "max rel diff", err, flush=True)
+    print("adlbl3 order", order, "nx", nx, "ny", ny, "max rel diff", err, flush=True)
     assert err < 1e-12
     assert np.max(np.abs(xg[sl] - xs[sl])) > 1e-6  # not yet converged: the comparison is of the iterates
 

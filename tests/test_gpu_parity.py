@@ -151,7 +151,10 @@ def test_mixed_precision_first_iteration(nx):
 def test_mixed_precision_tracks_fp64_developed(nx, ny, nz, ndev):
     """From a developed flow (ndev FP64 iterations) both precisions run 20
     more iterations: every field within 5e-5 relative and the res_mass
-    history within 1e-4 relative.  64x24x16 has 32 planes per colour, one
+    history within 1e-3 relative (res_mass sums face-flux imbalances, each
+    a difference of nearly equal fluxes, so its relative error is the
+    fields' relative error times flux / imbalance, ~10-100 here).
+    64x24x16 has 32 planes per colour, one
     full FP32 tile of the line kernel.  Only flows that settle qualify: in
     one that keeps oscillating (the 72x12x10 furnace, res_mass ~0.08) an
     FP64 run perturbed by 1e-7 leaves the unperturbed one as fast as the
@@ -172,6 +175,6 @@ def test_mixed_precision_tracks_fp64_developed(nx, ny, nz, ndev):
         runs[prec] = (c, h)
     errs = field_errors(runs[0][0], runs[1][0])
     print("mixed vs fp64 developed", errs, runs[0][1][-1], runs[1][1][-1], flush=True)
-    np.testing.assert_allclose(runs[1][1], runs[0][1], rtol=1e-4)
+    np.testing.assert_allclose(runs[1][1], runs[0][1], rtol=1e-3)
     for k, v in errs.items():
         assert v < 5e-5, (k, v)

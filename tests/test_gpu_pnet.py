@@ -1,0 +1,86 @@
+"""The p' guess network's BF16 matrix-core kernels (forward, input and
+weight gradients, Adam) against the CPU restatement oracle/pnet_oracle.py,
+which rounds to BF16 at the same points, so the two differ only by
+accumulation order (FP32 on the GPU, FP64 on the CPU) and the rare BF16
+rounding it flips.  Tolerances: forward output within 5e-3 relative L2,
+loss within 1e-3, each layer's weight and bias gradient within 1e-2
+relative L2."""
+import numpy as np
+import pytest
+
+import pnet_oracle as po
+from gfm_amd.pnet import PNet
+
+pytestmark = pytest.mark.gpu
+
+CH = (32, 32, 64, 64)
+
+
+def _batch(rng, B, D, H, W):
+    x = np.zeros((B, D, H, W, 2), dtype=np.float32)
+    x[..., 0] = rng.standard_normal((B, D, H, W))
+    x[..., 1] = rng.uniform(size=(B, D, H, W)) > 0.15
+    t = rng.standard_normal((B, D, H, W)).astype(np.float32)
+    return x, t
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_forward_matches_oracle():
+    rng = np.random.default_rng(0)
+    p = po.init_params(CH, seed=1).astype(np.float32)
+    x, _ = _batch(rng, 2, 8, 16, 24)
+    with PNet(CH, max_batch=2, max_cells=8 * 16 * 24) as net:
+        net.set_params(p)
+        assert np.array_equal(net.get_params(), p)
+        y = net.forward(x)
+    yo, _ = po.forward(p.astype(np.float64), CH, x.astype(np.float64), bf16=True)
+    err = _rel(y, yo)
+    print("forward rel err", err, flush=True)
+    assert err < 5e-3
+
+
+def test_gradients_match_oracle():
+    rng = np.random.default_rng(2)
+    p = po.init_params(CH, seed=3).astype(np.float32)
+    x, t = _batch(rng, 2, 8, 8, 16)
+    with PNet(CH, max_batch=2, max_cells=8 * 8 * 16) as net:
+        net.set_params(p)
+        loss, g = net.loss_grad(x, t)
+    lo, go = po.loss_grad(p.astype(np.float64), CH, x.astype(np.float64), t.astype(np.float64), bf16=True)
+    print("loss", loss, lo, flush=True)
+    assert loss == pytest.approx(lo, rel=1e-3)
+    layers, off_out, n = po.layout(CH)
+    for l, L in enumerate(layers):
+        ci = L["ca"] + L["cb"]
+        sw = slice(L["ow"], L["ow"] + 27 * L["co"] * ci)
+        sb = slice(L["ob"], L["ob"] + L["co"])
+        ew, eb = _rel(g[sw], go[sw]), _rel(g[sb], go[sb])
+        print(f"layer {l}: weight grad rel err {ew:.2e}, bias {eb:.2e}", flush=True)
+        assert ew < 1e-2 and eb < 1e-2, l
+    so = slice(off_out, n)
+    assert _rel(g[so], go[so]) < 1e-2
+
+
+def test_training_reduces_loss():
+    """Learn a fixed smoothing of the input (a 7-point average applied
+    twice): the mean loss of the last epoch is below a quarter of the
+    first's."""
+    rng = np.random.default_rng(4)
+    N, D, H, W = 16, 8, 16, 16
+    x = np.zeros((N, D, H, W, 2), dtype=np.float32)
+    x[..., 0] = rng.standard_normal((N, D, H, W))
+    x[..., 1] = 1.0
+    s = x[..., 0].astype(np.float64)
+    for _ in range(2):
+        sp = np.pad(s, ((0, 0), (1, 1), (1, 1), (1, 1)))
+        s = (sp[:, 1:-1, 1:-1, 1:-1] + sp[:, :-2, 1:-1, 1:-1] + sp[:, 2:, 1:-1, 1:-1] + sp[:, 1:-1, :-2, 1:-1]
+             + sp[:, 1:-1, 2:, 1:-1] + sp[:, 1:-1, 1:-1, :-2] + sp[:, 1:-1, 1:-1, 2:]) / 7.0
+    t = (s / s.std()).astype(np.float32)
+    with PNet(CH, max_batch=4, max_cells=D * H * W, seed=5) as net:
+        hist = net.train(x, t, epochs=12, batch=4, lr=2e-3, seed=6)
+    print("epoch losses", hist, flush=True)
+    assert np.all(np.isfinite(hist))
+    assert hist[-1] < 0.25 * hist[0]
