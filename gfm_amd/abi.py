@@ -39,6 +39,9 @@ class GfmCtrl(ct.Structure):
         ("gfex", D), ("gfin", D), ("fuel_out", D), ("tmean_ex", D), ("q_react", D), ("q_exhaust", D), ("q_wall", D),
         ("tave", D), ("tave_vol", D), ("tmax_c", D),
         ("t_av_exit", D), ("t_x_exit", D),
+        ("pcorr_mode", I32), ("pcorr_guess", I32), ("pcorr_max_sweeps", I32), ("pcorr_dump", I32),
+        ("pcorr_sweeps", I32), ("pcorr_fallback", I32),
+        ("pcorr_tol", D), ("pcorr_resid_start", D), ("pcorr_alpha", D),
     ]
 
     def copy(self):

@@ -34,7 +34,8 @@ static void set_err(const char *fmt, ...) {
 #include "gfm_pnet.hip"
 
 // residual slots in the device scalar block
-enum { RS_P_PRE = 0, RS_P = 1, RS_U_PRE = 2, RS_U = 5, RS_PC = 8, RS_GF_PRE = 10, RS_GF = 20, RS_IRC = 30 };
+enum { RS_P_PRE = 0, RS_P = 1, RS_U_PRE = 2, RS_U = 5, RS_PC = 8, RS_PC0 = 9, RS_GF_PRE = 10, RS_GF = 20, RS_IRC = 30,
+       RS_PCG = 60 };
 
 struct ProfRec {
   std::string name;
@@ -66,6 +67,14 @@ struct gfm_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int graph_key = -1;
+  // p' solved to a tolerance: the iteration runs as graph A (up to the
+  // initial guess), the relaxation loop, graph B (the rest)
+  hipGraph_t graph_b = nullptr;
+  hipGraphExec_t graph_exec_b = nullptr;
+  gfm_pnet *pnet = nullptr;     // learned initial guess (not owned)
+  double *pin = nullptr;        // pinned host word for residual checks
+  float *dump_x = nullptr, *dump_p = nullptr;  // training pair of the last p' solve
+  int pair_ready = 0;
 };
 static gfm_ctx *g_bound = nullptr;
 
@@ -276,6 +285,8 @@ static void pull_scalars(gfm_ctx *c, gfm_ctrl *k, const double *sc) {
   k->p_resid = r[RS_P];
   for (int q = 0; q < 3; q++) k->u_resid[q] = r[RS_U + q];
   k->pcorr_resid = r[RS_PC];
+  k->pcorr_alpha = sc[SC_PG_ALPHA];
+  k->pcorr_fallback = (int)sc[SC_PG_FALLBACK];
   for (int q = 0; q < 7; q++) k->gf_resid[q] = r[RS_GF + q];
   k->h_resid = r[RS_GF + 2];
   if (k->want_irc)
@@ -359,8 +370,8 @@ static void residual(gfm_ctx *c, const double *xt, int ib, int ie, int jb, int j
   LAUNCH(k_residual, b, xt, c->part, irc);
   hipLaunchKernelGGL(k_residual_finalize, dim3(1), dim3(256), 0, c->stream, c->part, (int)box_blocks(b), SC_RESID0 + slot, irc);
 }
-static void adlbl3(gfm_ctx *c, double *xt, int nelg, int ib, int ie, int jb, int je, int kb, int ke) {
-  const int nim = c->ctrl.ntimesg[nelg - 1];
+// nim sweeps of ADLBL3 on xt over the box (FP32 working copy in mixed precision)
+static void adlbl3_sweeps(gfm_ctx *c, double *xt, int nim, int ib, int ie, int jb, int je, int kb, int ke) {
   if (!c->dc.mixed) {
     sweeps<double>(c, xt, c->xt2, c->xt3, nim, ib, ie, jb, je, kb, ke);
   } else {
@@ -374,20 +385,89 @@ static void adlbl3(gfm_ctx *c, double *xt, int nelg, int ib, int ie, int jb, int
                   ke);
     LAUNCH(k_box_widen, box, c->xtf, xt);
   }
+}
+static void adlbl3(gfm_ctx *c, double *xt, int nelg, int ib, int ie, int jb, int je, int kb, int ke) {
+  adlbl3_sweeps(c, xt, c->ctrl.ntimesg[nelg - 1], ib, ie, jb, je, kb, ke);
   if (c->ctrl.want_irc) residual(c, xt, ib, ie, jb, je, kb, ke, RS_IRC + 2 * (nelg - 1), 1);
+}
+
+// Initial guess of a p' solve relaxed to a tolerance: calc_residual of
+// FZ = 0, then (learned guess) the network's prediction from BS/AP,
+// scaled by the alpha minimising ||BS - alpha*A g||, kept only if its
+// calc_residual is below that of zero.  All on the device (capturable).
+static int pcorr_guess(gfm_ctx *c, const Box &cen) {
+  DevCase &d = c->dc;
+  const gfm_ctrl &k = c->ctrl;
+  const int MPh = k.mp, NPh = k.np, LPh = k.lp;
+  residual(c, d.fz, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_PC0, 0);
+  if (k.pcorr_guess != GFM_PGUESS_LEARNED || !c->pnet) return GFM_OK;
+  gfm_pnet *n = c->pnet;
+  const int nx = cen.ni, ny = cen.nj, nz = cen.nk;
+  const long long V = (long long)nx * ny * nz;
+  const long long nb = box_blocks(cen);
+  LAUNCH(k_pg_gather, cen, n->xin, c->part);
+  hipLaunchKernelGGL(k_pg_rms, dim3(1), dim3(256), 0, c->stream, c->part, (int)nb);
+  hipLaunchKernelGGL(k_pg_norm, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, c->stream, n->xin, V);
+  if (c->prof.on) prof_begin(c);
+  int rc = pnet_forward_dev(n, c->stream, 1, nz, ny, nx);
+  if (rc) return rc;
+  if (c->prof.on) prof_end(c, "pnet_forward", 0.0);
+  LAUNCH(k_pg_scatter, cen, n->y);
+  LAUNCH(k_pg_ag, cen, c->part);
+  hipLaunchKernelGGL(k_pg_alpha, dim3(1), dim3(256), 0, c->stream, c->part, (int)nb);
+  LAUNCH(k_pg_scale, cen);
+  residual(c, d.fz, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_PCG, 0);
+  LAUNCH(k_pg_fallback, cen, RS_PCG, RS_PC0);
+  return GFM_OK;
+}
+
+// Relax p' one sweep at a time until calc_residual <= pcorr_tol (host
+// loop: one residual read-back per sweep).
+static int pcorr_relax(gfm_ctx *c) {
+  DevCase &d = c->dc;
+  gfm_ctrl &k = c->ctrl;
+  const int MPh = k.mp, NPh = k.np, LPh = k.lp;
+  auto resid = [&](double *r) {
+    residual(c, d.fz, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_PC, 0);
+    HCHECK(hipMemcpyAsync(c->pin, d.scal + SC_RESID0 + RS_PC, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HCHECK(hipStreamSynchronize(c->stream));
+    *r = *c->pin;
+    return GFM_OK;
+  };
+  double r;
+  int rc = resid(&r);
+  if (rc) return rc;
+  k.pcorr_resid_start = r;
+  int s = 0;
+  while (s < k.pcorr_max_sweeps && !(r <= k.pcorr_tol)) {
+    adlbl3_sweeps(c, d.fz, 1, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2);
+    s++;
+    if ((rc = resid(&r))) return rc;
+  }
+  k.pcorr_sweeps = s;
+  if (k.want_irc) residual(c, d.fz, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_IRC + 2 * (5 - 1), 1);
+  return GFM_OK;
 }
 
 // ---------------------------------------------------------------------
 // GFLOW, C/Gflow.f90:6-408
 // ---------------------------------------------------------------------
-static int gflow(gfm_ctx *c) {
+// stage 0: the whole iteration; 1: up to the initial guess of a p' solve
+// relaxed to a tolerance; 2: from the end of that solve on.
+static int gflow(gfm_ctx *c, int stage) {
   const gfm_ctrl &k = c->ctrl;
   DevCase &d = c->dc;
   const int MPh = k.mp, NPh = k.np, LPh = k.lp;
   Box all1 = make_box(2, MPh, 2, NPh, 2, LPh, 1);
   Box cen = make_box(4, MPh - 2, 4, NPh - 2, 4, LPh - 2, 2);
   Box cen_all = make_box(2, MPh, 2, NPh, 2, LPh, 2);
+  Box mom[4];
+  for (int iu1 = 1; iu1 <= 3; iu1++) {
+    int nx0 = iu1 == 1;
+    mom[iu1] = make_box(4 + nx0, MPh - 2 - nx0, d.njy1v[iu1], d.njy2v[iu1], d.lkz1v[iu1], d.lkz2v[iu1], 2);
+  }
 
+  if (stage != 2) {
   LAUNCH(k_theta_wall, all1);
   LAUNCH(k_gama_wall_zero, all1);
   double rt20 = k.reyg2 * sqrt(k.epsr) * pow(CMU, 0.25);
@@ -398,11 +478,6 @@ static int gflow(gfm_ctx *c) {
     int nx0 = iu1 == 1, nx1 = iu1 == 2, nx2 = iu1 == 3;
     Box fb = make_box(2 + nx0, MPh - nx0, 2 + nx1, NPh - nx1, 2 + nx2, LPh - nx2, 2);
     LAUNCH(k_copy_ug_fz, fb, iu1);
-  }
-  Box mom[4];
-  for (int iu1 = 1; iu1 <= 3; iu1++) {
-    int nx0 = iu1 == 1;
-    mom[iu1] = make_box(4 + nx0, MPh - 2 - nx0, d.njy1v[iu1], d.njy2v[iu1], d.lkz1v[iu1], d.lkz2v[iu1], 2);
   }
   for (int iu1 = 1; iu1 <= 3; iu1++) LAUNCH(k_momentum_assemble, mom[iu1], iu1);
   LAUNCH(k_pressure_assemble, cen);
@@ -420,11 +495,25 @@ static int gflow(gfm_ctx *c) {
     LAUNCH(k_bound_store_ug, b, iu1);
   }
   // p'
-  LAUNCH(k_pcorr_source, cen, c->part);
-  hipLaunchKernelGGL(k_pcorr_finalize, dim3(1), dim3(256), 0, c->stream, c->part, box_blocks(cen));
-  LAUNCH(k_zero_fz, cen_all);
-  adlbl3(c, d.fz, 5, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2);
-  residual(c, d.fz, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_PC, 0);
+    LAUNCH(k_pcorr_source, cen, c->part);
+    hipLaunchKernelGGL(k_pcorr_finalize, dim3(1), dim3(256), 0, c->stream, c->part, box_blocks(cen));
+    LAUNCH(k_zero_fz, cen_all);
+    if (k.pcorr_mode == GFM_PCORR_REFERENCE) {
+      adlbl3(c, d.fz, 5, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2);
+      residual(c, d.fz, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_PC, 0);
+    } else {
+      int rc = pcorr_guess(c, cen);
+      if (rc) return rc;
+    }
+  }
+  if (stage == 1) {
+    HCHECK(hipGetLastError());
+    return GFM_OK;
+  }
+  if (k.pcorr_dump) {
+    LAUNCH(k_pg_gather, cen, c->dump_x, c->part);
+    LAUNCH(k_pg_dump_p, cen, c->dump_p);
+  }
   for (int iu1 = 1; iu1 <= 3; iu1++) LAUNCH(k_vel_correct, mom[iu1], iu1);
   // EXTRV(0), FLX1D, factor, EXTRV(1)
   Box ex[3] = {make_box(3, MPh - 1, 2, NPh, 2, LPh, 2), make_box(2, MPh, 3, NPh - 1, 2, LPh, 2), make_box(2, MPh, 2, NPh, 3, LPh - 1, 2)};
@@ -616,6 +705,12 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
     if ((rc = dalloc(c, &xf, N))) return fail(rc);
     c->xtf = xf;
   }
+  HCHECK(hipHostMalloc((void **)&c->pin, 8 * sizeof(double), hipHostMallocDefault));
+  {
+    const long long cells = (long long)((k->mp - 6) / 2 + 1) * ((k->np - 6) / 2 + 1) * ((k->lp - 6) / 2 + 1);
+    if ((rc = dalloc(c, &c->dump_x, 2 * cells))) return fail(rc);
+    if ((rc = dalloc(c, &c->dump_p, cells))) return fail(rc);
+  }
   if ((rc = dalloc(c, &c->xt2, N))) return fail(rc);
   if ((rc = dalloc(c, &c->xt3, N))) return fail(rc);
   if ((rc = dalloc(c, &c->stage, N))) return fail(rc);
@@ -710,31 +805,52 @@ int gfm_hip_initsv_tail(gfm_ctx *c, gfm_ctrl *k) {
 static void drop_graph(gfm_ctx *c) {
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
+  if (c->graph_exec_b) (void)hipGraphExecDestroy(c->graph_exec_b);
+  if (c->graph_b) (void)hipGraphDestroy(c->graph_b);
   c->graph_exec = nullptr;
   c->graph = nullptr;
+  c->graph_exec_b = nullptr;
+  c->graph_b = nullptr;
   c->graph_key = -1;
 }
 
 // Launch one GFLOW.  The launch sequence depends only on the options in
 // the key, so it is captured once into a graph and replayed; itr_gas
 // reaches the kernels through the device scalar block.
+static int capture(gfm_ctx *c, int stage, hipGraph_t *gr, hipGraphExec_t *ex) {
+  HCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = gflow(c, stage);
+  hipGraph_t g0 = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g0);
+  if (rc) { if (g0) (void)hipGraphDestroy(g0); return rc; }
+  if (e != hipSuccess) { set_err("graph capture failed: %s", hipGetErrorString(e)); return GFM_E_DEVICE; }
+  *gr = g0;
+  HCHECK(hipGraphInstantiateWithFlags(ex, g0, 0));
+  return GFM_OK;
+}
 static int run_gflow(gfm_ctx *c) {
   hipLaunchKernelGGL(k_set_itr, dim3(1), dim3(1), 0, c->stream, c->ctrl.itr_gas);
-  if (!c->use_graph || c->prof.on) return gflow(c);
-  int key = c->ctrl.sweep_order | (c->ctrl.want_pre_residuals << 2) | (c->ctrl.want_irc << 3);
+  const gfm_ctrl &k = c->ctrl;
+  const bool split = k.pcorr_mode == GFM_PCORR_TOLERANCE;
+  int rc;
+  if (!c->use_graph || c->prof.on) {
+    if (!split) return gflow(c, 0);
+    if ((rc = gflow(c, 1))) return rc;
+    if ((rc = pcorr_relax(c))) return rc;
+    return gflow(c, 2);
+  }
+  const int key = k.sweep_order | (k.want_pre_residuals << 2) | (k.want_irc << 3) | ((int)split << 4) |
+                  ((k.pcorr_guess == GFM_PGUESS_LEARNED && c->pnet) << 5) | ((k.pcorr_dump != 0) << 6);
   if (!c->graph_exec || key != c->graph_key) {
     drop_graph(c);
-    HCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int rc = gflow(c);
-    hipGraph_t gr = nullptr;
-    hipError_t e = hipStreamEndCapture(c->stream, &gr);
-    if (rc) { if (gr) (void)hipGraphDestroy(gr); return rc; }
-    if (e != hipSuccess) { set_err("graph capture failed: %s", hipGetErrorString(e)); return GFM_E_DEVICE; }
-    c->graph = gr;
-    HCHECK(hipGraphInstantiateWithFlags(&c->graph_exec, c->graph, 0));
+    if ((rc = capture(c, split ? 1 : 0, &c->graph, &c->graph_exec))) return rc;
+    if (split && (rc = capture(c, 2, &c->graph_b, &c->graph_exec_b))) return rc;
     c->graph_key = key;
   }
   HCHECK(hipGraphLaunch(c->graph_exec, c->stream));
+  if (!split) return GFM_OK;
+  if ((rc = pcorr_relax(c))) return rc;
+  HCHECK(hipGraphLaunch(c->graph_exec_b, c->stream));
   return GFM_OK;
 }
 
@@ -746,12 +862,35 @@ int gfm_hip_gflow(gfm_ctx *c, gfm_ctrl *k) {
   c->ctrl.sweep_order = k->sweep_order;
   c->ctrl.want_pre_residuals = k->want_pre_residuals;
   c->ctrl.want_irc = k->want_irc;
+  if (k->pcorr_mode != GFM_PCORR_REFERENCE && k->pcorr_mode != GFM_PCORR_TOLERANCE) { set_err("bad pcorr_mode"); return GFM_E_ARG; }
+  if (k->pcorr_mode == GFM_PCORR_TOLERANCE && (k->pcorr_max_sweeps < 0 || !(k->pcorr_tol >= 0.0))) {
+    set_err("pcorr_max_sweeps must be >= 0 and pcorr_tol >= 0");
+    return GFM_E_ARG;
+  }
+  c->ctrl.pcorr_mode = k->pcorr_mode;
+  c->ctrl.pcorr_guess = k->pcorr_guess;
+  c->ctrl.pcorr_max_sweeps = k->pcorr_max_sweeps;
+  c->ctrl.pcorr_tol = k->pcorr_tol;
+  c->ctrl.pcorr_dump = k->pcorr_dump;
   if ((rc = run_gflow(c))) return rc;
   double sc[SC_COUNT];
   HCHECK(hipMemcpyAsync(sc, c->dc.scal, sizeof sc, hipMemcpyDeviceToHost, c->stream));
   HCHECK(hipStreamSynchronize(c->stream));
   pull_scalars(c, &c->ctrl, sc);
   pull_scalars(c, k, sc);
+  if (c->ctrl.pcorr_mode == GFM_PCORR_REFERENCE) {
+    c->ctrl.pcorr_sweeps = c->ctrl.ntimesg[4];
+    c->ctrl.pcorr_resid_start = 0.0;
+  }
+  if (c->ctrl.pcorr_mode == GFM_PCORR_REFERENCE || c->ctrl.pcorr_guess != GFM_PGUESS_LEARNED || !c->pnet) {
+    c->ctrl.pcorr_alpha = 0.0;
+    c->ctrl.pcorr_fallback = 0;
+  }
+  k->pcorr_sweeps = c->ctrl.pcorr_sweeps;
+  k->pcorr_resid_start = c->ctrl.pcorr_resid_start;
+  k->pcorr_alpha = c->ctrl.pcorr_alpha;
+  k->pcorr_fallback = c->ctrl.pcorr_fallback;
+  if (c->ctrl.pcorr_dump) c->pair_ready = 1;
   if (c->prof.on) prof_flush(c);
   return GFM_OK;
 }
@@ -829,6 +968,31 @@ int gfm_hip_calc_residual(gfm_ctx *c, const double *ap, const double *as, const 
   return GFM_OK;
 }
 
+int gfm_hip_set_pguess(gfm_ctx *c, gfm_pnet *net) {
+  if (!c) { set_err("null context"); return GFM_E_ARG; }
+  if (net) {
+    const int nx = (c->ctrl.mp - 6) / 2 + 1, ny = (c->ctrl.np - 6) / 2 + 1, nz = (c->ctrl.lp - 6) / 2 + 1;
+    int rc = pnet_check_grid(net, 1, nz, ny, nx);
+    if (rc) return rc;
+    if (net->device != c->device) { set_err("the network lives on device %d, the case on %d", net->device, c->device); return GFM_E_ARG; }
+  }
+  c->pnet = net;
+  drop_graph(c);
+  return GFM_OK;
+}
+
+int gfm_hip_pcorr_pair(gfm_ctx *c, float *x, float *p) {
+  if (!c || !x || !p) { set_err("null argument"); return GFM_E_ARG; }
+  if (!c->pair_ready) { set_err("no p' solve has run with pcorr_dump set"); return GFM_E_STATE; }
+  int rc = bind(c);
+  if (rc) return rc;
+  const long long cells = (long long)((c->ctrl.mp - 6) / 2 + 1) * ((c->ctrl.np - 6) / 2 + 1) * ((c->ctrl.lp - 6) / 2 + 1);
+  HCHECK(hipMemcpyAsync(x, c->dump_x, 2 * cells * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HCHECK(hipMemcpyAsync(p, c->dump_p, cells * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HCHECK(hipStreamSynchronize(c->stream));
+  return GFM_OK;
+}
+
 int gfm_hip_kernel_times(gfm_ctx *c, int enable, char *names, int name_len, double *ms, double *bytes, int *launches, int cap,
                          int *n) {
   if (!c) { set_err("null context"); return GFM_E_ARG; }
@@ -862,6 +1026,7 @@ int gfm_hip_finalize(gfm_ctx *c) {
   for (void *p : c->allocs) (void)hipFree(p);
   for (auto &e : c->prof.ev) (void)hipEventDestroy(e);
   drop_graph(c);
+  if (c->pin) (void)hipHostFree(c->pin);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   if (g_bound == c) g_bound = nullptr;
   delete c;

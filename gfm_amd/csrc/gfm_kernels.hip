@@ -1281,6 +1281,115 @@ __global__ void k_residual_finalize(const double *part, int nb, int slot, int ir
 }
 
 // ======================================================================
+// Learned initial guess of the p' solve.  No counterpart in the reference,
+// whose guess is FZ = 0 (C/Gslv3v.f90:157-161).  Cell v of the p' box
+// [4, MP-2]x[4, NP-2]x[4, LP-2] (step 2) is v = (kk*ny + jj)*nx + ii with
+// ii = (i-4)/2, jj = (j-4)/2, kk = (k-4)/2; a row is open when IBCELL <= 0
+// and |AP| > 1e-20 (the rows the line solver does not pin).
+// ======================================================================
+__device__ __forceinline__ long long pg_cell(int i, int j, int k) {
+  const int nx = (MP - 6) / 2 + 1, ny = (NP - 6) / 2 + 1;
+  return ((long long)((k - 4) >> 1) * ny + ((j - 4) >> 1)) * nx + ((i - 4) >> 1);
+}
+__device__ __forceinline__ bool pg_open(int i, int j, int k, long long q) { return IBC(i, j, k) <= 0 && fabs(CAP(q)) > SMALL20; }
+// network input x[2v] = BS/AP, x[2v+1] = open; partial sums of (BS/AP)^2
+__global__ void k_pg_gather(Box b, float *x, double *part) {
+  int i, j, k;
+  double s2 = 0.0;
+  if (box_pt(b, i, j, k)) {
+    const long long q = lidx(i, j, k), v = pg_cell(i, j, k);
+    const bool op = pg_open(i, j, k, q);
+    const double s = op ? CBS(q) / CAP(q) : 0.0;
+    x[2 * v] = (float)s;
+    x[2 * v + 1] = op ? 1.f : 0.f;
+    s2 = s * s;
+  }
+  const double w = wave_sum(s2);
+  if (threadIdx.x == 0) part[block_lin()] = w;
+}
+// scale of the input: root mean square of BS/AP over open rows
+__global__ void k_pg_rms(const double *part, int nb) {
+  __shared__ double ss[256];
+  double s = 0.0;
+  for (int q = threadIdx.x; q < nb; q += 256) s += part[q];
+  ss[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) ss[threadIdx.x] += ss[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double r = sqrt(ss[0] / fmax((double)g.ncells, 1.0));
+    g.scal[SC_PG_RMS] = r > 0.0 ? r : 1.0;
+    g.scal[SC_PG_ALPHA] = 0.0;
+    g.scal[SC_PG_FALLBACK] = 0.0;
+  }
+}
+__global__ void k_pg_norm(float *x, long long V) {
+  const long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (v < V) x[2 * v] = (float)((double)x[2 * v] / g.scal[SC_PG_RMS]);
+}
+// FZ = network output on open rows (FZ is zero elsewhere)
+__global__ void k_pg_scatter(Box b, const float *y) {
+  int i, j, k;
+  if (!box_pt(b, i, j, k)) return;
+  const long long q = lidx(i, j, k);
+  if (pg_open(i, j, k, q)) g.fz[q] = (double)y[pg_cell(i, j, k)];
+}
+// partial sums of BS.(A g) and (A g).(A g), (A g) = AP g - sum AS g_nb, FP64
+__global__ void k_pg_ag(Box b, double *part) {
+  int i, j, k;
+  double bag = 0.0, agag = 0.0;
+  if (box_pt(b, i, j, k)) {
+    const long long q = lidx(i, j, k);
+    if (pg_open(i, j, k, q)) {
+      const double ag = CAP(q) * FZ(i, j, k) - (CAS(0, q) * FZ(i - 2, j, k) + CAS(3, q) * FZ(i + 2, j, k) +
+                                                CAS(1, q) * FZ(i, j - 2, k) + CAS(4, q) * FZ(i, j + 2, k) +
+                                                CAS(2, q) * FZ(i, j, k - 2) + CAS(5, q) * FZ(i, j, k + 2));
+      bag = CBS(q) * ag;
+      agag = ag * ag;
+    }
+  }
+  const double w1 = wave_sum(bag), w2 = wave_sum(agag);
+  if (threadIdx.x == 0) {
+    part[block_lin()] = w1;
+    part[g.pstride + block_lin()] = w2;
+  }
+}
+__global__ void k_pg_alpha(const double *part, int nb) {
+  __shared__ double s1[256], s2[256];
+  double a = 0.0, c = 0.0;
+  for (int q = threadIdx.x; q < nb; q += 256) { a += part[q]; c += part[g.pstride + q]; }
+  s1[threadIdx.x] = a; s2[threadIdx.x] = c;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) { s1[threadIdx.x] += s1[threadIdx.x + o]; s2[threadIdx.x] += s2[threadIdx.x + o]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) g.scal[SC_PG_ALPHA] = s2[0] > 0.0 ? s1[0] / s2[0] : 0.0;
+}
+__global__ void k_pg_scale(Box b) {
+  int i, j, k;
+  if (!box_pt(b, i, j, k)) return;
+  const long long q = lidx(i, j, k);
+  g.fz[q] *= g.scal[SC_PG_ALPHA];
+}
+// keep the guess only if it lowers calc_residual below that of FZ = 0
+__global__ void k_pg_fallback(Box b, int slot_guess, int slot_zero) {
+  int i, j, k;
+  const bool worse = !(g.scal[SC_RESID0 + slot_guess] < g.scal[SC_RESID0 + slot_zero]);
+  if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0) g.scal[SC_PG_FALLBACK] = worse ? 1.0 : 0.0;
+  if (!worse || !box_pt(b, i, j, k)) return;
+  g.fz[lidx(i, j, k)] = 0.0;
+}
+// training pair of the last solve: p' on every cell of the p' box
+__global__ void k_pg_dump_p(Box b, float *p) {
+  int i, j, k;
+  if (!box_pt(b, i, j, k)) return;
+  p[pg_cell(i, j, k)] = (float)FZ(i, j, k);
+}
+
+// ======================================================================
 // ADLBL3 line relaxation, C/Adlbl3.f90:10-221.  Each y-line (along J) or
 // z-line (along K) of an x-plane is a tridiagonal system solved by
 // parallel cyclic reduction.  Rows that the reference's TDMA pins (blocked

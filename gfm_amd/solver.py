@@ -45,6 +45,8 @@ def lib():
         L.gfm_hip_adlbl3.argtypes = [P, ct.POINTER(GfmCtrl), ct.c_int, D, D, D, D] + [ct.c_int] * 6 + [D]
         L.gfm_hip_calc_residual.argtypes = [P, D, D, D, D] + [ct.c_int] * 6 + [D]
         L.gfm_hip_finalize.argtypes = [P]
+        L.gfm_hip_set_pguess.argtypes = [P, P]
+        L.gfm_hip_pcorr_pair.argtypes = [P, ct.POINTER(ct.c_float), ct.POINTER(ct.c_float)]
         L.gfm_hip_kernel_times.argtypes = [P, ct.c_int, ct.c_char_p, ct.c_int, D, D, ct.POINTER(ct.c_int), ct.c_int,
                                            ct.POINTER(ct.c_int)]
         _lib = L
@@ -104,6 +106,28 @@ class CombustionSpace:
         _check(lib().gfm_hip_calc_residual(self._ctx, _dp(ap), _dp(as_), _dp(bs), _dp(xt), *map(int, box), _dp(r)),
                "gfm_hip_calc_residual")
         return float(r[0])
+
+    def set_pguess(self, net):
+        """Attach a gfm_amd.pnet.PNet as the learned p' initial guess (None
+        detaches); used when ctrl.pcorr_mode = 1 and ctrl.pcorr_guess = 1."""
+        self._pnet = net
+        _check(lib().gfm_hip_set_pguess(self._ctx, net.handle if net is not None else None), "gfm_hip_set_pguess")
+
+    @property
+    def cells(self):
+        c = self.case.ctrl
+        return (c.lp - 6) // 2 + 1, (c.np - 6) // 2 + 1, (c.mp - 6) // 2 + 1
+
+    def pcorr_pair(self):
+        """(x [nz,ny,nx,2], p [nz,ny,nx]) of the last p' solve run with
+        ctrl.pcorr_dump = 1: x[...,0] = BS/AP, x[...,1] = open-row mask,
+        p = p' after the solve."""
+        nz, ny, nx = self.cells
+        x = np.zeros((nz, ny, nx, 2), dtype=np.float32)
+        p = np.zeros((nz, ny, nx), dtype=np.float32)
+        _check(lib().gfm_hip_pcorr_pair(self._ctx, x.ctypes.data_as(ct.POINTER(ct.c_float)),
+                                        p.ctypes.data_as(ct.POINTER(ct.c_float))), "gfm_hip_pcorr_pair")
+        return x, p
 
     def kernel_times(self, enable=-1, cap=128):
         """Per-kernel HIP-event timings: enable=1 resets and starts, 0 stops,

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define GFM_ABI_VERSION 1
+#define GFM_ABI_VERSION 2
 
 enum {
   GFM_OK = 0,
@@ -59,6 +59,20 @@ enum { GFM_ORDER_REFERENCE = 0, GFM_ORDER_ZEBRA = 1 };
  * balances/reductions and the enthalpy->temperature inversion are always
  * FP64. */
 enum { GFM_PREC_FP64 = 0, GFM_PREC_MIXED = 1 };
+
+/* p' solve (C/Gslv3v.f90:149-164).
+ *   GFM_PCORR_REFERENCE: zero initial guess, NTIMESG(5) sweeps (the reference).
+ *   GFM_PCORR_TOLERANCE: start from the initial guess of pcorr_guess, then
+ *     relax one sweep at a time until calc_residual <= pcorr_tol or
+ *     pcorr_max_sweeps sweeps.
+ * Initial guess in GFM_PCORR_TOLERANCE:
+ *   GFM_PGUESS_ZERO: FZ = 0.
+ *   GFM_PGUESS_LEARNED: the network attached by gfm_hip_set_pguess maps the
+ *     Jacobi-scaled mass source BS/AP to a p' field g; the guess is alpha*g
+ *     with alpha minimising ||BS - alpha*A g||_2 (FP64), and FZ = 0 is kept
+ *     instead when calc_residual(alpha*g) >= calc_residual(0). */
+enum { GFM_PCORR_REFERENCE = 0, GFM_PCORR_TOLERANCE = 1 };
+enum { GFM_PGUESS_ZERO = 0, GFM_PGUESS_LEARNED = 1 };
 
 /* Host arrays of one case, reference layout (see top of file).  Arrays of
  * rank 3 have MP*NP*LP entries; UG holds 3 and GF holds 9 consecutive
@@ -117,6 +131,14 @@ typedef struct gfm_ctrl {
   double gfex, gfin, fuel_out, tmean_ex, q_react, q_exhaust, q_wall;
   double tave, tave_vol, tmax_c;
   double t_av_exit, t_x_exit;     /* C/Sphase.f90:74-89 */
+  /* p' solve options (GFM_PCORR_*, GFM_PGUESS_*) and outputs */
+  int32_t pcorr_mode, pcorr_guess, pcorr_max_sweeps;
+  int32_t pcorr_dump;             /* keep (BS/AP, mask, p') of the last solve for gfm_hip_pcorr_pair */
+  int32_t pcorr_sweeps;           /* out: sweeps of the last p' solve */
+  int32_t pcorr_fallback;         /* out: 1 if the learned guess was replaced by zero */
+  double pcorr_tol;
+  double pcorr_resid_start;       /* out: calc_residual of the initial guess */
+  double pcorr_alpha;             /* out: scale of the learned guess */
 } gfm_ctrl;
 
 /* Opaque device-side context. */
@@ -166,6 +188,21 @@ int gfm_hip_adlbl3(gfm_ctx *ctx, const gfm_ctrl *ctrl, int nelg,
 int gfm_hip_calc_residual(gfm_ctx *ctx, const double *ap, const double *as, const double *bs,
                           const double *xt, int ib, int ie, int jb, int je, int kb, int ke,
                           double *resid);
+
+/* Attach the network of include/gfm_pnet.h for GFM_PGUESS_LEARNED (NULL
+ * detaches).  The case's cell grid (nx, ny, nz of the p' box) must be
+ * multiples of 8 and fit the network's buffers; the network must live on
+ * the context's device and is not owned by the context.  No counterpart in
+ * the reference (its guess is zero, C/Gslv3v.f90:157-161). */
+struct gfm_pnet;
+int gfm_hip_set_pguess(gfm_ctx *ctx, struct gfm_pnet *net);
+
+/* The training pair of the last p' solve run with pcorr_dump set (the pair
+ * of row "p' solve": BS after SOURCE(5) and FZ after the solve): for each
+ * cell of the p' box, v = ((k-4)/2*ny + (j-4)/2)*nx + (i-4)/2,
+ * x[2v] = BS/AP (0 where the row is pinned), x[2v+1] = 1 for open rows else
+ * 0, p[v] = p'. */
+int gfm_hip_pcorr_pair(gfm_ctx *ctx, float *x, float *p);
 
 /* Per-kernel timing with HIP events recorded on the library's stream around
  * each launch.  enable=1 clears the records and starts recording, enable=0

@@ -1262,8 +1262,26 @@ static void GSLV3V(void) {
   for (int i = 2; i <= MP; i += 2)
     for (int j = 2; j <= NP; j += 2)
       for (int k = 2; k <= LP; k += 2) FZ(i, j, k) = 0.0;
-  ADLBL3(FZ_, mi1, mi3, nj1, nj3, lk1, lk3);
-  C->pcorr_resid = CALC_RESIDUAL(FZ_, mi1, mi3, nj1, nj3, lk1, lk3);
+  if (C->pcorr_mode == GFM_PCORR_TOLERANCE) {
+    /* zero initial guess relaxed one sweep at a time to pcorr_tol (the
+       GPU's GFM_PCORR_TOLERANCE with GFM_PGUESS_ZERO; not in the reference) */
+    const int save = C->ntimesg[NELG - 1];
+    C->ntimesg[NELG - 1] = 1;
+    double r = CALC_RESIDUAL(FZ_, mi1, mi3, nj1, nj3, lk1, lk3);
+    C->pcorr_resid_start = r;
+    int ns = 0;
+    while (ns < C->pcorr_max_sweeps && !(r <= C->pcorr_tol)) {
+      ADLBL3(FZ_, mi1, mi3, nj1, nj3, lk1, lk3);
+      ns++;
+      r = CALC_RESIDUAL(FZ_, mi1, mi3, nj1, nj3, lk1, lk3);
+    }
+    C->ntimesg[NELG - 1] = save;
+    C->pcorr_sweeps = ns;
+    C->pcorr_resid = r;
+  } else {
+    ADLBL3(FZ_, mi1, mi3, nj1, nj3, lk1, lk3);
+    C->pcorr_resid = CALC_RESIDUAL(FZ_, mi1, mi3, nj1, nj3, lk1, lk3);
+  }
 
   /* velocity correction */
   for (int iu1 = 1; iu1 <= 3; iu1++) {
