@@ -592,8 +592,18 @@ def stretched_faces(length, n, ratio):
     return np.concatenate([left, right[1:]])
 
 
+def air_velocity(u_air0, fuel_u, fuel_u0, air_t, air_t0, excess_air, excess_air0=0.10):
+    """Air-port velocity of an operating point: the nominal velocity scaled
+    with the fuel velocity, with (1 + excess air) and with the preheat
+    temperature (constant mass ratio at ideal-gas density).  The nominal
+    port sizes of the furnaces cannot pass a stoichiometric air flow at the
+    stated velocities, so excess air is applied relative to the nominal
+    point rather than to the stoichiometric ratio."""
+    return u_air0 * (fuel_u / fuel_u0) * ((1.0 + excess_air) / (1.0 + excess_air0)) * (air_t / air_t0)
+
+
 def furnace_small(nx=48, ny=24, nz=16, seed=1, noise=0.01, maxgi=300, bgcon=1.0e-12, walls="isothermal",
-                  gray_kappa=0.3):
+                  gray_kappa=0.3, fuel_u=30.0, air_t=1100.0, excess_air=0.10):
     """3.0 x 1.2 x 0.8 m chamber, uniform cells, one CH4 port (0.1x0.1 m,
     30 m/s, 300 K) centred on the x=0 wall inside an air port (0.3x0.2 m,
     10 m/s, 1100 K), full-face exit at x=3 m; uniform initial field
@@ -601,8 +611,10 @@ def furnace_small(nx=48, ny=24, nz=16, seed=1, noise=0.01, maxgi=300, bgcon=1.0e
     1500 K and a gray radiation source with kappa=0.3 1/m (walls="adiabatic",
     gray_kappa=0 give the reference's behaviour before its first radiation
     pass).  For grids other than 48x24x16 the ports keep their physical
-    size."""
+    size.  fuel_u, air_t, excess_air set an operating point (air velocity
+    from air_velocity, nominal 10 m/s)."""
     Lx, Ly, Lz = 3.0, 1.2, 0.8
+    u_air = air_velocity(10.0, fuel_u, 30.0, air_t, 1100.0, excess_air)
     dy, dz = Ly / ny, Lz / nz
 
     def span(centre, width, d, n):
@@ -614,9 +626,9 @@ def furnace_small(nx=48, ny=24, nz=16, seed=1, noise=0.01, maxgi=300, bgcon=1.0e
     fy = span(0.6, 0.1, dy, ny); fz = span(0.4, 0.1, dz, nz)
     ay = span(0.6, 0.3, dy, ny); az = span(0.4, 0.2, dz, nz)
     fuel = Inlet(box=(2, cell_I(fy[0]), cell_I(fz[0]), 2, cell_I(fy[1]), cell_I(fz[1])), iu=1,
-                 T=300.0, yf=1.0, yo2=0.0, yn2=0.0, uvw=(30.0, 0.0, 0.0))
+                 T=300.0, yf=1.0, yo2=0.0, yn2=0.0, uvw=(fuel_u, 0.0, 0.0))
     air = Inlet(box=(2, cell_I(ay[0]), cell_I(az[0]), 2, cell_I(ay[1]), cell_I(az[1])), iu=1,
-                T=1100.0, yf=0.0, yo2=0.233, yn2=0.767, uvw=(10.0, 0.0, 0.0))
+                T=air_t, yf=0.0, yo2=0.233, yn2=0.767, uvw=(u_air, 0.0, 0.0))
     spec = CaseSpec(name=f"furnace {nx}x{ny}x{nz}", faces_x=uniform_faces(Lx, nx), faces_y=uniform_faces(Ly, ny),
                     faces_z=uniform_faces(Lz, nz), inlets=[fuel, air], t_init=1500.0, k_init=1.0e-2,
                     eps_init=1.0e-2, noise=noise, seed=seed, maxgi=maxgi, bgcon=bgcon,
@@ -625,12 +637,13 @@ def furnace_small(nx=48, ny=24, nz=16, seed=1, noise=0.01, maxgi=300, bgcon=1.0e
 
 
 def furnace_large(nx=192, ny=96, nz=64, seed=2, noise=0.01, maxgi=2000, bgcon=1.0e-5, walls="isothermal",
-                  gray_kappa=0.3, exit_port=None):
+                  gray_kappa=0.3, exit_port=None, fuel_u=25.0, air_t=1200.0, excess_air=0.10):
     """7.0 x 3.0 x 1.6 m furnace, y-faces stretched toward the burner walls
     (ratio 1.05), two opposed CH4 burner pairs on the y walls at x=1.5 m and
     x=4.0 m (fuel 25 m/s 300 K, air 12 m/s 1200 K; the air port is sized
     for ~10 % excess air), glass bottom (IBCELL 4) at 1450 K +- 20 K
-    sinusoidal in x, full-face exit at x=7 m."""
+    sinusoidal in x, full-face exit at x=7 m.  fuel_u, air_t, excess_air set
+    an operating point (air velocity from air_velocity, nominal 12 m/s)."""
     Lx, Ly, Lz = 7.0, 3.0, 1.6
     fx = uniform_faces(Lx, nx)
     fy = stretched_faces(Ly, ny, 1.05)
@@ -638,6 +651,7 @@ def furnace_large(nx=192, ny=96, nz=64, seed=2, noise=0.01, maxgi=2000, bgcon=1.
     dxc, dzc = Lx / nx, Lz / nz
     rng = np.random.default_rng(seed)
     phase = rng.uniform(0.0, 2.0 * math.pi)
+    u_air = air_velocity(12.0, fuel_u, 25.0, air_t, 1200.0, excess_air)
     inlets = []
     # fuel 1x1 cell at the port centre, air port 9 (x) by 10 (z) cells
     for xb in (1.5, 4.0):
@@ -645,9 +659,9 @@ def furnace_large(nx=192, ny=96, nz=64, seed=2, noise=0.01, maxgi=2000, bgcon=1.
         ck = int(round(0.5 * Lz / dzc)) - 1
         for side, jw, v in ((+1, 2, 1.0), (-1, 2 * ny + 4, -1.0)):
             fuel = Inlet(box=(cell_I(ci), jw, cell_I(ck), cell_I(ci), jw, cell_I(ck)), iu=2 * side,
-                         T=300.0, yf=1.0, yo2=0.0, yn2=0.0, uvw=(0.0, 25.0 * v, 0.0))
+                         T=300.0, yf=1.0, yo2=0.0, yn2=0.0, uvw=(0.0, fuel_u * v, 0.0))
             air = Inlet(box=(cell_I(ci - 4), jw, cell_I(ck - 4), cell_I(ci + 4), jw, cell_I(ck + 5)), iu=2 * side,
-                        T=1200.0, yf=0.0, yo2=0.233, yn2=0.767, uvw=(0.0, 12.0 * v, 0.0))
+                        T=air_t, yf=0.0, yo2=0.233, yn2=0.767, uvw=(0.0, u_air * v, 0.0))
             inlets += [fuel, air]
     spec = CaseSpec(name=f"furnace {nx}x{ny}x{nz} stretched", faces_x=fx, faces_y=fy, faces_z=fzf, inlets=inlets,
                     t_init=1500.0, k_init=1.0e-2, eps_init=1.0e-2, noise=noise, seed=seed, maxgi=maxgi, bgcon=bgcon,

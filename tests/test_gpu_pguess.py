@@ -100,3 +100,21 @@ def test_pair_dump():
     assert not np.any(mask & (ib > 0))
     assert np.all(x[..., 0][~mask] == 0.0) and np.all(p[~mask] == 0.0)
     assert np.count_nonzero(p[mask]) > 0.9 * mask.sum()
+
+
+def test_trained_network_reduces_sweeps():
+    """The committed network (trained on 48x24x16 furnace pairs) on an
+    operating point outside its training set: to the same tolerance the
+    p' solve needs fewer sweeps on average than from zero."""
+    import os
+    from gfm_amd import pguess
+    path = os.path.join(os.path.dirname(pguess.__file__), "data", "pguess_furnace48x24x16.npz")
+    op = pguess.operating_points(1, 4242)[0]
+    net = pguess.load(path, max_cells=48 * 24 * 16)
+    try:
+        z = pguess.run_point(op, net, 0, (48, 24, 16), 12, 1e-3, 1000)
+        l = pguess.run_point(op, net, 1, (48, 24, 16), 12, 1e-3, 1000)
+    finally:
+        net.close()
+    print("sweeps zero", z["sweeps"], "learned", l["sweeps"], flush=True)
+    assert np.mean(l["sweeps"]) < 0.7 * np.mean(z["sweeps"])
