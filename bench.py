@@ -54,6 +54,77 @@ def cpu_baseline(spec_args, iters=2):
                       f"oracle/gfm_oracle.c (gcc -O2) on 1 host core, {dt:.1f} s"}
 
 
+PGUESS_WEIGHTS = os.path.join(ROOT, "gfm_amd", "data", "pguess_furnace48x24x16.npz")
+THRESHOLDS = (1e-3, 1e-4, 1e-5)
+
+
+def configure(case, mode, pcorr_tol):
+    """plain FP64: the reference algorithm (FP64, p' from zero, 5 sweeps).
+    learned+FP32: FP32 assembly and sweeps, p' from the network's guess
+    relaxed to the calc_residual the reference's 5 sweeps reach."""
+    k = case.ctrl
+    k.sweep_order = 1
+    k.want_pre_residuals = 0
+    if mode == "fp64":
+        k.precision, k.pcorr_mode = 0, 0
+    elif mode == "mixed":  # FP32 assembly and sweeps, the reference's p' solve
+        k.precision, k.pcorr_mode = 1, 0
+    else:
+        k.precision, k.pcorr_mode, k.pcorr_guess = 1, 1, 1
+        k.pcorr_tol, k.pcorr_max_sweeps = pcorr_tol, 100
+
+
+def throughput(spec_args, mode, net, steps, warmup, pcorr_tol, sync):
+    """W untimed iterations, then K timed ones (the library synchronises its
+    stream at the end of every iteration, when it reads the residuals)."""
+    from gfm_amd import case as cs
+    from gfm_amd.solver import CombustionSpace
+    case = cs.furnace_large(**spec_args)
+    configure(case, mode, pcorr_tol)
+    cspace = CombustionSpace(case, device=0)
+    if net is not None:
+        cspace.set_pguess(net)
+    for _ in range(warmup):
+        cspace.gflow()
+    sync()
+    t0 = time.perf_counter()
+    hist = cspace.sphase(steps)
+    sync()
+    dt = time.perf_counter() - t0
+    return cspace, case, hist, dt
+
+
+def time_to_residual(spec_args, mode, net, cap, pcorr_tol):
+    """Iterate from the initial state until res_mass < 1e-5 or `cap`
+    iterations; wall time and iteration count at the first crossing of each
+    threshold (the graph capture of the first iteration included)."""
+    from gfm_amd import case as cs
+    from gfm_amd.solver import CombustionSpace
+    case = cs.furnace_large(**dict(spec_args, maxgi=cap, bgcon=min(THRESHOLDS)))
+    configure(case, mode, pcorr_tol)
+    out = {f"{t:g}": None for t in THRESHOLDS}
+    best = float("inf")
+    with CombustionSpace(case, device=0) as s:
+        if net is not None:
+            s.set_pguess(net)
+        t0 = time.perf_counter()
+        n = 0
+        while n < cap:
+            h = s.sphase(1)
+            if len(h) == 0:
+                break
+            n += 1
+            r = float(h[0])
+            best = min(best, r)
+            for t in THRESHOLDS:
+                if out[f"{t:g}"] is None and r < t:
+                    out[f"{t:g}"] = {"seconds": round(time.perf_counter() - t0, 3), "iterations": n}
+            if r < min(THRESHOLDS):
+                break
+        total = time.perf_counter() - t0
+    return {"first_below": out, "iterations_run": n, "seconds": round(total, 3), "min_res_mass": best}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +137,11 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--eager", action="store_true", help="launch kernels one by one instead of replaying a graph")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned+FP32 configuration")
+    ap.add_argument("--pcorr-tol", type=float, default=0.025,
+                    help="p' tolerance of the learned+FP32 configuration (calc_residual)")
+    ap.add_argument("--ttr-cap", type=int, default=2000,
+                    help="iteration cap of the time-to-residual runs (0 skips them)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run a torch device copy of this size (reference kernel for rocprof byte counters)")
     args = ap.parse_args()
@@ -74,9 +150,6 @@ def main():
               "a case shards as x-slabs across GPUs only in the design notes", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
-    from gfm_amd import case as cs
-    from gfm_amd.solver import CombustionSpace
     try:
         import torch
         sync = torch.cuda.synchronize
@@ -93,24 +166,20 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
     spec_args = dict(nx=args.nx, ny=args.ny, nz=args.nz, seed=2, maxgi=10 ** 7, bgcon=0.0)
-    case = cs.furnace_large(**spec_args)
-    case.ctrl.sweep_order = 1
-    case.ctrl.want_pre_residuals = 0
-    case.ctrl.precision = 1 if args.precision == "mixed" else 0
-    n = case.ncells
-    cspace = CombustionSpace(case, device=0)
-    for _ in range(args.warmup):
-        cspace.gflow()
-    sync()
-    t0 = time.perf_counter()
-    hist = cspace.sphase(args.steps)
-    sync()
-    dt = time.perf_counter() - t0
+    n = args.nx * args.ny * args.nz
+    net = None
+    if not args.no_learned and os.path.exists(PGUESS_WEIGHTS) and not (args.nx | args.ny | args.nz) & 7:
+        from gfm_amd import pguess
+        net = pguess.load(PGUESS_WEIGHTS, max_cells=n)
+    # the workload of the metric's value: plain FP64 (or FP32 sweeps with
+    # the reference's p' solve, --precision mixed)
+    cspace, case, hist, dt = throughput(spec_args, args.precision, None, args.steps, args.warmup, args.pcorr_tol, sync)
     # per-kernel HIP-event timing (kernels launched one by one on the
     # library's stream) over a short pass right after the timed region
     cspace.kernel_times(1)
     cspace.sphase(max(2, min(5, args.steps)))
     kt = cspace.kernel_times(0)
+    cspace.close()
     steps = len(hist)
     res_final = float(hist[-1]) if steps else float("nan")
     value = n * steps / dt / 1e6
@@ -134,9 +203,25 @@ def main():
                                f"({n} cells), zebra x-plane order", "cells": n, "res_mass_last": res_final},
         "roofline": roof,
     }
+    if net is not None and not args.no_learned:
+        c2, case2, h2, dt2 = throughput(spec_args, "learned", net, args.steps, args.warmup, args.pcorr_tol, sync)
+        c2.close()
+        line["learned_fp32"] = {
+            "value": round(n * len(h2) / dt2 / 1e6, 3), "unit": "Mcell-updates/s",
+            "ms_per_step": round(dt2 / max(len(h2), 1) * 1e3, 3), "pcorr_tol": args.pcorr_tol,
+            "pcorr_sweeps_last": case2.ctrl.pcorr_sweeps,
+            "description": "FP32 assembly and line sweeps (FP64 p' defect, balances, h->T); p' from the learned "
+                           "guess relaxed to the calc_residual the reference's 5 sweeps reach"}
+    if args.ttr_cap > 0:
+        ttr = {"cap_iterations": args.ttr_cap, "plain_fp64": time_to_residual(spec_args, "fp64", None, args.ttr_cap,
+                                                                               args.pcorr_tol)}
+        if net is not None and not args.no_learned:
+            ttr["learned_fp32"] = time_to_residual(spec_args, "learned", net, args.ttr_cap, args.pcorr_tol)
+        line["time_to_residual"] = ttr
+    if net is not None:
+        net.close()
     kt_sum = {k: {"ms": round(v[0], 3), "launches": v[2]} for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0])}
     print(json.dumps({"kernel_times_ms": kt_sum}), file=sys.stderr)
-    cspace.close()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(spec_args, args.cpu_iters)
     print(json.dumps(line), flush=True)
