@@ -81,6 +81,7 @@ enum {
   SC_QWALL,  // total convective heat to walls and melt surface (W)
   SC_ITR,  // itr_gas of the running iteration (read by kernels so a captured graph is reusable)
   SC_PG_RMS, SC_PG_ALPHA, SC_PG_FALLBACK,  // learned p' guess: input scale, guess scale, fallback flag
+  SC_PG_COS2,  // learned p' guess: fraction of ||BS||^2 the scaled guess removes
   SC_RESID0 = 32,  // 64 residual slots follow
   SC_COUNT = 128
 };

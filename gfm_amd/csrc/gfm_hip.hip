@@ -286,6 +286,7 @@ static void pull_scalars(gfm_ctx *c, gfm_ctrl *k, const double *sc) {
   for (int q = 0; q < 3; q++) k->u_resid[q] = r[RS_U + q];
   k->pcorr_resid = r[RS_PC];
   k->pcorr_alpha = sc[SC_PG_ALPHA];
+  k->pcorr_gain = sc[SC_PG_COS2];
   k->pcorr_fallback = (int)sc[SC_PG_FALLBACK];
   for (int q = 0; q < 7; q++) k->gf_resid[q] = r[RS_GF + q];
   k->h_resid = r[RS_GF + 2];
@@ -884,11 +885,13 @@ int gfm_hip_gflow(gfm_ctx *c, gfm_ctrl *k) {
   }
   if (c->ctrl.pcorr_mode == GFM_PCORR_REFERENCE || c->ctrl.pcorr_guess != GFM_PGUESS_LEARNED || !c->pnet) {
     c->ctrl.pcorr_alpha = 0.0;
+    c->ctrl.pcorr_gain = 0.0;
     c->ctrl.pcorr_fallback = 0;
   }
   k->pcorr_sweeps = c->ctrl.pcorr_sweeps;
   k->pcorr_resid_start = c->ctrl.pcorr_resid_start;
   k->pcorr_alpha = c->ctrl.pcorr_alpha;
+  k->pcorr_gain = c->ctrl.pcorr_gain;
   k->pcorr_fallback = c->ctrl.pcorr_fallback;
   if (c->ctrl.pcorr_dump) c->pair_ready = 1;
   if (c->prof.on) prof_flush(c);

@@ -1323,6 +1323,7 @@ __global__ void k_pg_rms(const double *part, int nb) {
     g.scal[SC_PG_RMS] = r > 0.0 ? r : 1.0;
     g.scal[SC_PG_ALPHA] = 0.0;
     g.scal[SC_PG_FALLBACK] = 0.0;
+    g.scal[SC_PG_COS2] = 0.0;
   }
 }
 __global__ void k_pg_norm(float *x, long long V) {
@@ -1336,10 +1337,10 @@ __global__ void k_pg_scatter(Box b, const float *y) {
   const long long q = lidx(i, j, k);
   if (pg_open(i, j, k, q)) g.fz[q] = (double)y[pg_cell(i, j, k)];
 }
-// partial sums of BS.(A g) and (A g).(A g), (A g) = AP g - sum AS g_nb, FP64
+// partial sums of BS.(A g), (A g).(A g) and BS.BS, (A g) = AP g - sum AS g_nb, FP64
 __global__ void k_pg_ag(Box b, double *part) {
   int i, j, k;
-  double bag = 0.0, agag = 0.0;
+  double bag = 0.0, agag = 0.0, bb = 0.0;
   if (box_pt(b, i, j, k)) {
     const long long q = lidx(i, j, k);
     if (pg_open(i, j, k, q)) {
@@ -1348,25 +1349,41 @@ __global__ void k_pg_ag(Box b, double *part) {
                                                 CAS(2, q) * FZ(i, j, k - 2) + CAS(5, q) * FZ(i, j, k + 2));
       bag = CBS(q) * ag;
       agag = ag * ag;
+      bb = CBS(q) * CBS(q);
     }
   }
-  const double w1 = wave_sum(bag), w2 = wave_sum(agag);
+  const double w1 = wave_sum(bag), w2 = wave_sum(agag), w3 = wave_sum(bb);
   if (threadIdx.x == 0) {
     part[block_lin()] = w1;
     part[g.pstride + block_lin()] = w2;
+    part[2 * g.pstride + block_lin()] = w3;
   }
 }
+// alpha = BS.(Ag) / (Ag).(Ag), which leaves ||BS||^2 (1 - cos^2) of the
+// residual, cos^2 = (BS.Ag)^2 / (||BS||^2 ||Ag||^2).  A guess that removes
+// less than half of ||BS||^2 (cos^2 < PG_MIN_COS2) gets alpha = 0: a
+// prediction nearly orthogonal to the source would be scaled arbitrarily.
+#define PG_MIN_COS2 0.5
 __global__ void k_pg_alpha(const double *part, int nb) {
-  __shared__ double s1[256], s2[256];
-  double a = 0.0, c = 0.0;
-  for (int q = threadIdx.x; q < nb; q += 256) { a += part[q]; c += part[g.pstride + q]; }
-  s1[threadIdx.x] = a; s2[threadIdx.x] = c;
+  __shared__ double s1[256], s2[256], s3[256];
+  double a = 0.0, c = 0.0, e = 0.0;
+  for (int q = threadIdx.x; q < nb; q += 256) { a += part[q]; c += part[g.pstride + q]; e += part[2 * g.pstride + q]; }
+  s1[threadIdx.x] = a; s2[threadIdx.x] = c; s3[threadIdx.x] = e;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) { s1[threadIdx.x] += s1[threadIdx.x + o]; s2[threadIdx.x] += s2[threadIdx.x + o]; }
+    if ((int)threadIdx.x < o) {
+      s1[threadIdx.x] += s1[threadIdx.x + o];
+      s2[threadIdx.x] += s2[threadIdx.x + o];
+      s3[threadIdx.x] += s3[threadIdx.x + o];
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) g.scal[SC_PG_ALPHA] = s2[0] > 0.0 ? s1[0] / s2[0] : 0.0;
+  if (threadIdx.x == 0) {
+    const double bag = s1[0], agag = s2[0], bb = s3[0];
+    const double cos2 = (agag > 0.0 && bb > 0.0) ? bag * bag / (agag * bb) : 0.0;
+    g.scal[SC_PG_COS2] = cos2;
+    g.scal[SC_PG_ALPHA] = cos2 >= PG_MIN_COS2 ? bag / agag : 0.0;
+  }
 }
 __global__ void k_pg_scale(Box b) {
   int i, j, k;

@@ -63,13 +63,14 @@ def make_case(op, grid=(48, 24, 16), seed=100, large=False, precision=0, tol=1e-
     return c
 
 
-def generate_pairs(points, grid=(48, 24, 16), iters=130, skip=2, tol=1e-3, max_sweeps=400, seed0=100, log=None):
+def generate_pairs(points, grid=(48, 24, 16), iters=130, skip=2, tol=1e-3, max_sweeps=400, seed0=100, log=None,
+                   large=False):
     """Run each operating point `iters` SIMPLER iterations with the p' solve
     relaxed to `tol` from zero; keep the pair of every iteration after the
     first `skip`.  Returns X [N,nz,ny,nx,2], T [N,nz,ny,nx] (normalised)."""
     X, T = [], []
     for q, op in enumerate(points):
-        c = make_case(op, grid, seed=seed0 + q, tol=tol, max_sweeps=max_sweeps, dump=1)
+        c = make_case(op, grid, seed=seed0 + q, large=large, tol=tol, max_sweeps=max_sweeps, dump=1)
         t0 = time.perf_counter()
         sw = []
         with CombustionSpace(c) as s:
@@ -131,6 +132,9 @@ def main(argv=None):
     t.add_argument("--lr", type=float, default=1e-3)
     t.add_argument("--tol", type=float, default=1e-3)
     t.add_argument("--seed", type=int, default=1)
+    t.add_argument("--large", action="store_true", help="pairs from the stretched 4-burner furnace instead of the "
+                                                        "single-burner chamber")
+    t.add_argument("--grid", type=int, nargs=3, default=None, help="nx ny nz (default 48 24 16, or 192 96 64 with --large)")
     e = sub.add_parser("eval")
     e.add_argument("--weights", required=True)
     e.add_argument("--points", type=int, default=8)
@@ -143,7 +147,8 @@ def main(argv=None):
     if a.cmd == "train":
         pts = operating_points(a.points, a.seed)
         t0 = time.perf_counter()
-        X, T = generate_pairs(pts, iters=a.pairs + 2, skip=2, tol=a.tol, log=log)
+        grid = tuple(a.grid) if a.grid else ((192, 96, 64) if a.large else (48, 24, 16))
+        X, T = generate_pairs(pts, grid=grid, iters=a.pairs + 2, skip=2, tol=a.tol, log=log, large=a.large)
         log(f"{len(X)} pairs in {time.perf_counter() - t0:.1f} s")
         net = PNet(DEFAULT_CHANNELS, max_batch=a.batch, max_cells=int(np.prod(X.shape[1:4])), seed=a.seed)
         log(f"network: {net.n_effective} parameters ({net.n_params} with padding)")
@@ -152,7 +157,8 @@ def main(argv=None):
         dt = time.perf_counter() - t0
         log(f"trained {a.epochs} epochs in {dt:.1f} s, loss per epoch {np.array2string(hist, precision=4)}")
         save(net, a.out, dict(points=a.points, pairs=len(X), epochs=a.epochs, batch=a.batch, lr=a.lr, tol=a.tol,
-                              seed=a.seed, loss=hist.tolist(), train_seconds=dt))
+                              seed=a.seed, loss=hist.tolist(), train_seconds=dt, grid=list(grid),
+                              geometry="large" if a.large else "small"))
         net.close()
     else:
         grid = (192, 96, 64) if a.large else (48, 24, 16)
