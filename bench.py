@@ -54,24 +54,26 @@ def cpu_baseline(spec_args, iters=2):
                       f"oracle/gfm_oracle.c (gcc -O2) on 1 host core, {dt:.1f} s"}
 
 
-PGUESS_WEIGHTS = os.path.join(ROOT, "gfm_amd", "data", "pguess_furnace48x24x16.npz")
+PGUESS_DIR = os.path.join(ROOT, "gfm_amd", "data")
 THRESHOLDS = (1e-3, 1e-4, 1e-5)
 
 
 def configure(case, mode, pcorr_tol):
-    """plain FP64: the reference algorithm (FP64, p' from zero, 5 sweeps).
-    learned+FP32: FP32 assembly and sweeps, p' from the network's guess
-    relaxed to the calc_residual the reference's 5 sweeps reach."""
+    """fp64: the reference algorithm (FP64, p' from zero, 5 sweeps).
+    mixed: FP32 assembly and sweeps, the reference's p' solve.
+    mixed_zero: FP32, p' from zero relaxed to pcorr_tol.
+    learned: FP32, p' from the network's guess relaxed to pcorr_tol."""
     k = case.ctrl
     k.sweep_order = 1
     k.want_pre_residuals = 0
     if mode == "fp64":
         k.precision, k.pcorr_mode = 0, 0
-    elif mode == "mixed":  # FP32 assembly and sweeps, the reference's p' solve
+    elif mode == "mixed":
         k.precision, k.pcorr_mode = 1, 0
     else:
-        k.precision, k.pcorr_mode, k.pcorr_guess = 1, 1, 1
-        k.pcorr_tol, k.pcorr_max_sweeps = pcorr_tol, 100
+        k.precision, k.pcorr_mode, k.pcorr_guess = 1, 1, 1 if mode == "learned" else 0
+        k.pcorr_tol, k.pcorr_max_sweeps = pcorr_tol, 200
+        k.pcorr_guess_accept = 0.5  # keep a guess only if it halves the start residual
 
 
 def throughput(spec_args, mode, net, steps, warmup, pcorr_tol, sync):
@@ -138,9 +140,9 @@ def main():
     ap.add_argument("--eager", action="store_true", help="launch kernels one by one instead of replaying a graph")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned+FP32 configuration")
-    ap.add_argument("--pcorr-tol", type=float, default=0.025,
+    ap.add_argument("--pcorr-tol", type=float, default=0.01,
                     help="p' tolerance of the learned+FP32 configuration (calc_residual)")
-    ap.add_argument("--ttr-cap", type=int, default=2000,
+    ap.add_argument("--ttr-cap", type=int, default=800,
                     help="iteration cap of the time-to-residual runs (0 skips them)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run a torch device copy of this size (reference kernel for rocprof byte counters)")
@@ -168,9 +170,12 @@ def main():
     spec_args = dict(nx=args.nx, ny=args.ny, nz=args.nz, seed=2, maxgi=10 ** 7, bgcon=0.0)
     n = args.nx * args.ny * args.nz
     net = None
-    if not args.no_learned and os.path.exists(PGUESS_WEIGHTS) and not (args.nx | args.ny | args.nz) & 7:
+    weights = os.path.join(PGUESS_DIR, f"pguess_furnace{args.nx}x{args.ny}x{args.nz}.npz")
+    if not os.path.exists(weights):
+        weights = os.path.join(PGUESS_DIR, "pguess_furnace48x24x16.npz")
+    if not args.no_learned and os.path.exists(weights) and not (args.nx | args.ny | args.nz) & 7:
         from gfm_amd import pguess
-        net = pguess.load(PGUESS_WEIGHTS, max_cells=n)
+        net = pguess.load(weights, max_cells=n)
     # the workload of the metric's value: plain FP64 (or FP32 sweeps with
     # the reference's p' solve, --precision mixed)
     cspace, case, hist, dt = throughput(spec_args, args.precision, None, args.steps, args.warmup, args.pcorr_tol, sync)
@@ -203,20 +208,22 @@ def main():
                                f"({n} cells), zebra x-plane order", "cells": n, "res_mass_last": res_final},
         "roofline": roof,
     }
-    if net is not None and not args.no_learned:
-        c2, case2, h2, dt2 = throughput(spec_args, "learned", net, args.steps, args.warmup, args.pcorr_tol, sync)
+    others = [("fp32", "mixed", None)]
+    if net is not None:
+        others += [("fp32_zero_guess_tol", "mixed_zero", None), ("learned_fp32", "learned", net)]
+    for name, mode, nn in others:
+        c2, case2, h2, dt2 = throughput(spec_args, mode, nn, args.steps, args.warmup, args.pcorr_tol, sync)
         c2.close()
-        line["learned_fp32"] = {
-            "value": round(n * len(h2) / dt2 / 1e6, 3), "unit": "Mcell-updates/s",
-            "ms_per_step": round(dt2 / max(len(h2), 1) * 1e3, 3), "pcorr_tol": args.pcorr_tol,
-            "pcorr_sweeps_last": case2.ctrl.pcorr_sweeps,
-            "description": "FP32 assembly and line sweeps (FP64 p' defect, balances, h->T); p' from the learned "
-                           "guess relaxed to the calc_residual the reference's 5 sweeps reach"}
+        line[name] = {"value": round(n * len(h2) / dt2 / 1e6, 3), "unit": "Mcell-updates/s",
+                      "ms_per_step": round(dt2 / max(len(h2), 1) * 1e3, 3)}
+        if mode != "mixed":
+            line[name].update(pcorr_tol=args.pcorr_tol, pcorr_sweeps_last=case2.ctrl.pcorr_sweeps)
+    if net is not None:
+        line["learned_fp32"]["network"] = os.path.relpath(weights, ROOT)
     if args.ttr_cap > 0:
-        ttr = {"cap_iterations": args.ttr_cap, "plain_fp64": time_to_residual(spec_args, "fp64", None, args.ttr_cap,
-                                                                               args.pcorr_tol)}
-        if net is not None and not args.no_learned:
-            ttr["learned_fp32"] = time_to_residual(spec_args, "learned", net, args.ttr_cap, args.pcorr_tol)
+        ttr = {"cap_iterations": args.ttr_cap}
+        for name, mode, nn in [("plain_fp64", "fp64", None)] + others:
+            ttr[name] = time_to_residual(spec_args, mode, nn, args.ttr_cap, args.pcorr_tol)
         line["time_to_residual"] = ttr
     if net is not None:
         net.close()

@@ -42,6 +42,7 @@ class GfmCtrl(ct.Structure):
         ("pcorr_mode", I32), ("pcorr_guess", I32), ("pcorr_max_sweeps", I32), ("pcorr_dump", I32),
         ("pcorr_sweeps", I32), ("pcorr_fallback", I32),
         ("pcorr_tol", D), ("pcorr_resid_start", D), ("pcorr_alpha", D), ("pcorr_gain", D),
+        ("pcorr_guess_accept", D),
     ]
 
     def copy(self):

@@ -418,7 +418,8 @@ static int pcorr_guess(gfm_ctx *c, const Box &cen) {
   hipLaunchKernelGGL(k_pg_alpha, dim3(1), dim3(256), 0, c->stream, c->part, (int)nb);
   LAUNCH(k_pg_scale, cen);
   residual(c, d.fz, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_PCG, 0);
-  LAUNCH(k_pg_fallback, cen, RS_PCG, RS_PC0);
+  const double accept = k.pcorr_guess_accept > 0.0 ? k.pcorr_guess_accept : 1.0;
+  LAUNCH(k_pg_fallback, cen, RS_PCG, RS_PC0, accept);
   return GFM_OK;
 }
 
@@ -873,6 +874,8 @@ int gfm_hip_gflow(gfm_ctx *c, gfm_ctrl *k) {
   c->ctrl.pcorr_max_sweeps = k->pcorr_max_sweeps;
   c->ctrl.pcorr_tol = k->pcorr_tol;
   c->ctrl.pcorr_dump = k->pcorr_dump;
+  if (c->ctrl.pcorr_guess_accept != k->pcorr_guess_accept) drop_graph(c);  // captured as a kernel argument
+  c->ctrl.pcorr_guess_accept = k->pcorr_guess_accept;
   if ((rc = run_gflow(c))) return rc;
   double sc[SC_COUNT];
   HCHECK(hipMemcpyAsync(sc, c->dc.scal, sizeof sc, hipMemcpyDeviceToHost, c->stream));

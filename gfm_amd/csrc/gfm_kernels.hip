@@ -1360,10 +1360,7 @@ __global__ void k_pg_ag(Box b, double *part) {
   }
 }
 // alpha = BS.(Ag) / (Ag).(Ag), which leaves ||BS||^2 (1 - cos^2) of the
-// residual, cos^2 = (BS.Ag)^2 / (||BS||^2 ||Ag||^2).  A guess that removes
-// less than half of ||BS||^2 (cos^2 < PG_MIN_COS2) gets alpha = 0: a
-// prediction nearly orthogonal to the source would be scaled arbitrarily.
-#define PG_MIN_COS2 0.5
+// residual, cos^2 = (BS.Ag)^2 / (||BS||^2 ||Ag||^2) (kept for diagnostics).
 __global__ void k_pg_alpha(const double *part, int nb) {
   __shared__ double s1[256], s2[256], s3[256];
   double a = 0.0, c = 0.0, e = 0.0;
@@ -1382,7 +1379,7 @@ __global__ void k_pg_alpha(const double *part, int nb) {
     const double bag = s1[0], agag = s2[0], bb = s3[0];
     const double cos2 = (agag > 0.0 && bb > 0.0) ? bag * bag / (agag * bb) : 0.0;
     g.scal[SC_PG_COS2] = cos2;
-    g.scal[SC_PG_ALPHA] = cos2 >= PG_MIN_COS2 ? bag / agag : 0.0;
+    g.scal[SC_PG_ALPHA] = agag > 0.0 ? bag / agag : 0.0;
   }
 }
 __global__ void k_pg_scale(Box b) {
@@ -1391,10 +1388,13 @@ __global__ void k_pg_scale(Box b) {
   const long long q = lidx(i, j, k);
   g.fz[q] *= g.scal[SC_PG_ALPHA];
 }
-// keep the guess only if it lowers calc_residual below that of FZ = 0
-__global__ void k_pg_fallback(Box b, int slot_guess, int slot_zero) {
+// keep the guess only if its calc_residual is below `accept` times that of
+// FZ = 0 (accept < 1 demands a margin: a prediction that leaves the
+// residual nearly as large as zero does can carry wrong long-wave content,
+// which the few sweeps of a loose tolerance do not remove)
+__global__ void k_pg_fallback(Box b, int slot_guess, int slot_zero, double accept) {
   int i, j, k;
-  const bool worse = !(g.scal[SC_RESID0 + slot_guess] < g.scal[SC_RESID0 + slot_zero]);
+  const bool worse = !(g.scal[SC_RESID0 + slot_guess] < accept * g.scal[SC_RESID0 + slot_zero]);
   if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0) g.scal[SC_PG_FALLBACK] = worse ? 1.0 : 0.0;
   if (!worse || !box_pt(b, i, j, k)) return;
   g.fz[lidx(i, j, k)] = 0.0;

@@ -64,10 +64,11 @@ def make_case(op, grid=(48, 24, 16), seed=100, large=False, precision=0, tol=1e-
 
 
 def generate_pairs(points, grid=(48, 24, 16), iters=130, skip=2, tol=1e-3, max_sweeps=400, seed0=100, log=None,
-                   large=False):
+                   large=False, every=1):
     """Run each operating point `iters` SIMPLER iterations with the p' solve
-    relaxed to `tol` from zero; keep the pair of every iteration after the
-    first `skip`.  Returns X [N,nz,ny,nx,2], T [N,nz,ny,nx] (normalised)."""
+    relaxed to `tol` from zero; keep the pair of every `every`-th iteration
+    after the first `skip`.  Returns X [N,nz,ny,nx,2], T [N,nz,ny,nx]
+    (normalised)."""
     X, T = [], []
     for q, op in enumerate(points):
         c = make_case(op, grid, seed=seed0 + q, large=large, tol=tol, max_sweeps=max_sweeps, dump=1)
@@ -77,7 +78,7 @@ def generate_pairs(points, grid=(48, 24, 16), iters=130, skip=2, tol=1e-3, max_s
             for it in range(iters):
                 s.sphase(1)
                 sw.append(c.ctrl.pcorr_sweeps)
-                if it >= skip:
+                if it >= skip and (it - skip) % every == 0:
                     x, p = s.pcorr_pair()
                     xn, pn = normalise(x, p)
                     X.append(xn)
@@ -134,6 +135,7 @@ def main(argv=None):
     t.add_argument("--seed", type=int, default=1)
     t.add_argument("--large", action="store_true", help="pairs from the stretched 4-burner furnace instead of the "
                                                         "single-burner chamber")
+    t.add_argument("--every", type=int, default=1, help="keep every n-th iteration's pair (pairs per point unchanged)")
     t.add_argument("--grid", type=int, nargs=3, default=None, help="nx ny nz (default 48 24 16, or 192 96 64 with --large)")
     e = sub.add_parser("eval")
     e.add_argument("--weights", required=True)
@@ -148,17 +150,22 @@ def main(argv=None):
         pts = operating_points(a.points, a.seed)
         t0 = time.perf_counter()
         grid = tuple(a.grid) if a.grid else ((192, 96, 64) if a.large else (48, 24, 16))
-        X, T = generate_pairs(pts, grid=grid, iters=a.pairs + 2, skip=2, tol=a.tol, log=log, large=a.large)
+        X, T = generate_pairs(pts, grid=grid, iters=a.pairs * a.every + 2, skip=2, tol=a.tol, log=log, large=a.large,
+                              every=a.every)
         log(f"{len(X)} pairs in {time.perf_counter() - t0:.1f} s")
         net = PNet(DEFAULT_CHANNELS, max_batch=a.batch, max_cells=int(np.prod(X.shape[1:4])), seed=a.seed)
         log(f"network: {net.n_effective} parameters ({net.n_params} with padding)")
         t0 = time.perf_counter()
-        hist = net.train(X, T, epochs=a.epochs, batch=a.batch, lr=a.lr, seed=a.seed)
+        hist = []
+        for ep in range(a.epochs):  # one call per epoch, so progress is visible
+            hist.append(float(net.train(X, T, epochs=1, batch=a.batch, lr=a.lr, seed=a.seed * 1000 + ep)[0]))
+            log(f"epoch {ep}: loss {hist[-1]:.4g} ({time.perf_counter() - t0:.1f} s)")
+        hist = np.array(hist)
         dt = time.perf_counter() - t0
         log(f"trained {a.epochs} epochs in {dt:.1f} s, loss per epoch {np.array2string(hist, precision=4)}")
         save(net, a.out, dict(points=a.points, pairs=len(X), epochs=a.epochs, batch=a.batch, lr=a.lr, tol=a.tol,
                               seed=a.seed, loss=hist.tolist(), train_seconds=dt, grid=list(grid),
-                              geometry="large" if a.large else "small"))
+                              geometry="large" if a.large else "small", every=a.every))
         net.close()
     else:
         grid = (192, 96, 64) if a.large else (48, 24, 16)

@@ -70,8 +70,10 @@ enum { GFM_PREC_FP64 = 0, GFM_PREC_MIXED = 1 };
  *   GFM_PGUESS_LEARNED: the network attached by gfm_hip_set_pguess maps the
  *     Jacobi-scaled mass source BS/AP to a p' field g; the guess is alpha*g
  *     with alpha minimising ||BS - alpha*A g||_2 (FP64).  FZ = 0 is kept
- *     instead when alpha*g removes less than half of ||BS||^2 or when
- *     calc_residual(alpha*g) >= calc_residual(0). */
+ *     instead unless calc_residual(alpha*g) < a * calc_residual(0), with
+ *     a = pcorr_guess_accept (1 when <= 0: the guess must lower the
+ *     residual; smaller values demand a margin, which guards a network
+ *     applied outside the states it was trained on). */
 enum { GFM_PCORR_REFERENCE = 0, GFM_PCORR_TOLERANCE = 1 };
 enum { GFM_PGUESS_ZERO = 0, GFM_PGUESS_LEARNED = 1 };
 
@@ -141,6 +143,7 @@ typedef struct gfm_ctrl {
   double pcorr_resid_start;       /* out: calc_residual of the initial guess */
   double pcorr_alpha;             /* out: scale of the learned guess */
   double pcorr_gain;              /* out: fraction of ||BS||^2 the scaled learned guess removes */
+  double pcorr_guess_accept;      /* in: acceptance ratio of the learned guess (see above) */
 } gfm_ctrl;
 
 /* Opaque device-side context. */
