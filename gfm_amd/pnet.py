@@ -113,3 +113,42 @@ class PNet:
             self.close()
         except Exception:
             pass
+
+
+def forward_flops(channels, d, h, w, batch=1):
+    """Multiply-add FLOPs (2 per MAC) of one forward pass: the ten 3x3x3
+    convolutions at their padded channel widths (what the matrix cores
+    execute) and the 1x1x1 output layer."""
+    c = channels
+    v = [batch * (d >> q) * (h >> q) * (w >> q) for q in range(4)]
+    convs = [(0, 32, c[0]), (0, c[0], c[0]), (1, c[0], c[1]), (1, c[1], c[1]), (2, c[1], c[2]), (2, c[2], c[2]),
+             (3, c[2], c[3]), (2, c[3] + c[2], c[2]), (1, c[2] + c[1], c[1]), (0, c[1] + c[0], c[0])]
+    return sum(2 * 27 * ci * co * v[lv] for lv, ci, co in convs) + 2 * c[0] * v[0]
+
+
+def _main():
+    import argparse
+    import time
+    ap = argparse.ArgumentParser(prog="python -m gfm_amd.pnet",
+                                 description="time the forward pass of the p' guess network on one grid")
+    ap.add_argument("--grid", type=int, nargs=3, default=(192, 96, 64), metavar=("NX", "NY", "NZ"))
+    ap.add_argument("--reps", type=int, default=20)
+    a = ap.parse_args()
+    nx, ny, nz = a.grid
+    rng = np.random.default_rng(0)
+    x = np.zeros((1, nz, ny, nx, 2), dtype=np.float32)
+    x[..., 0] = rng.standard_normal((1, nz, ny, nx))
+    x[..., 1] = 1.0
+    with PNet(max_batch=1, max_cells=nx * ny * nz) as net:
+        net.forward(x)
+        t0 = time.perf_counter()
+        for _ in range(a.reps):
+            net.forward(x)
+        dt = (time.perf_counter() - t0) / a.reps
+    fl = forward_flops(DEFAULT_CHANNELS, nz, ny, nx)
+    print(f"forward {nx}x{ny}x{nz}: {dt * 1e3:.3f} ms per pass including host copies, {fl / 1e9:.1f} GFLOP, "
+          f"{fl / dt / 1e12:.1f} TFLOP/s")
+
+
+if __name__ == "__main__":
+    _main()
