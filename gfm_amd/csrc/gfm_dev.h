@@ -8,15 +8,15 @@
 // x-plane in the line relaxation) so that planes of one colour are
 // adjacent in memory:
 //
-//   xpos(c)      = (c&1)*HX + (c>>1) + 15,
+//   xpos(c)      = (c&1)*HX + (c>>1) + 31,
 //   offset(I,J,K) = cls*S + ((K>>1)*NYC + (J>>1))*NXP + xpos(I>>1),
 //   cls = (I&1) | (J&1)<<1 | (K&1)<<2,  NXP = 2*HX,  S = NXP*NYC*NZC,
-//   HX = ceil((ceil(NXC/2) + 15)/16)*16,  NXC = MP/2+1.
+//   HX = ceil((ceil(NXC/2) + 31)/32)*32,  NXC = MP/2+1.
 //
-// HX is a multiple of 16 doubles (128 bytes, one L2 line) and the offset 15
-// puts the first interior plane of either colour (c = 2 or 3, c>>1 = 1) at
-// the start of a line, so a tile of 16 consecutive planes of one colour is
-// exactly one line of each row.  Kernels keep the reference's lattice
+// HX is a multiple of 32 elements (128 bytes of FP32, 256 of FP64) and the
+// offset 31 puts the first interior plane of either colour (c = 2 or 3,
+// c>>1 = 1) at element 32, so a tile of 16 consecutive FP64 planes or 32
+// FP32 planes of one colour is exactly one 128-byte L2 line of each row.  Kernels keep the reference's lattice
 // indices (I,J,K).  A wavefront walking consecutive I of one class touches
 // two contiguous segments; a wavefront walking consecutive planes of one
 // colour touches one.
@@ -27,14 +27,14 @@
 struct DevGeom {
   int MP, NP, LP;
   int NXC, NYC, NZC;
-  int HX, NXP;  // half-row stride and row length (multiples of 16)
+  int HX, NXP;  // half-row stride and row length (multiples of 32)
   long long S;  // entries per class
 };
 
 __host__ __device__ __forceinline__ long long lat_index(const DevGeom &gg, int i, int j, int k) {
   int cls = (i & 1) | ((j & 1) << 1) | ((k & 1) << 2);
   int c = i >> 1;
-  int xp = (c & 1) * gg.HX + (c >> 1) + 15;
+  int xp = (c & 1) * gg.HX + (c >> 1) + 31;
   return (long long)cls * gg.S + ((long long)(k >> 1) * gg.NYC + (j >> 1)) * gg.NXP + xp;
 }
 

@@ -297,12 +297,17 @@ static void pull_scalars(gfm_ctx *c, gfm_ctrl *k, const double *sc) {
 // ---------------------------------------------------------------------
 // ADLBL3 (C/Adlbl3.f90:10-97) and calc_residual
 // ---------------------------------------------------------------------
-// Line-kernel instance: tile width (16 planes, or 2 for fewer planes than
-// a tile) and rows per lane NU (lines of up to 32*NU rows).
-static int lines_lw(int npl) { return npl >= 16 ? 16 : 2; }
+// Line-kernel instance: tile width (one 128-byte line of each row: 16 FP64
+// or 32 FP32 planes; 2 for fewer planes than a tile) and rows per lane NU
+// (lines of up to 32*NU rows).
+static int lines_lw(int npl, size_t word) {
+  const int full = word == 4 ? 32 : 16;
+  return npl >= full ? full : 2;
+}
 static int lines_nu(int nmax) { return nmax <= 64 ? 2 : nmax <= 96 ? 3 : nmax <= 128 ? 4 : 8; }
 static size_t lines_shm(int lw, int nmax, size_t word) { return 3 * (size_t)lw * (nmax | 1) * word; }
-#define LINES_INSTANCES(X_) X_(16, 2) X_(16, 3) X_(16, 4) X_(16, 8) X_(2, 2) X_(2, 3) X_(2, 4) X_(2, 8)
+#define LINES_INSTANCES(X_) \
+  X_(32, 2) X_(32, 3) X_(32, 4) X_(32, 8) X_(16, 2) X_(16, 3) X_(16, 4) X_(16, 8) X_(2, 2) X_(2, 3) X_(2, 4) X_(2, 8)
 template <typename T>
 static void launch_lines(gfm_ctx *c, int lw, int nu, unsigned nblk, size_t shm, const T *xt, T *xy, T *xz, int pi0, int pdi,
                          int npl, int jb, int je, int kb, int ke) {
@@ -334,7 +339,7 @@ static void plane_op(gfm_ctx *c, T *xt, T *x2, T *x3, int pi0, int pdi, int npl,
   const double cells = (double)npl * nyl * nzl;
   // algorithmic bytes: 8 coefficients, one read and one write of the unknown per row and direction
   const double line_bytes = cells * 2.0 * (8.0 * sizeof(T) + 2.0 * sizeof(T));
-  const int lw = lines_lw(npl);
+  const int lw = lines_lw(npl, sizeof(T));
   const int ntile = (npl + lw - 1) / lw;
   const int nmax = ny > nz ? ny : nz;
   const int nu = lines_nu(nmax);
@@ -680,7 +685,7 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
   DevGeom &gg = c->dc.g;
   gg.MP = k->mp; gg.NP = k->np; gg.LP = k->lp;
   gg.NXC = k->mp / 2 + 1; gg.NYC = k->np / 2 + 1; gg.NZC = k->lp / 2 + 1;
-  gg.HX = ((gg.NXC + 1) / 2 + 15 + 15) / 16 * 16;
+  gg.HX = ((gg.NXC + 1) / 2 + 31 + 31) / 32 * 32;
   gg.NXP = 2 * gg.HX;
   gg.S = (long long)gg.NXP * gg.NYC * gg.NZC;
   c->ncls = gg.S * 8;
@@ -750,7 +755,7 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
   {
     // LDS of k_lines beyond the default 64 KB for long lines
     int nmax = (k->np > k->lp ? k->np : k->lp) / 2;
-    size_t shm = lines_shm(16, nmax, d.mixed ? sizeof(float) : sizeof(double));
+    size_t shm = lines_shm(d.mixed ? 32 : 16, nmax, d.mixed ? sizeof(float) : sizeof(double));
     if (shm > 64 * 1024) {
       if ((rc = d.mixed ? set_lines_attr<float>(shm) : set_lines_attr<double>(shm))) return fail(rc);
     }
