@@ -4,6 +4,9 @@ module gfm_hip_mod
   use gfm_types
   implicit none
   integer(c_int), parameter :: GFM_ORDER_REFERENCE = 0, GFM_ORDER_ZEBRA = 1
+  integer(c_int), parameter :: GFM_PREC_FP64 = 0, GFM_PREC_MIXED = 1
+  integer(c_int), parameter :: GFM_PCORR_REFERENCE = 0, GFM_PCORR_TOLERANCE = 1
+  integer(c_int), parameter :: GFM_PGUESS_ZERO = 0, GFM_PGUESS_LEARNED = 1
   interface
     integer(c_int) function gfm_hip_init(ctrl, h, device, ctx) bind(C, name="gfm_hip_init")
       import :: c_int, c_ptr, gfm_ctrl, gfm_host_arrays
@@ -63,6 +66,39 @@ module gfm_hip_mod
     integer(c_int) function gfm_hip_finalize(ctx) bind(C, name="gfm_hip_finalize")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function gfm_hip_set_pguess(ctx, net) bind(C, name="gfm_hip_set_pguess")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, net
+    end function
+    integer(c_int) function gfm_hip_pcorr_pair(ctx, x, p) bind(C, name="gfm_hip_pcorr_pair")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: ctx
+      real(c_float), intent(out) :: x(*), p(*)
+    end function
+    ! include/gfm_pnet.h: the network of the learned p' guess
+    integer(c_int) function gfm_pnet_create(device, ch, max_batch, max_cells, seed, net) bind(C, name="gfm_pnet_create")
+      import :: c_int, c_ptr, c_long_long, c_int64_t
+      integer(c_int), value :: device, max_batch
+      integer(c_int), intent(in) :: ch(4)
+      integer(c_long_long), value :: max_cells
+      integer(c_int64_t), value :: seed
+      type(c_ptr), intent(out) :: net
+    end function
+    integer(c_int) function gfm_pnet_set_params(net, p, n) bind(C, name="gfm_pnet_set_params")
+      import :: c_int, c_ptr, c_float, c_size_t
+      type(c_ptr), value :: net
+      real(c_float), intent(in) :: p(*)
+      integer(c_size_t), value :: n
+    end function
+    integer(c_int) function gfm_pnet_param_count(net, n_total, n_effective) bind(C, name="gfm_pnet_param_count")
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), value :: net
+      integer(c_size_t), intent(out) :: n_total, n_effective
+    end function
+    integer(c_int) function gfm_pnet_destroy(net) bind(C, name="gfm_pnet_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: net
     end function
     type(c_ptr) function gfm_hip_last_error() bind(C, name="gfm_hip_last_error")
       import :: c_ptr
