@@ -43,6 +43,7 @@ class GfmCtrl(ct.Structure):
         ("pcorr_sweeps", I32), ("pcorr_fallback", I32),
         ("pcorr_tol", D), ("pcorr_resid_start", D), ("pcorr_alpha", D), ("pcorr_gain", D),
         ("pcorr_guess_accept", D),
+        ("q_wall_bottom", D), ("q_wall_top", D),
     ]
 
     def copy(self):

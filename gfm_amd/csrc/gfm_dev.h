@@ -79,6 +79,7 @@ enum {
   SC_GFIN0, SC_GFIN, SC_FACTOR, SC_FADJ, SC_QEXH, SC_QREACT, SC_G0REF,
   SC_TAVE_NUM, SC_TAVE_DEN, SC_TVOL_NUM, SC_TVOL_DEN, SC_TMAX,
   SC_QWALL,  // total convective heat to walls and melt surface (W)
+  SC_QWALL_LO, SC_QWALL_HI,  // the parts through bottom (k-) and top (k+) faces
   SC_ITR,  // itr_gas of the running iteration (read by kernels so a captured graph is reusable)
   SC_PG_RMS, SC_PG_ALPHA, SC_PG_FALLBACK,  // learned p' guess: input scale, guess scale, fallback flag
   SC_PG_COS2,  // learned p' guess: fraction of ||BS||^2 the scaled guess removes

@@ -270,6 +270,7 @@ static void pull_scalars(gfm_ctx *c, gfm_ctrl *k, const double *sc) {
   k->gfex = sc[SC_GFEX]; k->fuel_out = sc[SC_FUEL_OUT]; k->tmean_ex = sc[SC_TMEAN_EX];
   k->gfin0 = sc[SC_GFIN0]; k->gfin = sc[SC_GFIN]; k->factor = sc[SC_FACTOR]; k->fadj = sc[SC_FADJ];
   k->q_exhaust = sc[SC_QEXH]; k->q_react = sc[SC_QREACT]; k->q_wall = sc[SC_QWALL];
+  k->q_wall_bottom = sc[SC_QWALL_LO]; k->q_wall_top = sc[SC_QWALL_HI];
   if (sc[SC_TAVE_DEN] > 0.0) {
     k->tave = sc[SC_TAVE_NUM] * k->t0 / sc[SC_TAVE_DEN];
     k->tave_vol = sc[SC_TVOL_NUM] * k->t0 / sc[SC_TVOL_DEN];
@@ -304,10 +305,11 @@ static int lines_lw(int npl, size_t word) {
   const int full = word == 4 ? 32 : 16;
   return npl >= full ? full : 2;
 }
-static int lines_nu(int nmax) { return nmax <= 64 ? 2 : nmax <= 96 ? 3 : nmax <= 128 ? 4 : 8; }
+static int lines_nu(int nmax) { return nmax <= 96 ? (nmax <= 64 ? 2 : 3) : nmax <= 128 ? 4 : nmax <= 160 ? 5 : nmax <= 192 ? 6 : 8; }
 static size_t lines_shm(int lw, int nmax, size_t word) { return 3 * (size_t)lw * (nmax | 1) * word; }
-#define LINES_INSTANCES(X_) \
-  X_(32, 2) X_(32, 3) X_(32, 4) X_(32, 8) X_(16, 2) X_(16, 3) X_(16, 4) X_(16, 8) X_(2, 2) X_(2, 3) X_(2, 4) X_(2, 8)
+#define LINES_INSTANCES(X_)                                                                       \
+  X_(32, 2) X_(32, 3) X_(32, 4) X_(32, 5) X_(32, 6) X_(32, 8) X_(16, 2) X_(16, 3) X_(16, 4) X_(16, 5) \
+  X_(16, 6) X_(16, 8) X_(2, 2) X_(2, 3) X_(2, 4) X_(2, 5) X_(2, 6) X_(2, 8)
 template <typename T>
 static void launch_lines(gfm_ctx *c, int lw, int nu, unsigned nblk, size_t shm, const T *xt, T *xy, T *xz, int pi0, int pdi,
                          int npl, int jb, int je, int kb, int ke) {
@@ -581,6 +583,8 @@ static int gflow(gfm_ctx *c, int stage) {
     if (nelg == 8) {
       hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->part, (int)box_blocks(cen), 1, SC_QREACT, 0);
       hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->part + c->dc.pstride, (int)box_blocks(cen), 1, SC_QWALL, 0);
+      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->part + 2 * c->dc.pstride, (int)box_blocks(cen), 1, SC_QWALL_LO, 0);
+      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->part + 3 * c->dc.pstride, (int)box_blocks(cen), 1, SC_QWALL_HI, 0);
     }
     if (k.want_pre_residuals) residual(c, d.fz, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_GF_PRE + l - 1, 0);
     adlbl3(c, d.fz, nelg, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2);

@@ -851,8 +851,9 @@ __device__ double Q_CONV_FACE(long long q, double ds, double twk) {
   }
   return fmax(g.h_g, h_t) * (tgas - twk);
 }
-// wall sink of one open cell, C/Source.f90:473-509; qw gets sum q_conv*area
-__device__ double WALL_SINK(int i, int j, int k, double &qw) {
+// wall sink of one open cell, C/Source.f90:473-509; qw gets sum q_conv*area,
+// qlo / qhi the part through the cell's bottom (k-) / top (k+) face
+__device__ double WALL_SINK(int i, int j, int k, double &qw, double &qlo, double &qhi) {
   double s = 0.0;
   long long q = lidx(i, j, k);
   for (int d = 0; d < 6; d++) {
@@ -865,6 +866,8 @@ __device__ double WALL_SINK(int i, int j, int k, double &qw) {
     else { dsw = fabs(Z(k + dk / 2) - Z(k)); dsc = DZ(k); area = DX(i) * DR(j); }
     double qc = Q_CONV_FACE(q, dsw, T_(i + di, j + dj, k + dk) * g.t0);
     qw += qc * area;
+    if (d == 4) qlo += qc * area;
+    if (d == 5) qhi += qc * area;
     s += qc / (dsc * g.r0 * g.qr0);
   }
   return s;
@@ -885,7 +888,7 @@ __global__ void k_copy_gf_fz(Box b, int l) {
 __global__ void k_scalar_assemble(Box b, int l, int nelg, int igs, double sgm, double *part) {
   const int itr_gas = (int)g.scal[SC_ITR];
   int i, j, k;
-  double qr = 0.0, qw = 0.0;
+  double qr = 0.0, qw = 0.0, qlo = 0.0, qhi = 0.0;
   if (box_pt(b, i, j, k)) {
     long long q = lidx(i, j, k);
     const double fz = g.fz[q];
@@ -951,7 +954,7 @@ __global__ void k_scalar_assemble(Box b, int l, int nelg, int igs, double sgm, d
         double vol0 = VOL_C(i, j, k) * g.vol0;
         qr = hr * g.qr0 * vol0;
         if (g.gray_radiation) bs += GRAY_QNET(i, j, k) / (vol0 * g.qr0);
-        if (g.isothermal_walls) bs -= WALL_SINK(i, j, k, qw);
+        if (g.isothermal_walls) bs -= WALL_SINK(i, j, k, qw, qlo, qhi);
         if (itr_gas > 1) bs = g.rf_hsource * bs + (1.0 - g.rf_hsource) * g.bsh[q];
         g.bsh[q] = bs;
       } else if (nelg == 9) {  // k, C/Source.f90:529-572
@@ -1012,8 +1015,13 @@ __global__ void k_scalar_assemble(Box b, int l, int nelg, int igs, double sgm, d
     }
   }
   if (nelg == 8) {
-    double v = wave_sum(qr), w = wave_sum(qw);
-    if (threadIdx.x == 0) { part[block_lin()] = v; part[g.pstride + block_lin()] = w; }
+    double v = wave_sum(qr), w = wave_sum(qw), lo = wave_sum(qlo), hi = wave_sum(qhi);
+    if (threadIdx.x == 0) {
+      part[block_lin()] = v;
+      part[g.pstride + block_lin()] = w;
+      part[2 * g.pstride + block_lin()] = lo;
+      part[3 * g.pstride + block_lin()] = hi;
+    }
   }
 }
 __global__ void k_bound_store_gf(Box b, int l) {

@@ -144,6 +144,8 @@ typedef struct gfm_ctrl {
   double pcorr_alpha;             /* out: scale of the learned guess */
   double pcorr_gain;              /* out: fraction of ||BS||^2 the scaled learned guess removes */
   double pcorr_guess_accept;      /* in: acceptance ratio of the learned guess (see above) */
+  double q_wall_bottom, q_wall_top; /* out: the parts of q_wall through cells' bottom (k-) and top (k+) faces:
+                                       glass/floor and crown */
 } gfm_ctrl;
 
 /* Opaque device-side context. */

@@ -481,6 +481,8 @@ static double wall_sink(int i, int j, int k, double *qw) {
     else { dsw = fabs(Z(k + dk / 2) - Z(k)); dsc = DZ(k); area = DX(i) * DR(j); }
     double q = q_conv_face(i, j, k, dsw, T(i + di, j + dj, k + dk) * C->t0);
     *qw += q * area;
+    if (d == 4) C->q_wall_bottom += q * area;
+    if (d == 5) C->q_wall_top += q * area;
     s += q / (dsc * C->r0 * C->qr0);
   }
   return s;
@@ -568,6 +570,8 @@ static void SOURCE(int mi1, int mi3, int nj1, int nj3, int lk1, int lk3) {
     if (nelg == 8) {
       C->q_react = 0.0;
       C->q_wall = 0.0;
+      C->q_wall_bottom = 0.0;
+      C->q_wall_top = 0.0;
       for (int i = mi1; i <= mi3; i += 2)
         for (int j = nj1; j <= nj3; j += 2)
           for (int k = lk1; k <= lk3; k += 2) {

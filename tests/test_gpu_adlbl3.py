@@ -29,12 +29,13 @@ def _setup(order, sweeps, nx=20, precision=0, ny=14, nz=12):
     return c, box
 
 
-@pytest.mark.parametrize("order,nx,ny", [(1, 20, 14), (0, 20, 14), (1, 72, 14), (1, 34, 100), (1, 34, 140)])
+@pytest.mark.parametrize("order,nx,ny", [(1, 20, 14), (0, 20, 14), (1, 72, 14), (1, 34, 100), (1, 34, 140), (1, 20, 180),
+                                         (1, 20, 220)])
 def test_adlbl3_matches_oracle(order, nx, ny):
     """Three sweeps from xt = 0 in the box: every value within 1e-12 of the
     oracle's (relative to the largest).  nx=72 (36 planes per colour) runs
-    the tiled line kernel; ny=100 and 140 give y-lines longer than 96 and
-    128 rows (4 and 8 rows per lane)."""
+    the tiled line kernel; ny = 100, 140, 180, 220 give y-lines of 4, 5, 6
+    and 8 rows per lane."""
     c, box = _setup(order, 3, nx, ny=ny)
     ap, as_, bs, xt, xs, sl = manufactured(c, box, seed=11)
     xo = xt.copy(order="F")

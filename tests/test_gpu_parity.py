@@ -118,6 +118,9 @@ def test_wall_heat_flux_matches():
     o, g, h_o, h_g = _run_pair(1, 3)
     assert o.ctrl.q_wall != 0.0
     assert g.ctrl.q_wall == pytest.approx(o.ctrl.q_wall, rel=1e-9)
+    assert o.ctrl.q_wall_top != 0.0 and o.ctrl.q_wall_bottom != 0.0
+    assert g.ctrl.q_wall_top == pytest.approx(o.ctrl.q_wall_top, rel=1e-9)
+    assert g.ctrl.q_wall_bottom == pytest.approx(o.ctrl.q_wall_bottom, rel=1e-9)
     assert g.ctrl.q_react == pytest.approx(o.ctrl.q_react, rel=1e-9)
 
 
