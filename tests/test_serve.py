@@ -1,0 +1,24 @@
+"""Batched serving (gfm_amd/serve.py): the sweep is reproducible and drawn
+from the stated sizes and operating ranges (CPU); a small sweep completes
+in a pool of worker processes sharing the GPU (GPU)."""
+import pytest
+
+from gfm_amd import serve
+
+
+def test_sweep_is_seeded_and_in_range():
+    a, b = serve.sweep(24, 5), serve.sweep(24, 5)
+    assert a == b
+    assert serve.sweep(24, 6) != a
+    for j in a:
+        assert tuple(j["grid"]) in serve.SIZES
+        assert 20.0 <= j["fuel_u"] <= 40.0 and 900.0 <= j["air_t"] <= 1300.0
+
+
+@pytest.mark.gpu
+def test_small_sweep_completes():
+    cases = [dict(index=q, grid=(64, 32, 24), fuel_u=25.0 + 5 * q, air_t=1000.0, seed=10 + q) for q in range(3)]
+    summary, recs = serve.serve(cases, workers=2, cap=4, precision="mixed", log=lambda m: print(m, flush=True))
+    assert summary["failed"] == 0
+    assert all(r["iterations"] == 4 for r in recs)
+    assert summary["cell_updates"] == 3 * 4 * 64 * 32 * 24
