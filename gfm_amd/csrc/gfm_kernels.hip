@@ -1501,6 +1501,27 @@ __device__ __forceinline__ void wave_sync() {
 
 template <typename T> struct LinePair { T a, c; };
 
+// One PCR step at a stride s = 32*DU >= 32: the rows e-s and e+s of row
+// e = l32 + 32u are this lane's rows u-DU and u+DU, so the step needs no
+// exchange between lanes.
+template <typename T, int NU, int DU>
+__device__ __forceinline__ void pcr_lane_step(T (&ra)[NU], T (&rc)[NU], T (&rd)[NU], int l32, int n, bool act) {
+  T na[NU], nc[NU], nd[NU];
+#pragma unroll
+  for (int u = 0; u < NU; u++) {
+    const int e = l32 + 32 * u;
+    na[u] = ra[u]; nc[u] = rc[u]; nd[u] = rd[u];
+    if (!act || e >= n) continue;
+    T am = 0, cm = 0, dm = 0, apl = 0, cp = 0, dp = 0;
+    if (u - DU >= 0) { am = ra[u - DU]; cm = rc[u - DU]; dm = rd[u - DU]; }
+    if (u + DU < NU && e + 32 * DU < n) { apl = ra[u + DU]; cp = rc[u + DU]; dp = rd[u + DU]; }
+    const T r = line_rcp<T>((T)1 - ra[u] * cm - rc[u] * apl);
+    na[u] = -ra[u] * am * r; nc[u] = -rc[u] * cp * r; nd[u] = (rd[u] - ra[u] * dm - rc[u] * dp) * r;
+  }
+#pragma unroll
+  for (int u = 0; u < NU; u++) { ra[u] = na[u]; rc[u] = nc[u]; rd[u] = nd[u]; }
+}
+
 #define LINES_NT 512
 template <int LW, typename T, int NU>
 __global__ __launch_bounds__(LINES_NT) void k_lines(const T *__restrict__ xt, T *__restrict__ xy, T *__restrict__ xz,
@@ -1573,7 +1594,8 @@ __global__ __launch_bounds__(LINES_NT) void k_lines(const T *__restrict__ xt, T 
       ra[u] = 0; rc[u] = 0; rd[u] = 0;
       if (act && e < n) { const LinePair<T> p = ac_[e]; ra[u] = p.a; rc[u] = p.c; rd[u] = d_[e]; }
     }
-    for (int s = 1; s < n; s <<= 1) {
+    int s = 1;
+    for (; s < n && s < 32; s <<= 1) {
 #pragma unroll
       for (int u = 0; u < NU; u++) {
         const int e = l32 + 32 * u;
@@ -1593,6 +1615,16 @@ __global__ __launch_bounds__(LINES_NT) void k_lines(const T *__restrict__ xt, T 
         if (act && e < n) { ac_[e] = LinePair<T>{ra[u], rc[u]}; d_[e] = rd[u]; }
       }
       wave_sync();
+    }
+    if (s < n) {  // strides 32, 64, 128: within a lane
+      if (NU > 1) pcr_lane_step<T, NU, 1>(ra, rc, rd, l32, n, act);
+      if (NU > 2 && 64 < n) pcr_lane_step<T, NU, 2>(ra, rc, rd, l32, n, act);
+      if (NU > 4 && 128 < n) pcr_lane_step<T, NU, 4>(ra, rc, rd, l32, n, act);
+#pragma unroll
+      for (int u = 0; u < NU; u++) {
+        const int e = l32 + 32 * u;
+        if (act && e < n) d_[e] = rd[u];
+      }
     }
   }
   __syncthreads();
