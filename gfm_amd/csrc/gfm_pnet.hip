@@ -39,13 +39,14 @@ __host__ __device__ __forceinline__ long long pgrid_cells(const PGrid &q) { retu
 // The input is the channel concatenation [xa (ca channels), xb (cb)], xa
 // optionally read at half resolution (nearest upsampling by 2).  Output
 // channels [0, split) go to ya (row length split), the rest to yb.
-// A wavefront computes 32 voxels x 16*NCT channels with
+// A wavefront computes MT*16 voxels x 16*NCT channels with
 // v_mfma_f32_16x16x32_bf16: A = 16 voxels x 32 input channels (each lane 8
 // consecutive channels of one voxel, one 16-byte load), B = 32 input
-// channels x 16 output channels from w, D rows = voxels, columns = output
-// channels.  256 threads = 4 wavefronts = 128 voxels per workgroup.
+// channels x 16 output channels from w (each B fragment feeds MT MFMAs),
+// D rows = voxels, columns = output channels.  256 threads = 4 wavefronts
+// = 64*MT voxels per workgroup.
 // ---------------------------------------------------------------------
-template <int NCT>
+template <int NCT, int MT>
 __global__ __launch_bounds__(256) void k_pconv(const pbf *__restrict__ xa, int ca, int up_a, const pbf *__restrict__ xb,
                                                int cb, const pbf *__restrict__ w, const float *__restrict__ bias,
                                                pbf *__restrict__ ya, int split, pbf *__restrict__ yb, int relu, PGrid pg,
@@ -54,11 +55,11 @@ __global__ __launch_bounds__(256) void k_pconv(const pbf *__restrict__ xa, int c
   const int cin = ca + cb;
   const long long V = pgrid_cells(pg);
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, r16 = l & 15, kg = l >> 4;
-  const long long vb = (long long)blockIdx.x * 128 + wave * 32;
-  int vx[2], vy[2], vz[2], vbt[2];
-  bool vin[2];
+  const long long vb = (long long)blockIdx.x * (64 * MT) + wave * (16 * MT);
+  int vx[MT], vy[MT], vz[MT], vbt[MT];
+  bool vin[MT];
 #pragma unroll
-  for (int mt = 0; mt < 2; mt++) {
+  for (int mt = 0; mt < MT; mt++) {
     long long v = vb + mt * 16 + r16;
     vin[mt] = v < V;
     long long r = vin[mt] ? v : 0;
@@ -68,9 +69,9 @@ __global__ __launch_bounds__(256) void k_pconv(const pbf *__restrict__ xa, int c
     vbt[mt] = (int)(r / pg.D);
   }
   const int Dh = pg.D >> 1, Hh = pg.H >> 1, Wh = pg.W >> 1;
-  pf4 acc[2][NCT];
+  pf4 acc[MT][NCT];
 #pragma unroll
-  for (int mt = 0; mt < 2; mt++)
+  for (int mt = 0; mt < MT; mt++)
 #pragma unroll
     for (int ct = 0; ct < NCT; ct++) acc[mt][ct] = pf4{0.f, 0.f, 0.f, 0.f};
   pbf8 zero8;
@@ -78,19 +79,19 @@ __global__ __launch_bounds__(256) void k_pconv(const pbf *__restrict__ xa, int c
   for (int e = 0; e < 8; e++) zero8[e] = __builtin_bit_cast(__bf16, (pbf)0);
   for (int t = 0; t < ntap; t++) {
     const int dz = (ntap == 1) ? 0 : t / 9 - 1, dy = (ntap == 1) ? 0 : (t / 3) % 3 - 1, dx = (ntap == 1) ? 0 : t % 3 - 1;
-    long long na[2], nb[2];
+    long long na[MT], nb[MT];
 #pragma unroll
-    for (int mt = 0; mt < 2; mt++) {
+    for (int mt = 0; mt < MT; mt++) {
       const int z = vz[mt] + dz, y = vy[mt] + dy, x = vx[mt] + dx;
       const bool in = vin[mt] && z >= 0 && z < pg.D && y >= 0 && y < pg.H && x >= 0 && x < pg.W;
       nb[mt] = in ? (((long long)vbt[mt] * pg.D + z) * pg.H + y) * pg.W + x : -1;
       na[mt] = !in ? -1 : up_a ? (((long long)vbt[mt] * Dh + (z >> 1)) * Hh + (y >> 1)) * Wh + (x >> 1) : nb[mt];
     }
     for (int ch = 0; ch < cin; ch += 32) {
-      pbf8 A[2];
+      const bool fa = ch < ca;
+      pbf8 A[MT];
 #pragma unroll
-      for (int mt = 0; mt < 2; mt++) {
-        const bool fa = ch < ca;
+      for (int mt = 0; mt < MT; mt++) {
         const long long nv = fa ? na[mt] : nb[mt];
         const pbf *src = fa ? xa + nv * ca + ch : xb + nv * cb + (ch - ca);
         A[mt] = nv >= 0 ? *reinterpret_cast<const pbf8 *>(src + 8 * kg) : zero8;
@@ -98,13 +99,13 @@ __global__ __launch_bounds__(256) void k_pconv(const pbf *__restrict__ xa, int c
 #pragma unroll
       for (int ct = 0; ct < NCT; ct++) {
         const pbf8 Bf = *reinterpret_cast<const pbf8 *>(w + ((long long)t * CO + ct * 16 + r16) * cin + ch + 8 * kg);
-        acc[0][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], Bf, acc[0][ct], 0, 0, 0);
-        acc[1][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], Bf, acc[1][ct], 0, 0, 0);
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++) acc[mt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[mt], Bf, acc[mt][ct], 0, 0, 0);
       }
     }
   }
 #pragma unroll
-  for (int mt = 0; mt < 2; mt++)
+  for (int mt = 0; mt < MT; mt++)
 #pragma unroll
     for (int ct = 0; ct < NCT; ct++) {
       const int co = ct * 16 + r16;
@@ -271,19 +272,34 @@ __global__ void k_prelu_bwd(pbf *__restrict__ gr, const pbf *__restrict__ y, lon
   long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i < n && !(pbf_to_f(y[i]) > 0.f)) gr[i] = 0;
 }
-// network input: [v][32] bf16 from x[v][2] (float), channels 2..31 zero
+// network input: [v][32] bf16 from x[v][2] (float), channels 2..31 zero;
+// one thread writes one voxel's 64 bytes
 __global__ void k_pinput(const float *__restrict__ x, pbf *__restrict__ x0, long long V) {
-  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i >= V * 32) return;
-  const int c = (int)(i & 31);
-  x0[i] = c < 2 ? pbf_from_f(x[(i >> 5) * 2 + c]) : (pbf)0;
+  long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  const float2 in = reinterpret_cast<const float2 *>(x)[v];
+  uint4 *o = reinterpret_cast<uint4 *>(x0 + v * 32);
+  o[0] = make_uint4((uint32_t)pbf_from_f(in.x) | ((uint32_t)pbf_from_f(in.y) << 16), 0u, 0u, 0u);
+  o[1] = make_uint4(0u, 0u, 0u, 0u);
+  o[2] = make_uint4(0u, 0u, 0u, 0u);
+  o[3] = make_uint4(0u, 0u, 0u, 0u);
 }
-// output layer y[v] = w . d0[v] + b (FP32)
+// output layer y[v] = w . d0[v] + b (FP32); C is a multiple of 8, read 16
+// bytes at a time
 __global__ void k_pout(const pbf *__restrict__ d0, int C, const float *__restrict__ wb, float *__restrict__ y, long long V) {
   long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (v >= V) return;
+  const uint4 *row = reinterpret_cast<const uint4 *>(d0 + v * C);
   float s = 0.f;
-  for (int c = 0; c < C; c++) s += wb[c] * pbf_to_f(d0[v * C + c]);
+  for (int q = 0; q < C / 8; q++) {
+    const uint4 u = row[q];
+    const uint32_t wd[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+      s += wb[8 * q + 2 * h] * pbf_to_f((pbf)(wd[h] & 0xFFFFu));
+      s += wb[8 * q + 2 * h + 1] * pbf_to_f((pbf)(wd[h] >> 16));
+    }
+  }
   y[v] = s + wb[C];
 }
 // loss terms: dy = 2 m (y - t) / n_open, per-block partial sums of m (y-t)^2
@@ -450,20 +466,21 @@ static int pnet_cast(gfm_pnet *n) {
   return GFM_OK;
 }
 
-template <int NCT>
+// voxel tiles per wavefront: accumulators stay near 64 registers
+template <int NCT, int MT>
 static void pconv_launch(hipStream_t s, const pbf *xa, int ca, int up_a, const pbf *xb, int cb, const pbf *w, const float *bias,
                          pbf *ya, int split, pbf *yb, int relu, PGrid pg, int ntap) {
   const long long V = pgrid_cells(pg);
-  hipLaunchKernelGGL(k_pconv<NCT>, dim3(pblocks(V, 128)), dim3(256), 0, s, xa, ca, up_a, xb, cb, w, bias, ya, split, yb, relu,
-                     pg, ntap);
+  hipLaunchKernelGGL((k_pconv<NCT, MT>), dim3(pblocks(V, 64 * MT)), dim3(256), 0, s, xa, ca, up_a, xb, cb, w, bias, ya, split,
+                     yb, relu, pg, ntap);
 }
 static int pconv(hipStream_t s, const pbf *xa, int ca, int up_a, const pbf *xb, int cb, const pbf *w, const float *bias,
                  pbf *ya, int split, pbf *yb, int co, int relu, PGrid pg, int ntap) {
   switch (co) {
-    case 32: pconv_launch<2>(s, xa, ca, up_a, xb, cb, w, bias, ya, split, yb, relu, pg, ntap); break;
-    case 64: pconv_launch<4>(s, xa, ca, up_a, xb, cb, w, bias, ya, split, yb, relu, pg, ntap); break;
-    case 96: pconv_launch<6>(s, xa, ca, up_a, xb, cb, w, bias, ya, split, yb, relu, pg, ntap); break;
-    case 128: pconv_launch<8>(s, xa, ca, up_a, xb, cb, w, bias, ya, split, yb, relu, pg, ntap); break;
+    case 32: pconv_launch<2, 4>(s, xa, ca, up_a, xb, cb, w, bias, ya, split, yb, relu, pg, ntap); break;
+    case 64: pconv_launch<4, 4>(s, xa, ca, up_a, xb, cb, w, bias, ya, split, yb, relu, pg, ntap); break;
+    case 96: pconv_launch<6, 2>(s, xa, ca, up_a, xb, cb, w, bias, ya, split, yb, relu, pg, ntap); break;
+    case 128: pconv_launch<8, 2>(s, xa, ca, up_a, xb, cb, w, bias, ya, split, yb, relu, pg, ntap); break;
     default: set_err("pnet: unsupported channel count %d", co); return GFM_E_ARG;
   }
   return GFM_OK;
@@ -484,7 +501,7 @@ static int pnet_forward_dev(gfm_pnet *n, hipStream_t s, int batch, int d, int h,
   PConv *k = n->conv;
   const float *P = n->p;
   int rc;
-  hipLaunchKernelGGL(k_pinput, dim3(pblocks(L[0].V * 32)), dim3(256), 0, s, n->xin, n->x0, L[0].V);
+  hipLaunchKernelGGL(k_pinput, dim3(pblocks(L[0].V)), dim3(256), 0, s, n->xin, n->x0, L[0].V);
 #define PCONV(l, xa, up, xb, ya, pg)                                                                                \
   if ((rc = pconv(s, xa, k[l].ca, up, xb, k[l].cb, k[l].wf, P + k[l].off_b, ya, k[l].co, nullptr, k[l].co, 1, pg, 27))) \
     return rc;
