@@ -59,6 +59,13 @@ struct gfm_ctx {
   double *xt2 = nullptr, *xt3 = nullptr, *part = nullptr, *stage = nullptr;
   float *xtf = nullptr;  // FP32 working copy of the unknown of ADLBL3 (mixed precision)
   int16_t *stage16 = nullptr;
+  // lattice positions of the GFLOW wall treatment: open with a wall
+  // neighbour, and wall (Fortran linear indices)
+  long long *wall_open = nullptr, *wall_pos = nullptr;
+  int n_wall_open = 0, n_wall_pos = 0;
+  // cells of EXTR: inlets and exits, walls (Fortran linear indices)
+  long long *extr_io = nullptr, *extr_wall = nullptr;
+  int n_extr_io = 0, n_extr_wall = 0;
   long long nlat = 0, ncls = 0;
   hipStream_t stream = nullptr;
   double sgm = 1.0;
@@ -173,6 +180,50 @@ static void fill_devcase_scalars(gfm_ctx *c, const gfm_ctrl *k) {
   d.gray_radiation = k->gray_radiation;
   d.wt0 = 1.0 / (0.5 / WF + 0.0 / WCO2 + 0.0 / WH2O + 0.49 / WO2 + 0.01 / WN2);
   d.stoich = 2.0 * WO2 / WF;
+}
+
+// Positions of the GFLOW wall treatment (C/Gflow.f90:16-130), whose loops
+// run over i, j, k = 2..MP, 2..NP, 2..LP: the open positions (IBCELL <= 0)
+// with a wall (IBCELL == 1) among their six lattice neighbours, where
+// theta and the wall functions act, and the wall positions, where GAMA is
+// zeroed.  A neighbour beyond MP, NP or LP counts as open, as the device
+// holds 0 there.
+static void wall_lists(const gfm_ctrl *k, const int16_t *ibf, std::vector<long long> &wo, std::vector<long long> &wp) {
+  const int MPh = k->mp, NPh = k->np, LPh = k->lp;
+  auto lin = [&](int i, int j, int kk) -> long long { return (((long long)kk - 1) * NPh + (j - 1)) * MPh + (i - 1); };
+  auto ib = [&](int i, int j, int kk) -> int {
+    if (i > MPh || j > NPh || kk > LPh) return 0;
+    return ibf[lin(i, j, kk)];
+  };
+  wo.clear();
+  wp.clear();
+  for (int kk = 2; kk <= LPh; kk++)
+    for (int j = 2; j <= NPh; j++)
+      for (int i = 2; i <= MPh; i++) {
+        const int b = ib(i, j, kk);
+        if (b == 1) { wp.push_back(lin(i, j, kk)); continue; }
+        if (b >= 1) continue;
+        if (ib(i - 1, j, kk) == 1 || ib(i + 1, j, kk) == 1 || ib(i, j - 1, kk) == 1 || ib(i, j + 1, kk) == 1 ||
+            ib(i, j, kk - 1) == 1 || ib(i, j, kk + 1) == 1)
+          wo.push_back(lin(i, j, kk));
+      }
+}
+
+// Cells of EXTR(1) (C/Extr.f90:31-250) in the cell box [2, MP]x[2, NP]x
+// [2, LP] (step 2): inlets and exits (IBCELL mod 10 = 2 or 3) and walls
+// (IBCELL mod 10 = 1).
+static void extr_lists(const gfm_ctrl *k, const int16_t *ibf, std::vector<long long> &io, std::vector<long long> &wl) {
+  const int MPh = k->mp, NPh = k->np, LPh = k->lp;
+  io.clear();
+  wl.clear();
+  for (int kk = 2; kk <= LPh; kk += 2)
+    for (int j = 2; j <= NPh; j += 2)
+      for (int i = 2; i <= MPh; i += 2) {
+        const long long f = (((long long)kk - 1) * NPh + (j - 1)) * MPh + (i - 1);
+        const int b = ibf[f] % 10;
+        if (b == 2 || b == 3) io.push_back(f);
+        else if (b == 1) wl.push_back(f);
+      }
 }
 
 // sticky INTP weights: replay the reference's INTP loop nest on the host
@@ -348,12 +399,22 @@ static void plane_op(gfm_ctx *c, T *xt, T *x2, T *x3, int pi0, int pdi, int npl,
   const size_t shm = lines_shm(lw, nmax, sizeof(T));
   const unsigned nblk = (unsigned)(ntile * (nz + ny));
   const long long tot = (long long)npl * nzl * nyl;
+  // adjacent same-colour planes form aligned pairs: the first plane's
+  // x position (lattice_index formula) is even
+  const int c0 = pi0 >> 1;
+  const bool pair_avg = pdi == 4 && npl > 1 && ((((c0 & 1) * c->dc.g.HX + (c0 >> 1) + 31) & 1) == 0);
   for (int l = 0; l < 2; l++) {
     if (c->prof.on) prof_begin(c);
     launch_lines<T>(c, lw, nu, nblk, shm, xt, x2, x3, pi0, pdi, npl, jb, je, kb, ke);
     if (c->prof.on) { prof_end(c, "k_lines", line_bytes); prof_begin(c); }
-    hipLaunchKernelGGL(k_line_avg<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, xt, x2, x3, pi0, pdi,
-                       npl, jb, nzl, kb, nyl);
+    if (pair_avg) {
+      const long long tot2 = (long long)((npl + 1) / 2) * nzl * nyl;
+      hipLaunchKernelGGL(k_line_avg2<T>, dim3((unsigned)((tot2 + 255) / 256)), dim3(256), 0, c->stream, xt, x2, x3, pi0, pdi,
+                         npl, jb, nzl, kb, nyl);
+    } else {
+      hipLaunchKernelGGL(k_line_avg<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, xt, x2, x3, pi0, pdi,
+                         npl, jb, nzl, kb, nyl);
+    }
     if (c->prof.on) prof_end(c, "k_line_avg", cells * 3.0 * sizeof(T));
   }
 }
@@ -376,7 +437,7 @@ static void sweeps(gfm_ctx *c, T *xt, T *x2, T *x3, int nim, int ib, int ie, int
 static void residual(gfm_ctx *c, const double *xt, int ib, int ie, int jb, int je, int kb, int ke, int slot, int irc) {
   Box b = make_box(ib, ie, jb, je, kb, ke, 2);
   LAUNCH(k_residual, b, xt, c->part, irc);
-  hipLaunchKernelGGL(k_residual_finalize, dim3(1), dim3(256), 0, c->stream, c->part, (int)box_blocks(b), SC_RESID0 + slot, irc);
+  hipLaunchKernelGGL(k_residual_finalize, dim3(1), dim3(FIN_NT), 0, c->stream, c->part, (int)box_blocks(b), SC_RESID0 + slot, irc);
 }
 // nim sweeps of ADLBL3 on xt over the box (FP32 working copy in mixed precision)
 static void adlbl3_sweeps(gfm_ctx *c, double *xt, int nim, int ib, int ie, int jb, int je, int kb, int ke) {
@@ -414,7 +475,7 @@ static int pcorr_guess(gfm_ctx *c, const Box &cen) {
   const long long V = (long long)nx * ny * nz;
   const long long nb = box_blocks(cen);
   LAUNCH(k_pg_gather, cen, n->xin, c->part);
-  hipLaunchKernelGGL(k_pg_rms, dim3(1), dim3(256), 0, c->stream, c->part, (int)nb);
+  hipLaunchKernelGGL(k_pg_rms, dim3(1), dim3(FIN_NT), 0, c->stream, c->part, (int)nb);
   hipLaunchKernelGGL(k_pg_norm, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, c->stream, n->xin, V);
   if (c->prof.on) prof_begin(c);
   int rc = pnet_forward_dev(n, c->stream, 1, nz, ny, nx);
@@ -422,7 +483,7 @@ static int pcorr_guess(gfm_ctx *c, const Box &cen) {
   if (c->prof.on) prof_end(c, "pnet_forward", 0.0);
   LAUNCH(k_pg_scatter, cen, n->y);
   LAUNCH(k_pg_ag, cen, c->part);
-  hipLaunchKernelGGL(k_pg_alpha, dim3(1), dim3(256), 0, c->stream, c->part, (int)nb);
+  hipLaunchKernelGGL(k_pg_alpha, dim3(1), dim3(FIN_NT), 0, c->stream, c->part, (int)nb);
   LAUNCH(k_pg_scale, cen);
   residual(c, d.fz, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_PCG, 0);
   const double accept = k.pcorr_guess_accept > 0.0 ? k.pcorr_guess_accept : 1.0;
@@ -458,6 +519,8 @@ static int pcorr_relax(gfm_ctx *c) {
   return GFM_OK;
 }
 
+static void extr(gfm_ctx *c);
+
 // ---------------------------------------------------------------------
 // GFLOW, C/Gflow.f90:6-408
 // ---------------------------------------------------------------------
@@ -467,7 +530,6 @@ static int gflow(gfm_ctx *c, int stage) {
   const gfm_ctrl &k = c->ctrl;
   DevCase &d = c->dc;
   const int MPh = k.mp, NPh = k.np, LPh = k.lp;
-  Box all1 = make_box(2, MPh, 2, NPh, 2, LPh, 1);
   Box cen = make_box(4, MPh - 2, 4, NPh - 2, 4, LPh - 2, 2);
   Box cen_all = make_box(2, MPh, 2, NPh, 2, LPh, 2);
   Box mom[4];
@@ -477,10 +539,19 @@ static int gflow(gfm_ctx *c, int stage) {
   }
 
   if (stage != 2) {
-  LAUNCH(k_theta_wall, all1);
-  LAUNCH(k_gama_wall_zero, all1);
-  double rt20 = k.reyg2 * sqrt(k.epsr) * pow(CMU, 0.25);
-  LAUNCH(k_wall_function, all1, rt20);
+  // C/Gflow.f90:16-130 over the positions where it changes a field
+  const double rt20 = k.reyg2 * sqrt(k.epsr) * pow(CMU, 0.25);
+  if (c->n_wall_open > 0) {
+    const dim3 gw((unsigned)((c->n_wall_open + 255) / 256));
+    hipLaunchKernelGGL(k_theta_wall, gw, dim3(256), 0, c->stream, c->wall_open, c->n_wall_open);
+  }
+  if (c->n_wall_pos > 0)
+    hipLaunchKernelGGL(k_gama_wall_zero, dim3((unsigned)((c->n_wall_pos + 255) / 256)), dim3(256), 0, c->stream, c->wall_pos,
+                       c->n_wall_pos);
+  if (c->n_wall_open > 0) {
+    const dim3 gw((unsigned)((c->n_wall_open + 255) / 256));
+    hipLaunchKernelGGL(k_wall_function, gw, dim3(256), 0, c->stream, c->wall_open, c->n_wall_open, rt20);
+  }
 
   // ---- GSLV3V ----
   for (int iu1 = 1; iu1 <= 3; iu1++) {
@@ -505,7 +576,7 @@ static int gflow(gfm_ctx *c, int stage) {
   }
   // p'
     LAUNCH(k_pcorr_source, cen, c->part);
-    hipLaunchKernelGGL(k_pcorr_finalize, dim3(1), dim3(256), 0, c->stream, c->part, box_blocks(cen));
+    hipLaunchKernelGGL(k_pcorr_finalize, dim3(1), dim3(FIN_NT), 0, c->stream, c->part, box_blocks(cen));
     LAUNCH(k_zero_fz, cen_all);
     if (k.pcorr_mode == GFM_PCORR_REFERENCE) {
       adlbl3(c, d.fz, 5, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2);
@@ -531,7 +602,7 @@ static int gflow(gfm_ctx *c, int stage) {
     // order and sum each into the scalar block
     for (int q = 0; q < 3; q++) {
       LAUNCH(k_extrv, ex[q], q + 1, 0, c->part);
-      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->part, (int)box_blocks(ex[q]), 5, SC_GFEX, q > 0);
+      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(FIN_NT), 0, c->stream, c->part, (int)box_blocks(ex[q]), 5, SC_GFEX, q > 0);
     }
   }
   hipLaunchKernelGGL(k_factor, dim3(1), dim3(1), 0, c->stream);
@@ -581,10 +652,10 @@ static int gflow(gfm_ctx *c, int stage) {
     if (e == 0 || c->sgm != sigma[e]) { c->sgm = sigma[e]; igs = 0; }
     LAUNCH(k_scalar_assemble, cen, l, nelg, igs, c->sgm, c->part);
     if (nelg == 8) {
-      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->part, (int)box_blocks(cen), 1, SC_QREACT, 0);
-      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->part + c->dc.pstride, (int)box_blocks(cen), 1, SC_QWALL, 0);
-      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->part + 2 * c->dc.pstride, (int)box_blocks(cen), 1, SC_QWALL_LO, 0);
-      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->part + 3 * c->dc.pstride, (int)box_blocks(cen), 1, SC_QWALL_HI, 0);
+      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(FIN_NT), 0, c->stream, c->part, (int)box_blocks(cen), 1, SC_QREACT, 0);
+      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(FIN_NT), 0, c->stream, c->part + c->dc.pstride, (int)box_blocks(cen), 1, SC_QWALL, 0);
+      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(FIN_NT), 0, c->stream, c->part + 2 * c->dc.pstride, (int)box_blocks(cen), 1, SC_QWALL_LO, 0);
+      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(FIN_NT), 0, c->stream, c->part + 3 * c->dc.pstride, (int)box_blocks(cen), 1, SC_QWALL_HI, 0);
     }
     if (k.want_pre_residuals) residual(c, d.fz, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_GF_PRE + l - 1, 0);
     adlbl3(c, d.fz, nelg, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2);
@@ -595,11 +666,8 @@ static int gflow(gfm_ctx *c, int stage) {
   // properties, T_ave, EXTR(1), INTP
   LAUNCH(k_props, cen_all);
   LAUNCH(k_tave, cen, c->part);
-  hipLaunchKernelGGL(k_tave_finalize, dim3(1), dim3(256), 0, c->stream, c->part, (int)box_blocks(cen));
-  LAUNCH(k_extr_inout, cen_all, c->part);
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->part, (int)box_blocks(cen_all), 1, SC_GFIN0, 0);
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->part + c->dc.pstride, (int)box_blocks(cen_all), 1, SC_QEXH, 0);
-  LAUNCH(k_extr_wall, cen_all);
+  hipLaunchKernelGGL(k_tave_finalize, dim3(1), dim3(FIN_NT), 0, c->stream, c->part, (int)box_blocks(cen));
+  extr(c);
 
   {
     Box b = make_box(2, MPh, 2, NPh, 3, LPh - 1, 2); LAUNCH(k_intp, b, 3);
@@ -613,6 +681,17 @@ static int gflow(gfm_ctx *c, int stage) {
   }
   HCHECK(hipGetLastError());
   return GFM_OK;
+}
+
+// EXTR(1) over the inlet/exit and wall cell lists, C/Extr.f90:31-250
+static void extr(gfm_ctx *c) {
+  const unsigned nbio = (unsigned)((c->n_extr_io + 63) / 64 > 0 ? (c->n_extr_io + 63) / 64 : 1);
+  hipLaunchKernelGGL(k_extr_inout, dim3(nbio), dim3(64), 0, c->stream, c->extr_io, c->n_extr_io, c->part);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(FIN_NT), 0, c->stream, c->part, (int)nbio, 1, SC_GFIN0, 0);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(FIN_NT), 0, c->stream, c->part + c->dc.pstride, (int)nbio, 1, SC_QEXH, 0);
+  if (c->n_extr_wall > 0)
+    hipLaunchKernelGGL(k_extr_wall, dim3((unsigned)((c->n_extr_wall + 255) / 256)), dim3(256), 0, c->stream, c->extr_wall,
+                       c->n_extr_wall);
 }
 
 static int initsv_tail(gfm_ctx *c) {
@@ -635,11 +714,7 @@ static int initsv_tail(gfm_ctx *c) {
   b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 3, 2, 1, 1, 1);
   for (int pi = 0; pi < 2; pi++)
     for (int pj = 0; pj < 2; pj++) { b = make_box(2 + pi, pi ? MPh - 1 : MPh, 2 + pj, pj ? NPh - 1 : NPh, 4, LPh, 2); LAUNCH(k_intpv, b, 3, 3, pi, pj, 0); }
-  Box cen_all = make_box(2, MPh, 2, NPh, 2, LPh, 2);
-  LAUNCH(k_extr_inout, cen_all, c->part);
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->part, (int)box_blocks(cen_all), 1, SC_GFIN0, 0);
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->part + c->dc.pstride, (int)box_blocks(cen_all), 1, SC_QEXH, 0);
-  LAUNCH(k_extr_wall, cen_all);
+  extr(c);
   b = make_box(2, MPh, 2, NPh, 3, LPh - 1, 2); LAUNCH(k_intp, b, 3);
   b = make_box(2, MPh, 3, NPh - 1, 2, LPh, 2); LAUNCH(k_intp, b, 2);
   b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intp, b, 2);
@@ -770,6 +845,19 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
   std::vector<int8_t> code;
   intp_codes(k, h->ibcell, code, gg);
   HCHECK(hipMemcpy(icode, code.data(), N, hipMemcpyHostToDevice));
+  {
+    std::vector<long long> ls[4];
+    wall_lists(k, h->ibcell, ls[0], ls[1]);
+    extr_lists(k, h->ibcell, ls[2], ls[3]);
+    long long **dst[4] = {&c->wall_open, &c->wall_pos, &c->extr_io, &c->extr_wall};
+    int *cnt[4] = {&c->n_wall_open, &c->n_wall_pos, &c->n_extr_io, &c->n_extr_wall};
+    for (int q = 0; q < 4; q++) {
+      if (ls[q].size() > 0x7fffffff) { set_err("position list too long"); return fail(GFM_E_ARG); }
+      *cnt[q] = (int)ls[q].size();
+      if ((rc = dalloc(c, dst[q], (long long)ls[q].size() + 1))) return fail(rc);
+      if (!ls[q].empty()) HCHECK(hipMemcpy(*dst[q], ls[q].data(), ls[q].size() * sizeof(long long), hipMemcpyHostToDevice));
+    }
+  }
   if ((rc = upload_state(c, k, h))) return fail(rc);
   *out = c;
   return GFM_OK;

@@ -98,6 +98,60 @@ __device__ __forceinline__ long long block_lin() {
   return blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
 }
 
+// ---------------- one-block finalisation of per-block partials ----------------
+// A FIN_NT-thread block reduces nb partials of each of NQ slots (slot q at
+// part + q*stride) to one value: a sum, or a maximum for the slots whose
+// bit is set in maxmask (those values are >= 0, so 0 is the identity).
+// Thread t takes partials t + FIN_NT*m into four accumulators used round
+// robin (four loads in flight per slot), then the wavefronts and the
+// block combine in a tree: a fixed order on every call.  The results are
+// valid in thread 0.
+#define FIN_NT 1024
+template <int NQ>
+__device__ void fin_reduce(const double *part, long long stride, int nb, unsigned maxmask, double (&r)[NQ]) {
+  __shared__ double sh[NQ][FIN_NT / 64];
+  double a[NQ][4];
+#pragma unroll
+  for (int q = 0; q < NQ; q++)
+#pragma unroll
+    for (int u = 0; u < 4; u++) a[q][u] = 0.0;
+  int b = threadIdx.x;
+  for (; b + 3 * FIN_NT < nb; b += 4 * FIN_NT) {
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        const double v = part[q * stride + b + u * FIN_NT];
+        a[q][u] = ((maxmask >> q) & 1) ? fmax(a[q][u], v) : a[q][u] + v;
+      }
+  }
+  for (; b < nb; b += FIN_NT) {
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      const double v = part[q * stride + b];
+      a[q][0] = ((maxmask >> q) & 1) ? fmax(a[q][0], v) : a[q][0] + v;
+    }
+  }
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const bool mx = (maxmask >> q) & 1;
+    double v = mx ? fmax(fmax(a[q][0], a[q][1]), fmax(a[q][2], a[q][3])) : (a[q][0] + a[q][1]) + (a[q][2] + a[q][3]);
+    v = mx ? wave_max(v) : wave_sum(v);
+    if (l == 0) sh[q][w] = v;
+  }
+  __syncthreads();
+  if (w == 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      const bool mx = (maxmask >> q) & 1;
+      const double v = l < FIN_NT / 64 ? sh[q][l] : 0.0;
+      r[q] = mx ? wave_max(v) : wave_sum(v);
+    }
+  }
+  __syncthreads();  // sh is free for the next call
+}
+
 // ======================================================================
 // geometry, C/Grid3.f90:310-471 (no half-blocked cells)
 // ======================================================================
@@ -260,9 +314,23 @@ __device__ double ZY1(const double *x, const double *y, double x1, int n) {  // 
 // ======================================================================
 // GFLOW wall treatment, C/Gflow.f90:16-130
 // ======================================================================
-__global__ void k_theta_wall(Box b) {
+// The three stages run over lists of lattice positions built once from
+// IBCELL on the host (Fortran linear index (i-1) + MP*((j-1) + NP*(k-1))):
+// the open positions with a wall neighbour (theta, wall functions) and the
+// wall positions (GAMA = 0).  Every other position of the reference's
+// loops leaves the fields unchanged.
+__device__ __forceinline__ bool list_pt(const long long *lst, int n, int &i, int &j, int &k) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return false;
+  const long long f = lst[t];
+  i = (int)(f % MP) + 1;
+  j = (int)((f / MP) % NP) + 1;
+  k = (int)(f / ((long long)MP * NP)) + 1;
+  return true;
+}
+__global__ void k_theta_wall(const long long *lst, int n) {
   int i, j, k;
-  if (!box_pt(b, i, j, k)) return;
+  if (!list_pt(lst, n, i, j, k)) return;
   if (IBC(i, j, k) >= 1) return;
   double g1, g2;
   if (IBC(i - 1, j, k) == 1) { g1 = (X(i) - X(i - 1)) * (X(i) - X(i - 1)); g2 = (X(i + 2) - X(i - 1)) * (X(i + 2) - X(i - 1)); TH(i - 1, j, k) = TH(i, j, k) * (g2 / (g2 - g1)) + TH(i + 2, j, k) * (g1 / (g1 - g2)); }
@@ -272,16 +340,16 @@ __global__ void k_theta_wall(Box b) {
   if (IBC(i, j, k - 1) == 1) { g1 = (Z(k) - Z(k - 1)) * (Z(k) - Z(k - 1)); g2 = (Z(k + 2) - Z(k - 1)) * (Z(k + 2) - Z(k - 1)); TH(i, j, k - 1) = TH(i, j, k) * (g2 / (g2 - g1)) + TH(i, j, k + 2) * (g1 / (g1 - g2)); }
   if (IBC(i, j, k + 1) == 1) { g1 = (Z(k) - Z(k + 1)) * (Z(k) - Z(k + 1)); g2 = (Z(k - 2) - Z(k + 1)) * (Z(k - 2) - Z(k + 1)); TH(i, j, k + 1) = TH(i, j, k) * (g2 / (g2 - g1)) + TH(i, j, k - 2) * (g1 / (g1 - g2)); }
 }
-__global__ void k_gama_wall_zero(Box b) {
+__global__ void k_gama_wall_zero(const long long *lst, int n) {
   int i, j, k;
-  if (!box_pt(b, i, j, k)) return;
+  if (!list_pt(lst, n, i, j, k)) return;
   if (IBC(i, j, k) == 1) GAMA(i, j, k) = 0.0;
 }
 // Wall functions.  Each wall lattice position next to open fluid has one
 // open neighbour in a rectangular furnace, so the updates do not collide.
-__global__ void k_wall_function(Box b, double rt20) {
+__global__ void k_wall_function(const long long *lst, int n, double rt20) {
   int i, j, k;
-  if (!box_pt(b, i, j, k)) return;
+  if (!list_pt(lst, n, i, j, k)) return;
   if (IBC(i, j, k) >= 1) return;
   double blvm = rt20 * DN(i, j, k) * sqrt(GF(i, j, k, IK));
   double vgam = 1.0 / g.reyg2;
@@ -582,27 +650,22 @@ __global__ void k_pcorr_source(Box b, double *part) {
     ((long long *)part)[2 * nb + bl] = key;
   }
 }
-__global__ void k_pcorr_finalize(const double *part, long long nb) {
-  __shared__ double ss[256];
-  __shared__ double sm[256];
-  __shared__ long long sk[256];
-  double s = 0.0, m = 0.0;
+__global__ __launch_bounds__(FIN_NT) void k_pcorr_finalize(const double *part, long long nb) {
+  __shared__ double sm[FIN_NT];
+  __shared__ long long sk[FIN_NT];
+  double rs[1];
+  fin_reduce<1>(part, 0, (int)nb, 0u, rs);
+  double m = 0.0;
   long long key = 0x7fffffffffffffffLL;
-  double s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  long long b0 = threadIdx.x;
-  for (; b0 + 768 < nb; b0 += 1024) { s += part[b0]; s1 += part[b0 + 256]; s2 += part[b0 + 512]; s3 += part[b0 + 768]; }
-  for (; b0 < nb; b0 += 256) s += part[b0];
-  s = (s + s1) + (s2 + s3);
-  for (long long b = threadIdx.x; b < nb; b += 256) {
+  for (long long b = threadIdx.x; b < nb; b += FIN_NT) {
     double r = part[nb + b];
     long long k2 = ((const long long *)part)[2 * nb + b];
     if (r > m || (r == m && k2 < key)) { m = r; key = k2; }
   }
-  ss[threadIdx.x] = s; sm[threadIdx.x] = m; sk[threadIdx.x] = key;
+  sm[threadIdx.x] = m; sk[threadIdx.x] = key;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
+  for (int o = FIN_NT / 2; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o) {
-      ss[threadIdx.x] += ss[threadIdx.x + o];
       double r2 = sm[threadIdx.x + o];
       long long k2 = sk[threadIdx.x + o];
       if (r2 > sm[threadIdx.x] || (r2 == sm[threadIdx.x] && k2 < sk[threadIdx.x])) { sm[threadIdx.x] = r2; sk[threadIdx.x] = k2; }
@@ -610,7 +673,7 @@ __global__ void k_pcorr_finalize(const double *part, long long nb) {
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    g.scal[SC_RES_MASS] = ss[0] / g.ncells;
+    g.scal[SC_RES_MASS] = rs[0] / g.ncells;
     g.scal[SC_RES_MAX] = sm[0];
     long long kk = sk[0];
     if (sm[0] > 0.0) {
@@ -713,24 +776,12 @@ __global__ void k_extrv(Box b, int iu, int mex, double *part) {
   }
 }
 
-// generic finalize: sum nb partials of slot q (stride 4096) into scal[dst]
-__global__ void k_sum_partials(const double *part, int nb, int nq, int dst0, int accumulate) {
-  __shared__ double ss[256];
+// generic finalize: sum nb partials of each of nq slots into scal[dst0+q]
+__global__ __launch_bounds__(FIN_NT) void k_sum_partials(const double *part, int nb, int nq, int dst0, int accumulate) {
   for (int q = 0; q < nq; q++) {
-    double s = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    const double *pp = part + (long long)q * g.pstride;
-    int b = threadIdx.x;
-    for (; b + 768 < nb; b += 1024) { s += pp[b]; s1 += pp[b + 256]; s2 += pp[b + 512]; s3 += pp[b + 768]; }
-    for (; b < nb; b += 256) s += pp[b];
-    s = (s + s1) + (s2 + s3);
-    ss[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-      if ((int)threadIdx.x < o) ss[threadIdx.x] += ss[threadIdx.x + o];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) g.scal[dst0 + q] = accumulate ? g.scal[dst0 + q] + ss[0] : ss[0];
-    __syncthreads();
+    double r[1];
+    fin_reduce<1>(part + (long long)q * g.pstride, 0, nb, 0u, r);
+    if (threadIdx.x == 0) g.scal[dst0 + q] = accumulate ? g.scal[dst0 + q] + r[0] : r[0];
   }
 }
 // FLX1D's GFIN and the outlet flow factor, C/Flx1d.f90:75-77, C/Gslv3v.f90:213-231
@@ -1065,20 +1116,11 @@ __global__ void k_tave(Box b, double *part) {
   double m = wave_max(s[4]);
   if (threadIdx.x == 0) part[4LL * g.pstride + block_lin()] = m;
 }
-__global__ void k_tave_finalize(const double *part, int nb) {
-  __shared__ double ss[256];
-  for (int n = 0; n < 5; n++) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nb; b += 256) s = (n == 4) ? fmax(s, part[(long long)n * g.pstride + b]) : s + part[(long long)n * g.pstride + b];
-    ss[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-      if ((int)threadIdx.x < o) ss[threadIdx.x] = (n == 4) ? fmax(ss[threadIdx.x], ss[threadIdx.x + o]) : ss[threadIdx.x] + ss[threadIdx.x + o];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) g.scal[SC_TAVE_NUM + n] = ss[0];
-    __syncthreads();
-  }
+__global__ __launch_bounds__(FIN_NT) void k_tave_finalize(const double *part, int nb) {
+  double r[5];
+  fin_reduce<5>(part, g.pstride, nb, 1u << 4, r);
+  if (threadIdx.x == 0)
+    for (int n = 0; n < 5; n++) g.scal[SC_TAVE_NUM + n] = r[n];
 }
 
 // ======================================================================
@@ -1112,11 +1154,13 @@ __device__ __forceinline__ void EXTR1F(double &f1, double &f2, int i, int j, int
   else if (ix == 1) { double d = X(i2) - X(i1); if (d != 0.0) { int i0 = (ibc0 == 2) ? (i + i1) / 2 : i; f2 = (X(i0) - X(i1)) / d; } }
   f1 = 1.0 - f2;
 }
-// inlets and exits (partials: GFIN0, q_exhaust)
-__global__ void k_extr_inout(Box b, double *part) {
+// inlets and exits (partials: GFIN0, q_exhaust), over the list of the
+// inlet and exit cells of the cell box [2, MP]x[2, NP]x[2, LP] (step 2);
+// every other cell of that box leaves the fields unchanged.
+__global__ void k_extr_inout(const long long *lst, int n, double *part) {
   int i, j, k;
   double s0 = 0.0, s1 = 0.0;
-  if (box_pt(b, i, j, k)) {
+  if (list_pt(lst, n, i, j, k)) {
     int ibc0 = IBC(i, j, k) % 10;
     if (ibc0 == 2 || ibc0 == 3) {
       int i1, j1, k1, i2, j2, k2;
@@ -1161,11 +1205,12 @@ __global__ void k_extr_inout(Box b, double *part) {
     }
   }
   double v0 = wave_sum(s0), v1 = wave_sum(s1);
-  if (threadIdx.x == 0) { part[block_lin()] = v0; part[g.pstride + block_lin()] = v1; }
+  if (threadIdx.x == 0) { part[blockIdx.x] = v0; part[g.pstride + blockIdx.x] = v1; }
 }
-__global__ void k_extr_wall(Box b) {
+// walls, over the list of the wall cells of the same box
+__global__ void k_extr_wall(const long long *lst, int n) {
   int i, j, k;
-  if (!box_pt(b, i, j, k)) return;
+  if (!list_pt(lst, n, i, j, k)) return;
   if ((IBC(i, j, k) % 10) != 1) return;
   int i1, j1, k1, i2, j2, k2;
   int ix = EXTIDX(1, i, j, k, i1, j1, k1, i2, j2, k2);
@@ -1263,28 +1308,15 @@ __global__ void k_residual(Box b, const double *xt, double *part, int irc) {
     if (threadIdx.x == 0) { part[block_lin()] = v; part[g.pstride + block_lin()] = m; }
   }
 }
-__global__ void k_residual_finalize(const double *part, int nb, int slot, int irc) {
-  __shared__ double ss[256];
-  __shared__ double sm[256];
-  double s = 0.0, m = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  int b = threadIdx.x;
+__global__ __launch_bounds__(FIN_NT) void k_residual_finalize(const double *part, int nb, int slot, int irc) {
   if (!irc) {
-    for (; b + 768 < nb; b += 1024) { s += part[b]; s1 += part[b + 256]; s2 += part[b + 512]; s3 += part[b + 768]; }
-  }
-  for (; b < nb; b += 256) {
-    s += part[b];
-    if (irc) m = fmax(m, part[g.pstride + b]);
-  }
-  s = (s + s1) + (s2 + s3);
-  ss[threadIdx.x] = s; sm[threadIdx.x] = m;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) { ss[threadIdx.x] += ss[threadIdx.x + o]; sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + o]); }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    if (!irc) g.scal[slot] = fmax(ss[0] / g.ncells, 1.0e-16);
-    else { g.scal[slot] = ss[0] / g.ncells; g.scal[slot + 1] = sm[0]; }
+    double r[1];
+    fin_reduce<1>(part, 0, nb, 0u, r);
+    if (threadIdx.x == 0) g.scal[slot] = fmax(r[0] / g.ncells, 1.0e-16);
+  } else {
+    double r[2];
+    fin_reduce<2>(part, g.pstride, nb, 2u, r);
+    if (threadIdx.x == 0) { g.scal[slot] = r[0] / g.ncells; g.scal[slot + 1] = r[1]; }
   }
 }
 
@@ -1316,19 +1348,12 @@ __global__ void k_pg_gather(Box b, float *x, double *part) {
   if (threadIdx.x == 0) part[block_lin()] = w;
 }
 // scale of the input: root mean square of BS/AP over open rows
-__global__ void k_pg_rms(const double *part, int nb) {
-  __shared__ double ss[256];
-  double s = 0.0;
-  for (int q = threadIdx.x; q < nb; q += 256) s += part[q];
-  ss[threadIdx.x] = s;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) ss[threadIdx.x] += ss[threadIdx.x + o];
-    __syncthreads();
-  }
+__global__ __launch_bounds__(FIN_NT) void k_pg_rms(const double *part, int nb) {
+  double r[1];
+  fin_reduce<1>(part, 0, nb, 0u, r);
   if (threadIdx.x == 0) {
-    const double r = sqrt(ss[0] / fmax((double)g.ncells, 1.0));
-    g.scal[SC_PG_RMS] = r > 0.0 ? r : 1.0;
+    const double rr = sqrt(r[0] / fmax((double)g.ncells, 1.0));
+    g.scal[SC_PG_RMS] = rr > 0.0 ? rr : 1.0;
     g.scal[SC_PG_ALPHA] = 0.0;
     g.scal[SC_PG_FALLBACK] = 0.0;
     g.scal[SC_PG_COS2] = 0.0;
@@ -1369,22 +1394,11 @@ __global__ void k_pg_ag(Box b, double *part) {
 }
 // alpha = BS.(Ag) / (Ag).(Ag), which leaves ||BS||^2 (1 - cos^2) of the
 // residual, cos^2 = (BS.Ag)^2 / (||BS||^2 ||Ag||^2) (kept for diagnostics).
-__global__ void k_pg_alpha(const double *part, int nb) {
-  __shared__ double s1[256], s2[256], s3[256];
-  double a = 0.0, c = 0.0, e = 0.0;
-  for (int q = threadIdx.x; q < nb; q += 256) { a += part[q]; c += part[g.pstride + q]; e += part[2 * g.pstride + q]; }
-  s1[threadIdx.x] = a; s2[threadIdx.x] = c; s3[threadIdx.x] = e;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) {
-      s1[threadIdx.x] += s1[threadIdx.x + o];
-      s2[threadIdx.x] += s2[threadIdx.x + o];
-      s3[threadIdx.x] += s3[threadIdx.x + o];
-    }
-    __syncthreads();
-  }
+__global__ __launch_bounds__(FIN_NT) void k_pg_alpha(const double *part, int nb) {
+  double r[3];
+  fin_reduce<3>(part, g.pstride, nb, 0u, r);
   if (threadIdx.x == 0) {
-    const double bag = s1[0], agag = s2[0], bb = s3[0];
+    const double bag = r[0], agag = r[1], bb = r[2];
     const double cos2 = (agag > 0.0 && bb > 0.0) ? bag * bag / (agag * bb) : 0.0;
     g.scal[SC_PG_COS2] = cos2;
     g.scal[SC_PG_ALPHA] = agag > 0.0 ? bag / agag : 0.0;
@@ -1437,6 +1451,30 @@ __global__ void k_line_avg(T *xt, const T *x2, const T *x3, int pi0, int pdi, in
   int i = pi0 + pdi * pl, j = jb + 2 * jj, k = kb + 2 * kk;
   long long q = lidx(i, j, k);
   xt[q] = (x2[q] + x3[q]) / (T)2;
+}
+
+// The same for two adjacent planes per thread (zebra order: planes of one
+// colour are adjacent in memory); the first plane's offset is even, so
+// the pair is one aligned 2-element vector.
+template <typename T>
+__global__ void k_line_avg2(T *xt, const T *x2, const T *x3, int pi0, int pdi, int npl, int jb, int nj, int kb, int nk) {
+  typedef T T2 __attribute__((ext_vector_type(2)));
+  const int nph = (npl + 1) / 2;
+  long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long tot = (long long)nph * nj * nk;
+  if (t >= tot) return;
+  int ph = (int)(t % nph);
+  long long r = t / nph;
+  int jj = (int)(r % nj), kk = (int)(r / nj);
+  int pl = 2 * ph;
+  int i = pi0 + pdi * pl, j = jb + 2 * jj, k = kb + 2 * kk;
+  long long q = lidx(i, j, k);
+  if (pl + 1 < npl) {
+    const T2 a = *reinterpret_cast<const T2 *>(x2 + q), b = *reinterpret_cast<const T2 *>(x3 + q);
+    *reinterpret_cast<T2 *>(xt + q) = (a + b) / (T)2;
+  } else {
+    xt[q] = (x2[q] + x3[q]) / (T)2;
+  }
 }
 
 // FP32 working copy of an unknown over a box (mixed precision) and back.
