@@ -56,7 +56,7 @@ struct gfm_ctx {
   gfm_ctrl ctrl;
   DevCase dc;
   std::vector<void *> allocs;
-  double *xt2 = nullptr, *xt3 = nullptr, *part = nullptr, *stage = nullptr;
+  double *xt2 = nullptr, *xt3 = nullptr, *xt4 = nullptr, *xt5 = nullptr, *part = nullptr, *stage = nullptr;
   float *xtf = nullptr;  // FP32 working copy of the unknown of ADLBL3 (mixed precision)
   int16_t *stage16 = nullptr;
   // lattice positions of the GFLOW wall treatment: open with a wall
@@ -362,13 +362,11 @@ static size_t lines_shm(int lw, int nmax, size_t word) { return 3 * (size_t)lw *
   X_(32, 2) X_(32, 3) X_(32, 4) X_(32, 5) X_(32, 6) X_(32, 8) X_(16, 2) X_(16, 3) X_(16, 4) X_(16, 5) \
   X_(16, 6) X_(16, 8) X_(2, 2) X_(2, 3) X_(2, 4) X_(2, 5) X_(2, 6) X_(2, 8)
 template <typename T>
-static void launch_lines(gfm_ctx *c, int lw, int nu, unsigned nblk, size_t shm, const T *xt, T *xy, T *xz, int pi0, int pdi,
-                         int npl, int jb, int je, int kb, int ke) {
-#define LINES_CASE(LW_, NU_)                                                                                        \
-  if (lw == LW_ && nu == NU_) {                                                                                     \
-    hipLaunchKernelGGL((k_lines<LW_, T, NU_>), dim3(nblk), dim3(LINES_NT), shm, c->stream, xt, xy, xz, pi0, pdi, npl, \
-                       jb, je, kb, ke);                                                                             \
-    return;                                                                                                         \
+static void launch_lines(gfm_ctx *c, int lw, int nu, unsigned nblk, size_t shm, const LineIO<T> &io) {
+#define LINES_CASE(LW_, NU_)                                                                           \
+  if (lw == LW_ && nu == NU_) {                                                                        \
+    hipLaunchKernelGGL((k_lines<LW_, T, NU_>), dim3(nblk), dim3(LINES_NT), shm, c->stream, io);        \
+    return;                                                                                            \
   }
   LINES_INSTANCES(LINES_CASE)
 #undef LINES_CASE
@@ -382,57 +380,93 @@ static int set_lines_attr(size_t shm) {
   return GFM_OK;
 }
 
-// One zebra half-step or one reference-order plane: the two passes of ADLZ
-// on npl planes i = pi0 + pdi*p, each pass a y/z line solve from the same
-// XT followed by the average.
+// One line-kernel pass: y-line and z-line solves on the planes
+// i = pi0 + pdi*p (p < npl) of the box, from the current values io
+// describes, into io.xy and io.xz.
 template <typename T>
-static void plane_op(gfm_ctx *c, T *xt, T *x2, T *x3, int pi0, int pdi, int npl, int jb, int je, int kb, int ke) {
-  const int nyl = (ke - kb) / 2 + 1, nzl = (je - jb) / 2 + 1;  // y-lines per plane, z-lines per plane
-  const int ny = nzl, nz = nyl;                                  // rows of a y-line, of a z-line
-  const double cells = (double)npl * nyl * nzl;
-  // algorithmic bytes: 8 coefficients, one read and one write of the unknown per row and direction
-  const double line_bytes = cells * 2.0 * (8.0 * sizeof(T) + 2.0 * sizeof(T));
-  const int lw = lines_lw(npl, sizeof(T));
-  const int ntile = (npl + lw - 1) / lw;
+static void line_pass(gfm_ctx *c, const LineIO<T> &io) {
+  const int nyl = (io.ke - io.kb) / 2 + 1, nzl = (io.je - io.jb) / 2 + 1;  // y-lines per plane, z-lines per plane
+  const int ny = nzl, nz = nyl;                                            // rows of a y-line, of a z-line
+  const double cells = (double)io.npl * nyl * nzl;
+  // algorithmic bytes per row and direction: 8 coefficients, the current
+  // value of the unknown (one word from xt, or the two words of a pair whose
+  // average the kernel forms) and the written solution
+  const double line_bytes = cells * 2.0 * (8.0 + (io.own2 ? 2.0 : 1.0) + 1.0) * sizeof(T);
+  const int lw = lines_lw(io.npl, sizeof(T));
+  const int ntile = (io.npl + lw - 1) / lw;
   const int nmax = ny > nz ? ny : nz;
-  const int nu = lines_nu(nmax);
   const size_t shm = lines_shm(lw, nmax, sizeof(T));
-  const unsigned nblk = (unsigned)(ntile * (nz + ny));
-  const long long tot = (long long)npl * nzl * nyl;
-  // adjacent same-colour planes form aligned pairs: the first plane's
+  const unsigned nblk = (unsigned)(8 * ntile * ((nz + 7) / 8 + (ny + 7) / 8));  // XCD-balanced order (k_lines)
+  if (c->prof.on) prof_begin(c);
+  launch_lines<T>(c, lw, lines_nu(nmax), nblk, shm, io);
+  if (c->prof.on) prof_end(c, "k_lines", line_bytes);
+}
+
+// XT = (X2 + X3)/2 on the planes i = pi0 + pdi*p (p < npl) of the box,
+// C/Adlbl3.f90:134-138.
+template <typename T>
+static void line_avg(gfm_ctx *c, T *xt, const T *x2, const T *x3, int pi0, int pdi, int npl, int jb, int je, int kb, int ke) {
+  const int nyl = (ke - kb) / 2 + 1, nzl = (je - jb) / 2 + 1;
+  const double cells = (double)npl * nyl * nzl;
+  // adjacent same-colour planes form aligned pairs when the first plane's
   // x position (lattice_index formula) is even
   const int c0 = pi0 >> 1;
-  const bool pair_avg = pdi == 4 && npl > 1 && ((((c0 & 1) * c->dc.g.HX + (c0 >> 1) + 31) & 1) == 0);
-  for (int l = 0; l < 2; l++) {
-    if (c->prof.on) prof_begin(c);
-    launch_lines<T>(c, lw, nu, nblk, shm, xt, x2, x3, pi0, pdi, npl, jb, je, kb, ke);
-    if (c->prof.on) { prof_end(c, "k_lines", line_bytes); prof_begin(c); }
-    if (pair_avg) {
-      const long long tot2 = (long long)((npl + 1) / 2) * nzl * nyl;
-      hipLaunchKernelGGL(k_line_avg2<T>, dim3((unsigned)((tot2 + 255) / 256)), dim3(256), 0, c->stream, xt, x2, x3, pi0, pdi,
-                         npl, jb, nzl, kb, nyl);
-    } else {
-      hipLaunchKernelGGL(k_line_avg<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, xt, x2, x3, pi0, pdi,
-                         npl, jb, nzl, kb, nyl);
-    }
-    if (c->prof.on) prof_end(c, "k_line_avg", cells * 3.0 * sizeof(T));
+  const bool vec = pdi == 4 && npl > 1 && ((((c0 & 1) * c->dc.g.HX + (c0 >> 1) + 31) & 1) == 0);
+  if (c->prof.on) prof_begin(c);
+  if (vec) {
+    const long long tot2 = (long long)((npl + 1) / 2) * nzl * nyl;
+    hipLaunchKernelGGL(k_line_avg2<T>, dim3((unsigned)((tot2 + 255) / 256)), dim3(256), 0, c->stream, xt, x2, x3, pi0, pdi, npl,
+                       jb, nzl, kb, nyl);
+  } else {
+    const long long tot = (long long)npl * nzl * nyl;
+    hipLaunchKernelGGL(k_line_avg<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, xt, x2, x3, pi0, pdi, npl,
+                       jb, nzl, kb, nyl);
   }
+  if (c->prof.on) prof_end(c, "k_line_avg", cells * 3.0 * sizeof(T));
 }
+
+// nim sweeps of ADLBL3 on xt over the box.  Reference order: plane by
+// plane, forward then backward, each plane the two passes of ADLZ, a y/z
+// line solve from the same XT followed by the average.  Zebra order: the
+// same on all planes of one colour at once, colours 0, 1, 0, 1 per sweep;
+// the averages are not stored between passes: each pass reads the current
+// values of a colour's planes as the average of the pair its last pass
+// wrote (x2/x3 or x4/x5), and xt receives them once, after the last pass.
 template <typename T>
-static void sweeps(gfm_ctx *c, T *xt, T *x2, T *x3, int nim, int ib, int ie, int jb, int je, int kb, int ke) {
+static void sweeps(gfm_ctx *c, T *xt, T *x2, T *x3, T *x4, T *x5, int nim, int ib, int ie, int jb, int je, int kb, int ke) {
   const int npl = (ie - ib) / 2 + 1;
-  for (int s = 0; s < nim; s++) {
-    if (c->ctrl.sweep_order == GFM_ORDER_REFERENCE) {
-      for (int i = ib; i <= ie; i += 2) plane_op<T>(c, xt, x2, x3, i, 0, 1, jb, je, kb, ke);
-      for (int i = ie; i >= ib; i -= 2) plane_op<T>(c, xt, x2, x3, i, 0, 1, jb, je, kb, ke);
-    } else {
-      for (int half = 0; half < 4; half++) {
-        int col = half & 1;
-        int n = (npl - col + 1) / 2;
-        if (n > 0) plane_op<T>(c, xt, x2, x3, ib + 2 * col, 4, n, jb, je, kb, ke);
+  if (c->ctrl.sweep_order == GFM_ORDER_REFERENCE) {
+    for (int s = 0; s < nim; s++)
+      for (int dirn = 0; dirn < 2; dirn++)
+        for (int q = 0; q < npl; q++) {
+          const int i = dirn == 0 ? ib + 2 * q : ie - 2 * q;
+          for (int l = 0; l < 2; l++) {
+            const LineIO<T> io{xt, nullptr, nullptr, nullptr, nullptr, x2, x3, i, 0, 1, ib, ie, jb, je, kb, ke};
+            line_pass<T>(c, io);
+            line_avg<T>(c, xt, x2, x3, i, 0, 1, jb, je, kb, ke);
+          }
+        }
+    return;
+  }
+  T *pr[2][2] = {{x2, x3}, {x4, x5}};
+  int cur[2] = {-1, -1};  // pair holding each colour's current values (-1: xt)
+  const int ncol[2] = {(npl + 1) / 2, npl / 2};
+  for (int s = 0; s < nim; s++)
+    for (int half = 0; half < 4; half++) {
+      const int col = half & 1;
+      if (ncol[col] == 0) continue;
+      for (int l = 0; l < 2; l++) {
+        const int own = cur[col], oth = cur[1 - col];
+        const int tgt = own == 0 ? 1 : 0;  // never the pair this pass reads its own planes from
+        const LineIO<T> io{xt, own >= 0 ? pr[own][0] : nullptr, own >= 0 ? pr[own][1] : nullptr,
+                           oth >= 0 ? pr[oth][0] : nullptr, oth >= 0 ? pr[oth][1] : nullptr, pr[tgt][0], pr[tgt][1],
+                           ib + 2 * col, 4, ncol[col], ib, ie, jb, je, kb, ke};
+        line_pass<T>(c, io);
+        cur[col] = tgt;
       }
     }
-  }
+  for (int col = 0; col < 2; col++)
+    if (cur[col] >= 0) line_avg<T>(c, xt, pr[cur[col]][0], pr[cur[col]][1], ib + 2 * col, 4, ncol[col], jb, je, kb, ke);
 }
 static void residual(gfm_ctx *c, const double *xt, int ib, int ie, int jb, int je, int kb, int ke, int slot, int irc) {
   Box b = make_box(ib, ie, jb, je, kb, ke, 2);
@@ -442,7 +476,7 @@ static void residual(gfm_ctx *c, const double *xt, int ib, int ie, int jb, int j
 // nim sweeps of ADLBL3 on xt over the box (FP32 working copy in mixed precision)
 static void adlbl3_sweeps(gfm_ctx *c, double *xt, int nim, int ib, int ie, int jb, int je, int kb, int ke) {
   if (!c->dc.mixed) {
-    sweeps<double>(c, xt, c->xt2, c->xt3, nim, ib, ie, jb, je, kb, ke);
+    sweeps<double>(c, xt, c->xt2, c->xt3, c->xt4, c->xt5, nim, ib, ie, jb, je, kb, ke);
   } else {
     // FP32 sweeps on a working copy of the box and the neighbours it reads
     // (one lattice step of the class outside the box); rows the sweeps pin
@@ -450,8 +484,8 @@ static void adlbl3_sweeps(gfm_ctx *c, double *xt, int nim, int ib, int ie, int j
     Box ext = make_box(ib - 2, ie + 2, jb - 2, je + 2, kb - 2, ke + 2, 2);
     Box box = make_box(ib, ie, jb, je, kb, ke, 2);
     LAUNCH(k_box_narrow, ext, xt, c->xtf);
-    sweeps<float>(c, c->xtf, reinterpret_cast<float *>(c->xt2), reinterpret_cast<float *>(c->xt3), nim, ib, ie, jb, je, kb,
-                  ke);
+    sweeps<float>(c, c->xtf, reinterpret_cast<float *>(c->xt2), reinterpret_cast<float *>(c->xt3),
+                  reinterpret_cast<float *>(c->xt4), reinterpret_cast<float *>(c->xt5), nim, ib, ie, jb, je, kb, ke);
     LAUNCH(k_box_widen, box, c->xtf, xt);
   }
 }
@@ -799,6 +833,8 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
   }
   if ((rc = dalloc(c, &c->xt2, N))) return fail(rc);
   if ((rc = dalloc(c, &c->xt3, N))) return fail(rc);
+  if ((rc = dalloc(c, &c->xt4, N))) return fail(rc);
+  if ((rc = dalloc(c, &c->xt5, N))) return fail(rc);
   if ((rc = dalloc(c, &c->stage, N))) return fail(rc);
   if ((rc = dalloc(c, &c->stage16, N))) return fail(rc);
   // partial sums: 5 slots of the largest reduction grid

@@ -1501,9 +1501,9 @@ __global__ void k_box_widen(Box b, const float *src, double *dst) {
 // odd row pitch.  Each wavefront then solves two lines, 32 lanes and up to
 // NU rows per lane each, by parallel cyclic reduction with
 // wavefront-scope synchronisation only, and the tile is written back
-// cooperatively.  Blocks [0, ntile*nz) do y-lines into xy, the rest
-// z-lines into xz.  T is the arithmetic and storage type of the sweep
-// (FP64, or FP32 in mixed precision, where xt is the FP32 working copy).
+// cooperatively.  A workgroup does y-lines into xy or z-lines into xz.  T
+// is the arithmetic and storage type of the sweep (FP64, or FP32 in mixed
+// precision, where xt is the FP32 working copy).
 template <typename T>
 __device__ __forceinline__ T coef_ap(long long q) {
   if constexpr (sizeof(T) == 4) return g.apf[q]; else return g.ap[q];
@@ -1561,16 +1561,42 @@ __device__ __forceinline__ void pcr_lane_step(T (&ra)[NU], T (&rc)[NU], T (&rd)[
 }
 
 #define LINES_NT 512
+// Operands of one line-kernel launch.  Inside the box the current values of
+// the unknown are either in xt or, when a pair is given, the average
+// (a + b)/2 of the two line solutions of the previous pass of that colour
+// (the value the separate average would have stored in xt); outside the
+// box they are always in xt.
+template <typename T>
+struct LineIO {
+  const T *xt;
+  const T *own2, *own3;  // own-colour planes as a pair, or null (xt)
+  const T *oth2, *oth3;  // other-colour planes as a pair, or null (xt)
+  T *xy, *xz;            // y-line and z-line solutions of the own-colour planes
+  int pi0, pdi, npl;     // own-colour planes i = pi0 + pdi*p, p < npl
+  int ib, ie, jb, je, kb, ke;  // the box
+};
 template <int LW, typename T, int NU>
-__global__ __launch_bounds__(LINES_NT) void k_lines(const T *__restrict__ xt, T *__restrict__ xy, T *__restrict__ xz,
-                                                    int pi0, int pdi, int npl, int jb, int je, int kb, int ke) {
+__global__ __launch_bounds__(LINES_NT) void k_lines(const LineIO<T> io) {
   extern __shared__ double smd[];
+  const T *__restrict__ xt = io.xt;
+  const int pi0 = io.pi0, pdi = io.pdi, npl = io.npl, jb = io.jb, je = io.je, kb = io.kb, ke = io.ke;
   const int ny = (je - jb) / 2 + 1, nz = (ke - kb) / 2 + 1;
   const int ntile = (npl + LW - 1) / LW;
-  int b = blockIdx.x, dir, n, o, tile;
+  // Workgroups go to the 8 XCDs round robin (workgroup b to XCD b mod 8).
+  // XCD x takes the y-lines at o in [x*nz/8, (x+1)*nz/8) and the z-lines at
+  // o in [x*ny/8, (x+1)*ny/8), all tiles: workgroups that read the same
+  // neighbour rows (adjacent o, adjacent tiles) share one L2, and the XCDs
+  // get equal numbers of rows to within one line per tile.  The grid holds
+  // 8*ntile*(ceil(nz/8) + ceil(ny/8)) workgroups; the surplus ones return.
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int oy0 = xcd * nz / 8, oy1 = (xcd + 1) * nz / 8, oz0 = xcd * ny / 8, oz1 = (xcd + 1) * ny / 8;
+  const int cy = ntile * (oy1 - oy0), cz = ntile * (oz1 - oz0);
+  int dir, n, o, tile, nlin;
   T *xo;
-  if (b < ntile * nz) { dir = 2; n = ny; tile = b % ntile; o = b / ntile; xo = xy; }
-  else { b -= ntile * nz; dir = 3; n = nz; tile = b % ntile; o = b / ntile; xo = xz; }
+  if (slot < cy) { dir = 2; n = ny; nlin = nz; tile = slot % ntile; o = oy0 + slot / ntile; xo = io.xy; }
+  else if (slot < cy + cz) { dir = 3; n = nz; nlin = ny; tile = (slot - cy) % ntile; o = oz0 + (slot - cy) / ntile; xo = io.xz; }
+  else return;
+  const bool own_p = io.own2 != nullptr, oth_p = io.oth2 != nullptr;
   const int npad = n | 1;
   LinePair<T> *AC = reinterpret_cast<LinePair<T> *>(smd);
   T *D = reinterpret_cast<T *>(AC + LW * npad);
@@ -1595,9 +1621,22 @@ __global__ __launch_bounds__(LINES_NT) void k_lines(const T *__restrict__ xt, T 
     const T ax_p = coef_as<T>(3, q), ax_m = coef_as<T>(0, q);
     const T ac_p = coef_as<T>(s_cp, q), ac_m = coef_as<T>(s_cm, q);
     const T al_m = coef_as<T>(s_lm, q), al_p = coef_as<T>(s_lp, q);
-    const T x0 = xt[q];
-    const T xxp = xt[lidx(i + 2, j, k)], xxm = xt[lidx(i - 2, j, k)];
-    const T xcp = xt[lidx(i, j + 2 - dj, k + 2 - dk)], xcm = xt[lidx(i, j - 2 + dj, k - 2 + dk)];
+    // current values: x0 and the cross-direction neighbours on the own
+    // plane, the x neighbours on the other colour's planes (the line-end
+    // neighbours lie outside the box).  Each is read from two arrays, a
+    // pair or xt twice, so that every load is issued unconditionally.
+    const long long qxp = lidx(i + 2, j, k), qxm = lidx(i - 2, j, k);
+    const long long qcp = lidx(i, j + 2 - dj, k + 2 - dk), qcm = lidx(i, j - 2 + dj, k - 2 + dk);
+    const bool pxp = oth_p && i + 2 <= io.ie, pxm = oth_p && i - 2 >= io.ib;
+    const bool pcp = own_p && o + 1 < nlin, pcm = own_p && o > 0;
+    const T x0a = (own_p ? io.own2 : xt)[q], x0b = (own_p ? io.own3 : xt)[q];
+    const T xxpa = (pxp ? io.oth2 : xt)[qxp], xxpb = (pxp ? io.oth3 : xt)[qxp];
+    const T xxma = (pxm ? io.oth2 : xt)[qxm], xxmb = (pxm ? io.oth3 : xt)[qxm];
+    const T xcpa = (pcp ? io.own2 : xt)[qcp], xcpb = (pcp ? io.own3 : xt)[qcp];
+    const T xcma = (pcm ? io.own2 : xt)[qcm], xcmb = (pcm ? io.own3 : xt)[qcm];
+    const T x0 = own_p ? (x0a + x0b) / (T)2 : x0a;
+    const T xxp = pxp ? (xxpa + xxpb) / (T)2 : xxpa, xxm = pxm ? (xxma + xxmb) / (T)2 : xxma;
+    const T xcp = pcp ? (xcpa + xcpb) / (T)2 : xcpa, xcm = pcm ? (xcma + xcmb) / (T)2 : xcma;
     const T xlm = (e == 0) ? xt[lidx(i, j - dj, k - dk)] : (T)0;
     const T xlp = (e == n - 1) ? xt[lidx(i, j + dj, k + dk)] : (T)0;
     T a, c, d;
