@@ -1237,6 +1237,21 @@ __global__ void k_extr_wall(const long long *lst, int n) {
 // precomputed per position in intp_code: 0 -> INTP1 weight, 1 -> 1.0,
 // 2 -> 0.0.  dir: 3 = from (k+1,k-1), 2 = (j+1,j-1), 1 = (i+1,i-1).
 // ======================================================================
+// The 28 neighbour values are loaded before anything is stored (the field
+// arrays may alias as far as the compiler knows, so stores between them
+// would serialise the loads); CONC and DNST_CALC then run on registers with
+// the same operations as on memory.
+__device__ __forceinline__ void conc_r(double y[9]) {  // CONC on gf values in registers, C/Prop.f90:28-59
+  const double yf = y[0], yo = y[1], yc = y[5], yn = y[6];
+  double g0 = yf + yo + yc + yn;
+  if (g0 > 1.0) {
+    g0 = g0 + y[8];
+    y[0] = yf / g0; y[1] = yo / g0; y[5] = yc / g0; y[6] = yn / g0;
+    y[8] = y[8] / g0;
+  } else {
+    y[8] = 1.0 - g0;
+  }
+}
 __global__ void k_intp(Box b, int dir) {
   int i, j, k;
   if (!box_pt(b, i, j, k)) return;
@@ -1250,16 +1265,38 @@ __global__ void k_intp(Box b, int dir) {
   if (dir == 3) { q1 = lidx(i, j, k + 1); q2 = lidx(i, j, k - 1); }
   else if (dir == 2) { q1 = lidx(i, j + 1, k); q2 = lidx(i, j - 1, k); }
   else { q1 = lidx(i + 1, j, k); q2 = lidx(i - 1, j, k); }
-  g.p[q] = a * g.p[q1] + bb * g.p[q2];
-  g.t[q] = a * g.t[q1] + bb * g.t[q2];
-  for (int l = 0; l < 7; l++) g.gf[l][q] = a * g.gf[l][q1] + bb * g.gf[l][q2];
-  CONC(q);
-  g.tmu[q] = a * g.tmu[q1] + bb * g.tmu[q2];
-  g.gcp[q] = a * g.gcp[q1] + bb * g.gcp[q2];
-  g.glam[q] = a * g.glam[q1] + bb * g.glam[q2];
-  g.gama[q] = a * g.gama[q1] + bb * g.gama[q2];
-  g.theta[q] = a * g.theta[q1] + bb * g.theta[q2];
-  DNST_CALC(q);
+  const double p1 = g.p[q1], p2 = g.p[q2], t1 = g.t[q1], t2 = g.t[q2];
+  double y1[7], y2[7];
+#pragma unroll
+  for (int l = 0; l < 7; l++) { y1[l] = g.gf[l][q1]; y2[l] = g.gf[l][q2]; }
+  const double mu1 = g.tmu[q1], mu2 = g.tmu[q2], cp1 = g.gcp[q1], cp2 = g.gcp[q2];
+  const double la1 = g.glam[q1], la2 = g.glam[q2], ga1 = g.gama[q1], ga2 = g.gama[q2];
+  const double th1 = g.theta[q1], th2 = g.theta[q2];
+  double y[9];
+  y[7] = 0.0;
+  y[8] = g.gf[8][q];
+  const double p = a * p1 + bb * p2;
+  const double t = a * t1 + bb * t2;
+#pragma unroll
+  for (int l = 0; l < 7; l++) y[l] = a * y1[l] + bb * y2[l];
+  conc_r(y);
+  const double mu = a * mu1 + bb * mu2, cp = a * cp1 + bb * cp2, la = a * la1 + bb * la2;
+  const double ga = a * ga1 + bb * ga2, th = a * th1 + bb * th2;
+  // DNST_CALC, C/Prop.f90:187-198
+  conc_r(y);
+  const double wt = 1.0 / (y[0] / WF + y[5] / WCO2 + y[8] / WH2O + y[1] / WO2 + y[6] / WN2);
+  const double temp = (1.0 + p) / t * wt / g.wt0;
+  g.p[q] = p;
+  g.t[q] = t;
+#pragma unroll
+  for (int l = 0; l < 7; l++) g.gf[l][q] = y[l];
+  g.gf[8][q] = y[8];
+  g.tmu[q] = mu;
+  g.gcp[q] = cp;
+  g.glam[q] = la;
+  g.gama[q] = ga;
+  g.theta[q] = th;
+  g.dnst[q] = fmax(temp, SMALL20);
 }
 
 // exit temperature of SPHASE, C/Sphase.f90:74-89 (partials: area, T*area, max T)
