@@ -71,6 +71,7 @@ struct gfm_ctx {
   double sgm = 1.0;
   // one SIMPLER iteration captured as a graph, replayed every iteration
   bool use_graph = true;
+  bool line_pcr = false;  // solve every line by PCR across 32 lanes (GFM_LINE_SOLVE=pcr)
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int graph_key = -1;
@@ -350,22 +351,29 @@ static void pull_scalars(gfm_ctx *c, gfm_ctrl *k, const double *sc) {
 // ADLBL3 (C/Adlbl3.f90:10-97) and calc_residual
 // ---------------------------------------------------------------------
 // Line-kernel instance: tile width (one 128-byte line of each row: 16 FP64
-// or 32 FP32 planes; 2 for fewer planes than a tile) and rows per lane NU
-// (lines of up to 32*NU rows).
+// or 32 FP32 planes; 2 for fewer planes than a tile) and the solve: lines of
+// up to 128 rows by 16 lanes with M contiguous rows each (M in 2, 4, 6, 8),
+// longer lines by PCR with NU rows per lane across 32 lanes (lines of up to
+// 32*NU rows).
 static int lines_lw(int npl, size_t word) {
   const int full = word == 4 ? 32 : 16;
   return npl >= full ? full : 2;
 }
 static int lines_nu(int nmax) { return nmax <= 96 ? (nmax <= 64 ? 2 : 3) : nmax <= 128 ? 4 : nmax <= 160 ? 5 : nmax <= 192 ? 6 : 8; }
+static int lines_m(int nmax, bool pcr) {
+  if (pcr || nmax > 128) return 0;
+  return nmax <= 32 ? 2 : nmax <= 64 ? 4 : nmax <= 96 ? 6 : 8;
+}
 static size_t lines_shm(int lw, int nmax, size_t word) { return 3 * (size_t)lw * (nmax | 1) * word; }
-#define LINES_INSTANCES(X_)                                                                       \
-  X_(32, 2) X_(32, 3) X_(32, 4) X_(32, 5) X_(32, 6) X_(32, 8) X_(16, 2) X_(16, 3) X_(16, 4) X_(16, 5) \
-  X_(16, 6) X_(16, 8) X_(2, 2) X_(2, 3) X_(2, 4) X_(2, 5) X_(2, 6) X_(2, 8)
+#define LINES_INSTANCES_LW(X_, LW_)                                                                       \
+  X_(LW_, 1, 2) X_(LW_, 1, 4) X_(LW_, 1, 6) X_(LW_, 1, 8) X_(LW_, 2, 0) X_(LW_, 3, 0) X_(LW_, 4, 0) X_(LW_, 5, 0) \
+  X_(LW_, 6, 0) X_(LW_, 8, 0)
+#define LINES_INSTANCES(X_) LINES_INSTANCES_LW(X_, 32) LINES_INSTANCES_LW(X_, 16) LINES_INSTANCES_LW(X_, 2)
 template <typename T>
-static void launch_lines(gfm_ctx *c, int lw, int nu, unsigned nblk, size_t shm, const LineIO<T> &io) {
-#define LINES_CASE(LW_, NU_)                                                                           \
-  if (lw == LW_ && nu == NU_) {                                                                        \
-    hipLaunchKernelGGL((k_lines<LW_, T, NU_>), dim3(nblk), dim3(LINES_NT), shm, c->stream, io);        \
+static void launch_lines(gfm_ctx *c, int lw, int nu, int m, unsigned nblk, size_t shm, const LineIO<T> &io) {
+#define LINES_CASE(LW_, NU_, M_)                                                                        \
+  if (lw == LW_ && (M_ ? m == M_ : m == 0 && nu == NU_)) {                                              \
+    hipLaunchKernelGGL((k_lines<LW_, T, NU_, M_>), dim3(nblk), dim3(LINES_NT), shm, c->stream, io);     \
     return;                                                                                            \
   }
   LINES_INSTANCES(LINES_CASE)
@@ -373,8 +381,9 @@ static void launch_lines(gfm_ctx *c, int lw, int nu, unsigned nblk, size_t shm, 
 }
 template <typename T>
 static int set_lines_attr(size_t shm) {
-#define LINES_ATTR(LW_, NU_) \
-  HCHECK(hipFuncSetAttribute((const void *)k_lines<LW_, T, NU_>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+#define LINES_ATTR(LW_, NU_, M_)                                                                            \
+  HCHECK(hipFuncSetAttribute((const void *)k_lines<LW_, T, NU_, M_>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                             (int)shm));
   LINES_INSTANCES(LINES_ATTR)
 #undef LINES_ATTR
   return GFM_OK;
@@ -398,7 +407,7 @@ static void line_pass(gfm_ctx *c, const LineIO<T> &io) {
   const size_t shm = lines_shm(lw, nmax, sizeof(T));
   const unsigned nblk = (unsigned)(8 * ntile * ((nz + 7) / 8 + (ny + 7) / 8));  // XCD-balanced order (k_lines)
   if (c->prof.on) prof_begin(c);
-  launch_lines<T>(c, lw, lines_nu(nmax), nblk, shm, io);
+  launch_lines<T>(c, lw, lines_nu(nmax), lines_m(nmax, c->line_pcr), nblk, shm, io);
   if (c->prof.on) prof_end(c, "k_lines", line_bytes);
 }
 
@@ -793,6 +802,8 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
   {
     const char *ev = getenv("GFM_HIP_GRAPH");
     c->use_graph = !(ev && ev[0] == '0');
+    const char *ls = getenv("GFM_LINE_SOLVE");
+    c->line_pcr = ls && strcmp(ls, "pcr") == 0;
   }
   c->nlat = nlat;
   DevGeom &gg = c->dc.g;

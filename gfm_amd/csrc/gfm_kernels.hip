@@ -1597,6 +1597,108 @@ __device__ __forceinline__ void pcr_lane_step(T (&ra)[NU], T (&rc)[NU], T (&rd)[
   for (int u = 0; u < NU; u++) { ra[u] = na[u]; rc[u] = nc[u]; rd[u] = nd[u]; }
 }
 
+// Value of lane l-K (K > 0) or l+|K| (K < 0) of the same 16-lane DPP row,
+// zero where that lane lies outside the row.  Every lane of the wavefront
+// must execute it.
+template <int K, typename T>
+__device__ __forceinline__ T row_shift(T v) {
+  constexpr int ctrl = K > 0 ? 0x110 + K : 0x100 - K;  // row_shr:K / row_shl:-K
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), ctrl, 0xf, 0xf, false));
+  } else {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, ctrl, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), ctrl, 0xf, 0xf, false);
+    return __builtin_bit_cast(T, (long long)(unsigned)lo | ((long long)hi << 32));
+  }
+}
+
+// Row r of a unit-diagonal tridiagonal system, a x_{r-S} + x_r + c x_{r+S} = d.
+template <typename T> struct TriRow { T a, c, d; };
+
+// One PCR elimination: row r with its neighbours at -S and +S
+// (zero rows where the neighbour does not exist), renormalised.
+template <typename T>
+__device__ __forceinline__ TriRow<T> pcr_combine(const TriRow<T> &r, const TriRow<T> &m, const TriRow<T> &p) {
+  const T q = line_rcp<T>((T)1 - r.a * m.c - r.c * p.a);
+  return TriRow<T>{-r.a * m.a * q, -r.c * p.c * q, (r.d - r.a * m.d - r.c * p.d) * q};
+}
+
+template <int K, typename T>
+__device__ __forceinline__ TriRow<T> row_shift3(const TriRow<T> &v) {
+  return TriRow<T>{row_shift<K>(v.a), row_shift<K>(v.c), row_shift<K>(v.d)};
+}
+
+// Reduced-system PCR step at lane stride K >= 1 (reduced-row stride 2K):
+// each lane's first and last rows combine with the same rows of lanes L-K
+// and L+K.
+template <int K, typename T>
+__device__ __forceinline__ void reduced_step(TriRow<T> &rs, TriRow<T> &re) {
+  const TriRow<T> sm = row_shift3<K>(rs), sp = row_shift3<-K>(rs);
+  const TriRow<T> em = row_shift3<K>(re), ep = row_shift3<-K>(re);
+  rs = pcr_combine(rs, sm, sp);
+  re = pcr_combine(re, em, ep);
+}
+
+// Solve of one line of up to 16*M rows (M >= 2) by 16 lanes of one DPP row,
+// lane L owning the contiguous rows s = L*M .. e = s+M-1 (unit-diagonal
+// rows a x_{i-1} + x_i + c x_{i+1} = d; rows past the line are zero and
+// decouple, since the last row of a line has c = 0).
+//  1. Forward elimination inside the lane: rows s+1..e as
+//     x_i + al_i x_s + ga_i x_{i+1} = de_i; row e is the lane's last
+//     reduced row, x_e + al_e x_s + ga_e x_{e+1} = de_e.
+//  2. x_{s+1} = p - q x_s - t x_e by back substitution of those rows;
+//     with row s it gives the first reduced row, x_s + A x_{s-1} + C x_e = D.
+//  3. The 32 reduced rows (s_0, e_0, s_1, e_1, ...) form a tridiagonal
+//     system, solved by parallel cyclic reduction across the lanes with
+//     DPP row shifts (reduced strides 1, 2, 4, 8, 16).
+//  4. Rows s+1..e-1 from step 1's relations, from e-1 down.
+// The solution replaces d.
+template <typename T, int M>
+__device__ __forceinline__ void line_partition_solve(T (&a)[M], T (&c)[M], T (&d)[M]) {
+  T al[M], ga[M], de[M];
+  al[1] = a[1]; ga[1] = c[1]; de[1] = d[1];
+#pragma unroll
+  for (int u = 2; u < M; u++) {
+    const T r = line_rcp<T>((T)1 - a[u] * ga[u - 1]);
+    al[u] = -a[u] * al[u - 1] * r;
+    ga[u] = c[u] * r;
+    de[u] = (d[u] - a[u] * de[u - 1]) * r;
+  }
+  T p = 0, q = 0, t = -1;  // M == 2: x_{s+1} = x_e
+  if constexpr (M > 2) {
+    p = de[M - 2]; q = al[M - 2]; t = ga[M - 2];
+#pragma unroll
+    for (int u = M - 3; u >= 1; u--) {
+      p = de[u] - ga[u] * p;
+      q = al[u] - ga[u] * q;
+      t = -ga[u] * t;
+    }
+  }
+  const T r0 = line_rcp<T>((T)1 - c[0] * q);
+  TriRow<T> rs{a[0] * r0, -c[0] * t * r0, (d[0] - c[0] * p) * r0};
+  TriRow<T> re{al[M - 1], ga[M - 1], de[M - 1]};
+  {  // reduced stride 1: s_L couples e_{L-1} and e_L; e_L couples s_L and s_{L+1}
+    const TriRow<T> em = row_shift3<1>(re), sp = row_shift3<-1>(rs);
+    const TriRow<T> ns = pcr_combine(rs, em, re);
+    const TriRow<T> ne = pcr_combine(re, rs, sp);
+    rs = ns; re = ne;
+  }
+  reduced_step<1>(rs, re);
+  reduced_step<2>(rs, re);
+  reduced_step<4>(rs, re);
+  reduced_step<8>(rs, re);
+  const T xs = rs.d, xe = re.d;
+  d[0] = xs;
+  d[M - 1] = xe;
+  T xn = xe;
+#pragma unroll
+  for (int u = M - 2; u >= 1; u--) {
+    xn = de[u] - al[u] * xs - ga[u] * xn;
+    d[u] = xn;
+  }
+}
+
 #define LINES_NT 512
 // Operands of one line-kernel launch.  Inside the box the current values of
 // the unknown are either in xt or, when a pair is given, the average
@@ -1612,7 +1714,7 @@ struct LineIO {
   int pi0, pdi, npl;     // own-colour planes i = pi0 + pdi*p, p < npl
   int ib, ie, jb, je, kb, ke;  // the box
 };
-template <int LW, typename T, int NU>
+template <int LW, typename T, int NU, int M>
 __global__ __launch_bounds__(LINES_NT) void k_lines(const LineIO<T> io) {
   extern __shared__ double smd[];
   const T *__restrict__ xt = io.xt;
@@ -1692,10 +1794,37 @@ __global__ __launch_bounds__(LINES_NT) void k_lines(const LineIO<T> io) {
     D[w * npad + e] = d;
   }
   __syncthreads();
+  const int wave = t >> 6, lane = t & 63;
+  if constexpr (M > 0) {
+    // Lines of up to 16*M rows: 16 lanes (one DPP row) per line, four lines
+    // per wavefront, each lane M contiguous rows (line_partition_solve).
+    // Lanes of absent lines solve zero rows, so that every lane of a
+    // wavefront takes part in the row shifts.
+    const int sub = lane >> 4, l16 = lane & 15;
+    for (int ln0 = 4 * wave; ln0 < LW; ln0 += 4 * (LINES_NT / 64)) {
+      const int ln = ln0 + sub;
+      const bool act = ln < LW && tile * LW + ln < npl;
+      const LinePair<T> *ac_ = AC + (act ? ln : 0) * npad;
+      T *d_ = D + (act ? ln : 0) * npad;
+      T ra[M], rc[M], rd[M];
+#pragma unroll
+      for (int u = 0; u < M; u++) {
+        const int e = l16 * M + u;
+        ra[u] = 0; rc[u] = 0; rd[u] = 0;
+        if (act && e < n) { const LinePair<T> p = ac_[e]; ra[u] = p.a; rc[u] = p.c; rd[u] = d_[e]; }
+      }
+      line_partition_solve<T, M>(ra, rc, rd);
+#pragma unroll
+      for (int u = 0; u < M; u++) {
+        const int e = l16 * M + u;
+        if (act && e < n) d_[e] = rd[u];
+      }
+    }
+  } else {
   // PCR on unit-diagonal rows: row e (a, 1, c | d) combined with rows e-s
   // and e+s gives (-a*a_m, 1 - a*c_m - c*a_p, -c*c_p | d - a*d_m - c*d_p),
   // renormalised.  After ceil(log2 n) steps every row is (0, 1, 0 | x).
-  const int wave = t >> 6, lane = t & 63, half = lane >> 5, l32 = lane & 31;
+  const int half = lane >> 5, l32 = lane & 31;
   for (int pr = wave; pr < LW / 2; pr += LINES_NT / 64) {
     const int ln = 2 * pr + half;
     const bool act = tile * LW + ln < npl;  // differs between the halves: every wave_sync stays outside it
@@ -1740,6 +1869,7 @@ __global__ __launch_bounds__(LINES_NT) void k_lines(const LineIO<T> io) {
         if (act && e < n) d_[e] = rd[u];
       }
     }
+  }
   }
   __syncthreads();
   for (int e = rg; e < n; e += NRG) {
