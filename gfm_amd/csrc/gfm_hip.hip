@@ -57,6 +57,7 @@ struct gfm_ctx {
   DevCase dc;
   std::vector<void *> allocs;
   double *xt2 = nullptr, *xt3 = nullptr, *xt4 = nullptr, *xt5 = nullptr, *part = nullptr, *stage = nullptr;
+  double *xg = nullptr;  // G of the line kernel between the two passes of a colour visit
   float *xtf = nullptr;  // FP32 working copy of the unknown of ADLBL3 (mixed precision)
   int16_t *stage16 = nullptr;
   // lattice positions of the GFLOW wall treatment: open with a wall
@@ -399,8 +400,11 @@ static void line_pass(gfm_ctx *c, const LineIO<T> &io) {
   const double cells = (double)io.npl * nyl * nzl;
   // algorithmic bytes per row and direction: 8 coefficients, the current
   // value of the unknown (one word from xt, or the two words of a pair whose
-  // average the kernel forms) and the written solution
-  const double line_bytes = cells * 2.0 * (8.0 + (io.own2 ? 2.0 : 1.0) + 1.0) * sizeof(T);
+  // average the kernel forms) and the written solution, plus one word per
+  // cell where G is stored; where G is read, 5 coefficients, G and the
+  // written solution
+  const double line_bytes = io.gr ? cells * 2.0 * 7.0 * sizeof(T)
+                                  : cells * (2.0 * (8.0 + (io.own2 ? 2.0 : 1.0) + 1.0) + (io.gw ? 1.0 : 0.0)) * sizeof(T);
   const int lw = lines_lw(io.npl, sizeof(T));
   const int ntile = (io.npl + lw - 1) / lw;
   const int nmax = ny > nz ? ny : nz;
@@ -442,7 +446,8 @@ static void line_avg(gfm_ctx *c, T *xt, const T *x2, const T *x3, int pi0, int p
 // values of a colour's planes as the average of the pair its last pass
 // wrote (x2/x3 or x4/x5), and xt receives them once, after the last pass.
 template <typename T>
-static void sweeps(gfm_ctx *c, T *xt, T *x2, T *x3, T *x4, T *x5, int nim, int ib, int ie, int jb, int je, int kb, int ke) {
+static void sweeps(gfm_ctx *c, T *xt, T *x2, T *x3, T *x4, T *x5, T *xg, int nim, int ib, int ie, int jb, int je, int kb,
+                   int ke) {
   const int npl = (ie - ib) / 2 + 1;
   if (c->ctrl.sweep_order == GFM_ORDER_REFERENCE) {
     for (int s = 0; s < nim; s++)
@@ -450,7 +455,7 @@ static void sweeps(gfm_ctx *c, T *xt, T *x2, T *x3, T *x4, T *x5, int nim, int i
         for (int q = 0; q < npl; q++) {
           const int i = dirn == 0 ? ib + 2 * q : ie - 2 * q;
           for (int l = 0; l < 2; l++) {
-            const LineIO<T> io{xt, nullptr, nullptr, nullptr, nullptr, x2, x3, i, 0, 1, ib, ie, jb, je, kb, ke};
+            const LineIO<T> io{xt, nullptr, nullptr, nullptr, nullptr, x2, x3, nullptr, nullptr, i, 0, 1, ib, ie, jb, je, kb, ke};
             line_pass<T>(c, io);
             line_avg<T>(c, xt, x2, x3, i, 0, 1, jb, je, kb, ke);
           }
@@ -459,6 +464,12 @@ static void sweeps(gfm_ctx *c, T *xt, T *x2, T *x3, T *x4, T *x5, int nim, int i
   }
   T *pr[2][2] = {{x2, x3}, {x4, x5}};
   int cur[2] = {-1, -1};  // pair holding each colour's current values (-1: xt)
+  {
+    const int nj = (je - jb) / 2 + 1, nk = (ke - kb) / 2 + 1;
+    const long long tot = 2LL * nj * nk;
+    hipLaunchKernelGGL(k_pair_xbound<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, xt, x2, x3, x4, x5, ib,
+                       ie, jb, nj, kb, nk);
+  }
   const int ncol[2] = {(npl + 1) / 2, npl / 2};
   for (int s = 0; s < nim; s++)
     for (int half = 0; half < 4; half++) {
@@ -469,6 +480,7 @@ static void sweeps(gfm_ctx *c, T *xt, T *x2, T *x3, T *x4, T *x5, int nim, int i
         const int tgt = own == 0 ? 1 : 0;  // never the pair this pass reads its own planes from
         const LineIO<T> io{xt, own >= 0 ? pr[own][0] : nullptr, own >= 0 ? pr[own][1] : nullptr,
                            oth >= 0 ? pr[oth][0] : nullptr, oth >= 0 ? pr[oth][1] : nullptr, pr[tgt][0], pr[tgt][1],
+                           l == 0 ? xg : nullptr, l == 1 ? xg : nullptr,
                            ib + 2 * col, 4, ncol[col], ib, ie, jb, je, kb, ke};
         line_pass<T>(c, io);
         cur[col] = tgt;
@@ -485,7 +497,7 @@ static void residual(gfm_ctx *c, const double *xt, int ib, int ie, int jb, int j
 // nim sweeps of ADLBL3 on xt over the box (FP32 working copy in mixed precision)
 static void adlbl3_sweeps(gfm_ctx *c, double *xt, int nim, int ib, int ie, int jb, int je, int kb, int ke) {
   if (!c->dc.mixed) {
-    sweeps<double>(c, xt, c->xt2, c->xt3, c->xt4, c->xt5, nim, ib, ie, jb, je, kb, ke);
+    sweeps<double>(c, xt, c->xt2, c->xt3, c->xt4, c->xt5, c->xg, nim, ib, ie, jb, je, kb, ke);
   } else {
     // FP32 sweeps on a working copy of the box and the neighbours it reads
     // (one lattice step of the class outside the box); rows the sweeps pin
@@ -494,7 +506,8 @@ static void adlbl3_sweeps(gfm_ctx *c, double *xt, int nim, int ib, int ie, int j
     Box box = make_box(ib, ie, jb, je, kb, ke, 2);
     LAUNCH(k_box_narrow, ext, xt, c->xtf);
     sweeps<float>(c, c->xtf, reinterpret_cast<float *>(c->xt2), reinterpret_cast<float *>(c->xt3),
-                  reinterpret_cast<float *>(c->xt4), reinterpret_cast<float *>(c->xt5), nim, ib, ie, jb, je, kb, ke);
+                  reinterpret_cast<float *>(c->xt4), reinterpret_cast<float *>(c->xt5), reinterpret_cast<float *>(c->xg), nim,
+                  ib, ie, jb, je, kb, ke);
     LAUNCH(k_box_widen, box, c->xtf, xt);
   }
 }
@@ -815,6 +828,10 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
   c->ncls = gg.S * 8;
   int rc = 0;
   auto fail = [&](int r) { gfm_hip_finalize(c); return r; };
+  if (c->ncls * (long long)sizeof(double) >= (1LL << 32)) {  // the line kernel addresses arrays by 32-bit byte offsets
+    set_err("lattice of %lld entries per array exceeds 4 GB of FP64", c->ncls);
+    return fail(GFM_E_ARG);
+  }
   HCHECK(hipStreamCreate(&c->stream));
   DevCase &d = c->dc;
   const long long N = c->ncls;
@@ -846,6 +863,7 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
   if ((rc = dalloc(c, &c->xt3, N))) return fail(rc);
   if ((rc = dalloc(c, &c->xt4, N))) return fail(rc);
   if ((rc = dalloc(c, &c->xt5, N))) return fail(rc);
+  if ((rc = dalloc(c, &c->xg, N))) return fail(rc);
   if ((rc = dalloc(c, &c->stage, N))) return fail(rc);
   if ((rc = dalloc(c, &c->stage16, N))) return fail(rc);
   // partial sums: 5 slots of the largest reduction grid

@@ -1514,6 +1514,20 @@ __global__ void k_line_avg2(T *xt, const T *x2, const T *x3, int pi0, int pdi, i
   }
 }
 
+// Copies xt on the x-planes i = ib-2 and ie+2 (box rows j, k) into the four
+// arrays of the two pairs, so that a pair's average there is xt's value.
+template <typename T>
+__global__ void k_pair_xbound(const T *xt, T *x2, T *x3, T *x4, T *x5, int ib, int ie, int jb, int nj, int kb, int nk) {
+  long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= 2LL * nj * nk) return;
+  const int side = (int)(t & 1);
+  const long long r = t >> 1;
+  const int j = jb + 2 * (int)(r % nj), k = kb + 2 * (int)(r / nj);
+  const long long q = lidx(side ? ie + 2 : ib - 2, j, k);
+  const T v = xt[q];
+  x2[q] = v; x3[q] = v; x4[q] = v; x5[q] = v;
+}
+
 // FP32 working copy of an unknown over a box (mixed precision) and back.
 __global__ void k_box_narrow(Box b, const double *src, float *dst) {
   int i, j, k;
@@ -1541,19 +1555,22 @@ __global__ void k_box_widen(Box b, const float *src, double *dst) {
 // cooperatively.  A workgroup does y-lines into xy or z-lines into xz.  T
 // is the arithmetic and storage type of the sweep (FP64, or FP32 in mixed
 // precision, where xt is the FP32 working copy).
-template <typename T>
-__device__ __forceinline__ T coef_ap(long long q) {
-  if constexpr (sizeof(T) == 4) return g.apf[q]; else return g.ap[q];
+template <typename V>
+__device__ __forceinline__ V ld_off(const V *base, unsigned byte_off) {
+  return *reinterpret_cast<const V *>(reinterpret_cast<const char *>(base) + byte_off);
 }
 template <typename T>
-__device__ __forceinline__ T coef_bs(long long q) {
-  if constexpr (sizeof(T) == 4) return g.bsf[q]; else return g.bs[q];
+__device__ __forceinline__ const T *coef_ap_base() {
+  if constexpr (sizeof(T) == 4) return g.apf; else return g.ap;
 }
 template <typename T>
-__device__ __forceinline__ T coef_as(int s, long long q) {
-  if constexpr (sizeof(T) == 4) return g.asf[s][q]; else return g.as[s][q];
+__device__ __forceinline__ const T *coef_bs_base() {
+  if constexpr (sizeof(T) == 4) return g.bsf; else return g.bs;
 }
-
+template <typename T>
+__device__ __forceinline__ const T *coef_as_base(int s) {
+  if constexpr (sizeof(T) == 4) return g.asf[s]; else return g.as[s];
+}
 // 1/x: the hardware reciprocal, refined by two Newton-Raphson steps in
 // FP64 (error below 2 ulp); the hardware FP32 reciprocal (1 ulp) as is.
 template <typename T> __device__ __forceinline__ T line_rcp(T x);
@@ -1634,16 +1651,19 @@ __device__ __forceinline__ TriRow<T> row_shift3(const TriRow<T> &v) {
 // and L+K.
 template <int K, typename T>
 __device__ __forceinline__ void reduced_step(TriRow<T> &rs, TriRow<T> &re) {
-  const TriRow<T> sm = row_shift3<K>(rs), sp = row_shift3<-K>(rs);
+  {
+    const TriRow<T> sm = row_shift3<K>(rs), sp = row_shift3<-K>(rs);
+    rs = pcr_combine(rs, sm, sp);
+  }
   const TriRow<T> em = row_shift3<K>(re), ep = row_shift3<-K>(re);
-  rs = pcr_combine(rs, sm, sp);
   re = pcr_combine(re, em, ep);
 }
 
 // Solve of one line of up to 16*M rows (M >= 2) by 16 lanes of one DPP row,
 // lane L owning the contiguous rows s = L*M .. e = s+M-1 (unit-diagonal
-// rows a x_{i-1} + x_i + c x_{i+1} = d; rows past the line are zero and
-// decouple, since the last row of a line has c = 0).
+// rows a x_{i-1} + x_i + c x_{i+1} = d, read from the lane's LDS rows acl,
+// dl, of which the first nrow exist; the others are zero and decouple,
+// since the last row of a line has c = 0).
 //  1. Forward elimination inside the lane: rows s+1..e as
 //     x_i + al_i x_s + ga_i x_{i+1} = de_i; row e is the lane's last
 //     reduced row, x_e + al_e x_s + ga_e x_{e+1} = de_e.
@@ -1653,17 +1673,23 @@ __device__ __forceinline__ void reduced_step(TriRow<T> &rs, TriRow<T> &re) {
 //     system, solved by parallel cyclic reduction across the lanes with
 //     DPP row shifts (reduced strides 1, 2, 4, 8, 16).
 //  4. Rows s+1..e-1 from step 1's relations, from e-1 down.
-// The solution replaces d.
+// The solution is written to dl (existing rows).  Every lane of the
+// wavefront must call it.
 template <typename T, int M>
-__device__ __forceinline__ void line_partition_solve(T (&a)[M], T (&c)[M], T (&d)[M]) {
+__device__ __forceinline__ void line_partition_solve(LinePair<T> *acl, T *dl, int nrow) {
   T al[M], ga[M], de[M];
-  al[1] = a[1]; ga[1] = c[1]; de[1] = d[1];
+  T a0 = 0, c0 = 0, d0 = 0;
+  if (nrow > 0) { const LinePair<T> r = acl[0]; a0 = r.a; c0 = r.c; d0 = dl[0]; }
+  al[1] = 0; ga[1] = 0; de[1] = 0;
+  if (nrow > 1) { const LinePair<T> r = acl[1]; al[1] = r.a; ga[1] = r.c; de[1] = dl[1]; }
 #pragma unroll
   for (int u = 2; u < M; u++) {
-    const T r = line_rcp<T>((T)1 - a[u] * ga[u - 1]);
-    al[u] = -a[u] * al[u - 1] * r;
-    ga[u] = c[u] * r;
-    de[u] = (d[u] - a[u] * de[u - 1]) * r;
+    T a = 0, c = 0, d = 0;
+    if (u < nrow) { const LinePair<T> r = acl[u]; a = r.a; c = r.c; d = dl[u]; }
+    const T r = line_rcp<T>((T)1 - a * ga[u - 1]);
+    al[u] = -a * al[u - 1] * r;
+    ga[u] = c * r;
+    de[u] = (d - a * de[u - 1]) * r;
   }
   T p = 0, q = 0, t = -1;  // M == 2: x_{s+1} = x_e
   if constexpr (M > 2) {
@@ -1675,8 +1701,8 @@ __device__ __forceinline__ void line_partition_solve(T (&a)[M], T (&c)[M], T (&d
       t = -ga[u] * t;
     }
   }
-  const T r0 = line_rcp<T>((T)1 - c[0] * q);
-  TriRow<T> rs{a[0] * r0, -c[0] * t * r0, (d[0] - c[0] * p) * r0};
+  const T r0 = line_rcp<T>((T)1 - c0 * q);
+  TriRow<T> rs{a0 * r0, -c0 * t * r0, (d0 - c0 * p) * r0};
   TriRow<T> re{al[M - 1], ga[M - 1], de[M - 1]};
   {  // reduced stride 1: s_L couples e_{L-1} and e_L; e_L couples s_L and s_{L+1}
     const TriRow<T> em = row_shift3<1>(re), sp = row_shift3<-1>(rs);
@@ -1689,33 +1715,48 @@ __device__ __forceinline__ void line_partition_solve(T (&a)[M], T (&c)[M], T (&d
   reduced_step<4>(rs, re);
   reduced_step<8>(rs, re);
   const T xs = rs.d, xe = re.d;
-  d[0] = xs;
-  d[M - 1] = xe;
+  if (nrow > 0) dl[0] = xs;
+  if (nrow > M - 1) dl[M - 1] = xe;
   T xn = xe;
 #pragma unroll
   for (int u = M - 2; u >= 1; u--) {
     xn = de[u] - al[u] * xs - ga[u] * xn;
-    d[u] = xn;
+    if (u < nrow) dl[u] = xn;
   }
 }
 
 #define LINES_NT 512
+// Waves per SIMD the line kernel is compiled for (register budget), and
+// rows of the load phase whose loads are issued together per thread.
+#ifndef LINES_MIN_WAVES
+#define LINES_MIN_WAVES 6
+#endif
+#ifndef LINES_LOAD_UNROLL
+#define LINES_LOAD_UNROLL 2
+#endif
 // Operands of one line-kernel launch.  Inside the box the current values of
 // the unknown are either in xt or, when a pair is given, the average
 // (a + b)/2 of the two line solutions of the previous pass of that colour
 // (the value the separate average would have stored in xt); outside the
-// box they are always in xt.
+// box they are in xt, and a pair also holds copies of xt on the two
+// x-planes just outside the box (k_pair_xbound).
 template <typename T>
 struct LineIO {
   const T *xt;
   const T *own2, *own3;  // own-colour planes as a pair, or null (xt)
   const T *oth2, *oth3;  // other-colour planes as a pair, or null (xt)
   T *xy, *xz;            // y-line and z-line solutions of the own-colour planes
+  // G = bs + AS(x-) x(i-2) + AS(x+) x(i+2) of the own-colour cells (x0 on
+  // pinned rows): stored by the y-line workgroups when gw is given (first
+  // pass of a colour visit), read instead of its terms when gr is given
+  // (second pass: the other colour's planes have not changed)
+  T *gw;
+  const T *gr;
   int pi0, pdi, npl;     // own-colour planes i = pi0 + pdi*p, p < npl
   int ib, ie, jb, je, kb, ke;  // the box
 };
 template <int LW, typename T, int NU, int M>
-__global__ __launch_bounds__(LINES_NT) void k_lines(const LineIO<T> io) {
+__global__ __launch_bounds__(LINES_NT, LINES_MIN_WAVES) void k_lines(const LineIO<T> io) {
   extern __shared__ double smd[];
   const T *__restrict__ xt = io.xt;
   const int pi0 = io.pi0, pdi = io.pdi, npl = io.npl, jb = io.jb, je = io.je, kb = io.kb, ke = io.ke;
@@ -1749,41 +1790,60 @@ __global__ __launch_bounds__(LINES_NT) void k_lines(const LineIO<T> io) {
   const int s_lm = (dir == 2) ? 1 : 2, s_lp = s_lm + 3;
   const int s_cm = (dir == 2) ? 2 : 1, s_cp = s_cm + 3;
   const int dj = (dir == 2) ? 2 : 0, dk = 2 - dj;  // lattice step along the line
-#pragma unroll 2
+  // arrays of the current values (uniform over the workgroup)
+  const bool pcp = own_p && o + 1 < nlin, pcm = own_p && o > 0;
+  const T *own2 = own_p ? io.own2 : xt, *own3 = own_p ? io.own3 : xt;
+  const T *oth2 = oth_p ? io.oth2 : xt, *oth3 = oth_p ? io.oth3 : xt;
+  const T *cp2 = pcp ? io.own2 : xt, *cp3 = pcp ? io.own3 : xt;
+  const T *cm2 = pcm ? io.own2 : xt, *cm3 = pcm ? io.own3 : xt;
+  const bool g_read = io.gr != nullptr, g_write = io.gw != nullptr && dir == 2;
+#pragma unroll LINES_LOAD_UNROLL
   for (int e = rg; e < n; e += NRG) {
     if (!valid) break;
     const int j = (dir == 2) ? jb + 2 * e : jb + 2 * o;
     const int k = (dir == 2) ? kb + 2 * o : kb + 2 * e;
-    const long long q = lidx(i, j, k);
-    const int code = g.ib[q];
-    const T ap = coef_ap<T>(q), bs = coef_bs<T>(q);
-    const T ax_p = coef_as<T>(3, q), ax_m = coef_as<T>(0, q);
-    const T ac_p = coef_as<T>(s_cp, q), ac_m = coef_as<T>(s_cm, q);
-    const T al_m = coef_as<T>(s_lm, q), al_p = coef_as<T>(s_lp, q);
-    // current values: x0 and the cross-direction neighbours on the own
-    // plane, the x neighbours on the other colour's planes (the line-end
-    // neighbours lie outside the box).  Each is read from two arrays, a
-    // pair or xt twice, so that every load is issued unconditionally.
-    const long long qxp = lidx(i + 2, j, k), qxm = lidx(i - 2, j, k);
-    const long long qcp = lidx(i, j + 2 - dj, k + 2 - dk), qcm = lidx(i, j - 2 + dj, k - 2 + dk);
-    const bool pxp = oth_p && i + 2 <= io.ie, pxm = oth_p && i - 2 >= io.ib;
-    const bool pcp = own_p && o + 1 < nlin, pcm = own_p && o > 0;
-    const T x0a = (own_p ? io.own2 : xt)[q], x0b = (own_p ? io.own3 : xt)[q];
-    const T xxpa = (pxp ? io.oth2 : xt)[qxp], xxpb = (pxp ? io.oth3 : xt)[qxp];
-    const T xxma = (pxm ? io.oth2 : xt)[qxm], xxmb = (pxm ? io.oth3 : xt)[qxm];
-    const T xcpa = (pcp ? io.own2 : xt)[qcp], xcpb = (pcp ? io.own3 : xt)[qcp];
-    const T xcma = (pcm ? io.own2 : xt)[qcm], xcmb = (pcm ? io.own3 : xt)[qcm];
-    const T x0 = own_p ? (x0a + x0b) / (T)2 : x0a;
-    const T xxp = pxp ? (xxpa + xxpb) / (T)2 : xxpa, xxm = pxm ? (xxma + xxmb) / (T)2 : xxma;
-    const T xcp = pcp ? (xcpa + xcpb) / (T)2 : xcpa, xcm = pcm ? (xcma + xcmb) / (T)2 : xcma;
+    // 32-bit byte offsets from uniform base pointers (every lattice array
+    // of the case is below 4 GB, checked at init)
+    const unsigned q = (unsigned)lidx(i, j, k), bq = q * (unsigned)sizeof(T);
+    const int code = ld_off(g.ib, 2u * q);
+    const T ap = ld_off(coef_ap_base<T>(), bq);
+    const T ac_p = ld_off(coef_as_base<T>(s_cp), bq), ac_m = ld_off(coef_as_base<T>(s_cm), bq);
+    const T al_m = ld_off(coef_as_base<T>(s_lm), bq), al_p = ld_off(coef_as_base<T>(s_lp), bq);
+    // current values: the cross-direction neighbours on the own plane (past
+    // the last line of the box they lie outside it, in xt), and for G the
+    // value x0 and the x neighbours on the other colour's planes (a pair
+    // also holds the planes just outside the box in x, copies of xt there,
+    // so these need no test).  Each value is read from two arrays, a pair or
+    // xt twice, so that every load is issued unconditionally.
+    const unsigned bcp = (unsigned)lidx(i, j + 2 - dj, k + 2 - dk) * (unsigned)sizeof(T);
+    const unsigned bcm = (unsigned)lidx(i, j - 2 + dj, k - 2 + dk) * (unsigned)sizeof(T);
+    const T xcpa = ld_off(cp2, bcp), xcpb = ld_off(cp3, bcp);
+    const T xcma = ld_off(cm2, bcm), xcmb = ld_off(cm3, bcm);
     const T xlm = (e == 0) ? xt[lidx(i, j - dj, k - dk)] : (T)0;
     const T xlp = (e == n - 1) ? xt[lidx(i, j + dj, k + dk)] : (T)0;
-    T a, c, d;
-    if (code >= 1 || fabs((double)ap) <= SMALL20) {
-      a = 0; c = 0; d = x0;
+    const bool pinned = code >= 1 || fabs((double)ap) <= SMALL20;
+    T gx;
+    if (g_read) {
+      gx = ld_off(io.gr, bq);
     } else {
-      T bsy = bs + ax_p * xxp + ax_m * xxm;
-      bsy = bsy + ac_p * xcp + ac_m * xcm;
+      const unsigned bxp = (unsigned)lidx(i + 2, j, k) * (unsigned)sizeof(T);
+      const unsigned bxm = (unsigned)lidx(i - 2, j, k) * (unsigned)sizeof(T);
+      const T bs = ld_off(coef_bs_base<T>(), bq);
+      const T ax_p = ld_off(coef_as_base<T>(3), bq), ax_m = ld_off(coef_as_base<T>(0), bq);
+      const T x0a = ld_off(own2, bq), x0b = ld_off(own3, bq);
+      const T xxpa = ld_off(oth2, bxp), xxpb = ld_off(oth3, bxp);
+      const T xxma = ld_off(oth2, bxm), xxmb = ld_off(oth3, bxm);
+      const T x0 = own_p ? (x0a + x0b) / (T)2 : x0a;
+      const T xxp = oth_p ? (xxpa + xxpb) / (T)2 : xxpa, xxm = oth_p ? (xxma + xxmb) / (T)2 : xxma;
+      gx = pinned ? x0 : bs + ax_p * xxp + ax_m * xxm;
+      if (g_write) *reinterpret_cast<T *>(reinterpret_cast<char *>(io.gw) + bq) = gx;
+    }
+    const T xcp = pcp ? (xcpa + xcpb) / (T)2 : xcpa, xcm = pcm ? (xcma + xcmb) / (T)2 : xcma;
+    T a, c, d;
+    if (pinned) {
+      a = 0; c = 0; d = gx;
+    } else {
+      T bsy = gx + ac_p * xcp + ac_m * xcm;
       T am = al_m, apn = al_p;
       if (e == 0) { bsy += am * xlm; am = 0; }
       if (e == n - 1) { bsy += apn * xlp; apn = 0; }
@@ -1804,21 +1864,10 @@ __global__ __launch_bounds__(LINES_NT) void k_lines(const LineIO<T> io) {
     for (int ln0 = 4 * wave; ln0 < LW; ln0 += 4 * (LINES_NT / 64)) {
       const int ln = ln0 + sub;
       const bool act = ln < LW && tile * LW + ln < npl;
-      const LinePair<T> *ac_ = AC + (act ? ln : 0) * npad;
+      LinePair<T> *ac_ = AC + (act ? ln : 0) * npad;
       T *d_ = D + (act ? ln : 0) * npad;
-      T ra[M], rc[M], rd[M];
-#pragma unroll
-      for (int u = 0; u < M; u++) {
-        const int e = l16 * M + u;
-        ra[u] = 0; rc[u] = 0; rd[u] = 0;
-        if (act && e < n) { const LinePair<T> p = ac_[e]; ra[u] = p.a; rc[u] = p.c; rd[u] = d_[e]; }
-      }
-      line_partition_solve<T, M>(ra, rc, rd);
-#pragma unroll
-      for (int u = 0; u < M; u++) {
-        const int e = l16 * M + u;
-        if (act && e < n) d_[e] = rd[u];
-      }
+      const int nrow = act ? min(max(n - l16 * M, 0), M) : 0;
+      line_partition_solve<T, M>(ac_ + l16 * M, d_ + l16 * M, nrow);
     }
   } else {
   // PCR on unit-diagonal rows: row e (a, 1, c | d) combined with rows e-s
