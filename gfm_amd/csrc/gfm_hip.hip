@@ -399,12 +399,12 @@ static void line_pass(gfm_ctx *c, const LineIO<T> &io) {
   const int ny = nzl, nz = nyl;                                            // rows of a y-line, of a z-line
   const double cells = (double)io.npl * nyl * nzl;
   // algorithmic bytes per row and direction: 8 coefficients, the current
-  // value of the unknown (one word from xt, or the two words of a pair whose
-  // average the kernel forms) and the written solution, plus one word per
-  // cell where G is stored; where G is read, 5 coefficients, G and the
-  // written solution
+  // values of the x neighbours (one word from xt, or the two words of a
+  // pair whose average the kernel forms) and the written solution, plus one
+  // word per cell where G is stored; where G is read, 5 coefficients, G and
+  // the written solution
   const double line_bytes = io.gr ? cells * 2.0 * 7.0 * sizeof(T)
-                                  : cells * (2.0 * (8.0 + (io.own2 ? 2.0 : 1.0) + 1.0) + (io.gw ? 1.0 : 0.0)) * sizeof(T);
+                                  : cells * (2.0 * (8.0 + (io.oth2 ? 2.0 : 1.0) + 1.0) + (io.gw ? 1.0 : 0.0)) * sizeof(T);
   const int lw = lines_lw(io.npl, sizeof(T));
   const int ntile = (io.npl + lw - 1) / lw;
   const int nmax = ny > nz ? ny : nz;

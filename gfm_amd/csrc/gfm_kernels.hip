@@ -1792,7 +1792,6 @@ __global__ __launch_bounds__(LINES_NT, LINES_MIN_WAVES) void k_lines(const LineI
   const int dj = (dir == 2) ? 2 : 0, dk = 2 - dj;  // lattice step along the line
   // arrays of the current values (uniform over the workgroup)
   const bool pcp = own_p && o + 1 < nlin, pcm = own_p && o > 0;
-  const T *own2 = own_p ? io.own2 : xt, *own3 = own_p ? io.own3 : xt;
   const T *oth2 = oth_p ? io.oth2 : xt, *oth3 = oth_p ? io.oth3 : xt;
   const T *cp2 = pcp ? io.own2 : xt, *cp3 = pcp ? io.own3 : xt;
   const T *cm2 = pcm ? io.own2 : xt, *cm3 = pcm ? io.own3 : xt;
@@ -1811,10 +1810,10 @@ __global__ __launch_bounds__(LINES_NT, LINES_MIN_WAVES) void k_lines(const LineI
     const T al_m = ld_off(coef_as_base<T>(s_lm), bq), al_p = ld_off(coef_as_base<T>(s_lp), bq);
     // current values: the cross-direction neighbours on the own plane (past
     // the last line of the box they lie outside it, in xt), and for G the
-    // value x0 and the x neighbours on the other colour's planes (a pair
-    // also holds the planes just outside the box in x, copies of xt there,
-    // so these need no test).  Each value is read from two arrays, a pair or
-    // xt twice, so that every load is issued unconditionally.
+    // x neighbours on the other colour's planes (a pair also holds the
+    // planes just outside the box in x, copies of xt there, so these need
+    // no test).  Each value is read from two arrays, a pair or xt twice, so
+    // that every load is issued unconditionally.
     const unsigned bcp = (unsigned)lidx(i, j + 2 - dj, k + 2 - dk) * (unsigned)sizeof(T);
     const unsigned bcm = (unsigned)lidx(i, j - 2 + dj, k - 2 + dk) * (unsigned)sizeof(T);
     const T xcpa = ld_off(cp2, bcp), xcpb = ld_off(cp3, bcp);
@@ -1830,12 +1829,11 @@ __global__ __launch_bounds__(LINES_NT, LINES_MIN_WAVES) void k_lines(const LineI
       const unsigned bxm = (unsigned)lidx(i - 2, j, k) * (unsigned)sizeof(T);
       const T bs = ld_off(coef_bs_base<T>(), bq);
       const T ax_p = ld_off(coef_as_base<T>(3), bq), ax_m = ld_off(coef_as_base<T>(0), bq);
-      const T x0a = ld_off(own2, bq), x0b = ld_off(own3, bq);
       const T xxpa = ld_off(oth2, bxp), xxpb = ld_off(oth3, bxp);
       const T xxma = ld_off(oth2, bxm), xxmb = ld_off(oth3, bxm);
-      const T x0 = own_p ? (x0a + x0b) / (T)2 : x0a;
       const T xxp = oth_p ? (xxpa + xxpb) / (T)2 : xxpa, xxm = oth_p ? (xxma + xxmb) / (T)2 : xxma;
-      gx = pinned ? x0 : bs + ax_p * xxp + ax_m * xxm;
+      // a pinned row keeps its value, which stays in xt for the whole call
+      gx = pinned ? ld_off(xt, bq) : bs + ax_p * xxp + ax_m * xxm;
       if (g_write) *reinterpret_cast<T *>(reinterpret_cast<char *>(io.gw) + bq) = gx;
     }
     const T xcp = pcp ? (xcpa + xcpb) / (T)2 : xcpa, xcm = pcm ? (xcma + xcmb) / (T)2 : xcma;
