@@ -3,6 +3,7 @@
 // that the __constant__ case block is shared without relocatable device
 // code.
 #include "gfm_kernels.hip"
+#include <hip/hip_ext.h>
 #include "../../include/gfm_hip.h"
 
 #include <math.h>
@@ -122,6 +123,24 @@ static void prof_begin(gfm_ctx *c) {
 static void prof_end(gfm_ctx *c, const char *name, double bytes) {
   Prof &p = c->prof;
   (void)hipEventRecord(p.ev[2 * p.used + 1], c->stream);
+  p.pending.push_back({prof_rec(c, name), bytes});
+  p.used++;
+}
+// The event pair of the next record, for a launch that records the
+// kernel's own start and end (hipExtLaunchKernelGGL): the span excludes
+// the launch latency that stream-recorded events around a launch include.
+static void prof_ext_events(gfm_ctx *c, hipEvent_t *e0, hipEvent_t *e1) {
+  Prof &p = c->prof;
+  if ((size_t)(2 * p.used + 2) > p.ev.size()) prof_flush(c);
+  if (p.ev.empty()) {
+    p.ev.resize(8192);
+    for (auto &e : p.ev) (void)hipEventCreate(&e);
+  }
+  *e0 = p.ev[2 * p.used];
+  *e1 = p.ev[2 * p.used + 1];
+}
+static void prof_ext_done(gfm_ctx *c, const char *name, double bytes) {
+  Prof &p = c->prof;
   p.pending.push_back({prof_rec(c, name), bytes});
   p.used++;
 }
@@ -371,11 +390,12 @@ static size_t lines_shm(int lw, int nmax, size_t word) { return 3 * (size_t)lw *
   X_(LW_, 6, 0) X_(LW_, 8, 0)
 #define LINES_INSTANCES(X_) LINES_INSTANCES_LW(X_, 32) LINES_INSTANCES_LW(X_, 16) LINES_INSTANCES_LW(X_, 2)
 template <typename T>
-static void launch_lines(gfm_ctx *c, int lw, int nu, int m, unsigned nblk, size_t shm, const LineIO<T> &io) {
-#define LINES_CASE(LW_, NU_, M_)                                                                        \
-  if (lw == LW_ && (M_ ? m == M_ : m == 0 && nu == NU_)) {                                              \
-    hipLaunchKernelGGL((k_lines<LW_, T, NU_, M_>), dim3(nblk), dim3(LINES_NT), shm, c->stream, io);     \
-    return;                                                                                            \
+static void launch_lines(gfm_ctx *c, int lw, int nu, int m, unsigned nblk, size_t shm, hipEvent_t e0, hipEvent_t e1,
+                         const LineIO<T> &io) {
+#define LINES_CASE(LW_, NU_, M_)                                                                                   \
+  if (lw == LW_ && (M_ ? m == M_ : m == 0 && nu == NU_)) {                                                         \
+    hipExtLaunchKernelGGL((k_lines<LW_, T, NU_, M_>), dim3(nblk), dim3(LINES_NT), shm, c->stream, e0, e1, 0, io); \
+    return;                                                                                                       \
   }
   LINES_INSTANCES(LINES_CASE)
 #undef LINES_CASE
@@ -410,9 +430,10 @@ static void line_pass(gfm_ctx *c, const LineIO<T> &io) {
   const int nmax = ny > nz ? ny : nz;
   const size_t shm = lines_shm(lw, nmax, sizeof(T));
   const unsigned nblk = (unsigned)(8 * ntile * ((nz + 7) / 8 + (ny + 7) / 8));  // XCD-balanced order (k_lines)
-  if (c->prof.on) prof_begin(c);
-  launch_lines<T>(c, lw, lines_nu(nmax), lines_m(nmax, c->line_pcr), nblk, shm, io);
-  if (c->prof.on) prof_end(c, "k_lines", line_bytes);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (c->prof.on) prof_ext_events(c, &e0, &e1);
+  launch_lines<T>(c, lw, lines_nu(nmax), lines_m(nmax, c->line_pcr), nblk, shm, e0, e1, io);
+  if (c->prof.on) prof_ext_done(c, "k_lines", line_bytes);
 }
 
 // XT = (X2 + X3)/2 on the planes i = pi0 + pdi*p (p < npl) of the box,
