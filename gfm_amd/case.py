@@ -185,6 +185,40 @@ def load_kinetic(nr0=100):
     return {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in out.items()}
 
 
+def dnst_calc_at(arr, sl):
+    """CONC then DNST_CALC on the lattice positions sl (C/Prop.f90:28-59,
+    187-198): H2O = 1 - sum of the others (all renormalised when that sum
+    exceeds 1), rho = (1 + p)/T * W_mix/W0, at least 1e-20."""
+    P, T, DN, GF = arr["p"], arr["t"], arr["dnst"], arr["gf"]
+    y = GF[sl]
+    s = y[..., IYF - 1] + y[..., IYO2 - 1] + y[..., IYCO2 - 1] + y[..., IYN2 - 1]
+    over = s > 1.0
+    if np.any(over):
+        s2 = s + y[..., IYH2O - 1]
+        for q in (IYF, IYO2, IYCO2, IYN2, IYH2O):
+            y[..., q - 1] = np.where(over, y[..., q - 1] / s2, y[..., q - 1])
+    y[..., IYH2O - 1] = np.where(over, y[..., IYH2O - 1], 1.0 - s)
+    GF[sl] = y
+    y = GF[sl]
+    wt = 1.0 / (y[..., IYF - 1] / WF + y[..., IYCO2 - 1] / WCO2 + y[..., IYH2O - 1] / WH2O +
+                y[..., IYO2 - 1] / WO2 + y[..., IYN2 - 1] / WN2)
+    DN[sl] = np.maximum((1.0 + P[sl]) / T[sl] * wt / WT0, 1e-20)
+
+
+def ther_at(arr, c, sl):
+    """THER on the lattice positions sl (C/Prop.f90:130-154): eps >= 1e-12,
+    mu_t = C_mu Re eps_r/(2 tau) k^2/eps rho, lambda = 1, c_p = CPF1,
+    Gamma = 2/Re (mu_t + 1)."""
+    GF, DN = arr["gf"], arr["dnst"]
+    GF[sl + (IEPS - 1,)] = np.maximum(GF[sl + (IEPS - 1,)], 1.0e-12)
+    kk = GF[sl + (IK - 1,)]
+    ee = GF[sl + (IEPS - 1,)]
+    arr["tmu"][sl] = CMU * c.reyg * c.epsr / (2.0 * c.taur) * kk ** 2 / ee * DN[sl]
+    arr["glam"][sl] = 1.0
+    arr["gcp"][sl] = CPF1
+    arr["gama"][sl] = 2.0 / c.reyg * (arr["tmu"][sl] + 1.0)
+
+
 class Case:
     """Host state of one case: ``ctrl`` (GfmCtrl) and ``arrays`` (dict of
     Fortran-ordered numpy arrays on the doubled lattice)."""
@@ -339,28 +373,10 @@ def build(spec: CaseSpec) -> Case:
     UG, GF = arr["ug"], arr["gf"]
 
     def dnst_calc(sl):
-        y = GF[sl]
-        s = y[..., IYF - 1] + y[..., IYO2 - 1] + y[..., IYCO2 - 1] + y[..., IYN2 - 1]
-        over = s > 1.0
-        if np.any(over):
-            s2 = s + y[..., IYH2O - 1]
-            for q in (IYF, IYO2, IYCO2, IYN2, IYH2O):
-                y[..., q - 1] = np.where(over, y[..., q - 1] / s2, y[..., q - 1])
-        y[..., IYH2O - 1] = np.where(over, y[..., IYH2O - 1], 1.0 - s)
-        GF[sl] = y
-        y = GF[sl]
-        wt = 1.0 / (y[..., IYF - 1] / WF + y[..., IYCO2 - 1] / WCO2 + y[..., IYH2O - 1] / WH2O +
-                    y[..., IYO2 - 1] / WO2 + y[..., IYN2 - 1] / WN2)
-        DN[sl] = np.maximum((1.0 + P[sl]) / T[sl] * wt / WT0, 1e-20)
+        dnst_calc_at(arr, sl)
 
     def ther(sl):
-        GF[sl + (IEPS - 1,)] = np.maximum(GF[sl + (IEPS - 1,)], 1.0e-12)
-        kk = GF[sl + (IK - 1,)]
-        ee = GF[sl + (IEPS - 1,)]
-        TMU[sl] = CMU * c.reyg * c.epsr / (2.0 * c.taur) * kk ** 2 / ee * DN[sl]
-        GLAM[sl] = 1.0
-        GCP[sl] = CPF1
-        GAMA[sl] = 2.0 / c.reyg * (TMU[sl] + 1.0)
+        ther_at(arr, c, sl)
 
     def tke(sl):
         g = UG[sl + (0,)] ** 2 + UG[sl + (1,)] ** 2 + UG[sl + (2,)] ** 2

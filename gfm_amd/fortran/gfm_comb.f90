@@ -2,8 +2,11 @@
 !
 ! Reads a case state (reference layout) from a stream file, runs SPHASE on
 ! the GPU for the requested iterations, writes the state back and prints
-! the res_mass history (the convg log line of C/Sphase.f90:53).
-!   usage: gfm_comb <state-in> <state-out> <iterations> [sweep_order]
+! the res_mass history (the convg log line of C/Sphase.f90:53).  With a
+! fifth argument it also writes the restart record rg####c.d there (SAV3F),
+! with a sixth the convg log of every iteration (convg####c.plt, header of
+! C/Utility.f90:1066-1080, record of C/Sphase.f90:53).
+!   usage: gfm_comb <state-in> <state-out> <iterations> [sweep_order [rg-out [convg-out]]]
 program gfm_comb
   use iso_c_binding
   use gfm_types
@@ -25,11 +28,25 @@ program gfm_comb
   if (order >= 0) ctrl%sweep_order = order
   ctrl%itr_gend = ctrl%itr_gas + niter
   poll_every = max(1, min(100, niter))
+  if (command_argument_count() >= 6) then
+    call get_command_argument(6, arg)
+    open (newunit=nug, file=trim(arg), status='replace')
+    log_convg = .true.
+    write (nug, *) '# Combustion space mean and max mass residual - log plot'
+    write (nug, *) '# PlotData'
+    write (nug, *) '# iteration  average mass residual  max mass residual'
+    poll_every = 1
+  end if
   call SPHASE
+  if (log_convg) close (nug)
   do i = 1, n_hist
     write (*, '(i7,es26.17e3)') ctrl%itr_gas - n_hist + i, res_hist(i)
   end do
   call write_state(trim(fout))
+  if (command_argument_count() >= 5) then
+    call get_command_argument(5, arg)
+    if (len_trim(arg) > 0) call SAV3F(trim(arg))
+  end if
   call gpu_close()
 contains
   subroutine read_state(path)

@@ -27,6 +27,14 @@ module gfm_state
   type(gfm_ctrl), target :: ctrl
   type(gfm_host_arrays), target :: harr
   character(len=256) :: casedir = '.'
+  ! restart-record state the combustion-space loop does not use
+  ! (C/Var_mod.f90:455,468,812,866; C/Setup.f90:755-756)
+  real(c_double), allocatable :: T_SURF_PREV(:, :)
+  integer :: i1st_rad_qe_done = 0, itr_rad = 0, interval_rad = 100
+  integer, parameter :: LEND = 7
+  ! convg log (C/Sphase.f90:53), written by SPHASE on unit nug when log_convg
+  integer :: nug = 0
+  logical :: log_convg = .false.
 contains
   subroutine allocate_state(m, n, l, nr)
     integer, intent(in) :: m, n, l, nr
@@ -38,6 +46,47 @@ contains
     allocate (CRATE(MP, NP, LP), HRATE(MP, NP, LP), BS_HPREV(MP, NP, LP))
     allocate (UG(MP, NP, LP, 3), GF(MP, NP, LP, 9))
     allocate (RTIM(NR0), DELRT(NR0), HRT(NR0), DELR(NR0), TIMED(NR0), HRD(NR0))
+    allocate (T_SURF_PREV(MP / 2, NP / 2))
+    T_SURF_PREV = 1500.0d0
+  end subroutine
+
+  ! SAV3F: the restart record rg####c.d of the combustion space
+  ! (C/Sav3f.f90:21-88, single phase): per cell centre (K innermost) the
+  ! record P, U(I+1), V(J+1), W(K+1), T, THETA (face index MP-1, NP-1, LP-1
+  ! on the last layer) and the record GF(:,:,:,1:LEND); then bs_hprev,
+  ! T_surf_prev, (FACTOR, FADJ), the radiation flags, itr_gas,
+  ! (itr_rad, itr_gend), interval_rad.
+  subroutine SAV3F(filename)
+    character(len=*), intent(in) :: filename
+    integer :: u, i, j, k, iu, ju, ku, l
+    integer(c_int32_t) :: qe, rad, itg, itr2(2), irad
+    real(c_double) :: rec6(6), ff(2)
+    open (newunit=u, file=filename, form='unformatted', access='sequential', status='replace')
+    do i = 2, MP, 2
+      iu = merge(MP - 1, i + 1, i == MP)
+      do j = 2, NP, 2
+        ju = merge(NP - 1, j + 1, j == NP)
+        do k = 2, LP, 2
+          ku = merge(LP - 1, k + 1, k == LP)
+          rec6 = [P(i, j, k), UG(iu, j, k, 1), UG(i, ju, k, 2), UG(i, j, ku, 3), T(i, j, k), THETA(i, j, k)]
+          write (u) rec6
+          write (u) (GF(i, j, k, l), l = 1, LEND)
+        end do
+      end do
+    end do
+    write (u) BS_HPREV
+    write (u) T_SURF_PREV
+    ff = [ctrl%factor, ctrl%fadj]
+    write (u) ff
+    qe = i1st_rad_qe_done; rad = ctrl%i1st_rad_done; itg = ctrl%itr_gas
+    itr2 = [int(itr_rad, c_int32_t), int(ctrl%itr_gend, c_int32_t)]
+    irad = interval_rad
+    write (u) qe
+    write (u) rad
+    write (u) itg
+    write (u) itr2
+    write (u) irad
+    close (u)
   end subroutine
 
   subroutine bind_host_arrays()
@@ -114,6 +163,9 @@ contains
         res_hist(n_hist + 1:n_hist + n_done) = chunk(1:n_done)
         n_hist = n_hist + n_done
       end if
+      ! the convg line of every iteration (needs poll_every = 1 for the
+      ! per-iteration maximum)
+      if (log_convg .and. n_done == 1) write (nug, '(i7,4e25.16)') ctrl%itr_gas, ctrl%res_mass, ctrl%res_mass_max
       if (n_done == 0) exit
       inquire (file=trim(casedir)//'/runstop.dat', exist=stop_req)
       if (stop_req) exit

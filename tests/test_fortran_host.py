@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from gfm_amd import case as cs
+from gfm_amd import plt, restart
 from gfm_amd.solver import CombustionSpace
 from gfm_amd.statefile import read_state, write_state
 
@@ -41,9 +42,10 @@ def test_fortran_sphase_matches_ctypes(tmp_path):
     base = cs.furnace_small(nx=16, ny=12, nz=10, seed=3)
     base.ctrl.sweep_order = 1
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    frg, fcv = tmp_path / "rg0001c.d", tmp_path / "convg0001c.plt"
     write_state(base, fin)
-    r = subprocess.run([EXE, str(fin), str(fout), "6", "1"], capture_output=True, text=True, timeout=120,
-                       cwd=str(tmp_path))
+    r = subprocess.run([EXE, str(fin), str(fout), "6", "1", str(frg), str(fcv)], capture_output=True, text=True,
+                       timeout=120, cwd=str(tmp_path))
     print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
     assert r.returncode == 0
     assert not (tmp_path / "runend.txt").exists()
@@ -59,3 +61,11 @@ def test_fortran_sphase_matches_ctypes(tmp_path):
         np.testing.assert_array_equal(out.arrays[k], g.arrays[k])
     np.testing.assert_array_equal(out.arrays["ug"], g.arrays["ug"])
     np.testing.assert_array_equal(out.arrays["gf"], g.arrays["gf"])
+    # the host's SAV3F writes the restart record the Python writer does
+    assert open(frg, "rb").read() == restart.records_bytes(restart.records_from_case(out))
+    # and its convg log has the reference's header and record format
+    lines = open(fcv).read().split("\n")
+    assert lines[:3] == plt.HEADERS["convg"]
+    for i, ln in enumerate(lines[3:9]):
+        assert ln[:7] == plt.fortran_i(i + 1, 7)
+        assert ln[7:32] == plt.fortran_e(hist_c[i], 25, 16)
