@@ -55,7 +55,7 @@ def cpu_baseline(spec_args, iters=2):
 
 
 PGUESS_DIR = os.path.join(ROOT, "gfm_amd", "data")
-THRESHOLDS = (1e-3, 1e-4, 1e-5)
+THRESHOLDS = (1e-3, 1e-4, 5e-5, 1e-5)
 
 
 def configure(case, mode, pcorr_tol):
@@ -72,7 +72,7 @@ def configure(case, mode, pcorr_tol):
         k.precision, k.pcorr_mode = 1, 0
     else:
         k.precision, k.pcorr_mode, k.pcorr_guess = 1, 1, 1 if mode == "learned" else 0
-        k.pcorr_tol, k.pcorr_max_sweeps = pcorr_tol, 200
+        k.pcorr_tol, k.pcorr_max_sweeps = pcorr_tol, 400
         k.pcorr_guess_accept = 0.5  # keep a guess only if it halves the start residual
 
 
@@ -110,7 +110,7 @@ def time_to_residual(spec_args, mode, net, cap, pcorr_tol):
         if net is not None:
             s.set_pguess(net)
         t0 = time.perf_counter()
-        n = 0
+        n = sweeps = 0
         while n < cap:
             h = s.sphase(1)
             if len(h) == 0:
@@ -118,13 +118,15 @@ def time_to_residual(spec_args, mode, net, cap, pcorr_tol):
             n += 1
             r = float(h[0])
             best = min(best, r)
+            sweeps += case.ctrl.pcorr_sweeps
             for t in THRESHOLDS:
                 if out[f"{t:g}"] is None and r < t:
                     out[f"{t:g}"] = {"seconds": round(time.perf_counter() - t0, 3), "iterations": n}
             if r < min(THRESHOLDS):
                 break
         total = time.perf_counter() - t0
-    return {"first_below": out, "iterations_run": n, "seconds": round(total, 3), "min_res_mass": best}
+    return {"first_below": out, "iterations_run": n, "seconds": round(total, 3), "min_res_mass": best,
+            "mean_pcorr_sweeps": round(sweeps / max(n, 1), 2)}
 
 
 def main():
@@ -142,6 +144,9 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned+FP32 configuration")
     ap.add_argument("--pcorr-tol", type=float, default=0.01,
                     help="p' tolerance of the learned+FP32 configuration (calc_residual)")
+    ap.add_argument("--pcorr-tol-tight", type=float, default=1e-3,
+                    help="p' tolerance of the second pair of time-to-residual runs (FP32 from zero, learned+FP32); "
+                         "0 skips them")
     ap.add_argument("--ttr-cap", type=int, default=800,
                     help="iteration cap of the time-to-residual runs (0 skips them)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -221,9 +226,16 @@ def main():
     if net is not None:
         line["learned_fp32"]["network"] = os.path.relpath(weights, ROOT)
     if args.ttr_cap > 0:
-        ttr = {"cap_iterations": args.ttr_cap}
+        ttr = {"cap_iterations": args.ttr_cap, "pcorr_tol": args.pcorr_tol}
         for name, mode, nn in [("plain_fp64", "fp64", None)] + others:
             ttr[name] = time_to_residual(spec_args, mode, nn, args.ttr_cap, args.pcorr_tol)
+        if net is not None and args.pcorr_tol_tight > 0:
+            # the p' solve relaxed further: ~100-150 sweeps per solve, where
+            # the guess saves more than its inference costs
+            ttr["pcorr_tol_tight"] = args.pcorr_tol_tight
+            for name, mode in (("fp32_zero_guess_tol_tight", "mixed_zero"), ("learned_fp32_tol_tight", "learned")):
+                ttr[name] = time_to_residual(spec_args, mode, net if mode == "learned" else None, args.ttr_cap,
+                                             args.pcorr_tol_tight)
         line["time_to_residual"] = ttr
     if net is not None:
         net.close()
