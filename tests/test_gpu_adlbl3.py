@@ -29,14 +29,17 @@ def _setup(order, sweeps, nx=20, precision=0, ny=14, nz=12):
     return c, box
 
 
-@pytest.mark.parametrize("order,nx,ny", [(1, 20, 14), (0, 20, 14), (1, 72, 14), (1, 34, 100), (1, 34, 140), (1, 20, 180),
-                                         (1, 20, 220)])
-def test_adlbl3_matches_oracle(order, nx, ny):
+@pytest.mark.parametrize("order,nx,ny,nz", [(1, 20, 14, 12), (0, 20, 14, 12), (1, 72, 14, 12), (1, 34, 50, 12),
+                                            (1, 34, 80, 12), (1, 20, 96, 64), (1, 34, 100, 12), (1, 34, 140, 12),
+                                            (1, 20, 180, 12), (1, 20, 220, 12)])
+def test_adlbl3_matches_oracle(order, nx, ny, nz):
     """Three sweeps from xt = 0 in the box: every value within 1e-12 of the
     oracle's (relative to the largest).  nx=72 (36 planes per colour) runs
-    the tiled line kernel; ny = 100, 140, 180, 220 give y-lines of 4, 5, 6
-    and 8 rows per lane."""
-    c, box = _setup(order, 3, nx, ny=ny)
+    the tiled line kernel.  Lines of up to 128 rows take the partition
+    solve with 2, 4, 6, 8 rows per lane (ny = 14, 50, 80, 100; the 96x64
+    plane of the 192x96x64 furnace: 6 and 4); ny = 140, 180, 220 the PCR
+    solve with 5, 6 and 8 rows per lane."""
+    c, box = _setup(order, 3, nx, ny=ny, nz=nz)
     ap, as_, bs, xt, xs, sl = manufactured(c, box, seed=11)
     xo = xt.copy(order="F")
     oracle.adlbl3(c, 4, ap, as_, bs, xo, box)
@@ -44,19 +47,19 @@ def test_adlbl3_matches_oracle(order, nx, ny):
         xg = xt.copy(order="F")
         s.adlbl3(4, ap, as_, bs, xg, box)
     err = np.max(np.abs(xg - xo)) / np.max(np.abs(xo))
-    print("adlbl3 order", order, "nx", nx, "ny", ny, "max rel diff", err, flush=True)
+    print("adlbl3 order", order, "nx", nx, "ny", ny, "nz", nz, "max rel diff", err, flush=True)
     assert err < 1e-12
     assert np.max(np.abs(xg[sl] - xs[sl])) > 1e-6  # not yet converged: the comparison is of the iterates
 
 
-@pytest.mark.parametrize("nx", [20, 72])
-def test_adlbl3_fp32_matches_oracle(nx):
+@pytest.mark.parametrize("nx,ny,nz", [(20, 14, 12), (72, 14, 12), (20, 96, 64)])
+def test_adlbl3_fp32_matches_oracle(nx, ny, nz):
     """Mixed-precision context: coefficients rounded to FP32 and FP32 line
     sweeps (nx=72: tiles of 32 planes of one colour and a partial tile).
     After three sweeps every value within 2e-5 of the FP64 oracle's
     (relative to the largest); after 60 the manufactured solution is met
     to 2e-5 (the FP32 rounding floor of the line solves)."""
-    c, box = _setup(1, 3, nx, precision=1)
+    c, box = _setup(1, 3, nx, precision=1, ny=ny, nz=nz)
     ap, as_, bs, xt, xs, sl = manufactured(c, box, seed=14)
     xo = xt.copy(order="F")
     oracle.adlbl3(c, 4, ap, as_, bs, xo, box)
