@@ -1529,6 +1529,9 @@ __global__ void k_pair_xbound(const T *xt, T *x2, T *x3, T *x4, T *x5, int ib, i
 }
 
 // FP32 working copy of an unknown over a box (mixed precision) and back.
+// The copy back skips the rows the line kernel pins (cell code >= 1 or
+// |a_p| <= 1e-20, the same test on the same FP32 a_p), which keep their FP64
+// values.
 __global__ void k_box_narrow(Box b, const double *src, float *dst) {
   int i, j, k;
   if (!box_pt(b, i, j, k)) return;
@@ -1539,6 +1542,7 @@ __global__ void k_box_widen(Box b, const float *src, double *dst) {
   int i, j, k;
   if (!box_pt(b, i, j, k)) return;
   long long q = lidx(i, j, k);
+  if (g.ib[q] >= 1 || fabs((double)g.apf[q]) <= SMALL20) return;
   dst[q] = (double)src[q];
 }
 

@@ -78,6 +78,28 @@ def test_adlbl3_fp32_matches_oracle(nx, ny, nz):
     assert sol < 2e-5
 
 
+def test_fp32_sweeps_keep_fp64_values_of_pinned_rows():
+    """In a mixed-precision context the FP32 working copy is written back
+    only where the line kernel does not pin the row: rows with a_p = 0
+    inside the box keep FP64 values that FP32 cannot hold."""
+    c, box = _setup(1, 3, 20, precision=1)
+    ap, as_, bs, xt, xs, sl = manufactured(c, box, seed=15)
+    rng = np.random.default_rng(3)
+    ii = rng.integers(0, xt[sl].shape[0], 12)
+    jj = rng.integers(0, xt[sl].shape[1], 12)
+    kk = rng.integers(0, xt[sl].shape[2], 12)
+    pos = (box[0] - 1 + 2 * ii, box[2] - 1 + 2 * jj, box[4] - 1 + 2 * kk)
+    ap[pos] = 0.0
+    vals = 1.0 / 3.0 + 1e-9 * np.arange(12)
+    xt[pos] = vals
+    assert np.all(vals.astype(np.float32).astype(np.float64) != vals)
+    with CombustionSpace(c, initsv=False) as s:
+        s.adlbl3(4, ap, as_, bs, xt, box)
+    np.testing.assert_array_equal(xt[pos], vals)
+    opn = (c.arrays["ibcell"][sl] <= 0) & (ap[sl] > 0)
+    assert np.max(np.abs(xt[sl][opn])) > 0  # the open rows were relaxed
+
+
 @pytest.mark.parametrize("order", [1, 0])
 def test_adlbl3_reaches_solution(order):
     c, box = _setup(order, 60)
