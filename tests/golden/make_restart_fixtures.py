@@ -6,7 +6,10 @@ where the reference is mounted; the outputs are committed):
   TC21 case (lattice 40x48x36 from gd0041c.dat), gzip-compressed, with its
   SHA-256 in restart_fixtures.json;
 * plt/<name>.plt -- the first lines of the case's convergence logs
-  (convg, gresid, gresid_xtra, Tave of case0041), CRLF line ends kept.
+  (convg, gresid, gresid_xtra, Tave of case0041), CRLF line ends kept;
+* grid/gd0041c.dat, grid/gd1206c.dat -- two grid files (the TC21 furnace
+  and the 154x28x30 case), with the cell count and melt-surface area their
+  runs logged (summary####c.txt lines 3-4) in grid_fixtures.json.
 """
 import gzip
 import hashlib
@@ -34,6 +37,17 @@ def main(ref=REF):
         open(os.path.join(HERE, "plt", name + ".plt"), "wb").write(b"\n".join(lines) + b"\n")
     json.dump(meta, open(os.path.join(HERE, "restart_fixtures.json"), "w"), indent=1)
     print(meta)
+    grid = {}
+    os.makedirs(os.path.join(HERE, "grid"), exist_ok=True)
+    for cn in ("0041", "1206"):
+        d = os.path.join(ref, "case" + cn)
+        text = open(os.path.join(d, f"gd{cn}c.dat"), "rb").read()
+        open(os.path.join(HERE, "grid", f"gd{cn}c.dat"), "wb").write(text)
+        lines = open(os.path.join(d, f"summary{cn}c.txt")).read().splitlines()
+        grid[f"gd{cn}c.dat"] = {"ncells": int(lines[2].split(":")[1]),
+                                "area_melt_surf": float(lines[3].split(":")[1].split()[0])}
+    json.dump(grid, open(os.path.join(HERE, "grid_fixtures.json"), "w"), indent=1)
+    print(grid)
 
 
 if __name__ == "__main__":
