@@ -7,9 +7,10 @@ where the reference is mounted; the outputs are committed):
   SHA-256 in restart_fixtures.json;
 * plt/<name>.plt -- the first lines of the case's convergence logs
   (convg, gresid, gresid_xtra, Tave of case0041), CRLF line ends kept;
-* grid/gd0041c.dat, grid/gd1206c.dat -- two grid files (the TC21 furnace
-  and the 154x28x30 case), with the cell count and melt-surface area their
-  runs logged (summary####c.txt lines 3-4) in grid_fixtures.json.
+* grid/gd0041c.dat, grid/gd1206c.dat and the sbc files beside them -- two
+  cases (the TC21 furnace and the 154x28x30 case), with the cell count,
+  melt-surface area and inflow rates their runs logged (summary####c.txt
+  lines 3-10) in grid_fixtures.json.
 """
 import gzip
 import hashlib
@@ -41,11 +42,18 @@ def main(ref=REF):
     os.makedirs(os.path.join(HERE, "grid"), exist_ok=True)
     for cn in ("0041", "1206"):
         d = os.path.join(ref, "case" + cn)
-        text = open(os.path.join(d, f"gd{cn}c.dat"), "rb").read()
-        open(os.path.join(HERE, "grid", f"gd{cn}c.dat"), "wb").write(text)
+        for kind in ("gd", "sbc"):
+            text = open(os.path.join(d, f"{kind}{cn}c.dat"), "rb").read()
+            open(os.path.join(HERE, "grid", f"{kind}{cn}c.dat"), "wb").write(text)
         lines = open(os.path.join(d, f"summary{cn}c.txt")).read().splitlines()
-        grid[f"gd{cn}c.dat"] = {"ncells": int(lines[2].split(":")[1]),
-                                "area_melt_surf": float(lines[3].split(":")[1].split()[0])}
+        logged = {}
+        for ln in lines[:12]:
+            for key, name in (("Fuel in flow rate", "fr_f"), ("Oxygen in flow rate", "fr_ox"),
+                              ("Nitrogen in flow rate", "fr_n2"), ("Heat value in with fuel", "heat_in")):
+                if key in ln:
+                    logged[name] = float(ln.split(":")[1].split()[0])
+        grid[f"gd{cn}c.dat"] = dict(ncells=int(lines[2].split(":")[1]),
+                                    area_melt_surf=float(lines[3].split(":")[1].split()[0]), **logged)
     json.dump(grid, open(os.path.join(HERE, "grid_fixtures.json"), "w"), indent=1)
     print(grid)
 
