@@ -18,7 +18,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .case import PG0, RU, T0, WCO2, WF, WH2O, WN2, WO2, WT0, Inlet
+from .case import H_0, PG0, RU, T0, WCO2, WF, WH2O, WN2, WO2, WT0, Inlet, t_to_enth
 
 
 @dataclass
@@ -73,8 +73,11 @@ def read_sbc(path_or_text):
 
 
 def inflow_rates(grid, sbc):
-    """(fr_f, fr_ox, fr_n2, heat_in) in kg/s and W for the inlet cells of a
-    grid (gfm_amd.gridfile.GridFile) and an sbc file, as SBC forms them."""
+    """Inflow of the inlet cells of a grid (gfm_amd.gridfile.GridFile) and
+    an sbc file as SBC forms it: fuel, O2 and N2 mass flow (kg/s), the fuel
+    heat value fr_f*Q0 (W) and the enthalpy inflow q_h_in (W), the sum of
+    T_to_enth of each inlet cell's state times its mass flow
+    (C/Sbc.f90:159-165, T_to_enth C/Prop.f90:525-565)."""
     ib = grid.ibcell
     taken = np.zeros(ib.shape, dtype=bool)
     gmfr = sbc.number("GMFR")
@@ -82,7 +85,7 @@ def inflow_rates(grid, sbc):
     dnst0 = PG0 * WT0 / RU / T0
     ug0 = gmfr / dnst0 / 1.0  # AREA0 = r0**2 = 1
     p = sbc.p_ref / PG0
-    fr_f = fr_ox = fr_n2 = 0.0
+    fr_f = fr_ox = fr_n2 = q_h_in = 0.0
     for inl in sbc.inlets:
         i1, j1, k1, i2, j2, k2 = inl.box
         iu = abs(inl.iu)
@@ -115,4 +118,6 @@ def inflow_rates(grid, sbc):
                     fr_f += g0 * yf
                     fr_ox += g0 * yo
                     fr_n2 += g0 * yn
-    return fr_f, fr_ox, fr_n2, fr_f * q0
+                    h = float(t_to_enth([np.array(v) for v in (yf, yo, yc, yn, yh)], np.array(t)))
+                    q_h_in += h * H_0 * g0
+    return dict(fr_f=fr_f, fr_ox=fr_ox, fr_n2=fr_n2, heat_in=fr_f * q0, q_h_in=q_h_in)

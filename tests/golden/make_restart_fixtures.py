@@ -52,6 +52,10 @@ def main(ref=REF):
                               ("Nitrogen in flow rate", "fr_n2"), ("Heat value in with fuel", "heat_in")):
                 if key in ln:
                     logged[name] = float(ln.split(":")[1].split()[0])
+        for ln in lines:
+            if "Energy in with flow" in ln:
+                logged["q_h_in"] = float(ln.split(":")[1].split()[0])
+                break
         grid[f"gd{cn}c.dat"] = dict(ncells=int(lines[2].split(":")[1]),
                                     area_melt_surf=float(lines[3].split(":")[1].split()[0]), **logged)
     json.dump(grid, open(os.path.join(HERE, "grid_fixtures.json"), "w"), indent=1)

@@ -1,7 +1,8 @@
 """Boundary-condition files sbc####c.dat (gfm_amd/sbcfile.py): the inlet
-blocks and the inflow rates SBC forms from them (C/Sbc.f90:19-170) against
-the rates the reference's runs logged (C/Utility.f90:101-105), and the
-case builder's own inflow against the same function."""
+blocks and the inflow SBC forms from them (C/Sbc.f90:19-170) against what
+the reference's runs logged (C/Utility.f90:101-105,131: mass inflow, fuel
+heat value, enthalpy inflow -- the last pins T_to_enth at the inlet
+states), and the case builder's own inflow against the same function."""
 import glob
 import json
 import os
@@ -34,30 +35,41 @@ def test_inflow_rates_match_the_logged_ones(cn):
     want = json.load(open(os.path.join(GOLD, "grid_fixtures.json")))[f"gd{cn}c.dat"]
     g = gfile.read_gd(os.path.join(GOLD, "grid", f"gd{cn}c.dat"))
     s = sf.read_sbc(os.path.join(GOLD, "grid", f"sbc{cn}c.dat"))
-    got = dict(zip(("fr_f", "fr_ox", "fr_n2", "heat_in"), sf.inflow_rates(g, s)))
+    got = sf.inflow_rates(g, s)
     for k in ("fr_f", "fr_ox", "fr_n2", "heat_in"):
         assert got[k] == pytest.approx(want[k], rel=1e-13), k
+    if cn == "0041":  # the enthalpy inflow (case 1206's log does not match its own inlets)
+        assert got["q_h_in"] == pytest.approx(want["q_h_in"], rel=1e-13)
 
 
 @pytest.mark.skipif(not os.path.isdir(REF), reason="reference data files not mounted")
 def test_every_reference_case_gives_its_logged_inflow():
-    n = 0
+    n = nq = 0
     for sm in glob.glob(os.path.join(REF, "case*", "summary*c.txt")):
         d = os.path.dirname(sm)
         gd, sb = glob.glob(os.path.join(d, "gd*c.dat")), glob.glob(os.path.join(d, "sbc*c.dat"))
         if not gd or not sb:
             continue
         logged = {}
-        for ln in open(sm).read().splitlines()[:12]:
+        for ln in open(sm).read().splitlines():
             if "Fuel in flow rate" in ln:
                 logged["f"] = float(ln.split(":")[1].split()[0])
             if "Total in flow rate" in ln:
                 logged["t"] = float(ln.split(":")[1].split()[0])
-        f, o, nn, _ = sf.inflow_rates(gfile.read_gd(gd[0]), sf.read_sbc(sb[0]))
-        assert f == pytest.approx(logged["f"], rel=1e-13), d
-        assert f + o + nn == pytest.approx(logged["t"], rel=1e-13), d
+            if "Energy in with flow" in ln and "q" not in logged:
+                logged["q"] = float(ln.split(":")[1].split()[0])
+        r = sf.inflow_rates(gfile.read_gd(gd[0]), sf.read_sbc(sb[0]))
+        assert r["fr_f"] == pytest.approx(logged["f"], rel=1e-13), d
+        assert r["fr_f"] + r["fr_ox"] + r["fr_n2"] == pytest.approx(logged["t"], rel=1e-13), d
+        # enthalpy inflow: reproduced for every run of the current code; the
+        # runs kept as "oldrun"/"bak" log 2 % less, and cases 1206/1207 log
+        # 3.8e5 W for 13 kg/s of mostly 1400 K air, which their inlet files do
+        # not give whatever the air temperature
+        if not ("oldrun" in d or "bak" in d or "case120" in d):
+            assert r["q_h_in"] == pytest.approx(logged["q"], rel=1e-13), d
+            nq += 1
         n += 1
-    assert n >= 15
+    assert n >= 15 and nq >= 10
 
 
 @pytest.mark.parametrize("make", [lambda: cs.furnace_small(nx=16, ny=12, nz=10, seed=3),
@@ -71,5 +83,6 @@ def test_case_builder_inflow_is_sbc_inflow(make):
     g = SimpleNamespace(ibcell=a["ibcell"], dx=a["dx"], dr=a["dr"], dz=a["dz"])
     s = sf.SbcFile({"GMFR": repr(c.ctrl.gmfr), "Q0": repr(c.spec.q0)}, (c.ctrl.i_ref, c.ctrl.j_ref, c.ctrl.k_ref),
                    0.0, c.spec.inlets)
-    f, o, n, _ = sf.inflow_rates(g, s)
-    assert (f, o, n) == pytest.approx((c.meta["fr_f"], c.meta["fr_ox"], c.meta["fr_n2"]), rel=1e-15)
+    r = sf.inflow_rates(g, s)
+    assert (r["fr_f"], r["fr_ox"], r["fr_n2"]) == pytest.approx((c.meta["fr_f"], c.meta["fr_ox"], c.meta["fr_n2"]),
+                                                                rel=1e-15)
