@@ -1553,9 +1553,11 @@ __global__ void k_box_widen(Box b, const float *src, double *dst) {
 // rows of the tile cooperatively, every load of a row issued before any is
 // used, forms the tridiagonal rows of ADLZY/ADLYZ (C/Adlbl3.f90:147-221)
 // and stores them in LDS normalised to a unit diagonal, [line][row] with an
-// odd row pitch.  Each wavefront then solves two lines, 32 lanes and up to
-// NU rows per lane each, by parallel cyclic reduction with
-// wavefront-scope synchronisation only, and the tile is written back
+// odd row pitch.  The lines are then solved -- with M > 0 by the partition
+// solve, four lines per wavefront, 16 lanes of M contiguous rows per line
+// (line_partition_solve); with M = 0 by parallel cyclic reduction, two lines
+// per wavefront, 32 lanes and up to NU rows per lane each, with
+// wavefront-scope synchronisation only -- and the tile is written back
 // cooperatively.  A workgroup does y-lines into xy or z-lines into xz.  T
 // is the arithmetic and storage type of the sweep (FP64, or FP32 in mixed
 // precision, where xt is the FP32 working copy).
