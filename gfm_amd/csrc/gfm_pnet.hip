@@ -33,6 +33,16 @@ struct PGrid {
 };
 __host__ __device__ __forceinline__ long long pgrid_cells(const PGrid &q) { return (long long)q.B * q.D * q.H * q.W; }
 
+// Raw buffer over an activation tensor for the convolution's gathers: the
+// range check (offsets from 2^31 up read as zero) stands for the padding.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pconv_rsrc(const pbf *base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<pbf *>(base), (short)0, 0x7FFFFFFF, 0x00020000);
+}
+__device__ __forceinline__ pbf8 pconv_load(__amdgpu_buffer_rsrc_t rs, unsigned byte_off) {
+  typedef unsigned pu4 __attribute__((ext_vector_type(4)));
+  return __builtin_bit_cast(pbf8, (pu4)__builtin_amdgcn_raw_buffer_load_b128(rs, (int)byte_off, 0, 0));
+}
+
 // ---------------------------------------------------------------------
 // 3x3x3 (or 1-tap) convolution as an implicit GEMM on the matrix cores:
 // out[v][co] = act(bias[co] + sum_{tap,ci} in(v,tap)[ci] * w[tap][co][ci]).
@@ -43,66 +53,127 @@ __host__ __device__ __forceinline__ long long pgrid_cells(const PGrid &q) { retu
 // v_mfma_f32_16x16x32_bf16: A = 16 voxels x 32 input channels (each lane 8
 // consecutive channels of one voxel, one 16-byte load), B = 32 input
 // channels x 16 output channels from w (each B fragment feeds MT MFMAs),
-// D rows = voxels, columns = output channels.  256 threads = 4 wavefronts
-// = 64*MT voxels per workgroup.
+// D rows = voxels, columns = output channels.  256 threads = 4 wavefronts:
+// with KS = 1 each takes its own MT*16 voxels (64*MT per workgroup); with
+// KS = 4, for the coarse grids, the four share one tile of MT*16 voxels and
+// each runs a quarter of the (tap, chunk) steps, their sums added through
+// LDS in a fixed order.
 // ---------------------------------------------------------------------
-template <int NCT, int MT>
-__global__ __launch_bounds__(256) void k_pconv(const pbf *__restrict__ xa, int ca, int up_a, const pbf *__restrict__ xb,
-                                               int cb, const pbf *__restrict__ w, const float *__restrict__ bias,
-                                               pbf *__restrict__ ya, int split, pbf *__restrict__ yb, int relu, PGrid pg,
-                                               int ntap) {
-  constexpr int CO = NCT * 16;
+template <int NCT, int MT, int KS>
+__global__ __launch_bounds__(256, NCT <= 2 ? 4 : 2) void k_pconv(const pbf *__restrict__ xa, int ca, int up_a,
+                                                                  const pbf *__restrict__ xb, int cb,
+                                                                  const pbf *__restrict__ w, const float *__restrict__ bias,
+                                                                  pbf *__restrict__ ya, int split, pbf *__restrict__ yb,
+                                                                  int relu, PGrid pg, int ntap) {
+  static_assert(KS == 1 || KS == 4, "steps split over 1 or 4 wavefronts");
+  constexpr int CO = NCT * 16, VG = 4 / KS;  // VG voxel tiles per workgroup
   const int cin = ca + cb;
-  const long long V = pgrid_cells(pg);
+  const unsigned V = (unsigned)pgrid_cells(pg);  // pconv() keeps V and every byte offset below 2^31
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, r16 = l & 15, kg = l >> 4;
-  const long long vb = (long long)blockIdx.x * (64 * MT) + wave * (16 * MT);
-  int vx[MT], vy[MT], vz[MT], vbt[MT];
-  bool vin[MT];
+  const int kw = wave % KS;  // which part of the steps this wavefront runs
+  const unsigned vb = blockIdx.x * (VG * 16u * MT) + (wave / KS) * (16u * MT);
+  const int Dh = pg.D >> 1, Hh = pg.H >> 1, Wh = pg.W >> 1;
+  // Per voxel tile, computed once: the lane's byte offsets of its own voxel
+  // in xa and xb (its 8-channel group included), the inverted in-grid mask
+  // of the 27 taps (bit t, t = 9(dz+1) + 3(dy+1) + dx+1) and, for a
+  // half-resolution xa, the parities of z, y, x.  Inside the tap loop a
+  // neighbour then costs a uniform base shift plus, for the lane, an OR of
+  // its mask bit into bit 31: a buffer load past the descriptor's range
+  // returns zero, which is the zero padding of the convolution.
+  unsigned ofa[MT], ofb[MT], ntm[MT];
+  bool pz[MT], py[MT], px[MT];
 #pragma unroll
   for (int mt = 0; mt < MT; mt++) {
-    long long v = vb + mt * 16 + r16;
-    vin[mt] = v < V;
-    long long r = vin[mt] ? v : 0;
-    vx[mt] = (int)(r % pg.W); r /= pg.W;
-    vy[mt] = (int)(r % pg.H); r /= pg.H;
-    vz[mt] = (int)(r % pg.D);
-    vbt[mt] = (int)(r / pg.D);
+    const unsigned v = vb + mt * 16 + r16;
+    const bool in = v < V;
+    unsigned r = in ? v : 0;
+    const int x = (int)(r % (unsigned)pg.W); r /= (unsigned)pg.W;
+    const int y = (int)(r % (unsigned)pg.H); r /= (unsigned)pg.H;
+    const int z = (int)(r % (unsigned)pg.D);
+    const int bt = (int)(r / (unsigned)pg.D);
+    const unsigned vo = in ? v : 0;
+    ofb[mt] = (vo * (unsigned)cb + 8u * kg) * 2u;
+    const unsigned va = up_a ? (((unsigned)(bt * Dh + (z >> 1)) * Hh + (y >> 1)) * Wh + (x >> 1)) : vo;
+    ofa[mt] = (va * (unsigned)ca + 8u * kg) * 2u;
+    pz[mt] = z & 1; py[mt] = y & 1; px[mt] = x & 1;
+    const unsigned mx = (x > 0) | 2u | ((unsigned)(x < pg.W - 1) << 2);
+    const unsigned my = (y > 0) | 2u | ((unsigned)(y < pg.H - 1) << 2);
+    const unsigned mz = (z > 0) | 2u | ((unsigned)(z < pg.D - 1) << 2);
+    const unsigned ey = ((my & 1u) ? 7u : 0u) | ((my & 2u) ? 070u : 0u) | ((my & 4u) ? 0700u : 0u);
+    const unsigned ez = ((mz & 1u) ? 0x1FFu : 0u) | ((mz & 2u) ? 0x1FFu << 9 : 0u) | ((mz & 4u) ? 0x1FFu << 18 : 0u);
+    const unsigned tm = (ntap == 1) ? 1u : (mx * 01111111111u) & (ey * 0x40201u) & ez;
+    ntm[mt] = in ? ~tm : ~0u;
   }
-  const int Dh = pg.D >> 1, Hh = pg.H >> 1, Wh = pg.W >> 1;
   pf4 acc[MT][NCT];
 #pragma unroll
   for (int mt = 0; mt < MT; mt++)
 #pragma unroll
     for (int ct = 0; ct < NCT; ct++) acc[mt][ct] = pf4{0.f, 0.f, 0.f, 0.f};
-  pbf8 zero8;
-#pragma unroll
-  for (int e = 0; e < 8; e++) zero8[e] = __builtin_bit_cast(__bf16, (pbf)0);
-  for (int t = 0; t < ntap; t++) {
+  // byte strides of a half-resolution xa, and the lane's offset in w
+  const int sza = Hh * Wh * ca * 2, sya = Wh * ca * 2, sxa = ca * 2;
+  const unsigned wl = (unsigned)(r16 * cin + 8 * kg) * 2u;
+  // One step per (tap, 32-channel chunk).  The fragments of the next step
+  // are loaded before the MFMAs of the current one, so a wavefront keeps
+  // its gathers in flight while its matrix cores work.
+  const int nch = cin / 32, nstep = ntap * nch;
+  auto fetch = [&](int st, pbf8 (&A)[MT], pbf8 (&Bf)[NCT]) {
+    const int t = st / nch, ch = (st - t * nch) * 32;
     const int dz = (ntap == 1) ? 0 : t / 9 - 1, dy = (ntap == 1) ? 0 : (t / 3) % 3 - 1, dx = (ntap == 1) ? 0 : t % 3 - 1;
-    long long na[MT], nb[MT];
-#pragma unroll
-    for (int mt = 0; mt < MT; mt++) {
-      const int z = vz[mt] + dz, y = vy[mt] + dy, x = vx[mt] + dx;
-      const bool in = vin[mt] && z >= 0 && z < pg.D && y >= 0 && y < pg.H && x >= 0 && x < pg.W;
-      nb[mt] = in ? (((long long)vbt[mt] * pg.D + z) * pg.H + y) * pg.W + x : -1;
-      na[mt] = !in ? -1 : up_a ? (((long long)vbt[mt] * Dh + (z >> 1)) * Hh + (y >> 1)) * Wh + (x >> 1) : nb[mt];
-    }
-    for (int ch = 0; ch < cin; ch += 32) {
-      const bool fa = ch < ca;
-      pbf8 A[MT];
+    const long long toff = ((long long)dz * pg.H + dy) * pg.W + dx;  // neighbour shift in voxels
+    if (ch < ca && up_a) {
+      const __amdgpu_buffer_rsrc_t rs = pconv_rsrc(xa + ch);
 #pragma unroll
       for (int mt = 0; mt < MT; mt++) {
-        const long long nv = fa ? na[mt] : nb[mt];
-        const pbf *src = fa ? xa + nv * ca + ch : xb + nv * cb + (ch - ca);
-        A[mt] = nv >= 0 ? *reinterpret_cast<const pbf8 *>(src + 8 * kg) : zero8;
+        // floor((z+dz)/2) - floor(z/2) is -1, 0 or 1 by dz and the parity of z
+        const int oz = dz == 0 ? 0 : dz < 0 ? (pz[mt] ? 0 : -sza) : (pz[mt] ? sza : 0);
+        const int oy = dy == 0 ? 0 : dy < 0 ? (py[mt] ? 0 : -sya) : (py[mt] ? sya : 0);
+        const int ox = dx == 0 ? 0 : dx < 0 ? (px[mt] ? 0 : -sxa) : (px[mt] ? sxa : 0);
+        A[mt] = pconv_load(rs, (ofa[mt] + (unsigned)(oz + oy + ox)) | ((ntm[mt] >> t) << 31));
       }
+    } else {
+      const bool fa = ch < ca;
+      const __amdgpu_buffer_rsrc_t rs = fa ? pconv_rsrc(xa + toff * ca + ch) : pconv_rsrc(xb + toff * cb + (ch - ca));
 #pragma unroll
-      for (int ct = 0; ct < NCT; ct++) {
-        const pbf8 Bf = *reinterpret_cast<const pbf8 *>(w + ((long long)t * CO + ct * 16 + r16) * cin + ch + 8 * kg);
-#pragma unroll
-        for (int mt = 0; mt < MT; mt++) acc[mt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[mt], Bf, acc[mt][ct], 0, 0, 0);
-      }
+      for (int mt = 0; mt < MT; mt++) A[mt] = pconv_load(rs, (fa ? ofa[mt] : ofb[mt]) | ((ntm[mt] >> t) << 31));
     }
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++)
+      Bf[ct] = *reinterpret_cast<const pbf8 *>(reinterpret_cast<const char *>(w + ((long long)t * CO + ct * 16) * cin + ch) + wl);
+  };
+  auto mma = [&](const pbf8 (&A)[MT], const pbf8 (&Bf)[NCT]) {
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+      for (int mt = 0; mt < MT; mt++) acc[mt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[mt], Bf[ct], acc[mt][ct], 0, 0, 0);
+  };
+  // two fragment sets used in turn (no register copies between steps)
+  pbf8 A0[MT], B0[NCT], A1[MT], B1[NCT];
+  const int st1 = (kw + 1) * nstep / KS;
+  int st = kw * nstep / KS;
+  fetch(st, A0, B0);
+  for (; st + 1 < st1; st += 2) {
+    fetch(st + 1, A1, B1);
+    mma(A0, B0);
+    if (st + 2 < st1) fetch(st + 2, A0, B0);
+    mma(A1, B1);
+  }
+  if (st < st1) mma(A0, B0);
+  if constexpr (KS > 1) {
+    __shared__ pf4 red[VG][KS - 1][MT * NCT][64];
+    if (kw > 0) {
+#pragma unroll
+      for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++) red[wave / KS][kw - 1][mt * NCT + ct][l] = acc[mt][ct];
+    }
+    __syncthreads();
+    if (kw > 0) return;
+#pragma unroll
+    for (int j = 0; j < KS - 1; j++)
+#pragma unroll
+      for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++) acc[mt][ct] += red[wave / KS][j][mt * NCT + ct][l];
   }
 #pragma unroll
   for (int mt = 0; mt < MT; mt++)
@@ -112,12 +183,12 @@ __global__ __launch_bounds__(256) void k_pconv(const pbf *__restrict__ xa, int c
       const float bv = bias ? bias[co] : 0.f;
 #pragma unroll
       for (int e = 0; e < 4; e++) {
-        const long long v = vb + mt * 16 + 4 * kg + e;
+        const unsigned v = vb + mt * 16 + 4 * kg + e;
         if (v >= V) continue;
         float val = acc[mt][ct][e] + bv;
         if (relu) val = fmaxf(val, 0.f);
-        if (co < split) ya[v * split + co] = pbf_from_f(val);
-        else yb[v * (CO - split) + (co - split)] = pbf_from_f(val);
+        if (co < split) ya[(size_t)v * split + co] = pbf_from_f(val);
+        else yb[(size_t)v * (CO - split) + (co - split)] = pbf_from_f(val);
       }
     }
 }
@@ -466,16 +537,56 @@ static int pnet_cast(gfm_pnet *n) {
   return GFM_OK;
 }
 
-// voxel tiles per wavefront: accumulators stay near 64 registers
+// The voxel tile by grid size, so that a launch has about 1,024
+// workgroups or more: MT = 4 (accumulators near 64 registers) on the fine
+// grids, MT = 2 or 1 on the middle ones, and on the coarsest MT = 1 with
+// the steps split over the four wavefronts of a workgroup.  The
+// environment variable GFM_PCONV_TILE = 4, 2, 1 or 1x4 fixes the choice
+// for every grid (4 means 2 for 96 and 128 output channels).  The choices
+// differ only in FP32 summation order: with KS = 4 four partial sums over
+// the steps are added at the end.
+static int pconv_tile_env() {
+  const char *e = getenv("GFM_PCONV_TILE");
+  if (!e || !*e || !strcmp(e, "auto")) return 0;
+  if (!strcmp(e, "4")) return 4;
+  if (!strcmp(e, "2")) return 2;
+  if (!strcmp(e, "1")) return 1;
+  if (!strcmp(e, "1x4")) return 14;
+  return -1;
+}
+template <int NCT, int MT, int KS>
+static void pconv_launch1(hipStream_t s, const pbf *xa, int ca, int up_a, const pbf *xb, int cb, const pbf *w,
+                          const float *bias, pbf *ya, int split, pbf *yb, int relu, PGrid pg, int ntap) {
+  const long long V = pgrid_cells(pg);
+  hipLaunchKernelGGL((k_pconv<NCT, MT, KS>), dim3(pblocks(V, 16 * MT * (4 / KS))), dim3(256), 0, s, xa, ca, up_a, xb, cb,
+                     w, bias, ya, split, yb, relu, pg, ntap);
+}
 template <int NCT, int MT>
 static void pconv_launch(hipStream_t s, const pbf *xa, int ca, int up_a, const pbf *xb, int cb, const pbf *w, const float *bias,
                          pbf *ya, int split, pbf *yb, int relu, PGrid pg, int ntap) {
   const long long V = pgrid_cells(pg);
-  hipLaunchKernelGGL((k_pconv<NCT, MT>), dim3(pblocks(V, 64 * MT)), dim3(256), 0, s, xa, ca, up_a, xb, cb, w, bias, ya, split,
-                     yb, relu, pg, ntap);
+  const int tile = pconv_tile_env();
+  if (tile == 4 || (tile == 0 && V >= 1024LL * 64 * MT))
+    pconv_launch1<NCT, MT, 1>(s, xa, ca, up_a, xb, cb, w, bias, ya, split, yb, relu, pg, ntap);
+  else if (tile == 2 || (tile == 0 && V >= 1024LL * 64 * 2))
+    pconv_launch1<NCT, 2, 1>(s, xa, ca, up_a, xb, cb, w, bias, ya, split, yb, relu, pg, ntap);
+  else if (tile == 1 || (tile == 0 && V >= 1024LL * 64))
+    pconv_launch1<NCT, 1, 1>(s, xa, ca, up_a, xb, cb, w, bias, ya, split, yb, relu, pg, ntap);
+  else
+    pconv_launch1<NCT, 1, 4>(s, xa, ca, up_a, xb, cb, w, bias, ya, split, yb, relu, pg, ntap);
 }
 static int pconv(hipStream_t s, const pbf *xa, int ca, int up_a, const pbf *xb, int cb, const pbf *w, const float *bias,
                  pbf *ya, int split, pbf *yb, int co, int relu, PGrid pg, int ntap) {
+  // the kernel's gathers use 32-bit byte offsets below 2^31
+  const long long V = pgrid_cells(pg);
+  if (V * 2 * std::max(std::max(ca, cb), 1) >= (1LL << 31)) {
+    set_err("pnet: %lld voxels x %d channels exceed the convolution's 2 GiB tensor limit", V, std::max(ca, cb));
+    return GFM_E_ARG;
+  }
+  if (pconv_tile_env() < 0) {
+    set_err("pnet: GFM_PCONV_TILE must be auto, 4, 2, 1 or 1x4");
+    return GFM_E_ARG;
+  }
   switch (co) {
     case 32: pconv_launch<2, 4>(s, xa, ca, up_a, xb, cb, w, bias, ya, split, yb, relu, pg, ntap); break;
     case 64: pconv_launch<4, 4>(s, xa, ca, up_a, xb, cb, w, bias, ya, split, yb, relu, pg, ntap); break;

@@ -4,7 +4,10 @@ which rounds to BF16 at the same points, so the two differ only by
 accumulation order (FP32 on the GPU, FP64 on the CPU) and the rare BF16
 rounding it flips.  Tolerances: forward output within 5e-3 relative L2,
 loss within 1e-3, each layer's weight and bias gradient within 1e-2
-relative L2."""
+relative L2.  Each test runs every voxel tile of the convolution
+(GFM_PCONV_TILE: 4, 2 or 1 tiles of 16 voxels per wavefront, or one tile
+with the steps split over four wavefronts) as well as the choice made by
+grid size, which on these small grids is the split one."""
 import numpy as np
 import pytest
 
@@ -28,40 +31,72 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
-def test_forward_matches_oracle():
-    rng = np.random.default_rng(0)
-    p = po.init_params(CH, seed=1).astype(np.float32)
-    x, _ = _batch(rng, 2, 8, 16, 24)
-    with PNet(CH, max_batch=2, max_cells=8 * 16 * 24) as net:
-        net.set_params(p)
-        assert np.array_equal(net.get_params(), p)
-        y = net.forward(x)
+TILES = ["auto", "4", "2", "1", "1x4"]
+
+
+def _forward_case(shape, seed):
+    rng = np.random.default_rng(seed)
+    p = po.init_params(CH, seed=seed + 1).astype(np.float32)
+    x, _ = _batch(rng, *shape)
     yo, _ = po.forward(p.astype(np.float64), CH, x.astype(np.float64), bf16=True)
+    return p, x, yo
+
+
+def test_forward_matches_oracle(monkeypatch):
+    p, x, yo = _forward_case((2, 8, 16, 24), 0)
+    for tile in TILES:
+        monkeypatch.setenv("GFM_PCONV_TILE", tile)
+        with PNet(CH, max_batch=2, max_cells=8 * 16 * 24) as net:
+            net.set_params(p)
+            assert np.array_equal(net.get_params(), p)
+            y = net.forward(x)
+        err = _rel(y, yo)
+        print(f"tile {tile}: forward rel err {err:.2e}", flush=True)
+        assert err < 5e-3, tile
+
+
+def test_forward_matches_oracle_large_grid():
+    """A 64x64x64 grid, where the choice by grid size takes 4 tiles per
+    wavefront on the finest level and the split tile on the coarser ones."""
+    p, x, yo = _forward_case((1, 64, 64, 64), 7)
+    with PNet(CH, max_batch=1, max_cells=64 ** 3) as net:
+        net.set_params(p)
+        y = net.forward(x)
     err = _rel(y, yo)
-    print("forward rel err", err, flush=True)
+    print("forward 64^3 rel err", err, flush=True)
     assert err < 5e-3
 
 
-def test_gradients_match_oracle():
+def test_unknown_tile_is_rejected(monkeypatch):
+    monkeypatch.setenv("GFM_PCONV_TILE", "3")
+    x = np.zeros((1, 8, 8, 8, 2), dtype=np.float32)
+    with PNet(CH, max_batch=1, max_cells=512) as net:
+        with pytest.raises(RuntimeError, match="GFM_PCONV_TILE"):
+            net.forward(x)
+
+
+def test_gradients_match_oracle(monkeypatch):
     rng = np.random.default_rng(2)
     p = po.init_params(CH, seed=3).astype(np.float32)
     x, t = _batch(rng, 2, 8, 8, 16)
-    with PNet(CH, max_batch=2, max_cells=8 * 8 * 16) as net:
-        net.set_params(p)
-        loss, g = net.loss_grad(x, t)
     lo, go = po.loss_grad(p.astype(np.float64), CH, x.astype(np.float64), t.astype(np.float64), bf16=True)
-    print("loss", loss, lo, flush=True)
-    assert loss == pytest.approx(lo, rel=1e-3)
     layers, off_out, n = po.layout(CH)
-    for l, L in enumerate(layers):
-        ci = L["ca"] + L["cb"]
-        sw = slice(L["ow"], L["ow"] + 27 * L["co"] * ci)
-        sb = slice(L["ob"], L["ob"] + L["co"])
-        ew, eb = _rel(g[sw], go[sw]), _rel(g[sb], go[sb])
-        print(f"layer {l}: weight grad rel err {ew:.2e}, bias {eb:.2e}", flush=True)
-        assert ew < 1e-2 and eb < 1e-2, l
-    so = slice(off_out, n)
-    assert _rel(g[so], go[so]) < 1e-2
+    for tile in TILES:
+        monkeypatch.setenv("GFM_PCONV_TILE", tile)
+        with PNet(CH, max_batch=2, max_cells=8 * 8 * 16) as net:
+            net.set_params(p)
+            loss, g = net.loss_grad(x, t)
+        print(f"tile {tile}: loss", loss, lo, flush=True)
+        assert loss == pytest.approx(lo, rel=1e-3)
+        for l, L in enumerate(layers):
+            ci = L["ca"] + L["cb"]
+            sw = slice(L["ow"], L["ow"] + 27 * L["co"] * ci)
+            sb = slice(L["ob"], L["ob"] + L["co"])
+            ew, eb = _rel(g[sw], go[sw]), _rel(g[sb], go[sb])
+            print(f"  layer {l}: weight grad rel err {ew:.2e}, bias {eb:.2e}", flush=True)
+            assert ew < 1e-2 and eb < 1e-2, (tile, l)
+        so = slice(off_out, n)
+        assert _rel(g[so], go[so]) < 1e-2, tile
 
 
 def test_training_reduces_loss():
