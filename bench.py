@@ -7,7 +7,10 @@ over the 192x96x64 stretched furnace (1,179,648 cells), all state resident
 in HBM.  Prints one JSON line with Mcell-updates/s (cells x iterations /
 s), the roofline figure of the line-relaxation kernel measured with HIP
 events during the timed region, and the CPU baseline (the oracle, one core,
-on a bounded sample of the same workload).
+on a bounded sample of the same workload).  Beside the plain FP64 value it
+reports FP32 configurations with the p' solve relaxed to a tolerance (from
+zero, from the learned guess, with the x-plane block correction) and the
+time each takes to reach res_mass 1e-3, 1e-4, 5e-5 and 1e-5.
 """
 import argparse
 import json
@@ -62,18 +65,24 @@ def configure(case, mode, pcorr_tol):
     """fp64: the reference algorithm (FP64, p' from zero, 5 sweeps).
     mixed: FP32 assembly and sweeps, the reference's p' solve.
     mixed_zero: FP32, p' from zero relaxed to pcorr_tol.
-    learned: FP32, p' from the network's guess relaxed to pcorr_tol."""
+    learned: FP32, p' from the network's guess relaxed to pcorr_tol.
+    A suffix _bc adds the x-plane block correction before each p' sweep
+    (mixed_zero_bc, learned_bc; fp64_zero_bc is FP64 with p' from zero
+    relaxed to pcorr_tol with it)."""
     k = case.ctrl
     k.sweep_order = 1
     k.want_pre_residuals = 0
-    if mode == "fp64":
+    base, bc = (mode[:-3], 1) if mode.endswith("_bc") else (mode, 0)
+    if base == "fp64":
         k.precision, k.pcorr_mode = 0, 0
-    elif mode == "mixed":
+    elif base == "mixed":
         k.precision, k.pcorr_mode = 1, 0
     else:
-        k.precision, k.pcorr_mode, k.pcorr_guess = 1, 1, 1 if mode == "learned" else 0
+        k.precision = 0 if base == "fp64_zero" else 1
+        k.pcorr_mode, k.pcorr_guess = 1, 1 if base == "learned" else 0
         k.pcorr_tol, k.pcorr_max_sweeps = pcorr_tol, 400
         k.pcorr_guess_accept = 0.5  # keep a guess only if it halves the start residual
+        k.pcorr_block_corr = bc
 
 
 def throughput(spec_args, mode, net, steps, warmup, pcorr_tol, sync):
@@ -145,8 +154,8 @@ def main():
     ap.add_argument("--pcorr-tol", type=float, default=0.01,
                     help="p' tolerance of the learned+FP32 configuration (calc_residual)")
     ap.add_argument("--pcorr-tol-tight", type=float, default=1e-3,
-                    help="p' tolerance of the second pair of time-to-residual runs (FP32 from zero, learned+FP32); "
-                         "0 skips them")
+                    help="p' tolerance of the second set of time-to-residual runs (FP32 from zero, learned+FP32, "
+                         "learned+FP32 with the block correction); 0 skips them")
     ap.add_argument("--ttr-cap", type=int, default=800,
                     help="iteration cap of the time-to-residual runs (0 skips them)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -216,6 +225,7 @@ def main():
     others = [("fp32", "mixed", None)]
     if net is not None:
         others += [("fp32_zero_guess_tol", "mixed_zero", None), ("learned_fp32", "learned", net)]
+    others += [("fp32_zero_guess_tol_bc", "mixed_zero_bc", None)]
     for name, mode, nn in others:
         c2, case2, h2, dt2 = throughput(spec_args, mode, nn, args.steps, args.warmup, args.pcorr_tol, sync)
         c2.close()
@@ -223,19 +233,24 @@ def main():
                       "ms_per_step": round(dt2 / max(len(h2), 1) * 1e3, 3)}
         if mode != "mixed":
             line[name].update(pcorr_tol=args.pcorr_tol, pcorr_sweeps_last=case2.ctrl.pcorr_sweeps)
+        if mode.endswith("_bc"):
+            line[name]["block_correction"] = True
     if net is not None:
         line["learned_fp32"]["network"] = os.path.relpath(weights, ROOT)
     if args.ttr_cap > 0:
         ttr = {"cap_iterations": args.ttr_cap, "pcorr_tol": args.pcorr_tol}
         for name, mode, nn in [("plain_fp64", "fp64", None)] + others:
             ttr[name] = time_to_residual(spec_args, mode, nn, args.ttr_cap, args.pcorr_tol)
+            ttr[name]["block_correction"] = mode.endswith("_bc")
         if net is not None and args.pcorr_tol_tight > 0:
             # the p' solve relaxed further: ~100-150 sweeps per solve, where
             # the guess saves more than its inference costs
             ttr["pcorr_tol_tight"] = args.pcorr_tol_tight
-            for name, mode in (("fp32_zero_guess_tol_tight", "mixed_zero"), ("learned_fp32_tol_tight", "learned")):
-                ttr[name] = time_to_residual(spec_args, mode, net if mode == "learned" else None, args.ttr_cap,
-                                             args.pcorr_tol_tight)
+            for name, mode in (("fp32_zero_guess_tol_tight", "mixed_zero"), ("learned_fp32_tol_tight", "learned"),
+                               ("learned_fp32_tol_tight_bc", "learned_bc")):
+                ttr[name] = time_to_residual(spec_args, mode, net if mode.startswith("learned") else None,
+                                             args.ttr_cap, args.pcorr_tol_tight)
+                ttr[name]["block_correction"] = mode.endswith("_bc")
         line["time_to_residual"] = ttr
     if net is not None:
         net.close()

@@ -17,7 +17,7 @@ class GfmCtrl(ct.Structure):
         ("mp", I32), ("np", I32), ("lp", I32), ("nr0", I32),
         ("ncells", I32), ("i_ref", I32), ("j_ref", I32), ("k_ref", I32),
         ("sweep_order", I32), ("precision", I32), ("want_pre_residuals", I32), ("want_irc", I32),
-        ("i1st_rad_done", I32), ("isothermal_walls", I32), ("gray_radiation", I32), ("pad0", I32),
+        ("i1st_rad_done", I32), ("isothermal_walls", I32), ("gray_radiation", I32), ("pcorr_block_corr", I32),
         ("itr_gas", I32), ("itr_gend", I32), ("maxgi", I32), ("pad1", I32),
         ("bgcon", D), ("factor", D), ("fadj", D),
         ("t0", D), ("trdg", D), ("h_0", D), ("r0", D), ("area0", D), ("vol0", D),

@@ -59,6 +59,7 @@ struct gfm_ctx {
   std::vector<void *> allocs;
   double *xt2 = nullptr, *xt3 = nullptr, *xt4 = nullptr, *xt5 = nullptr, *part = nullptr, *stage = nullptr;
   double *xg = nullptr;  // G of the line kernel between the two passes of a colour visit
+  double *bc_part = nullptr, *bc_d = nullptr;  // x-plane block correction of the p' box: chunk sums, plane values
   float *xtf = nullptr;  // FP32 working copy of the unknown of ADLBL3 (mixed precision)
   int16_t *stage16 = nullptr;
   // lattice positions of the GFLOW wall treatment: open with a wall
@@ -568,8 +569,25 @@ static int pcorr_guess(gfm_ctx *c, const Box &cen) {
   return GFM_OK;
 }
 
+// x-plane block correction of xt on the box (p' solve relaxed to a
+// tolerance): partial sums, chunk sums and plane solve, update.
+static int block_corr_x(gfm_ctx *c, double *xt, int ib, int ie, int jb, int je, int kb, int ke) {
+  const Box b = make_box(ib, ie, jb, je, kb, ke, 2);
+  if ((size_t)BC_NV * b.ni * sizeof(double) > 65536) {
+    set_err("block correction: %d x-planes exceed the plane solve's 64 KB of LDS", b.ni);
+    return GFM_E_ARG;
+  }
+  const int nchunk = (b.nj * b.nk + BC_ROWS - 1) / BC_ROWS;
+  hipLaunchKernelGGL(k_bc_part, dim3((b.ni + 63) / 64, nchunk), dim3(256), 0, c->stream, b, xt, c->bc_part);
+  hipLaunchKernelGGL(k_bc_solve, dim3(1), dim3(FIN_NT), BC_NV * b.ni * sizeof(double), c->stream, b.ni, nchunk, c->bc_part,
+                     c->bc_d);
+  LAUNCH(k_bc_apply, b, xt, c->bc_d);
+  return GFM_OK;
+}
+
 // Relax p' one sweep at a time until calc_residual <= pcorr_tol (host
-// loop: one residual read-back per sweep).
+// loop: one residual read-back per sweep), each sweep preceded by the
+// x-plane block correction when pcorr_block_corr is set.
 static int pcorr_relax(gfm_ctx *c) {
   DevCase &d = c->dc;
   gfm_ctrl &k = c->ctrl;
@@ -587,6 +605,7 @@ static int pcorr_relax(gfm_ctx *c) {
   k.pcorr_resid_start = r;
   int s = 0;
   while (s < k.pcorr_max_sweeps && !(r <= k.pcorr_tol)) {
+    if (k.pcorr_block_corr && (rc = block_corr_x(c, d.fz, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2))) return rc;
     adlbl3_sweeps(c, d.fz, 1, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2);
     s++;
     if ((rc = resid(&r))) return rc;
@@ -879,6 +898,9 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
     const long long cells = (long long)((k->mp - 6) / 2 + 1) * ((k->np - 6) / 2 + 1) * ((k->lp - 6) / 2 + 1);
     if ((rc = dalloc(c, &c->dump_x, 2 * cells))) return fail(rc);
     if ((rc = dalloc(c, &c->dump_p, cells))) return fail(rc);
+    const long long ni = (k->mp - 6) / 2 + 1, nrow = (long long)((k->np - 6) / 2 + 1) * ((k->lp - 6) / 2 + 1);
+    if ((rc = dalloc(c, &c->bc_part, ((nrow + BC_ROWS - 1) / BC_ROWS) * BC_NV * ni))) return fail(rc);
+    if ((rc = dalloc(c, &c->bc_d, ni))) return fail(rc);
   }
   if ((rc = dalloc(c, &c->xt2, N))) return fail(rc);
   if ((rc = dalloc(c, &c->xt3, N))) return fail(rc);
@@ -1056,6 +1078,7 @@ int gfm_hip_gflow(gfm_ctx *c, gfm_ctrl *k) {
   c->ctrl.pcorr_guess = k->pcorr_guess;
   c->ctrl.pcorr_max_sweeps = k->pcorr_max_sweeps;
   c->ctrl.pcorr_tol = k->pcorr_tol;
+  c->ctrl.pcorr_block_corr = k->pcorr_block_corr;
   c->ctrl.pcorr_dump = k->pcorr_dump;
   if (c->ctrl.pcorr_guess_accept != k->pcorr_guess_accept) drop_graph(c);  // captured as a kernel argument
   c->ctrl.pcorr_guess_accept = k->pcorr_guess_accept;

@@ -1358,6 +1358,106 @@ __global__ __launch_bounds__(FIN_NT) void k_residual_finalize(const double *part
 }
 
 // ======================================================================
+// Additive correction over x-planes (block correction; Settari and Aziz
+// 1973, Patankar 1981 sec. 5.4) before each sweep of a p' solve relaxed to
+// a tolerance.  No counterpart in the reference.  Plane q of the box
+// (i = i0 + 2q) gets one value d_q, added to its free cells (in the box,
+// IBCELL <= 0, |AP| > 1e-20), with
+//   A_q d_q - W_q d_{q-1} - E_q d_{q+1} = R_q,
+//   A_q = sum (a_p - a_nb of the free in-plane neighbours),
+//   W_q, E_q = sum of a_x- (a_x+) toward a free neighbour,
+//   R_q = sum of the residuals a_nb x_nb + b - a_p x,
+// sums over the plane's free cells, so that afterwards the residuals of
+// each plane's free cells sum to zero, for every plane but the last, the
+// reference plane (d = 0, its equation not solved): the p' equation has no
+// coefficient toward a pinned cell and is singular, fixed up to a constant
+// (with coefficients rounded to FP32 only nearly so, and solving the full
+// system would add a large constant).  Three launches, all in a fixed
+// order: per-plane partial sums over chunks of BC_ROWS (j,k) rows, the
+// chunk sums and the Thomas solve in one workgroup, the update.
+// ======================================================================
+#define BC_ROWS 64
+#define BC_NV 5  // A, W, E, R, free-cell count
+__device__ __forceinline__ bool bc_free(const Box &b, int i, int j, int k) {
+  const int ti = (i - b.i0) >> 1, tj = (j - b.j0) >> 1, tk = (k - b.k0) >> 1;
+  if (i < b.i0 || ti >= b.ni || j < b.j0 || tj >= b.nj || k < b.k0 || tk >= b.nk) return false;
+  const long long q = lidx(i, j, k);
+  return g.ib[q] <= 0 && fabs(CAP(q)) > SMALL20;
+}
+// grid (ceil(ni/64), ceil(nj*nk/BC_ROWS)), 256 threads: lane = plane,
+// wavefront w takes rows w, w+4, ... of the chunk; part[(c*BC_NV + v)*ni + q]
+__global__ __launch_bounds__(256) void k_bc_part(Box b, const double *xt, double *part) {
+  __shared__ double red[3][BC_NV][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int q = blockIdx.x * 64 + lane;
+  const int nrow = b.nj * b.nk, r0 = blockIdx.y * BC_ROWS;
+  double acc[BC_NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (q < b.ni) {
+    const int i = b.i0 + 2 * q;
+    for (int r = r0 + w; r < r0 + BC_ROWS && r < nrow; r += 4) {
+      const int j = b.j0 + 2 * (r % b.nj), k = b.k0 + 2 * (r / b.nj);
+      if (!bc_free(b, i, j, k)) continue;
+      const long long c = lidx(i, j, k);
+      const double a0 = CAS(0, c), a3 = CAS(3, c), a1 = CAS(1, c), a4 = CAS(4, c), a2 = CAS(2, c), a5 = CAS(5, c);
+      const double t1 = a0 * xt[lidx(i - 2, j, k)], t2 = a3 * xt[lidx(i + 2, j, k)];
+      const double t3 = a1 * xt[lidx(i, j - 2, k)], t4 = a4 * xt[lidx(i, j + 2, k)];
+      const double t5 = a2 * xt[lidx(i, j, k - 2)], t6 = a5 * xt[lidx(i, j, k + 2)];
+      const double apc = CAP(c), t7 = apc * xt[c], t8 = CBS(c);
+      acc[3] += t1 + t2 + t3 + t4 + t5 + t6 + t8 - t7;
+      double a = apc;
+      if (bc_free(b, i, j - 2, k)) a -= a1;
+      if (bc_free(b, i, j + 2, k)) a -= a4;
+      if (bc_free(b, i, j, k - 2)) a -= a2;
+      if (bc_free(b, i, j, k + 2)) a -= a5;
+      acc[0] += a;
+      if (bc_free(b, i - 2, j, k)) acc[1] += a0;
+      if (bc_free(b, i + 2, j, k)) acc[2] += a3;
+      acc[4] += 1.0;
+    }
+  }
+  if (w > 0)
+#pragma unroll
+    for (int v = 0; v < BC_NV; v++) red[w - 1][v][lane] = acc[v];
+  __syncthreads();
+  if (w == 0 && q < b.ni)
+#pragma unroll
+    for (int v = 0; v < BC_NV; v++)
+      part[((long long)blockIdx.y * BC_NV + v) * b.ni + q] = ((acc[v] + red[0][v][lane]) + red[1][v][lane]) + red[2][v][lane];
+}
+// one workgroup of FIN_NT threads, dynamic LDS BC_NV*ni doubles: the chunk
+// sums in chunk order, then the Thomas solve by thread 0 into d[ni].  A
+// plane without free cells, the last plane, and a plane whose pivot falls
+// below 1e-12 of its diagonal (closing a block of planes decoupled from the
+// rest) get 0.
+__global__ __launch_bounds__(FIN_NT) void k_bc_solve(int ni, int nchunk, const double *part, double *d) {
+  extern __shared__ double bcs[];  // sums[v][q] for v < BC_NV, then cp[q]
+  for (int it = threadIdx.x; it < BC_NV * ni; it += FIN_NT) {
+    const int v = it / ni, q = it - v * ni;
+    double s = 0.0;
+    for (int c = 0; c < nchunk; c++) s += part[((long long)c * BC_NV + v) * ni + q];
+    bcs[it] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double *A = bcs, *W = bcs + ni, *E = bcs + 2 * ni, *R = bcs + 3 * ni, *nf = bcs + 4 * ni;
+  double *cp = nf;  // the counts are read before cp overwrites them
+  for (int q = 0; q < ni; q++) {
+    if (nf[q] == 0.0) { A[q] = 1.0; W[q] = E[q] = R[q] = 0.0; }
+    const double m = A[q] + (q > 0 ? W[q] * cp[q - 1] : 0.0);
+    if (q == ni - 1 || !(fabs(m) > 1.0e-12 * fabs(A[q]))) { cp[q] = 0.0; R[q] = 0.0; continue; }
+    cp[q] = -E[q] / m;
+    R[q] = (R[q] + (q > 0 ? W[q] * R[q - 1] : 0.0)) / m;
+  }
+  d[ni - 1] = R[ni - 1];
+  for (int q = ni - 2; q >= 0; q--) d[q] = R[q] - cp[q] * d[q + 1];
+}
+__global__ void k_bc_apply(Box b, double *xt, const double *d) {
+  int i, j, k;
+  if (!box_pt(b, i, j, k) || !bc_free(b, i, j, k)) return;
+  xt[lidx(i, j, k)] += d[(i - b.i0) >> 1];
+}
+
+// ======================================================================
 // Learned initial guess of the p' solve.  No counterpart in the reference,
 // whose guess is FZ = 0 (C/Gslv3v.f90:157-161).  Cell v of the p' box
 // [4, MP-2]x[4, NP-2]x[4, LP-2] (step 2) is v = (kk*ny + jj)*nx + ii with

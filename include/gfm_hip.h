@@ -73,7 +73,16 @@ enum { GFM_PREC_FP64 = 0, GFM_PREC_MIXED = 1 };
  *     instead unless calc_residual(alpha*g) < a * calc_residual(0), with
  *     a = pcorr_guess_accept (1 when <= 0: the guess must lower the
  *     residual; smaller values demand a margin, which guards a network
- *     applied outside the states it was trained on). */
+ *     applied outside the states it was trained on).
+ * With pcorr_block_corr = 1 each sweep of GFM_PCORR_TOLERANCE is preceded by
+ * an additive correction over x-planes (block correction: Settari and Aziz
+ * 1973; Patankar 1981, sec. 5.4): one value per x-plane of the box, added
+ * to the plane's unpinned cells, chosen so that the residuals of each
+ * plane's unpinned cells sum to zero (a tridiagonal system over the planes,
+ * FP64), for every plane but the last: the last plane is the reference
+ * (value 0), because the p' equation, with no coefficient toward a pinned
+ * cell, is singular.  Pinned cells (IBCELL >= 1 or |AP| <= 1e-20) and cells
+ * outside the box keep their values. */
 enum { GFM_PCORR_REFERENCE = 0, GFM_PCORR_TOLERANCE = 1 };
 enum { GFM_PGUESS_ZERO = 0, GFM_PGUESS_LEARNED = 1 };
 
@@ -108,7 +117,8 @@ typedef struct gfm_ctrl {
   int32_t i1st_rad_done;          /* 0: adiabatic walls, C/Extr.f90:194 */
   int32_t isothermal_walls;       /* 1: wall heat sink of C/Source.f90:473-509 with fixed wall T */
   int32_t gray_radiation;         /* 1: gray emission/absorption source term */
-  int32_t pad0;
+  int32_t pcorr_block_corr;       /* 1: x-plane additive correction before each sweep of a p' solve
+                                     relaxed to a tolerance (GFM_PCORR_TOLERANCE; not in the reference) */
   /* iteration state */
   int32_t itr_gas, itr_gend, maxgi, pad1;
   double bgcon;

@@ -792,6 +792,75 @@ static double CALC_RESIDUAL(const double *xt, int ib, int ie, int jb, int je, in
   return dmax(resid, 1.0e-16);
 }
 
+/* Additive correction over x-planes (block correction; Settari and Aziz
+ * 1973, Patankar 1981 sec. 5.4; not in the reference, used by the p' solve
+ * relaxed to a tolerance when pcorr_block_corr is set).  Plane q of the box
+ * (i = ib + 2q) gets one value d_q, added to its unpinned cells, such that
+ * the residuals r = sum a_nb x_nb + b - a_p x of each plane's unpinned
+ * cells sum to zero afterwards, for every plane but the last:
+ *   A_q d_q - W_q d_{q-1} - E_q d_{q+1} = R_q,
+ *   A_q = sum (a_p - a_nb of the in-plane neighbours that are corrected),
+ *   W_q, E_q = sum of a_x- (a_x+) whose neighbour is corrected, R_q = sum r,
+ * sums over the plane's unpinned cells.  A cell is corrected when it lies
+ * in the box and is not pinned (IBCELL >= 1 or |AP| <= 1e-20, the rows
+ * ADLBL3 keeps).  Planes without unpinned cells get d = 0.  The last plane
+ * is the reference: its d is 0 and its equation is not solved, because
+ * the p' equation has no coefficient toward a pinned cell and is singular
+ * (its solution is fixed up to a constant; with coefficients rounded to
+ * FP32 it is only nearly so, and solving the full system would add a
+ * large constant).  The other planes' system is solved by the Thomas
+ * algorithm; a pivot below 1e-12 of its diagonal (a plane closing a block
+ * of planes decoupled from the rest) gets d = 0 in the same way.  Sums run
+ * over k, then j, then i ascending. */
+static int BC_FREE(int i, int j, int k, int ib, int ie, int jb, int je, int kb, int ke) {
+  if (i < ib || i > ie || j < jb || j > je || k < kb || k > ke) return 0;
+  return IBCELL(i, j, k) <= 0 && fabs(AP(i, j, k)) > small20;
+}
+static void BLOCK_CORR_X(double *xt, int ib, int ie, int jb, int je, int kb, int ke) {
+  const int np = (ie - ib) / 2 + 1;
+  double *A = calloc((size_t)np, sizeof(double)), *W = calloc((size_t)np, sizeof(double));
+  double *E = calloc((size_t)np, sizeof(double)), *R = calloc((size_t)np, sizeof(double));
+  double *cp = calloc((size_t)np, sizeof(double)), *dp = calloc((size_t)np, sizeof(double));
+  int *nfree = calloc((size_t)np, sizeof(int));
+  for (int k = kb; k <= ke; k += 2)
+    for (int j = jb; j <= je; j += 2)
+      for (int q = 0; q < np; q++) {
+        const int i = ib + 2 * q;
+        if (!BC_FREE(i, j, k, ib, ie, jb, je, kb, ke)) continue;
+        double t1 = AS(i, j, k, 1, 1) * XTA(i - 2, j, k), t2 = AS(i, j, k, 1, 2) * XTA(i + 2, j, k);
+        double t3 = AS(i, j, k, 2, 1) * XTA(i, j - 2, k), t4 = AS(i, j, k, 2, 2) * XTA(i, j + 2, k);
+        double t5 = AS(i, j, k, 3, 1) * XTA(i, j, k - 2), t6 = AS(i, j, k, 3, 2) * XTA(i, j, k + 2);
+        double t7 = AP(i, j, k) * XTA(i, j, k), t8 = BS(i, j, k);
+        R[q] += t1 + t2 + t3 + t4 + t5 + t6 + t8 - t7;
+        double a = AP(i, j, k);
+        if (BC_FREE(i, j - 2, k, ib, ie, jb, je, kb, ke)) a -= AS(i, j, k, 2, 1);
+        if (BC_FREE(i, j + 2, k, ib, ie, jb, je, kb, ke)) a -= AS(i, j, k, 2, 2);
+        if (BC_FREE(i, j, k - 2, ib, ie, jb, je, kb, ke)) a -= AS(i, j, k, 3, 1);
+        if (BC_FREE(i, j, k + 2, ib, ie, jb, je, kb, ke)) a -= AS(i, j, k, 3, 2);
+        A[q] += a;
+        if (BC_FREE(i - 2, j, k, ib, ie, jb, je, kb, ke)) W[q] += AS(i, j, k, 1, 1);
+        if (BC_FREE(i + 2, j, k, ib, ie, jb, je, kb, ke)) E[q] += AS(i, j, k, 1, 2);
+        nfree[q]++;
+      }
+  for (int q = 0; q < np; q++) {
+    if (nfree[q] == 0) { A[q] = 1.0; W[q] = E[q] = R[q] = 0.0; }
+    const double m = A[q] + (q > 0 ? W[q] * cp[q - 1] : 0.0);
+    if (q == np - 1 || !(fabs(m) > 1.0e-12 * fabs(A[q]))) { cp[q] = 0.0; dp[q] = 0.0; continue; }
+    cp[q] = -E[q] / m;
+    dp[q] = (R[q] + (q > 0 ? W[q] * dp[q - 1] : 0.0)) / m;
+  }
+  {
+    for (int q = np - 2; q >= 0; q--) dp[q] -= cp[q] * dp[q + 1];
+    for (int k = kb; k <= ke; k += 2)
+      for (int j = jb; j <= je; j += 2)
+        for (int q = 0; q < np; q++) {
+          const int i = ib + 2 * q;
+          if (BC_FREE(i, j, k, ib, ie, jb, je, kb, ke)) XTA(i, j, k) += dp[q];
+        }
+  }
+  free(A); free(W); free(E); free(R); free(cp); free(dp); free(nfree);
+}
+
 /* ======================================================================
  * GSLVX1: momentum coefficients and SIMPLER pseudo-velocity,
  * C/Gslv3x.f90:9-113
@@ -1268,13 +1337,16 @@ static void GSLV3V(void) {
       for (int k = 2; k <= LP; k += 2) FZ(i, j, k) = 0.0;
   if (C->pcorr_mode == GFM_PCORR_TOLERANCE) {
     /* zero initial guess relaxed one sweep at a time to pcorr_tol (the
-       GPU's GFM_PCORR_TOLERANCE with GFM_PGUESS_ZERO; not in the reference) */
+       GPU's GFM_PCORR_TOLERANCE with GFM_PGUESS_ZERO; not in the reference),
+       each sweep preceded by the x-plane block correction when
+       pcorr_block_corr is set */
     const int save = C->ntimesg[NELG - 1];
     C->ntimesg[NELG - 1] = 1;
     double r = CALC_RESIDUAL(FZ_, mi1, mi3, nj1, nj3, lk1, lk3);
     C->pcorr_resid_start = r;
     int ns = 0;
     while (ns < C->pcorr_max_sweeps && !(r <= C->pcorr_tol)) {
+      if (C->pcorr_block_corr) BLOCK_CORR_X(FZ_, mi1, mi3, nj1, nj3, lk1, lk3);
       ADLBL3(FZ_, mi1, mi3, nj1, nj3, lk1, lk3);
       ns++;
       r = CALC_RESIDUAL(FZ_, mi1, mi3, nj1, nj3, lk1, lk3);
@@ -1566,6 +1638,17 @@ int oracle_adlbl3(gfm_ctrl *c, const gfm_host_arrays *h, int nelg, const double 
   memcpy(BS_, bs, sizeof(double) * NN);
   NELG = nelg;
   ADLBL3(xt, ib, ie, jb, je, kb, ke);
+  return GFM_OK;
+}
+/* BLOCK_CORR_X on caller arrays (x-plane additive correction of xt). */
+int oracle_block_corr_x(gfm_ctrl *c, const gfm_host_arrays *h, const double *ap, const double *as, const double *bs,
+                        double *xt, int ib, int ie, int jb, int je, int kb, int ke) {
+  int rc = bind(c, h);
+  if (rc) return rc;
+  memcpy(AP_, ap, sizeof(double) * NN);
+  memcpy(AS_, as, sizeof(double) * NN * 6);
+  memcpy(BS_, bs, sizeof(double) * NN);
+  BLOCK_CORR_X(xt, ib, ie, jb, je, kb, ke);
   return GFM_OK;
 }
 double oracle_calc_residual(gfm_ctrl *c, const gfm_host_arrays *h, const double *ap, const double *as,

@@ -28,6 +28,7 @@ def lib():
         _lib.oracle_sphase.restype = ct.c_int
         _lib.oracle_initsv_tail.restype = ct.c_int
         _lib.oracle_adlbl3.restype = ct.c_int
+        _lib.oracle_block_corr_x.restype = ct.c_int
         _lib.oracle_calc_residual.restype = ct.c_double
         _lib.oracle_ctrl_size.restype = ct.c_size_t
         _lib.oracle_da.restype = ct.c_double
@@ -72,6 +73,16 @@ def adlbl3(case, nelg, ap, as_, bs, xt, box):
     rc = lib().oracle_adlbl3(ct.byref(case.ctrl), ct.byref(h), int(nelg), _dp(ap), _dp(as_), _dp(bs), _dp(xt), *map(int, box))
     if rc:
         raise RuntimeError(f"oracle_adlbl3 failed: {rc}")
+
+
+def block_corr_x(case, ap, as_, bs, xt, box):
+    """The x-plane additive correction of xt in place (the p' solve's
+    block correction; no reference counterpart)."""
+    from gfm_amd.abi import host_arrays
+    h = host_arrays(case.arrays)
+    rc = lib().oracle_block_corr_x(ct.byref(case.ctrl), ct.byref(h), _dp(ap), _dp(as_), _dp(bs), _dp(xt), *map(int, box))
+    if rc:
+        raise RuntimeError(f"oracle_block_corr_x failed: {rc}")
 
 
 def calc_residual(case, ap, as_, bs, xt, box):

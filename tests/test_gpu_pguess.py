@@ -2,7 +2,12 @@
 initial guess.
 
 * Zero guess: GPU and oracle take the same number of sweeps and agree on
-  every field to 1e-9 after two iterations (same plane order).
+  every field to 1e-9 after two iterations (same plane order), with and
+  without the x-plane block correction before each sweep.
+* Block correction: on a developed 48x24x16 furnace it cuts the sweeps to
+  tolerance 1e-3 at least threefold, in FP64 and in mixed precision, and
+  res_mass after 8 iterations is no worse than 1.5 times that of the plain
+  solves.
 * Learned guess (an untrained network: the guard must hold for any
   prediction): the solve still ends at or below the tolerance, the start
   residual is never worse than that of the zero guess, and the iteration
@@ -21,18 +26,20 @@ from helpers import field_errors
 pytestmark = pytest.mark.gpu
 
 
-def _case(tol=1e-3, max_sweeps=200, guess=0, nx=16, ny=16, nz=8):
+def _case(tol=1e-3, max_sweeps=200, guess=0, nx=16, ny=16, nz=8, bc=0):
     c = cs.furnace_small(nx=nx, ny=ny, nz=nz, seed=3, maxgi=10 ** 7)
     c.ctrl.sweep_order = 1
     c.ctrl.pcorr_mode = 1
     c.ctrl.pcorr_guess = guess
     c.ctrl.pcorr_tol = tol
     c.ctrl.pcorr_max_sweeps = max_sweeps
+    c.ctrl.pcorr_block_corr = bc
     return c
 
 
-def test_tolerance_zero_guess_matches_oracle():
-    base = _case()
+@pytest.mark.parametrize("bc", [0, 1])
+def test_tolerance_zero_guess_matches_oracle(bc):
+    base = _case(bc=bc)
     o, g = base.clone(), base.clone()
     oracle.initsv_tail(o)
     with CombustionSpace(g) as s:
@@ -45,10 +52,35 @@ def test_tolerance_zero_guess_matches_oracle():
             assert g.ctrl.pcorr_resid == pytest.approx(o.ctrl.pcorr_resid, rel=1e-8)
         s.download()
     print("sweeps (oracle, gpu)", sweeps, flush=True)
-    assert all(a == b and a > 5 for a, b in sweeps)
+    assert all(a == b and a > (0 if bc else 5) for a, b in sweeps)
     errs = field_errors(o, g)
     for k, v in errs.items():
         assert v < 1e-9, (k, v)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_block_correction_cuts_sweeps(precision):
+    base = cs.furnace_large(nx=48, ny=24, nz=16, seed=2)
+    base.ctrl.sweep_order = 1
+    base.ctrl.precision = precision
+    with CombustionSpace(base) as s:
+        s.sphase(20)  # develop the flow with the reference p' solve
+        s.download()
+    res, rm = {}, {}
+    for bc in (0, 1):
+        c = base.clone()
+        c.ctrl.pcorr_mode, c.ctrl.pcorr_tol, c.ctrl.pcorr_max_sweeps, c.ctrl.pcorr_block_corr = 1, 1e-3, 400, bc
+        sw, h = [], []
+        with CombustionSpace(c) as s:
+            for it in range(8):
+                h += list(s.sphase(1))
+                assert c.ctrl.pcorr_resid <= 1e-3
+                sw.append(c.ctrl.pcorr_sweeps)
+        res[bc], rm[bc] = sw, h
+    print("sweeps without / with block correction", res, "res_mass", rm, flush=True)
+    assert sum(res[1]) * 3 <= sum(res[0])
+    # the corrected solves serve the outer iteration as well as the plain ones
+    assert rm[1][-1] <= 1.5 * rm[0][-1]
 
 
 def test_learned_guess_guard_and_tolerance():

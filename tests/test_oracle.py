@@ -17,6 +17,13 @@ term, a wrong sign, a swapped index -- fails it):
                                   blocked cells untouched); calc_residual
                                   against an independent numpy evaluation.
   SOURCE(5) res_mass              closed form for a single perturbed face.
+  x-plane block correction        its defining property: afterwards the
+  (no reference counterpart)      residuals of each plane's unpinned cells
+                                  sum to zero (numpy evaluation) on every
+                                  plane but the last (the reference plane,
+                                  left unchanged), every such cell moved by
+                                  its plane's one value, and pinned and
+                                  out-of-box cells unchanged.
   case generator                  closed-form lattice coordinates, brute-force
                                   cell-code propagation on a tiny grid,
                                   inflow = GMFR, nondimensional groups.
@@ -151,6 +158,94 @@ def test_adlbl3_manufactured_solution(order):
     assert np.max(np.abs(xt[sl][opn] - xs[sl][opn])) < 1e-10
     assert np.all(xt[sl][~opn] == 0.0)  # blocked rows keep their value
     assert r1 < 1e-10 * r0
+
+
+def _np_residual(c, ap, as_, bs, xt, box):
+    """Residual sum a_nb x_nb + b - a_p x on the box's cells (numpy)."""
+    ib, ie, jb, je, kb, ke = box
+    sl = (slice(ib - 1, ie, 2), slice(jb - 1, je, 2), slice(kb - 1, ke, 2))
+
+    def sh(di, dj, dk):
+        return xt[ib - 1 + di:ie + di:2, jb - 1 + dj:je + dj:2, kb - 1 + dk:ke + dk:2]
+    r = (as_[..., 0, 0][sl] * sh(-2, 0, 0) + as_[..., 0, 1][sl] * sh(2, 0, 0) + as_[..., 1, 0][sl] * sh(0, -2, 0)
+         + as_[..., 1, 1][sl] * sh(0, 2, 0) + as_[..., 2, 0][sl] * sh(0, 0, -2) + as_[..., 2, 1][sl] * sh(0, 0, 2)
+         + bs[sl] - ap[sl] * xt[sl])
+    return r, sl
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_block_corr_x_zeroes_plane_residual_sums(seed):
+    c = cs.furnace_small(nx=12, ny=8, nz=6, seed=seed)
+    oracle.initsv_tail(c)
+    box = (4, c.ctrl.mp - 2, 4, c.ctrl.np - 2, 4, c.ctrl.lp - 2)
+    ap, as_, bs, xt, xs, sl = manufactured(c, box, seed=20 + seed)
+    rng = np.random.default_rng(seed)
+    xt[sl] = rng.uniform(-1, 1, xt[sl].shape)
+    # pin a few open interior rows through |AP| <= 1e-20 as well
+    ibc = c.arrays["ibcell"]
+    opn = np.argwhere(ibc[sl] <= 0)
+    for q in opn[rng.choice(len(opn), 5, replace=False)]:
+        ap[tuple(2 * q + np.array([box[0] - 1, box[2] - 1, box[4] - 1]))] = 0.0
+    x0 = xt.copy(order="F")
+    r0, _ = _np_residual(c, ap, as_, bs, x0, box)
+    free = (ibc[sl] <= 0) & (np.abs(ap[sl]) > 1e-20)
+    oracle.block_corr_x(c, ap, as_, bs, xt, box)
+    r1, _ = _np_residual(c, ap, as_, bs, xt, box)
+    d = xt - x0
+    outside = np.ones(xt.shape, bool)
+    outside[sl] = False
+    assert np.all(d[outside] == 0.0)
+    assert np.all(d[sl][~free] == 0.0)
+    scale = np.abs(r0).max()
+    planes = 0
+    last = free.shape[0] - 1
+    assert np.all(d[sl][last] == 0.0)  # the reference plane
+    for q in range(last):
+        f = free[q]
+        if not f.any():
+            continue
+        planes += 1
+        dq = d[sl][q][f]
+        assert np.ptp(dq) <= 1e-14 * max(1.0, np.abs(dq).max()), q  # one value per plane
+        assert abs(r1[q][f].sum()) <= 1e-11 * scale * f.sum(), q
+        assert abs(r0[q][f].sum()) > 1e-3 * scale  # there was something to correct
+        assert np.abs(dq).max() > 0.0
+    assert planes >= 8
+
+
+def test_block_corr_x_nearly_singular_system_stays_bounded():
+    """A p'-like system: no coefficient toward pinned or out-of-box cells
+    and a_p = sum a_nb up to FP32 rounding (relative 1e-7), so the plane
+    system is singular but for rounding.  The correction stays of the size
+    of the solution (solving the full system would divide the total residual
+    by the rounding-level last pivot) and still zeroes the plane sums."""
+    c = cs.furnace_small(nx=12, ny=8, nz=6, seed=4)
+    oracle.initsv_tail(c)
+    box = (4, c.ctrl.mp - 2, 4, c.ctrl.np - 2, 4, c.ctrl.lp - 2)
+    ap, as_, bs, xt, xs, sl = manufactured(c, box, seed=30)
+    ibc = c.arrays["ibcell"]
+    ib, ie, jb, je, kb, ke = box
+    free = np.zeros(ibc.shape, bool)
+    free[sl] = ibc[sl] <= 0
+    fp = np.pad(free, 2)
+    shp = ibc.shape
+    for d, (di, dj, dk) in enumerate([(-2, 0, 0), (0, -2, 0), (0, 0, -2), (2, 0, 0), (0, 2, 0), (0, 0, 2)]):
+        nbf = fp[2 + di:2 + di + shp[0], 2 + dj:2 + dj + shp[1], 2 + dk:2 + dk + shp[2]]
+        as_[..., d % 3, d // 3][~nbf] = 0.0
+    rng = np.random.default_rng(5)
+    ap[sl] = as_[sl].sum(axis=(-1, -2)) * (1.0 + 1e-7 * rng.uniform(-1, 1, ap[sl].shape))
+    xt[sl] = rng.uniform(-1, 1, xt[sl].shape)
+    x0 = xt.copy(order="F")
+    oracle.block_corr_x(c, ap, as_, bs, xt, box)
+    d = (xt - x0)[sl]
+    r1, _ = _np_residual(c, ap, as_, bs, xt, box)
+    fr = free[sl]
+    print("max |d|", np.abs(d).max(), flush=True)
+    assert 0.0 < np.abs(d).max() < 10.0 * (np.abs(xs).max() + 1.0)
+    scale = np.abs(_np_residual(c, ap, as_, bs, x0, box)[0]).max()
+    for q in range(fr.shape[0] - 1):
+        if fr[q].any():
+            assert abs(r1[q][fr[q]].sum()) <= 1e-9 * scale * fr[q].sum(), q
 
 
 def test_calc_residual_definition():
