@@ -59,7 +59,8 @@ struct gfm_ctx {
   std::vector<void *> allocs;
   double *xt2 = nullptr, *xt3 = nullptr, *xt4 = nullptr, *xt5 = nullptr, *part = nullptr, *stage = nullptr;
   double *xg = nullptr;  // G of the line kernel between the two passes of a colour visit
-  double *bc_part = nullptr, *bc_d = nullptr;  // x-plane block correction of the p' box: chunk sums, plane values
+  double *bc_part = nullptr, *bc_sums = nullptr, *bc_d = nullptr;  // x-plane block correction of the p' box:
+                                                                   // chunk sums, plane sums, plane values
   float *xtf = nullptr;  // FP32 working copy of the unknown of ADLBL3 (mixed precision)
   int16_t *stage16 = nullptr;
   // lattice positions of the GFLOW wall treatment: open with a wall
@@ -570,16 +571,18 @@ static int pcorr_guess(gfm_ctx *c, const Box &cen) {
 }
 
 // x-plane block correction of xt on the box (p' solve relaxed to a
-// tolerance): partial sums, chunk sums and plane solve, update.
+// tolerance): partial sums, chunk sums, plane solve, update.
 static int block_corr_x(gfm_ctx *c, double *xt, int ib, int ie, int jb, int je, int kb, int ke) {
   const Box b = make_box(ib, ie, jb, je, kb, ke, 2);
-  if ((size_t)BC_NV * b.ni * sizeof(double) > 65536) {
-    set_err("block correction: %d x-planes exceed the plane solve's 64 KB of LDS", b.ni);
+  const int nchunk = (b.nj * b.nk + BC_ROWS - 1) / BC_ROWS;
+  if (nchunk > 65535 || (size_t)BC_NV * b.ni * sizeof(double) > 65536) {
+    set_err("block correction: box of %d x-planes and %d x %d rows exceeds the kernels' grid or LDS", b.ni, b.nj, b.nk);
     return GFM_E_ARG;
   }
-  const int nchunk = (b.nj * b.nk + BC_ROWS - 1) / BC_ROWS;
-  hipLaunchKernelGGL(k_bc_part, dim3((b.ni + 63) / 64, nchunk), dim3(256), 0, c->stream, b, xt, c->bc_part);
-  hipLaunchKernelGGL(k_bc_solve, dim3(1), dim3(FIN_NT), BC_NV * b.ni * sizeof(double), c->stream, b.ni, nchunk, c->bc_part,
+  hipLaunchKernelGGL(k_bc_part, dim3((b.ni + 63) / 64, nchunk), dim3(64 * BC_ROWS), 0, c->stream, b, xt, c->bc_part);
+  hipLaunchKernelGGL(k_bc_sum, dim3((b.ni + 63) / 64, BC_NV), dim3(1024), 0, c->stream, b.ni, nchunk, c->bc_part,
+                     c->bc_sums);
+  hipLaunchKernelGGL(k_bc_solve, dim3(1), dim3(256), BC_NV * b.ni * sizeof(double), c->stream, b.ni, c->bc_sums,
                      c->bc_d);
   LAUNCH(k_bc_apply, b, xt, c->bc_d);
   return GFM_OK;
@@ -900,6 +903,7 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
     if ((rc = dalloc(c, &c->dump_p, cells))) return fail(rc);
     const long long ni = (k->mp - 6) / 2 + 1, nrow = (long long)((k->np - 6) / 2 + 1) * ((k->lp - 6) / 2 + 1);
     if ((rc = dalloc(c, &c->bc_part, ((nrow + BC_ROWS - 1) / BC_ROWS) * BC_NV * ni))) return fail(rc);
+    if ((rc = dalloc(c, &c->bc_sums, BC_NV * ni))) return fail(rc);
     if ((rc = dalloc(c, &c->bc_d, ni))) return fail(rc);
   }
   if ((rc = dalloc(c, &c->xt2, N))) return fail(rc);

@@ -1372,84 +1372,120 @@ __global__ __launch_bounds__(FIN_NT) void k_residual_finalize(const double *part
 // reference plane (d = 0, its equation not solved): the p' equation has no
 // coefficient toward a pinned cell and is singular, fixed up to a constant
 // (with coefficients rounded to FP32 only nearly so, and solving the full
-// system would add a large constant).  Three launches, all in a fixed
+// system would add a large constant).  Four launches, all in a fixed
 // order: per-plane partial sums over chunks of BC_ROWS (j,k) rows, the
-// chunk sums and the Thomas solve in one workgroup, the update.
+// chunk sums, the Thomas solve, the update.
 // ======================================================================
-#define BC_ROWS 64
-#define BC_NV 5  // A, W, E, R, free-cell count
+#define BC_ROWS 16  // (j,k) rows per workgroup of k_bc_part: one per wavefront
+#define BC_NV 5     // A, W, E, R, free-cell count
 __device__ __forceinline__ bool bc_free(const Box &b, int i, int j, int k) {
   const int ti = (i - b.i0) >> 1, tj = (j - b.j0) >> 1, tk = (k - b.k0) >> 1;
   if (i < b.i0 || ti >= b.ni || j < b.j0 || tj >= b.nj || k < b.k0 || tk >= b.nk) return false;
   const long long q = lidx(i, j, k);
   return g.ib[q] <= 0 && fabs(CAP(q)) > SMALL20;
 }
-// grid (ceil(ni/64), ceil(nj*nk/BC_ROWS)), 256 threads: lane = plane,
-// wavefront w takes rows w, w+4, ... of the chunk; part[(c*BC_NV + v)*ni + q]
-__global__ __launch_bounds__(256) void k_bc_part(Box b, const double *xt, double *part) {
-  __shared__ double red[3][BC_NV][64];
+// grid (ceil(ni/64), ceil(nj*nk/BC_ROWS)), 64*BC_ROWS threads: lane = plane,
+// wavefront w = row of the chunk, one cell per thread; the rows' sums are
+// added in row order.  part[(c*BC_NV + v)*ni + q]
+__global__ __launch_bounds__(64 * BC_ROWS) void k_bc_part(Box b, const double *xt, double *part) {
+  __shared__ double red[BC_ROWS][BC_NV][64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int q = blockIdx.x * 64 + lane;
-  const int nrow = b.nj * b.nk, r0 = blockIdx.y * BC_ROWS;
+  const int r = blockIdx.y * BC_ROWS + w;
   double acc[BC_NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  if (q < b.ni) {
-    const int i = b.i0 + 2 * q;
-    for (int r = r0 + w; r < r0 + BC_ROWS && r < nrow; r += 4) {
-      const int j = b.j0 + 2 * (r % b.nj), k = b.k0 + 2 * (r / b.nj);
-      if (!bc_free(b, i, j, k)) continue;
+  if (q < b.ni && r < b.nj * b.nk) {
+    const int i = b.i0 + 2 * q, j = b.j0 + 2 * (r % b.nj), k = b.k0 + 2 * (r / b.nj);
+    if (bc_free(b, i, j, k)) {
       const long long c = lidx(i, j, k);
       const double a0 = CAS(0, c), a3 = CAS(3, c), a1 = CAS(1, c), a4 = CAS(4, c), a2 = CAS(2, c), a5 = CAS(5, c);
       const double t1 = a0 * xt[lidx(i - 2, j, k)], t2 = a3 * xt[lidx(i + 2, j, k)];
       const double t3 = a1 * xt[lidx(i, j - 2, k)], t4 = a4 * xt[lidx(i, j + 2, k)];
       const double t5 = a2 * xt[lidx(i, j, k - 2)], t6 = a5 * xt[lidx(i, j, k + 2)];
       const double apc = CAP(c), t7 = apc * xt[c], t8 = CBS(c);
-      acc[3] += t1 + t2 + t3 + t4 + t5 + t6 + t8 - t7;
+      acc[3] = t1 + t2 + t3 + t4 + t5 + t6 + t8 - t7;
       double a = apc;
       if (bc_free(b, i, j - 2, k)) a -= a1;
       if (bc_free(b, i, j + 2, k)) a -= a4;
       if (bc_free(b, i, j, k - 2)) a -= a2;
       if (bc_free(b, i, j, k + 2)) a -= a5;
-      acc[0] += a;
-      if (bc_free(b, i - 2, j, k)) acc[1] += a0;
-      if (bc_free(b, i + 2, j, k)) acc[2] += a3;
-      acc[4] += 1.0;
+      acc[0] = a;
+      if (bc_free(b, i - 2, j, k)) acc[1] = a0;
+      if (bc_free(b, i + 2, j, k)) acc[2] = a3;
+      acc[4] = 1.0;
     }
   }
-  if (w > 0)
 #pragma unroll
-    for (int v = 0; v < BC_NV; v++) red[w - 1][v][lane] = acc[v];
+  for (int v = 0; v < BC_NV; v++) red[w][v][lane] = acc[v];
   __syncthreads();
   if (w == 0 && q < b.ni)
 #pragma unroll
-    for (int v = 0; v < BC_NV; v++)
-      part[((long long)blockIdx.y * BC_NV + v) * b.ni + q] = ((acc[v] + red[0][v][lane]) + red[1][v][lane]) + red[2][v][lane];
+    for (int v = 0; v < BC_NV; v++) {
+      double t = red[0][v][lane];
+      for (int rr = 1; rr < BC_ROWS; rr++) t += red[rr][v][lane];
+      part[((long long)blockIdx.y * BC_NV + v) * b.ni + q] = t;
+    }
 }
-// one workgroup of FIN_NT threads, dynamic LDS BC_NV*ni doubles: the chunk
-// sums in chunk order, then the Thomas solve by thread 0 into d[ni].  A
-// plane without free cells, the last plane, and a plane whose pivot falls
-// below 1e-12 of its diagonal (closing a block of planes decoupled from the
-// rest) get 0.
-__global__ __launch_bounds__(FIN_NT) void k_bc_solve(int ni, int nchunk, const double *part, double *d) {
-  extern __shared__ double bcs[];  // sums[v][q] for v < BC_NV, then cp[q]
-  for (int it = threadIdx.x; it < BC_NV * ni; it += FIN_NT) {
-    const int v = it / ni, q = it - v * ni;
-    double s = 0.0;
-    for (int c = 0; c < nchunk; c++) s += part[((long long)c * BC_NV + v) * ni + q];
-    bcs[it] = s;
+// grid (ceil(ni/64), BC_NV), 1024 threads: plane q = 64*bx + (t mod 64),
+// quantity v = by, slice s = t / 64 of the chunks (c = s, s+16, ... with two
+// accumulators), the 16 slices added in order: sums[v*ni + q].
+__global__ __launch_bounds__(1024) void k_bc_sum(int ni, int nchunk, const double *part, double *sums) {
+  __shared__ double red[16][64];
+  const int lane = threadIdx.x & 63, sl = threadIdx.x >> 6, v = blockIdx.y;
+  const int q = blockIdx.x * 64 + lane;
+  double s0 = 0.0, s1 = 0.0;
+  if (q < ni) {
+    int c = sl;
+    for (; c + 16 < nchunk; c += 32) {
+      s0 += part[((long long)c * BC_NV + v) * ni + q];
+      s1 += part[((long long)(c + 16) * BC_NV + v) * ni + q];
+    }
+    if (c < nchunk) s0 += part[((long long)c * BC_NV + v) * ni + q];
   }
+  red[sl][lane] = s0 + s1;
   __syncthreads();
-  if (threadIdx.x != 0) return;
+  if (sl == 0 && q < ni) {
+    double t = red[0][lane];
+    for (int r = 1; r < 16; r++) t += red[r][lane];
+    sums[(long long)v * ni + q] = t;
+  }
+}
+// one workgroup of 256 threads, dynamic LDS BC_NV*ni doubles: the sums
+// (A, W, E, R, count) staged in LDS, the Thomas solve over the planes by
+// thread 0 there, d[ni] written by all.  A plane without free cells, the
+// last plane, and a plane whose pivot falls below 1e-12 of its diagonal
+// (closing a block of planes decoupled from the rest) get 0.
+__global__ __launch_bounds__(256) void k_bc_solve(int ni, const double *sums, double *d) {
+  extern __shared__ double bcs[];
+  for (int t = threadIdx.x; t < BC_NV * ni; t += 256) bcs[t] = sums[t];
+  __syncthreads();
   double *A = bcs, *W = bcs + ni, *E = bcs + 2 * ni, *R = bcs + 3 * ni, *nf = bcs + 4 * ni;
   double *cp = nf;  // the counts are read before cp overwrites them
-  for (int q = 0; q < ni; q++) {
-    if (nf[q] == 0.0) { A[q] = 1.0; W[q] = E[q] = R[q] = 0.0; }
-    const double m = A[q] + (q > 0 ? W[q] * cp[q - 1] : 0.0);
-    if (q == ni - 1 || !(fabs(m) > 1.0e-12 * fabs(A[q]))) { cp[q] = 0.0; R[q] = 0.0; continue; }
-    cp[q] = -E[q] / m;
-    R[q] = (R[q] + (q > 0 ? W[q] * R[q - 1] : 0.0)) / m;
+  if (threadIdx.x == 0) {
+    // c' and d' of the previous plane stay in registers (the chain of the
+    // elimination), the plane's sums are independent loads
+    double cpm = 0.0, dpm = 0.0;
+    for (int q = 0; q < ni; q++) {
+      double a = A[q], w = W[q], e = E[q], r = R[q];
+      if (nf[q] == 0.0) { a = 1.0; w = e = r = 0.0; }
+      const double m = a + (q > 0 ? w * cpm : 0.0);
+      double cq = 0.0, dq = 0.0;
+      if (q < ni - 1 && fabs(m) > 1.0e-12 * fabs(a)) {
+        cq = -e / m;
+        dq = (r + (q > 0 ? w * dpm : 0.0)) / m;
+      }
+      cp[q] = cq;
+      R[q] = dq;
+      cpm = cq;
+      dpm = dq;
+    }
+    double dn = R[ni - 1];
+    for (int q = ni - 2; q >= 0; q--) {
+      dn = R[q] - cp[q] * dn;
+      R[q] = dn;
+    }
   }
-  d[ni - 1] = R[ni - 1];
-  for (int q = ni - 2; q >= 0; q--) d[q] = R[q] - cp[q] * d[q + 1];
+  __syncthreads();
+  for (int q = threadIdx.x; q < ni; q += 256) d[q] = R[q];
 }
 __global__ void k_bc_apply(Box b, double *xt, const double *d) {
   int i, j, k;
