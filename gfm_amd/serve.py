@@ -11,7 +11,13 @@ Reported: aggregate cell-updates/s over the whole sweep, per-case wall time
 to the first iteration below 1e-3, 1e-4 and 1e-5, and the p50/p95 time from
 the start of the sweep to each case's completion.
 
+With --pcorr-tol the p' solve of every case is relaxed to that
+calc_residual tolerance from zero (GFM_PCORR_TOLERANCE), with --block-corr
+each sweep preceded by the x-plane block correction; without it the solve
+is the reference's fixed sweeps.
+
   python -m gfm_amd.serve [--cases 24 --workers 8 --cap 200 --precision mixed]
+                          [--pcorr-tol 0.01 --block-corr]
 """
 import argparse
 import json
@@ -48,6 +54,9 @@ def _run_case(job, cfg, t_sweep0):
     k.sweep_order = 1
     k.want_pre_residuals = 0
     k.precision = 1 if cfg["precision"] == "mixed" else 0
+    if cfg.get("pcorr_tol", 0.0) > 0.0:
+        k.pcorr_mode, k.pcorr_tol, k.pcorr_max_sweeps = 1, cfg["pcorr_tol"], 400
+        k.pcorr_block_corr = 1 if cfg.get("block_corr") else 0
     first = {f"{t:g}": None for t in THRESHOLDS}
     t0 = time.perf_counter()
     n = 0
@@ -81,7 +90,7 @@ def _worker(jobs, results, cfg, t_sweep0):
             results.put(dict(index=job["index"], error=f"{type(e).__name__}: {e}"))
 
 
-def serve(cases, workers, cap, precision, log=print):
+def serve(cases, workers, cap, precision, log=print, pcorr_tol=0.0, block_corr=False):
     """Run the sweep; returns the summary and the per-case records."""
     ctx = mp.get_context("spawn")
     jobs, results = ctx.Queue(), ctx.Queue()
@@ -89,7 +98,7 @@ def serve(cases, workers, cap, precision, log=print):
         jobs.put(job)
     for _ in range(workers):
         jobs.put(None)
-    cfg = dict(cap=cap, precision=precision)
+    cfg = dict(cap=cap, precision=precision, pcorr_tol=pcorr_tol, block_corr=block_corr)
     t0w = time.time()
     t0 = time.perf_counter()
     procs = [ctx.Process(target=_worker, args=(jobs, results, cfg, t0w)) for _ in range(workers)]
@@ -111,6 +120,7 @@ def serve(cases, workers, cap, precision, log=print):
     work = sum(r["cells"] * r["iterations"] for r in ok)
     done = np.array([r["done_at"] for r in ok]) if ok else np.array([np.nan])
     summary = dict(cases=len(cases), failed=len(recs) - len(ok), workers=workers, precision=precision, cap=cap,
+                   pcorr_tol=pcorr_tol or None, block_correction=bool(block_corr and pcorr_tol),
                    wall_seconds=round(wall, 3), cell_updates=work,
                    aggregate_mcell_updates_per_s=round(work / wall / 1e6, 3),
                    completion_p50_s=round(float(np.percentile(done, 50)), 3),
@@ -126,12 +136,18 @@ def main(argv=None):
     ap.add_argument("--cap", type=int, default=200, help="iteration cap per case")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--pcorr-tol", type=float, default=0.0,
+                    help="relax p' to this calc_residual from zero (0: the reference's fixed sweeps)")
+    ap.add_argument("--block-corr", action="store_true", help="x-plane block correction before each p' sweep")
     a = ap.parse_args(argv)
     if a.workers < 1 or a.workers > 15:
         print("serve: 1..15 worker processes share the GPU", file=sys.stderr)
         return 2
+    if a.block_corr and not a.pcorr_tol > 0.0:
+        print("serve: --block-corr needs --pcorr-tol", file=sys.stderr)
+        return 2
     summary, recs = serve(sweep(a.cases, a.seed), a.workers, a.cap, a.precision,
-                          log=lambda m: print(m, flush=True))
+                          log=lambda m: print(m, flush=True), pcorr_tol=a.pcorr_tol, block_corr=a.block_corr)
     print(json.dumps({"summary": summary, "cases": recs}))
     return 0
 

@@ -15,10 +15,17 @@ def test_sweep_is_seeded_and_in_range():
         assert 20.0 <= j["fuel_u"] <= 40.0 and 900.0 <= j["air_t"] <= 1300.0
 
 
+def test_block_corr_needs_a_tolerance():
+    assert serve.main(["--block-corr"]) == 2
+
+
 @pytest.mark.gpu
-def test_small_sweep_completes():
+@pytest.mark.parametrize("tol,bc", [(0.0, False), (0.01, True)])
+def test_small_sweep_completes(tol, bc):
     cases = [dict(index=q, grid=(64, 32, 24), fuel_u=25.0 + 5 * q, air_t=1000.0, seed=10 + q) for q in range(3)]
-    summary, recs = serve.serve(cases, workers=2, cap=4, precision="mixed", log=lambda m: print(m, flush=True))
+    summary, recs = serve.serve(cases, workers=2, cap=4, precision="mixed", log=lambda m: print(m, flush=True),
+                                pcorr_tol=tol, block_corr=bc)
     assert summary["failed"] == 0
+    assert summary["block_correction"] == bc
     assert all(r["iterations"] == 4 for r in recs)
     assert summary["cell_updates"] == 3 * 4 * 64 * 32 * 24
