@@ -658,7 +658,9 @@ static int gflow(gfm_ctx *c, int stage) {
     Box fb = make_box(2 + nx0, MPh - nx0, 2 + nx1, NPh - nx1, 2 + nx2, LPh - nx2, 2);
     LAUNCH(k_copy_ug_fz, fb, iu1);
   }
-  for (int iu1 = 1; iu1 <= 3; iu1++) LAUNCH(k_momentum_assemble, mom[iu1], iu1);
+  LAUNCH(k_momentum_assemble<1>, mom[1]);
+  LAUNCH(k_momentum_assemble<2>, mom[2]);
+  LAUNCH(k_momentum_assemble<3>, mom[3]);
   LAUNCH(k_pressure_assemble, cen);
   if (k.want_pre_residuals) residual(c, d.p, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_P_PRE, 0);
   adlbl3(c, d.p, 4, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2);

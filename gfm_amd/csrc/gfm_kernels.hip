@@ -438,8 +438,13 @@ __global__ void k_copy_ug_fz(Box b, int iu1) {
   g.fz[q] = g.ug[iu1 - 1][q];
 }
 
-// SOURCE(IU1) -> SRC1 fused with GSLVX1 for one face (C/Gslv3x.f90:26-111)
-__global__ void k_momentum_assemble(Box b, int iu1) {
+// SOURCE(IU1) -> SRC1 fused with GSLVX1 for one face (C/Gslv3x.f90:26-111).
+// The velocity component is a template parameter, so the per-direction
+// arrays below (vis, dxn, dx0, dx1, ffl) are indexed by constants and stay
+// in registers.
+template <int IU1>
+__global__ void k_momentum_assemble(Box b) {
+  constexpr int iu1 = IU1;
   int i, j, k;
   if (!box_pt(b, i, j, k)) return;
   long long q = lidx(i, j, k);
