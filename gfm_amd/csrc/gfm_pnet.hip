@@ -42,6 +42,20 @@ __device__ __forceinline__ pbf8 pconv_load(__amdgpu_buffer_rsrc_t rs, unsigned b
   typedef unsigned pu4 __attribute__((ext_vector_type(4)));
   return __builtin_bit_cast(pbf8, (pu4)__builtin_amdgcn_raw_buffer_load_b128(rs, (int)byte_off, 0, 0));
 }
+// A fragment moved across the 16 lanes of each DPP row: CTRL 0x111
+// (row_shr:1) gives lane l the value of lane l-1, 0x101 (row_shl:1) that of
+// lane l+1, a lane whose source lies outside the row keeping `old`;
+// 0x121 / 0x12F (row_ror:1 / row_ror:15) rotate: lane l gets lane (l-1)
+// mod 16 / (l+1) mod 16.  Every lane of the wavefront must execute it.
+template <int CTRL>
+__device__ __forceinline__ pbf8 pconv_dpp(pbf8 v, pbf8 old) {
+  typedef int pi4 __attribute__((ext_vector_type(4)));
+  const pi4 a = __builtin_bit_cast(pi4, v), o = __builtin_bit_cast(pi4, old);
+  pi4 r;
+#pragma unroll
+  for (int e = 0; e < 4; e++) r[e] = __builtin_amdgcn_update_dpp(o[e], a[e], CTRL, 0xf, 0xf, false);
+  return __builtin_bit_cast(pbf8, r);
+}
 
 // ---------------------------------------------------------------------
 // 3x3x3 (or 1-tap) convolution as an implicit GEMM on the matrix cores:
@@ -146,6 +160,88 @@ __global__ __launch_bounds__(256, NCT <= 2 ? 4 : 2) void k_pconv(const pbf *__re
 #pragma unroll
       for (int mt = 0; mt < MT; mt++) acc[mt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[mt], Bf[ct], acc[mt][ct], 0, 0, 0);
   };
+  // When the wavefront's 16*MT voxels lie in one x-row (W a multiple of
+  // 16*MT: the tiles start at multiples of 16*MT), the three x-taps of a
+  // (dz, dy) row share their voxels: one step per (row tap, chunk) loads the
+  // centre fragments (dx = 0) and one edge fragment, whose lane 0 holds
+  // voxel x-1 of the first tile and lane 15 voxel x+1 of the last (the other
+  // lanes read past the range: no traffic), and forms dx = -1 and +1 by
+  // moving the centre fragments one lane within each 16-lane row, the lane
+  // at a tile's edge taking the neighbouring tile's edge lane.  A voxel's
+  // x-1 and x+1 then come from the lanes that loaded them with the in-grid
+  // test of their own dx = 0 tap, which is the test of this voxel's dx = -1
+  // or +1 tap; the first and last voxels of a row are edge-fragment lanes,
+  // tested by their own tap bits.
+  const bool xrow = ntap == 27 && pg.W % (16 * MT) == 0;
+  if (xrow) {
+    auto fetch_row = [&](int st, pbf8 (&C)[MT], pbf8 &E) {
+      const int rt = st / nch, ch = (st - rt * nch) * 32;
+      const int dz = rt / 3 - 1, dy = rt % 3 - 1, t0 = 3 * rt;  // taps t0, t0+1, t0+2: dx = -1, 0, +1
+      const long long toff = ((long long)dz * pg.H + dy) * pg.W;
+      const bool el = r16 == 0, er = r16 == 15;
+      if (ch < ca && up_a) {
+        const __amdgpu_buffer_rsrc_t rs = pconv_rsrc(xa + ch);
+        // the row's voxels share z and y, so their parities
+        const int oz = dz == 0 ? 0 : dz < 0 ? (pz[0] ? 0 : -sza) : (pz[0] ? sza : 0);
+        const int oy = dy == 0 ? 0 : dy < 0 ? (py[0] ? 0 : -sya) : (py[0] ? sya : 0);
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++)
+          C[mt] = pconv_load(rs, (ofa[mt] + (unsigned)(oz + oy)) | ((ntm[mt] >> (t0 + 1)) << 31));
+        const int ox = el ? (px[0] ? 0 : -sxa) : (px[MT - 1] ? sxa : 0);
+        const unsigned eo = (el ? ofa[0] : ofa[MT - 1]) + (unsigned)(oz + oy + ox);
+        const unsigned eb = el ? (ntm[0] >> t0) : (ntm[MT - 1] >> (t0 + 2));
+        E = pconv_load(rs, (el || er) ? (eo | (eb << 31)) : 0x80000000u);
+      } else {
+        const bool fa = ch < ca;
+        const __amdgpu_buffer_rsrc_t rs = fa ? pconv_rsrc(xa + toff * ca + ch) : pconv_rsrc(xb + toff * cb + (ch - ca));
+        const unsigned cs = (unsigned)(fa ? ca : cb) * 2u;  // bytes per voxel
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++) C[mt] = pconv_load(rs, (fa ? ofa[mt] : ofb[mt]) | ((ntm[mt] >> (t0 + 1)) << 31));
+        // voxel x-1 of the first voxel of a row is out of the grid, so its
+        // tap bit is set whenever the offset below would wrap
+        const unsigned eo = el ? (fa ? ofa[0] : ofb[0]) - cs : (fa ? ofa[MT - 1] : ofb[MT - 1]) + cs;
+        const unsigned eb = el ? (ntm[0] >> t0) : (ntm[MT - 1] >> (t0 + 2));
+        E = pconv_load(rs, (el || er) ? (eo | (eb << 31)) : 0x80000000u);
+      }
+    };
+    // the weights of a step's three taps (the same for every wavefront,
+    // mostly L1 hits) are loaded tap by tap when the step runs
+    auto mma_row = [&](int st, const pbf8 (&C)[MT], const pbf8 &E) {
+      const int rt = st / nch, ch = (st - rt * nch) * 32, t0 = 3 * rt;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        pbf8 Bw[NCT];
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++)
+          Bw[ct] = *reinterpret_cast<const pbf8 *>(
+              reinterpret_cast<const char *>(w + ((long long)(t0 + d) * CO + ct * 16) * cin + ch) + wl);
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++) {
+          pbf8 A = C[mt];
+          if (d == 0) {
+            const pbf8 lo = mt == 0 ? E : pconv_dpp<0x121>(C[mt > 0 ? mt - 1 : 0], C[mt > 0 ? mt - 1 : 0]);
+            A = pconv_dpp<0x111>(C[mt], lo);
+          } else if (d == 2) {
+            const pbf8 hi = mt == MT - 1 ? E : pconv_dpp<0x12F>(C[mt < MT - 1 ? mt + 1 : mt], C[mt < MT - 1 ? mt + 1 : mt]);
+            A = pconv_dpp<0x101>(C[mt], hi);
+          }
+#pragma unroll
+          for (int ct = 0; ct < NCT; ct++) acc[mt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, Bw[ct], acc[mt][ct], 0, 0, 0);
+        }
+      }
+    };
+    const int nrs = 9 * nch, rs1 = (kw + 1) * nrs / KS;
+    int s = kw * nrs / KS;
+    pbf8 C0[MT], E0, C1[MT], E1;
+    fetch_row(s, C0, E0);
+    for (; s + 1 < rs1; s += 2) {
+      fetch_row(s + 1, C1, E1);
+      mma_row(s, C0, E0);
+      if (s + 2 < rs1) fetch_row(s + 2, C0, E0);
+      mma_row(s + 1, C1, E1);
+    }
+    if (s < rs1) mma_row(s, C0, E0);
+  } else {
   // two fragment sets used in turn (no register copies between steps)
   pbf8 A0[MT], B0[NCT], A1[MT], B1[NCT];
   const int st1 = (kw + 1) * nstep / KS;
@@ -158,6 +254,7 @@ __global__ __launch_bounds__(256, NCT <= 2 ? 4 : 2) void k_pconv(const pbf *__re
     mma(A1, B1);
   }
   if (st < st1) mma(A0, B0);
+  }
   if constexpr (KS > 1) {
     __shared__ pf4 red[VG][KS - 1][MT * NCT][64];
     if (kw > 0) {
