@@ -751,7 +751,15 @@ static int gflow(gfm_ctx *c, int stage) {
     LAUNCH(k_copy_gf_fz, cen_all, l);
     int igs = 1;
     if (e == 0 || c->sgm != sigma[e]) { c->sgm = sigma[e]; igs = 0; }
-    LAUNCH(k_scalar_assemble, cen, l, nelg, igs, c->sgm, c->part);
+    switch (nelg) {
+      case 6: LAUNCH(k_scalar_assemble<6>, cen, l, igs, c->sgm, c->part); break;
+      case 7: LAUNCH(k_scalar_assemble<7>, cen, l, igs, c->sgm, c->part); break;
+      case 8: LAUNCH(k_scalar_assemble<8>, cen, l, igs, c->sgm, c->part); break;
+      case 9: LAUNCH(k_scalar_assemble<9>, cen, l, igs, c->sgm, c->part); break;
+      case 10: LAUNCH(k_scalar_assemble<10>, cen, l, igs, c->sgm, c->part); break;
+      case 11: LAUNCH(k_scalar_assemble<11>, cen, l, igs, c->sgm, c->part); break;
+      default: LAUNCH(k_scalar_assemble<12>, cen, l, igs, c->sgm, c->part); break;
+    }
     if (nelg == 8) {
       hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(FIN_NT), 0, c->stream, c->part, (int)box_blocks(cen), 1, SC_QREACT, 0);
       hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(FIN_NT), 0, c->stream, c->part + c->dc.pstride, (int)box_blocks(cen), 1, SC_QWALL, 0);

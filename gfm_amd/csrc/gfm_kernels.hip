@@ -941,7 +941,11 @@ __global__ void k_copy_gf_fz(Box b, int l) {
 // source + flux coefficients (GSLV3G0) + volume and relaxation terms of one
 // scalar equation at one cell.  igs=0 recomputes AS.  part[] collects
 // q_react for the enthalpy equation.
-__global__ void k_scalar_assemble(Box b, int l, int nelg, int igs, double sgm, double *part) {
+// The equation NELG is a template parameter: each instance keeps only its
+// own source branch (fewer registers, no branches on it).
+template <int NELG>
+__global__ void k_scalar_assemble(Box b, int l, int igs, double sgm, double *part) {
+  constexpr int nelg = NELG;
   const int itr_gas = (int)g.scal[SC_ITR];
   int i, j, k;
   double qr = 0.0, qw = 0.0, qlo = 0.0, qhi = 0.0;
