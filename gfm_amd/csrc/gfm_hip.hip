@@ -779,13 +779,13 @@ static int gflow(gfm_ctx *c, int stage) {
   extr(c);
 
   {
-    Box b = make_box(2, MPh, 2, NPh, 3, LPh - 1, 2); LAUNCH(k_intp, b, 3);
-    b = make_box(2, MPh, 3, NPh - 1, 2, LPh, 2); LAUNCH(k_intp, b, 2);
-    b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intp, b, 2);
+    Box b = make_box(2, MPh, 2, NPh, 3, LPh - 1, 2); LAUNCH(k_intp<3>, b);
+    b = make_box(2, MPh, 3, NPh - 1, 2, LPh, 2); LAUNCH(k_intp<2>, b);
+    b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intp<2>, b);
     for (int pj = 0; pj < 2; pj++)
       for (int pk = 0; pk < 2; pk++) {
         b = make_box(3, MPh - 1, 2 + pj, pj ? NPh - 1 : NPh, 2 + pk, pk ? LPh - 1 : LPh, 2);
-        LAUNCH(k_intp, b, 1);
+        LAUNCH(k_intp<1>, b);
       }
   }
   HCHECK(hipGetLastError());
@@ -824,11 +824,11 @@ static int initsv_tail(gfm_ctx *c) {
   for (int pi = 0; pi < 2; pi++)
     for (int pj = 0; pj < 2; pj++) { b = make_box(2 + pi, pi ? MPh - 1 : MPh, 2 + pj, pj ? NPh - 1 : NPh, 4, LPh, 2); LAUNCH(k_intpv, b, 3, 3, pi, pj, 0); }
   extr(c);
-  b = make_box(2, MPh, 2, NPh, 3, LPh - 1, 2); LAUNCH(k_intp, b, 3);
-  b = make_box(2, MPh, 3, NPh - 1, 2, LPh, 2); LAUNCH(k_intp, b, 2);
-  b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intp, b, 2);
+  b = make_box(2, MPh, 2, NPh, 3, LPh - 1, 2); LAUNCH(k_intp<3>, b);
+  b = make_box(2, MPh, 3, NPh - 1, 2, LPh, 2); LAUNCH(k_intp<2>, b);
+  b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intp<2>, b);
   for (int pj = 0; pj < 2; pj++)
-    for (int pk = 0; pk < 2; pk++) { b = make_box(3, MPh - 1, 2 + pj, pj ? NPh - 1 : NPh, 2 + pk, pk ? LPh - 1 : LPh, 2); LAUNCH(k_intp, b, 1); }
+    for (int pk = 0; pk < 2; pk++) { b = make_box(3, MPh - 1, 2 + pj, pj ? NPh - 1 : NPh, 2 + pk, pk ? LPh - 1 : LPh, 2); LAUNCH(k_intp<1>, b); }
   HCHECK(hipGetLastError());
   return GFM_OK;
 }

@@ -1261,7 +1261,10 @@ __device__ __forceinline__ void conc_r(double y[9]) {  // CONC on gf values in r
     y[8] = 1.0 - g0;
   }
 }
-__global__ void k_intp(Box b, int dir) {
+// The direction DIR of the two source points is a template parameter.
+template <int DIR>
+__global__ void k_intp(Box b) {
+  constexpr int dir = DIR;
   int i, j, k;
   if (!box_pt(b, i, j, k)) return;
   int bc = IBC(i, j, k);
