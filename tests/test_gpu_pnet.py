@@ -42,11 +42,14 @@ def _forward_case(shape, seed):
     return p, x, yo
 
 
-def test_forward_matches_oracle(monkeypatch):
-    p, x, yo = _forward_case((2, 8, 16, 24), 0)
+@pytest.mark.parametrize("w", [24, 32])
+def test_forward_matches_oracle(monkeypatch, w):
+    """W = 24: no tile width divides a row, every tap is gathered; W = 32:
+    the x-taps of tiles 2, 1 and 1x4 come from lane shifts of one row."""
+    p, x, yo = _forward_case((2, 8, 16, w), 0)
     for tile in TILES:
         monkeypatch.setenv("GFM_PCONV_TILE", tile)
-        with PNet(CH, max_batch=2, max_cells=8 * 16 * 24) as net:
+        with PNet(CH, max_batch=2, max_cells=8 * 16 * w) as net:
             net.set_params(p)
             assert np.array_equal(net.get_params(), p)
             y = net.forward(x)
@@ -75,15 +78,18 @@ def test_unknown_tile_is_rejected(monkeypatch):
             net.forward(x)
 
 
-def test_gradients_match_oracle(monkeypatch):
+@pytest.mark.parametrize("w", [16, 64])
+def test_gradients_match_oracle(monkeypatch, w):
+    """W = 16: lane-shifted x-taps for tiles 1 and 1x4 only; W = 64: for every
+    tile, the 96- and 128-channel input gradients included."""
     rng = np.random.default_rng(2)
     p = po.init_params(CH, seed=3).astype(np.float32)
-    x, t = _batch(rng, 2, 8, 8, 16)
+    x, t = _batch(rng, 2, 8, 8, w)
     lo, go = po.loss_grad(p.astype(np.float64), CH, x.astype(np.float64), t.astype(np.float64), bf16=True)
     layers, off_out, n = po.layout(CH)
     for tile in TILES:
         monkeypatch.setenv("GFM_PCONV_TILE", tile)
-        with PNet(CH, max_batch=2, max_cells=8 * 8 * 16) as net:
+        with PNet(CH, max_batch=2, max_cells=8 * 8 * w) as net:
             net.set_params(p)
             loss, g = net.loss_grad(x, t)
         print(f"tile {tile}: loss", loss, lo, flush=True)
