@@ -18,7 +18,7 @@ class GfmCtrl(ct.Structure):
         ("ncells", I32), ("i_ref", I32), ("j_ref", I32), ("k_ref", I32),
         ("sweep_order", I32), ("precision", I32), ("want_pre_residuals", I32), ("want_irc", I32),
         ("i1st_rad_done", I32), ("isothermal_walls", I32), ("gray_radiation", I32), ("pcorr_block_corr", I32),
-        ("itr_gas", I32), ("itr_gend", I32), ("maxgi", I32), ("pad1", I32),
+        ("itr_gas", I32), ("itr_gend", I32), ("maxgi", I32), ("want_plane_flows", I32),
         ("bgcon", D), ("factor", D), ("fadj", D),
         ("t0", D), ("trdg", D), ("h_0", D), ("r0", D), ("area0", D), ("vol0", D),
         ("reyg", D), ("reyg2", D), ("eun", D), ("epsr", D), ("taur", D), ("ug0", D), ("dnst0", D),

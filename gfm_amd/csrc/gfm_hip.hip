@@ -61,6 +61,8 @@ struct gfm_ctx {
   double *xg = nullptr;  // G of the line kernel between the two passes of a colour visit
   double *bc_part = nullptr, *bc_sums = nullptr, *bc_d = nullptr;  // x-plane block correction of the p' box:
                                                                    // chunk sums, plane sums, plane values
+  double *flow_part = nullptr, *flow_x = nullptr, *flow_y = nullptr, *flow_z = nullptr;  // FLX1D partials and sums
+  int flows_formed = 0;
   float *xtf = nullptr;  // FP32 working copy of the unknown of ADLBL3 (mixed precision)
   int16_t *stage16 = nullptr;
   // lattice positions of the GFLOW wall treatment: open with a wall
@@ -618,6 +620,26 @@ static int pcorr_relax(gfm_ctx *c) {
   return GFM_OK;
 }
 
+// FLX1D's per-plane flow rates (C/Flx1d.f90:16-73) into flow_x (2 per
+// x-face plane), flow_y and flow_z (4 per plane: mass +/-, enthalpy +/-).
+static void plane_flows(gfm_ctx *c) {
+  const gfm_ctrl &k = c->ctrl;
+  const int MPh = k.mp, NPh = k.np, LPh = k.lp;
+  const Box bx = make_box(3, MPh - 1, 2, NPh, 2, LPh, 2);
+  const int nch = (bx.nj * bx.nk + FLOW_ROWS - 1) / FLOW_ROWS;
+  hipLaunchKernelGGL(k_flow_x, dim3((bx.ni + 63) / 64, nch), dim3(1024), 0, c->stream, bx, c->flow_part);
+  hipLaunchKernelGGL(k_flow_sum, dim3((2 * bx.ni + 63) / 64), dim3(1024), 0, c->stream, nch, 2 * bx.ni, c->flow_part,
+                     c->flow_x);
+  const Box by = make_box(4, MPh - 2, 3, NPh - 1, 4, LPh - 2, 2);
+  LAUNCH(k_flow_yz<2>, by, c->flow_part);
+  hipLaunchKernelGGL(k_flow_sum, dim3((4 * by.nj + 63) / 64), dim3(1024), 0, c->stream, by.nk * ((by.ni + 63) / 64), 4 * by.nj,
+                     c->flow_part, c->flow_y);
+  const Box bz = make_box(4, MPh - 2, 4, NPh - 2, 3, LPh - 1, 2);
+  LAUNCH(k_flow_yz<3>, bz, c->flow_part);
+  hipLaunchKernelGGL(k_flow_sum, dim3((4 * bz.nk + 63) / 64), dim3(1024), 0, c->stream, bz.nj * ((bz.ni + 63) / 64), 4 * bz.nk,
+                     c->flow_part, c->flow_z);
+}
+
 static void extr(gfm_ctx *c);
 
 // ---------------------------------------------------------------------
@@ -706,6 +728,7 @@ static int gflow(gfm_ctx *c, int stage) {
       hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(FIN_NT), 0, c->stream, c->part, (int)box_blocks(ex[q]), 5, SC_GFEX, q > 0);
     }
   }
+  if (k.want_plane_flows) plane_flows(c);
   hipLaunchKernelGGL(k_factor, dim3(1), dim3(1), 0, c->stream);
   for (int q = 0; q < 3; q++) LAUNCH(k_extrv, ex[q], q + 1, 1, c->part);
   // INTPV
@@ -915,6 +938,16 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
     if ((rc = dalloc(c, &c->bc_part, ((nrow + BC_ROWS - 1) / BC_ROWS) * BC_NV * ni))) return fail(rc);
     if ((rc = dalloc(c, &c->bc_sums, BC_NV * ni))) return fail(rc);
     if ((rc = dalloc(c, &c->bc_d, ni))) return fail(rc);
+    // FLX1D: x planes over all (J, K) rows; y and z planes over interior rows
+    const long long px = (k->mp - 2) / 2, py = (k->np - 2) / 2, pz = (k->lp - 2) / 2;  // face planes 3..M-1
+    const long long cx = (k->np / 2) * (long long)(k->lp / 2), ix = (k->mp - 6) / 2 + 1;
+    const long long nbx = (ix + 63) / 64;
+    const long long fx = ((cx + FLOW_ROWS - 1) / FLOW_ROWS) * 2 * px;
+    const long long fy = ((k->lp - 6) / 2 + 1) * nbx * 4 * py, fz = ((k->np - 6) / 2 + 1) * nbx * 4 * pz;
+    if ((rc = dalloc(c, &c->flow_part, std::max(fx, std::max(fy, fz))))) return fail(rc);
+    if ((rc = dalloc(c, &c->flow_x, 2 * px))) return fail(rc);
+    if ((rc = dalloc(c, &c->flow_y, 4 * py))) return fail(rc);
+    if ((rc = dalloc(c, &c->flow_z, 4 * pz))) return fail(rc);
   }
   if ((rc = dalloc(c, &c->xt2, N))) return fail(rc);
   if ((rc = dalloc(c, &c->xt3, N))) return fail(rc);
@@ -1061,7 +1094,8 @@ static int run_gflow(gfm_ctx *c) {
     return gflow(c, 2);
   }
   const int key = k.sweep_order | (k.want_pre_residuals << 2) | (k.want_irc << 3) | ((int)split << 4) |
-                  ((k.pcorr_guess == GFM_PGUESS_LEARNED && c->pnet) << 5) | ((k.pcorr_dump != 0) << 6);
+                  ((k.pcorr_guess == GFM_PGUESS_LEARNED && c->pnet) << 5) | ((k.pcorr_dump != 0) << 6) |
+                  ((k.want_plane_flows != 0) << 7);
   if (!c->graph_exec || key != c->graph_key) {
     drop_graph(c);
     if ((rc = capture(c, split ? 1 : 0, &c->graph, &c->graph_exec))) return rc;
@@ -1083,6 +1117,7 @@ int gfm_hip_gflow(gfm_ctx *c, gfm_ctrl *k) {
   c->ctrl.sweep_order = k->sweep_order;
   c->ctrl.want_pre_residuals = k->want_pre_residuals;
   c->ctrl.want_irc = k->want_irc;
+  c->ctrl.want_plane_flows = k->want_plane_flows;
   if (k->pcorr_mode != GFM_PCORR_REFERENCE && k->pcorr_mode != GFM_PCORR_TOLERANCE) { set_err("bad pcorr_mode"); return GFM_E_ARG; }
   if (k->pcorr_mode == GFM_PCORR_TOLERANCE && (k->pcorr_max_sweeps < 0 || !(k->pcorr_tol >= 0.0))) {
     set_err("pcorr_max_sweeps must be >= 0 and pcorr_tol >= 0");
@@ -1117,6 +1152,7 @@ int gfm_hip_gflow(gfm_ctx *c, gfm_ctrl *k) {
   k->pcorr_gain = c->ctrl.pcorr_gain;
   k->pcorr_fallback = c->ctrl.pcorr_fallback;
   if (c->ctrl.pcorr_dump) c->pair_ready = 1;
+  if (c->ctrl.want_plane_flows) c->flows_formed = 1;
   if (c->prof.on) prof_flush(c);
   return GFM_OK;
 }
@@ -1216,6 +1252,33 @@ int gfm_hip_pcorr_pair(gfm_ctx *c, float *x, float *p) {
   HCHECK(hipMemcpyAsync(x, c->dump_x, 2 * cells * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   HCHECK(hipMemcpyAsync(p, c->dump_p, cells * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   HCHECK(hipStreamSynchronize(c->stream));
+  return GFM_OK;
+}
+
+int gfm_hip_plane_flows(gfm_ctx *c, double *flx, double *fly, double *flyh, double *flux_z, double *fluxh_z) {
+  if (!c) { set_err("null context"); return GFM_E_ARG; }
+  if (!c->flows_formed) { set_err("no iteration has run with want_plane_flows set"); return GFM_E_STATE; }
+  int rc = bind(c);
+  if (rc) return rc;
+  const int mp = c->ctrl.mp, np = c->ctrl.np, lp = c->ctrl.lp;
+  const int px = (mp - 2) / 2, py = (np - 2) / 2, pz = (lp - 2) / 2;
+  std::vector<double> fx(2 * px), fy(4 * py), fz(4 * pz);
+  HCHECK(hipMemcpyAsync(fx.data(), c->flow_x, fx.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HCHECK(hipMemcpyAsync(fy.data(), c->flow_y, fy.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HCHECK(hipMemcpyAsync(fz.data(), c->flow_z, fz.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HCHECK(hipStreamSynchronize(c->stream));
+  // (M,2) column-major, plane I = 3 + 2q at row I-1
+  auto put = [](double *dst, int m, const std::vector<double> &src, int np_, int nv, int off) {
+    if (!dst) return;
+    std::fill(dst, dst + 2 * (size_t)m, 0.0);
+    for (int q = 0; q < np_; q++)
+      for (int v = 0; v < 2; v++) dst[(2 + 2 * q) + (size_t)m * v] = src[(size_t)nv * q + off + v];
+  };
+  put(flx, mp, fx, px, 2, 0);
+  put(fly, np, fy, py, 4, 0);
+  put(flyh, np, fy, py, 4, 2);
+  put(flux_z, lp, fz, pz, 4, 0);
+  put(fluxh_z, lp, fz, pz, 4, 2);
   return GFM_OK;
 }
 

@@ -1499,6 +1499,82 @@ __global__ __launch_bounds__(256) void k_bc_solve(int ni, const double *sums, do
   __syncthreads();
   for (int q = threadIdx.x; q < ni; q += 256) d[q] = R[q];
 }
+// ======================================================================
+// FLX1D, C/Flx1d.f90:16-73: per face plane, the gas mass flow through it
+// in the + and - direction (and for y and z the enthalpy flow), from
+// theta*rho*u*AREA_C*GMFR of each face whose cell code is not 1 or >= 4.
+// Partials part[term*nitem + item] with item = plane*nv + value, summed in
+// term order by k_flow_sum.
+// ======================================================================
+__device__ __forceinline__ double flow_g0(int i, int j, int k, int d) {
+  const int ib = IBC(i, j, k);
+  if (ib == 1 || ib >= 4) return 0.0;
+  const long long q = lidx(i, j, k);
+  return g.theta[q] * g.dnst[q] * g.ug[d - 1][q] * AREA_C(i, j, k, d) * g.gmfr;
+}
+#define FLOW_ROWS 64  // (j,k) rows per workgroup of k_flow_x, 4 per wavefront
+// x-face planes I = b.i0 + 2q (lane q), rows (J, K) of the box; 1,024
+// threads, the 16 wavefronts' sums added in order; item = 2q + (-/+).
+__global__ __launch_bounds__(1024) void k_flow_x(Box b, double *part) {
+  __shared__ double red[16][2][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int q = blockIdx.x * 64 + lane;
+  const int nrow = b.nj * b.nk, r0 = blockIdx.y * FLOW_ROWS;
+  double gp = 0.0, gm = 0.0;
+  if (q < b.ni) {
+    const int i = b.i0 + 2 * q;
+    for (int r = r0 + w; r < r0 + FLOW_ROWS && r < nrow; r += 16) {
+      const double g0 = flow_g0(i, b.j0 + 2 * (r % b.nj), b.k0 + 2 * (r / b.nj), 1);
+      if (g0 >= 0.0) gp += g0; else gm -= g0;
+    }
+  }
+  red[w][0][lane] = gp;
+  red[w][1][lane] = gm;
+  __syncthreads();
+  if (w == 0 && q < b.ni)
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+      double t = red[0][v][lane];
+      for (int r = 1; r < 16; r++) t += red[r][v][lane];
+      part[(long long)blockIdx.y * 2 * b.ni + 2 * q + v] = t;
+    }
+}
+// y-face (DIR 2) or z-face (DIR 3) planes: one wavefront per (x chunk, j,
+// k) of the box, its 64 faces along x summed; the plane is blockIdx.y (y)
+// or blockIdx.z (z), the term (row, x chunk) the other; item = 4*plane +
+// (mass +, mass -, enthalpy +, enthalpy -).
+template <int DIR>
+__global__ void k_flow_yz(Box b, double *part) {
+  int i, j, k;
+  double g0 = 0.0;
+  if (box_pt(b, i, j, k)) g0 = flow_g0(i, j, k, DIR);
+  const double h0 = g0 != 0.0 ? g0 * g.gf[IH - 1][lidx(i, j, k)] * g.h_0 : 0.0;
+  const double v0 = wave_sum(g0 >= 0.0 ? g0 : 0.0), v1 = wave_sum(g0 < 0.0 ? -g0 : 0.0);
+  const double v2 = wave_sum(g0 >= 0.0 ? h0 : 0.0), v3 = wave_sum(g0 < 0.0 ? -h0 : 0.0);
+  if (threadIdx.x != 0) return;
+  const int plane = DIR == 2 ? blockIdx.y : blockIdx.z, row = DIR == 2 ? blockIdx.z : blockIdx.y;
+  const int nplane = DIR == 2 ? b.nj : b.nk;
+  double *o = part + ((long long)row * gridDim.x + blockIdx.x) * 4 * nplane + 4 * plane;
+  o[0] = v0; o[1] = v1; o[2] = v2; o[3] = v3;
+}
+// dst[item] = sum over terms of part[term*nitem + item], in term order per
+// slice of 16; grid ceil(nitem/64), 1,024 threads.
+__global__ __launch_bounds__(1024) void k_flow_sum(int nterm, int nitem, const double *part, double *dst) {
+  __shared__ double red[16][64];
+  const int lane = threadIdx.x & 63, sl = threadIdx.x >> 6;
+  const int it = blockIdx.x * 64 + lane;
+  double s = 0.0;
+  if (it < nitem)
+    for (int t = sl; t < nterm; t += 16) s += part[(long long)t * nitem + it];
+  red[sl][lane] = s;
+  __syncthreads();
+  if (sl == 0 && it < nitem) {
+    double t = red[0][lane];
+    for (int r = 1; r < 16; r++) t += red[r][lane];
+    dst[it] = t;
+  }
+}
+
 __global__ void k_bc_apply(Box b, double *xt, const double *d) {
   int i, j, k;
   if (!box_pt(b, i, j, k) || !bc_free(b, i, j, k)) return;

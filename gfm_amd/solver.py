@@ -47,6 +47,7 @@ def lib():
         L.gfm_hip_finalize.argtypes = [P]
         L.gfm_hip_set_pguess.argtypes = [P, P]
         L.gfm_hip_pcorr_pair.argtypes = [P, ct.POINTER(ct.c_float), ct.POINTER(ct.c_float)]
+        L.gfm_hip_plane_flows.argtypes = [P, D, D, D, D, D]
         L.gfm_hip_kernel_times.argtypes = [P, ct.c_int, ct.c_char_p, ct.c_int, D, D, ct.POINTER(ct.c_int), ct.c_int,
                                            ct.POINTER(ct.c_int)]
         _lib = L
@@ -128,6 +129,20 @@ class CombustionSpace:
         _check(lib().gfm_hip_pcorr_pair(self._ctx, x.ctypes.data_as(ct.POINTER(ct.c_float)),
                                         p.ctypes.data_as(ct.POINTER(ct.c_float))), "gfm_hip_pcorr_pair")
         return x, p
+
+    def plane_flows(self):
+        """FLX1D's flow rates of the last iteration run with
+        ctrl.want_plane_flows = 1 (C/Flx1d.f90:16-73): dict of (M, 2) arrays
+        flx, fly, flyh, flux_z, fluxh_z, row I-1 for face plane I, columns
+        + and - direction (kg/s; enthalpy flows in W)."""
+        mp, np_, lp = self.case.shape
+        out = {k: np.zeros((m, 2), order="F") for k, m in
+               (("flx", mp), ("fly", np_), ("flyh", np_), ("flux_z", lp), ("fluxh_z", lp))}
+        dp = ct.POINTER(ct.c_double)
+        _check(lib().gfm_hip_plane_flows(self._ctx, *(out[k].ctypes.data_as(dp)
+                                                      for k in ("flx", "fly", "flyh", "flux_z", "fluxh_z"))),
+               "gfm_hip_plane_flows")
+        return out
 
     def kernel_times(self, enable=-1, cap=128):
         """Per-kernel HIP-event timings: enable=1 resets and starts, 0 stops,

@@ -120,7 +120,8 @@ typedef struct gfm_ctrl {
   int32_t pcorr_block_corr;       /* 1: x-plane additive correction before each sweep of a p' solve
                                      relaxed to a tolerance (GFM_PCORR_TOLERANCE; not in the reference) */
   /* iteration state */
-  int32_t itr_gas, itr_gend, maxgi, pad1;
+  int32_t itr_gas, itr_gend, maxgi;
+  int32_t want_plane_flows;       /* form FLX1D's per-plane flow rates each iteration (gfm_hip_plane_flows) */
   double bgcon;
   double factor, fadj;
   /* reference scales and nondimensional groups */
@@ -220,6 +221,17 @@ int gfm_hip_set_pguess(gfm_ctx *ctx, struct gfm_pnet *net);
  * x[2v] = BS/AP (0 where the row is pinned), x[2v+1] = 1 for open rows else
  * 0, p[v] = p'. */
 int gfm_hip_pcorr_pair(gfm_ctx *ctx, float *x, float *p);
+
+/* FLX1D's cross-sectional flow rates (C/Flx1d.f90:16-73; formed after
+ * EXTRV(0) of the velocity update, C/Gslv3v.f90:210-211, and read by the
+ * printout C/Sprint.f90:219-228) of the last iteration run with
+ * want_plane_flows set: gas mass flow (kg/s) through each x-face plane
+ * I = 3..MP-1 in +x and -x, flx(MP,2); the same through the y-face planes,
+ * fly(NP,2), and z-face planes, flux_z(LP,2); the enthalpy flows (W) in y
+ * and z, flyh(NP,2), fluxh_z(LP,2).  Column-major with 1-based plane index
+ * as in the reference (entries at even indices are 0).  Any pointer may be
+ * NULL.  GFM_E_STATE if no iteration has formed them. */
+int gfm_hip_plane_flows(gfm_ctx *ctx, double *flx, double *fly, double *flyh, double *flux_z, double *fluxh_z);
 
 /* Per-kernel timing with HIP events recorded on the library's stream around
  * each launch.  enable=1 clears the records and starts recording, enable=0

@@ -774,6 +774,58 @@ static void ADLBL3(double *xt, int ib, int ie, int jb, int je, int kb, int ke) {
   }
 }
 /* calc_residual, C/Adlbl3.f90:239-271 */
+/* FLX1D, C/Flx1d.f90:16-73: the gas mass flow through each x-face plane
+ * I = 3..MP-1 (over J = 2..NP, K = 2..LP), y-face plane J = 3..NP-1 (over
+ * I = 4..MP-2, K = LKZ1..LKZ2) and z-face plane K = 3..LP-1 (over
+ * I = 4..MP-2, J = NJY1..NJY2), split into the + and - direction, and for
+ * y and z the enthalpy flow; faces of wall (1) and melt-surface (4) cells
+ * are skipped.  Arrays (M,2) column-major, 1-based plane index; the
+ * reference reads them only in its printout (C/Sprint.f90:219-228). */
+static double *FLX_, *FLY_, *FLYH_, *FLZ_, *FLZH_;
+static int flx_formed;
+static void FLX1D(void) {
+  FLX_ = realloc(FLX_, sizeof(double) * 2 * MP);
+  FLY_ = realloc(FLY_, sizeof(double) * 2 * NP);
+  FLYH_ = realloc(FLYH_, sizeof(double) * 2 * NP);
+  FLZ_ = realloc(FLZ_, sizeof(double) * 2 * LP);
+  FLZH_ = realloc(FLZH_, sizeof(double) * 2 * LP);
+  memset(FLX_, 0, sizeof(double) * 2 * MP);
+  memset(FLY_, 0, sizeof(double) * 2 * NP);
+  memset(FLYH_, 0, sizeof(double) * 2 * NP);
+  memset(FLZ_, 0, sizeof(double) * 2 * LP);
+  memset(FLZH_, 0, sizeof(double) * 2 * LP);
+  const int ih = 3;
+  for (int i = 3; i <= MPM1; i += 2)
+    for (int j = 2; j <= NP; j += 2)
+      for (int k = 2; k <= LP; k += 2) {
+        const int ib = IBCELL(i, j, k);
+        if (ib == 1 || ib >= 4) continue;
+        const double g0 = THETA(i, j, k) * DNST(i, j, k) * UG(i, j, k, 1) * AREA_C(i, j, k, 1) * C->gmfr;
+        if (g0 >= 0.0) FLX_[i - 1] += g0; else FLX_[i - 1 + MP] -= g0;
+      }
+  for (int j = 3; j <= NPM1; j += 2)
+    for (int i = 4; i <= MPM2; i += 2)
+      for (int k = LKZ1; k <= LKZ2; k += 2) {
+        const int ib = IBCELL(i, j, k);
+        if (ib == 1 || ib >= 4) continue;
+        const double g0 = THETA(i, j, k) * DNST(i, j, k) * UG(i, j, k, 2) * AREA_C(i, j, k, 2) * C->gmfr;
+        const double h0 = g0 * GF(i, j, k, ih) * C->h_0;
+        if (g0 >= 0.0) { FLY_[j - 1] += g0; FLYH_[j - 1] += h0; }
+        else { FLY_[j - 1 + NP] -= g0; FLYH_[j - 1 + NP] -= h0; }
+      }
+  for (int k = 3; k <= LPM1; k += 2)
+    for (int i = 4; i <= MPM2; i += 2)
+      for (int j = NJY1; j <= NJY2; j += 2) {
+        const int ib = IBCELL(i, j, k);
+        if (ib == 1 || ib >= 4) continue;
+        const double g0 = THETA(i, j, k) * DNST(i, j, k) * UG(i, j, k, 3) * AREA_C(i, j, k, 3) * C->gmfr;
+        const double h0 = g0 * GF(i, j, k, ih) * C->h_0;
+        if (g0 >= 0.0) { FLZ_[k - 1] += g0; FLZH_[k - 1] += h0; }
+        else { FLZ_[k - 1 + LP] -= g0; FLZH_[k - 1 + LP] -= h0; }
+      }
+  flx_formed = 1;
+}
+
 static double CALC_RESIDUAL(const double *xt, int ib, int ie, int jb, int je, int kb, int ke) {
   double resid = 0.0;
   for (int k = kb; k <= ke; k += 2)
@@ -1379,6 +1431,7 @@ static void GSLV3V(void) {
         }
   }
   EXTRV(0);
+  if (C->want_plane_flows) FLX1D();
   C->gfin = C->gfin0 * C->gmfr; /* FLX1D, C/Flx1d.f90:75-77 */
   if (C->itr_gas > 10) {
     if (C->gfex < small20) { C->factor = 1.0; C->fadj = 0.0; }
@@ -1649,6 +1702,23 @@ int oracle_block_corr_x(gfm_ctrl *c, const gfm_host_arrays *h, const double *ap,
   memcpy(AS_, as, sizeof(double) * NN * 6);
   memcpy(BS_, bs, sizeof(double) * NN);
   BLOCK_CORR_X(xt, ib, ie, jb, je, kb, ke);
+  return GFM_OK;
+}
+/* FLX1D on the caller's state; and the flows the last FLX1D formed (during
+ * an iteration with want_plane_flows set, or this call). */
+int oracle_flx1d(gfm_ctrl *c, const gfm_host_arrays *h) {
+  int rc = bind(c, h);
+  if (rc) return rc;
+  FLX1D();
+  return GFM_OK;
+}
+int oracle_plane_flows(double *flx, double *fly, double *flyh, double *flz, double *flzh) {
+  if (!flx_formed) return GFM_E_STATE;
+  if (flx) memcpy(flx, FLX_, sizeof(double) * 2 * MP);
+  if (fly) memcpy(fly, FLY_, sizeof(double) * 2 * NP);
+  if (flyh) memcpy(flyh, FLYH_, sizeof(double) * 2 * NP);
+  if (flz) memcpy(flz, FLZ_, sizeof(double) * 2 * LP);
+  if (flzh) memcpy(flzh, FLZH_, sizeof(double) * 2 * LP);
   return GFM_OK;
 }
 double oracle_calc_residual(gfm_ctrl *c, const gfm_host_arrays *h, const double *ap, const double *as,

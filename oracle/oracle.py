@@ -29,6 +29,8 @@ def lib():
         _lib.oracle_initsv_tail.restype = ct.c_int
         _lib.oracle_adlbl3.restype = ct.c_int
         _lib.oracle_block_corr_x.restype = ct.c_int
+        _lib.oracle_flx1d.restype = ct.c_int
+        _lib.oracle_plane_flows.restype = ct.c_int
         _lib.oracle_calc_residual.restype = ct.c_double
         _lib.oracle_ctrl_size.restype = ct.c_size_t
         _lib.oracle_da.restype = ct.c_double
@@ -83,6 +85,27 @@ def block_corr_x(case, ap, as_, bs, xt, box):
     rc = lib().oracle_block_corr_x(ct.byref(case.ctrl), ct.byref(h), _dp(ap), _dp(as_), _dp(bs), _dp(xt), *map(int, box))
     if rc:
         raise RuntimeError(f"oracle_block_corr_x failed: {rc}")
+
+
+def flx1d(case):
+    """FLX1D on the case's current state (no iteration)."""
+    from gfm_amd.abi import host_arrays
+    h = host_arrays(case.arrays)
+    rc = lib().oracle_flx1d(ct.byref(case.ctrl), ct.byref(h))
+    if rc:
+        raise RuntimeError(f"oracle_flx1d failed: {rc}")
+
+
+def plane_flows(case):
+    """The flows the last FLX1D formed: dict of (M, 2) arrays flx, fly, flyh,
+    flux_z, fluxh_z (Fortran order, 1-based plane index i at row i-1)."""
+    mp, np_, lp = case.shape
+    out = {k: np.zeros((m, 2), order="F") for k, m in
+           (("flx", mp), ("fly", np_), ("flyh", np_), ("flux_z", lp), ("fluxh_z", lp))}
+    rc = lib().oracle_plane_flows(*(_dp(out[k]) for k in ("flx", "fly", "flyh", "flux_z", "fluxh_z")))
+    if rc:
+        raise RuntimeError(f"oracle_plane_flows failed: {rc}")
+    return out
 
 
 def calc_residual(case, ap, as_, bs, xt, box):

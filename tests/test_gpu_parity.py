@@ -13,7 +13,7 @@ import pytest
 
 import oracle
 from gfm_amd import case as cs
-from gfm_amd.solver import CombustionSpace
+from gfm_amd.solver import CombustionSpace, GfmError
 from helpers import field_errors
 
 pytestmark = pytest.mark.gpu
@@ -122,6 +122,29 @@ def test_wall_heat_flux_matches():
     assert g.ctrl.q_wall_top == pytest.approx(o.ctrl.q_wall_top, rel=1e-9)
     assert g.ctrl.q_wall_bottom == pytest.approx(o.ctrl.q_wall_bottom, rel=1e-9)
     assert g.ctrl.q_react == pytest.approx(o.ctrl.q_react, rel=1e-9)
+
+
+@pytest.mark.parametrize("iters", [1, 3])
+def test_plane_flows_match_oracle(iters):
+    """FLX1D's per-plane flow rates (formed inside the iteration, after the
+    velocity correction) against the oracle's, every plane and direction."""
+    base = cs.furnace_small(nx=16, ny=12, nz=10, seed=3)
+    base.ctrl.sweep_order = 1
+    base.ctrl.want_plane_flows = 1
+    o, g = base.clone(), base.clone()
+    oracle.initsv_tail(o)
+    with CombustionSpace(g) as cspace:
+        with pytest.raises(GfmError):  # none formed yet
+            cspace.plane_flows()
+        oracle.sphase(o, iters)
+        cspace.sphase(iters)
+        got = cspace.plane_flows()
+    want = oracle.plane_flows(o)
+    for k in want:
+        scale = np.abs(want[k]).max()
+        print(k, "max |gpu - oracle| / max", np.abs(got[k] - want[k]).max() / scale, flush=True)
+        assert scale > 0.0, k
+        np.testing.assert_allclose(got[k], want[k], rtol=0, atol=1e-9 * scale)
 
 
 @pytest.mark.parametrize("nx", [16, 72])

@@ -17,6 +17,8 @@ term, a wrong sign, a swapped index -- fails it):
                                   blocked cells untouched); calc_residual
                                   against an independent numpy evaluation.
   SOURCE(5) res_mass              closed form for a single perturbed face.
+  FLX1D (C/Flx1d.f90:16-73)       an independent numpy evaluation of the
+                                  per-plane flow sums on a developed state.
   x-plane block correction        its defining property: afterwards the
   (no reference counterpart)      residuals of each plane's unpinned cells
                                   sum to zero (numpy evaluation) on every
@@ -246,6 +248,45 @@ def test_block_corr_x_nearly_singular_system_stays_bounded():
     for q in range(fr.shape[0] - 1):
         if fr[q].any():
             assert abs(r1[q][fr[q]].sum()) <= 1e-9 * scale * fr[q].sum(), q
+
+
+def test_flx1d_matches_numpy():
+    c = cs.furnace_small(nx=10, ny=8, nz=6, seed=3)
+    oracle.initsv_tail(c)
+    oracle.sphase(c, 3)
+    oracle.flx1d(c)
+    got = oracle.plane_flows(c)
+    a, k = c.arrays, c.ctrl
+    mp, np_, lp = c.shape
+    ib = a["ibcell"].astype(int)
+    skip = (ib == 1) | (ib >= 4)
+    dx, dr, rs, dz = a["dx"], a["dr"], a["rs"], a["dz"]
+    area = [dr[None, :, None] * rs[None, :, None] * dz[None, None, :],
+            dx[:, None, None] * dz[None, None, :],
+            dx[:, None, None] * dr[None, :, None] * rs[None, :, None]]
+    g = [np.where(skip | (ib == 1), 0.0, a["theta"] * a["dnst"] * a["ug"][..., d] * area[d] * k.gmfr) for d in range(3)]
+    h = a["gf"][..., 2] * k.h_0
+    # x-faces: I = 3..MP-1 over J = 2..NP, K = 2..LP (1-based)
+    for I in range(3, mp, 2):
+        v = g[0][I - 1, 1::2, 1::2]
+        want = (v[v >= 0].sum(), -v[v < 0].sum())
+        np.testing.assert_allclose(got["flx"][I - 1], want, rtol=1e-12, atol=1e-300)
+    for J in range(3, np_, 2):
+        v = g[1][3:mp - 2:2, J - 1, 3:lp - 2:2]
+        hv = v * h[3:mp - 2:2, J - 1, 3:lp - 2:2]
+        want = (v[v >= 0].sum(), -v[v < 0].sum())
+        np.testing.assert_allclose(got["fly"][J - 1], want, rtol=1e-12, atol=1e-300)
+        np.testing.assert_allclose(got["flyh"][J - 1], (hv[v >= 0].sum(), -hv[v < 0].sum()), rtol=1e-12, atol=1e-300)
+    for K in range(3, lp, 2):
+        v = g[2][3:mp - 2:2, 3:np_ - 2:2, K - 1]
+        hv = v * h[3:mp - 2:2, 3:np_ - 2:2, K - 1]
+        want = (v[v >= 0].sum(), -v[v < 0].sum())
+        np.testing.assert_allclose(got["flux_z"][K - 1], want, rtol=1e-12, atol=1e-300)
+        np.testing.assert_allclose(got["fluxh_z"][K - 1], (hv[v >= 0].sum(), -hv[v < 0].sum()), rtol=1e-12, atol=1e-300)
+    # even indices are not planes
+    assert np.all(got["flx"][1::2] == 0.0) and np.all(got["fly"][1::2] == 0.0)
+    # the flow enters through the inlet (x = 0 wall) and leaves through the exit face
+    assert got["flx"][2, 0] > 0.0 and got["flx"][mp - 2, 0] > 0.0
 
 
 def test_calc_residual_definition():
