@@ -76,6 +76,14 @@ module gfm_hip_mod
       type(c_ptr), value :: ctx
       real(c_float), intent(out) :: x(*), p(*)
     end function
+    ! FLX1D's flow rates of the last iteration run with want_plane_flows set:
+    ! flx(MP,2), fly(NP,2), flyh(NP,2), flux_z(LP,2), fluxh_z(LP,2)
+    integer(c_int) function gfm_hip_plane_flows(ctx, flx, fly, flyh, flux_z, fluxh_z) &
+        bind(C, name="gfm_hip_plane_flows")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: flx(*), fly(*), flyh(*), flux_z(*), fluxh_z(*)
+    end function
     ! include/gfm_pnet.h: the network of the learned p' guess
     integer(c_int) function gfm_pnet_create(device, ch, max_batch, max_cells, seed, net) bind(C, name="gfm_pnet_create")
       import :: c_int, c_ptr, c_long_long, c_int64_t
