@@ -153,6 +153,9 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned+FP32 configuration")
     ap.add_argument("--pcorr-tol", type=float, default=0.01,
                     help="p' tolerance of the learned+FP32 configuration (calc_residual)")
+    ap.add_argument("--pcorr-tol-bc", type=float, default=0.05,
+                    help="p' tolerance of FP32 from zero with the block correction (at 0.02-0.1 every solve is one "
+                         "corrected sweep)")
     ap.add_argument("--pcorr-tol-tight", type=float, default=1e-3,
                     help="p' tolerance of the second set of time-to-residual runs (FP32 from zero, learned+FP32, "
                          "learned+FP32 with the block correction); 0 skips them")
@@ -226,13 +229,16 @@ def main():
     if net is not None:
         others += [("fp32_zero_guess_tol", "mixed_zero", None), ("learned_fp32", "learned", net)]
     others += [("fp32_zero_guess_tol_bc", "mixed_zero_bc", None)]
+    def tol_of(mode):
+        return args.pcorr_tol_bc if mode == "mixed_zero_bc" else args.pcorr_tol
+
     for name, mode, nn in others:
-        c2, case2, h2, dt2 = throughput(spec_args, mode, nn, args.steps, args.warmup, args.pcorr_tol, sync)
+        c2, case2, h2, dt2 = throughput(spec_args, mode, nn, args.steps, args.warmup, tol_of(mode), sync)
         c2.close()
         line[name] = {"value": round(n * len(h2) / dt2 / 1e6, 3), "unit": "Mcell-updates/s",
                       "ms_per_step": round(dt2 / max(len(h2), 1) * 1e3, 3)}
         if mode != "mixed":
-            line[name].update(pcorr_tol=args.pcorr_tol, pcorr_sweeps_last=case2.ctrl.pcorr_sweeps)
+            line[name].update(pcorr_tol=tol_of(mode), pcorr_sweeps_last=case2.ctrl.pcorr_sweeps)
         if mode.endswith("_bc"):
             line[name]["block_correction"] = True
     if net is not None:
@@ -240,8 +246,9 @@ def main():
     if args.ttr_cap > 0:
         ttr = {"cap_iterations": args.ttr_cap, "pcorr_tol": args.pcorr_tol}
         for name, mode, nn in [("plain_fp64", "fp64", None)] + others:
-            ttr[name] = time_to_residual(spec_args, mode, nn, args.ttr_cap, args.pcorr_tol)
+            ttr[name] = time_to_residual(spec_args, mode, nn, args.ttr_cap, tol_of(mode))
             ttr[name]["block_correction"] = mode.endswith("_bc")
+            ttr[name]["pcorr_tol"] = tol_of(mode) if mode not in ("fp64", "mixed") else None
         if net is not None and args.pcorr_tol_tight > 0:
             # the p' solve relaxed further: ~100-150 sweeps per solve, where
             # the guess saves more than its inference costs
