@@ -376,22 +376,22 @@ static void pull_scalars(gfm_ctx *c, gfm_ctrl *k, const double *sc) {
 // ---------------------------------------------------------------------
 // Line-kernel instance: tile width (one 128-byte line of each row: 16 FP64
 // or 32 FP32 planes; 2 for fewer planes than a tile) and the solve: lines of
-// up to 128 rows by 16 lanes with M contiguous rows each (M in 2, 4, 6, 8),
-// longer lines by PCR with NU rows per lane across 32 lanes (lines of up to
-// 32*NU rows).
+// up to 160 rows by 16 lanes with M contiguous rows each (M in 2, 4, 6, 8,
+// 10), longer lines by PCR with NU rows per lane across 32 lanes (lines of
+// up to 32*NU rows).
 static int lines_lw(int npl, size_t word) {
   const int full = word == 4 ? 32 : 16;
   return npl >= full ? full : 2;
 }
 static int lines_nu(int nmax) { return nmax <= 96 ? (nmax <= 64 ? 2 : 3) : nmax <= 128 ? 4 : nmax <= 160 ? 5 : nmax <= 192 ? 6 : 8; }
 static int lines_m(int nmax, bool pcr) {
-  if (pcr || nmax > 128) return 0;
-  return nmax <= 32 ? 2 : nmax <= 64 ? 4 : nmax <= 96 ? 6 : 8;
+  if (pcr || nmax > 160) return 0;
+  return nmax <= 32 ? 2 : nmax <= 64 ? 4 : nmax <= 96 ? 6 : nmax <= 128 ? 8 : 10;
 }
 static size_t lines_shm(int lw, int nmax, size_t word) { return 3 * (size_t)lw * (nmax | 1) * word; }
 #define LINES_INSTANCES_LW(X_, LW_)                                                                       \
-  X_(LW_, 1, 2) X_(LW_, 1, 4) X_(LW_, 1, 6) X_(LW_, 1, 8) X_(LW_, 2, 0) X_(LW_, 3, 0) X_(LW_, 4, 0) X_(LW_, 5, 0) \
-  X_(LW_, 6, 0) X_(LW_, 8, 0)
+  X_(LW_, 1, 2) X_(LW_, 1, 4) X_(LW_, 1, 6) X_(LW_, 1, 8) X_(LW_, 1, 10) X_(LW_, 2, 0) X_(LW_, 3, 0) X_(LW_, 4, 0) \
+  X_(LW_, 5, 0) X_(LW_, 6, 0) X_(LW_, 8, 0)
 #define LINES_INSTANCES(X_) LINES_INSTANCES_LW(X_, 32) LINES_INSTANCES_LW(X_, 16) LINES_INSTANCES_LW(X_, 2)
 template <typename T>
 static void launch_lines(gfm_ctx *c, int lw, int nu, int m, unsigned nblk, size_t shm, hipEvent_t e0, hipEvent_t e1,

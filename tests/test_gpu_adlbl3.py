@@ -31,14 +31,15 @@ def _setup(order, sweeps, nx=20, precision=0, ny=14, nz=12):
 
 @pytest.mark.parametrize("order,nx,ny,nz", [(1, 20, 14, 12), (0, 20, 14, 12), (1, 72, 14, 12), (1, 34, 50, 12),
                                             (1, 34, 80, 12), (1, 20, 96, 64), (1, 34, 100, 12), (1, 34, 140, 12),
-                                            (1, 20, 180, 12), (1, 20, 220, 12)])
+                                            (1, 20, 160, 12), (1, 20, 180, 12), (1, 20, 220, 12)])
 def test_adlbl3_matches_oracle(order, nx, ny, nz):
     """Three sweeps from xt = 0 in the box: every value within 1e-12 of the
     oracle's (relative to the largest).  nx=72 (36 planes per colour) runs
-    the tiled line kernel.  Lines of up to 128 rows take the partition
-    solve with 2, 4, 6, 8 rows per lane (ny = 14, 50, 80, 100; the 96x64
-    plane of the 192x96x64 furnace: 6 and 4); ny = 140, 180, 220 the PCR
-    solve with 5, 6 and 8 rows per lane."""
+    the tiled line kernel.  Lines of up to 160 rows take the partition
+    solve with 2, 4, 6, 8, 10 rows per lane (ny = 14, 50, 80, 100, 140 and
+    160 -- the y-lines of the 384x160x96 furnace; the 96x64 plane of the
+    192x96x64 furnace: 6 and 4); ny = 180, 220 the PCR solve with 6 and 8
+    rows per lane."""
     c, box = _setup(order, 3, nx, ny=ny, nz=nz)
     ap, as_, bs, xt, xs, sl = manufactured(c, box, seed=11)
     xo = xt.copy(order="F")
@@ -52,10 +53,11 @@ def test_adlbl3_matches_oracle(order, nx, ny, nz):
     assert np.max(np.abs(xg[sl] - xs[sl])) > 1e-6  # not yet converged: the comparison is of the iterates
 
 
-@pytest.mark.parametrize("nx,ny,nz", [(20, 14, 12), (72, 14, 12), (20, 96, 64)])
+@pytest.mark.parametrize("nx,ny,nz", [(20, 14, 12), (72, 14, 12), (20, 96, 64), (20, 160, 24)])
 def test_adlbl3_fp32_matches_oracle(nx, ny, nz):
     """Mixed-precision context: coefficients rounded to FP32 and FP32 line
-    sweeps (nx=72: tiles of 32 planes of one colour and a partial tile).
+    sweeps (nx=72: tiles of 32 planes of one colour and a partial tile;
+    ny=160: the partition solve with 10 rows per lane).
     After three sweeps every value within 2e-5 of the FP64 oracle's
     (relative to the largest); after 60 the manufactured solution is met
     to 2e-5 (the FP32 rounding floor of the line solves)."""
