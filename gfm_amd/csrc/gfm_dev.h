@@ -84,6 +84,7 @@ enum {
   SC_PG_RMS, SC_PG_ALPHA, SC_PG_FALLBACK,  // learned p' guess: input scale, guess scale, fallback flag
   SC_PG_COS2,  // learned p' guess: fraction of ||BS||^2 the scaled guess removes
   SC_RESID0 = 32,  // 64 residual slots follow
+  SC_TEXIT_AREA = 96, SC_TEXIT_SUM, SC_TEXIT_MAX,  // exit cells: area, T*area, max T (SPHASE's exit temperatures)
   SC_COUNT = 128
 };
 

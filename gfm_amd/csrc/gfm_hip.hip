@@ -352,6 +352,8 @@ static void pull_scalars(gfm_ctx *c, gfm_ctrl *k, const double *sc) {
     k->tave_vol = sc[SC_TVOL_NUM] * k->t0 / sc[SC_TVOL_DEN];
     k->tmax_c = sc[SC_TMAX] * k->t0;
   }
+  k->t_av_exit = sc[SC_TEXIT_AREA] > 0.0 ? sc[SC_TEXIT_SUM] * k->t0 / sc[SC_TEXIT_AREA] : 0.0;
+  k->t_x_exit = sc[SC_TEXIT_MAX] * k->t0;
   const double *r = sc + SC_RESID0;
   if (k->want_pre_residuals) {
     k->p_resid_pre = r[RS_P_PRE];
@@ -811,6 +813,9 @@ static int gflow(gfm_ctx *c, int stage) {
         LAUNCH(k_intp<1>, b);
       }
   }
+  // SPHASE's exit temperatures after the GFLOW call, C/Sphase.f90:74-89
+  LAUNCH(k_exit_t, cen_all, c->part);
+  hipLaunchKernelGGL(k_exit_t_finalize, dim3(1), dim3(FIN_NT), 0, c->stream, c->part, (int)box_blocks(cen_all));
   HCHECK(hipGetLastError());
   return GFM_OK;
 }

@@ -1325,6 +1325,12 @@ __global__ void k_exit_t(Box b, double *part) {
   double v0 = wave_sum(s0), v1 = wave_sum(s1), v2 = wave_max(mx);
   if (threadIdx.x == 0) { part[block_lin()] = v0; part[g.pstride + block_lin()] = v1; part[2 * g.pstride + block_lin()] = v2; }
 }
+__global__ __launch_bounds__(FIN_NT) void k_exit_t_finalize(const double *part, int nb) {
+  double r[3];
+  fin_reduce<3>(part, g.pstride, nb, 1u << 2, r);
+  if (threadIdx.x == 0)
+    for (int n = 0; n < 3; n++) g.scal[SC_TEXIT_AREA + n] = r[n];
+}
 
 // ======================================================================
 // residual of an assembled system, calc_residual C/Adlbl3.f90:239-271

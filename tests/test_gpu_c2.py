@@ -1,0 +1,158 @@
+"""GPU parity against the oracle on the stretched-furnace type of the
+192x96x64 benchmark, at reduced sizes, and from developed states.
+
+``furnace_large`` keeps what distinguishes the benchmark furnace from the
+uniform chamber: y faces stretched toward the burner walls (ratio 1.05),
+two opposed burner pairs on the y walls (inlets with IU = +-2), a melt
+surface floor (IBCELL 4, C/Grid3.f90:228-231) at 1450 K +- 20 K and walls
+and crown at 1600 K.  That exercises the melt-surface branches of INTPV,
+EXTR and the wall sink, the y-inlet branches of SBC/EXTR/INTP and the
+non-uniform metrics (AREA_C, VOL_C, DXAV; C/Grid3.f90:310-471).
+
+48x24x16 runs the tiled line kernel (24 planes per colour: one full tile
+of 16 and a partial one); 32x96x16 has y lines of 96 rows (partition
+solve with 6 rows per lane).
+
+The developed-state tests run the GPU for 150 iterations, hand the state to
+both the oracle and a fresh device context, and compare 3 more iterations:
+itr_gas is then past 10, so the exit mass-flow correction FACTOR
+(C/Gslv3v.f90:213-231) is live, and every balance scalar of the iteration
+is compared (exit flow GFEX and fuel, mean exit temperature, inflow GFIN0,
+exhaust enthalpy, mean and maximum gas temperatures C/Utility.f90:1485-1513,
+SPHASE's exit temperatures C/Sphase.f90:74-89, wall heat and reaction
+heat) together with the IRC residual block RESG (C/Adlbl3.f90:60-94).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from gfm_amd import case as cs
+from gfm_amd.solver import CombustionSpace
+from helpers import field_errors
+
+pytestmark = pytest.mark.gpu
+
+SCALARS = ("factor", "gfex", "fuel_out", "tmean_ex", "gfin0", "gfin", "q_exhaust", "tave", "tave_vol", "tmax_c",
+           "t_av_exit", "t_x_exit", "q_wall", "q_wall_bottom", "q_wall_top", "q_react", "res_mass", "res_mass_max",
+           "p_resid", "pcorr_resid", "h_resid")
+
+
+def large(nx, ny, nz, order=1, **kw):
+    c = cs.furnace_large(nx=nx, ny=ny, nz=nz, seed=3, **kw)
+    c.ctrl.sweep_order = order
+    return c
+
+
+def _pair(base, iters, initsv=True):
+    o, g = base.clone(), base.clone()
+    if initsv:
+        oracle.initsv_tail(o)
+    h_o = oracle.sphase(o, iters)
+    with CombustionSpace(g, initsv=initsv) as s:
+        h_g = s.sphase(iters)
+        s.download()
+    return o, g, h_o, h_g
+
+
+def _rel(a, b):
+    return abs(a - b) / max(abs(a), abs(b), 1e-300)
+
+
+def _check_scalars(o, g, tol):
+    bad = []
+    for name in SCALARS:
+        a, b = getattr(o.ctrl, name), getattr(g.ctrl, name)
+        print(f"  {name:14s} oracle {a:.16e} gpu {b:.16e} rel {_rel(a, b):.2e}", flush=True)
+        if _rel(a, b) > tol:
+            bad.append((name, a, b))
+    for q in range(3):
+        if _rel(o.ctrl.u_resid[q], g.ctrl.u_resid[q]) > tol:
+            bad.append(("u_resid", q, o.ctrl.u_resid[q], g.ctrl.u_resid[q]))
+    for q in range(7):
+        if _rel(o.ctrl.gf_resid[q], g.ctrl.gf_resid[q]) > tol:
+            bad.append(("gf_resid", q, o.ctrl.gf_resid[q], g.ctrl.gf_resid[q]))
+    assert (o.ctrl.imx, o.ctrl.jmx, o.ctrl.kmx) == (g.ctrl.imx, g.ctrl.jmx, g.ctrl.kmx)
+    assert not bad, bad
+
+
+def _check_resg(o, g, tol):
+    """RESG(NELG, 1:2) of the equations solved (NELG 1..12): sums and maxima
+    of |r| after each ADLBL3.  An entry far below the largest of its column
+    is rounding noise of an equation already solved to round-off (the
+    enthalpy of a case whose walls hold it fixed); those are compared
+    against that floor."""
+    ro = np.array(o.ctrl.resg).reshape(2, 20)[:, :12]
+    rg = np.array(g.ctrl.resg).reshape(2, 20)[:, :12]
+    print("  RESG oracle", ro.tolist(), "\n  RESG gpu   ", rg.tolist(), flush=True)
+    assert np.all(ro[:, [0, 1, 2, 3, 4, 5, 7]] > 0.0)
+    for col in range(2):
+        floor = 1e-9 * np.abs(ro[col]).max()
+        np.testing.assert_allclose(rg[col], ro[col], rtol=tol, atol=floor)
+
+
+@pytest.mark.parametrize("dims", [(48, 24, 16), (32, 96, 16)])
+@pytest.mark.parametrize("iters,tol", [(1, 1e-9), (5, 1e-7)])
+def test_stretched_furnace_matches_oracle(dims, iters, tol):
+    base = large(*dims)
+    base.ctrl.want_irc = 1
+    o, g, h_o, h_g = _pair(base, iters)
+    errs = field_errors(o, g)
+    print("C2-type", dims, "iterations", iters, errs, h_o, h_g, flush=True)
+    np.testing.assert_allclose(h_g, h_o, rtol=tol)
+    for k, v in errs.items():
+        assert v < tol, (k, v)
+    _check_scalars(o, g, tol)
+    _check_resg(o, g, tol)
+
+
+def test_stretched_furnace_reference_order():
+    """The reference's plane order (forward then backward GS over x-planes)
+    on the GPU against the oracle in the same order."""
+    base = large(48, 24, 16, order=0)
+    o, g, h_o, h_g = _pair(base, 1)
+    errs = field_errors(o, g)
+    print("C2-type reference order", errs, h_o, h_g, flush=True)
+    np.testing.assert_allclose(h_g, h_o, rtol=1e-9)
+    for k, v in errs.items():
+        assert v < 1e-9, (k, v)
+
+
+@pytest.mark.parametrize("which", ["chamber48x24x16", "stretched48x24x16"])
+def test_developed_state_matches_oracle(which):
+    if which.startswith("chamber"):
+        base = cs.furnace_small(nx=48, ny=24, nz=16, seed=3, maxgi=10 ** 7)
+    else:
+        base = cs.furnace_large(nx=48, ny=24, nz=16, seed=3, maxgi=10 ** 7)
+    base.ctrl.sweep_order = 1
+    base.ctrl.want_irc = 1
+    with CombustionSpace(base) as s:
+        s.sphase(150)
+        s.download()
+    assert base.ctrl.itr_gas == 150
+    o, g, h_o, h_g = _pair(base, 3, initsv=False)
+    errs = field_errors(o, g)
+    print(which, "developed, 3 more iterations", errs, h_o, h_g, "factor", o.ctrl.factor, g.ctrl.factor, flush=True)
+    assert o.ctrl.factor != 1.0  # the exit-flow correction acts
+    np.testing.assert_allclose(h_g, h_o, rtol=1e-8)
+    for k, v in errs.items():
+        assert v < 1e-8, (k, v)
+    _check_scalars(o, g, 1e-8)
+    _check_resg(o, g, 1e-8)
+
+
+def test_convergence_exit_matches_oracle():
+    """SPHASE leaves its loop at the first iteration with res_mass < bgcon
+    (C/Sphase.f90:193) and records it as itr_gend, on the GPU as in the
+    oracle."""
+    base = large(48, 24, 16)
+    probe = base.clone()
+    oracle.initsv_tail(probe)
+    h = oracle.sphase(probe, 6)
+    # the first iteration whose res_mass undercuts every earlier one
+    k = next(q for q in range(1, len(h)) if h[q] < h[:q].min())
+    base.ctrl.bgcon = 0.5 * (h[k] + h[:k].min())
+    o, g, h_o, h_g = _pair(base, 20)
+    print("bgcon", base.ctrl.bgcon, h_o, h_g, flush=True)
+    assert len(h_o) == len(h_g) == k + 1
+    assert o.ctrl.itr_gend == g.ctrl.itr_gend == o.ctrl.itr_gas == g.ctrl.itr_gas
+    assert h_g[-1] < base.ctrl.bgcon and np.all(h_g[:-1] >= base.ctrl.bgcon)
