@@ -54,22 +54,30 @@ def _pair(base, iters, initsv=True):
     return o, g, h_o, h_g
 
 
-def _rel(a, b):
-    return abs(a - b) / max(abs(a), abs(b), 1e-300)
+# calc_residual returns at least 1e-16 (C/Adlbl3.f90:268); an equation
+# relaxed to round-off (the enthalpy with its wall cells held) reads
+# 1e-16 on one side and rounding noise just above it on the other
+RESID_FLOOR = 1e-14
+RESIDUALS = ("p_resid", "pcorr_resid", "h_resid")
+
+
+def _rel(a, b, floor=0.0):
+    return max(abs(a - b) - floor, 0.0) / max(abs(a), abs(b), 1e-300)
 
 
 def _check_scalars(o, g, tol):
     bad = []
     for name in SCALARS:
         a, b = getattr(o.ctrl, name), getattr(g.ctrl, name)
+        fl = RESID_FLOOR if name in RESIDUALS else 0.0
         print(f"  {name:14s} oracle {a:.16e} gpu {b:.16e} rel {_rel(a, b):.2e}", flush=True)
-        if _rel(a, b) > tol:
+        if _rel(a, b, fl) > tol:
             bad.append((name, a, b))
     for q in range(3):
-        if _rel(o.ctrl.u_resid[q], g.ctrl.u_resid[q]) > tol:
+        if _rel(o.ctrl.u_resid[q], g.ctrl.u_resid[q], RESID_FLOOR) > tol:
             bad.append(("u_resid", q, o.ctrl.u_resid[q], g.ctrl.u_resid[q]))
     for q in range(7):
-        if _rel(o.ctrl.gf_resid[q], g.ctrl.gf_resid[q]) > tol:
+        if _rel(o.ctrl.gf_resid[q], g.ctrl.gf_resid[q], RESID_FLOOR) > tol:
             bad.append(("gf_resid", q, o.ctrl.gf_resid[q], g.ctrl.gf_resid[q]))
     assert (o.ctrl.imx, o.ctrl.jmx, o.ctrl.kmx) == (g.ctrl.imx, g.ctrl.jmx, g.ctrl.kmx)
     assert not bad, bad
@@ -80,14 +88,20 @@ def _check_resg(o, g, tol):
     of |r| after each ADLBL3.  An entry far below the largest of its column
     is rounding noise of an equation already solved to round-off (the
     enthalpy of a case whose walls hold it fixed); those are compared
-    against that floor."""
+    against that floor.  eps (NELG 10) is not compared: its sum and maximum
+    both come from the wall-adjacent rows that the log-law pins through
+    BIG = 1e20 (C/Source.f90:590-614), where |r| is BIG times the last bit
+    of eps and follows the line solve's rounding (TDMA in the oracle,
+    partition or PCR here) while eps itself agrees to 1e-15."""
     ro = np.array(o.ctrl.resg).reshape(2, 20)[:, :12]
     rg = np.array(g.ctrl.resg).reshape(2, 20)[:, :12]
     print("  RESG oracle", ro.tolist(), "\n  RESG gpu   ", rg.tolist(), flush=True)
     assert np.all(ro[:, [0, 1, 2, 3, 4, 5, 7]] > 0.0)
+    keep = [q for q in range(12) if q != 9]
     for col in range(2):
-        floor = 1e-9 * np.abs(ro[col]).max()
-        np.testing.assert_allclose(rg[col], ro[col], rtol=tol, atol=floor)
+        floor = 1e-9 * np.abs(ro[col][keep]).max()
+        np.testing.assert_allclose(rg[col][keep], ro[col][keep], rtol=tol, atol=floor)
+    assert np.all(np.isfinite(rg[:, 9]))
 
 
 @pytest.mark.parametrize("dims", [(48, 24, 16), (32, 96, 16)])
