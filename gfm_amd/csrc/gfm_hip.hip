@@ -11,6 +11,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -862,6 +863,66 @@ static int initsv_tail(gfm_ctx *c) {
 }
 
 // ---------------------------------------------------------------------
+// calc_residual allocating nothing (gfm_hip_calc_residual_noalloc): the
+// system in device memory in the reference layout, partial sums in a
+// static device array.
+// ---------------------------------------------------------------------
+#define NOALLOC_NT 256
+#define NOALLOC_MAX_DEVICES 64
+// Per-row partial sums; host code never passes this name to a kernel (it
+// is a host-side stand-in there), only its per-device address from
+// hipGetSymbolAddress.
+static __device__ __align__(256) double scratch[GFM_NOALLOC_MAX_ROWS];
+static double *g_scratch_dev[NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// One workgroup per (J, K) row of the box: the normalised residual terms
+// of calc_residual (C/Adlbl3.f90:239-271) of the row's cells, each lane
+// summing its cells in i order, the lanes' sums added in a fixed order.
+__global__ __launch_bounds__(NOALLOC_NT) void k_resid_rows(int mp, int np, long long nlat, int ni, int nj, int ib, int jb,
+                                                           int kb, const int16_t *ibc, const double *ap, const double *as,
+                                                           const double *bs, const double *xt, double *part) {
+  const int r = blockIdx.x;
+  const int j = jb + 2 * (r % nj), k = kb + 2 * (r / nj);
+  const long long base = ((long long)(k - 1) * np + (j - 1)) * mp - 1;  // linear index of (i, j, k) is base + i
+  const long long sj = 2LL * mp, sk = 2LL * mp * np;
+  double s = 0.0;
+  for (int t = threadIdx.x; t < ni; t += NOALLOC_NT) {
+    const long long q = base + ib + 2 * t;
+    if (ibc[q] > 0) continue;
+    const double t1 = as[q] * xt[q - 2], t2 = as[3 * nlat + q] * xt[q + 2];
+    const double t3 = as[nlat + q] * xt[q - sj], t4 = as[4 * nlat + q] * xt[q + sj];
+    const double t5 = as[2 * nlat + q] * xt[q - sk], t6 = as[5 * nlat + q] * xt[q + sk];
+    const double t7 = ap[q] * xt[q], t8 = bs[q];
+    const double eq_max =
+        fmax(fmax(fmax(fabs(t1), fabs(t2)), fmax(fabs(t3), fabs(t4))), fmax(fmax(fabs(t5), fabs(t6)), fmax(fabs(t7), fabs(t8))));
+    s += fabs(t1 + t2 + t3 + t4 + t5 + t6 + t8 - t7) / eq_max;
+  }
+  __shared__ double sh[NOALLOC_NT / 64];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = 0.0;
+    for (int w = 0; w < NOALLOC_NT / 64; w++) v += sh[w];
+    part[r] = v;
+  }
+}
+// resid = max(sum of the row sums / ncells, 1e-16), summed in a fixed order
+__global__ __launch_bounds__(1024) void k_resid_rows_finalize(const double *part, int nrows, int ncells, double *resid) {
+  __shared__ double sh[1024];
+  double v = 0.0;
+  for (int r = threadIdx.x; r < nrows; r += 1024) v += part[r];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) resid[0] = fmax(sh[0] / ncells, 1.0e-16);
+}
+
+// ---------------------------------------------------------------------
 // C-ABI
 // ---------------------------------------------------------------------
 extern "C" {
@@ -1232,6 +1293,50 @@ int gfm_hip_calc_residual(gfm_ctx *c, const double *ap, const double *as, const 
   HCHECK(hipMemcpyAsync(sc, d.scal, sizeof sc, hipMemcpyDeviceToHost, c->stream));
   HCHECK(hipStreamSynchronize(c->stream));
   *resid = sc[SC_RESID0 + 63];
+  return GFM_OK;
+}
+
+int gfm_hip_calc_residual_noalloc(int mp, int np, int lp, int ncells, const int16_t *ibcell, const double *ap,
+                                  const double *as, const double *bs, const double *xt, int ib, int ie, int jb, int je,
+                                  int kb, int ke, double *resid, void *stream) {
+  // every check before the first HIP call
+  if (mp < 8 || np < 8 || lp < 8 || (mp & 1) || (np & 1) || (lp & 1)) {
+    set_err("bad lattice dimensions %d %d %d", mp, np, lp);
+    return GFM_E_ARG;
+  }
+  auto ok = [](int b, int e, int m) { return b >= 3 && b <= e && e <= m - 2 && ((e - b) & 1) == 0; };
+  if (!ok(ib, ie, mp) || !ok(jb, je, np) || !ok(kb, ke, lp)) {
+    set_err("box must satisfy 3 <= begin <= end <= M-2 with end-begin even on each axis");
+    return GFM_E_ARG;
+  }
+  const int ni = (ie - ib) / 2 + 1, nj = (je - jb) / 2 + 1, nk = (ke - kb) / 2 + 1;
+  const long long nrows = (long long)nj * nk;
+  if (nrows > GFM_NOALLOC_MAX_ROWS) {
+    set_err("box of %lld (J,K) rows exceeds the %d partial sums of the static device array", nrows, GFM_NOALLOC_MAX_ROWS);
+    return GFM_E_ARG;
+  }
+  if (ncells <= 0) { set_err("ncells must be positive"); return GFM_E_ARG; }
+  if (!ibcell || !ap || !as || !bs || !xt || !resid) { set_err("null argument"); return GFM_E_ARG; }
+  int dev = -1;
+  HCHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) { set_err("device %d beyond the %d supported", dev, NOALLOC_MAX_DEVICES); return GFM_E_DEVICE; }
+  double *part = nullptr;
+  {
+    // the array's address on this device, looked up once with the device current
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+      void *p = nullptr;
+      HCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+      g_scratch_dev[dev] = (double *)p;
+    }
+    part = g_scratch_dev[dev];
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const long long nlat = (long long)mp * np * lp;
+  hipLaunchKernelGGL(k_resid_rows, dim3((unsigned)nrows), dim3(NOALLOC_NT), 0, st, mp, np, nlat, ni, nj, ib, jb, kb, ibcell, ap,
+                     as, bs, xt, part);
+  hipLaunchKernelGGL(k_resid_rows_finalize, dim3(1), dim3(1024), 0, st, (const double *)part, (int)nrows, ncells, resid);
+  HCHECK(hipGetLastError());
   return GFM_OK;
 }
 

@@ -207,6 +207,27 @@ int gfm_hip_calc_residual(gfm_ctx *ctx, const double *ap, const double *as, cons
                           const double *xt, int ib, int ie, int jb, int je, int kb, int ke,
                           double *resid);
 
+/* calc_residual, allocating nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  A
+ * variant of gfm_hip_calc_residual that needs no context: the system is
+ * already in device memory in the reference layout (column-major doubled
+ * lattice (mp,np,lp), FP64; as holds the 6 blocks AS(:,:,:,1..3,1..2);
+ * ibcell int16), ncells is NCELLS, the box is as for gfm_hip_adlbl3, and
+ * the result (one double) is written to the device pointer resid.  The
+ * two kernels are enqueued on `stream` (a hipStream_t; NULL = the default
+ * stream) and the call returns without synchronising.  Partial sums (one
+ * per (J,K) row of the box) go to a static device array of
+ * GFM_NOALLOC_MAX_ROWS doubles (512 KB), one per GPU, held until the
+ * process ends; a box with more rows is rejected with GFM_E_ARG.  Calls
+ * that overlap on one GPU share that array and race: serialise them (one
+ * stream, or events).  FP64 always (a mixed-precision context's
+ * gfm_hip_calc_residual rounds the coefficients to FP32; this does not). */
+#define GFM_NOALLOC_MAX_ROWS 65536
+int gfm_hip_calc_residual_noalloc(int mp, int np, int lp, int ncells, const int16_t *ibcell, const double *ap,
+                                  const double *as, const double *bs, const double *xt, int ib, int ie, int jb, int je,
+                                  int kb, int ke, double *resid, void *stream);
+
 /* Attach the network of include/gfm_pnet.h for GFM_PGUESS_LEARNED (NULL
  * detaches).  The case's cell grid (nx, ny, nz of the p' box) must be
  * multiples of 8 and fit the network's buffers; the network must live on

@@ -50,3 +50,20 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(solver, "LIB_PATH", str(tmp_path / "nope.so"))
     with pytest.raises(solver.GfmError):
         solver.lib()
+
+
+def test_noalloc_rejects_oversized_box_before_any_device_call():
+    """The allocation-free calc_residual checks its box against the static
+    partial-sum array (one entry per (J,K) row) before any HIP call, so the
+    rejection runs without a GPU.  The pointers are never dereferenced."""
+    lib = solver.lib()
+    f = lib.gfm_hip_calc_residual_noalloc
+    f.argtypes = [ct.c_int] * 4 + [ct.c_void_p] * 5 + [ct.c_int] * 6 + [ct.c_void_p] * 2
+    f.restype = ct.c_int
+    fake = ct.c_void_p(16)
+    n = 600  # 297 x 297 rows > 65536
+    rc = f(8, n, n, 1, fake, fake, fake, fake, fake, 3, 5, 3, n - 3, 3, n - 3, fake, None)
+    assert rc == 1
+    assert b"65536" in lib.gfm_hip_last_error()
+    rc = f(8, 20, 20, 1, fake, fake, fake, fake, fake, 3, 6, 3, 17, 3, 17, fake, None)  # odd box extent
+    assert rc == 1
