@@ -19,8 +19,10 @@
  * diagnostic sums (FLX/FLY/FLUX_Z plane fluxes of FLX1D, the Info/gresid
  * file writes) are not formed.
  *
- * Parity pinning: see the header of tests/test_oracle.py for which closed
- * forms, invariants and reference-held data pin each function here.
+ * Parity pinning: see the headers of tests/test_oracle.py and
+ * tests/test_oracle_pins.py for which closed forms, invariants and
+ * reference-held data pin each function here (oracle_probe runs single
+ * stages for the latter).
  */
 #include <math.h>
 #include <stdint.h>
@@ -66,6 +68,8 @@ static gfm_ctrl *C;
 static double SGM = 1.0;
 static int NELG = 0;
 static double T_new_state = 0.0; /* module variable T_new of enth_to_T */
+static double *RMASS_ = NULL;     /* per-cell r_mass of the last SOURCE(5) (-1 where not formed), for diagnosis */
+static int STOP_STAGE = 0;        /* probes: 1 = GFLOW returns after the momentum assembly, 2 = after the velocity correction */
 
 #define IX3(i, j, k) ((((size_t)(k)-1) * (size_t)NP + ((size_t)(j)-1)) * (size_t)MP + ((size_t)(i)-1))
 #define X(i) X_[(i)-1]
@@ -498,7 +502,10 @@ static void SOURCE(int mi1, int mi3, int nj1, int nj3, int lk1, int lk3) {
       for (int j = nj1; j <= nj3; j += 2)
         for (int k = lk1; k <= lk3; k += 2) SRC1(i, j, k, nelg);
   } else if (nelg == 4 || nelg == 5) {
-    if (nelg == 5) { C->res_mass = 0.0; C->res_mass_max = 0.0; C->imx = 2; C->jmx = 2; C->kmx = 2; }
+    if (nelg == 5) {
+      C->res_mass = 0.0; C->res_mass_max = 0.0; C->imx = 2; C->jmx = 2; C->kmx = 2;
+      if (RMASS_) for (size_t q = 0; q < NN; q++) RMASS_[q] = -1.0;
+    }
     for (int i = mi1; i <= mi3; i += 2)
       for (int j = nj1; j <= nj3; j += 2)
         for (int k = lk1; k <= lk3; k += 2) {
@@ -523,6 +530,7 @@ static void SOURCE(int mi1, int mi3, int nj1, int nj3, int lk1, int lk3) {
             double fmax = dmax(dmax(dmax(fabs(frx1), fabs(frx2)), dmax(fabs(fry1), fabs(fry2))),
                                dmax(dmax(fabs(frz1), fabs(frz2)), evms));
             r_mass = (fmax == 0.0) ? 0.0 : fabs(BS(i, j, k)) * C->eun / fmax;
+            if (RMASS_) RMASS_[IX3(i, j, k)] = r_mass;
             if (C->res_mass_max < r_mass) { C->imx = i; C->jmx = j; C->kmx = k; C->res_mass_max = r_mass; }
             C->res_mass += r_mass;
           }
@@ -1307,6 +1315,7 @@ static void GSLV3V(void) {
         for (int k = 2 + nx[2]; k <= LP - nx[2]; k += 2) FZ(i, j, k) = UG(i, j, k, iu1);
   }
   for (int iu1 = 1; iu1 <= 3; iu1++) { NELG = iu1; GSLVX1(iu1); }
+  if (STOP_STAGE == 1) return;
 
   /* pressure equation */
   NELG = 4;
@@ -1430,6 +1439,7 @@ static void GSLV3V(void) {
           BOUND(&UG(i, j, k, iu1), iu1, 1);
         }
   }
+  if (STOP_STAGE == 2) return;
   EXTRV(0);
   if (C->want_plane_flows) FLX1D();
   C->gfin = C->gfin0 * C->gmfr; /* FLX1D, C/Flx1d.f90:75-77 */
@@ -1567,6 +1577,7 @@ static void GFLOW(void) {
       }
 
   GSLV3V();
+  if (STOP_STAGE) return;
 
   /* pressure referencing */
   int i = C->i_ref, j = C->j_ref, k = C->k_ref;
@@ -1644,6 +1655,12 @@ static int bind(gfm_ctrl *c, const gfm_host_arrays *h) {
   static size_t cap = 0;
   if (cap != NN) {
     free(FZ_); free(BS_); free(SFP_); free(AP_); free(AS_); free(XT2_); free(XT3_);
+    if (RMASS_) {
+      free(RMASS_);
+      RMASS_ = calloc(NN, sizeof(double));
+      if (!RMASS_) return GFM_E_DEVICE;
+      for (size_t q = 0; q < NN; q++) RMASS_[q] = -1.0;
+    }
     FZ_ = calloc(NN, sizeof(double)); BS_ = calloc(NN, sizeof(double)); SFP_ = calloc(NN, sizeof(double));
     AP_ = calloc(NN, sizeof(double)); AS_ = calloc(NN * 6, sizeof(double));
     XT2_ = calloc((size_t)(NP + 4) * (LP + 4), sizeof(double)); XT3_ = calloc((size_t)(NP + 4) * (LP + 4), sizeof(double));
@@ -1737,6 +1754,55 @@ void oracle_scratch(double *ap, double *as, double *bs, double *fz) {
   if (bs) memcpy(bs, BS_, sizeof(double) * NN);
   if (fz) memcpy(fz, FZ_, sizeof(double) * NN);
 }
+/* Per-cell r_mass of the last SOURCE(5) (C/Source.f90:76-95) on the
+ * lattice, -1 at positions where none is formed (blocked, exit-adjacent,
+ * non-centre).  Recording starts with the first call (enable = 1). */
+int oracle_rmass(int enable, double *out) {
+  if (enable && !RMASS_) {
+    RMASS_ = calloc(NN, sizeof(double));
+    if (!RMASS_) return GFM_E_DEVICE;
+    for (size_t q = 0; q < NN; q++) RMASS_[q] = -1.0;
+  }
+  if (out && RMASS_) memcpy(out, RMASS_, sizeof(double) * NN);
+  return GFM_OK;
+}
+/* Single stages of the iteration on the caller's state, for the tests that
+ * pin the oracle against closed forms and invariants:
+ *   1 CONC, 2 DNST_CALC, 3 THER at lattice position (i,j,k);
+ *   4 q_conv_face(i,j,k, ds = io[0], T_wall = io[1] K) -> io[2];
+ *   5 GFLOW up to and including the momentum assembly (GSLVX1 for u, v, w:
+ *     AP/AS/BS of the face lattices, UG = pseudo-velocities);
+ *   6 GFLOW up to and including the velocity correction (before EXTRV):
+ *     UG corrected, FZ = p', BS = SOURCE(5)'s mass source, res_mass set;
+ *   7 SOURCE of equation NELG = i over the cell box (BS, SFP; CRATE/HRATE
+ *     for the fuel);
+ *   8 INTP; 9 INTPV.
+ * AP/AS/BS/FZ/SFP are read back with oracle_scratch / oracle_sfp. */
+int oracle_probe(gfm_ctrl *c, const gfm_host_arrays *h, int what, int i, int j, int k, double *io) {
+  int rc = bind(c, h);
+  if (rc) return rc;
+  switch (what) {
+    case 1: CONC(i, j, k); break;
+    case 2: DNST_CALC(i, j, k); break;
+    case 3: THER(i, j, k); break;
+    case 4: io[2] = q_conv_face(i, j, k, io[0], io[1]); break;
+    case 5:
+    case 6:
+      STOP_STAGE = what - 4;
+      GFLOW();
+      STOP_STAGE = 0;
+      break;
+    case 7:
+      NELG = i;
+      SOURCE(4, MPM2, NJY1, NJY2, LKZ1, LKZ2);
+      break;
+    case 8: INTP(); break;
+    case 9: INTPV(); break;
+    default: return GFM_E_ARG;
+  }
+  return GFM_OK;
+}
+void oracle_sfp(double *sfp) { memcpy(sfp, SFP_, sizeof(double) * NN); }
 int oracle_abi_version(void) { return GFM_ABI_VERSION; }
 size_t oracle_ctrl_size(void) { return sizeof(gfm_ctrl); }
 

@@ -30,6 +30,8 @@ def lib():
         _lib.oracle_adlbl3.restype = ct.c_int
         _lib.oracle_block_corr_x.restype = ct.c_int
         _lib.oracle_flx1d.restype = ct.c_int
+        _lib.oracle_rmass.restype = ct.c_int
+        _lib.oracle_probe.restype = ct.c_int
         _lib.oracle_plane_flows.restype = ct.c_int
         _lib.oracle_calc_residual.restype = ct.c_double
         _lib.oracle_ctrl_size.restype = ct.c_size_t
@@ -67,6 +69,45 @@ def sphase(case, n_iters):
     if n < 0:
         raise RuntimeError(f"oracle_sphase failed: {-n}")
     return hist[:n]
+
+
+def rmass(case, enable=True):
+    """Per-cell r_mass of the last SOURCE(5) as a lattice array (-1 where
+    none is formed); recording starts with the first call."""
+    out = np.zeros(case.shape, order="F")
+    rc = lib().oracle_rmass(int(enable), _dp(out))
+    if rc:
+        raise RuntimeError(f"oracle_rmass failed: {rc}")
+    return out
+
+
+PROBES = {"conc": 1, "dnst_calc": 2, "ther": 3, "q_conv": 4, "momentum": 5, "vel_correct": 6, "source": 7,
+          "intp": 8, "intpv": 9}
+
+
+def probe(case, what, i=0, j=0, k=0, io=None):
+    """One stage of the iteration on the case's state (oracle_probe):
+    'conc', 'dnst_calc', 'ther' at lattice position (i, j, k); 'q_conv'
+    with io = [ds, T_wall] (returns q); 'momentum' (GFLOW through the
+    momentum assembly), 'vel_correct' (through the velocity correction),
+    'source' (SOURCE of equation NELG = i), 'intp', 'intpv'."""
+    from gfm_amd.abi import host_arrays
+    h = host_arrays(case.arrays)
+    buf = np.zeros(3) if io is None else np.array(list(io) + [0.0] * (3 - len(io)), dtype=np.float64)
+    rc = lib().oracle_probe(ct.byref(case.ctrl), ct.byref(h), PROBES[what], int(i), int(j), int(k), _dp(buf))
+    if rc:
+        raise RuntimeError(f"oracle_probe({what}) failed: {rc}")
+    return float(buf[2])
+
+
+def scratch(case):
+    """The oracle's AP, AS(...,3,2), BS, FZ and SFP arrays after a probe."""
+    shp = case.shape
+    ap, bs, fz, sfp = (np.zeros(shp, order="F") for _ in range(4))
+    as_ = np.zeros(shp + (3, 2), order="F")
+    lib().oracle_scratch(_dp(ap), _dp(as_), _dp(bs), _dp(fz))
+    lib().oracle_sfp(_dp(sfp))
+    return dict(ap=ap, as_=as_, bs=bs, fz=fz, sfp=sfp)
 
 
 def adlbl3(case, nelg, ap, as_, bs, xt, box):
