@@ -22,6 +22,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# Algorithmic byte model of one cell through one SIMPLER outer iteration
+# (words of 8 bytes in FP64, 4 in FP32 for the assembly and sweeps): 198
+# words of assembly, 820 of line relaxation (41 sweeps x 2 line directions
+# x (8 coefficients + 1 read + 1 write)), 108 of post-solve residuals, 12 of
+# velocity correction, 8 of interpolation, plus 88 bytes of FP64 property
+# update.  FP64: 9,256 B per cell-update; mixed: about 4,692 B.
+MODEL_BYTES = {"fp64": 1146 * 8 + 88, "mixed": 4692}
+LINE_WORDS = 820          # line-relaxation term per cell-iteration
+LINE_LAUNCHES_PER_SWEEP = 8  # zebra order: 4 colour visits x 2 passes, each launch one colour (half the cells)
 PMC_SUMMARY = os.path.join("profiles", "pmc_k_lines_furnace192x96x64.md")
 
 
@@ -52,8 +61,19 @@ def cpu_baseline(spec_args, iters=2):
     t = time.perf_counter()
     n = len(oracle.sphase(c, iters))
     dt = time.perf_counter() - t
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": c.ncells * n / dt / 1e6, "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{n} SIMPLER iterations of the same 192x96x64 case, reference x-plane order, "
+            "cpu_model": model, "nproc": os.cpu_count(),
+            "sample": f"{n} SIMPLER iterations of the same {c.ctrl.mp // 2 - 2}x{c.ctrl.np // 2 - 2}x"
+                      f"{c.ctrl.lp // 2 - 2} case, reference x-plane order, "
                       f"oracle/gfm_oracle.c (gcc -O2) on 1 host core, {dt:.1f} s"}
 
 
@@ -208,13 +228,20 @@ def main():
 
     ms, by, la = kt.get("k_lines", (0.0, 0.0, 0))
     avg_s = ms / 1e3 / max(la, 1)
-    per_launch = by / max(la, 1)
+    word = 8 if args.precision == "fp64" else 4
+    # the line term of the byte model per launch: 20 words per cell and
+    # sweep (2 directions x 10 words), spread over the 8 launches of a sweep
+    # of which each covers half the cells (one colour)
+    per_launch = LINE_WORDS / 41 / LINE_LAUNCHES_PER_SWEEP * 2 * (n / 2) * word
     achieved = per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    kernel_model = by / max(la, 1)  # the kernel's own count: coefficients re-read by every pass
     traffic = measured_traffic(args.precision, (args.nx, args.ny, args.nz))
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "k_lines", "avg_launch_us": round(avg_s * 1e6, 2), "bytes_per_launch": per_launch,
-            "launches": la, "traffic_source": PMC_SUMMARY if traffic else None}
+            "kernel": "k_lines", "avg_launch_us": round(avg_s * 1e6, 2), "bytes_per_launch": round(per_launch),
+            "launches": la, "traffic_source": PMC_SUMMARY if traffic else None,
+            "kernel_model_bytes_per_launch": round(kernel_model),
+            "kernel_model_gbs": round(kernel_model / avg_s / 1e9, 1) if avg_s > 0 else None}
     line = {
         "metric": "time to residual 1e-5 & Mcell-updates/s: learned+FP32 vs plain FP64, 1 GPU",
         "value": round(value, 3), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
@@ -224,6 +251,12 @@ def main():
                                f"stretched furnace {args.nx}x{args.ny}x{args.nz} "
                                f"({n} cells), zebra x-plane order", "cells": n, "res_mass_last": res_final},
         "roofline": roof,
+        # the whole iteration against the byte model (9,256 B per cell-update in FP64)
+        "iteration_roofline": {"model_bytes_per_cell_update": MODEL_BYTES[args.precision],
+                               "achieved": round(MODEL_BYTES[args.precision] * n * steps / dt / 1e9, 1),
+                               "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                               "frac": round(MODEL_BYTES[args.precision] * n * steps / dt / 1e9 / HBM_PEAK_GBS, 4),
+                               "ceiling_mcell_updates_s": round(HBM_PEAK_GBS * 1e9 / MODEL_BYTES[args.precision] / 1e6, 1)},
     }
     others = [("fp32", "mixed", None)]
     if net is not None:
