@@ -619,7 +619,8 @@ def air_velocity(u_air0, fuel_u, fuel_u0, air_t, air_t0, excess_air, excess_air0
 
 
 def furnace_small(nx=48, ny=24, nz=16, seed=1, noise=0.01, maxgi=300, bgcon=1.0e-12, walls="isothermal",
-                  gray_kappa=0.3, fuel_u=30.0, air_t=1100.0, excess_air=0.10):
+                  gray_kappa=0.3, fuel_u=30.0, air_t=1100.0, excess_air=0.10, air_port=(0.3, 0.2), air_u=None,
+                  reacting=True):
     """3.0 x 1.2 x 0.8 m chamber, uniform cells, one CH4 port (0.1x0.1 m,
     30 m/s, 300 K) centred on the x=0 wall inside an air port (0.3x0.2 m,
     10 m/s, 1100 K), full-face exit at x=3 m; uniform initial field
@@ -628,7 +629,10 @@ def furnace_small(nx=48, ny=24, nz=16, seed=1, noise=0.01, maxgi=300, bgcon=1.0e
     gray_kappa=0 give the reference's behaviour before its first radiation
     pass).  For grids other than 48x24x16 the ports keep their physical
     size.  fuel_u, air_t, excess_air set an operating point (air velocity
-    from air_velocity, nominal 10 m/s)."""
+    from air_velocity, nominal 10 m/s, or air_u).  air_port (y, z extent in
+    m) sizes the air port; (1.2, 0.8) makes it the whole x=0 wall around
+    the fuel port (a coflow chamber).  reacting=False sends air at 300 K
+    through the fuel port (no reaction)."""
     Lx, Ly, Lz = 3.0, 1.2, 0.8
     u_air = air_velocity(10.0, fuel_u, 30.0, air_t, 1100.0, excess_air)
     dy, dz = Ly / ny, Lz / nz
@@ -640,9 +644,12 @@ def furnace_small(nx=48, ny=24, nz=16, seed=1, noise=0.01, maxgi=300, bgcon=1.0e
         return max(a, 0), min(b, n - 1)
 
     fy = span(0.6, 0.1, dy, ny); fz = span(0.4, 0.1, dz, nz)
-    ay = span(0.6, 0.3, dy, ny); az = span(0.4, 0.2, dz, nz)
+    ay = span(0.6, air_port[0], dy, ny); az = span(0.4, air_port[1], dz, nz)
+    if air_u is not None:
+        u_air = air_u
+    fy_f, fy_o2, fy_n2 = (1.0, 0.0, 0.0) if reacting else (0.0, 0.233, 0.767)
     fuel = Inlet(box=(2, cell_I(fy[0]), cell_I(fz[0]), 2, cell_I(fy[1]), cell_I(fz[1])), iu=1,
-                 T=300.0, yf=1.0, yo2=0.0, yn2=0.0, uvw=(fuel_u, 0.0, 0.0))
+                 T=300.0, yf=fy_f, yo2=fy_o2, yn2=fy_n2, uvw=(fuel_u, 0.0, 0.0))
     air = Inlet(box=(2, cell_I(ay[0]), cell_I(az[0]), 2, cell_I(ay[1]), cell_I(az[1])), iu=1,
                 T=air_t, yf=0.0, yo2=0.233, yn2=0.767, uvw=(u_air, 0.0, 0.0))
     spec = CaseSpec(name=f"furnace {nx}x{ny}x{nz}", faces_x=uniform_faces(Lx, nx), faces_y=uniform_faces(Ly, ny),

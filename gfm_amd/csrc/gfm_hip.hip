@@ -168,13 +168,26 @@ static int dalloc(gfm_ctx *c, T **p, size_t n) {
   return GFM_OK;
 }
 
+// box kernels (box_pt): one x row per row of workgroups
 static dim3 box_grid(const Box &b) { return dim3((b.ni + 63) / 64, b.nj, b.nk); }
 static long long box_blocks(const Box &b) { return (long long)((b.ni + 63) / 64) * b.nj * b.nk; }
+// flattened box kernels (box_ptf): the (i, j) rows of each k-plane along x
+static dim3 box_grid_f(const Box &b) { return dim3((unsigned)(((long long)b.ni * b.nj + 63) / 64), b.nk); }
+static long long box_blocks_f(const Box &b) { return (((long long)b.ni * b.nj + 63) / 64) * b.nk; }
 #define LAUNCH(kern, b, ...)                                                              \
   do {                                                                                    \
     if ((b).ni > 0 && (b).nj > 0 && (b).nk > 0) {                                         \
       if (c->prof.on) prof_begin(c);                                                      \
       hipLaunchKernelGGL(kern, box_grid(b), dim3(64), 0, c->stream, b, ##__VA_ARGS__);   \
+      if (c->prof.on) prof_end(c, #kern, 0.0);                                            \
+    }                                                                                     \
+  } while (0)
+// the same for the flattened box kernels (box_ptf)
+#define LAUNCHF(kern, b, ...)                                                              \
+  do {                                                                                    \
+    if ((b).ni > 0 && (b).nj > 0 && (b).nk > 0) {                                         \
+      if (c->prof.on) prof_begin(c);                                                      \
+      hipLaunchKernelGGL(kern, box_grid_f(b), dim3(64), 0, c->stream, b, ##__VA_ARGS__);   \
       if (c->prof.on) prof_end(c, #kern, 0.0);                                            \
     }                                                                                     \
   } while (0)
@@ -532,11 +545,11 @@ static void adlbl3_sweeps(gfm_ctx *c, double *xt, int nim, int ib, int ie, int j
     // keep their FP64 values.
     Box ext = make_box(ib - 2, ie + 2, jb - 2, je + 2, kb - 2, ke + 2, 2);
     Box box = make_box(ib, ie, jb, je, kb, ke, 2);
-    LAUNCH(k_box_narrow, ext, xt, c->xtf);
+    LAUNCHF(k_box_narrow, ext, xt, c->xtf);
     sweeps<float>(c, c->xtf, reinterpret_cast<float *>(c->xt2), reinterpret_cast<float *>(c->xt3),
                   reinterpret_cast<float *>(c->xt4), reinterpret_cast<float *>(c->xt5), reinterpret_cast<float *>(c->xg), nim,
                   ib, ie, jb, je, kb, ke);
-    LAUNCH(k_box_widen, box, c->xtf, xt);
+    LAUNCHF(k_box_widen, box, c->xtf, xt);
   }
 }
 static void adlbl3(gfm_ctx *c, double *xt, int nelg, int ib, int ie, int jb, int je, int kb, int ke) {
@@ -681,7 +694,7 @@ static int gflow(gfm_ctx *c, int stage) {
   for (int iu1 = 1; iu1 <= 3; iu1++) {
     int nx0 = iu1 == 1, nx1 = iu1 == 2, nx2 = iu1 == 3;
     Box fb = make_box(2 + nx0, MPh - nx0, 2 + nx1, NPh - nx1, 2 + nx2, LPh - nx2, 2);
-    LAUNCH(k_copy_ug_fz, fb, iu1);
+    LAUNCHF(k_copy_ug_fz, fb, iu1);
   }
   LAUNCH(k_momentum_assemble<1>, mom[1]);
   LAUNCH(k_momentum_assemble<2>, mom[2]);
@@ -690,7 +703,7 @@ static int gflow(gfm_ctx *c, int stage) {
   if (k.want_pre_residuals) residual(c, d.p, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_P_PRE, 0);
   adlbl3(c, d.p, 4, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2);
   residual(c, d.p, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_P, 0);
-  LAUNCH(k_bound_p, cen);
+  LAUNCHF(k_bound_p, cen);
   for (int iu1 = 1; iu1 <= 3; iu1++) {
     const Box &b = mom[iu1];
     int ie = b.i0 + 2 * (b.ni - 1), je = b.j0 + 2 * (b.nj - 1), ke = b.k0 + 2 * (b.nk - 1);
@@ -698,12 +711,12 @@ static int gflow(gfm_ctx *c, int stage) {
     if (k.want_pre_residuals) residual(c, d.fz, b.i0, ie, b.j0, je, b.k0, ke, RS_U_PRE + iu1 - 1, 0);
     adlbl3(c, d.fz, iu1, b.i0, ie, b.j0, je, b.k0, ke);
     residual(c, d.fz, b.i0, ie, b.j0, je, b.k0, ke, RS_U + iu1 - 1, 0);
-    LAUNCH(k_bound_store_ug, b, iu1);
+    LAUNCHF(k_bound_store_ug, b, iu1);
   }
   // p'
     LAUNCH(k_pcorr_source, cen, c->part);
     hipLaunchKernelGGL(k_pcorr_finalize, dim3(1), dim3(FIN_NT), 0, c->stream, c->part, box_blocks(cen));
-    LAUNCH(k_zero_fz, cen_all);
+    LAUNCHF(k_zero_fz, cen_all);
     if (k.pcorr_mode == GFM_PCORR_REFERENCE) {
       adlbl3(c, d.fz, 5, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2);
       residual(c, d.fz, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_PC, 0);
@@ -727,42 +740,42 @@ static int gflow(gfm_ctx *c, int stage) {
     // three directions accumulate into one set of partials: run them in
     // order and sum each into the scalar block
     for (int q = 0; q < 3; q++) {
-      LAUNCH(k_extrv, ex[q], q + 1, 0, c->part);
-      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(FIN_NT), 0, c->stream, c->part, (int)box_blocks(ex[q]), 5, SC_GFEX, q > 0);
+      LAUNCHF(k_extrv, ex[q], q + 1, 0, c->part);
+      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(FIN_NT), 0, c->stream, c->part, (int)box_blocks_f(ex[q]), 5, SC_GFEX, q > 0);
     }
   }
   if (k.want_plane_flows) plane_flows(c);
   hipLaunchKernelGGL(k_factor, dim3(1), dim3(1), 0, c->stream);
-  for (int q = 0; q < 3; q++) LAUNCH(k_extrv, ex[q], q + 1, 1, c->part);
+  for (int q = 0; q < 3; q++) LAUNCHF(k_extrv, ex[q], q + 1, 1, c->part);
   // INTPV
   {
     // IU=1
     Box b;
-    b = make_box(3, MPh - 1, 3, NPh - 1, 2, LPh, 2); LAUNCH(k_intpv, b, 1, 2, 1, 1, 0);
-    b = make_box(3, MPh - 1, 2, NPh, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 1, 3, 1, 0, 1);
-    b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 1, 3, 1, 1, 1);
+    b = make_box(3, MPh - 1, 3, NPh - 1, 2, LPh, 2); LAUNCHF(k_intpv, b, 1, 2, 1, 1, 0);
+    b = make_box(3, MPh - 1, 2, NPh, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 1, 3, 1, 0, 1);
+    b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 1, 3, 1, 1, 1);
     for (int pj = 0; pj < 2; pj++)
       for (int pk = 0; pk < 2; pk++) {
         b = make_box(4, MPh, 2 + pj, pj ? NPh - 1 : NPh, 2 + pk, pk ? LPh - 1 : LPh, 2);
-        LAUNCH(k_intpv, b, 1, 1, 0, pj, pk);
+        LAUNCHF(k_intpv, b, 1, 1, 0, pj, pk);
       }
     // IU=2
-    b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 2, 3, 0, 1, 1);
-    b = make_box(3, MPh - 1, 3, NPh - 1, 2, LPh, 2); LAUNCH(k_intpv, b, 2, 1, 1, 1, 0);
-    b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 2, 1, 1, 1, 1);
+    b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 2, 3, 0, 1, 1);
+    b = make_box(3, MPh - 1, 3, NPh - 1, 2, LPh, 2); LAUNCHF(k_intpv, b, 2, 1, 1, 1, 0);
+    b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 2, 1, 1, 1, 1);
     for (int pi = 0; pi < 2; pi++)
       for (int pk = 0; pk < 2; pk++) {
         b = make_box(2 + pi, pi ? MPh - 1 : MPh, 4, NPh, 2 + pk, pk ? LPh - 1 : LPh, 2);
-        LAUNCH(k_intpv, b, 2, 2, pi, 0, pk);
+        LAUNCHF(k_intpv, b, 2, 2, pi, 0, pk);
       }
     // IU=3
-    b = make_box(3, MPh - 1, 2, NPh, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 3, 1, 1, 0, 1);
-    b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 3, 2, 0, 1, 1);
-    b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 3, 2, 1, 1, 1);
+    b = make_box(3, MPh - 1, 2, NPh, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 3, 1, 1, 0, 1);
+    b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 3, 2, 0, 1, 1);
+    b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 3, 2, 1, 1, 1);
     for (int pi = 0; pi < 2; pi++)
       for (int pj = 0; pj < 2; pj++) {
         b = make_box(2 + pi, pi ? MPh - 1 : MPh, 2 + pj, pj ? NPh - 1 : NPh, 4, LPh, 2);
-        LAUNCH(k_intpv, b, 3, 3, pi, pj, 0);
+        LAUNCHF(k_intpv, b, 3, 3, pi, pj, 0);
       }
   }
   // pressure reference
@@ -774,7 +787,7 @@ static int gflow(gfm_ctx *c, int stage) {
   const double sigma[7] = {0.7, 0.7, 0.7, 1.0, 1.0, 0.7, 0.7};
   for (int e = 0; e < 7; e++) {
     int l = Ls[e], nelg = nelgs[e];
-    LAUNCH(k_copy_gf_fz, cen_all, l);
+    LAUNCHF(k_copy_gf_fz, cen_all, l);
     int igs = 1;
     if (e == 0 || c->sgm != sigma[e]) { c->sgm = sigma[e]; igs = 0; }
     switch (nelg) {
@@ -795,11 +808,11 @@ static int gflow(gfm_ctx *c, int stage) {
     if (k.want_pre_residuals) residual(c, d.fz, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_GF_PRE + l - 1, 0);
     adlbl3(c, d.fz, nelg, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2);
     residual(c, d.fz, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_GF + l - 1, 0);
-    LAUNCH(k_bound_store_gf, cen, l);
+    LAUNCHF(k_bound_store_gf, cen, l);
   }
   c->sgm = 1.0;  // GSLV3V resets SGM at the next iteration
   // properties, T_ave, EXTR(1), INTP
-  LAUNCH(k_props, cen_all);
+  LAUNCHF(k_props, cen_all);
   LAUNCH(k_tave, cen, c->part);
   hipLaunchKernelGGL(k_tave_finalize, dim3(1), dim3(FIN_NT), 0, c->stream, c->part, (int)box_blocks(cen));
   extr(c);
@@ -837,21 +850,21 @@ static int initsv_tail(gfm_ctx *c) {
   const int MPh = k.mp, NPh = k.np, LPh = k.lp;
   // INTPV (same launches as in gflow)
   Box b;
-  b = make_box(3, MPh - 1, 3, NPh - 1, 2, LPh, 2); LAUNCH(k_intpv, b, 1, 2, 1, 1, 0);
-  b = make_box(3, MPh - 1, 2, NPh, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 1, 3, 1, 0, 1);
-  b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 1, 3, 1, 1, 1);
+  b = make_box(3, MPh - 1, 3, NPh - 1, 2, LPh, 2); LAUNCHF(k_intpv, b, 1, 2, 1, 1, 0);
+  b = make_box(3, MPh - 1, 2, NPh, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 1, 3, 1, 0, 1);
+  b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 1, 3, 1, 1, 1);
   for (int pj = 0; pj < 2; pj++)
-    for (int pk = 0; pk < 2; pk++) { b = make_box(4, MPh, 2 + pj, pj ? NPh - 1 : NPh, 2 + pk, pk ? LPh - 1 : LPh, 2); LAUNCH(k_intpv, b, 1, 1, 0, pj, pk); }
-  b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 2, 3, 0, 1, 1);
-  b = make_box(3, MPh - 1, 3, NPh - 1, 2, LPh, 2); LAUNCH(k_intpv, b, 2, 1, 1, 1, 0);
-  b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 2, 1, 1, 1, 1);
+    for (int pk = 0; pk < 2; pk++) { b = make_box(4, MPh, 2 + pj, pj ? NPh - 1 : NPh, 2 + pk, pk ? LPh - 1 : LPh, 2); LAUNCHF(k_intpv, b, 1, 1, 0, pj, pk); }
+  b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 2, 3, 0, 1, 1);
+  b = make_box(3, MPh - 1, 3, NPh - 1, 2, LPh, 2); LAUNCHF(k_intpv, b, 2, 1, 1, 1, 0);
+  b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 2, 1, 1, 1, 1);
   for (int pi = 0; pi < 2; pi++)
-    for (int pk = 0; pk < 2; pk++) { b = make_box(2 + pi, pi ? MPh - 1 : MPh, 4, NPh, 2 + pk, pk ? LPh - 1 : LPh, 2); LAUNCH(k_intpv, b, 2, 2, pi, 0, pk); }
-  b = make_box(3, MPh - 1, 2, NPh, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 3, 1, 1, 0, 1);
-  b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 3, 2, 0, 1, 1);
-  b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intpv, b, 3, 2, 1, 1, 1);
+    for (int pk = 0; pk < 2; pk++) { b = make_box(2 + pi, pi ? MPh - 1 : MPh, 4, NPh, 2 + pk, pk ? LPh - 1 : LPh, 2); LAUNCHF(k_intpv, b, 2, 2, pi, 0, pk); }
+  b = make_box(3, MPh - 1, 2, NPh, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 3, 1, 1, 0, 1);
+  b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 3, 2, 0, 1, 1);
+  b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 3, 2, 1, 1, 1);
   for (int pi = 0; pi < 2; pi++)
-    for (int pj = 0; pj < 2; pj++) { b = make_box(2 + pi, pi ? MPh - 1 : MPh, 2 + pj, pj ? NPh - 1 : NPh, 4, LPh, 2); LAUNCH(k_intpv, b, 3, 3, pi, pj, 0); }
+    for (int pj = 0; pj < 2; pj++) { b = make_box(2 + pi, pi ? MPh - 1 : MPh, 2 + pj, pj ? NPh - 1 : NPh, 4, LPh, 2); LAUNCHF(k_intpv, b, 3, 3, pi, pj, 0); }
   extr(c);
   b = make_box(2, MPh, 2, NPh, 3, LPh - 1, 2); LAUNCH(k_intp<3>, b);
   b = make_box(2, MPh, 3, NPh - 1, 2, LPh, 2); LAUNCH(k_intp<2>, b);

@@ -76,12 +76,29 @@ __device__ __forceinline__ void SAS(int s, long long q, double v) { if (g.mixed)
 
 enum { IYF = 1, IYO2 = 2, IH = 3, IK = 4, IEPS = 5, IYCO2 = 6, IYN2 = 7, IYH2O = 9 };
 
+// Position of the thread in a box launched on box_grid: one x row of the
+// box per row of workgroups, blockIdx.y and blockIdx.z being j and k
+// (uniform over the wavefront: the stencil kernels form their indices in
+// scalar registers).
 __device__ __forceinline__ bool box_pt(const Box &b, int &i, int &j, int &k) {
   int tx = blockIdx.x * blockDim.x + threadIdx.x;
   if (tx >= b.ni) return false;
   i = b.i0 + b.step * tx;
   j = b.j0 + b.step * (int)blockIdx.y;
   k = b.k0 + b.step * (int)blockIdx.z;
+  return true;
+}
+// The same on box_grid_f, the (i, j) rows of a k-plane flattened along x,
+// so that only the last wavefront of a plane has idle lanes: for the light
+// kernels over boxes of 193-195 positions per row, whose last wavefront per
+// row would otherwise hold 1-3 live lanes; blockIdx.y is k.
+__device__ __forceinline__ bool box_ptf(const Box &b, int &i, int &j, int &k) {
+  const int tx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tx >= b.ni * b.nj) return false;
+  const int jj = tx / b.ni;
+  i = b.i0 + b.step * (tx - jj * b.ni);
+  j = b.j0 + b.step * jj;
+  k = b.k0 + b.step * (int)blockIdx.y;
   return true;
 }
 
@@ -433,7 +450,7 @@ __device__ void ECELL(int i, int j, int k, int iu, double ffl[3][2]) {  // C/Gsl
 // FZ = UG on a face class, C/Gslv3v.f90:26-34
 __global__ void k_copy_ug_fz(Box b, int iu1) {
   int i, j, k;
-  if (!box_pt(b, i, j, k)) return;
+  if (!box_ptf(b, i, j, k)) return;
   long long q = lidx(i, j, k);
   g.fz[q] = g.ug[iu1 - 1][q];
 }
@@ -584,7 +601,7 @@ __global__ void k_pressure_assemble(Box b) {
 }
 __global__ void k_bound_p(Box b) {
   int i, j, k;
-  if (!box_pt(b, i, j, k)) return;
+  if (!box_ptf(b, i, j, k)) return;
   if (IBC(i, j, k) >= 1) return;
   long long q = lidx(i, j, k);
   g.p[q] = BOUND_P(g.p[q]);
@@ -605,7 +622,7 @@ __global__ void k_starred(Box b, int iu1) {
 }
 __global__ void k_bound_store_ug(Box b, int iu1) {
   int i, j, k;
-  if (!box_pt(b, i, j, k)) return;
+  if (!box_ptf(b, i, j, k)) return;
   if (IBC(i, j, k) >= 1) return;
   long long q = lidx(i, j, k);
   double f = BOUND_U(g.fz[q]);
@@ -692,7 +709,7 @@ __global__ __launch_bounds__(FIN_NT) void k_pcorr_finalize(const double *part, l
 }
 __global__ void k_zero_fz(Box b) {
   int i, j, k;
-  if (!box_pt(b, i, j, k)) return;
+  if (!box_ptf(b, i, j, k)) return;
   FZ(i, j, k) = 0.0;
 }
 // velocity correction, C/Gslv3v.f90:179-204
@@ -717,7 +734,7 @@ __global__ void k_vel_correct(Box b, int iu1) {
 __global__ void k_extrv(Box b, int iu, int mex, double *part) {
   int i, j, k;
   double s[5] = {0, 0, 0, 0, 0};
-  if (box_pt(b, i, j, k) && (IBC(i, j, k) % 10) == 3) {
+  if (box_ptf(b, i, j, k) && (IBC(i, j, k) % 10) == 3) {
     int done = 0;
     for (int m = -1; m <= 1 && !done; m += 2) {
       int ia = i, ja = j, ka = k;
@@ -820,7 +837,7 @@ __device__ __forceinline__ double INTP1(int i, int j, int k, int ix) {
 }
 __global__ void k_intpv(Box b, int iu, int ix, int pi, int pj, int pk) {
   int i, j, k;
-  if (!box_pt(b, i, j, k)) return;
+  if (!box_ptf(b, i, j, k)) return;
   // box covers a class; parities fixed per launch
   (void)pi; (void)pj; (void)pk;
   int bc = IBC(i, j, k);
@@ -934,7 +951,7 @@ __device__ double WALL_SINK(int i, int j, int k, double &qw, double &qlo, double
 // ======================================================================
 __global__ void k_copy_gf_fz(Box b, int l) {
   int i, j, k;
-  if (!box_pt(b, i, j, k)) return;
+  if (!box_ptf(b, i, j, k)) return;
   long long q = lidx(i, j, k);
   g.fz[q] = g.gf[l - 1][q];
 }
@@ -1086,7 +1103,7 @@ __global__ void k_scalar_assemble(Box b, int l, int igs, double sgm, double *par
 }
 __global__ void k_bound_store_gf(Box b, int l) {
   int i, j, k;
-  if (!box_pt(b, i, j, k)) return;
+  if (!box_ptf(b, i, j, k)) return;
   if (IBC(i, j, k) >= 1) return;
   long long q = lidx(i, j, k);
   double f = BOUND_GF(g.fz[q], l);
@@ -1095,9 +1112,9 @@ __global__ void k_bound_store_gf(Box b, int l) {
 }
 
 // property update of open cells, C/Gflow.f90:302-382
-__global__ void k_props(Box b) {
+__global__ __launch_bounds__(64) void k_props(Box b) {
   int i, j, k;
-  if (!box_pt(b, i, j, k)) return;
+  if (!box_ptf(b, i, j, k)) return;
   if (IBC(i, j, k) >= 1) return;
   long long q = lidx(i, j, k);
   CONC(q);
@@ -1764,13 +1781,13 @@ __global__ void k_pair_xbound(const T *xt, T *x2, T *x3, T *x4, T *x5, int ib, i
 // values.
 __global__ void k_box_narrow(Box b, const double *src, float *dst) {
   int i, j, k;
-  if (!box_pt(b, i, j, k)) return;
+  if (!box_ptf(b, i, j, k)) return;
   long long q = lidx(i, j, k);
   dst[q] = (float)src[q];
 }
 __global__ void k_box_widen(Box b, const float *src, double *dst) {
   int i, j, k;
-  if (!box_pt(b, i, j, k)) return;
+  if (!box_ptf(b, i, j, k)) return;
   long long q = lidx(i, j, k);
   if (g.ib[q] >= 1 || fabs((double)g.apf[q]) <= SMALL20) return;
   dst[q] = (double)src[q];

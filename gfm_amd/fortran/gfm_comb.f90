@@ -7,6 +7,12 @@
 ! with a sixth the convg log of every iteration (convg####c.plt, header of
 ! C/Utility.f90:1066-1080, record of C/Sphase.f90:53).
 !   usage: gfm_comb <state-in> <state-out> <iterations> [sweep_order [rg-out [convg-out]]]
+! The staged form runs one ADLBL3 on a system read from a file, called as
+! the reference calls it (C/Gslv3v.f90:91-94: calc_residual, ADLBL3,
+! calc_residual, implicit interfaces, NELG from module state):
+!   gfm_comb adlbl3 <state-in> <system-in> <out>
+! system-in (stream): int32 NELG, int32 IB,IE,JB,JE,KB,KE, then AP, AS, BS
+! and XT in the reference layout; out: the two residuals, then XT.
 program gfm_comb
   use iso_c_binding
   use gfm_types
@@ -15,6 +21,11 @@ program gfm_comb
   implicit none
   character(len=512) :: fin, fout, arg
   integer :: niter, order, i
+  call get_command_argument(1, arg)
+  if (trim(arg) == 'adlbl3') then
+    call staged_adlbl3()
+    stop
+  end if
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
   call get_command_argument(3, arg)
@@ -49,6 +60,32 @@ program gfm_comb
   end if
   call gpu_close()
 contains
+  subroutine staged_adlbl3()
+    character(len=512) :: fstate, fsys, fo
+    integer(c_int32_t) :: hdr(7)
+    integer :: u, ib, ie, jb, je, kb, ke
+    real(c_double) :: r0, r1
+    real(c_double), allocatable :: xt(:, :, :)
+    call get_command_argument(2, fstate)
+    call get_command_argument(3, fsys)
+    call get_command_argument(4, fo)
+    call read_state(trim(fstate))
+    allocate (AP(MP, NP, LP), AS(MP, NP, LP, 3, 2), BS(MP, NP, LP), xt(MP, NP, LP))
+    open (newunit=u, file=trim(fsys), access='stream', form='unformatted', status='old', action='read')
+    read (u) hdr
+    read (u) AP, AS, BS, xt
+    close (u)
+    NELG = hdr(1)
+    ib = hdr(2); ie = hdr(3); jb = hdr(4); je = hdr(5); kb = hdr(6); ke = hdr(7)
+    call calc_residual(xt, r0, ib, ie, jb, je, kb, ke)
+    call ADLBL3(xt, ib, ie, jb, je, kb, ke)
+    call calc_residual(xt, r1, ib, ie, jb, je, kb, ke)
+    open (newunit=u, file=trim(fo), access='stream', form='unformatted', status='replace', action='write')
+    write (u) r0, r1, xt
+    close (u)
+    call gpu_close()
+  end subroutine
+
   subroutine read_state(path)
     character(len=*), intent(in) :: path
     character(len=8) :: magic

@@ -3,13 +3,16 @@
 ! Module gfm_state holds the subset of the reference's MODULE GBL_VAR
 ! (C/Var_mod.f90:3-889) that the combustion-space loop reads and writes,
 ! in the reference layout (1-based, column-major, doubled lattice).
-! Module gfm_solver keeps the reference's subroutine interface:
-!   SPHASE                      C/Sphase.f90:10   (called C/A0main.f90:204)
-!   GFLOW                       C/Gflow.f90:6     (called C/Sphase.f90:51)
-!   ADLBL3(XT,IB,IE,JB,JE,KB,KE) C/Adlbl3.f90:10
-! and runs them on the GPU through ISO_C_BINDING (gfm_hip_mod).  A non-zero
-! return from the library ends the run through stop_run, as the reference's
-! fatal paths do (C/A0main.f90:415-424).
+! Module gfm_solver runs the loop on the GPU through ISO_C_BINDING
+! (gfm_hip_mod); gfm_external.f90 puts the reference's subroutine interface
+! on it as external subroutines, so the reference's implicit-interface call
+! sites link unchanged:
+!   SPHASE                               C/Sphase.f90:10   (called C/A0main.f90:204)
+!   GFLOW                                C/Gflow.f90:6     (called C/Sphase.f90:51)
+!   ADLBL3(XT,IB,IE,JB,JE,KB,KE)         C/Adlbl3.f90:10   (called C/Gslv3v.f90:93,133,163, C/Gslv3g.f90:96)
+!   calc_residual(xt,resid,ib,ie,jb,je,kb,ke) C/Adlbl3.f90:239 (called C/Gslv3v.f90:91,94,132,135,164)
+! A non-zero return from the library ends the run through stop_run, as the
+! reference's fatal paths do (C/A0main.f90:415-424).
 module gfm_state
   use iso_c_binding
   use gfm_types
@@ -22,8 +25,10 @@ module gfm_state
   real(c_double), allocatable, target :: CRATE(:, :, :), HRATE(:, :, :), BS_HPREV(:, :, :)
   real(c_double), allocatable, target :: UG(:, :, :, :), GF(:, :, :, :)
   real(c_double), allocatable, target :: RTIM(:), DELRT(:), HRT(:), DELR(:), TIMED(:), HRD(:)
-  ! ADLBL3's module inputs (C/Adlbl3.f90 reads AP, AS, BS from GBL_VAR)
+  ! ADLBL3's module inputs (C/Adlbl3.f90 reads AP, AS, BS and the equation
+  ! index NELG from GBL_VAR, C/Var_mod.f90:567)
   real(c_double), allocatable, target :: AP(:, :, :), AS(:, :, :, :, :), BS(:, :, :)
+  integer :: NELG = 0
   type(gfm_ctrl), target :: ctrl
   type(gfm_host_arrays), target :: harr
   character(len=256) :: casedir = '.'
@@ -147,10 +152,10 @@ contains
     gpu_ctx = c_null_ptr
   end subroutine
 
-  ! SPHASE: the combustion-space loop (C/Sphase.f90:37-235) on the device,
+  ! the combustion-space loop (C/Sphase.f90:37-235) on the device,
   ! returning to the host every poll_every iterations to honour runstop.dat
   ! (C/Sphase.f90:125-131).  The module arrays are current on return.
-  subroutine SPHASE
+  subroutine gpu_sphase
     integer(c_int) :: n_done
     real(c_double), allocatable :: chunk(:)
     logical :: stop_req
@@ -173,17 +178,17 @@ contains
     call gpu_sync_to_host()
   end subroutine
 
-  ! GFLOW: one SIMPLER iteration (C/Gflow.f90:6-408) at itr_gas
-  subroutine GFLOW
+  ! one SIMPLER iteration (C/Gflow.f90:6-408) at itr_gas
+  subroutine gpu_gflow
     if (.not. c_associated(gpu_ctx)) call gpu_open(0)
     call check(gfm_hip_gflow(gpu_ctx, ctrl), 'gfm_hip_gflow')
   end subroutine
 
-  ! ADLBL3 (C/Adlbl3.f90:10): line relaxation of XT with the module's AP,
-  ! AS and BS and NTIMESG(NELG) sweeps; RESG(NELG,1:2) back in ctrl%resg.
-  subroutine ADLBL3(XT, IB, IE, JB, JE, KB, KE, NELG)
+  ! line relaxation of XT (C/Adlbl3.f90:10) with the module's AP, AS, BS
+  ! and NTIMESG(NELG) sweeps; RESG(NELG,1:2) back in ctrl%resg
+  subroutine gpu_adlbl3(XT, IB, IE, JB, JE, KB, KE)
     real(c_double), intent(inout) :: XT(MP, NP, LP)
-    integer, intent(in) :: IB, IE, JB, JE, KB, KE, NELG
+    integer, intent(in) :: IB, IE, JB, JE, KB, KE
     real(c_double) :: resg(2)
     if (.not. c_associated(gpu_ctx)) call gpu_open(0)
     call check(gfm_hip_adlbl3(gpu_ctx, ctrl, int(NELG, c_int), AP, AS, BS, XT, int(IB, c_int), int(IE, c_int), &
@@ -192,5 +197,15 @@ contains
       ctrl%resg(NELG) = resg(1)
       ctrl%resg(20 + NELG) = resg(2)
     end if
+  end subroutine
+
+  ! calc_residual (C/Adlbl3.f90:239-271) of XT with the module's AP, AS, BS
+  subroutine gpu_calc_residual(XT, RESID, IB, IE, JB, JE, KB, KE)
+    real(c_double), intent(in) :: XT(MP, NP, LP)
+    real(c_double), intent(out) :: RESID
+    integer, intent(in) :: IB, IE, JB, JE, KB, KE
+    if (.not. c_associated(gpu_ctx)) call gpu_open(0)
+    call check(gfm_hip_calc_residual(gpu_ctx, AP, AS, BS, XT, int(IB, c_int), int(IE, c_int), int(JB, c_int), &
+                                     int(JE, c_int), int(KB, c_int), int(KE, c_int), RESID), 'gfm_hip_calc_residual')
   end subroutine
 end module gfm_solver
