@@ -27,6 +27,9 @@ term, a wrong sign, index or factor) fails.
    104-266,621-654)                  W_CO2/W_F, O2 = -2 W_O2/W_F crate,
                                      N2 = 0, and the element balance of
                                      CH4 + 2 O2 -> CO2 + 2 H2O closes.
+  enthalpy-source history           with a known history field the relaxed
+  (C/Source.f90:511-520)             source is rf_h raw + (1 - rf_h) history,
+                                     stored as the next history.
   CONC, DNST_CALC, THER              the ideal-gas law in SI units, mass
   (C/Prop.f90:28-59,130-198)         fractions summing to one, mu_t =
                                      C_mu rho k^2/eps in SI over the
@@ -485,3 +488,34 @@ def test_conc_dnst_ther_closed_forms():
     mu_ref = 2.0 * c.ctrl.dnst0 * c.ctrl.ug0 * c.ctrl.r0 / c.ctrl.reyg
     assert A["tmu"][i, j, k] == pytest.approx(0.09 * rho_si * k_si ** 2 / e_si / mu_ref, rel=1e-12)
     assert A["gama"][i, j, k] == pytest.approx(2.0 / c.ctrl.reyg * (A["tmu"][i, j, k] + 1.0), rel=1e-15)
+
+
+def test_enthalpy_source_history_relaxation():
+    """From the second iteration on the enthalpy source is relaxed with its
+    history, b = rf_h b_raw + (1 - rf_h) b_prev, and stored as the next
+    history (C/Source.f90:511-520): with the history set to a known field,
+    the relaxed source is that blend of the raw one (the first iteration's,
+    which is not relaxed) and the field."""
+    c = _uniform()
+    oracle.initsv_tail(c)
+    oracle.sphase(c, 3)
+    sl = _cell_slices(c)
+    opn = c.arrays["ibcell"][sl] <= 0
+    c.ctrl.itr_gas = 1
+    oracle.probe(c, "source", 8)
+    raw = oracle.scratch(c)["bs"][sl].copy()
+    assert np.abs(raw[opn]).max() > 0.0
+    np.testing.assert_array_equal(c.arrays["bs_hprev"][sl][opn], raw[opn])
+    rng = np.random.default_rng(11)
+    hist = rng.uniform(-1.0, 1.0, raw.shape) * np.abs(raw[opn]).max()
+    view = c.arrays["bs_hprev"][sl]
+    view[...] = hist
+    c.arrays["bs_hprev"][sl] = view
+    c.ctrl.itr_gas = 5
+    oracle.probe(c, "source", 8)
+    got = oracle.scratch(c)["bs"][sl]
+    rf = c.ctrl.rf_hsource
+    assert rf == pytest.approx(0.3)
+    np.testing.assert_allclose(got[opn], rf * raw[opn] + (1.0 - rf) * hist[opn], rtol=1e-13,
+                               atol=1e-13 * np.abs(raw[opn]).max())
+    np.testing.assert_array_equal(c.arrays["bs_hprev"][sl][opn], got[opn])
