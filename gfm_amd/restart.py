@@ -262,3 +262,129 @@ def load_rg(case, path, **kw):
     """read_restart_data: apply rg####c.d at path to the case's host state."""
     mp, np_, lp = case.shape
     return apply_records(case, read_records(path, mp, np_, lp), **kw)
+
+
+# ----------------------------------------------------------------------
+# Minor-species restart record rs####c.d
+# ----------------------------------------------------------------------
+# Written by sav3fm (C/Sav3f.f90:96-116), read by INITSVM
+# (C/Initsv.f90:329-380): per cell centre, I = 2..MP, J, K in steps of 2
+# (K innermost), one record GFM(I,J,K,1:NSP0), SMF(I,J,K); then the whole
+# lattice arrays bs_soot_prev and sfp_soot_prev, RRMS(MP/2, NP/2, LP/2, NRM0)
+# (the minor-species reaction rates, C/Setup.f90:911) and itr_ms.
+NSP0 = 7  # minor species, C/Var_mod.f90:218
+NRM0 = 5  # minor-species reactions of RRMS
+
+
+@dataclass
+class MinorSpeciesRecords:
+    """The contents of one rs####c.d file, in record order."""
+    mp: int
+    np: int
+    lp: int
+    gfm: np.ndarray             # (cells, nsp0): GFM(I,J,K,1:NSP0) per cell centre, K innermost
+    smf: np.ndarray             # (cells,): SMF, the soot mass fraction
+    bs_soot_prev: np.ndarray    # (mp, np, lp), Fortran order
+    sfp_soot_prev: np.ndarray   # (mp, np, lp), Fortran order
+    rrms: np.ndarray            # (mp//2, np//2, lp//2, nrm0), Fortran order
+    itr_ms: int = 0
+
+    @property
+    def nsp0(self):
+        return self.gfm.shape[1]
+
+
+def read_rs(src, mp, np_, lp, nsp0=NSP0, nrm0=NRM0):
+    """Parse rs####c.d (a path or the file's bytes) for a lattice of
+    (mp, np_, lp); raises ValueError on any record that does not match."""
+    buf = src if isinstance(src, (bytes, bytearray, memoryview)) else open(src, "rb").read()
+    buf = bytes(buf)
+    ncell = (mp // 2) * (np_ // 2) * (lp // 2)
+    n = 8 * (nsp0 + 1)
+    dt = np.dtype([("m0", "<i4"), ("v", "<f8", (nsp0 + 1,)), ("m1", "<i4")])
+    if len(buf) < ncell * dt.itemsize:
+        raise ValueError(f"file of {len(buf)} bytes is shorter than the {ncell} cell records")
+    cells = np.frombuffer(buf, dtype=dt, count=ncell)
+    for m in ("m0", "m1"):
+        bad = np.nonzero(cells[m] != n)[0]
+        if len(bad):
+            raise ValueError(f"cell record {bad[0]}: marker {cells[m][bad[0]]}, expected {n}")
+    r = _Reader(buf)
+    r.off = ncell * dt.itemsize
+    nlat = mp * np_ * lp
+    bsp = r.doubles(nlat).reshape((mp, np_, lp), order="F")
+    sfp = r.doubles(nlat).reshape((mp, np_, lp), order="F")
+    rr = r.doubles(ncell * nrm0).reshape((mp // 2, np_ // 2, lp // 2, nrm0), order="F")
+    (itr_ms,) = r.ints(1)
+    if r.off != len(buf):
+        raise ValueError(f"{len(buf) - r.off} bytes after the last record")
+    v = cells["v"]
+    return MinorSpeciesRecords(mp, np_, lp, v[:, :nsp0].copy(), v[:, nsp0].copy(), bsp, sfp, rr, itr_ms)
+
+
+def rs_bytes(rec):
+    """The rs####c.d file of a MinorSpeciesRecords, as sav3fm writes it."""
+    n = 8 * (rec.nsp0 + 1)
+    dt = np.dtype([("m0", "<i4"), ("v", "<f8", (rec.nsp0 + 1,)), ("m1", "<i4")])
+    cells = np.zeros(len(rec.smf), dtype=dt)
+    cells["m0"] = n
+    cells["m1"] = n
+    cells["v"][:, :rec.nsp0] = rec.gfm
+    cells["v"][:, rec.nsp0] = rec.smf
+    out = [cells.tobytes()]
+    for a in (rec.bs_soot_prev, rec.sfp_soot_prev, rec.rrms):
+        p = np.asarray(a, dtype="<f8").tobytes(order="F")
+        out.append(struct.pack("<i", len(p)) + p + struct.pack("<i", len(p)))
+    p = struct.pack("<i", int(rec.itr_ms))
+    out.append(struct.pack("<i", 4) + p + struct.pack("<i", 4))
+    return b"".join(out)
+
+
+def write_rs(path, rec):
+    with open(path, "wb") as f:
+        f.write(rs_bytes(rec))
+
+
+def initsvm(case, rec=None):
+    """The minor-species fields INITSVM sets up (C/Initsv.f90:329-380):
+    returns (gfm, smf) on the lattice, gfm of shape (MP, NP, LP, NSP0).
+    Fresh start (rec None): every cell centre takes fuel, O2, H2O, CO2 and
+    N2 from the major species into slots 1, 2, 4, 5, 6, the rest zero, and
+    smf = 0.  Restart: the file's values, except at inlet cells, where the
+    five come from the major species again and slots 3 and 7 are zeroed."""
+    from .case import IYF, IYO2, IYH2O, IYCO2, IYN2
+    mp, np_, lp = case.shape
+    nsp0 = NSP0 if rec is None else rec.nsp0
+    gfm = np.zeros((mp, np_, lp, nsp0), order="F")
+    smf = np.zeros((mp, np_, lp), order="F")
+    GF = case.arrays["gf"]
+    ib = case.arrays["ibcell"]
+    cen = (slice(1, None, 2),) * 3
+    take = ((0, IYF), (1, IYO2), (3, IYH2O), (4, IYCO2), (5, IYN2))
+    if rec is None:
+        for s, l in take:
+            gfm[cen + (s,)] = GF[cen + (l - 1,)]
+        return gfm, smf
+    if (rec.mp, rec.np, rec.lp) != (mp, np_, lp):
+        raise ValueError("restart record and case lattices differ")
+    I, J, K, _, _, _ = _cell_index(mp, np_, lp)
+    gfm[I, J, K, :] = rec.gfm
+    smf[I, J, K] = rec.smf
+    inl = ib[I, J, K] == 2
+    for s, l in take:
+        gfm[I[inl], J[inl], K[inl], s] = GF[I[inl], J[inl], K[inl], l - 1]
+    gfm[I[inl], J[inl], K[inl], 2] = 0.0
+    gfm[I[inl], J[inl], K[inl], 6] = 0.0
+    return gfm, smf
+
+
+def rs_records_from(gfm, smf, bs_soot_prev=None, sfp_soot_prev=None, rrms=None, itr_ms=0):
+    """MinorSpeciesRecords of lattice fields (what sav3fm writes)."""
+    mp, np_, lp, nsp0 = gfm.shape
+    I, J, K, _, _, _ = _cell_index(mp, np_, lp)
+    z = lambda: np.zeros((mp, np_, lp), order="F")
+    return MinorSpeciesRecords(mp, np_, lp, gfm[I, J, K, :].copy(), smf[I, J, K].copy(),
+                               z() if bs_soot_prev is None else bs_soot_prev,
+                               z() if sfp_soot_prev is None else sfp_soot_prev,
+                               np.zeros((mp // 2, np_ // 2, lp // 2, NRM0), order="F") if rrms is None else rrms,
+                               itr_ms)

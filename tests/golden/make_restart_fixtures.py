@@ -4,7 +4,8 @@ where the reference is mounted; the outputs are committed):
 
 * rg0041c.d.gz -- combustion/case0041/rg0041c.d, the restart record of the
   TC21 case (lattice 40x48x36 from gd0041c.dat), gzip-compressed, with its
-  SHA-256 in restart_fixtures.json;
+  SHA-256 in restart_fixtures.json; rs0041c.d.gz the same run's
+  minor-species record;
 * plt/<name>.plt -- the first lines of the case's convergence logs
   (convg, gresid, gresid_xtra, Tave of case0041), CRLF line ends kept;
 * grid/gd0041c.dat, grid/gd1206c.dat and the sbc files beside them -- two
@@ -32,6 +33,11 @@ def main(ref=REF):
     mp, np_, lp = (int(v) for v in hdr[2].split()[1:4])
     meta = {"rg0041c.d": {"mp": mp, "np": np_, "lp": lp, "lend": 7, "bytes": len(raw),
                           "sha256": hashlib.sha256(raw).hexdigest()}}
+    raw_rs = open(os.path.join(case, "rs0041c.d"), "rb").read()
+    with gzip.GzipFile(os.path.join(HERE, "rs0041c.d.gz"), "wb", compresslevel=9, mtime=0) as f:
+        f.write(raw_rs)
+    meta["rs0041c.d"] = {"mp": mp, "np": np_, "lp": lp, "bytes": len(raw_rs),
+                         "sha256": hashlib.sha256(raw_rs).hexdigest()}
     os.makedirs(os.path.join(HERE, "plt"), exist_ok=True)
     for name in ("convg0041c", "gresid0041c", "gresid_xtra0041c", "Tave0041c"):
         lines = open(os.path.join(case, name + ".plt"), "rb").read().split(b"\n")[:PLT_LINES]

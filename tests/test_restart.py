@@ -179,3 +179,103 @@ def test_melt_surface_temperature_is_kept_unless_in_run(tmp_path):
     assert np.all(b.arrays["t"][surf] == 0.123)
     rs.load_rg(b, path, in_run=1)
     np.testing.assert_array_equal(b.arrays["t"][surf], a.arrays["t"][surf])
+
+
+# ---------------- minor-species record rs####c.d ----------------
+def _case_dims(d):
+    return _gd_dims(glob.glob(os.path.join(d, "gd*c.dat"))[0])
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="reference data files not mounted")
+def test_every_reference_minor_species_file_round_trips():
+    """Every rs####c.d the reference ships (17 files, two lattices), parsed
+    with the lattice of its gd file and written back: the same bytes."""
+    n = 0
+    for f in sorted(glob.glob(os.path.join(REF, "case*", "rs*c.d"))):
+        raw = open(f, "rb").read()
+        rec = rs.read_rs(raw, *_case_dims(os.path.dirname(f)))
+        assert rec.gfm.shape[1] == rs.NSP0 and rec.rrms.shape[-1] == rs.NRM0
+        assert rs.rs_bytes(rec) == raw, f
+        n += 1
+    assert n == 17
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="reference data files not mounted")
+@pytest.mark.parametrize("case", ["case0041", "case1207"])
+def test_minor_species_slots_are_the_major_species(case):
+    """GFM slots 1, 2, 5, 6 hold fuel, O2, CO2 and N2 (C/Initsv.f90:342-346):
+    in the same run's rg and rs files they are equal on the same cells (the
+    cells the minor-species solve leaves alone), and the pure-fuel cells of
+    rg carry fuel 1 in rs."""
+    d = os.path.join(REF, case)
+    dims = _case_dims(d)
+    g = rs.read_records(glob.glob(os.path.join(d, "rg*c.d"))[0], *dims)
+    m = rs.read_rs(glob.glob(os.path.join(d, "rs*c.d"))[0], *dims)
+    same = [m.gfm[:, slot] == g.gf[:, l - 1] for slot, l in ((0, 1), (1, 2), (4, 6), (5, 7))]
+    assert np.count_nonzero(same[0]) > 1000
+    for s in same[1:]:
+        np.testing.assert_array_equal(s, same[0])
+    pure = g.gf[:, 0] == 1.0
+    assert np.count_nonzero(pure) > 0 and np.all(m.gfm[pure, 0] == 1.0)
+
+
+def test_minor_species_record_rejects_corruption():
+    c = cs.furnace_small(nx=6, ny=4, nz=4, seed=1)
+    gfm, smf = rs.initsvm(c)
+    raw = rs.rs_bytes(rs.rs_records_from(gfm, smf, itr_ms=7))
+    rec = rs.read_rs(raw, *c.shape)
+    assert rec.itr_ms == 7 and rs.rs_bytes(rec) == raw
+    bad = bytearray(raw)
+    bad[0] = 65
+    with pytest.raises(ValueError):
+        rs.read_rs(bytes(bad), *c.shape)
+    with pytest.raises(ValueError):
+        rs.read_rs(raw[:-3], *c.shape)
+    with pytest.raises(ValueError):
+        rs.read_rs(raw, c.shape[0] + 2, c.shape[1], c.shape[2])
+
+
+def test_initsvm_fresh_start_and_restart_rules():
+    """INITSVM (C/Initsv.f90:329-380): a fresh start copies fuel, O2, H2O,
+    CO2, N2 into slots 1, 2, 4, 5, 6 at every cell centre; a restart takes
+    the file's values except at inlet cells, which take the major species
+    again with slots 3 and 7 zeroed."""
+    c = cs.furnace_small(nx=8, ny=6, nz=6, seed=2)
+    gfm, smf = rs.initsvm(c)
+    cen = (slice(1, None, 2),) * 3
+    GF = c.arrays["gf"]
+    for s, l in ((0, cs.IYF), (1, cs.IYO2), (3, cs.IYH2O), (4, cs.IYCO2), (5, cs.IYN2)):
+        np.testing.assert_array_equal(gfm[cen + (s,)], GF[cen + (l - 1,)])
+    assert np.all(gfm[cen + (2,)] == 0.0) and np.all(gfm[cen + (6,)] == 0.0) and np.all(smf == 0.0)
+    rng = np.random.default_rng(3)
+    filed = rng.uniform(0.0, 1.0, gfm.shape)
+    soot = rng.uniform(0.0, 1e-3, smf.shape)
+    rec = rs.read_rs(rs.rs_bytes(rs.rs_records_from(filed, soot, itr_ms=12)), *c.shape)
+    g2, s2 = rs.initsvm(c, rec)
+    ib = c.arrays["ibcell"]
+    inl = np.zeros(ib.shape, dtype=bool)
+    inl[cen] = ib[cen] == 2
+    opn = np.zeros(ib.shape, dtype=bool)
+    opn[cen] = ib[cen] != 2
+    assert inl.any()
+    np.testing.assert_array_equal(g2[opn], filed[opn])
+    np.testing.assert_array_equal(s2[cen], soot[cen])
+    np.testing.assert_array_equal(g2[inl][:, 0], GF[..., cs.IYF - 1][inl])
+    np.testing.assert_array_equal(g2[inl][:, 3], GF[..., cs.IYH2O - 1][inl])
+    assert np.all(g2[inl][:, 2] == 0.0) and np.all(g2[inl][:, 6] == 0.0)
+
+
+def test_committed_minor_species_record_round_trips():
+    """case0041's rs0041c.d (committed compressed, with its SHA-256): parsed
+    and written back byte for byte, and its fuel slot equal to rg0041c.d's
+    fuel on the cells the minor-species solve leaves alone."""
+    meta = json.load(open(os.path.join(GOLD, "restart_fixtures.json")))
+    raw = gzip.open(os.path.join(GOLD, "rs0041c.d.gz")).read()
+    assert hashlib.sha256(raw).hexdigest() == meta["rs0041c.d"]["sha256"]
+    m = meta["rs0041c.d"]
+    rec = rs.read_rs(raw, m["mp"], m["np"], m["lp"])
+    assert rs.rs_bytes(rec) == raw
+    assert rec.itr_ms == 38700 and rec.rrms.shape == (20, 24, 18, 5)
+    g_meta, g_raw = _rg0041()
+    g = rs.read_records(g_raw, g_meta["mp"], g_meta["np"], g_meta["lp"], g_meta["lend"])
+    assert np.count_nonzero(rec.gfm[:, 0] == g.gf[:, 0]) == 2895
