@@ -44,6 +44,7 @@ class GfmCtrl(ct.Structure):
         ("pcorr_tol", D), ("pcorr_resid_start", D), ("pcorr_alpha", D), ("pcorr_gain", D),
         ("pcorr_guess_accept", D),
         ("q_wall_bottom", D), ("q_wall_top", D),
+        ("bv_f", D * 2), ("ba_f", D * 6),
     ]
 
     def copy(self):

@@ -120,6 +120,13 @@ class CaseSpec:
     gray_kappa: float = 0.0     # gray absorption coefficient (1/m) of the radiation source term; 0 -> off
     bgcon: float = 1.0e-5
     maxgi: int = 2000
+    # half-blocked cells (grid-file digits 9 and 8, C/Grid3.f90:243-256):
+    # list of ((c0, c1), (d0, d1), (e0, e1), code) with 0-based inclusive
+    # cell ranges and code -1 or -2; the &GRD factors BV_F(1:2) and
+    # BA_F(1:3,1:2) (column-major) of their volumes and face areas
+    half_blocked: list = field(default_factory=list)
+    bv_f: tuple = (1.0, 1.0)
+    ba_f: tuple = (1.0,) * 6
     meta: dict = field(default_factory=dict)
 
 
@@ -268,6 +275,11 @@ def build(spec: CaseSpec) -> Case:
         cen[-1, jy[0]:jy[-1] + 1, kz[0]:kz[-1] + 1] = 3
     if spec.melt_bottom_T is not None:
         cen[1:-1, 1:-1, 0] = 4
+    for (c0, c1), (d0, d1), (e0, e1), code in spec.half_blocked:
+        if code not in (-1, -2):
+            raise ValueError(code)
+        sub = cen[1 + c0:2 + c1, 1 + d0:2 + d1, 1 + e0:2 + e1]
+        sub[sub == 0] = code
     for inl in spec.inlets:
         i1, j1, k1, i2, j2, k2 = inl.box
         for I in range(i1, i2 + 1, 2):
@@ -353,6 +365,10 @@ def build(spec: CaseSpec) -> Case:
     c.rf_hsource = RF_HSOURCE
     for q in range(20):
         c.ntimesg[q] = NTIMESG[q]
+    for q in range(2):
+        c.bv_f[q] = spec.bv_f[q]
+    for q in range(6):
+        c.ba_f[q] = spec.ba_f[q]
     c.sigma_sb = 5.668e-8
     c.h_g = 10.0
     c.kappa_gray = spec.gray_kappa
@@ -620,7 +636,7 @@ def air_velocity(u_air0, fuel_u, fuel_u0, air_t, air_t0, excess_air, excess_air0
 
 def furnace_small(nx=48, ny=24, nz=16, seed=1, noise=0.01, maxgi=300, bgcon=1.0e-12, walls="isothermal",
                   gray_kappa=0.3, fuel_u=30.0, air_t=1100.0, excess_air=0.10, air_port=(0.3, 0.2), air_u=None,
-                  reacting=True):
+                  reacting=True, half_blocked=(), bv_f=(1.0, 1.0), ba_f=(1.0,) * 6):
     """3.0 x 1.2 x 0.8 m chamber, uniform cells, one CH4 port (0.1x0.1 m,
     30 m/s, 300 K) centred on the x=0 wall inside an air port (0.3x0.2 m,
     10 m/s, 1100 K), full-face exit at x=3 m; uniform initial field
@@ -632,7 +648,8 @@ def furnace_small(nx=48, ny=24, nz=16, seed=1, noise=0.01, maxgi=300, bgcon=1.0e
     from air_velocity, nominal 10 m/s, or air_u).  air_port (y, z extent in
     m) sizes the air port; (1.2, 0.8) makes it the whole x=0 wall around
     the fuel port (a coflow chamber).  reacting=False sends air at 300 K
-    through the fuel port (no reaction)."""
+    through the fuel port (no reaction).  half_blocked, bv_f, ba_f: cells
+    of code -1/-2 and their &GRD factors (CaseSpec)."""
     Lx, Ly, Lz = 3.0, 1.2, 0.8
     u_air = air_velocity(10.0, fuel_u, 30.0, air_t, 1100.0, excess_air)
     dy, dz = Ly / ny, Lz / nz
@@ -655,7 +672,8 @@ def furnace_small(nx=48, ny=24, nz=16, seed=1, noise=0.01, maxgi=300, bgcon=1.0e
     spec = CaseSpec(name=f"furnace {nx}x{ny}x{nz}", faces_x=uniform_faces(Lx, nx), faces_y=uniform_faces(Ly, ny),
                     faces_z=uniform_faces(Lz, nz), inlets=[fuel, air], t_init=1500.0, k_init=1.0e-2,
                     eps_init=1.0e-2, noise=noise, seed=seed, maxgi=maxgi, bgcon=bgcon,
-                    t_wall=1500.0 if walls == "isothermal" else None, gray_kappa=gray_kappa)
+                    t_wall=1500.0 if walls == "isothermal" else None, gray_kappa=gray_kappa,
+                    half_blocked=list(half_blocked), bv_f=tuple(bv_f), ba_f=tuple(ba_f))
     return build(spec)
 
 

@@ -70,6 +70,7 @@ struct DevCase {
   double t_wall, kappa_gray, sigma_sb, h_g;
   int i1st_rad_done, isothermal_walls, gray_radiation;
   double wt0, stoich;
+  double bv_f[2], ba_f[6];  // half-blocked cells: volume and face-area factors (&GRD BV_F, BA_F)
 };
 
 // device scalar block indices

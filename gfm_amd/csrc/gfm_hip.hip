@@ -217,6 +217,8 @@ static void fill_devcase_scalars(gfm_ctx *c, const gfm_ctrl *k) {
   d.i1st_rad_done = k->i1st_rad_done;
   d.isothermal_walls = k->isothermal_walls;
   d.gray_radiation = k->gray_radiation;
+  for (int q = 0; q < 2; q++) d.bv_f[q] = k->bv_f[q];
+  for (int q = 0; q < 6; q++) d.ba_f[q] = k->ba_f[q];
   d.wt0 = 1.0 / (0.5 / WF + 0.0 / WCO2 + 0.0 / WH2O + 0.49 / WO2 + 0.01 / WN2);
   d.stoich = 2.0 * WO2 / WF;
 }
@@ -959,7 +961,7 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
   if (k->precision != GFM_PREC_FP64 && k->precision != GFM_PREC_MIXED) { set_err("bad precision mode %d", k->precision); return GFM_E_ARG; }
   const long long nlat = (long long)k->mp * k->np * k->lp;
   for (long long q = 0; q < nlat; q++)
-    if (h->ibcell[q] < 0) { set_err("half-blocked cells (IBCELL<0) are not supported"); return GFM_E_ARG; }
+    if (h->ibcell[q] < -2) { set_err("cell code %d below -2 (half-blocked cells are -1 and -2)", (int)h->ibcell[q]); return GFM_E_ARG; }
   int ndev = 0;
   HCHECK(hipGetDeviceCount(&ndev));
   if (device < 0 || device >= ndev) { set_err("device %d not present (%d devices)", device, ndev); return GFM_E_DEVICE; }

@@ -172,13 +172,40 @@ __device__ void fin_reduce(const double *part, long long stride, int nb, unsigne
 // ======================================================================
 // geometry, C/Grid3.f90:310-471 (no half-blocked cells)
 // ======================================================================
+// Face area and volume, C/Grid3.f90:389-471: a half-blocked cell (code -1
+// or -2) scales its own face areas by BA_F(ix, n) and its volume by BV_F(n);
+// a cell of positive code other than a wall takes BA_F of a half-blocked
+// neighbour two lattice steps away along ix.
 __device__ __forceinline__ double AREA_C(int i, int j, int k, int ix) {
-  if (IBC(i, j, k) == 1) return 0.0;
-  if (ix == 1) return DR(j) * RS(j) * DZ(k);
-  if (ix == 2) return DX(i) * DZ(k);
-  return DX(i) * DR(j) * RS(j);
+  const int b = IBC(i, j, k);
+  if (b == 1) return 0.0;
+  double a;
+  if (ix == 1) a = DR(j) * RS(j) * DZ(k);
+  else if (ix == 2) a = DX(i) * DZ(k);
+  else a = DX(i) * DR(j) * RS(j);
+  if (b == 0) return a;
+  if (b <= -1) return a * g.ba_f[(ix - 1) + 3 * (-b - 1)];
+  int nb;
+  if (ix == 1) {
+    if (i <= MP - 2 && IBC(i + 2, j, k) <= -1) nb = -IBC(i + 2, j, k);
+    else if (i >= 4 && IBC(i - 2, j, k) <= -1) nb = -IBC(i - 2, j, k);
+    else return a;
+  } else if (ix == 2) {
+    if (j <= NP - 2 && IBC(i, j + 2, k) <= -1) nb = -IBC(i, j + 2, k);
+    else if (j >= 4 && IBC(i, j - 2, k) <= -1) nb = -IBC(i, j - 2, k);
+    else return a;
+  } else {
+    if (k <= LP - 2 && IBC(i, j, k + 2) <= -1) nb = -IBC(i, j, k + 2);
+    else if (k >= 4 && IBC(i, j, k - 2) <= -1) nb = -IBC(i, j, k - 2);
+    else return a;
+  }
+  return a * g.ba_f[(ix - 1) + 3 * (nb - 1)];
 }
-__device__ __forceinline__ double VOL_C0(int i, int j, int k) { return DX(i) * DR(j) * RS(j) * DZ(k); }
+__device__ __forceinline__ double VOL_C0(int i, int j, int k) {
+  const double v = DX(i) * DR(j) * RS(j) * DZ(k);
+  const int b = IBC(i, j, k);
+  return b <= -1 ? v * g.bv_f[-b - 1] : v;
+}
 __device__ double VOL_C(int i, int j, int k) {
   double vm, vp;
   if (i & 1) {
@@ -556,7 +583,19 @@ __global__ void k_momentum_assemble(Box b) {
       g0 += fld * FZ(i2, j2, k2);
     }
   }
-  double vol = (ffl[iu1 - 1][0] + ffl[iu1 - 1][1]) * AREA_C(i, j, k, iu1);  // VOL1
+  // VOL1 (C/Gslv3x.f90:266-286): less half of a half-blocked neighbour cell
+  const double area1 = AREA_C(i, j, k, iu1);
+  double vol = (ffl[iu1 - 1][0] + ffl[iu1 - 1][1]) * area1;
+  if (iu1 == 1) {
+    if (IBC(i - 1, j, k) <= -1) vol -= DX(i - 1) * area1 / 2.0;
+    if (IBC(i + 1, j, k) <= -1) vol -= DX(i + 1) * area1 / 2.0;
+  } else if (iu1 == 2) {
+    if (IBC(i, j - 1, k) <= -1) vol -= RS(j - 1) * DR(j - 1) * area1 / 2.0;
+    if (IBC(i, j + 1, k) <= -1) vol -= RS(j + 1) * DR(j + 1) * area1 / 2.0;
+  } else {
+    if (IBC(i, j, k - 1) <= -1) vol -= DZ(k - 1) * area1 / 2.0;
+    if (IBC(i, j, k + 1) <= -1) vol -= DZ(k + 1) * area1 / 2.0;
+  }
   bs *= vol;
   ap += (0.0 - 0.0 - sfp) * vol;
   SBS(q, bs);

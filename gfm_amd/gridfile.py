@@ -103,6 +103,27 @@ def _namelist(text):
     return out
 
 
+def grd_factors(grd):
+    """BV_F(1:2) and BA_F(1:3,1:2) (column-major, 6 values) of a parsed
+    &GRD namelist, 1 where it gives none (C/Grid3.f90:252-255): whole arrays
+    as lists, or single elements as NAME(i) / NAME(i,j)."""
+    bv, ba = [1.0, 1.0], [1.0] * 6
+    for key, vals in grd.items():
+        m = re.fullmatch(r"(BV_F|BA_F)(?:\((\d+)(?:,(\d+))?\))?", key)
+        if not m:
+            continue
+        dst = bv if m.group(1) == "BV_F" else ba
+        if m.group(2) is None:
+            start = 0
+        elif m.group(1) == "BV_F":
+            start = int(m.group(2)) - 1
+        else:
+            start = (int(m.group(2)) - 1) + 3 * (int(m.group(3) or 1) - 1)
+        for q, v in enumerate(vals):
+            dst[start + q] = v
+    return tuple(bv), tuple(ba)
+
+
 def read_gd(path_or_text):
     """Parse a gd file (path, or its text) as GRID3 does."""
     text = path_or_text
@@ -175,6 +196,11 @@ def write_gd(path, case, title="Combustion Space", surf=(0.0, 0.0)):
         for ic in range(mp // 2):
             row = "".join(str(int(v) + 10 if v < 0 else int(v)) for v in codes[ic, :, kc])
             lines.append(f"{2 * ic + 2:4d}  {row}")
-    lines += ["", "&GRD", "/", ""]
+    bv, ba = tuple(k.bv_f), tuple(k.ba_f)
+    if bv == (1.0, 1.0) and ba == (1.0,) * 6:
+        lines += ["", "&GRD", "/", ""]
+    else:
+        lines += ["", "&GRD", " BV_F=" + ",".join(repr(float(v)) for v in bv) + ",",
+                  " BA_F=" + ",".join(repr(float(v)) for v in ba), "/", ""]
     with open(path, "w") as f:
         f.write("\n".join(lines))

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define GFM_ABI_VERSION 2
+#define GFM_ABI_VERSION 3
 
 enum {
   GFM_OK = 0,
@@ -157,6 +157,12 @@ typedef struct gfm_ctrl {
   double pcorr_guess_accept;      /* in: acceptance ratio of the learned guess (see above) */
   double q_wall_bottom, q_wall_top; /* out: the parts of q_wall through cells' bottom (k-) and top (k+) faces:
                                        glass/floor and crown */
+  /* half-blocked cells (IBCELL -1 and -2, grid-file digits 9 and 8): the
+   * &GRD factors BV_F(1:2) of their volume and BA_F(1:3,1:2) of their face
+   * areas (column-major, BA_F(ix,n) = ba_f[(ix-1) + 3*(n-1)]), read by
+   * GRID3 (C/Grid3.f90:12-13,252-255) and applied by VOL_C0, AREA_C and VOL1
+   * (C/Grid3.f90:389-471, C/Gslv3x.f90:269-286).  1 where no such cells. */
+  double bv_f[2], ba_f[6];
 } gfm_ctrl;
 
 /* Opaque device-side context. */

@@ -14,7 +14,7 @@
  *
  * Deliberate restrictions (checked at entry, documented in the design
  * notes): single phase (NDNP=0), Cartesian (CYL=.false.), turbulent,
- * reacting, steady, SIMPLER; no half-blocked cells (IBCELL>=0); no
+ * reacting, steady, SIMPLER; cell codes from -2 (half-blocked) up; no
  * radiation transport (only the optional gray source term).  Unused
  * diagnostic sums (FLX/FLY/FLUX_Z plane fluxes of FLX1D, the Info/gresid
  * file writes) are not formed.
@@ -109,13 +109,38 @@ static inline int imod10(int a) { return a % 10; } /* Fortran MOD keeps the sign
 /* ======================================================================
  * Geometry, C/Grid3.f90:310-471 (no half-blocked cells)
  * ====================================================================== */
+/* AREA_C, C/Grid3.f90:427-471; VOL_C0, C/Grid3.f90:410-421.  Half-blocked
+ * cells (IBCELL -1, -2) scale by the &GRD factors BA_F(ix,n) and BV_F(n). */
 static double AREA_C(int i, int j, int k, int ix) {
-  if (IBCELL(i, j, k) == 1) return 0.0;
-  if (ix == 1) return DR(j) * RS(j) * DZ(k);
-  if (ix == 2) return DX(i) * DZ(k);
-  return DX(i) * DR(j) * RS(j);
+  const int b = IBCELL(i, j, k);
+  if (b == 1) return 0.0;
+  double area;
+  if (ix == 1) area = DR(j) * RS(j) * DZ(k);
+  else if (ix == 2) area = DX(i) * DZ(k);
+  else area = DX(i) * DR(j) * RS(j);
+  if (b == 0) return area;
+  if (b <= -1) return area * C->ba_f[(ix - 1) + 3 * (-b - 1)];
+  int ibc;
+  if (ix == 1) {
+    if (i <= MPM2 && IBCELL(i + 2, j, k) <= -1) ibc = -IBCELL(i + 2, j, k);
+    else if (i >= 4 && IBCELL(i - 2, j, k) <= -1) ibc = -IBCELL(i - 2, j, k);
+    else return area;
+  } else if (ix == 2) {
+    if (j <= NPM2 && IBCELL(i, j + 2, k) <= -1) ibc = -IBCELL(i, j + 2, k);
+    else if (j >= 4 && IBCELL(i, j - 2, k) <= -1) ibc = -IBCELL(i, j - 2, k);
+    else return area;
+  } else {
+    if (k <= LPM2 && IBCELL(i, j, k + 2) <= -1) ibc = -IBCELL(i, j, k + 2);
+    else if (k >= 4 && IBCELL(i, j, k - 2) <= -1) ibc = -IBCELL(i, j, k - 2);
+    else return area;
+  }
+  return area * C->ba_f[(ix - 1) + 3 * (ibc - 1)];
 }
-static double VOL_C0(int i, int j, int k) { return DX(i) * DR(j) * RS(j) * DZ(k); }
+static double VOL_C0(int i, int j, int k) {
+  double vol = DX(i) * DR(j) * RS(j) * DZ(k);
+  if (IBCELL(i, j, k) <= -1) vol *= C->bv_f[-IBCELL(i, j, k) - 1];
+  return vol;
+}
 static double VOL_C(int i, int j, int k) {
   double vm, vp;
   if (i & 1) {
@@ -358,8 +383,21 @@ static void ECELL(int i, int j, int k, int iu, double ffl[3][2]) {
     }
   }
 }
+/* VOL1, C/Gslv3x.f90:266-286 */
 static double VOL1(int i, int j, int k, int ix, double ffl[3][2]) {
-  return (ffl[ix - 1][0] + ffl[ix - 1][1]) * AREA_C(i, j, k, ix);
+  double area = AREA_C(i, j, k, ix);
+  double vol = (ffl[ix - 1][0] + ffl[ix - 1][1]) * area;
+  if (ix == 1) {
+    if (IBCELL(i - 1, j, k) <= -1) vol = vol - DX(i - 1) * area / 2.0;
+    if (IBCELL(i + 1, j, k) <= -1) vol = vol - DX(i + 1) * area / 2.0;
+  } else if (ix == 2) {
+    if (IBCELL(i, j - 1, k) <= -1) vol = vol - RS(j - 1) * DR(j - 1) * area / 2.0;
+    if (IBCELL(i, j + 1, k) <= -1) vol = vol - RS(j + 1) * DR(j + 1) * area / 2.0;
+  } else {
+    if (IBCELL(i, j, k - 1) <= -1) vol = vol - DZ(k - 1) * area / 2.0;
+    if (IBCELL(i, j, k + 1) <= -1) vol = vol - DZ(k + 1) * area / 2.0;
+  }
+  return vol;
 }
 /* GSLV3X1, C/Gslv3x.f90:119-139 */
 static double GSLV3X1(int i, int j, int k, int iu) {
@@ -1651,7 +1689,7 @@ static int bind(gfm_ctrl *c, const gfm_host_arrays *h) {
   wt0 = 1.0 / (0.5 / wf + 0.0 / wco2 + 0.0 / wh2o + 0.49 / wo2 + 0.01 / wn2);
   stoich = 2.0 * wo2 / wf;
   for (size_t q = 0; q < NN; q++)
-    if (IBC[q] < 0) return GFM_E_ARG;
+    if (IBC[q] < -2) return GFM_E_ARG;
   static size_t cap = 0;
   if (cap != NN) {
     free(FZ_); free(BS_); free(SFP_); free(AP_); free(AS_); free(XT2_); free(XT3_);
@@ -1776,7 +1814,7 @@ int oracle_rmass(int enable, double *out) {
  *     UG corrected, FZ = p', BS = SOURCE(5)'s mass source, res_mass set;
  *   7 SOURCE of equation NELG = i over the cell box (BS, SFP; CRATE/HRATE
  *     for the fuel);
- *   8 INTP; 9 INTPV.
+ *   8 INTP; 9 INTPV; 10 AREA_C(i,j,k, ix = io[0]) -> io[2]; 11 VOL_C(i,j,k) -> io[2].
  * AP/AS/BS/FZ/SFP are read back with oracle_scratch / oracle_sfp. */
 int oracle_probe(gfm_ctrl *c, const gfm_host_arrays *h, int what, int i, int j, int k, double *io) {
   int rc = bind(c, h);
@@ -1798,6 +1836,8 @@ int oracle_probe(gfm_ctrl *c, const gfm_host_arrays *h, int what, int i, int j, 
       break;
     case 8: INTP(); break;
     case 9: INTPV(); break;
+    case 10: io[2] = AREA_C(i, j, k, (int)io[0]); break;
+    case 11: io[2] = VOL_C(i, j, k); break;
     default: return GFM_E_ARG;
   }
   return GFM_OK;

@@ -82,7 +82,7 @@ def rmass(case, enable=True):
 
 
 PROBES = {"conc": 1, "dnst_calc": 2, "ther": 3, "q_conv": 4, "momentum": 5, "vel_correct": 6, "source": 7,
-          "intp": 8, "intpv": 9}
+          "intp": 8, "intpv": 9, "area_c": 10, "vol_c": 11}
 
 
 def probe(case, what, i=0, j=0, k=0, io=None):

@@ -36,7 +36,7 @@ def test_library_exports_header_symbols():
 def test_ctrl_layout():
     lib = solver.lib()
     assert lib.gfm_hip_ctrl_size() == ct.sizeof(GfmCtrl)
-    assert lib.gfm_hip_abi_version() == 2
+    assert lib.gfm_hip_abi_version() == 3
     assert ct.sizeof(GfmHostArrays) == 28 * 8
 
 

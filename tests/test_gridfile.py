@@ -74,3 +74,22 @@ def test_malformed_grid_is_rejected():
     bad[j] = bad[j][:-3]  # a cell-map row short of digits
     with pytest.raises(ValueError):
         gfile.read_gd("\n".join(bad))
+
+
+def test_half_blocked_cells_round_trip_with_their_factors(tmp_path):
+    """Cells of code -1 and -2 are written as digits 9 and 8 and the &GRD
+    factors BV_F, BA_F as a namelist; GRID3's reading rules give them back
+    (C/Grid3.f90:243-256), as open cells for NCELLS."""
+    hb = [((6, 8), (3, 5), (2, 4), -1), ((15, 15), (7, 9), (6, 7), -2)]
+    bv, ba = (0.5, 0.7), (0.6, 0.8, 0.9, 0.75, 0.65, 0.85)
+    c = cs.furnace_small(nx=16, ny=12, nz=10, seed=3, half_blocked=hb, bv_f=bv, ba_f=ba)
+    p = tmp_path / "gd0001c.dat"
+    gfile.write_gd(p, c)
+    text = p.read_text()
+    assert "9" in text.split("Computational and blocked cells")[1].split("&GRD")[0]
+    g = gfile.read_gd(str(p))
+    np.testing.assert_array_equal(g.ibcell, c.arrays["ibcell"])
+    assert g.ncells == c.ctrl.ncells
+    assert gfile.grd_factors(g.grd) == (bv, ba)
+    # element forms of the namelist
+    assert gfile.grd_factors({"BA_F(2,2)": [0.5], "BV_F(2)": [0.25]}) == ((1.0, 0.25), (1.0,) * 4 + (0.5, 1.0))

@@ -519,3 +519,90 @@ def test_enthalpy_source_history_relaxation():
     np.testing.assert_allclose(got[opn], rf * raw[opn] + (1.0 - rf) * hist[opn], rtol=1e-13,
                                atol=1e-13 * np.abs(raw[opn]).max())
     np.testing.assert_array_equal(c.arrays["bs_hprev"][sl][opn], got[opn])
+
+
+# ---------------------------------------------------------------------
+# half-blocked cells (IBCELL -1, -2; C/Grid3.f90:243-256,389-471)
+# ---------------------------------------------------------------------
+HALF_BLOCKED = [((6, 8), (3, 5), (2, 4), -1), ((15, 15), (7, 9), (6, 7), -2), ((0, 0), (5, 5), (4, 4), -1)]
+BV_F = (0.5, 0.7)
+BA_F = (0.6, 0.8, 0.9, 0.75, 0.65, 0.85)
+
+
+def half_blocked_case():
+    return cs.furnace_small(nx=16, ny=12, nz=10, seed=3, half_blocked=HALF_BLOCKED, bv_f=BV_F, ba_f=BA_F)
+
+
+def _base_area(c, I, J, K, ix):
+    A = c.arrays
+    dx, dr, dz, rs = A["dx"][I - 1], A["dr"][J - 1], A["dz"][K - 1], A["rs"][J - 1]
+    return (dr * rs * dz, dx * dz, dx * dr * rs)[ix - 1]
+
+
+def test_half_blocked_cells_are_open_and_not_propagated():
+    c = half_blocked_case()
+    ib = c.arrays["ibcell"]
+    assert ib[2 * 6 + 3, 2 * 3 + 3, 2 * 2 + 3] == -1 and ib[2 * 15 + 3, 2 * 7 + 3, 2 * 6 + 3] == -2
+    # a half-blocked cell's faces keep the open code (GRID3 propagates codes >= 1 only)
+    assert ib[2 * 6 + 3 + 1, 2 * 3 + 3, 2 * 2 + 3] == 0
+    plain = cs.furnace_small(nx=16, ny=12, nz=10, seed=3)
+    assert c.ctrl.ncells == plain.ctrl.ncells
+
+
+def test_half_blocked_area_and_volume_closed_forms():
+    c = half_blocked_case()
+    oracle.initsv_tail(c)
+    ba = np.array(BA_F).reshape(2, 3)  # [n-1][ix-1]
+    mp = c.shape[0]
+    cases = []
+    I, J, K = 4 + 2 * 7, 4 + 2 * 4, 4 + 2 * 3          # inside the -1 block
+    for ix in (1, 2, 3):
+        cases.append(((I, J, K), ix, _base_area(c, I, J, K, ix) * ba[0][ix - 1]))
+    I2, J2, K2 = 4 + 2 * 15, 4 + 2 * 8, 4 + 2 * 6      # a -2 cell on the last plane
+    for ix in (1, 2, 3):
+        cases.append(((I2, J2, K2), ix, _base_area(c, I2, J2, K2, ix) * ba[1][ix - 1]))
+    # the exit cell beyond it (code 3) takes BA_F of its half-blocked x neighbour
+    assert c.arrays["ibcell"][mp - 1, J2 - 1, K2 - 1] == 3
+    cases.append(((mp, J2, K2), 1, _base_area(c, mp, J2, K2, 1) * ba[1][0]))
+    # the inlet cell next to the -1 cell at the inlet wall, likewise
+    assert c.arrays["ibcell"][1, 4 + 2 * 5 - 1, 4 + 2 * 4 - 1] == 2
+    cases.append(((2, 4 + 2 * 5, 4 + 2 * 4), 1, _base_area(c, 2, 4 + 2 * 5, 4 + 2 * 4, 1) * ba[0][0]))
+    # an open cell next to the block: its own area
+    cases.append(((I + 6, J, K), 1, _base_area(c, I + 6, J, K, 1)))
+    for (i, j, k), ix, want in cases:
+        got = oracle.probe(c, "area_c", i, j, k, io=[ix, 0.0])
+        assert got == pytest.approx(want, rel=1e-15), ((i, j, k), ix)
+    A = c.arrays
+    v0 = lambda i, j, k: A["dx"][i - 1] * A["dr"][j - 1] * A["rs"][j - 1] * A["dz"][k - 1]
+    assert oracle.probe(c, "vol_c", I, J, K) == pytest.approx(v0(I, J, K) * BV_F[0], rel=1e-15)
+    assert oracle.probe(c, "vol_c", I2, J2, K2) == pytest.approx(v0(I2, J2, K2) * BV_F[1], rel=1e-15)
+    # x face between the last -1 cell of the block and the open cell beyond:
+    # half of each cell's volume, the half-blocked one scaled by BV_F
+    Ie = 4 + 2 * 8
+    want = 0.5 * v0(Ie, J, K) * BV_F[0] + 0.5 * v0(Ie + 2, J, K)
+    assert oracle.probe(c, "vol_c", Ie + 1, J, K) == pytest.approx(want, rel=1e-15)
+
+
+def test_half_blocked_staggered_volume_of_the_buoyancy_source():
+    """VOL1 (C/Gslv3x.f90:266-286): a w face next to a half-blocked cell
+    loses half of that cell's height times the face area, so the buoyancy
+    source there is rho g ((Z(k+1) - Z(k-1)) - DZ(k +- 1)/2) A."""
+    c = half_blocked_case()
+    A = c.arrays
+    A["theta"][...] = 1.0
+    A["dnst"][...] = 1.0
+    A["gama"][...] = 0.02
+    A["ug"][...] = 0.0
+    g = c.ctrl.gra[2]
+    oracle.probe(c, "momentum")
+    bs = oracle.scratch(c)["bs"]
+    I, J = 4 + 2 * 7, 4 + 2 * 4
+    z, dz = A["z"], A["dz"]
+    for K, below, above in ((4 + 2 * 2 - 1, False, True), (4 + 2 * 3 + 1, True, True), (4 + 2 * 4 + 1, True, False)):
+        area = _base_area(c, I, J, K, 3)
+        vol = (z[K] - z[K - 2]) * area  # Z(K+1) - Z(K-1), 1-based
+        if below:
+            vol -= dz[K - 2] * area / 2.0
+        if above:
+            vol -= dz[K] * area / 2.0
+        assert bs[I - 1, J - 1, K - 1] == pytest.approx(g * vol, rel=1e-12), K
