@@ -72,6 +72,43 @@ GRID1 = ("x", "dx", "r", "dr", "rs", "z", "dz")
 KIN = ("rtim", "delrt", "hrt", "delr", "timed", "hrd")
 
 
+class GfmMsCtrl(ct.Structure):
+    """gfm_ms_ctrl: the minor-species inputs and outputs (MSFLOW/GSLV3M)."""
+    _fields_ = [
+        ("maxms", I32), ("react", I32), ("ntimesm", I32), ("itr_ms", I32), ("igs0", I32), ("pad0", I32),
+        ("bmcon", D), ("ssigma", D), ("rfm", D * 8),
+        ("aform", D), ("aoxid", D), ("esf_r", D), ("eso_r", D), ("t0_sf", D), ("t0_so", D),
+        ("rf_soot_form_source", D), ("rf_soot_oxid_source", D), ("soot_cap", D), ("dnst_soot", D), ("vol_tot", D),
+        ("iterations", I32), ("converged", I32), ("smf_clipped_cnt", I32), ("pad1", I32),
+        ("resid_ms_pre", D * 8), ("resid_ms", D * 8),
+        ("bal_ms", D), ("soot_tot", D), ("smf_max", D), ("smf_max_noclip", D), ("rr_f", D), ("rr_c", D),
+        ("gfm_0", D * 7), ("gfmex", D), ("ave_chg", D), ("svf_r", D),
+    ]
+
+    def copy(self):
+        c = GfmMsCtrl()
+        ct.memmove(ct.byref(c), ct.byref(self), ct.sizeof(GfmMsCtrl))
+        return c
+
+
+class GfmMsArrays(ct.Structure):
+    _fields_ = [("gfm", DP), ("smf", DP), ("bs_soot_prev", DP), ("sfp_soot_prev", DP), ("rrms", DP)]
+
+
+MS_FIELDS = ("gfm", "smf", "bs_soot_prev", "sfp_soot_prev", "rrms")
+
+
+def ms_arrays(arrs):
+    """GfmMsArrays pointing at the numpy arrays of a minor-species state
+    (keep ``arrs`` alive while the structure is used)."""
+    a = GfmMsArrays()
+    for k in MS_FIELDS:
+        v = arrs[k]
+        assert v.dtype == np.float64 and v.flags["F_CONTIGUOUS"], k
+        setattr(a, k, v.ctypes.data_as(DP))
+    return a
+
+
 def _ptr(a, ctype=ct.c_double):
     assert a.flags["F_CONTIGUOUS"] or a.ndim == 1, "arrays must be Fortran-ordered"
     return a.ctypes.data_as(ct.POINTER(ctype))

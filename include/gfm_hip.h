@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define GFM_ABI_VERSION 3
+#define GFM_ABI_VERSION 4
 
 enum {
   GFM_OK = 0,
@@ -260,6 +260,96 @@ int gfm_hip_pcorr_pair(gfm_ctx *ctx, float *x, float *p);
  * NULL.  GFM_E_STATE if no iteration has formed them. */
 int gfm_hip_plane_flows(gfm_ctx *ctx, double *flx, double *fly, double *flyh, double *flux_z, double *fluxh_z);
 
+/* ---------------------------------------------------------------------
+ * Minor species and soot (MS=1 runs): the sub-species solve that the
+ * reference runs on the converged gas field,
+ *
+ *   call initsvm; call msflow; call sav3fm      C/A0main.f90:219-222, C/Sphase.f90:216-218
+ *   SUBROUTINE INITSVM                           C/Initsv.f90:329-380
+ *   SUBROUTINE MSFLOW                            C/Msflow.f90:10-169
+ *   SUBROUTINE GSLV3M                            C/Gslv3m.f90:8-286 (SPIN :292-305)
+ *   SUBROUTINE SOURCEM(MI1,MI3,NJ1,NJ3,LK1,LK3)  C/Sourcem.f90:8-122 (new soot model)
+ *   subroutine flx1dm / extrm / intpm            C/Flx1d.f90:114-158, C/Extr.f90:258-305, C/Intp.f90:348-419
+ *
+ * Seven lumped species GFM(:,:,:,1..7) = CH4, O2, CO, H2O, CO2, N2, NO and
+ * the soot mass fraction smf, transported in the gas field the context
+ * holds, with the five-reaction kinetics of GSLV3M (rates RRMS(MZ,NZ,LZ,
+ * 1..5), MZ = MP/2 ..., of which 1, 2 and 4 are formed) and the soot
+ * formation/oxidation source of SOURCEM.  The species equations reuse the
+ * gas solver's coefficient arrays: the rates of the first GSLV3M after
+ * INITSVM read the coefficients AS that the last GFLOW left (its last
+ * scalar equation), as the reference's GBL_VAR AS does; the species
+ * coefficients (SSIGMA) are assembled once after INITSVM and kept (AP1).
+ * Device memory for these fields is allocated at the first call of any
+ * gfm_hip_ms_* entry point. */
+#define GFM_NSP0 7
+#define GFM_NRM0 5
+typedef struct gfm_ms_ctrl {
+  /* inputs (C/Setup.f90:161-185, the relaxation file C/Setup.f90:478-493) */
+  int32_t maxms;              /* MAXMS: GSLV3M calls per MSFLOW */
+  int32_t react;              /* REACT: solve the soot equation */
+  int32_t ntimesm;            /* NTIMESM: ADLBL3 sweeps of every species and of soot */
+  int32_t itr_ms;             /* in/out: minor-species iteration count (rs record) */
+  int32_t igs0;               /* in/out: 0 = assemble the species coefficients at the next GSLV3M */
+  int32_t pad0;
+  double bmcon;               /* BMCON: every resid_ms below it ends MSFLOW (after 10 iterations) */
+  double ssigma;              /* SSIGMA: Schmidt number of the species equations */
+  double rfm[8];              /* RFM(1:8): relaxation of species 1..7 and soot (RFMC = 1 - RFM) */
+  double aform, aoxid;        /* soot formation / oxidation pre-exponential constants */
+  double esf_r, eso_r;        /* Esf/Ru, Eso/Ru (K) */
+  double t0_sf, t0_so;        /* soot formation / oxidation cut-off temperatures (K) */
+  double rf_soot_form_source, rf_soot_oxid_source;  /* relaxation of the soot sources */
+  double soot_cap;            /* largest soot mass fraction kept */
+  double dnst_soot;           /* soot density (kg/m3) */
+  double vol_tot;             /* VOL_TOT: volume of the open cells (m3), C/Grid3.f90:286-294 */
+  /* outputs of the last GSLV3M / MSFLOW */
+  int32_t iterations;         /* GSLV3M calls of the last gfm_hip_msflow */
+  int32_t converged;          /* 1 if that call met MSFLOW's convergence test */
+  int32_t smf_clipped_cnt;    /* cells clipped at soot_cap */
+  int32_t pad1;
+  double resid_ms_pre[8], resid_ms[8];  /* calc_residual before / after each solve (8 = soot) */
+  double bal_ms;              /* volume-weighted |1 - sum GFM| / VOL_TOT */
+  double soot_tot;            /* soot mass (kg) */
+  double smf_max, smf_max_noclip;
+  double rr_f, rr_c;          /* sum of the CH4 and CO rates times their molar masses */
+  double gfm_0[7];            /* species flow through the exit planes, GFM_0(:,2) (kg/s) */
+  double gfmex;               /* sum of gfm_0 */
+  double ave_chg;             /* mean |change of gfm_0| over the last iteration */
+  double svf_r;               /* area-mean soot volume fraction of the exit cells */
+} gfm_ms_ctrl;
+
+/* Host arrays of the minor-species state, reference layout:
+ * gfm (MP,NP,LP,7), smf, bs_soot_prev, sfp_soot_prev (MP,NP,LP),
+ * rrms (MP/2,NP/2,LP/2,5). */
+typedef struct gfm_ms_arrays {
+  double *gfm, *smf, *bs_soot_prev, *sfp_soot_prev, *rrms;
+} gfm_ms_arrays;
+
+/* Copy the minor-species state to / from the device (the arrays sav3fm
+ * writes and INITSVM reads, C/Sav3f.f90:96-116, C/Initsv.f90:352-372; the
+ * counters itr_ms and igs0 travel in ms). */
+int gfm_hip_ms_upload(gfm_ctx *ctx, const gfm_ms_ctrl *ms, const gfm_ms_arrays *a);
+int gfm_hip_ms_download(gfm_ctx *ctx, gfm_ms_ctrl *ms, gfm_ms_arrays *a);
+
+/* INITSVM (C/Initsv.f90:329-380) on the device.  restart = 0: a new start
+ * (GFM = 0 and smf = 0 everywhere, then the cell centres take CH4, O2,
+ * H2O, CO2 and N2 from the gas field; the rates, the soot-source history
+ * and itr_ms are kept).  restart = 1: the state of the last
+ * gfm_hip_ms_upload is the restart record; inlet cells (IBCELL 2) take
+ * the five from the gas field and CO = NO = 0, then extrm.  Both end with
+ * intpm and igs0 = 0. */
+int gfm_hip_initsvm(gfm_ctx *ctx, gfm_ms_ctrl *ms, int restart);
+
+/* One GSLV3M call (C/Gslv3m.f90:8-286): rates, the seven species
+ * equations, the balance, and (react) the soot equation.  Does not
+ * advance itr_ms. */
+int gfm_hip_gslv3m(gfm_ctx *ctx, gfm_ms_ctrl *ms);
+
+/* MSFLOW (C/Msflow.f90:29-162): up to maxms iterations of itr_ms += 1,
+ * GSLV3M, flx1dm, the exit soot fraction, the convergence test (itr_ms >
+ * 10 and every resid_ms <= bmcon) and extrm + intpm. */
+int gfm_hip_msflow(gfm_ctx *ctx, gfm_ms_ctrl *ms);
+
 /* Per-kernel timing with HIP events recorded on the library's stream around
  * each launch.  enable=1 clears the records and starts recording, enable=0
  * stops; every call returns up to `cap` records accumulated so far: name
@@ -274,8 +364,10 @@ int gfm_hip_finalize(gfm_ctx *ctx);
 
 const char *gfm_hip_last_error(void);
 int gfm_hip_abi_version(void);
-/* sizeof(gfm_ctrl) as compiled into the library, for binding checks. */
+/* sizeof(gfm_ctrl) / sizeof(gfm_ms_ctrl) as compiled into the library, for
+ * binding checks. */
 size_t gfm_hip_ctrl_size(void);
+size_t gfm_hip_ms_ctrl_size(void);
 
 #ifdef __cplusplus
 }

@@ -786,9 +786,8 @@ static void ADLZ(double *xt, int i, int jb, int je, int kb, int ke) {
   }
 }
 /* ADLBL3, C/Adlbl3.f90:10-97.  order selects the x-plane visiting order. */
-static void ADLBL3(double *xt, int ib, int ie, int jb, int je, int kb, int ke) {
+static void ADLBL3_SWEEPS(double *xt, int nim, int ib, int ie, int jb, int je, int kb, int ke) {
   const int ld = 2;
-  int nim = C->ntimesg[NELG - 1];
   for (int ns = 1; ns <= nim; ns++) {
     if (C->sweep_order == GFM_ORDER_REFERENCE) {
       for (int i = ib; i <= ie; i += ld) ADLZ(xt, i, jb, je, kb, ke);
@@ -800,6 +799,12 @@ static void ADLBL3(double *xt, int ib, int ie, int jb, int je, int kb, int ke) {
       }
     }
   }
+}
+/* ADLBL3 of a gas-phase equation: NTIMESG(NELG) sweeps, then the IRC
+ * residual block (RESG). */
+static void ADLBL3(double *xt, int ib, int ie, int jb, int je, int kb, int ke) {
+  const int ld = 2;
+  ADLBL3_SWEEPS(xt, C->ntimesg[NELG - 1], ib, ie, jb, je, kb, ke);
   if (C->want_irc) {
     double res0 = 0.0, resx = 0.0;
     for (int i = ib; i <= ie; i += ld)
@@ -1854,3 +1859,526 @@ double oracle_mass_residual(gfm_ctrl *c, const gfm_host_arrays *h) {
   SOURCE(4, MPM2, NJY1, NJY2, LKZ1, LKZ2);
   return c->res_mass;
 }
+
+/* ======================================================================
+ * Minor species and soot (MS = 1): INITSVM C/Initsv.f90:329-380, MSFLOW
+ * C/Msflow.f90:10-169, GSLV3M C/Gslv3m.f90:8-286 with SPIN :292-305,
+ * SOURCEM C/Sourcem.f90:8-122 (the new soot model, ioldsoot = 0),
+ * flx1dm C/Flx1d.f90:114-158, extrm C/Extr.f90:258-305, intpm/intp0m
+ * C/Intp.f90:348-419.  Constants of C/Var_mod.f90:215-235.
+ * ====================================================================== */
+static const double wms[7] = {16.0426, 31.9988, 28.0104, 18.0152, 44.0098, 28.0134, 30.0061};
+static const double ck0_m[5] = {1.66e15, 7.94e14, 2.24e14, 7.94e22, 9.33e14};
+static const double beta_m[5] = {0.0, 0.0, 0.0, 0.0, -0.5};
+static const double odr[5][2] = {{1.46, 0.5217}, {1.6904, 1.57}, {1.0, 0.0}, {0.721, 4.0111}, {1.0, 0.5}};
+static const double erc[5] = {20643.0, 11613.0, 62281.0, 44369.0, 68899.0};
+static const double sml1 = 1.0e-25;
+
+static gfm_ms_ctrl *MSC;
+static double *GFM_, *SMF_, *BSSP_, *SFPSP_, *RRMS_, *AP1_;
+static int MZ, NZ, LZ;
+static double FLXM_[3][7], FLYM_[2][7]; /* FLXM(5,:,2), FLXM(MP-5,:,1), FLXM(MP-3,:,1); FLYM(5,:,2), FLYM(NP-3,:,1) */
+#define GFM(i, j, k, l) GFM_[IX3(i, j, k) + NN * (size_t)((l)-1)]
+#define SMF(i, j, k) SMF_[IX3(i, j, k)]
+#define BSSP(i, j, k) BSSP_[IX3(i, j, k)]
+#define SFPSP(i, j, k) SFPSP_[IX3(i, j, k)]
+#define AP1(i, j, k) AP1_[IX3(i, j, k)]
+#define RRMS(id, jd, kd, n) \
+  RRMS_[(((size_t)(kd)-1) * (size_t)NZ + ((jd)-1)) * (size_t)MZ + ((id)-1) + (size_t)MZ * NZ * LZ * (size_t)((n)-1)]
+
+static int ms_bind(gfm_ms_ctrl *ms, const gfm_ms_arrays *a) {
+  if (!ms || !a || !a->gfm || !a->smf || !a->bs_soot_prev || !a->sfp_soot_prev || !a->rrms) return GFM_E_ARG;
+  MSC = ms;
+  GFM_ = a->gfm; SMF_ = a->smf; BSSP_ = a->bs_soot_prev; SFPSP_ = a->sfp_soot_prev; RRMS_ = a->rrms;
+  MZ = MP / 2; NZ = NP / 2; LZ = LP / 2;
+  static size_t cap = 0;
+  if (cap != NN) {
+    free(AP1_);
+    AP1_ = calloc(NN, sizeof(double));
+    if (!AP1_) return GFM_E_DEVICE;
+    cap = NN;
+  }
+  return GFM_OK;
+}
+
+/* SPIN, C/Gslv3m.f90:292-305: the inflow of species lsp from the six
+ * neighbours through the coefficients AS, in kmol/s */
+static double SPIN(int i, int j, int k, int lsp) {
+  double g11 = AS(i, j, k, 1, 1) * GFM(i - 2, j, k, lsp), g12 = AS(i, j, k, 1, 2) * GFM(i + 2, j, k, lsp);
+  double g21 = AS(i, j, k, 2, 1) * GFM(i, j - 2, k, lsp), g22 = AS(i, j, k, 2, 2) * GFM(i, j + 2, k, lsp);
+  double g31 = AS(i, j, k, 3, 1) * GFM(i, j, k - 2, lsp), g32 = AS(i, j, k, 3, 2) * GFM(i, j, k + 2, lsp);
+  double f = g11 + g12 + g21 + g22 + g31 + g32;
+  return f * C->gmfr / wms[lsp - 1];
+}
+
+/* The reaction rates of GSLV3M, C/Gslv3m.f90:22-115 (IRR0 = 5).  RR2 is a
+ * local of GSLV3M that carries over from cell to cell: a cell whose CH4 or
+ * O2 concentration is at most 1e-25 skips reaction 1 and then reduces the
+ * O2 inflow of reaction 2 by 1.5 times the RR2 the previous open cell left
+ * (its NO rate, reactions 4 + 5).  The reference leaves RR2 undefined at
+ * the first cell of a call; here it starts at 0. */
+static void MS_RATES(void) {
+  const int mi1 = 4, mi3 = MPM2, nj1 = NJY1, nj3 = NJY2, lk1 = LKZ1, lk3 = LKZ2;
+  double rr1, rr2 = 0.0, rr3, rr_f = 0.0, rr_c = 0.0;
+  for (int i = mi1; i <= mi3; i += 2) {
+    int id2 = i / 2;
+    for (int j = nj1; j <= nj3; j += 2) {
+      int jd2 = j / 2;
+      for (int k = lk1; k <= lk3; k += 2) {
+        if (IBCELL(i, j, k) > 0) continue;
+        int kd2 = k / 2;
+        double tmp = T(i, j, k) * C->t0;
+        double dnst1 = DNST(i, j, k) * C->dnst0;
+        double vol = DX(i) * DR(j) * DZ(k) * C->vol0;
+        double gfm1 = GFM(i, j, k, 1) / wms[0] * dnst1, gfm2 = GFM(i, j, k, 2) / wms[1] * dnst1;
+        double gfm3 = GFM(i, j, k, 3) / wms[2] * dnst1, gfm5 = GFM(i, j, k, 5) / wms[4] * dnst1;
+        double gfm6 = GFM(i, j, k, 6) / wms[5] * dnst1;
+        /* reaction 1: CH4 + 1.5 O2 -> CO + 2 H2O */
+        double fxms1 = SPIN(i, j, k, 1), fxms2 = SPIN(i, j, k, 2), fxms3, fxms5;
+        if (gfm1 <= sml1 || gfm2 <= sml1) {
+          RRMS(id2, jd2, kd2, 1) = 0.0;
+        } else {
+          rr1 = pow(gfm1, odr[0][0]) * pow(gfm2, odr[0][1]);
+          rr2 = ck0_m[0] * rr1 * exp(-erc[0] / tmp) * vol;
+          if (rr2 > fxms1) rr2 = fxms1;
+          if (fxms2 / 1.5 < rr2) rr2 = fxms2 / 1.5;
+          RRMS(id2, jd2, kd2, 1) = rr2 * 0.3 + RRMS(id2, jd2, kd2, 1) * 0.7;
+        }
+        /* reaction 2: CO + 0.5 O2 -> CO2 */
+        fxms2 = fxms2 - rr2 * 1.5;
+        if (fxms2 > small20 && gfm3 > small20) {
+          rr1 = pow(gfm3, odr[1][0]) * pow(gfm2, odr[1][1]);
+          rr2 = ck0_m[1] * rr1 * exp(-erc[1] / tmp) * vol;
+          if (fxms2 * 2 < rr2) rr2 = fxms2 * 2;
+          fxms3 = SPIN(i, j, k, 3);
+          rr2 = dmin(rr2, fxms3);
+        } else {
+          rr2 = 0.0;
+        }
+        /* reaction 3: CO2 -> CO + 0.5 O2 (the net CO oxidation rate is RR2 - RR3) */
+        if (gfm5 > small20) {
+          rr1 = pow(gfm5, odr[2][0]);
+          rr3 = ck0_m[2] * rr1 * exp(-erc[2] / tmp) * vol;
+          fxms5 = SPIN(i, j, k, 5);
+          rr3 = dmin(rr3, fxms5);
+        } else {
+          rr3 = 0.0;
+        }
+        rr1 = rr2 - rr3;
+        if (rr1 < 0) rr1 = 0.0;
+        RRMS(id2, jd2, kd2, 2) = rr1 * 0.3 + RRMS(id2, jd2, kd2, 2) * 0.7;
+        rr_f += RRMS(id2, jd2, kd2, 1) * wms[0];
+        rr_c += RRMS(id2, jd2, kd2, 2) * wms[2];
+        /* reactions 4 and 5: N2 + O2 -> 2 NO */
+        if (gfm2 > small20 && gfm3 > small20 && gfm6 > small20) {
+          rr1 = pow(gfm3, odr[3][0]) * pow(gfm2, odr[3][1]) * GFM(i, j, k, 6) / 0.725;
+          rr2 = ck0_m[3] * rr1 * exp(-erc[3] / tmp) * vol;
+        } else {
+          rr2 = 0.0;
+        }
+        if (gfm2 > small20 && gfm6 > small20) {
+          rr1 = pow(gfm6, odr[4][0]) * pow(gfm2, odr[4][1]);
+          rr3 = ck0_m[4] * rr1 * exp(-erc[4] / tmp) * pow(tmp, beta_m[4]);
+          rr3 = rr3 * vol;
+        } else {
+          rr3 = 0.0;
+        }
+        rr2 = rr2 + rr3;
+        RRMS(id2, jd2, kd2, 4) = rr2;
+      }
+    }
+  }
+  MSC->rr_f = rr_f;
+  MSC->rr_c = rr_c;
+}
+
+/* SOURCEM, C/Sourcem.f90:8-122.  nelm 1..7: BS = the species' net
+ * production (kg/s, volume included in the rates) / GMFR, SFP = 0.
+ * nelm 11: the soot formation (BS, per volume) and oxidation (SFP)
+ * sources of the new model, relaxed against the previous call's after the
+ * first minor-species iteration; both arrays are cleared everywhere first
+ * and saved whole as the next call's history. */
+static void SOURCEM(int nelm, int mi1, int mi3, int nj1, int nj3, int lk1, int lk3) {
+  if (nelm != 11) {
+    for (int i = mi1; i <= mi3; i += 2) {
+      int id2 = i / 2;
+      for (int j = nj1; j <= nj3; j += 2) {
+        int jd2 = j / 2;
+        for (int k = lk1; k <= lk3; k += 2) {
+          SFP(i, j, k) = 0.0;
+          BS(i, j, k) = 0.0;
+          if (IBCELL(i, j, k) > 0) continue;
+          int kd2 = k / 2;
+          double g0;
+          switch (nelm) {
+            case 1: g0 = -RRMS(id2, jd2, kd2, 1); break;
+            case 2: g0 = -RRMS(id2, jd2, kd2, 1) * 1.5 - RRMS(id2, jd2, kd2, 2) * 0.5 - RRMS(id2, jd2, kd2, 4); break;
+            case 3: g0 = RRMS(id2, jd2, kd2, 1) - RRMS(id2, jd2, kd2, 2); break;
+            case 4: g0 = RRMS(id2, jd2, kd2, 1) * 2.0; break;
+            case 5: g0 = RRMS(id2, jd2, kd2, 2); break;
+            case 6: g0 = -RRMS(id2, jd2, kd2, 4); break;
+            default: g0 = RRMS(id2, jd2, kd2, 4) * 2.0; break;
+          }
+          BS(i, j, k) = g0 * wms[nelm - 1] / C->gmfr;
+        }
+      }
+    }
+    return;
+  }
+  const double snorm = C->r0 / (C->dnst0 * C->ug0);
+  const double e0_sf = (MSC->t0_sf < small20) ? 0.0 : exp(-MSC->esf_r / MSC->t0_sf);
+  const double e0_so = (MSC->t0_so < small20) ? 0.0 : sqrt(MSC->t0_so) * exp(-MSC->eso_r / MSC->t0_so);
+  memset(BS_, 0, sizeof(double) * NN);
+  memset(SFP_, 0, sizeof(double) * NN);
+  for (int i = mi1; i <= mi3; i += 2)
+    for (int j = nj1; j <= nj3; j += 2)
+      for (int k = lk1; k <= lk3; k += 2) {
+        double tcell = T(i, j, k) * C->t0;
+        if (IBCELL(i, j, k) > 0 || !MSC->react) continue;
+        if (tcell > MSC->t0_sf) {
+          BS(i, j, k) = MSC->aform * GFM(i, j, k, 1) * (exp(-MSC->esf_r / tcell) - e0_sf) * snorm;
+          if (MSC->itr_ms > 1)
+            BS(i, j, k) = MSC->rf_soot_form_source * BS(i, j, k) + (1.0 - MSC->rf_soot_form_source) * BSSP(i, j, k);
+        }
+        if (tcell > MSC->t0_so) {
+          SFP(i, j, k) = -(MSC->aoxid * GFM(i, j, k, 2) * (sqrt(tcell) * exp(-MSC->eso_r / tcell) - e0_so) * snorm);
+          if (MSC->itr_ms > 1)
+            SFP(i, j, k) = MSC->rf_soot_oxid_source * SFP(i, j, k) + (1.0 - MSC->rf_soot_oxid_source) * SFPSP(i, j, k);
+        }
+      }
+  memcpy(BSSP_, BS_, sizeof(double) * NN);
+  memcpy(SFPSP_, SFP_, sizeof(double) * NN);
+}
+
+/* GSLV3M, C/Gslv3m.f90:8-286 */
+static void GSLV3M(void) {
+  const int mi1 = 4, mi3 = MPM2, nj1 = NJY1, nj3 = NJY2, lk1 = LKZ1, lk3 = LKZ2;
+  MS_RATES();
+  for (int l = 1; l <= 7; l++) {
+    for (int i = 2; i <= MP; i += 2)
+      for (int j = 2; j <= NP; j += 2)
+        for (int k = 2; k <= LP; k += 2) FZ(i, j, k) = GFM(i, j, k, l);
+    SOURCEM(l, mi1, mi3, nj1, nj3, lk1, lk3);
+    SGM = MSC->ssigma;
+    if (MSC->igs0 == 0) {
+      GSLV3G0(mi1, mi3, nj1, nj3, lk1, lk3, 0);
+      memcpy(AP1_, AP_, sizeof(double) * NN);
+      MSC->igs0 = 1;
+    } else {
+      memcpy(AP_, AP1_, sizeof(double) * NN);
+    }
+    const double rfm = MSC->rfm[l - 1], rfmc = 1.0 - MSC->rfm[l - 1];
+    for (int i = mi1; i <= mi3; i += 2)
+      for (int j = nj1; j <= nj3; j += 2)
+        for (int k = lk1; k <= lk3; k += 2) {
+          if (IBCELL(i, j, k) >= 1) continue;
+          double vol = VOL_C(i, j, k);
+          AP(i, j, k) = AP(i, j, k) - (SFP(i, j, k) * vol);
+          AP(i, j, k) = AP(i, j, k) / rfm;
+          BS(i, j, k) = BS(i, j, k) + rfmc * FZ(i, j, k) * AP(i, j, k);
+        }
+    MSC->resid_ms_pre[l - 1] = CALC_RESIDUAL(FZ_, mi1, mi3, nj1, nj3, lk1, lk3);
+    ADLBL3_SWEEPS(FZ_, MSC->ntimesm, mi1, mi3, nj1, nj3, lk1, lk3);
+    MSC->resid_ms[l - 1] = CALC_RESIDUAL(FZ_, mi1, mi3, nj1, nj3, lk1, lk3);
+    for (int i = mi1; i <= mi3; i += 2)
+      for (int j = nj1; j <= nj3; j += 2)
+        for (int k = lk1; k <= lk3; k += 2) {
+          if (IBCELL(i, j, k) >= 1) continue;
+          if (FZ(i, j, k) > 1.0) FZ(i, j, k) = 1.0;
+          if (FZ(i, j, k) < 0) FZ(i, j, k) = 0.0;
+          if (l == 3 && FZ(i, j, k) > 0.1) FZ(i, j, k) = 0.1;
+          if (l == 7 && FZ(i, j, k) > 1.0e-2) FZ(i, j, k) = 1.0e-2;
+          GFM(i, j, k, l) = rfmc * GFM(i, j, k, l) + rfm * FZ(i, j, k);
+          if (GFM(i, j, k, l) < small20) GFM(i, j, k, l) = 0.0;
+        }
+  }
+  double bal = 0.0;
+  for (int i = mi1; i <= mi3; i += 2)
+    for (int j = nj1; j <= nj3; j += 2)
+      for (int k = lk1; k <= lk3; k += 2) {
+        if (IBCELL(i, j, k) >= 1) continue;
+        double g0 = 0.0;
+        for (int l = 1; l <= 7; l++) g0 += GFM(i, j, k, l);
+        bal += fabs(1.0 - g0) * VOL_C(i, j, k);
+      }
+  MSC->bal_ms = bal * C->vol0 / MSC->vol_tot;
+
+  if (!MSC->react) return;
+  for (int i = 2; i <= MP; i += 2)
+    for (int j = 2; j <= NP; j += 2)
+      for (int k = 2; k <= LP; k += 2) FZ(i, j, k) = SMF(i, j, k);
+  SOURCEM(11, mi1, mi3, nj1, nj3, lk1, lk3);
+  SGM = MSC->ssigma;
+  GSLV3G0(mi1, mi3, nj1, nj3, lk1, lk3, 1);
+  const double rfm = MSC->rfm[7], rfmc = 1.0 - MSC->rfm[7];
+  for (int i = mi1; i <= mi3; i += 2)
+    for (int j = nj1; j <= nj3; j += 2)
+      for (int k = lk1; k <= lk3; k += 2) {
+        if (IBCELL(i, j, k) >= 1) continue;
+        double vol = VOL_C(i, j, k);
+        BS(i, j, k) = BS(i, j, k) * vol;
+        AP(i, j, k) = AP(i, j, k) - (SFP(i, j, k) * vol);
+        AP(i, j, k) = AP(i, j, k) / rfm;
+        BS(i, j, k) = BS(i, j, k) + rfmc * FZ(i, j, k) * AP(i, j, k);
+      }
+  MSC->resid_ms_pre[7] = CALC_RESIDUAL(FZ_, mi1, mi3, nj1, nj3, lk1, lk3);
+  ADLBL3_SWEEPS(FZ_, MSC->ntimesm, mi1, mi3, nj1, nj3, lk1, lk3);
+  MSC->resid_ms[7] = CALC_RESIDUAL(FZ_, mi1, mi3, nj1, nj3, lk1, lk3);
+  double soot_tot = 0.0, smf_max = 0.0, smf_max_noclip = 0.0;
+  int clipped = 0;
+  for (int i = mi1; i <= mi3; i += 2)
+    for (int j = nj1; j <= nj3; j += 2)
+      for (int k = lk1; k <= lk3; k += 2) {
+        if (IBCELL(i, j, k) >= 1) continue;
+        smf_max_noclip = dmax(smf_max_noclip, FZ(i, j, k));
+        double g0;
+        if (FZ(i, j, k) > MSC->soot_cap) { g0 = MSC->soot_cap; clipped++; }
+        else if (FZ(i, j, k) > small20) g0 = FZ(i, j, k);
+        else g0 = 0.0;
+        smf_max = dmax(smf_max, g0);
+        SMF(i, j, k) = rfm * g0 + rfmc * SMF(i, j, k);
+        double vol = DX(i) * DR(j) * DZ(k);
+        soot_tot = soot_tot + SMF(i, j, k) * DNST(i, j, k) * C->dnst0 * vol;
+      }
+  MSC->soot_tot = soot_tot;
+  MSC->smf_max = smf_max;
+  MSC->smf_max_noclip = smf_max_noclip;
+  MSC->smf_clipped_cnt = clipped;
+}
+
+/* flx1dm, C/Flx1d.f90:114-158: the species flow (kg/s) through x-face
+ * planes (+x in FLXM(:,:,1), -x in (:,:,2)) and y-face planes; kept for the
+ * planes MSFLOW reads (x: 5, MP-5, MP-3; y: 5, NP-3) */
+static void FLX1DM(void) {
+  memset(FLXM_, 0, sizeof FLXM_);
+  memset(FLYM_, 0, sizeof FLYM_);
+  const int xi[3] = {5, MP - 5, MP - 3}, xd[3] = {2, 1, 1}, yj[2] = {5, NP - 3}, yd[2] = {2, 1};
+  for (int i = 3; i <= MPM1; i += 2)
+    for (int j = 4; j <= NPM2; j += 2)
+      for (int k = LKZ1; k <= LKZ2; k += 2) {
+        if (IBCELL(i, j, k) == 1) continue;
+        double area = AREA_C(i, j, k, 1);
+        double g0 = THETA(i, j, k) * DNST(i, j, k) * UG(i, j, k, 1) * area * C->gmfr;
+        int id = (g0 >= 0.0) ? 1 : 2;
+        for (int q = 0; q < 3; q++)
+          if (i == xi[q] && id == xd[q])
+            for (int l = 1; l <= 7; l++) FLXM_[q][l - 1] += fabs(g0) * GFM(i, j, k, l);
+      }
+  for (int j = 3; j <= NPM1; j += 2)
+    for (int i = 4; i <= MPM2; i += 2)
+      for (int k = LKZ1; k <= LKZ2; k += 2) {
+        if (IBCELL(i, j, k) == 1) continue;
+        double area = AREA_C(i, j, k, 2);
+        double g0 = THETA(i, j, k) * DNST(i, j, k) * UG(i, j, k, 2) * area * C->gmfr;
+        int id = (g0 >= 0.0) ? 1 : 2;
+        for (int q = 0; q < 2; q++)
+          if (j == yj[q] && id == yd[q])
+            for (int l = 1; l <= 7; l++) FLYM_[q][l - 1] += fabs(g0) * GFM(i, j, k, l);
+      }
+}
+
+/* extrm, C/Extr.f90:258-305: exit cells extrapolate the species from the
+ * interior (floored at 0; the y-face toward the interior cell takes the
+ * same value) and copy smf; wall cells copy their non-wall neighbour and
+ * the face between.  Wall cells without a non-wall neighbour copy the
+ * cell two steps up in k as it was before this loop reached it. */
+static void EXTRM(void) {
+  int i1 = 2, j1 = 2, k1 = 2, i2 = 2, j2 = 2, k2 = 2, ix = 4;
+  double f1, f2;
+  for (int i = 2; i <= MP; i += 2)
+    for (int j = 2; j <= NP; j += 2)
+      for (int k = 2; k <= LP; k += 2) {
+        if (imod10(IBCELL(i, j, k)) != 3) continue;
+        EXTIDX(0, i, j, k, &i1, &j1, &k1, &i2, &j2, &k2, &ix);
+        EXTR1(&f1, &f2, i, j, k, i1, j1, k1, i2, j2, k2, ix);
+        for (int l = 1; l <= 7; l++) {
+          double g0 = GFM(i1, j1, k1, l) * f1 + GFM(i2, j2, k2, l) * f2;
+          GFM(i, j, k, l) = dmax(g0, 0.0);
+          int jj = (j + j1) / 2;
+          GFM(i, jj, k, l) = GFM(i, j, k, l);
+        }
+        SMF(i, j, k) = SMF(i1, j1, k1);
+      }
+  for (int i = 2; i <= MP; i += 2)
+    for (int j = 2; j <= NP; j += 2)
+      for (int k = 2; k <= LP; k += 2) {
+        if (imod10(IBCELL(i, j, k)) != 1) continue;
+        EXTIDX(1, i, j, k, &i1, &j1, &k1, &i2, &j2, &k2, &ix);
+        if (ix == 4) { i1 = i; j1 = j; k1 = imin(LP, k + 2); } /* EXT1's indices after an unsuccessful search */
+        EXTR1(&f1, &f2, i, j, k, i1, j1, k1, i2, j2, k2, ix);
+        for (int l = 1; l <= 7; l++) {
+          GFM(i, j, k, l) = GFM(i1, j1, k1, l);
+          if (ix == 1) GFM((i + i1) / 2, j, k, l) = GFM(i, j, k, l);
+          else if (ix == 2) GFM(i, (j + j1) / 2, k, l) = GFM(i, j, k, l);
+          else GFM(i, j, (k + k1) / 2, l) = GFM(i, j, k, l);
+        }
+      }
+}
+
+/* intp0m, C/Intp.f90:396-419 */
+static void INTP0M(double *f1, int i, int j, int k, int i1, int j1, int k1, int i2, int j2, int k2) {
+  if (IBCELL(i, j, k) == 3) return;
+  if (IBCELL(i, j, k) == 2) {
+    if (IBCELL(i1, j1, k1) == 2 && IBCELL(i2, j2, k2) == 0) *f1 = 1.0;
+    else if (IBCELL(i1, j1, k1) == 0 && IBCELL(i2, j2, k2) == 2) *f1 = 0.0;
+  }
+  double a = *f1, b = 1.0 - a;
+  for (int l = 1; l <= 7; l++) GFM(i, j, k, l) = a * GFM(i1, j1, k1, l) + b * GFM(i2, j2, k2, l);
+}
+/* intpm, C/Intp.f90:348-390 (the loop nest of INTP) */
+static void INTPM(void) {
+  for (int i = 2; i <= MP; i += 2) {
+    for (int k = 3; k <= LPM1; k += 2) {
+      double f1 = INTP1(i, 0, k, 3);
+      for (int j = 2; j <= NP; j += 2) {
+        if (IBCELL(i, j, k) == 1) continue;
+        INTP0M(&f1, i, j, k, i, j, k + 1, i, j, k - 1);
+      }
+    }
+    for (int j = 3; j <= NPM1; j += 2) {
+      double f1 = INTP1(i, j, 0, 2);
+      for (int k = 2; k <= LP; k++) {
+        if (IBCELL(i, j, k) == 1) continue;
+        INTP0M(&f1, i, j, k, i, j + 1, k, i, j - 1, k);
+      }
+    }
+  }
+  for (int i = 3; i <= MPM1; i += 2) {
+    double f1 = INTP1(i, 0, 0, 1);
+    for (int k = 2; k <= LP; k++)
+      for (int j = 2; j <= NP; j++) {
+        if (IBCELL(i, j, k) == 1) continue;
+        INTP0M(&f1, i, j, k, i + 1, j, k, i - 1, j, k);
+      }
+  }
+}
+
+/* INITSVM, C/Initsv.f90:329-380.  restart = 0: new start; 1: the arrays
+ * hold the restart record (cell centres of GFM and smf, the soot-source
+ * history, RRMS; itr_ms in the control block). */
+static void INITSVM(int restart) {
+  if (!restart) {
+    memset(SMF_, 0, sizeof(double) * NN);
+    memset(GFM_, 0, sizeof(double) * NN * 7);
+    for (int i = 2; i <= MP; i += 2)
+      for (int j = 2; j <= NP; j += 2)
+        for (int k = 2; k <= LP; k += 2) {
+          GFM(i, j, k, 1) = GF(i, j, k, iyf);
+          GFM(i, j, k, 2) = GF(i, j, k, iyo2);
+          GFM(i, j, k, 4) = GF(i, j, k, iyh2o);
+          GFM(i, j, k, 5) = GF(i, j, k, iyco2);
+          GFM(i, j, k, 6) = GF(i, j, k, iyn2);
+        }
+  } else {
+    for (int i = 2; i <= MP; i += 2)
+      for (int j = 2; j <= NP; j += 2)
+        for (int k = 2; k <= LP; k += 2) {
+          if (IBCELL(i, j, k) != 2) continue;
+          GFM(i, j, k, 1) = GF(i, j, k, iyf);
+          GFM(i, j, k, 2) = GF(i, j, k, iyo2);
+          GFM(i, j, k, 4) = GF(i, j, k, iyh2o);
+          GFM(i, j, k, 5) = GF(i, j, k, iyco2);
+          GFM(i, j, k, 6) = GF(i, j, k, iyn2);
+          GFM(i, j, k, 3) = 0.0;
+          GFM(i, j, k, 7) = 0.0;
+        }
+    EXTRM();
+  }
+  MSC->igs0 = 0;
+  INTPM();
+}
+
+/* MSFLOW, C/Msflow.f90:29-162 (without the screen, file and GUI output) */
+static void MSFLOW(void) {
+  double gfm0[7];
+  FLX1DM();
+  for (int l = 0; l < 7; l++) gfm0[l] = FLXM_[0][l] + FLXM_[1][l];
+  MSC->igs0 = 0;
+  MSC->iterations = 0;
+  MSC->converged = 0;
+  for (int it = 1; it <= MSC->maxms; it++) {
+    MSC->itr_ms += 1;
+    MSC->iterations = it;
+    GSLV3M();
+    FLX1DM();
+    double gfmex = 0.0, chg = 0.0;
+    for (int l = 0; l < 7; l++) {
+      double g = FLXM_[0][l] + FLXM_[2][l] + FLYM_[0][l] + FLYM_[1][l];
+      MSC->gfm_0[l] = g;
+      gfmex += g;
+      chg += fabs(g - gfm0[l]);
+      gfm0[l] = g;
+    }
+    MSC->gfmex = gfmex;
+    MSC->ave_chg = chg / 7;
+    double svf = 0.0, g1 = 0.0;
+    for (int i = 2; i <= MP; i += 2)
+      for (int j = 2; j <= NP; j += 2)
+        for (int k = 2; k <= LP; k += 2) {
+          if (IBCELL(i, j, k) != 3) continue;
+          double g0 = AREA_C(i, j, k, 3);
+          if (g0 < small20) g0 = AREA_C(i, j, k, 2);
+          if (g0 < small20) g0 = AREA_C(i, j, k, 1);
+          g1 += g0;
+          svf += SMF(i, j, k) * DNST(i, j, k) * C->dnst0 / MSC->dnst_soot * g0;
+        }
+    MSC->svf_r = svf / g1;
+    int done = 0;
+    if (MSC->itr_ms > 10) {
+      done = 1;
+      for (int l = 0; l < 8; l++)
+        if (MSC->resid_ms[l] > MSC->bmcon) done = 0;
+    }
+    if (done) { MSC->converged = 1; break; }
+    EXTRM();
+    INTPM();
+  }
+}
+
+int oracle_initsvm(gfm_ctrl *c, const gfm_host_arrays *h, gfm_ms_ctrl *ms, const gfm_ms_arrays *a, int restart) {
+  int rc = bind(c, h);
+  if (!rc) rc = ms_bind(ms, a);
+  if (rc) return rc;
+  INITSVM(restart);
+  return GFM_OK;
+}
+int oracle_gslv3m(gfm_ctrl *c, const gfm_host_arrays *h, gfm_ms_ctrl *ms, const gfm_ms_arrays *a) {
+  int rc = bind(c, h);
+  if (!rc) rc = ms_bind(ms, a);
+  if (rc) return rc;
+  GSLV3M();
+  return GFM_OK;
+}
+int oracle_msflow(gfm_ctrl *c, const gfm_host_arrays *h, gfm_ms_ctrl *ms, const gfm_ms_arrays *a) {
+  int rc = bind(c, h);
+  if (!rc) rc = ms_bind(ms, a);
+  if (rc) return rc;
+  MSFLOW();
+  return GFM_OK;
+}
+/* single stages for the pins: 1 the rates (RRMS and rr_f/rr_c), 2 SOURCEM
+ * of species nelm (BS/SFP via oracle_scratch/oracle_sfp), 3 extrm, 4 intpm,
+ * 5 flx1dm -> out[0..34] = FLXM(5,:,2), FLXM(MP-5,:,1), FLXM(MP-3,:,1),
+ * FLYM(5,:,2), FLYM(NP-3,:,1) */
+int oracle_ms_probe(gfm_ctrl *c, const gfm_host_arrays *h, gfm_ms_ctrl *ms, const gfm_ms_arrays *a, int what, int nelm,
+                    double *out) {
+  int rc = bind(c, h);
+  if (!rc) rc = ms_bind(ms, a);
+  if (rc) return rc;
+  switch (what) {
+    case 1: MS_RATES(); break;
+    case 2: SOURCEM(nelm, 4, MPM2, NJY1, NJY2, LKZ1, LKZ2); break;
+    case 3: EXTRM(); break;
+    case 4: INTPM(); break;
+    case 5:
+      FLX1DM();
+      for (int l = 0; l < 7; l++) {
+        out[l] = FLXM_[0][l]; out[7 + l] = FLXM_[1][l]; out[14 + l] = FLXM_[2][l];
+        out[21 + l] = FLYM_[0][l]; out[28 + l] = FLYM_[1][l];
+      }
+      break;
+    default: return GFM_E_ARG;
+  }
+  return GFM_OK;
+}
+size_t oracle_ms_ctrl_size(void) { return sizeof(gfm_ms_ctrl); }

@@ -35,6 +35,9 @@ def lib():
         _lib.oracle_plane_flows.restype = ct.c_int
         _lib.oracle_calc_residual.restype = ct.c_double
         _lib.oracle_ctrl_size.restype = ct.c_size_t
+        _lib.oracle_ms_ctrl_size.restype = ct.c_size_t
+        for f in ("oracle_initsvm", "oracle_gslv3m", "oracle_msflow", "oracle_ms_probe"):
+            getattr(_lib, f).restype = ct.c_int
         _lib.oracle_da.restype = ct.c_double
         _lib.oracle_da.argtypes = [ct.c_double, ct.c_double]
         _lib.oracle_tgamf.restype = ct.c_double
@@ -177,3 +180,41 @@ def zy1(x, y, x1):
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.ascontiguousarray(y, dtype=np.float64)
     return lib().oracle_zy1(_dp(x), _dp(y), float(x1), len(x))
+
+
+# ---- minor species (MSFLOW / GSLV3M / SOURCEM / INITSVM) ----
+def _ms_call(fn, case, ms, *extra):
+    from gfm_amd.abi import host_arrays, ms_arrays
+    h = host_arrays(case.arrays)
+    a = ms_arrays(ms.arrays)
+    rc = getattr(lib(), fn)(ct.byref(case.ctrl), ct.byref(h), ct.byref(ms.ctrl), ct.byref(a), *extra)
+    if rc:
+        raise RuntimeError(f"{fn} failed: {rc}")
+
+
+def initsvm(case, ms, restart=False):
+    """INITSVM on the host state (ms: gfm_amd.minor.MinorSpecies)."""
+    _ms_call("oracle_initsvm", case, ms, int(bool(restart)))
+
+
+def gslv3m(case, ms):
+    """One GSLV3M on the host state.  Its rates read the oracle's AS as the
+    last GFLOW run through the oracle left them."""
+    _ms_call("oracle_gslv3m", case, ms)
+
+
+def msflow(case, ms):
+    _ms_call("oracle_msflow", case, ms)
+
+
+MS_PROBES = {"rates": 1, "sourcem": 2, "extrm": 3, "intpm": 4, "flx1dm": 5}
+
+
+def ms_probe(case, ms, what, nelm=0):
+    """One stage of the minor-species solve: 'rates' (RRMS, rr_f, rr_c),
+    'sourcem' (BS/SFP of species nelm or soot 11, read with scratch()),
+    'extrm', 'intpm', 'flx1dm' (returns the 35 plane flows MSFLOW reads:
+    FLXM(5,:,2), FLXM(MP-5,:,1), FLXM(MP-3,:,1), FLYM(5,:,2), FLYM(NP-3,:,1))."""
+    out = np.zeros(35)
+    _ms_call("oracle_ms_probe", case, ms, MS_PROBES[what], int(nelm), _dp(out))
+    return out
