@@ -86,7 +86,13 @@ enum {
   SC_PG_COS2,  // learned p' guess: fraction of ||BS||^2 the scaled guess removes
   SC_RESID0 = 32,  // 64 residual slots follow
   SC_TEXIT_AREA = 96, SC_TEXIT_SUM, SC_TEXIT_MAX,  // exit cells: area, T*area, max T (SPHASE's exit temperatures)
-  SC_COUNT = 128
+  // minor species: calc_residual before / after each of the 8 solves, the
+  // rate sums, the balance, the soot reductions, the exit soot fraction and
+  // the 5 x 7 plane flows MSFLOW reads
+  SC_MS_RESPRE = 128, SC_MS_RES = 136,
+  SC_MS_RRF = 144, SC_MS_RRC, SC_MS_BAL, SC_MS_SMFMAXNC, SC_MS_SMFMAX, SC_MS_CLIP, SC_MS_SOOT, SC_MS_SVF_A, SC_MS_SVF_S,
+  SC_MS_FLX = 160,
+  SC_COUNT = 256
 };
 
 struct Box {  // lattice index box with a uniform step

@@ -52,8 +52,11 @@ struct Prof {
   int used = 0;
 };
 
+struct MsHost;
 struct gfm_ctx {
   int device = 0;
+  std::vector<int16_t> ibh;  // cell codes, reference layout (lists built after init)
+  MsHost *ms = nullptr;       // minor-species state, at the first gfm_hip_ms_* call
   Prof prof;
   gfm_ctrl ctrl;
   DevCase dc;
@@ -940,6 +943,8 @@ __global__ __launch_bounds__(1024) void k_resid_rows_finalize(const double *part
 // ---------------------------------------------------------------------
 // C-ABI
 // ---------------------------------------------------------------------
+#include "gfm_minor.hip"
+
 extern "C" {
 
 const char *gfm_hip_last_error(void) { return g_err.c_str(); }
@@ -1076,6 +1081,7 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
     }
   }
   // cell codes and INTP weight codes
+  c->ibh.assign(h->ibcell, h->ibcell + nlat);
   HCHECK(hipMemcpy(c->stage16, h->ibcell, nlat * sizeof(int16_t), hipMemcpyHostToDevice));
   hipLaunchKernelGGL(k_scatter_i16, dim3((unsigned)((nlat + 255) / 256)), dim3(256), 0, c->stream, c->stage16, ibd, nlat);
   std::vector<int8_t> code;
@@ -1443,6 +1449,7 @@ int gfm_hip_finalize(gfm_ctx *c) {
   if (c->pin) (void)hipHostFree(c->pin);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   if (g_bound == c) g_bound = nullptr;
+  delete c->ms;
   delete c;
   return GFM_OK;
 }

@@ -14,7 +14,7 @@ import os
 
 import numpy as np
 
-from .abi import GfmCtrl, host_arrays
+from .abi import GfmCtrl, GfmMsCtrl, host_arrays, ms_arrays
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgfm_hip.so")
 _lib = None
@@ -48,6 +48,11 @@ def lib():
         L.gfm_hip_set_pguess.argtypes = [P, P]
         L.gfm_hip_pcorr_pair.argtypes = [P, ct.POINTER(ct.c_float), ct.POINTER(ct.c_float)]
         L.gfm_hip_plane_flows.argtypes = [P, D, D, D, D, D]
+        L.gfm_hip_ms_upload.argtypes = [P, ct.POINTER(GfmMsCtrl), P]
+        L.gfm_hip_ms_download.argtypes = [P, ct.POINTER(GfmMsCtrl), P]
+        L.gfm_hip_initsvm.argtypes = [P, ct.POINTER(GfmMsCtrl), ct.c_int]
+        L.gfm_hip_gslv3m.argtypes = [P, ct.POINTER(GfmMsCtrl)]
+        L.gfm_hip_msflow.argtypes = [P, ct.POINTER(GfmMsCtrl)]
         L.gfm_hip_kernel_times.argtypes = [P, ct.c_int, ct.c_char_p, ct.c_int, D, D, ct.POINTER(ct.c_int), ct.c_int,
                                            ct.POINTER(ct.c_int)]
         _lib = L
@@ -107,6 +112,29 @@ class CombustionSpace:
         _check(lib().gfm_hip_calc_residual(self._ctx, _dp(ap), _dp(as_), _dp(bs), _dp(xt), *map(int, box), _dp(r)),
                "gfm_hip_calc_residual")
         return float(r[0])
+
+    # -- minor species (INITSVM, GSLV3M, MSFLOW) on a gfm_amd.minor.MinorSpecies --
+    def ms_upload(self, ms):
+        a = ms_arrays(ms.arrays)
+        _check(lib().gfm_hip_ms_upload(self._ctx, ct.byref(ms.ctrl), ct.cast(ct.byref(a), ct.c_void_p)), "gfm_hip_ms_upload")
+
+    def ms_download(self, ms):
+        a = ms_arrays(ms.arrays)
+        _check(lib().gfm_hip_ms_download(self._ctx, ct.byref(ms.ctrl), ct.cast(ct.byref(a), ct.c_void_p)),
+               "gfm_hip_ms_download")
+        return ms
+
+    def initsvm(self, ms, restart=False):
+        """INITSVM on the device: a new start from the gas field, or
+        (restart) from the state of the last ms_upload."""
+        _check(lib().gfm_hip_initsvm(self._ctx, ct.byref(ms.ctrl), int(bool(restart))), "gfm_hip_initsvm")
+
+    def gslv3m(self, ms):
+        _check(lib().gfm_hip_gslv3m(self._ctx, ct.byref(ms.ctrl)), "gfm_hip_gslv3m")
+
+    def msflow(self, ms):
+        _check(lib().gfm_hip_msflow(self._ctx, ct.byref(ms.ctrl)), "gfm_hip_msflow")
+        return ms.ctrl.iterations
 
     def set_pguess(self, net):
         """Attach a gfm_amd.pnet.PNet as the learned p' initial guess (None

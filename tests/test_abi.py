@@ -9,7 +9,7 @@ import subprocess
 import pytest
 
 from gfm_amd import solver
-from gfm_amd.abi import GfmCtrl, GfmHostArrays
+from gfm_amd.abi import GfmCtrl, GfmHostArrays, GfmMsArrays, GfmMsCtrl
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -36,13 +36,17 @@ def test_library_exports_header_symbols():
 def test_ctrl_layout():
     lib = solver.lib()
     assert lib.gfm_hip_ctrl_size() == ct.sizeof(GfmCtrl)
-    assert lib.gfm_hip_abi_version() == 3
+    assert lib.gfm_hip_abi_version() == 4
     assert ct.sizeof(GfmHostArrays) == 28 * 8
+    lib.gfm_hip_ms_ctrl_size.restype = ct.c_size_t
+    assert lib.gfm_hip_ms_ctrl_size() == ct.sizeof(GfmMsCtrl)
+    assert ct.sizeof(GfmMsArrays) == 5 * 8
 
 
 def test_oracle_layout():
     import oracle
     assert oracle.lib().oracle_ctrl_size() == ct.sizeof(GfmCtrl)
+    assert oracle.lib().oracle_ms_ctrl_size() == ct.sizeof(GfmMsCtrl)
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
@@ -67,3 +71,20 @@ def test_noalloc_rejects_oversized_box_before_any_device_call():
     assert b"65536" in lib.gfm_hip_last_error()
     rc = f(8, 20, 20, 1, fake, fake, fake, fake, fake, 3, 6, 3, 17, 3, 17, fake, None)  # odd box extent
     assert rc == 1
+
+
+def test_minor_species_entry_points_reject_bad_controls_before_any_device_call():
+    """The gfm_hip_ms_* entry points check the control block first: a null
+    context or a relaxation factor outside (0, 1] is rejected without a
+    GPU."""
+    lib = solver.lib()
+    from gfm_amd import case as cs, minor
+    c = cs.furnace_small(nx=8, ny=6, nz=6, seed=1)
+    m = minor.default_ctrl(c)
+    for f in (lib.gfm_hip_gslv3m, lib.gfm_hip_msflow):
+        f.argtypes = [ct.c_void_p, ct.POINTER(GfmMsCtrl)]
+        assert f(None, ct.byref(m)) == 1
+    bad = m.copy()
+    bad.rfm[3] = 0.0
+    lib.gfm_hip_initsvm.argtypes = [ct.c_void_p, ct.POINTER(GfmMsCtrl), ct.c_int]
+    assert lib.gfm_hip_initsvm(ct.c_void_p(1), ct.byref(bad), 0) == 1
