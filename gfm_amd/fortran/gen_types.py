@@ -7,7 +7,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
-from gfm_amd.abi import GfmCtrl, GfmHostArrays  # noqa: E402
+from gfm_amd.abi import GfmCtrl, GfmHostArrays, GfmMsArrays, GfmMsCtrl  # noqa: E402
 
 
 def ftype(t):
@@ -33,8 +33,10 @@ def emit(cls, fname):
 
 
 lines = ["! Generated by gen_types.py from gfm_amd/abi.py -- do not edit.",
-         "! ISO_C_BINDING mirrors of gfm_ctrl and gfm_host_arrays (include/gfm_hip.h).",
+         "! ISO_C_BINDING mirrors of gfm_ctrl, gfm_host_arrays, gfm_ms_ctrl and gfm_ms_arrays",
+         "! (include/gfm_hip.h).",
          "module gfm_types", "  use iso_c_binding", "  implicit none"]
 lines += emit(GfmCtrl, "gfm_ctrl") + emit(GfmHostArrays, "gfm_host_arrays")
+lines += emit(GfmMsCtrl, "gfm_ms_ctrl") + emit(GfmMsArrays, "gfm_ms_arrays")
 lines += ["end module gfm_types", ""]
 open(os.path.join(HERE, "gfm_types.f90"), "w").write("\n".join(lines))

@@ -11,6 +11,8 @@
 ! the reference calls it (C/Gslv3v.f90:91-94: calc_residual, ADLBL3,
 ! calc_residual, implicit interfaces, NELG from module state):
 !   gfm_comb adlbl3 <state-in> <system-in> <out>
+! and the minor-species solve (INITSVM, MSFLOW, SAV3FM):
+!   gfm_comb msflow <state-in> <ms-ctrl-in> <rs-out> [rs-in]
 ! system-in (stream): int32 NELG, int32 IB,IE,JB,JE,KB,KE, then AP, AS, BS
 ! and XT in the reference layout; out: the two residuals, then XT.
 program gfm_comb
@@ -24,6 +26,10 @@ program gfm_comb
   call get_command_argument(1, arg)
   if (trim(arg) == 'adlbl3') then
     call staged_adlbl3()
+    stop
+  end if
+  if (trim(arg) == 'msflow') then
+    call minor_species()
     stop
   end if
   call get_command_argument(1, fin)
@@ -83,6 +89,34 @@ contains
     open (newunit=u, file=trim(fo), access='stream', form='unformatted', status='replace', action='write')
     write (u) r0, r1, xt
     close (u)
+    call gpu_close()
+  end subroutine
+
+  ! the minor-species solve as the main program runs it after SPHASE
+  ! (C/A0main.f90:219-222): INITSVM, MSFLOW, SAV3FM
+  !   gfm_comb msflow <state-in> <ms-ctrl-in> <rs-out> [rs-in]
+  ! ms-ctrl-in (stream): the gfm_ms_ctrl block; with rs-in INITSVM restarts
+  ! from that record (IRSTYPM = 1).  Prints itr_ms, the residuals and bal_ms.
+  subroutine minor_species()
+    character(len=512) :: fstate, fms, fo
+    integer :: u, l
+    call get_command_argument(2, fstate)
+    call get_command_argument(3, fms)
+    call get_command_argument(4, fo)
+    call read_state(trim(fstate))
+    open (newunit=u, file=trim(fms), access='stream', form='unformatted', status='old', action='read')
+    read (u) msc
+    close (u)
+    if (command_argument_count() >= 5) then
+      call get_command_argument(5, rs_file)
+      IRSTYPM = 1
+    end if
+    call INITSVM
+    call MSFLOW
+    call SAV3FM(trim(fo))
+    write (*, '(a,i8,a,i5)') 'itr_ms', msc%itr_ms, ' iterations', msc%iterations
+    write (*, '(a,8es26.17e3)') 'resid_ms', (msc%resid_ms(l), l = 1, 8)
+    write (*, '(a,es26.17e3)') 'bal_ms', msc%bal_ms
     call gpu_close()
   end subroutine
 

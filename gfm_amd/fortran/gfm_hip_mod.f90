@@ -84,6 +84,35 @@ module gfm_hip_mod
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: flx(*), fly(*), flyh(*), flux_z(*), fluxh_z(*)
     end function
+    ! minor species (INITSVM, GSLV3M, MSFLOW on the device)
+    integer(c_int) function gfm_hip_ms_upload(ctx, ms, a) bind(C, name="gfm_hip_ms_upload")
+      import :: c_int, c_ptr, gfm_ms_ctrl, gfm_ms_arrays
+      type(c_ptr), value :: ctx
+      type(gfm_ms_ctrl), intent(in) :: ms
+      type(gfm_ms_arrays), intent(in) :: a
+    end function
+    integer(c_int) function gfm_hip_ms_download(ctx, ms, a) bind(C, name="gfm_hip_ms_download")
+      import :: c_int, c_ptr, gfm_ms_ctrl, gfm_ms_arrays
+      type(c_ptr), value :: ctx
+      type(gfm_ms_ctrl), intent(inout) :: ms
+      type(gfm_ms_arrays), intent(in) :: a
+    end function
+    integer(c_int) function gfm_hip_initsvm(ctx, ms, restart) bind(C, name="gfm_hip_initsvm")
+      import :: c_int, c_ptr, gfm_ms_ctrl
+      type(c_ptr), value :: ctx
+      type(gfm_ms_ctrl), intent(inout) :: ms
+      integer(c_int), value :: restart
+    end function
+    integer(c_int) function gfm_hip_gslv3m(ctx, ms) bind(C, name="gfm_hip_gslv3m")
+      import :: c_int, c_ptr, gfm_ms_ctrl
+      type(c_ptr), value :: ctx
+      type(gfm_ms_ctrl), intent(inout) :: ms
+    end function
+    integer(c_int) function gfm_hip_msflow(ctx, ms) bind(C, name="gfm_hip_msflow")
+      import :: c_int, c_ptr, gfm_ms_ctrl
+      type(c_ptr), value :: ctx
+      type(gfm_ms_ctrl), intent(inout) :: ms
+    end function
     ! include/gfm_pnet.h: the network of the learned p' guess
     integer(c_int) function gfm_pnet_create(device, ch, max_batch, max_cells, seed, net) bind(C, name="gfm_pnet_create")
       import :: c_int, c_ptr, c_long_long, c_int64_t

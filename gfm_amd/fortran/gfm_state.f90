@@ -11,6 +11,8 @@
 !   GFLOW                                C/Gflow.f90:6     (called C/Sphase.f90:51)
 !   ADLBL3(XT,IB,IE,JB,JE,KB,KE)         C/Adlbl3.f90:10   (called C/Gslv3v.f90:93,133,163, C/Gslv3g.f90:96)
 !   calc_residual(xt,resid,ib,ie,jb,je,kb,ke) C/Adlbl3.f90:239 (called C/Gslv3v.f90:91,94,132,135,164)
+!   INITSVM, MSFLOW, GSLV3M              C/Initsv.f90:329, C/Msflow.f90:10, C/Gslv3m.f90:8
+!                                        (called C/A0main.f90:220-221, C/Sphase.f90:216-217)
 ! A non-zero return from the library ends the run through stop_run, as the
 ! reference's fatal paths do (C/A0main.f90:415-424).
 module gfm_state
@@ -40,6 +42,17 @@ module gfm_state
   ! convg log (C/Sphase.f90:53), written by SPHASE on unit nug when log_convg
   integer :: nug = 0
   logical :: log_convg = .false.
+  ! minor species (MS = 1; C/Var_mod.f90:218-235,824-848): GFM(:,:,:,1..7) =
+  ! CH4, O2, CO, H2O, CO2, N2, NO, the soot mass fraction, the soot-source
+  ! history and the reaction rates; msc carries MAXMS, BMCON, the relaxation
+  ! factors and the soot constants in and itr_ms and the residuals out
+  integer, parameter :: NSP0 = 7, NRM0 = 5
+  real(c_double), allocatable, target :: GFM(:, :, :, :), SMF(:, :, :), BS_SOOT_PREV(:, :, :), SFP_SOOT_PREV(:, :, :)
+  real(c_double), allocatable, target :: RRMS(:, :, :, :)
+  type(gfm_ms_ctrl), target :: msc
+  type(gfm_ms_arrays), target :: marr
+  integer :: IRSTYPM = 0                  ! 1: INITSVM restarts from rs_file when it exists
+  character(len=512) :: rs_file = 'rs.d'  ! the rs####c.d record INITSVM reads and SAV3FM writes
 contains
   subroutine allocate_state(m, n, l, nr)
     integer, intent(in) :: m, n, l, nr
@@ -91,6 +104,59 @@ contains
     write (u) itg
     write (u) itr2
     write (u) irad
+    close (u)
+  end subroutine
+
+  subroutine allocate_ms_state()
+    if (allocated(GFM)) return
+    allocate (GFM(MP, NP, LP, NSP0), SMF(MP, NP, LP), BS_SOOT_PREV(MP, NP, LP), SFP_SOOT_PREV(MP, NP, LP))
+    allocate (RRMS(MP / 2, NP / 2, LP / 2, NRM0))
+    GFM = 0.0d0; SMF = 0.0d0; BS_SOOT_PREV = 0.0d0; SFP_SOOT_PREV = 0.0d0; RRMS = 0.0d0
+    marr%gfm = c_loc(GFM); marr%smf = c_loc(SMF); marr%bs_soot_prev = c_loc(BS_SOOT_PREV)
+    marr%sfp_soot_prev = c_loc(SFP_SOOT_PREV); marr%rrms = c_loc(RRMS)
+  end subroutine
+
+  ! SAV3FM: the minor-species restart record rs####c.d (C/Sav3f.f90:96-116):
+  ! per cell centre (K innermost) GFM(1:NSP0) and smf, then bs_soot_prev,
+  ! sfp_soot_prev, RRMS and itr_ms
+  subroutine SAV3FM(filename)
+    character(len=*), intent(in) :: filename
+    integer :: u, i, j, k, l
+    integer(c_int32_t) :: itr
+    open (newunit=u, file=filename, form='unformatted', access='sequential', status='replace')
+    do i = 2, MP, 2
+      do j = 2, NP, 2
+        do k = 2, LP, 2
+          write (u) (GFM(i, j, k, l), l = 1, NSP0), SMF(i, j, k)
+        end do
+      end do
+    end do
+    write (u) BS_SOOT_PREV
+    write (u) SFP_SOOT_PREV
+    write (u) RRMS
+    itr = msc%itr_ms
+    write (u) itr
+    close (u)
+  end subroutine
+
+  ! the reads of INITSVM's restart branch (C/Initsv.f90:352-372)
+  subroutine read_rs(filename)
+    character(len=*), intent(in) :: filename
+    integer :: u, i, j, k, l
+    integer(c_int32_t) :: itr
+    open (newunit=u, file=filename, form='unformatted', access='sequential', status='old', action='read')
+    do i = 2, MP, 2
+      do j = 2, NP, 2
+        do k = 2, LP, 2
+          read (u) (GFM(i, j, k, l), l = 1, NSP0), SMF(i, j, k)
+        end do
+      end do
+    end do
+    read (u) BS_SOOT_PREV
+    read (u) SFP_SOOT_PREV
+    read (u) RRMS
+    read (u) itr
+    msc%itr_ms = itr
     close (u)
   end subroutine
 
@@ -197,6 +263,37 @@ contains
       ctrl%resg(NELG) = resg(1)
       ctrl%resg(20 + NELG) = resg(2)
     end if
+  end subroutine
+
+  ! INITSVM (C/Initsv.f90:329-380) on the device: a restart from rs_file
+  ! when IRSTYPM = 1 and the file exists, else a new start from the gas field
+  subroutine gpu_initsvm
+    logical :: exst
+    integer(c_int) :: restart
+    if (.not. c_associated(gpu_ctx)) call gpu_open(0)
+    call allocate_ms_state()
+    inquire (file=trim(rs_file), exist=exst)
+    restart = 0
+    if (IRSTYPM /= 0 .and. exst) then
+      call read_rs(trim(rs_file))
+      restart = 1
+    end if
+    call check(gfm_hip_ms_upload(gpu_ctx, msc, marr), 'gfm_hip_ms_upload')
+    call check(gfm_hip_initsvm(gpu_ctx, msc, restart), 'gfm_hip_initsvm')
+    call check(gfm_hip_ms_download(gpu_ctx, msc, marr), 'gfm_hip_ms_download')
+  end subroutine
+
+  ! GSLV3M (C/Gslv3m.f90:8-286) and MSFLOW (C/Msflow.f90:10-169) on the
+  ! device state INITSVM set up; the module arrays are current on return
+  subroutine gpu_gslv3m
+    if (.not. allocated(GFM)) call gpu_initsvm()
+    call check(gfm_hip_gslv3m(gpu_ctx, msc), 'gfm_hip_gslv3m')
+    call check(gfm_hip_ms_download(gpu_ctx, msc, marr), 'gfm_hip_ms_download')
+  end subroutine
+  subroutine gpu_msflow
+    if (.not. allocated(GFM)) call gpu_initsvm()
+    call check(gfm_hip_msflow(gpu_ctx, msc), 'gfm_hip_msflow')
+    call check(gfm_hip_ms_download(gpu_ctx, msc, marr), 'gfm_hip_ms_download')
   end subroutine
 
   ! calc_residual (C/Adlbl3.f90:239-271) of XT with the module's AP, AS, BS

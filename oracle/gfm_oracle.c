@@ -2382,3 +2382,10 @@ int oracle_ms_probe(gfm_ctrl *c, const gfm_host_arrays *h, gfm_ms_ctrl *ms, cons
   return GFM_OK;
 }
 size_t oracle_ms_ctrl_size(void) { return sizeof(gfm_ms_ctrl); }
+/* Zero the oracle's coefficient arrays AP, AS, BS (the state of a device
+ * context before its first assembly), for comparisons with a fresh one. */
+void oracle_reset_coefficients(void) {
+  if (AP_) memset(AP_, 0, sizeof(double) * NN);
+  if (AS_) memset(AS_, 0, sizeof(double) * NN * 6);
+  if (BS_) memset(BS_, 0, sizeof(double) * NN);
+}

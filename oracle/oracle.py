@@ -192,6 +192,16 @@ def _ms_call(fn, case, ms, *extra):
         raise RuntimeError(f"{fn} failed: {rc}")
 
 
+def reset_coefficients(case):
+    """Bind the case and zero the oracle's AP, AS, BS (a fresh device
+    context holds zeros before its first assembly)."""
+    from gfm_amd.abi import host_arrays
+    h = host_arrays(case.arrays)
+    buf = np.zeros(3)
+    lib().oracle_probe(ct.byref(case.ctrl), ct.byref(h), 11, 4, 4, 4, _dp(buf))  # binds (VOL_C probe)
+    lib().oracle_reset_coefficients()
+
+
 def initsvm(case, ms, restart=False):
     """INITSVM on the host state (ms: gfm_amd.minor.MinorSpecies)."""
     _ms_call("oracle_initsvm", case, ms, int(bool(restart)))

@@ -72,15 +72,16 @@ def test_fortran_sphase_matches_ctypes(tmp_path):
 
 
 def test_reference_names_are_external_subroutines():
-    """SPHASE, GFLOW, ADLBL3 and calc_residual are external subroutines (not
-    module procedures), so the reference's implicit-interface call sites
-    (C/A0main.f90:204, C/Sphase.f90:51, C/Gslv3v.f90:91-94,133,163,
-    C/Gslv3g.f90:96) link to them by their plain names."""
+    """SPHASE, GFLOW, ADLBL3, calc_residual, INITSVM, MSFLOW and GSLV3M are
+    external subroutines (not module procedures), so the reference's
+    implicit-interface call sites (C/A0main.f90:204,220-221, C/Sphase.f90:
+    51,216-217, C/Gslv3v.f90:91-94,133,163, C/Gslv3g.f90:96, C/Msflow.f90:51)
+    link to them by their plain names."""
     if not os.path.exists(EXE):
         pytest.skip("gfm_amd/fortran is not built")
     out = subprocess.run(["nm", EXE], capture_output=True, text=True, check=True).stdout
     defined = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
-    for name in ("sphase_", "gflow_", "adlbl3_", "calc_residual_"):
+    for name in ("sphase_", "gflow_", "adlbl3_", "calc_residual_", "initsvm_", "msflow_", "gslv3m_"):
         assert name in defined, name
 
 
@@ -119,3 +120,51 @@ def test_fortran_adlbl3_reference_call_sequence(tmp_path):
     assert r0_f == r0_c and r1_f == r1_c
     assert r1_c < r0_c
     np.testing.assert_array_equal(xt_f, xt_c)
+
+
+@pytest.mark.gpu
+def test_fortran_minor_species_matches_oracle(tmp_path):
+    """INITSVM, MSFLOW and SAV3FM through the Fortran host's external
+    subroutines (C/A0main.f90:219-222) in two processes, a new start and a
+    restart from the record the first wrote (IRSTYPM = 1), against the
+    oracle running the same sequence on the same state (no GFLOW before
+    either: the rates of the first GSLV3M read zero coefficients on both
+    sides).  The rs####c.d files agree with the oracle's arrays to 1e-11
+    (12 iterations each, inside the horizon over which the minor-species
+    iteration keeps rounding differences small)."""
+    import oracle
+    from gfm_amd import minor
+    from gfm_amd import restart as rs
+    assert os.path.exists(EXE), "build gfm_amd/fortran first"
+    base = cs.furnace_small(nx=16, ny=12, nz=10, seed=3)
+    fin = tmp_path / "in.bin"
+    write_state(base, fin)
+    kw = dict(maxms=12, bmcon=1e-30, aform=0.00353, aoxid=30.0)
+    fms = tmp_path / "ms.bin"
+    fms.write_bytes(bytes(minor.MinorSpecies(base, **kw).ctrl))
+    rs1, rs2 = tmp_path / "rs1.d", tmp_path / "rs2.d"
+    for args in ([str(fin), str(fms), str(rs1)], [str(fin), str(fms), str(rs2), str(rs1)]):
+        r = subprocess.run([EXE, "msflow", *args], capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+        print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+        assert r.returncode == 0
+    o = base.clone()
+    oracle.initsv_tail(o)
+    m1 = minor.MinorSpecies(o, **kw)
+    oracle.reset_coefficients(o)
+    oracle.initsvm(o, m1)
+    oracle.msflow(o, m1)
+    m2 = minor.MinorSpecies(o, **kw)
+    m2.load_rs(rs.read_rs(rs1.read_bytes(), *base.shape))
+    oracle.reset_coefficients(o)
+    oracle.initsvm(o, m2, restart=True)
+    oracle.msflow(o, m2)
+    for m, f, itr in ((m1, rs1, 12), (m2, rs2, 24)):
+        rec = rs.read_rs(f.read_bytes(), *base.shape)
+        assert rec.itr_ms == itr == m.ctrl.itr_ms
+        want = rs.rs_records_from(m.arrays["gfm"], m.arrays["smf"], m.arrays["bs_soot_prev"], m.arrays["sfp_soot_prev"],
+                                  m.arrays["rrms"], m.ctrl.itr_ms)
+        for name in ("gfm", "smf", "bs_soot_prev", "sfp_soot_prev", "rrms"):
+            a, b = getattr(want, name), getattr(rec, name)
+            e = float(np.abs(a - b).max() / max(np.abs(a).max(), 1e-300))
+            print(f.name, name, f"{e:.1e}", flush=True)
+            assert e < 1e-11, (f.name, name, e)
