@@ -105,9 +105,11 @@ def configure(case, mode, pcorr_tol):
         k.pcorr_block_corr = bc
 
 
-def throughput(spec_args, mode, net, steps, warmup, pcorr_tol, sync):
+def throughput(spec_args, mode, net, steps, warmup, pcorr_tol, sync, autotune=0):
     """W untimed iterations, then K timed ones (the library synchronises its
-    stream at the end of every iteration, when it reads the residuals)."""
+    stream at the end of every iteration, when it reads the residuals).
+    With autotune > 0 the launch shapes of the per-position box kernels are
+    chosen first (6*autotune untimed iterations); returns their table."""
     from gfm_amd import case as cs
     from gfm_amd.solver import CombustionSpace
     case = cs.furnace_large(**spec_args)
@@ -115,6 +117,7 @@ def throughput(spec_args, mode, net, steps, warmup, pcorr_tol, sync):
     cspace = CombustionSpace(case, device=0)
     if net is not None:
         cspace.set_pguess(net)
+    cspace.tuning = cspace.autotune(autotune) if autotune > 0 else ""
     for _ in range(warmup):
         cspace.gflow()
     sync()
@@ -181,6 +184,8 @@ def main():
                          "learned+FP32 with the block correction); 0 skips them")
     ap.add_argument("--ttr-cap", type=int, default=800,
                     help="iteration cap of the time-to-residual runs (0 skips them)")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="iterations per candidate of the launch-shape autotuner before the warm-up (0: default shapes)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run a torch device copy of this size (reference kernel for rocprof byte counters)")
     args = ap.parse_args()
@@ -215,7 +220,9 @@ def main():
         net = pguess.load(weights, max_cells=n)
     # the workload of the metric's value: plain FP64 (or FP32 sweeps with
     # the reference's p' solve, --precision mixed)
-    cspace, case, hist, dt = throughput(spec_args, args.precision, None, args.steps, args.warmup, args.pcorr_tol, sync)
+    cspace, case, hist, dt = throughput(spec_args, args.precision, None, args.steps, args.warmup, args.pcorr_tol, sync,
+                                        args.autotune)
+    shapes = cspace.tuning
     # per-kernel HIP-event timing (kernels launched one by one on the
     # library's stream) over a short pass right after the timed region
     cspace.kernel_times(1)
@@ -251,6 +258,8 @@ def main():
                                f"stretched furnace {args.nx}x{args.ny}x{args.nz} "
                                f"({n} cells), zebra x-plane order", "cells": n, "res_mass_last": res_final},
         "roofline": roof,
+        "autotune": {"iterations_per_candidate": args.autotune,
+                     "launch_shapes": [ln for ln in shapes.splitlines()[1:]] if shapes else None},
         # the whole iteration against the byte model (9,256 B per cell-update in FP64)
         "iteration_roofline": {"model_bytes_per_cell_update": MODEL_BYTES[args.precision],
                                "achieved": round(MODEL_BYTES[args.precision] * n * steps / dt / 1e9, 1),
@@ -266,7 +275,7 @@ def main():
         return args.pcorr_tol_bc if mode == "mixed_zero_bc" else args.pcorr_tol
 
     for name, mode, nn in others:
-        c2, case2, h2, dt2 = throughput(spec_args, mode, nn, args.steps, args.warmup, tol_of(mode), sync)
+        c2, case2, h2, dt2 = throughput(spec_args, mode, nn, args.steps, args.warmup, tol_of(mode), sync, args.autotune)
         c2.close()
         line[name] = {"value": round(n * len(h2) / dt2 / 1e6, 3), "unit": "Mcell-updates/s",
                       "ms_per_step": round(dt2 / max(len(h2), 1) * 1e3, 3)}

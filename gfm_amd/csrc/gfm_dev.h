@@ -97,11 +97,13 @@ enum {
 
 struct Box {  // lattice index box with a uniform step
   int i0, ni, j0, nj, k0, nk, step;
+  int map;  // thread mapping of tunable box kernels (0: one x row per row of workgroups, 1: (i, j) flattened)
 };
 
 static inline Box make_box(int i0, int i1, int j0, int j1, int k0, int k1, int step) {
   Box b;
   b.i0 = i0; b.j0 = j0; b.k0 = k0; b.step = step;
+  b.map = 0;
   b.ni = (i1 >= i0) ? (i1 - i0) / step + 1 : 0;
   b.nj = (j1 >= j0) ? (j1 - j0) / step + 1 : 0;
   b.nk = (k1 >= k0) ? (k1 - k0) / step + 1 : 0;

@@ -11,6 +11,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -53,8 +54,17 @@ struct Prof {
 };
 
 struct MsHost;
+struct Shape {  // launch shape of a tunable box kernel: thread mapping (Box::map) and workgroup size
+  int map, bs;
+};
 struct gfm_ctx {
   int device = 0;
+  // launch shapes of the tunable box kernels chosen by gfm_hip_autotune
+  // (kernel name -> shape), the candidate under trial (-1: none) and each
+  // kernel's largest workgroup
+  std::map<std::string, Shape> tuned;
+  std::map<std::string, int> tune_maxbs, tune_map0;
+  int tune_cand = -1;
   std::vector<int16_t> ibh;  // cell codes, reference layout (lists built after init)
   MsHost *ms = nullptr;       // minor-species state, at the first gfm_hip_ms_* call
   Prof prof;
@@ -184,6 +194,45 @@ static long long box_blocks_f(const Box &b) { return (((long long)b.ni * b.nj + 
       hipLaunchKernelGGL(kern, box_grid(b), dim3(64), 0, c->stream, b, ##__VA_ARGS__);   \
       if (c->prof.on) prof_end(c, #kern, 0.0);                                            \
     }                                                                                     \
+  } while (0)
+// Box kernels whose launch shape (thread mapping and workgroup size) an
+// autotuner may choose (gfm_hip_autotune): MAP0 is the default mapping
+// (0: box_pt rows, 1: box_ptf flattened) at 64 threads.  Every shape hands
+// each position to one thread, so results do not depend on the shape.
+static const Shape TUNE_CANDS[6] = {{0, 64}, {0, 128}, {0, 256}, {1, 64}, {1, 128}, {1, 256}};
+static dim3 box_grid_bs(const Box &b, int bs) {
+  return b.map ? dim3((unsigned)(((long long)b.ni * b.nj + bs - 1) / bs), b.nk) : dim3((b.ni + bs - 1) / bs, b.nj, b.nk);
+}
+template <typename K>
+static Shape tune_shape(gfm_ctx *c, const char *name, K kern, int map0) {
+  Shape def = {map0, 64};
+  if (c->tune_cand >= 0) {
+    c->tune_map0[name] = map0;
+    auto it = c->tune_maxbs.find(name);
+    int mx = 64;
+    if (it == c->tune_maxbs.end()) {
+      hipFuncAttributes a;
+      if (hipFuncGetAttributes(&a, reinterpret_cast<const void *>(kern)) == hipSuccess) mx = a.maxThreadsPerBlock;
+      c->tune_maxbs[name] = mx;
+    } else {
+      mx = it->second;
+    }
+    const Shape s = TUNE_CANDS[c->tune_cand];
+    return s.bs <= mx ? s : def;
+  }
+  auto it = c->tuned.find(name);
+  return it == c->tuned.end() ? def : it->second;
+}
+#define LAUNCHT(kern, map0, b, ...)                                                           \
+  do {                                                                                       \
+    if ((b).ni > 0 && (b).nj > 0 && (b).nk > 0) {                                            \
+      Box b_ = (b);                                                                          \
+      const Shape sh_ = tune_shape(c, #kern, kern, map0);                                    \
+      b_.map = sh_.map;                                                                      \
+      if (c->prof.on) prof_begin(c);                                                         \
+      hipLaunchKernelGGL(kern, box_grid_bs(b_, sh_.bs), dim3(sh_.bs), 0, c->stream, b_, ##__VA_ARGS__); \
+      if (c->prof.on) prof_end(c, #kern, 0.0);                                               \
+    }                                                                                        \
   } while (0)
 // the same for the flattened box kernels (box_ptf)
 #define LAUNCHF(kern, b, ...)                                                              \
@@ -550,11 +599,11 @@ static void adlbl3_sweeps(gfm_ctx *c, double *xt, int nim, int ib, int ie, int j
     // keep their FP64 values.
     Box ext = make_box(ib - 2, ie + 2, jb - 2, je + 2, kb - 2, ke + 2, 2);
     Box box = make_box(ib, ie, jb, je, kb, ke, 2);
-    LAUNCHF(k_box_narrow, ext, xt, c->xtf);
+    LAUNCHT(k_box_narrow, 1, ext, xt, c->xtf);
     sweeps<float>(c, c->xtf, reinterpret_cast<float *>(c->xt2), reinterpret_cast<float *>(c->xt3),
                   reinterpret_cast<float *>(c->xt4), reinterpret_cast<float *>(c->xt5), reinterpret_cast<float *>(c->xg), nim,
                   ib, ie, jb, je, kb, ke);
-    LAUNCHF(k_box_widen, box, c->xtf, xt);
+    LAUNCHT(k_box_widen, 1, box, c->xtf, xt);
   }
 }
 static void adlbl3(gfm_ctx *c, double *xt, int nelg, int ib, int ie, int jb, int je, int kb, int ke) {
@@ -699,29 +748,29 @@ static int gflow(gfm_ctx *c, int stage) {
   for (int iu1 = 1; iu1 <= 3; iu1++) {
     int nx0 = iu1 == 1, nx1 = iu1 == 2, nx2 = iu1 == 3;
     Box fb = make_box(2 + nx0, MPh - nx0, 2 + nx1, NPh - nx1, 2 + nx2, LPh - nx2, 2);
-    LAUNCHF(k_copy_ug_fz, fb, iu1);
+    LAUNCHT(k_copy_ug_fz, 1, fb, iu1);
   }
-  LAUNCH(k_momentum_assemble<1>, mom[1]);
-  LAUNCH(k_momentum_assemble<2>, mom[2]);
-  LAUNCH(k_momentum_assemble<3>, mom[3]);
-  LAUNCH(k_pressure_assemble, cen);
+  LAUNCHT(k_momentum_assemble<1>, 0, mom[1]);
+  LAUNCHT(k_momentum_assemble<2>, 0, mom[2]);
+  LAUNCHT(k_momentum_assemble<3>, 0, mom[3]);
+  LAUNCHT(k_pressure_assemble, 0, cen);
   if (k.want_pre_residuals) residual(c, d.p, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_P_PRE, 0);
   adlbl3(c, d.p, 4, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2);
   residual(c, d.p, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_P, 0);
-  LAUNCHF(k_bound_p, cen);
+  LAUNCHT(k_bound_p, 1, cen);
   for (int iu1 = 1; iu1 <= 3; iu1++) {
     const Box &b = mom[iu1];
     int ie = b.i0 + 2 * (b.ni - 1), je = b.j0 + 2 * (b.nj - 1), ke = b.k0 + 2 * (b.nk - 1);
-    LAUNCH(k_starred, b, iu1);
+    LAUNCHT(k_starred, 0, b, iu1);
     if (k.want_pre_residuals) residual(c, d.fz, b.i0, ie, b.j0, je, b.k0, ke, RS_U_PRE + iu1 - 1, 0);
     adlbl3(c, d.fz, iu1, b.i0, ie, b.j0, je, b.k0, ke);
     residual(c, d.fz, b.i0, ie, b.j0, je, b.k0, ke, RS_U + iu1 - 1, 0);
-    LAUNCHF(k_bound_store_ug, b, iu1);
+    LAUNCHT(k_bound_store_ug, 1, b, iu1);
   }
   // p'
     LAUNCH(k_pcorr_source, cen, c->part);
     hipLaunchKernelGGL(k_pcorr_finalize, dim3(1), dim3(FIN_NT), 0, c->stream, c->part, box_blocks(cen));
-    LAUNCHF(k_zero_fz, cen_all);
+    LAUNCHT(k_zero_fz, 1, cen_all);
     if (k.pcorr_mode == GFM_PCORR_REFERENCE) {
       adlbl3(c, d.fz, 5, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2);
       residual(c, d.fz, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_PC, 0);
@@ -738,7 +787,7 @@ static int gflow(gfm_ctx *c, int stage) {
     LAUNCH(k_pg_gather, cen, c->dump_x, c->part);
     LAUNCH(k_pg_dump_p, cen, c->dump_p);
   }
-  for (int iu1 = 1; iu1 <= 3; iu1++) LAUNCH(k_vel_correct, mom[iu1], iu1);
+  for (int iu1 = 1; iu1 <= 3; iu1++) LAUNCHT(k_vel_correct, 0, mom[iu1], iu1);
   // EXTRV(0), FLX1D, factor, EXTRV(1)
   Box ex[3] = {make_box(3, MPh - 1, 2, NPh, 2, LPh, 2), make_box(2, MPh, 3, NPh - 1, 2, LPh, 2), make_box(2, MPh, 2, NPh, 3, LPh - 1, 2)};
   {
@@ -756,31 +805,31 @@ static int gflow(gfm_ctx *c, int stage) {
   {
     // IU=1
     Box b;
-    b = make_box(3, MPh - 1, 3, NPh - 1, 2, LPh, 2); LAUNCHF(k_intpv, b, 1, 2, 1, 1, 0);
-    b = make_box(3, MPh - 1, 2, NPh, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 1, 3, 1, 0, 1);
-    b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 1, 3, 1, 1, 1);
+    b = make_box(3, MPh - 1, 3, NPh - 1, 2, LPh, 2); LAUNCHT(k_intpv, 1, b, 1, 2, 1, 1, 0);
+    b = make_box(3, MPh - 1, 2, NPh, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 1, 3, 1, 0, 1);
+    b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 1, 3, 1, 1, 1);
     for (int pj = 0; pj < 2; pj++)
       for (int pk = 0; pk < 2; pk++) {
         b = make_box(4, MPh, 2 + pj, pj ? NPh - 1 : NPh, 2 + pk, pk ? LPh - 1 : LPh, 2);
-        LAUNCHF(k_intpv, b, 1, 1, 0, pj, pk);
+        LAUNCHT(k_intpv, 1, b, 1, 1, 0, pj, pk);
       }
     // IU=2
-    b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 2, 3, 0, 1, 1);
-    b = make_box(3, MPh - 1, 3, NPh - 1, 2, LPh, 2); LAUNCHF(k_intpv, b, 2, 1, 1, 1, 0);
-    b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 2, 1, 1, 1, 1);
+    b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 2, 3, 0, 1, 1);
+    b = make_box(3, MPh - 1, 3, NPh - 1, 2, LPh, 2); LAUNCHT(k_intpv, 1, b, 2, 1, 1, 1, 0);
+    b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 2, 1, 1, 1, 1);
     for (int pi = 0; pi < 2; pi++)
       for (int pk = 0; pk < 2; pk++) {
         b = make_box(2 + pi, pi ? MPh - 1 : MPh, 4, NPh, 2 + pk, pk ? LPh - 1 : LPh, 2);
-        LAUNCHF(k_intpv, b, 2, 2, pi, 0, pk);
+        LAUNCHT(k_intpv, 1, b, 2, 2, pi, 0, pk);
       }
     // IU=3
-    b = make_box(3, MPh - 1, 2, NPh, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 3, 1, 1, 0, 1);
-    b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 3, 2, 0, 1, 1);
-    b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 3, 2, 1, 1, 1);
+    b = make_box(3, MPh - 1, 2, NPh, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 3, 1, 1, 0, 1);
+    b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 3, 2, 0, 1, 1);
+    b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 3, 2, 1, 1, 1);
     for (int pi = 0; pi < 2; pi++)
       for (int pj = 0; pj < 2; pj++) {
         b = make_box(2 + pi, pi ? MPh - 1 : MPh, 2 + pj, pj ? NPh - 1 : NPh, 4, LPh, 2);
-        LAUNCHF(k_intpv, b, 3, 3, pi, pj, 0);
+        LAUNCHT(k_intpv, 1, b, 3, 3, pi, pj, 0);
       }
   }
   // pressure reference
@@ -792,17 +841,17 @@ static int gflow(gfm_ctx *c, int stage) {
   const double sigma[7] = {0.7, 0.7, 0.7, 1.0, 1.0, 0.7, 0.7};
   for (int e = 0; e < 7; e++) {
     int l = Ls[e], nelg = nelgs[e];
-    LAUNCHF(k_copy_gf_fz, cen_all, l);
+    LAUNCHT(k_copy_gf_fz, 1, cen_all, l);
     int igs = 1;
     if (e == 0 || c->sgm != sigma[e]) { c->sgm = sigma[e]; igs = 0; }
     switch (nelg) {
-      case 6: LAUNCH(k_scalar_assemble<6>, cen, l, igs, c->sgm, c->part); break;
-      case 7: LAUNCH(k_scalar_assemble<7>, cen, l, igs, c->sgm, c->part); break;
+      case 6: LAUNCHT(k_scalar_assemble<6>, 0, cen, l, igs, c->sgm, c->part); break;
+      case 7: LAUNCHT(k_scalar_assemble<7>, 0, cen, l, igs, c->sgm, c->part); break;
       case 8: LAUNCH(k_scalar_assemble<8>, cen, l, igs, c->sgm, c->part); break;
-      case 9: LAUNCH(k_scalar_assemble<9>, cen, l, igs, c->sgm, c->part); break;
-      case 10: LAUNCH(k_scalar_assemble<10>, cen, l, igs, c->sgm, c->part); break;
-      case 11: LAUNCH(k_scalar_assemble<11>, cen, l, igs, c->sgm, c->part); break;
-      default: LAUNCH(k_scalar_assemble<12>, cen, l, igs, c->sgm, c->part); break;
+      case 9: LAUNCHT(k_scalar_assemble<9>, 0, cen, l, igs, c->sgm, c->part); break;
+      case 10: LAUNCHT(k_scalar_assemble<10>, 0, cen, l, igs, c->sgm, c->part); break;
+      case 11: LAUNCHT(k_scalar_assemble<11>, 0, cen, l, igs, c->sgm, c->part); break;
+      default: LAUNCHT(k_scalar_assemble<12>, 0, cen, l, igs, c->sgm, c->part); break;
     }
     if (nelg == 8) {
       hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(FIN_NT), 0, c->stream, c->part, (int)box_blocks(cen), 1, SC_QREACT, 0);
@@ -813,23 +862,23 @@ static int gflow(gfm_ctx *c, int stage) {
     if (k.want_pre_residuals) residual(c, d.fz, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_GF_PRE + l - 1, 0);
     adlbl3(c, d.fz, nelg, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2);
     residual(c, d.fz, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_GF + l - 1, 0);
-    LAUNCHF(k_bound_store_gf, cen, l);
+    LAUNCHT(k_bound_store_gf, 1, cen, l);
   }
   c->sgm = 1.0;  // GSLV3V resets SGM at the next iteration
   // properties, T_ave, EXTR(1), INTP
-  LAUNCHF(k_props, cen_all);
+  LAUNCHT(k_props, 1, cen_all);
   LAUNCH(k_tave, cen, c->part);
   hipLaunchKernelGGL(k_tave_finalize, dim3(1), dim3(FIN_NT), 0, c->stream, c->part, (int)box_blocks(cen));
   extr(c);
 
   {
-    Box b = make_box(2, MPh, 2, NPh, 3, LPh - 1, 2); LAUNCH(k_intp<3>, b);
-    b = make_box(2, MPh, 3, NPh - 1, 2, LPh, 2); LAUNCH(k_intp<2>, b);
-    b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intp<2>, b);
+    Box b = make_box(2, MPh, 2, NPh, 3, LPh - 1, 2); LAUNCHT(k_intp<3>, 0, b);
+    b = make_box(2, MPh, 3, NPh - 1, 2, LPh, 2); LAUNCHT(k_intp<2>, 0, b);
+    b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHT(k_intp<2>, 0, b);
     for (int pj = 0; pj < 2; pj++)
       for (int pk = 0; pk < 2; pk++) {
         b = make_box(3, MPh - 1, 2 + pj, pj ? NPh - 1 : NPh, 2 + pk, pk ? LPh - 1 : LPh, 2);
-        LAUNCH(k_intp<1>, b);
+        LAUNCHT(k_intp<1>, 0, b);
       }
   }
   // SPHASE's exit temperatures after the GFLOW call, C/Sphase.f90:74-89
@@ -855,27 +904,27 @@ static int initsv_tail(gfm_ctx *c) {
   const int MPh = k.mp, NPh = k.np, LPh = k.lp;
   // INTPV (same launches as in gflow)
   Box b;
-  b = make_box(3, MPh - 1, 3, NPh - 1, 2, LPh, 2); LAUNCHF(k_intpv, b, 1, 2, 1, 1, 0);
-  b = make_box(3, MPh - 1, 2, NPh, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 1, 3, 1, 0, 1);
-  b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 1, 3, 1, 1, 1);
+  b = make_box(3, MPh - 1, 3, NPh - 1, 2, LPh, 2); LAUNCHT(k_intpv, 1, b, 1, 2, 1, 1, 0);
+  b = make_box(3, MPh - 1, 2, NPh, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 1, 3, 1, 0, 1);
+  b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 1, 3, 1, 1, 1);
   for (int pj = 0; pj < 2; pj++)
-    for (int pk = 0; pk < 2; pk++) { b = make_box(4, MPh, 2 + pj, pj ? NPh - 1 : NPh, 2 + pk, pk ? LPh - 1 : LPh, 2); LAUNCHF(k_intpv, b, 1, 1, 0, pj, pk); }
-  b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 2, 3, 0, 1, 1);
-  b = make_box(3, MPh - 1, 3, NPh - 1, 2, LPh, 2); LAUNCHF(k_intpv, b, 2, 1, 1, 1, 0);
-  b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 2, 1, 1, 1, 1);
+    for (int pk = 0; pk < 2; pk++) { b = make_box(4, MPh, 2 + pj, pj ? NPh - 1 : NPh, 2 + pk, pk ? LPh - 1 : LPh, 2); LAUNCHT(k_intpv, 1, b, 1, 1, 0, pj, pk); }
+  b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 2, 3, 0, 1, 1);
+  b = make_box(3, MPh - 1, 3, NPh - 1, 2, LPh, 2); LAUNCHT(k_intpv, 1, b, 2, 1, 1, 1, 0);
+  b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 2, 1, 1, 1, 1);
   for (int pi = 0; pi < 2; pi++)
-    for (int pk = 0; pk < 2; pk++) { b = make_box(2 + pi, pi ? MPh - 1 : MPh, 4, NPh, 2 + pk, pk ? LPh - 1 : LPh, 2); LAUNCHF(k_intpv, b, 2, 2, pi, 0, pk); }
-  b = make_box(3, MPh - 1, 2, NPh, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 3, 1, 1, 0, 1);
-  b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 3, 2, 0, 1, 1);
-  b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHF(k_intpv, b, 3, 2, 1, 1, 1);
+    for (int pk = 0; pk < 2; pk++) { b = make_box(2 + pi, pi ? MPh - 1 : MPh, 4, NPh, 2 + pk, pk ? LPh - 1 : LPh, 2); LAUNCHT(k_intpv, 1, b, 2, 2, pi, 0, pk); }
+  b = make_box(3, MPh - 1, 2, NPh, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 3, 1, 1, 0, 1);
+  b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 3, 2, 0, 1, 1);
+  b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 3, 2, 1, 1, 1);
   for (int pi = 0; pi < 2; pi++)
-    for (int pj = 0; pj < 2; pj++) { b = make_box(2 + pi, pi ? MPh - 1 : MPh, 2 + pj, pj ? NPh - 1 : NPh, 4, LPh, 2); LAUNCHF(k_intpv, b, 3, 3, pi, pj, 0); }
+    for (int pj = 0; pj < 2; pj++) { b = make_box(2 + pi, pi ? MPh - 1 : MPh, 2 + pj, pj ? NPh - 1 : NPh, 4, LPh, 2); LAUNCHT(k_intpv, 1, b, 3, 3, pi, pj, 0); }
   extr(c);
-  b = make_box(2, MPh, 2, NPh, 3, LPh - 1, 2); LAUNCH(k_intp<3>, b);
-  b = make_box(2, MPh, 3, NPh - 1, 2, LPh, 2); LAUNCH(k_intp<2>, b);
-  b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCH(k_intp<2>, b);
+  b = make_box(2, MPh, 2, NPh, 3, LPh - 1, 2); LAUNCHT(k_intp<3>, 0, b);
+  b = make_box(2, MPh, 3, NPh - 1, 2, LPh, 2); LAUNCHT(k_intp<2>, 0, b);
+  b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHT(k_intp<2>, 0, b);
   for (int pj = 0; pj < 2; pj++)
-    for (int pk = 0; pk < 2; pk++) { b = make_box(3, MPh - 1, 2 + pj, pj ? NPh - 1 : NPh, 2 + pk, pk ? LPh - 1 : LPh, 2); LAUNCH(k_intp<1>, b); }
+    for (int pk = 0; pk < 2; pk++) { b = make_box(3, MPh - 1, 2 + pj, pj ? NPh - 1 : NPh, 2 + pk, pk ? LPh - 1 : LPh, 2); LAUNCHT(k_intp<1>, 0, b); }
   HCHECK(hipGetLastError());
   return GFM_OK;
 }
@@ -1241,6 +1290,62 @@ int gfm_hip_gflow(gfm_ctx *c, gfm_ctrl *k) {
   if (c->ctrl.pcorr_dump) c->pair_ready = 1;
   if (c->ctrl.want_plane_flows) c->flows_formed = 1;
   if (c->prof.on) prof_flush(c);
+  return GFM_OK;
+}
+
+// Autotuner of the box kernels' launch shapes: iters GFLOW iterations per
+// candidate shape (all tunable kernels on the same candidate, each kernel
+// timed with events around its launches), then each kernel keeps its
+// fastest candidate.  The iterations are ordinary iterations of the run
+// (itr_gas advances by 6 * iters): every shape computes the same values.
+int gfm_hip_autotune(gfm_ctx *c, gfm_ctrl *k, int iters, char *report, int report_len) {
+  if (!c || !k || iters < 1) { set_err("null argument or iters < 1"); return GFM_E_ARG; }
+  int rc = bind(c);
+  if (rc) return rc;
+  const int NC = (int)(sizeof TUNE_CANDS / sizeof TUNE_CANDS[0]);
+  std::map<std::string, std::vector<double>> us;
+  const bool prof_was = c->prof.on;
+  if (prof_was) prof_flush(c);
+  std::vector<ProfRec> saved = c->prof.recs;
+  c->tuned.clear();
+  drop_graph(c);
+  for (int cand = 0; cand < NC && rc == GFM_OK; cand++) {
+    c->tune_cand = cand;
+    c->prof.recs.clear();
+    c->prof.on = true;
+    for (int it = 0; it < iters; it++) {
+      k->itr_gas += 1;
+      if ((rc = gfm_hip_gflow(c, k))) break;
+      if (!isfinite(k->res_mass)) { set_err("non-finite res_mass at iteration %d", k->itr_gas); rc = GFM_E_NUMERIC; break; }
+    }
+    prof_flush(c);
+    for (const ProfRec &r : c->prof.recs)
+      if (c->tune_maxbs.count(r.name) && r.launches > 0) {
+        auto &v = us[r.name];
+        v.resize(NC, -1.0);
+        v[cand] = 1e3 * r.ms / (double)r.launches;
+      }
+  }
+  c->tune_cand = -1;
+  c->prof.recs = saved;
+  c->prof.on = prof_was;
+  c->ctrl.itr_gas = k->itr_gas;
+  if (rc) { c->tuned.clear(); drop_graph(c); return rc; }
+  std::string rep = "kernel                          default(us)  best(us)  map  workgroup\n";
+  for (auto &kv : us) {
+    const int mx = c->tune_maxbs[kv.first], map0 = c->tune_map0[kv.first];
+    const int def = map0 ? 3 : 0;
+    int best = def;
+    for (int q = 0; q < NC; q++)
+      if (TUNE_CANDS[q].bs <= mx && kv.second[q] > 0.0 && kv.second[q] < kv.second[best]) best = q;
+    c->tuned[kv.first] = TUNE_CANDS[best];
+    char line[160];
+    snprintf(line, sizeof line, "%-30s %12.2f %9.2f %4d %10d\n", kv.first.c_str(), kv.second[def], kv.second[best],
+             TUNE_CANDS[best].map, TUNE_CANDS[best].bs);
+    rep += line;
+  }
+  if (report && report_len > 0) snprintf(report, (size_t)report_len, "%s", rep.c_str());
+  drop_graph(c);  // the next iteration captures the chosen shapes
   return GFM_OK;
 }
 

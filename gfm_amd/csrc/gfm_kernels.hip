@@ -102,6 +102,14 @@ __device__ __forceinline__ bool box_ptf(const Box &b, int &i, int &j, int &k) {
   return true;
 }
 
+// The mapping of a box kernel whose launch shape an autotuner may choose:
+// b.map 0 = box_pt on box_grid, 1 = box_ptf on box_grid_f, any block size.
+// Each position is handled by one thread either way, so the results do not
+// depend on the choice.
+__device__ __forceinline__ bool box_at(const Box &b, int &i, int &j, int &k) {
+  return b.map ? box_ptf(b, i, j, k) : box_pt(b, i, j, k);
+}
+
 // ---------------- wave-level reductions (blockDim == 64) ----------------
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
@@ -477,7 +485,7 @@ __device__ void ECELL(int i, int j, int k, int iu, double ffl[3][2]) {  // C/Gsl
 // FZ = UG on a face class, C/Gslv3v.f90:26-34
 __global__ void k_copy_ug_fz(Box b, int iu1) {
   int i, j, k;
-  if (!box_ptf(b, i, j, k)) return;
+  if (!box_at(b, i, j, k)) return;
   long long q = lidx(i, j, k);
   g.fz[q] = g.ug[iu1 - 1][q];
 }
@@ -490,7 +498,7 @@ template <int IU1>
 __global__ void k_momentum_assemble(Box b) {
   constexpr int iu1 = IU1;
   int i, j, k;
-  if (!box_pt(b, i, j, k)) return;
+  if (!box_at(b, i, j, k)) return;
   long long q = lidx(i, j, k);
   if (IBC(i, j, k) >= 1) { SBS(q, 0.0); return; }
   // ---- SRC1 ----
@@ -607,7 +615,7 @@ __global__ void k_momentum_assemble(Box b) {
 // C/Source.f90:35-74
 __global__ void k_pressure_assemble(Box b) {
   int i, j, k;
-  if (!box_pt(b, i, j, k)) return;
+  if (!box_at(b, i, j, k)) return;
   long long q = lidx(i, j, k);
   if (IBC(i, j, k) >= 1) { SBS(q, g.p[q]); return; }
   double ax = AREA_C(i, j, k, 1), ay = AREA_C(i, j, k, 2), az = AREA_C(i, j, k, 3);
@@ -640,7 +648,7 @@ __global__ void k_pressure_assemble(Box b) {
 }
 __global__ void k_bound_p(Box b) {
   int i, j, k;
-  if (!box_ptf(b, i, j, k)) return;
+  if (!box_at(b, i, j, k)) return;
   if (IBC(i, j, k) >= 1) return;
   long long q = lidx(i, j, k);
   g.p[q] = BOUND_P(g.p[q]);
@@ -648,7 +656,7 @@ __global__ void k_bound_p(Box b) {
 // starred-momentum right-hand side, C/Gslv3v.f90:116-130
 __global__ void k_starred(Box b, int iu1) {
   int i, j, k;
-  if (!box_pt(b, i, j, k)) return;
+  if (!box_at(b, i, j, k)) return;
   if (IBC(i, j, k) >= 1) return;
   long long q = lidx(i, j, k);
   double ap = CAP(q) / g.rf[iu1 - 1];
@@ -661,7 +669,7 @@ __global__ void k_starred(Box b, int iu1) {
 }
 __global__ void k_bound_store_ug(Box b, int iu1) {
   int i, j, k;
-  if (!box_ptf(b, i, j, k)) return;
+  if (!box_at(b, i, j, k)) return;
   if (IBC(i, j, k) >= 1) return;
   long long q = lidx(i, j, k);
   double f = BOUND_U(g.fz[q]);
@@ -748,13 +756,13 @@ __global__ __launch_bounds__(FIN_NT) void k_pcorr_finalize(const double *part, l
 }
 __global__ void k_zero_fz(Box b) {
   int i, j, k;
-  if (!box_ptf(b, i, j, k)) return;
+  if (!box_at(b, i, j, k)) return;
   FZ(i, j, k) = 0.0;
 }
 // velocity correction, C/Gslv3v.f90:179-204
 __global__ void k_vel_correct(Box b, int iu1) {
   int i, j, k;
-  if (!box_pt(b, i, j, k)) return;
+  if (!box_at(b, i, j, k)) return;
   long long q = lidx(i, j, k);
   const double apq = CAP(q);
   if (IBC(i, j, k) >= 1 || fabs(apq) < SMALL20) return;
@@ -876,7 +884,7 @@ __device__ __forceinline__ double INTP1(int i, int j, int k, int ix) {
 }
 __global__ void k_intpv(Box b, int iu, int ix, int pi, int pj, int pk) {
   int i, j, k;
-  if (!box_ptf(b, i, j, k)) return;
+  if (!box_at(b, i, j, k)) return;
   // box covers a class; parities fixed per launch
   (void)pi; (void)pj; (void)pk;
   int bc = IBC(i, j, k);
@@ -990,7 +998,7 @@ __device__ double WALL_SINK(int i, int j, int k, double &qw, double &qlo, double
 // ======================================================================
 __global__ void k_copy_gf_fz(Box b, int l) {
   int i, j, k;
-  if (!box_ptf(b, i, j, k)) return;
+  if (!box_at(b, i, j, k)) return;
   long long q = lidx(i, j, k);
   g.fz[q] = g.gf[l - 1][q];
 }
@@ -1005,7 +1013,7 @@ __global__ void k_scalar_assemble(Box b, int l, int igs, double sgm, double *par
   const int itr_gas = (int)g.scal[SC_ITR];
   int i, j, k;
   double qr = 0.0, qw = 0.0, qlo = 0.0, qhi = 0.0;
-  if (box_pt(b, i, j, k)) {
+  if (box_at(b, i, j, k)) {
     long long q = lidx(i, j, k);
     const double fz = g.fz[q];
     if (IBC(i, j, k) >= 1) {
@@ -1142,7 +1150,7 @@ __global__ void k_scalar_assemble(Box b, int l, int igs, double sgm, double *par
 }
 __global__ void k_bound_store_gf(Box b, int l) {
   int i, j, k;
-  if (!box_ptf(b, i, j, k)) return;
+  if (!box_at(b, i, j, k)) return;
   if (IBC(i, j, k) >= 1) return;
   long long q = lidx(i, j, k);
   double f = BOUND_GF(g.fz[q], l);
@@ -1153,7 +1161,7 @@ __global__ void k_bound_store_gf(Box b, int l) {
 // property update of open cells, C/Gflow.f90:302-382
 __global__ __launch_bounds__(64) void k_props(Box b) {
   int i, j, k;
-  if (!box_ptf(b, i, j, k)) return;
+  if (!box_at(b, i, j, k)) return;
   if (IBC(i, j, k) >= 1) return;
   long long q = lidx(i, j, k);
   CONC(q);
@@ -1322,7 +1330,7 @@ template <int DIR>
 __global__ void k_intp(Box b) {
   constexpr int dir = DIR;
   int i, j, k;
-  if (!box_pt(b, i, j, k)) return;
+  if (!box_at(b, i, j, k)) return;
   int bc = IBC(i, j, k);
   if (bc == 1 || bc == 3) return;
   long long q = lidx(i, j, k);
@@ -1820,13 +1828,13 @@ __global__ void k_pair_xbound(const T *xt, T *x2, T *x3, T *x4, T *x5, int ib, i
 // values.
 __global__ void k_box_narrow(Box b, const double *src, float *dst) {
   int i, j, k;
-  if (!box_ptf(b, i, j, k)) return;
+  if (!box_at(b, i, j, k)) return;
   long long q = lidx(i, j, k);
   dst[q] = (float)src[q];
 }
 __global__ void k_box_widen(Box b, const float *src, double *dst) {
   int i, j, k;
-  if (!box_ptf(b, i, j, k)) return;
+  if (!box_at(b, i, j, k)) return;
   long long q = lidx(i, j, k);
   if (g.ib[q] >= 1 || fabs((double)g.apf[q]) <= SMALL20) return;
   dst[q] = (double)src[q];

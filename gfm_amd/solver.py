@@ -48,6 +48,7 @@ def lib():
         L.gfm_hip_set_pguess.argtypes = [P, P]
         L.gfm_hip_pcorr_pair.argtypes = [P, ct.POINTER(ct.c_float), ct.POINTER(ct.c_float)]
         L.gfm_hip_plane_flows.argtypes = [P, D, D, D, D, D]
+        L.gfm_hip_autotune.argtypes = [P, ct.POINTER(GfmCtrl), ct.c_int, ct.c_char_p, ct.c_int]
         L.gfm_hip_ms_upload.argtypes = [P, ct.POINTER(GfmMsCtrl), P]
         L.gfm_hip_ms_download.argtypes = [P, ct.POINTER(GfmMsCtrl), P]
         L.gfm_hip_initsvm.argtypes = [P, ct.POINTER(GfmMsCtrl), ct.c_int]
@@ -112,6 +113,14 @@ class CombustionSpace:
         _check(lib().gfm_hip_calc_residual(self._ctx, _dp(ap), _dp(as_), _dp(bs), _dp(xt), *map(int, box), _dp(r)),
                "gfm_hip_calc_residual")
         return float(r[0])
+
+    def autotune(self, iters=1):
+        """gfm_hip_autotune: 6*iters ordinary iterations that time every
+        launch shape of the per-position box kernels; returns the table of
+        choices (text)."""
+        buf = ct.create_string_buffer(8192)
+        _check(lib().gfm_hip_autotune(self._ctx, ct.byref(self.case.ctrl), int(iters), buf, len(buf)), "gfm_hip_autotune")
+        return buf.value.decode()
 
     # -- minor species (INITSVM, GSLV3M, MSFLOW) on a gfm_amd.minor.MinorSpecies --
     def ms_upload(self, ms):

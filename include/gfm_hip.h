@@ -196,6 +196,19 @@ int gfm_hip_gflow(gfm_ctx *ctx, gfm_ctrl *ctrl);
  * *n_done. */
 int gfm_hip_sphase(gfm_ctx *ctx, gfm_ctrl *ctrl, int max_iters, double *res_hist, int *n_done);
 
+/* Autotuner of the launch shapes (thread mapping: one x row of the box per
+ * row of workgroups, or the (i, j) rows of a plane flattened; workgroups of
+ * 64, 128 or 256 threads) of the per-position box kernels (assembly,
+ * interpolation, properties, copies): `iters` SIMPLER iterations per
+ * candidate shape, each kernel timed with events, then each keeps its
+ * fastest shape for the rest of the context's life.  No counterpart in the
+ * reference.  The iterations are ordinary iterations of the run (itr_gas
+ * advances by 6*iters; ctrl carries the last one's outputs): every shape
+ * computes the same values bit for bit.  A table of the choices (kernel,
+ * default and best microseconds per launch, mapping, workgroup size) is
+ * written to report (may be NULL). */
+int gfm_hip_autotune(gfm_ctx *ctx, gfm_ctrl *ctrl, int iters, char *report, int report_len);
+
 /* ADLBL3 on a host-provided system (C/Adlbl3.f90:10-97): AP, AS(…,3,2),
  * BS and XT on the doubled lattice, NTIMESG(nelg) sweeps over the index
  * box [ib..ie]x[jb..je]x[kb..ke] (stride 2, 1-based; 3 <= begin <= end <=
