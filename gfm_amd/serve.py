@@ -7,7 +7,9 @@ processes that share the GPU: the solver binds a case's constants
 per device in each process, so concurrent cases need separate processes;
 the GPU overlaps their kernels.  Workers take cases largest first from a
 shared queue (longest-processing-time order), so the pool drains evenly.
-Reported: aggregate cell-updates/s over the whole sweep and over the cases
+Each case first runs the launch-shape autotuner (--autotune iterations per
+candidate; they count as the case's first iterations).  Reported:
+aggregate cell-updates/s over the whole sweep and over the cases
 that did not stall, per-case wall time to the first iteration below each
 residual level of LEVELS (1e-2 ... 1e-5), the lowest level every
 non-stalled case reached with each case's time to it, and the p50/p95 time
@@ -101,6 +103,12 @@ def _run_case(job, cfg, t_sweep0):
         with CombustionSpace(c) as s:
             if net is not None:
                 s.set_pguess(net)
+            if cfg.get("autotune", 0) > 0 and cfg["cap"] > 6 * cfg["autotune"]:
+                # the autotuner's iterations are ordinary iterations of the case
+                s.autotune(cfg["autotune"])
+                n = 6 * cfg["autotune"]
+                res = float(k.res_mass)
+                hist.append(res)
             while n < cfg["cap"]:
                 h = s.sphase(min(10, cfg["cap"] - n))
                 if len(h) == 0:
@@ -148,7 +156,7 @@ def lowest_common_level(recs):
     return best, {r["index"]: r["first_below"][f"{best:g}"] for r in recs}
 
 
-def serve(cases, workers, cap, precision, log=print, pcorr_tol=0.0, block_corr=False, learned=False):
+def serve(cases, workers, cap, precision, log=print, pcorr_tol=0.0, block_corr=False, learned=False, autotune=0):
     """Run the sweep; returns the summary and the per-case records."""
     ctx = mp.get_context("spawn")
     jobs, results = ctx.Queue(), ctx.Queue()
@@ -156,7 +164,8 @@ def serve(cases, workers, cap, precision, log=print, pcorr_tol=0.0, block_corr=F
         jobs.put(job)
     for _ in range(workers):
         jobs.put(None)
-    cfg = dict(cap=cap, precision=precision, pcorr_tol=pcorr_tol, block_corr=block_corr, learned=learned)
+    cfg = dict(cap=cap, precision=precision, pcorr_tol=pcorr_tol, block_corr=block_corr, learned=learned,
+               autotune=autotune)
     t0w = time.time()
     t0 = time.perf_counter()
     procs = [ctx.Process(target=_worker, args=(jobs, results, cfg, t0w)) for _ in range(workers)]
@@ -184,6 +193,7 @@ def serve(cases, workers, cap, precision, log=print, pcorr_tol=0.0, block_corr=F
     summary = dict(cases=len(cases), failed=len(recs) - len(ok), workers=workers, precision=precision, cap=cap,
                    pcorr_tol=(pcorr_tol or (0.01 if learned else 0.0)) or None,
                    block_correction=bool(block_corr and pcorr_tol), learned_guess=bool(learned),
+                   autotune_iterations_per_candidate=autotune,
                    wall_seconds=round(wall, 3), cell_updates=work,
                    aggregate_mcell_updates_per_s=round(work / wall / 1e6, 3),
                    stalled_cases=[r["index"] for r in ok if r["stalled"]],
@@ -207,6 +217,8 @@ def main(argv=None):
                     help="relax p' to this calc_residual from zero (0: the reference's fixed sweeps)")
     ap.add_argument("--block-corr", action="store_true", help="x-plane block correction before each p' sweep")
     ap.add_argument("--learned", action="store_true", help="the learned p' guess (p' relaxed to --pcorr-tol, 0.01)")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="iterations per candidate of each case's launch-shape autotuner (0: default shapes)")
     a = ap.parse_args(argv)
     if a.workers < 1 or a.workers > 15:
         print("serve: 1..15 worker processes share the GPU", file=sys.stderr)
@@ -216,7 +228,7 @@ def main(argv=None):
         return 2
     summary, recs = serve(sweep(a.cases, a.seed), a.workers, a.cap, a.precision,
                           log=lambda m: print(m, flush=True), pcorr_tol=a.pcorr_tol, block_corr=a.block_corr,
-                          learned=a.learned)
+                          learned=a.learned, autotune=a.autotune)
     print(json.dumps({"summary": summary, "cases": recs}))
     return 0
 
