@@ -109,6 +109,49 @@ def ms_arrays(arrs):
     return a
 
 
+NWL_MAX = 16  # GFM_NWL_MAX
+RADSRC_NONE, RADSRC_GRAY, RADSRC_FIELD = 0, 1, 2  # GFM_RADSRC_*
+
+
+class GfmRadCtrl(ct.Structure):
+    """gfm_rad_ctrl: inputs and outputs of a radiation pass (RAD_EMIS/RAD0/QRNF)."""
+    _fields_ = [
+        ("nwl", I32), ("have_crown", I32), ("wl", D * NWL_MAX), ("ri", D),
+        ("eps_w", D), ("eps_m", D), ("eps_c", D), ("height_to_ceiling", D), ("dnst_soot", D),
+        ("l_ab", I32), ("n_emit", I32), ("carry_unset", I32), ("angle_oob", I32),
+        ("tot_qe_h2o_co2", D), ("tot_qe_soot", D), ("qe_sum_from_open", D),
+        ("qa_sum_to_melt", D), ("qa_sum_to_wall", D), ("qa_sum_to_open", D), ("qa_sum_to_exit", D),
+        ("qa_sum_to_inlet", D), ("qeg0", D), ("qeg_g", D), ("qeg_w", D), ("qeg_s", D), ("qeg_io", D),
+    ]
+
+    def copy(self):
+        c = GfmRadCtrl()
+        ct.memmove(ct.byref(c), ct.byref(self), ct.sizeof(GfmRadCtrl))
+        return c
+
+
+class GfmRadArrays(ct.Structure):
+    _fields_ = [("qe", DP), ("qa", DP), ("qal", DP), ("akl", DP)]
+
+
+RAD_FIELDS = ("qe", "qa", "qal", "akl")
+RAD_SUMS = ("tot_qe_h2o_co2", "tot_qe_soot", "qe_sum_from_open", "qa_sum_to_melt", "qa_sum_to_wall",
+            "qa_sum_to_open", "qa_sum_to_exit", "qa_sum_to_inlet", "qeg0", "qeg_g", "qeg_w", "qeg_s", "qeg_io")
+
+
+def rad_arrays(arrs):
+    """GfmRadArrays pointing at the numpy arrays of a radiation state (a
+    missing or None entry is a null pointer)."""
+    a = GfmRadArrays()
+    for k in RAD_FIELDS:
+        v = arrs.get(k)
+        if v is None:
+            continue
+        assert v.dtype == np.float64 and v.flags["F_CONTIGUOUS"], k
+        setattr(a, k, v.ctypes.data_as(DP))
+    return a
+
+
 def _ptr(a, ctype=ct.c_double):
     assert a.flags["F_CONTIGUOUS"] or a.ndim == 1, "arrays must be Fortran-ordered"
     return a.ctypes.data_as(ct.POINTER(ctype))

@@ -709,3 +709,51 @@ def furnace_large(nx=192, ny=96, nz=64, seed=2, noise=0.01, maxgi=2000, bgcon=1.
                     melt_bottom_T=lambda xx: 1450.0 + 20.0 * math.sin(2.0 * math.pi * xx / Lx + phase),
                     t_wall=1600.0 if walls == "isothermal" else None, gray_kappa=gray_kappa, exit_yz=exit_port)
     return build(spec)
+
+
+def from_grid(grid, gmfr):
+    """A case on a reference grid file (gridfile.read_gd): the lattice,
+    the cell codes and the &GRD factors of the file, SETUP's scales for the
+    mass flow gmfr (kg/s, the sbc file's GMFR), uniform fields (T = T0,
+    rho = rho0).  The inlet fills of SBC are not applied: this holds the
+    geometry on which a saved state (restart.apply_records, the rs record)
+    is read, e.g. for a radiation pass on a reference run's state."""
+    from .abi import GfmCtrl
+    from .gridfile import grd_factors
+    mp, np_, lp = grid.shape
+    shp = (mp, np_, lp)
+    dnst0 = PG0 * WT0 / RU / T0
+    ug0 = gmfr / dnst0 / 1.0
+    tk0 = CTEA * ug0 ** 2
+    epsi0 = CTDA * tk0 ** 2
+    c = GfmCtrl()
+    c.mp, c.np, c.lp, c.nr0 = mp, np_, lp, 100
+    c.ncells = grid.ncells
+    c.sweep_order, c.precision = 1, 0
+    c.t0, c.trdg, c.h_0 = T0, TL0 / T0, H_0
+    c.r0 = c.area0 = c.vol0 = 1.0
+    c.reyg = 2.0 * dnst0 * ug0 / GMU_REF
+    c.reyg2 = c.reyg / 2.0
+    c.eun = PG0 / (dnst0 * ug0 ** 2)
+    c.epsr = tk0 / ug0 ** 2
+    c.taur = epsi0 / (tk0 * ug0)
+    c.ug0, c.dnst0, c.gmfr, c.tk0, c.epsi0 = ug0, dnst0, gmfr, tk0, epsi0
+    c.qr0 = H_0 * dnst0 * ug0
+    bv, ba = grd_factors(grid.grd)
+    for q in range(2):
+        c.bv_f[q] = bv[q]
+    for q in range(6):
+        c.ba_f[q] = ba[q]
+    c.sigma_sb = 5.668e-8
+    c.factor = 1.0
+    arr = {"x": grid.x.copy(), "dx": grid.dx.copy(), "r": grid.r.copy(), "dr": grid.dr.copy(), "rs": np.ones(np_),
+           "z": grid.z.copy(), "dz": grid.dz.copy(), "ibcell": np.array(grid.ibcell, dtype=np.int16, order="F")}
+    for k in FIELD3:
+        arr[k] = np.zeros(shp, order="F")
+    arr["dnst"][:] = 1.0
+    arr["t"][:] = 1.0
+    arr["theta"][:] = 1.0
+    arr["ug"] = np.zeros(shp + (3,), order="F")
+    arr["gf"] = np.zeros(shp + (9,), order="F")
+    arr.update(load_kinetic(c.nr0))
+    return Case(c, arr, None)

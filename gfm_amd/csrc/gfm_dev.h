@@ -22,6 +22,7 @@
 // colour touches one.
 #pragma once
 #include <hip/hip_runtime.h>
+#include "../../include/gfm_hip.h"
 #include <stdint.h>
 
 struct DevGeom {
@@ -71,6 +72,9 @@ struct DevCase {
   int i1st_rad_done, isothermal_walls, gray_radiation;
   double wt0, stoich;
   double bv_f[2], ba_f[6];  // half-blocked cells: volume and face-area factors (&GRD BV_F, BA_F)
+  // QA and QE (W) of the last radiation pass on the cell centres, (id-1) + MZ*((jd-1) + NZ*(kd-1));
+  // null until one has run (enthalpy source GFM_RADSRC_FIELD)
+  const double *rad_qa, *rad_qe;
 };
 
 // device scalar block indices

@@ -54,6 +54,7 @@ struct Prof {
 };
 
 struct MsHost;
+struct RadHost;
 struct Shape {  // launch shape of a tunable box kernel: thread mapping (Box::map) and workgroup size
   int map, bs;
 };
@@ -67,6 +68,7 @@ struct gfm_ctx {
   int tune_cand = -1;
   std::vector<int16_t> ibh;  // cell codes, reference layout (lists built after init)
   MsHost *ms = nullptr;       // minor-species state, at the first gfm_hip_ms_* call
+  RadHost *rad = nullptr;     // radiation fields, at the first gfm_hip_rad_* call
   Prof prof;
   gfm_ctrl ctrl;
   DevCase dc;
@@ -993,6 +995,7 @@ __global__ __launch_bounds__(1024) void k_resid_rows_finalize(const double *part
 // C-ABI
 // ---------------------------------------------------------------------
 #include "gfm_minor.hip"
+#include "gfm_rad.hip"
 
 extern "C" {
 
@@ -1254,6 +1257,20 @@ int gfm_hip_gflow(gfm_ctx *c, gfm_ctrl *k) {
   c->ctrl.want_pre_residuals = k->want_pre_residuals;
   c->ctrl.want_irc = k->want_irc;
   c->ctrl.want_plane_flows = k->want_plane_flows;
+  if (k->gray_radiation != c->ctrl.gray_radiation || k->i1st_rad_done != c->ctrl.i1st_rad_done) {
+    // the radiation source and the wall treatment of EXTR change after the first radiation pass
+    if (k->gray_radiation < GFM_RADSRC_NONE || k->gray_radiation > GFM_RADSRC_FIELD) { set_err("bad gray_radiation"); return GFM_E_ARG; }
+    if (k->gray_radiation == GFM_RADSRC_FIELD && !c->dc.rad_qa) {
+      set_err("gray_radiation = GFM_RADSRC_FIELD needs a radiation pass (gfm_hip_rad_emis) or gfm_hip_rad_upload first");
+      return GFM_E_STATE;
+    }
+    c->ctrl.gray_radiation = k->gray_radiation;
+    c->ctrl.i1st_rad_done = k->i1st_rad_done;
+    c->dc.gray_radiation = k->gray_radiation;
+    c->dc.i1st_rad_done = k->i1st_rad_done;
+    g_bound = nullptr;
+    if ((rc = bind(c))) return rc;
+  }
   if (k->pcorr_mode != GFM_PCORR_REFERENCE && k->pcorr_mode != GFM_PCORR_TOLERANCE) { set_err("bad pcorr_mode"); return GFM_E_ARG; }
   if (k->pcorr_mode == GFM_PCORR_TOLERANCE && (k->pcorr_max_sweeps < 0 || !(k->pcorr_tol >= 0.0))) {
     set_err("pcorr_max_sweeps must be >= 0 and pcorr_tol >= 0");
@@ -1555,6 +1572,7 @@ int gfm_hip_finalize(gfm_ctx *c) {
   if (c->stream) (void)hipStreamDestroy(c->stream);
   if (g_bound == c) g_bound = nullptr;
   delete c->ms;
+  delete c->rad;
   delete c;
   return GFM_OK;
 }

@@ -1077,7 +1077,12 @@ __global__ void k_scalar_assemble(Box b, int l, int igs, double sgm, double *par
         sfp = -0.0;
         double vol0 = VOL_C(i, j, k) * g.vol0;
         qr = hr * g.qr0 * vol0;
-        if (g.gray_radiation) bs += GRAY_QNET(i, j, k) / (vol0 * g.qr0);
+        if (g.gray_radiation == GFM_RADSRC_GRAY) {
+          bs += GRAY_QNET(i, j, k) / (vol0 * g.qr0);
+        } else if (g.gray_radiation == GFM_RADSRC_FIELD) {  // q_rad = QA - QE, C/Source.f90:437-447
+          const long long cc = (long long)(i / 2 - 1) + (long long)(MP / 2) * ((j / 2 - 1) + (long long)(NP / 2) * (k / 2 - 1));
+          bs += (g.rad_qa[cc] - g.rad_qe[cc]) / (vol0 * g.qr0);
+        }
         if (g.isothermal_walls) bs -= WALL_SINK(i, j, k, qw, qlo, qhi);
         if (itr_gas > 1) bs = g.rf_hsource * bs + (1.0 - g.rf_hsource) * g.bsh[q];
         g.bsh[q] = bs;

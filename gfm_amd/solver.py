@@ -14,7 +14,7 @@ import os
 
 import numpy as np
 
-from .abi import GfmCtrl, GfmMsCtrl, host_arrays, ms_arrays
+from .abi import GfmCtrl, GfmMsCtrl, GfmRadCtrl, host_arrays, ms_arrays, rad_arrays
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgfm_hip.so")
 _lib = None
@@ -54,6 +54,11 @@ def lib():
         L.gfm_hip_initsvm.argtypes = [P, ct.POINTER(GfmMsCtrl), ct.c_int]
         L.gfm_hip_gslv3m.argtypes = [P, ct.POINTER(GfmMsCtrl)]
         L.gfm_hip_msflow.argtypes = [P, ct.POINTER(GfmMsCtrl)]
+        L.gfm_hip_rad_emis.argtypes = [P, ct.POINTER(GfmRadCtrl)]
+        L.gfm_hip_rad_qe_only.argtypes = [P, ct.POINTER(GfmRadCtrl)]
+        L.gfm_hip_rad_download.argtypes = [P, ct.POINTER(GfmRadCtrl), P]
+        L.gfm_hip_rad_upload.argtypes = [P, P]
+        L.gfm_hip_rad_ctrl_size.restype = ct.c_size_t
         L.gfm_hip_kernel_times.argtypes = [P, ct.c_int, ct.c_char_p, ct.c_int, D, D, ct.POINTER(ct.c_int), ct.c_int,
                                            ct.POINTER(ct.c_int)]
         _lib = L
@@ -144,6 +149,32 @@ class CombustionSpace:
     def msflow(self, ms):
         _check(lib().gfm_hip_msflow(self._ctx, ct.byref(ms.ctrl)), "gfm_hip_msflow")
         return ms.ctrl.iterations
+
+    # ---- radiation transport (RAD_EMIS / rad_qe_only) ----
+    def rad_emis(self, rad, download=True):
+        """One full radiation pass on the device state (the minor species
+        must be on the device: ms_upload or initsvm first).  rad:
+        gfm_amd.radiation.Radiation, whose ctrl receives the sums and, with
+        download, whose arrays receive QE, QA, QAL and AKL."""
+        _check(lib().gfm_hip_rad_emis(self._ctx, ct.byref(rad.ctrl)), "gfm_hip_rad_emis")
+        return self.rad_download(rad) if download else rad
+
+    def rad_qe_only(self, rad, download=True):
+        """rad_qe_only: QE of every open cell (rad.ctrl.qeg0 its sum)."""
+        _check(lib().gfm_hip_rad_qe_only(self._ctx, ct.byref(rad.ctrl)), "gfm_hip_rad_qe_only")
+        return self.rad_download(rad) if download else rad
+
+    def rad_download(self, rad):
+        a = rad_arrays(rad.arrays)
+        _check(lib().gfm_hip_rad_download(self._ctx, ct.byref(rad.ctrl), ct.cast(ct.byref(a), ct.c_void_p)),
+               "gfm_hip_rad_download")
+        return rad
+
+    def rad_upload(self, rad):
+        """QE and QA of rad.arrays onto the device (the fields of a saved
+        radiation pass) for the enthalpy source GFM_RADSRC_FIELD."""
+        a = rad_arrays(rad.arrays)
+        _check(lib().gfm_hip_rad_upload(self._ctx, ct.cast(ct.byref(a), ct.c_void_p)), "gfm_hip_rad_upload")
 
     def set_pguess(self, net):
         """Attach a gfm_amd.pnet.PNet as the learned p' initial guess (None

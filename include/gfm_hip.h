@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define GFM_ABI_VERSION 4
+#define GFM_ABI_VERSION 5
 
 enum {
   GFM_OK = 0,
@@ -116,7 +116,7 @@ typedef struct gfm_ctrl {
   int32_t want_irc;               /* IRC residual block of ADLBL3 (RESG) */
   int32_t i1st_rad_done;          /* 0: adiabatic walls, C/Extr.f90:194 */
   int32_t isothermal_walls;       /* 1: wall heat sink of C/Source.f90:473-509 with fixed wall T */
-  int32_t gray_radiation;         /* 1: gray emission/absorption source term */
+  int32_t gray_radiation;         /* GFM_RADSRC_*: radiation source of the enthalpy equation */
   int32_t pcorr_block_corr;       /* 1: x-plane additive correction before each sweep of a p' solve
                                      relaxed to a tolerance (GFM_PCORR_TOLERANCE; not in the reference) */
   /* iteration state */
@@ -362,6 +362,73 @@ int gfm_hip_gslv3m(gfm_ctx *ctx, gfm_ms_ctrl *ms);
  * GSLV3M, flx1dm, the exit soot fraction, the convergence test (itr_ms >
  * 10 and every resid_ms <= bmcon) and extrm + intpm. */
 int gfm_hip_msflow(gfm_ctx *ctx, gfm_ms_ctrl *ms);
+
+/* ---------------------------------------------------------------------
+ * Radiation transport (ID_RAD > 0): one full radiation pass of RAD_EMIS
+ * (C/Rad_emis.f90:130-229) up to the wall exchange.  The walls with an open
+ * neighbour are marked by their normal (C/Rad_emis.f90:155-186); RAD0
+ * (C/Rad0.f90:12-172) forms the spectral absorption coefficients AKL of
+ * every open cell from H2O, CO2 and soot (RAD_GAS/AH2O/ACO2/A1/CHIWHI/
+ * BANDA C/Rad_gas.f90:16-396, SOOTA/SOC C/Rad_soot.f90:26-69) and the
+ * band fractions of its emission (DEBX/RIF C/Rad_fun.f90:45-140); the
+ * volume loop of QRNF (C/Qrn.f90:58-200) traces every open emitter above
+ * 300 K through the enclosure (QREF C/Qre.f90:31-505 with QRTX, QRTY,
+ * QRTZ): its band powers leave the six faces, cross the domain slab by
+ * slab, spread across each slab by the angles the slab's far face
+ * subtends, are absorbed in open cells in proportion to AKL and deposited
+ * whole in boundary cells.  QRN1 (C/Qrn.f90:245-290) sums QAL over the
+ * bands into QA.  The emission-only pass rad_qe_only (C/Rad_emis.f90:
+ * 411-466) is RAD0 with ID_RAD = 1: QE of every open cell, QA kept.
+ * The gas state, the minor species H2O and CO2 and the soot fraction are
+ * the context's (gfm_hip_ms_upload / gfm_hip_initsvm before the first
+ * call).  QE and QA stay on the device for the enthalpy source
+ * (gray_radiation = GFM_RADSRC_FIELD: C/Source.f90:437-447).
+ * ------------------------------------------------------------------- */
+#define GFM_NWL_MAX 16
+enum {
+  GFM_RADSRC_NONE = 0,   /* gfm_ctrl.gray_radiation: no radiation source */
+  GFM_RADSRC_GRAY = 1,   /* the gray closure (reference extension) */
+  GFM_RADSRC_FIELD = 2   /* QA - QE of the last radiation pass, C/Source.f90:437-447 */
+};
+typedef struct gfm_rad_ctrl {
+  /* inputs: &RADIN and the band list of the sbc file (C/Sbc.f90:316-355) */
+  int32_t nwl;                 /* NWL: spectral bands (1..GFM_NWL_MAX) */
+  int32_t have_crown;          /* 1: a crown (walls above height_to_ceiling take eps_c) */
+  double wl[GFM_NWL_MAX];      /* WL: band wavelengths (m), the listed microns times 1e-6 */
+  double ri;                   /* RI: refraction index */
+  double eps_w, eps_m, eps_c;  /* wall, melt-surface and crown emissivity (their AKL) */
+  double height_to_ceiling;    /* z (m) from which walls take eps_c */
+  double dnst_soot;            /* soot density (kg/m3), C/Var_mod.f90:137 */
+  /* outputs */
+  int32_t l_ab;                /* open cells counted by QRNF */
+  int32_t n_emit;              /* emitters traced (open, T > 300 K) */
+  int32_t carry_unset;         /* wall splits on an emitter's own column whose column the reference would
+                                  carry over from an earlier emitter (not reproduced: 0 expected) */
+  int32_t angle_oob;           /* angle reads past the lattice (boundary cells that are open; 0 expected) */
+  double tot_qe_h2o_co2;       /* RAD0: gas emission (W) */
+  double tot_qe_soot;          /* RAD0: soot emission (W) */
+  double qe_sum_from_open;     /* QRNF: emission of the traced emitters (W) */
+  double qa_sum_to_melt, qa_sum_to_wall, qa_sum_to_open, qa_sum_to_exit, qa_sum_to_inlet;  /* (W) */
+  double qeg0;                 /* QE of the open cells (W); emission-only pass: the same from RAD0 */
+  double qeg_g, qeg_w, qeg_s, qeg_io;  /* QRN1: QA of open, wall, melt-surface, inlet and exit cells (W) */
+} gfm_rad_ctrl;
+
+/* Host arrays of the radiation fields, reference layout (cell centres):
+ * qe, qa (MP/2,NP/2,LP/2) in W; qal, akl (MP/2,NP/2,LP/2,nwl). */
+typedef struct gfm_rad_arrays {
+  double *qe, *qa, *qal, *akl;
+} gfm_rad_arrays;
+
+/* The full pass (wall marks, RAD0 with ID_RAD = 10, QRNF's volume loop,
+ * QRN1).  Outputs in rc; QE, QA, QAL and AKL stay on the device. */
+int gfm_hip_rad_emis(gfm_ctx *ctx, gfm_rad_ctrl *rc);
+/* rad_qe_only: QE of every open cell from RAD0 with ID_RAD = 1 (qeg0). */
+int gfm_hip_rad_qe_only(gfm_ctx *ctx, gfm_rad_ctrl *rc);
+/* Copy the device's QE, QA, QAL and AKL to the host (any pointer may be
+ * null) / QE and QA from the host (a radiation record, rr####c.d). */
+int gfm_hip_rad_download(gfm_ctx *ctx, const gfm_rad_ctrl *rc, gfm_rad_arrays *a);
+int gfm_hip_rad_upload(gfm_ctx *ctx, const gfm_rad_arrays *a);
+size_t gfm_hip_rad_ctrl_size(void);
 
 /* Per-kernel timing with HIP events recorded on the library's stream around
  * each launch.  enable=1 clears the records and starts recording, enable=0

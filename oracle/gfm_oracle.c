@@ -475,6 +475,9 @@ double oracle_emf(double xkl, double xl0) {
  * T_wall with the same emissivity (Kirchhoff), so QA-QE =
  * SA*sigma*EMF*(Tw^4 - T^4).  Cells below 300 K do not emit
  * (TG_MN, C/Rad0.f90:42). */
+/* QA and QE (cell centres, W) of the enthalpy source GFM_RADSRC_FIELD; set by oracle_rad_field */
+static const double *RADF_QA = NULL, *RADF_QE = NULL;
+void oracle_rad_field(const double *qa, const double *qe) { RADF_QA = qa; RADF_QE = qe; }
 static double gray_qnet(int i, int j, int k) {
   double tk = T(i, j, k) * C->t0;
   double vol = VOL_C(i, j, k);
@@ -657,7 +660,12 @@ static void SOURCE(int mi1, int mi3, int nj1, int nj3, int lk1, int lk3) {
             SFP(i, j, k) = -0.0 / cpmix;
             double vol = VOL_C(i, j, k) * C->vol0;
             C->q_react += HRATE(i, j, k) * C->qr0 * vol;
-            if (C->gray_radiation) BS(i, j, k) += gray_qnet(i, j, k) / (vol * C->qr0);
+            if (C->gray_radiation == GFM_RADSRC_GRAY) BS(i, j, k) += gray_qnet(i, j, k) / (vol * C->qr0);
+            else if (C->gray_radiation == GFM_RADSRC_FIELD && RADF_QA) {
+              /* q_rad = QA - QE of the last radiation pass, C/Source.f90:437-447 */
+              const size_t cc = (((size_t)k / 2 - 1) * (size_t)(NP / 2) + ((size_t)j / 2 - 1)) * (size_t)(MP / 2) + ((size_t)i / 2 - 1);
+              BS(i, j, k) += (RADF_QA[cc] - RADF_QE[cc]) / (vol * C->qr0);
+            }
             if (C->isothermal_walls) BS(i, j, k) -= wall_sink(i, j, k, &C->q_wall);
           }
       if (C->itr_gas > 1) {
@@ -2389,3 +2397,893 @@ void oracle_reset_coefficients(void) {
   if (AS_) memset(AS_, 0, sizeof(double) * NN * 6);
   if (BS_) memset(BS_, 0, sizeof(double) * NN);
 }
+
+/* ======================================================================
+ * Radiation transport of one full radiation pass, RAD_EMIS C/Rad_emis.f90:
+ * 130-229 up to QRNF's volume loop: the wall marks (C/Rad_emis.f90:155-
+ * 186), RAD0 (C/Rad0.f90:12-172) with RAD_GAS, AH2O, ACO2, A1, CHIWHI,
+ * BANDA (C/Rad_gas.f90:16-396), SOOTA, SOC (C/Rad_soot.f90:26-69), PER,
+ * RIF, EMF, DEBX (C/Rad_fun.f90:14-140); QRNF's emitter loop (C/Qrn.f90:
+ * 58-200) with QREF (C/Qre.f90:31-505) and QRTX, QRTY, QRTZ
+ * (C/Qrtx.f90:15-255, C/Qrty.f90:9-224, C/Qrtz.f90:15-234); QRN1's sums
+ * (C/Qrn.f90:245-290).  rad_qe_only (C/Rad_emis.f90:411-466) is RAD0 with
+ * ID_RAD = 1.  Constants of C/Var_mod.f90:133-137, :18.
+ *
+ * Reference state that carries over and is kept here:
+ *  - AKLS (soot absorption) is zeroed once per RAD0 call and SOOTA leaves
+ *    it unchanged in a cell with a soot volume fraction below 1e-14, so
+ *    such a cell takes the AKLS of the last cell before it (loop order I,
+ *    J, K innermost) that had soot;
+ *  - QTX1, Q2M, QTX2 and their flags are zeroed once per axis of an
+ *    emitter, not between its two directions;
+ *  - the wall split of the second spreading pass of QRTX (QRTY, QRTZ)
+ *    writes next to the column J2 = J1 -+ 2 (I2 = I1 -+ 2) toward the
+ *    emitter; on the emitter's own column J2 (I2) is not assigned and keeps
+ *    the value it last had: from an earlier split, or as the first pass's
+ *    inner loop variable.  The reference's local variables are static (one
+ *    set per subroutine), so that value is carried from call to call; here
+ *    too, and RAD_STALE_PREV counts the uses whose value came from an
+ *    earlier emitter.
+ * ====================================================================== */
+static const double hck = 1.439e-2, t0_r = 1.0e2, p0_r = 1.01325e5, pg0_r = 1.01325e5, sig_sb = 5.668e-8;
+static const double pi_r = 3.141592653589793;
+static gfm_rad_ctrl *RC;
+static int NWL, ID_RAD;
+static int16_t *IBR_;                   /* cell codes with the wall marks (a copy; the caller's are unchanged) */
+static double *AKL_, *QEL_, *QAL_, *QE_, *QA_;
+static double WL[GFM_NWL_MAX], DEB[GFM_NWL_MAX], DEB0[GFM_NWL_MAX], AKLG[GFM_NWL_MAX], AKLS[GFM_NWL_MAX],
+    AKL0[GFM_NWL_MAX], E0R[GFM_NWL_MAX], QEL0[GFM_NWL_MAX], QEL1[GFM_NWL_MAX];
+static double SGN, T_RAD, TT0, P_RAD, YCO2, YH2O, FVS, OPL, W_R, PE;
+static double WNH2O[5][3], WNCO2[6][3], WNG[11][2], AAH2O[5], AACO2[6];
+static double ANG0[3];
+static double *ANGA_[3];                /* ANG1(MP,2), ANG2(NP,2), ANG3(LP,2): [axis][idx-1 + M*(col-1)] */
+static double *QTX1_, *Q2M_, *QTX2_;    /* plane arrays of the current axis, (b/2, c/2, L) */
+static unsigned char *IQT1_, *IQTM_, *IQT2_;
+static int PLB, PLC;                    /* plane extents of the current axis (cells) */
+static double qa_sum_bnd, qa_sum_vol, qa_to_open_c, qa_to_melt_c, qa_to_inlet_c, qa_to_exit_c, qa_to_wall_c;
+static int RAD_STALE[3];                /* part-1 inner loop variable of QRTX, QRTY, QRTZ (static in the reference) */
+static int RAD_STALE_SET[3];            /* set during the current call */
+long long RAD_STALE_PREV = 0, RAD_STALE_USES = 0, RAD_OOB = 0;
+
+#define IBR(i, j, k) ((int)IBR_[IX3(i, j, k)])
+#define CIX(id, jd, kd) ((((size_t)(kd)-1) * (size_t)NZ + ((size_t)(jd)-1)) * (size_t)MZ + ((size_t)(id)-1))
+#define CNN ((size_t)MZ * NZ * LZ)
+#define AKL(id, jd, kd, l) AKL_[CIX(id, jd, kd) + CNN * (size_t)((l)-1)]
+#define QEL(id, jd, kd, l) QEL_[CIX(id, jd, kd) + CNN * (size_t)((l)-1)]
+#define QAL(id, jd, kd, l) QAL_[CIX(id, jd, kd) + CNN * (size_t)((l)-1)]
+#define QE(id, jd, kd) QE_[CIX(id, jd, kd)]
+#define QA(id, jd, kd) QA_[CIX(id, jd, kd)]
+#define PIX(bd, cd) (((size_t)(cd)-1) * (size_t)PLB + ((size_t)(bd)-1))
+#define QTX1(bd, cd, l) QTX1_[PIX(bd, cd) + (size_t)PLB * PLC * ((l)-1)]
+#define Q2M(bd, cd, l) Q2M_[PIX(bd, cd) + (size_t)PLB * PLC * ((l)-1)]
+#define QTX2(bd, cd, l) QTX2_[PIX(bd, cd) + (size_t)PLB * PLC * ((l)-1)]
+#define IQT1(bd, cd) IQT1_[PIX(bd, cd)]
+#define IQTM(bd, cd) IQTM_[PIX(bd, cd)]
+#define IQT2(bd, cd) IQT2_[PIX(bd, cd)]
+
+static int axis_len(int d) { return d == 1 ? MP : d == 2 ? NP : LP; }
+static double axis_x(int d, int n) { return d == 1 ? X(n) : d == 2 ? R(n) : Z(n); }
+static double *ANGP(int d, int n, int col) { return &ANGA_[d - 1][(size_t)(n - 1) + (size_t)axis_len(d) * (col - 1)]; }
+#define ANG(d, n, col) (*ANGP(d, n, col))
+/* atand (the degree arc tangent the reference calls) of h >= 0 by reduction
+ * to |t| <= tan(pi/8) (1/h above 1, (h-1)/(h+1) above tan(pi/8)) and the
+ * series t * sum_{n=0..20} (-t^2)^n/(2n+1) in Horner form: the device code
+ * evaluates the same operations in the same order without contraction, so
+ * the many angle comparisons of QRT that tie in exact arithmetic resolve
+ * the same way on both. */
+static double rad_atan(double h) {
+  double off = 0.0, sgn = 1.0, off2 = 0.0;
+  if (h > 1.0) { h = 1.0 / h; off = pi_r / 2.0; sgn = -1.0; }
+  double t = h;
+  if (h > 0.41421356237309503) { t = (h - 1.0) / (h + 1.0); off2 = pi_r / 4.0; }
+  const double t2 = t * t;
+  double s = 1.0 / 41.0;
+  for (int n = 19; n >= 0; n--) s = 1.0 / (double)(2 * n + 1) - t2 * s;
+  return off + sgn * (off2 + t * s);
+}
+static double atand_(double h) { return rad_atan(h) * (180.0 / pi_r); }
+static double ipow3(double x) { return x * x * x; }
+static double ipow4(double x) { double x2 = x * x; return x2 * x2; }
+
+/* PER, C/Rad_fun.f90:14-40: m!/(m-n)! */
+static double PER(int m, int n) {
+  double per = 1.0;
+  if (m < n) return NAN;
+  if (m == 0 || n == 0) return per;
+  for (int i = m - n + 1; i <= m; i++) per *= i;
+  return per;
+}
+/* RIF, C/Rad_fun.f90:45-91: (15/pi^4) * sum_n e^{-(a+n)x} sum_i m!/(m-i)! x^{m-i}/(a+n)^{i+1},
+ * the blackbody fraction above x = hc/(k lambda T) for m = 3, a = 0 */
+static double RIF(int m, double a, double x) {
+  double rif = 0.0, y[8];
+  if (m < 0 || a < 0.0 || m > 7) return NAN;
+  if (x > 1.0e16) return 0.0;
+  if (x == 0.0) {
+    int n = (int)(pow(0.5e6, 1.0 / (m + 1)) - a);
+    double g0 = 0.0;
+    for (int i = 1; i <= n; i++) g0 += 1.0 / pow(a + i, m + 1);
+    rif = PER(m, m) * (g0 + 0.5 * (1.0 / pow(a + n + 1, m) + 1.0 / pow(a + n, m)) / m);
+  } else {
+    for (int i = 0; i <= m; i++) y[i] = PER(m, i);
+    for (int n = 1; n <= 1000; n++) {
+      double g0 = 0.0;
+      for (int i = 0; i <= m; i++) {
+        double xp = 1.0, ap = 1.0;
+        for (int q = 0; q < m - i; q++) xp *= x;
+        for (int q = 0; q < i + 1; q++) ap *= (a + n);
+        g0 += y[i] * xp / ap;
+      }
+      g0 *= exp(-(a + n) * x);
+      rif += g0;
+      if (fabs(g0) <= small16) break;
+    }
+  }
+  rif = rif * 1.5e1 / ipow4(pi_r);
+  if (rif > 1.0) rif = 1.0;
+  return rif;
+}
+/* DEBX, C/Rad_fun.f90:122-140: the fraction of the blackbody emission at
+ * tg0 in each band (bands below 1e-3 join the next) */
+static void DEBX(double tg0) {
+  double eb0 = 0.0;
+  for (int l = 1; l <= NWL - 1; l++) {
+    double x1 = DEB0[l - 1] / tg0;
+    double eb1 = RIF(3, 0.0, x1);
+    DEB[l - 1] = eb1 - eb0;
+    if (DEB[l - 1] <= 1.0e-3) DEB[l - 1] = 0.0;
+    else eb0 = eb1;
+  }
+  DEB[NWL - 1] = 1.0 - eb0;
+}
+/* CHIWHI, C/Rad_gas.f90:308-353 (Weiner and Edwards 1968, eqs A16-A20) */
+static void CHIWHI(double tt, const double v[3], const int ng[3], const int nd[3], double *chi, double *whi) {
+  if (nd[0] != 0 || nd[1] != 0 || nd[2] != 0) {
+    double ud = 0.0, xd = 0.0, xy = 0.0, yn = 0.0;
+    for (int k = 0; k < 3; k++) {
+      double u = hck * v[k] / tt;
+      int nv = 0;
+      if (nd[k] < 0) nv = abs(nd[k]);
+      int ndk = abs(nd[k]);
+      double y1 = 0.0;
+      for (;;) {
+        double xgas = exp(-u * nv) / PER(ng[k] - 1, ng[k] - 1);
+        double x1 = xgas * PER(nv + ng[k] + ndk - 1, ng[k] + ndk - 1);
+        double x2 = xgas * PER(nv + ng[k] - 1, ng[k] - 1);
+        if (x1 < 1.0e-10) break;
+        xy += x1;
+        xd += x2;
+        y1 += sqrt(x1);
+        nv++;
+        if (nv >= 100) break;
+      }
+      yn += y1 * y1;
+      ud -= u * nd[k];
+    }
+    *chi = (1.0 - exp(ud)) * xy / xd;
+    *whi = yn / xy / sqrt(TT0);
+  } else {
+    *chi = 1.0;
+    *whi = 1.0 / sqrt(TT0);
+  }
+}
+/* BANDA, C/Rad_gas.f90:360-396 (Edwards and Menard 1964); a negative
+ * argument stops the reference's run: here it gives NAN */
+static double BANDA(double alp, double bta, double a0) {
+  if (a0 <= 0.0) return 0.0;
+  if (alp < 0.0 || bta < 0.0 || a0 < 0.0 || W_R * PE < 0.0) return NAN;
+  if (W_R * PE == 0.0) return 0.0;
+  double eta = bta * PE, tau = alp * W_R / a0, a_s;
+  if (eta >= 1.0) {
+    a_s = (tau <= 1.0) ? tau : 1.0 + log(tau);
+  } else {
+    if (tau <= eta) a_s = tau;
+    else if (tau <= 1.0 / eta) a_s = 2.0 * sqrt(eta * tau) - eta;
+    else a_s = log(eta * tau) + 2.0 - eta;
+  }
+  return a0 * a_s;
+}
+/* A1, C/Rad_gas.f90:284-300 */
+static double A1R(const double v[3], const int ng[3], const double b[3], const int nd[3]) {
+  double chi0, whi0, chi, whi;
+  CHIWHI(t0_r, v, ng, nd, &chi0, &whi0);
+  CHIWHI(T_RAD, v, ng, nd, &chi, &whi);
+  double c3 = b[2] * sqrt(T_RAD / t0_r);
+  double c1 = b[0] * chi / chi0;
+  double c2 = 2.0 * sqrt(c1 * c3 * b[1] * whi / whi0);
+  return BANDA(c1, c2, c3);
+}
+/* AH2O, C/Rad_gas.f90:149-213: band absorptances of H2O (1/m) */
+static void AH2O(void) {
+  static const int ng[3] = {1, 1, 1};
+  static const double v[3] = {3.652e5, 1.595e5, 3.756e5};
+  double aa[5], b[3], chi0, whi0, chi, whi;
+  int nd[3];
+  for (int n = 0; n < 5; n++) AAH2O[n] = 0.0;
+  if (YH2O <= 0.0 || T_RAD <= 3.0e2) return;
+  PE = P_RAD + P_RAD * YH2O * (0.86e1 / TT0 - 0.5);
+  W_R = P_RAD * YH2O * OPL / T_RAD / 8.208e-2 * wh2o;
+  b[0] = 0.52e4 * exp(-0.176e2 / TT0); b[1] = 0.14311; b[2] = 0.284e2;
+  nd[0] = nd[1] = nd[2] = 0;
+  aa[0] = A1R(v, ng, b, nd);
+  b[0] = 0.412e2; b[1] = 0.9427e-1; b[2] = 0.564e2;
+  nd[1] = 1;
+  aa[1] = A1R(v, ng, b, nd);
+  b[0] = 0.19; b[1] = 0.13219; b[2] = 0.6e2;
+  nd[1] = 2;
+  CHIWHI(t0_r, v, ng, nd, &chi0, &whi0);
+  CHIWHI(T_RAD, v, ng, nd, &chi, &whi);
+  double t1 = b[0] * chi / chi0, t4 = b[1] * whi / whi0;
+  b[0] = 0.23e1;
+  nd[0] = 1; nd[1] = 0;
+  CHIWHI(t0_r, v, ng, nd, &chi0, &whi0);
+  CHIWHI(T_RAD, v, ng, nd, &chi, &whi);
+  double t2 = b[0] * chi / chi0, t5 = b[1] * whi / whi0;
+  b[0] = 0.224e2;
+  nd[0] = 0; nd[2] = 1;
+  CHIWHI(t0_r, v, ng, nd, &chi0, &whi0);
+  CHIWHI(T_RAD, v, ng, nd, &chi, &whi);
+  double t3 = b[0] * chi / chi0, t6 = b[1] * whi / whi0;
+  double c1 = t1 + t2 + t3, c3 = b[2] * TT0;
+  double bta = 2.0 * sqrt(c3) * (sqrt(t1 * t4) + sqrt(t2 * t5) + sqrt(t3 * t6));
+  aa[2] = BANDA(c1, bta, c3);
+  b[0] = 3.0; b[1] = 0.8169e-1; b[2] = 0.431e2;
+  nd[1] = 1;
+  aa[3] = A1R(v, ng, b, nd);
+  b[0] = 0.25e1; b[1] = 0.11628; b[2] = 0.32e2;
+  nd[0] = 1; nd[1] = 0;
+  aa[4] = A1R(v, ng, b, nd);
+  for (int n = 0; n < 5; n++) AAH2O[n] = aa[n] * 1.0e2;
+}
+/* ACO2, C/Rad_gas.f90:224-277: band absorptances of CO2 (1/m); with no
+ * CO2 path AACO2 keeps its last values, as in the reference */
+static void ACO2(void) {
+  static const int ng[3] = {1, 2, 1};
+  static const double v[3] = {1.351e5, 6.67e4, 2.396e5};
+  double aa[6] = {0, 0, 0, 0, 0, 0}, b[3];
+  int nd[3];
+  W_R = P_RAD * YCO2 * OPL / T_RAD / 8.208e-2 * wco2;
+  if (W_R <= 0.0) return;
+  PE = pow(P_RAD + P_RAD * YCO2 * 0.3, 0.7);
+  b[0] = 0.19e2; b[1] = 0.6157e-1; b[2] = 0.127e2;
+  nd[0] = 0; nd[1] = 1; nd[2] = 0;
+  aa[0] = A1R(v, ng, b, nd);
+  PE = pow(P_RAD + P_RAD * YCO2 * 0.3, 0.8);
+  b[0] = 0.36; b[1] = 0.4017e-1; b[2] = 0.134e2;
+  nd[0] = -1; nd[1] = 0; nd[2] = 1;
+  aa[1] = A1R(v, ng, b, nd);
+  b[0] = 0.24; b[1] = 0.11888; b[2] = 0.101e2;
+  aa[2] = A1R(v, ng, b, nd);
+  b[0] = 0.11e3; b[1] = 0.24723; b[2] = 0.112e2;
+  nd[0] = 0;
+  aa[3] = A1R(v, ng, b, nd);
+  PE = pow(P_RAD + P_RAD * YCO2 * 0.3, 0.65);
+  b[0] = 4.0; b[1] = 0.13341; b[2] = 23.5;
+  nd[0] = 1;
+  aa[4] = A1R(v, ng, b, nd);
+  b[0] = 0.66e-1; b[1] = 0.39305; b[2] = 0.345e2;
+  nd[0] = 2;
+  aa[5] = A1R(v, ng, b, nd);
+  for (int n = 0; n < 6; n++) AACO2[n] = aa[n] * 1.0e2;
+}
+/* SOC, C/Rad_soot.f90:49-69: optical constants of soot (Lee and Tien 1981) */
+static void SOC(double wl0, double *xn_k, double *x2nk) {
+  static const double socf[3] = {7.2e26, 4.07e27, 4.47e28}, soch[3] = {1.2e15, 5.90e15, 5.60e15},
+                      socw[3] = {0.0, 1.25e15, 7.25e15};
+  const double c_0 = 2.9984e8, e2me0 = 3.185e3;
+  *xn_k = 1.0;
+  *x2nk = 0.0;
+  double w = 2.0 * pi_r * c_0 / wl0;
+  for (int j = 0; j < 3; j++) {
+    double a = socw[j] * socw[j] - w * w;
+    double b = a * a + w * w * soch[j] * soch[j];
+    if (b < 1.0e-20) return;
+    double g0 = socf[j] * e2me0;
+    *xn_k += g0 * a / b;
+    *x2nk += g0 * soch[j] * w / b;
+  }
+}
+/* SOOTA, C/Rad_soot.f90:26-37: unchanged below a volume fraction of 1e-14 */
+static void SOOTA(void) {
+  if (FVS < 1.0e-14) return;
+  for (int n = 1; n <= NWL; n++) {
+    double wl0 = WL[n - 1], xn_k, x2nk;
+    SOC(wl0, &xn_k, &x2nk);
+    double g0 = 18.0 * pi_r * x2nk / ((xn_k + 2.0) * (xn_k + 2.0) + x2nk * x2nk);
+    AKLS[n - 1] = g0 * FVS / wl0;
+  }
+}
+/* RAD_GAS, C/Rad_gas.f90:16-141: AKL0 = AKLG + AKLS */
+static void RAD_GAS(void) {
+  memset(WNH2O, 0, sizeof(WNH2O));
+  memset(WNCO2, 0, sizeof(WNCO2));
+  memset(WNG, 0, sizeof(WNG));
+  for (int n = 0; n < NWL; n++) AKLG[n] = 0.0;
+  if (YH2O > 1.0e-4) {
+    WNH2O[0][0] = 0.0;
+    WNH2O[1][1] = 1.60e5; WNH2O[2][1] = 3.76e5; WNH2O[3][1] = 5.35e5; WNH2O[4][1] = 7.25e5;
+    AH2O();
+    WNH2O[0][1] = AAH2O[0] / 2;
+    WNH2O[0][2] = AAH2O[0];
+    for (int n = 2; n <= 5; n++) {
+      double g0 = AAH2O[n - 1] / 2;
+      WNH2O[n - 1][0] = WNH2O[n - 1][1] - g0;
+      WNH2O[n - 1][2] = WNH2O[n - 1][1] + g0;
+      if (WNH2O[n - 2][2] > WNH2O[n - 1][0]) WNH2O[n - 2][2] = WNH2O[n - 1][0];
+    }
+  }
+  if (YCO2 > 1.0e-4) {
+    WNCO2[0][1] = 6.67e4; WNCO2[1][1] = 9.60e4; WNCO2[2][1] = 1.06e5; WNCO2[3][2] = 2.41e5;
+    WNCO2[4][1] = 3.66e5; WNCO2[5][1] = 5.20e5;
+    ACO2();
+    for (int n = 1; n <= 6; n++) {
+      double g0 = AACO2[n - 1] / 2;
+      if (n == 1 && g0 > WNCO2[0][1]) {
+        WNCO2[0][0] = 0.0;
+        WNCO2[0][2] = AACO2[0];
+      } else if (n == 4) {
+        WNCO2[3][1] = WNCO2[3][2] - g0;
+        WNCO2[3][0] = WNCO2[3][1] - g0;
+      } else {
+        WNCO2[n - 1][0] = WNCO2[n - 1][1] - g0;
+        WNCO2[n - 1][2] = WNCO2[n - 1][1] + g0;
+      }
+      if (n <= 1) continue;
+      if (WNCO2[n - 2][2] > WNCO2[n - 1][0]) WNCO2[n - 2][2] = WNCO2[n - 1][0];
+    }
+  }
+  SOOTA();
+  for (int n = 0; n < 5; n++) { WNG[n][0] = WNH2O[n][0]; WNG[n][1] = WNH2O[n][2]; }
+  for (int n = 0; n < 6; n++) { WNG[5 + n][0] = WNCO2[n][0]; WNG[5 + n][1] = WNCO2[n][2]; }
+  for (int n = 1; n <= 10; n++) {
+    for (int n2 = n + 1; n2 <= 11; n2++) {
+      if (WNG[n - 1][0] < WNG[n2 - 1][0]) continue;
+      double g0 = WNG[n - 1][0]; WNG[n - 1][0] = WNG[n2 - 1][0]; WNG[n2 - 1][0] = g0;
+      g0 = WNG[n - 1][1]; WNG[n - 1][1] = WNG[n2 - 1][1]; WNG[n2 - 1][1] = g0;
+    }
+    if (n <= 1) continue;
+    if (WNG[n - 2][1] > WNG[n - 1][0]) WNG[n - 2][1] = WNG[n - 1][0];
+  }
+  double wl2 = 10.0 / WL[0];
+  for (int n = 0; n < NWL; n++) E0R[n] = 0.0;
+  int n0 = 11;
+  for (int n = 1; n <= NWL; n++) {
+    double wl1 = (n == NWL) ? 0.0 : 1.0 / WL[n - 1];
+    if (wl2 <= wl1 || WNG[10][1] <= wl1) {
+      wl2 = wl1;
+      continue;
+    }
+    for (int n1 = n0; n1 >= 1; n1--) {
+      if (WNG[n1 - 1][1] <= wl1) {
+        n0 = n1;
+        break;
+      }
+      if (WNG[n1 - 1][1] >= wl1 && WNG[n1 - 1][1] <= wl2) {
+        if (WNG[n1 - 1][0] < wl1) {
+          E0R[n - 1] += WNG[n1 - 1][1] - wl1;
+          WNG[n1 - 1][1] = wl1;
+          n0 = n1;
+          break;
+        } else {
+          E0R[n - 1] += WNG[n1 - 1][1] - WNG[n1 - 1][0];
+        }
+      }
+    }
+    if (E0R[n - 1] <= 0.0 || wl2 <= wl1) {
+      AKLG[n - 1] = 0.0;
+    } else {
+      E0R[n - 1] = E0R[n - 1] / (wl2 - wl1);
+      double g0 = 1.0 - E0R[n - 1];
+      if (g0 <= 1.0e-20) g0 = 1.0e-20;
+      AKLG[n - 1] = -log(g0) / OPL;
+    }
+    wl2 = wl1;
+  }
+  for (int n = 0; n < NWL; n++) AKL0[n] = AKLG[n] + AKLS[n];
+}
+/* RAD0, C/Rad0.f90:12-172 (T in K).  id_rad 10: AKL of every cell and the
+ * emission sums; 1: QE of every open cell into QE_ (rad_qe_only). */
+static void RAD0(void) {
+  SGN = sig_sb * RC->ri * RC->ri;
+  for (int l = 0; l < NWL; l++) DEB0[l] = hck / RC->ri / WL[l];
+  RC->tot_qe_h2o_co2 = 0.0;
+  RC->tot_qe_soot = 0.0;
+  memset(AKL_, 0, sizeof(double) * CNN * NWL);
+  for (int l = 0; l < NWL; l++) AKLG[l] = AKLS[l] = AKL0[l] = E0R[l] = DEB[l] = 0.0;
+  double qeg0 = 0.0;
+  for (int i = 2; i <= MP; i += 2) {
+    int id2 = i / 2;
+    for (int j = 2; j <= NP; j += 2) {
+      int jd2 = j / 2;
+      for (int k = 2; k <= LP; k += 2) {
+        int kd2 = k / 2;
+        T_RAD = T(i, j, k) * C->t0;
+        if (IBR(i, j, k) <= 0 && T_RAD >= 300.0) {
+          TT0 = sqrt(T_RAD / t0_r);
+          P_RAD = (1.0 + P(i, j, k)) * pg0_r / p0_r;
+          double sum = 0.0;
+          for (int n = 1; n <= 7; n++) sum += GFM(i, j, k, n) / wms[n - 1];
+          double wmix_ms = 1.0 / sum;
+          YCO2 = GFM(i, j, k, 5) * wmix_ms / wco2;
+          YH2O = GFM(i, j, k, 4) * wmix_ms / wh2o;
+          FVS = SMF(i, j, k) * DNST(i, j, k) * C->dnst0 / RC->dnst_soot;
+          for (int l = 1; l <= NWL; l++) AKL(id2, jd2, kd2, l) = 0.0;
+          for (int l = 0; l < NWL; l++) AKL0[l] = 0.0;
+          double vol = VOL_C(i, j, k);
+          double sa = ((DX(i) * DR(j)) + (DR(j) * DZ(k)) + (DX(i) * DZ(k))) * 2.0;
+          if (sa <= small16) continue;
+          OPL = 6.0 * vol / sa;
+          if (OPL <= 1.0e-10) continue;
+          RAD_GAS();
+          DEBX(T_RAD);
+          double area_sgn_t4 = sa * SGN * ipow4(T_RAD);
+          if (ID_RAD == 10) {
+            double sg = 0.0, ss = 0.0;
+            for (int l = 1; l <= NWL; l++) {
+              AKL(id2, jd2, kd2, l) = oracle_emf(AKL0[l - 1], OPL);
+              sg += DEB[l - 1] * oracle_emf(AKLG[l - 1], OPL);
+              ss += DEB[l - 1] * oracle_emf(AKLS[l - 1], OPL);
+            }
+            RC->tot_qe_h2o_co2 += area_sgn_t4 * sg;
+            RC->tot_qe_soot += area_sgn_t4 * ss;
+          } else {
+            double s = 0.0;
+            for (int l = 1; l <= NWL; l++) s += DEB[l - 1] * oracle_emf(AKL0[l - 1], OPL);
+            QE(id2, jd2, kd2) = area_sgn_t4 * s;
+            qeg0 += QE(id2, jd2, kd2);
+          }
+        } else if (IBR(i, j, k) == 4) {
+          for (int l = 1; l <= NWL; l++) AKL(id2, jd2, kd2, l) = RC->eps_m;
+        } else if (IBR(i, j, k) > 100) {
+          double e = (Z(k) >= RC->height_to_ceiling || (k == LP && RC->have_crown == 0)) ? RC->eps_c : RC->eps_w;
+          for (int l = 1; l <= NWL; l++) AKL(id2, jd2, kd2, l) = e;
+        }
+      }
+    }
+  }
+  if (ID_RAD != 10) RC->qeg0 = qeg0;
+}
+
+/* Cell (i,j,k) of slab position a on axis A and plane position (b, c) on
+ * axes B, C. */
+static void rad_cell(int A, int B, int C_, int a, int b, int c, int *i, int *j, int *k) {
+  int v[4];
+  v[A] = a; v[B] = b; v[C_] = c;
+  *i = v[1]; *j = v[2]; *k = v[3];
+}
+
+/* One slab step of QRTX (A = x, B = y, C = z), QRTY (y, x, z) or QRTZ (z, x,
+ * y): the energy QTX1 that left slab a10 toward slab a1 spreads across the
+ * plane by the angles of slab a1's far face, first along B row by row
+ * (rows of C from 4 to the last but one) into Q2M, then along C column by
+ * column into QTX2.  mk1/mk2: the wall marks of faces normal to A (101/111,
+ * 201/211, 301/311); split_gt: the wall split of the second pass needs a
+ * cell code above it (100 QRTX, 99 QRTY, 1 QRTZ). */
+static void QRT(int A, int B, int C_, int a0, int b0, int c0, int a10, int a1, int mk1, int mk2, int split_gt, int s) {
+  const int Ma = axis_len(A), Mb = axis_len(B), Mc = axis_len(C_);
+  int a2 = a1 + (a1 - a10) / 2;
+  if (a2 > Ma) a2 = Ma;
+  double q11[GFM_NWL_MAX], q12[GFM_NWL_MAX];
+  /* the angles the far face subtends at the emitter (SANG2 expanded) */
+  for (int bm = -1; bm <= 1; bm += 2) {
+    int bend = bm < 0 ? 2 : Mb;
+    for (int bb = b0 + bm; bm < 0 ? bb >= bend : bb <= bend; bb += 2 * bm) {
+      double dist = fabs(axis_x(A, a2) - axis_x(A, a0)), ang;
+      if (dist < small16) ang = 90.0;
+      else ang = atand_(fabs(axis_x(B, bb) - axis_x(B, b0)) / dist);
+      if (ang >= ANG0[B - 1]) break;
+      ANG(B, bb, 2) = ang;
+    }
+  }
+  for (int cm = -1; cm <= 1; cm += 2) {
+    int cend = cm < 0 ? 2 : Mc;
+    for (int cc = c0 + cm; cm < 0 ? cc >= cend : cc <= cend; cc += 2 * cm) {
+      double dist = fabs(axis_x(A, a2) - axis_x(A, a0)), ang;
+      if (dist < small16) ang = 90.0;
+      else ang = atand_(fabs(axis_x(C_, cc) - axis_x(C_, c0)) / dist);
+      if (ang >= ANG0[C_ - 1]) break;
+      ANG(C_, cc, 2) = ang;
+    }
+  }
+  /* first pass: along B, rows c1 = 4 .. Mc-2 */
+  for (int c1 = 4; c1 <= Mc - 2; c1 += 2) {
+    if (c1 < c0 && ANG(C_, c1 + 1, 2) >= ANG0[C_ - 1]) continue;
+    if (c1 > c0 && ANG(C_, c1 - 1, 2) >= ANG0[C_ - 1]) break;
+    int cd2 = c1 / 2;
+    for (int bpm = -2; bpm <= 2; bpm += 4) {
+      double a11 = 0.0, a12 = 0.0;
+      int bend = bpm < 0 ? 2 : Mb;
+      for (int b1 = b0; bpm < 0 ? b1 >= bend - bpm : b1 <= bend - bpm; b1 += bpm) {
+        if (a11 >= ANG0[B - 1]) break;
+        int b12 = b1 + bpm / 2;
+        a12 = ANG(B, b12, 1);
+        double dang = a12 - a11;
+        if (dang <= 1.0e-10) { a11 = a12; continue; }
+        double a21 = a11;
+        int b1d2 = b1 / 2;
+        if (IQT1(b1d2, cd2)) { a11 = a12; continue; }
+        if (b1 == b0) {
+          for (int l = 1; l <= NWL; l++) q11[l - 1] = QTX1(b1d2, cd2, l) * 0.5;
+          if (bpm > 0) for (int l = 1; l <= NWL; l++) QTX1(b1d2, cd2, l) = 0.0;
+        } else {
+          for (int l = 1; l <= NWL; l++) { q11[l - 1] = QTX1(b1d2, cd2, l); QTX1(b1d2, cd2, l) = 0.0; }
+        }
+        if (b1 != b0 || bpm > 0) IQT1(b1d2, cd2) = 1;
+        for (int l = 0; l < NWL; l++) q12[l] = q11[l];
+        int b2;
+        for (b2 = b1; bpm < 0 ? b2 >= bend : b2 <= bend; b2 += bpm) {
+          int b2d2 = b2 / 2, ii, jj, kk;
+          rad_cell(A, B, C_, a1, b2, c1, &ii, &jj, &kk);
+          if (IBR(ii, jj, kk) < 1) {
+            int b2m = b2 + bpm / 2;
+            if (b2m < 1 || b2m > Mb) { RAD_OOB++; b2m = b2m < 1 ? 1 : Mb; }
+            double a22 = ANG(B, b2m, 2);
+            if (a22 <= a21) continue;
+            if (a22 < a12) {
+              double fij = (a22 - a21) / dang;
+              for (int l = 1; l <= NWL; l++) {
+                if (q11[l - 1] < small16) continue;
+                double q21 = q11[l - 1] * fij;
+                Q2M(b2d2, cd2, l) += q21;
+                q12[l - 1] -= q21;
+              }
+              IQTM(b2d2, cd2) = 0;
+              a21 = a22;
+              continue;
+            }
+          }
+          for (int l = 1; l <= NWL; l++) Q2M(b2d2, cd2, l) += q12[l - 1];
+          IQTM(b2d2, cd2) = 0;
+          break;
+        }
+        RAD_STALE[s] = b2;
+        RAD_STALE_SET[s] = 1;
+        a11 = a12;
+      }
+    }
+  }
+  /* second pass: along C, columns b1 = 2 .. Mb */
+  for (int b1 = 2; b1 <= Mb; b1 += 2) {
+    if (b1 < b0 && ANG(B, b1 + 1, 2) >= ANG0[B - 1]) continue;
+    if (b1 > b0 && ANG(B, b1 - 1, 2) >= ANG0[B - 1]) break;
+    int bd2 = b1 / 2;
+    for (int cm = -2; cm <= 2; cm += 4) {
+      double a11 = 0.0, a12 = 0.0;
+      int cend = cm < 0 ? 2 : Mc;
+      for (int c1 = c0; cm < 0 ? c1 >= cend - cm : c1 <= cend - cm; c1 += cm) {
+        if (a11 >= ANG0[C_ - 1]) break;
+        int c12 = c1 + cm / 2;
+        a12 = ANG(C_, c12, 1);
+        double dang = a12 - a11;
+        if (dang <= 1.0e-10) { a11 = a12; continue; }
+        double a21 = a11;
+        int cd2 = c1 / 2;
+        if (IQTM(bd2, cd2)) { a11 = a12; continue; }
+        if (c1 == c0) {
+          for (int l = 1; l <= NWL; l++) q11[l - 1] = Q2M(bd2, cd2, l) * 0.5;
+          if (cm > 0) for (int l = 1; l <= NWL; l++) Q2M(bd2, cd2, l) = 0.0;
+        } else {
+          for (int l = 1; l <= NWL; l++) { q11[l - 1] = Q2M(bd2, cd2, l); Q2M(bd2, cd2, l) = 0.0; }
+        }
+        if (c1 != c0 || cm > 0) IQTM(bd2, cd2) = 1;
+        int ii, jj, kk;
+        rad_cell(A, B, C_, a1, b1, c1, &ii, &jj, &kk);
+        const int ibc1 = IBR(ii, jj, kk);
+        if (ibc1 == mk1 || ibc1 == mk2 || ibc1 == 4) {
+          for (int l = 1; l <= NWL; l++) QTX2(bd2, cd2, l) += q11[l - 1];
+          IQT2(bd2, cd2) = 0;
+          a11 = a12;
+          continue;
+        }
+        for (int l = 0; l < NWL; l++) q12[l] = q11[l];
+        for (int c2 = c1; cm < 0 ? c2 >= cend : c2 <= cend; c2 += cm) {
+          int c2m = c2 + cm / 2, c2d2 = c2 / 2;
+          rad_cell(A, B, C_, a1, b1, c2, &ii, &jj, &kk);
+          const int ibc2 = IBR(ii, jj, kk);
+          int deposit = 0;
+          if (ibc1 <= 0 && ibc2 >= 1) {
+            deposit = 1;
+          } else if (ibc1 > split_gt && ibc2 == 1) {
+            int b2;
+            if (b1 < b0) b2 = b1 + 2;
+            else if (b1 > b0) b2 = b1 - 2;
+            else {
+              b2 = RAD_STALE[s];
+              RAD_STALE_USES++;
+              if (!RAD_STALE_SET[s]) RAD_STALE_PREV++;
+            }
+            RAD_STALE[s] = b2;  /* the split assigns the reference's variable too */
+            if (b2 < 2 || b2 > Mb) { RAD_OOB++; b2 = b2 < 2 ? 2 : Mb; }  /* a completed first-pass loop leaves it past the end */
+            int b2d2 = b2 / 2, c2d2m = c2d2 - cm / 2;
+            for (int l = 1; l <= NWL; l++) {
+              if (q12[l - 1] < small16) continue;
+              double q21 = q12[l - 1] * 0.5;
+              QTX2(b2d2, c2d2, l) += q21;
+              QTX2(bd2, c2d2m, l) += q21;
+            }
+            IQT2(b2d2, c2d2) = 0;
+            IQT2(bd2, c2d2m) = 0;
+            break;
+          } else if (c2m > Mc || c2m < 2) {
+            deposit = 1;
+          } else {
+            double a22 = ANG(C_, c2m, 2);
+            if ((a22 - a21) <= 1.0e-10) continue;
+            if (a22 < a12) {
+              double fij = (a22 - a21) / dang;
+              for (int l = 1; l <= NWL; l++) {
+                if (q11[l - 1] < small16) continue;
+                double q21 = q11[l - 1] * fij;
+                QTX2(bd2, c2d2, l) += q21;
+                q12[l - 1] -= q21;
+              }
+              IQT2(bd2, c2d2) = 0;
+              a21 = a22;
+              continue;
+            }
+            deposit = 1;
+          }
+          if (deposit) {
+            for (int l = 1; l <= NWL; l++) QTX2(bd2, c2d2, l) += q12[l - 1];
+            IQT2(bd2, c2d2) = 0;
+            break;
+          }
+        }
+        a11 = a12;
+      }
+    }
+  }
+}
+
+/* One axis of QREF (C/Qre.f90:95-505): both directions along A from the
+ * emitter, slab by slab (the QRT step, then absorption in slab a: open
+ * cells take QTX2*AKL and pass the rest on in QTX1, boundary cells and the
+ * last slab take all). */
+static void QREF_AXIS(int A, int B, int C_, int i0, int j0, int k0, int mk1, int mk2, int split_gt, int s) {
+  int v0[4] = {0, i0, j0, k0};
+  const int a0 = v0[A], b0 = v0[B], c0 = v0[C_];
+  const int Ma = axis_len(A), Mb = axis_len(B), Mc = axis_len(C_);
+  const int i0d2 = i0 / 2, j0d2 = j0 / 2, k0d2 = k0 / 2;
+  PLB = Mb / 2; PLC = Mc / 2;
+  const size_t pn = (size_t)PLB * PLC;
+  memset(QTX1_, 0, sizeof(double) * pn * NWL);
+  memset(Q2M_, 0, sizeof(double) * pn * NWL);
+  memset(QTX2_, 0, sizeof(double) * pn * NWL);
+  memset(IQT1_, 1, pn); memset(IQTM_, 1, pn); memset(IQT2_, 1, pn);
+  const double (*dd[4]) = {NULL, DX_, DR_, DZ_};
+  for (int l = 1; l <= NWL; l++) QEL1[l - 1] = QEL0[l - 1] * dd[B][b0 - 1] * dd[C_][c0 - 1];
+  for (int apm = -2; apm <= 2; apm += 4) {
+    for (int l = 1; l <= NWL; l++) QTX1(b0 / 2, c0 / 2, l) = QEL1[l - 1];
+    IQT1(b0 / 2, c0 / 2) = 0;
+    for (int l = 1; l <= NWL; l++) QEL(i0d2, j0d2, k0d2, l) += QEL1[l - 1];
+    /* sfac of the emitter's own corner */
+    int v1[4] = {0, i0 + apm / 2, j0 + apm / 2, k0 + apm / 2};
+    ANG0[A - 1] = 0.0;
+    double bb = fabs(axis_x(A, v1[A]) - axis_x(A, a0));
+    if (bb < small16) {
+      ANG0[B - 1] = 90.0;
+      ANG0[C_ - 1] = 90.0;
+    } else {
+      ANG0[B - 1] = atand_(fabs(axis_x(B, v1[B]) - axis_x(B, b0)) / bb);
+      ANG0[C_ - 1] = atand_(fabs(axis_x(C_, v1[C_]) - axis_x(C_, c0)) / bb);
+    }
+    for (int n = 1; n <= Mb; n++) ANG(B, n, 1) = ANG(B, n, 2) = ANG0[B - 1];
+    ANG(B, b0, 1) = ANG(B, b0, 2) = 0.0;
+    for (int n = 1; n <= Mc; n++) ANG(C_, n, 1) = ANG(C_, n, 2) = ANG0[C_ - 1];
+    ANG(C_, c0, 1) = ANG(C_, c0, 2) = 0.0;
+    const int aend = apm < 0 ? 2 : Ma;
+    for (int a = a0 + apm; apm < 0 ? a >= aend : a <= aend; a += apm) {
+      QRT(A, B, C_, a0, b0, c0, a - apm, a, mk1, mk2, split_gt, s);
+      for (int b = 2; b <= Mb; b += 2) {
+        int bd2 = b / 2;
+        for (int c = 2; c <= Mc; c += 2) {
+          int cd2 = c / 2;
+          if (IQT2(bd2, cd2)) continue;
+          int i, j, k;
+          rad_cell(A, B, C_, a, b, c, &i, &j, &k);
+          for (int l = 1; l <= NWL; l++) {
+            if (QTX2(bd2, cd2, l) < small16) continue;
+            const int ib = IBR(i, j, k);
+            if (ib >= 1 || a == aend) {
+              QAL(i / 2, j / 2, k / 2, l) += QTX2(bd2, cd2, l);
+              qa_sum_bnd += QTX2(bd2, cd2, l);
+              if (ib == 4) qa_to_melt_c += QTX2(bd2, cd2, l);
+              else if (ib == 3) qa_to_exit_c += QTX2(bd2, cd2, l);
+              else if (ib == 2) qa_to_inlet_c += QTX2(bd2, cd2, l);
+              else qa_to_wall_c += QTX2(bd2, cd2, l);
+            } else {
+              double qa0 = QTX2(bd2, cd2, l) * AKL(i / 2, j / 2, k / 2, l);
+              QAL(i / 2, j / 2, k / 2, l) += qa0;
+              QTX1(bd2, cd2, l) = QTX2(bd2, cd2, l) - qa0;
+              IQT1(bd2, cd2) = 0;
+              qa_sum_vol += qa0;
+              qa_to_open_c += qa0;
+            }
+            QTX2(bd2, cd2, l) = 0.0;
+            IQT2(bd2, cd2) = 1;
+          }
+        }
+      }
+      for (int n = 1; n <= Mb; n++) ANG(B, n, 1) = ANG(B, n, 2);
+      for (int n = 1; n <= Mc; n++) ANG(C_, n, 1) = ANG(C_, n, 2);
+    }
+  }
+}
+
+/* QREF, C/Qre.f90:31-505, for an open emitter (i0,j0,k0) with DEB set */
+static void QREF(int i0, int j0, int k0) {
+  const int i0d2 = i0 / 2, j0d2 = j0 / 2, k0d2 = k0 / 2;
+  qa_sum_bnd = qa_sum_vol = 0.0;
+  qa_to_open_c = qa_to_melt_c = qa_to_inlet_c = qa_to_exit_c = qa_to_wall_c = 0.0;
+  for (int l = 1; l <= NWL; l++) QEL(i0d2, j0d2, k0d2, l) = 0.0;
+  if (IBR(i0, j0, k0) >= 1) return;
+  double tk = T(i0, j0, k0) * C->t0;
+  double g0 = SGN * ipow4(tk);
+  for (int l = 1; l <= NWL; l++) QEL0[l - 1] = DEB[l - 1] * AKL(i0d2, j0d2, k0d2, l) * g0;
+  for (int s = 0; s < 3; s++) RAD_STALE_SET[s] = 0;
+  QREF_AXIS(1, 2, 3, i0, j0, k0, 101, 111, 100, 0);
+  QREF_AXIS(2, 1, 3, i0, j0, k0, 201, 211, 99, 1);
+  QREF_AXIS(3, 1, 2, i0, j0, k0, 301, 311, 1, 2);
+  double qe = QEL(i0d2, j0d2, k0d2, 1);
+  for (int l = 2; l <= NWL; l++) qe += QEL(i0d2, j0d2, k0d2, l);
+  QE(i0d2, j0d2, k0d2) = qe;
+  RC->qe_sum_from_open += qe;
+}
+
+/* the wall marks of RAD_EMIS, C/Rad_emis.f90:155-186, on the copy IBR_ */
+static void rad_mark_walls(void) {
+  for (int i = 2; i <= MP; i += 2)
+    for (int j = 2; j <= NP; j += 2)
+      for (int k = 2; k <= LP; k += 2) {
+        if (IBR(i, j, k) != 1) continue;
+        int km2 = imax(k - 2, 2), kp2 = imin(k + 2, LP), jm2 = imax(j - 2, 2), jp2 = imin(j + 2, NP);
+        int im2 = imax(i - 2, 2), ip2 = imin(i + 2, MP);
+        int16_t *c = &IBR_[IX3(i, j, k)];
+        if (IBR(i, j, km2) <= 0) *c = 301;
+        else if (IBR(i, j, kp2) <= 0) *c = 311;
+        else if (IBR(i, jm2, k) <= 0) *c = 201;
+        else if (IBR(i, jp2, k) <= 0) *c = 211;
+        else if (IBR(im2, j, k) <= 0) *c = 101;
+        else if (IBR(ip2, j, k) <= 0) *c = 111;
+      }
+}
+
+static int rad_bind(gfm_ms_ctrl *ms, const gfm_ms_arrays *a, gfm_rad_ctrl *rc, const gfm_rad_arrays *ra) {
+  static gfm_ms_ctrl msd;
+  static double *dummy = NULL;
+  static size_t cap = 0, capn = 0;
+  if (!rc || !ra || !ra->qe || !ra->qa || !ra->qal || !ra->akl || !a || !a->gfm || !a->smf) return GFM_E_ARG;
+  if (rc->nwl < 1 || rc->nwl > GFM_NWL_MAX || rc->ri <= 0.0 || rc->dnst_soot <= 0.0) return GFM_E_ARG;
+  for (int l = 0; l < rc->nwl; l++)
+    if (!(rc->wl[l] > 0.0)) return GFM_E_ARG;
+  if (!ms) ms = &msd;
+  gfm_ms_arrays aa = *a;
+  if (!aa.bs_soot_prev || !aa.sfp_soot_prev || !aa.rrms) {
+    if (!dummy || cap != NN) { free(dummy); dummy = calloc(NN, sizeof(double)); cap = NN; }
+    if (!dummy) return GFM_E_DEVICE;
+    if (!aa.bs_soot_prev) aa.bs_soot_prev = dummy;
+    if (!aa.sfp_soot_prev) aa.sfp_soot_prev = dummy;
+    if (!aa.rrms) aa.rrms = dummy;
+  }
+  int r = ms_bind(ms, &aa);
+  if (r) return r;
+  RC = rc;
+  NWL = rc->nwl;
+  for (int l = 0; l < NWL; l++) WL[l] = rc->wl[l];
+  AKL_ = ra->akl; QAL_ = ra->qal; QE_ = ra->qe; QA_ = ra->qa;
+  if (capn != NN) {
+    free(IBR_); free(QEL_); free(ANGA_[0]); free(ANGA_[1]); free(ANGA_[2]);
+    free(QTX1_); free(Q2M_); free(QTX2_); free(IQT1_); free(IQTM_); free(IQT2_);
+    size_t pmax = (size_t)(imax(imax(MP * NP, MP * LP), NP * LP) / 4 + 1);
+    IBR_ = malloc(sizeof(int16_t) * NN);
+    QEL_ = calloc(CNN * GFM_NWL_MAX, sizeof(double));
+    ANGA_[0] = calloc(2 * (size_t)MP, sizeof(double)); ANGA_[1] = calloc(2 * (size_t)NP, sizeof(double));
+    ANGA_[2] = calloc(2 * (size_t)LP, sizeof(double));
+    QTX1_ = calloc(pmax * GFM_NWL_MAX, sizeof(double)); Q2M_ = calloc(pmax * GFM_NWL_MAX, sizeof(double));
+    QTX2_ = calloc(pmax * GFM_NWL_MAX, sizeof(double));
+    IQT1_ = malloc(pmax); IQTM_ = malloc(pmax); IQT2_ = malloc(pmax);
+    if (!IBR_ || !QEL_ || !ANGA_[0] || !ANGA_[1] || !ANGA_[2] || !QTX1_ || !Q2M_ || !QTX2_ || !IQT1_ || !IQTM_ || !IQT2_)
+      return GFM_E_DEVICE;
+    capn = NN;
+  }
+  memcpy(IBR_, IBC, sizeof(int16_t) * NN);
+  return GFM_OK;
+}
+
+/* RAD_EMIS up to QRN1: wall marks, RAD0 (ID_RAD 10), QRNF's volume loop. */
+int oracle_rad_emis(gfm_ctrl *c, const gfm_host_arrays *h, const gfm_ms_arrays *a, gfm_rad_ctrl *rc,
+                    const gfm_rad_arrays *ra) {
+  int r = bind(c, h);
+  if (!r) r = rad_bind(NULL, a, rc, ra);
+  if (r) return r;
+  rad_mark_walls();
+  ID_RAD = 10;
+  RAD0();
+  memset(QE_, 0, sizeof(double) * CNN);
+  memset(QEL_, 0, sizeof(double) * CNN * NWL);
+  memset(QAL_, 0, sizeof(double) * CNN * NWL);
+  memset(QA_, 0, sizeof(double) * CNN);
+  for (int d = 0; d < 3; d++) memset(ANGA_[d], 0, sizeof(double) * 2 * axis_len(d + 1));
+  rc->qe_sum_from_open = 0.0;
+  rc->qa_sum_to_melt = rc->qa_sum_to_exit = rc->qa_sum_to_inlet = rc->qa_sum_to_wall = rc->qa_sum_to_open = 0.0;
+  double qeg0 = 0.0;
+  int l_ab = 0, n_emit = 0;
+  const long long prev0 = RAD_STALE_PREV, oob0 = RAD_OOB;
+  for (int i = 2; i <= MP; i += 2)
+    for (int j = 2; j <= NP; j += 2)
+      for (int k = 2; k <= LP; k += 2) {
+        const int ibc0 = IBR(i, j, k);
+        if (ibc0 < 1) l_ab++;
+        if (ibc0 >= 1) continue;
+        if (T(i, j, k) * C->t0 <= 300.0) continue;
+        DEBX(T(i, j, k) * C->t0);
+        QREF(i, j, k);
+        n_emit++;
+        rc->qa_sum_to_melt += qa_to_melt_c;
+        rc->qa_sum_to_exit += qa_to_exit_c;
+        rc->qa_sum_to_inlet += qa_to_inlet_c;
+        rc->qa_sum_to_wall += qa_to_wall_c;
+        rc->qa_sum_to_open += qa_to_open_c;
+        if (ibc0 <= 0) qeg0 += QE(i / 2, j / 2, k / 2);
+      }
+  rc->l_ab = l_ab;
+  rc->n_emit = n_emit;
+  rc->carry_unset = (int32_t)(RAD_STALE_PREV - prev0);
+  rc->angle_oob = (int32_t)(RAD_OOB - oob0);
+  rc->qeg0 = qeg0;
+  /* QRN1, C/Qrn.f90:245-290 */
+  double qg = 0.0, qio = 0.0, qs = 0.0, qw = 0.0;
+  for (int i = 2; i <= MP; i += 2)
+    for (int j = 2; j <= NP; j += 2)
+      for (int k = 2; k <= LP; k += 2) {
+        const int ibc0 = IBR(i, j, k);
+        double g0 = 0.0;
+        for (int l = 1; l <= NWL; l++) g0 += QAL(i / 2, j / 2, k / 2, l);
+        QA(i / 2, j / 2, k / 2) = g0;
+        if (ibc0 <= 0) qg += g0;
+        else if (ibc0 == 2 || ibc0 == 3) qio += g0;
+        else if (ibc0 == 4) qs += g0;
+        else qw += g0;
+      }
+  rc->qeg_g = qg; rc->qeg_io = qio; rc->qeg_s = qs; rc->qeg_w = qw;
+  return GFM_OK;
+}
+
+/* rad_qe_only, C/Rad_emis.f90:411-466: QE of every open cell (RAD0, ID_RAD 1);
+ * QE = 0 elsewhere (qe = qe2). */
+int oracle_rad_qe_only(gfm_ctrl *c, const gfm_host_arrays *h, const gfm_ms_arrays *a, gfm_rad_ctrl *rc,
+                       const gfm_rad_arrays *ra) {
+  int r = bind(c, h);
+  if (!r) r = rad_bind(NULL, a, rc, ra);
+  if (r) return r;
+  ID_RAD = 1;
+  memset(QE_, 0, sizeof(double) * CNN);
+  RAD0();
+  return GFM_OK;
+}
+/* single functions for the pins: 1 DEBX(x[0]) -> out[0..nwl-1]; 2 RIF(3, 0, x[0]); 3 the band model of one
+ * cell state x = (T K, P atm - 1, wmix-free YH2O, YCO2, FVS, OPL) -> out = AKLG, AKLS (nwl each) */
+int oracle_rad_probe(gfm_rad_ctrl *rc, int what, const double *x, double *out) {
+  if (!rc || rc->nwl < 1 || rc->nwl > GFM_NWL_MAX) return GFM_E_ARG;
+  RC = rc;
+  NWL = rc->nwl;
+  for (int l = 0; l < NWL; l++) WL[l] = rc->wl[l];
+  SGN = sig_sb * rc->ri * rc->ri;
+  for (int l = 0; l < NWL; l++) DEB0[l] = hck / rc->ri / WL[l];
+  switch (what) {
+    case 1: DEBX(x[0]); for (int l = 0; l < NWL; l++) out[l] = DEB[l]; break;
+    case 2: out[0] = RIF(3, 0.0, x[0]); break;
+    case 3:
+      for (int l = 0; l < NWL; l++) AKLS[l] = 0.0;
+      T_RAD = x[0]; TT0 = sqrt(T_RAD / t0_r); P_RAD = 1.0 + x[1];
+      YH2O = x[2]; YCO2 = x[3]; FVS = x[4]; OPL = x[5];
+      RAD_GAS();
+      for (int l = 0; l < NWL; l++) { out[l] = AKLG[l]; out[NWL + l] = AKLS[l]; }
+      break;
+    default: return GFM_E_ARG;
+  }
+  return GFM_OK;
+}
+size_t oracle_rad_ctrl_size(void) { return sizeof(gfm_rad_ctrl); }
+/* uses of the carried wall-split column (total, whose value came from an earlier call), out-of-range angle reads */
+void oracle_rad_stats(long long *out) { out[0] = RAD_STALE_USES; out[1] = RAD_STALE_PREV; out[2] = RAD_OOB; }

@@ -228,3 +228,52 @@ def ms_probe(case, ms, what, nelm=0):
     out = np.zeros(35)
     _ms_call("oracle_ms_probe", case, ms, MS_PROBES[what], int(nelm), _dp(out))
     return out
+
+
+# ---- radiation transport (RAD_EMIS / RAD0 / QRNF / rad_qe_only) ----
+def _rad_call(fn, case, ms, rad):
+    from gfm_amd.abi import host_arrays, ms_arrays, rad_arrays
+    h = host_arrays(case.arrays)
+    a = ms_arrays(ms.arrays)
+    ra = rad_arrays(rad.arrays)
+    rc = getattr(lib(), fn)(ct.byref(case.ctrl), ct.byref(h), ct.byref(a), ct.byref(rad.ctrl), ct.byref(ra))
+    if rc:
+        raise RuntimeError(f"{fn} failed: {rc}")
+
+
+def rad_emis(case, ms, rad):
+    """One full radiation pass up to QRN1 on the host state (ms: the minor
+    species with H2O, CO2 and soot; rad: gfm_amd.radiation.Radiation)."""
+    _rad_call("oracle_rad_emis", case, ms, rad)
+
+
+def rad_qe_only(case, ms, rad):
+    _rad_call("oracle_rad_qe_only", case, ms, rad)
+
+
+RAD_PROBES = {"debx": 1, "rif": 2, "bands": 3}
+
+
+def rad_probe(rad_ctrl, what, x):
+    """'debx' (x = [T K]) -> DEB per band; 'rif' (x = [hc/(k lambda T)]) ->
+    RIF(3, 0, x); 'bands' (x = [T K, P - 1 atm, YH2O, YCO2, soot volume
+    fraction, OPL m]) -> (AKLG, AKLS)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = rad_ctrl.nwl
+    out = np.zeros(2 * n)
+    rc = lib().oracle_rad_probe(ct.byref(rad_ctrl), RAD_PROBES[what], _dp(x), _dp(out))
+    if rc:
+        raise RuntimeError(f"oracle_rad_probe failed: {rc}")
+    if what == "rif":
+        return out[0]
+    if what == "debx":
+        return out[:n]
+    return out[:n], out[n:]
+
+
+def rad_stats():
+    """(uses of the carried wall-split column, those whose value came from
+    an earlier call, out-of-range angle reads) since the library loaded."""
+    out = (ct.c_longlong * 3)()
+    lib().oracle_rad_stats(out)
+    return tuple(out)
