@@ -537,6 +537,7 @@ struct QrefAx {  // one axis of an item: slab axis A, first spreading axis B, se
   int A, B, C, a0, b0, c0, Ma, Mb, Mc, PLB, PLC, mk1, mk2, split_gt;
   double *angB, *angC;  // LDS, [(n-1) + M*(col-1)]
   unsigned char *iqt1, *iqtm, *iqt2;
+  short *code;                // LDS: cell codes of the current slab, [p]
   double *qtx1, *q2m, *qtx2;  // global scratch, [l*pn + p]
   long long pn;
   double ang0B, ang0C;
@@ -618,10 +619,8 @@ __device__ void rad_qrt(const RadDev &r, const QrefAx &x, int a10, int a1, int &
           for (int l = 0; l < NB; l++) q12[l] = q11[l];
           int b2;
           for (b2 = b1; bpm < 0 ? b2 >= bend : b2 <= bend; b2 += bpm) {
-            int i, j, k;
-            rad_abc_ijk(x, a1, b2, c1, i, j, k);
             const long long p2 = rad_p(x, b2, c1);
-            if (r.ibc[rad_cc(r, i, j, k)] < 1) {
+            if (x.code[p2] < 1) {
               int b2m = b2 + bpm / 2;
               if (b2m < 1 || b2m > x.Mb) { atomicAdd(&cnt[1], 1); b2m = b2m < 1 ? 1 : x.Mb; }
               const double a22 = x.angB[(b2m - 1) + x.Mb];
@@ -697,9 +696,7 @@ __device__ void rad_qrt(const RadDev &r, const QrefAx &x, int a10, int a1, int &
                 if (c1 != x.c0 || cm > 0) x.q2m[l * x.pn + p1] = 0.0;
               }
             if (c1 != x.c0 || cm > 0) x.iqtm[p1] = 1;
-            int i, j, k;
-            rad_abc_ijk(x, a1, b1, c1, i, j, k);
-            const int ibc1 = r.ibc[rad_cc(r, i, j, k)];
+            const int ibc1 = x.code[p1];
             if (ibc1 == x.mk1 || ibc1 == x.mk2 || ibc1 == 4) {
 #pragma unroll
               for (int l = 0; l < NB; l++)
@@ -712,9 +709,8 @@ __device__ void rad_qrt(const RadDev &r, const QrefAx &x, int a10, int a1, int &
             for (int l = 0; l < NB; l++) q12[l] = q11[l];
             for (int c2 = c1; cm < 0 ? c2 >= cend : c2 <= cend; c2 += cm) {
               const int c2m = c2 + cm / 2;
-              rad_abc_ijk(x, a1, b1, c2, i, j, k);
-              const int ibc2 = r.ibc[rad_cc(r, i, j, k)];
               const long long p2 = rad_p(x, b1, c2);
+              const int ibc2 = x.code[p2];
               bool deposit = false;
               if (ibc1 <= 0 && ibc2 >= 1) {
                 deposit = true;
@@ -786,12 +782,14 @@ __device__ void rad_qrt(const RadDev &r, const QrefAx &x, int a10, int a1, int &
 
 // QREF for (emitter, axis) items, C/Qre.f90:95-505: the two directions
 // along the axis from the emitter, slab by slab.  Dynamic LDS: the angle
-// arrays of the three axes (2*(MP+NP+LP) doubles) and the three flag planes.
+// arrays of the three axes (2*(MP+NP+LP) doubles), the cell codes of the
+// current slab and the three flag planes.
 template <int NB>
 __global__ __launch_bounds__(64) void k_qref(RadDev r) {
   extern __shared__ double rsm[];
   const int lane = threadIdx.x;
-  unsigned char *flg = reinterpret_cast<unsigned char *>(rsm + 2 * (MP + NP + LP));
+  short *codes = reinterpret_cast<short *>(rsm + 2 * (MP + NP + LP));
+  unsigned char *flg = reinterpret_cast<unsigned char *>(codes + r.pmax);
   const long long nitem = r.ncell * 3;
   double *slot = r.scr + (long long)blockIdx.x * 3 * r.pmax * r.nwl;
   for (long long item = blockIdx.x; item < nitem; item += gridDim.x) {
@@ -820,6 +818,7 @@ __global__ __launch_bounds__(64) void k_qref(RadDev r) {
     x.angB = rsm + rad_pick(x.B, 0, 2 * MP, 2 * (MP + NP));
     x.angC = rsm + rad_pick(x.C, 0, 2 * MP, 2 * (MP + NP));
     x.iqt1 = flg; x.iqtm = flg + x.pn; x.iqt2 = flg + 2 * x.pn;
+    x.code = codes;
     x.qtx1 = slot; x.q2m = slot + r.pmax * r.nwl; x.qtx2 = slot + 2 * r.pmax * r.nwl;
     for (long long p = lane; p < x.pn * r.nwl; p += 64) { x.qtx1[p] = 0.0; x.q2m[p] = 0.0; x.qtx2[p] = 0.0; }
     for (long long p = lane; p < 3 * x.pn; p += 64) flg[p] = 1;
@@ -848,6 +847,12 @@ __global__ __launch_bounds__(64) void k_qref(RadDev r) {
       __syncthreads();
       const int ae = dir == 0 ? 2 : x.Ma;
       for (int a = x.a0 + apm; apm < 0 ? a >= ae : a <= ae; a += apm) {
+        for (int p = lane; p < x.pn; p += 64) {
+          int i, j, k;
+          rad_abc_ijk(x, a, 2 * (p % x.PLB) + 2, 2 * (p / x.PLB) + 2, i, j, k);
+          codes[p] = r.ibc[rad_cc(r, i, j, k)];
+        }
+        __syncthreads();
         rad_qrt<NB>(r, x, a - apm, a, stale, stale_set, r.cnt);
         // absorption in slab a
         for (long long p = lane; p < x.pn; p += 64) {
@@ -856,7 +861,7 @@ __global__ __launch_bounds__(64) void k_qref(RadDev r) {
           int i, j, k;
           rad_abc_ijk(x, a, b, c, i, j, k);
           const long long ce = rad_cc(r, i, j, k);
-          const int code = r.ibc[ce];
+          const int code = codes[p];
           for (int l = 0; l < r.nwl; l++) {
             const double v = x.qtx2[l * x.pn + p];
             if (v < SMALL16) continue;
@@ -1017,7 +1022,7 @@ static int rad_alloc(gfm_ctx *c, int nwl) {
     h->nslot = (int)ns;
     h->nwl_cap = nwl;
   }
-  h->lds = (size_t)2 * (k.mp + k.np + k.lp) * sizeof(double) + (size_t)3 * r.pmax;
+  h->lds = (size_t)2 * (k.mp + k.np + k.lp) * sizeof(double) + (size_t)r.pmax * sizeof(short) + (size_t)3 * r.pmax;
   return GFM_OK;
 }
 
