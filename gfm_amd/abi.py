@@ -152,6 +152,73 @@ def rad_arrays(arrs):
     return a
 
 
+NEX_MAX = 32      # GFM_NEX_MAX
+NWALG_MAX = 64    # GFM_NWALG_MAX
+I16P = ct.POINTER(ct.c_int16)
+
+
+class GfmWallCtrl(ct.Structure):
+    """gfm_wall_ctrl: inputs, carried state and outputs of the wall exchange (C/Qwall.f90)."""
+    _fields_ = [
+        ("maxri1", I32), ("maxri2", I32), ("minri2", I32), ("preset_vf", I32),
+        ("have_crown", I32), ("nex", I32), ("nwal0", I32), ("isoot_cal", I32),
+        ("eps_w", D), ("eps_m", D), ("eps_c", D), ("height_to_ceiling", D),
+        ("t_a", D), ("h_a", D), ("w_d", D), ("w_k", D), ("h_g", D),
+        ("wall_radiosity_tolerance", D), ("radiation_boundary_tolerance", D),
+        ("exh_box", (I32 * 6) * NEX_MAX),
+        ("exh_orient", I32 * NEX_MAX), ("exh_direct", I32 * NEX_MAX), ("exh_type", I32 * NEX_MAX),
+        ("exh_frac", D * NEX_MAX), ("exh_fixed", D * NEX_MAX),
+        ("w1_d", D * NWALG_MAX), ("w1_k", D * NWALG_MAX), ("w1_ha", D * NWALG_MAX), ("w1_ta", D * NWALG_MAX),
+        ("q_melt_req", D), ("rf_aform", D), ("rf_afover", D),
+        ("i1st_rad_done", I32), ("iterw", I32),
+        ("avg_wall_T", D), ("aform", D), ("qa_s0", D), ("qa_s1", D),
+        ("b_cnt", I32), ("iwall_count", I32), ("imelt_count", I32), ("inlet_count", I32), ("iexit_count", I32),
+        ("pad0", I32),
+        ("wall_area_total", D), ("melt_area_total", D), ("inlet_area_total", D), ("exit_area_total", D),
+        ("boundary_area_total", D),
+        ("sweeps", I32), ("outer", I32),
+        ("q_incident_total", D), ("q_incident_wall", D), ("q_incident_inlet", D), ("q_incident_exit", D),
+        ("q_incident_melt", D), ("qc_avg", D),
+        ("qconv_total", D), ("qconv_avg", D), ("q_wall_loss", D), ("qamb_avg", D), ("q_melt", D), ("qrs_avg", D),
+        ("q_rad_inlet", D), ("q_rad_outlet", D), ("qm_flame", D), ("qm_bnds", D), ("qm_conv", D),
+        ("rel_rad_chg_last", D),
+    ]
+
+    def copy(self):
+        c = GfmWallCtrl()
+        ct.memmove(ct.byref(c), ct.byref(self), ct.sizeof(GfmWallCtrl))
+        return c
+
+
+WALL_I16 = ("b_typ", "b_i", "b_j", "b_k", "b_orient", "b_direct", "bc_i", "bc_j", "bc_k", "b_exh")
+WALL_F64 = ("patch_area", "b_face_area", "emis", "radiosity", "q_amb", "q_conv", "qc", "qeb", "b_t")
+WALL_OTHER = ("vf", "vf_sum", "qrs", "wall_rad_t", "conv_log")
+
+
+class GfmWallArrays(ct.Structure):
+    _fields_ = ([(k, I16P) for k in WALL_I16] + [(k, DP) for k in WALL_F64] + [(k, DP) for k in WALL_OTHER] +
+                [("log_cap", I32), ("pad", I32)])
+
+
+def wall_arrays(arrs):
+    """GfmWallArrays pointing at the numpy arrays of a wall state (a missing
+    or None entry is a null pointer)."""
+    a = GfmWallArrays()
+    for k in WALL_I16:
+        v = arrs.get(k)
+        if v is not None:
+            assert v.dtype == np.int16 and v.flags["C_CONTIGUOUS"], k
+            setattr(a, k, v.ctypes.data_as(I16P))
+    for k in WALL_F64 + WALL_OTHER:
+        v = arrs.get(k)
+        if v is not None:
+            assert v.dtype == np.float64 and v.flags["F_CONTIGUOUS"], k
+            setattr(a, k, v.ctypes.data_as(DP))
+    log = arrs.get("conv_log")
+    a.log_cap = 0 if log is None else int(log.size)
+    return a
+
+
 def _ptr(a, ctype=ct.c_double):
     assert a.flags["F_CONTIGUOUS"] or a.ndim == 1, "arrays must be Fortran-ordered"
     return a.ctypes.data_as(ct.POINTER(ctype))

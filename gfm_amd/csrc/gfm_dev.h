@@ -75,6 +75,9 @@ struct DevCase {
   // QA and QE (W) of the last radiation pass on the cell centres, (id-1) + MZ*((jd-1) + NZ*(kd-1));
   // null until one has run (enthalpy source GFM_RADSRC_FIELD)
   const double *rad_qa, *rad_qe;
+  // q_conv (W/m2) of each wall and melt patch of the last wall exchange at its face (lattice), 0 elsewhere;
+  // null until one has run (the enthalpy sink of C/Source.f90:473-509 while i1st_rad_done > 0)
+  const double *wall_qcf;
 };
 
 // device scalar block indices

@@ -67,6 +67,8 @@ struct gfm_ctx {
   std::map<std::string, int> tune_maxbs, tune_map0;
   int tune_cand = -1;
   std::vector<int16_t> ibh;  // cell codes, reference layout (lists built after init)
+  std::vector<double> hx, hdx, hr, hdr, hz, hdz;  // the 1-D grid (1-based index i at [i-1])
+  struct WallHost *wall = nullptr;  // wall exchange, at gfm_hip_wall_setup
   MsHost *ms = nullptr;       // minor-species state, at the first gfm_hip_ms_* call
   RadHost *rad = nullptr;     // radiation fields, at the first gfm_hip_rad_* call
   Prof prof;
@@ -996,6 +998,7 @@ __global__ __launch_bounds__(1024) void k_resid_rows_finalize(const double *part
 // ---------------------------------------------------------------------
 #include "gfm_minor.hip"
 #include "gfm_rad.hip"
+#include "gfm_wall.hip"
 
 extern "C" {
 
@@ -1115,6 +1118,9 @@ int gfm_hip_init(const gfm_ctrl *k, const gfm_host_arrays *h, int device, gfm_ct
     HCHECK(hipMemcpy(g1[q], h1[q], n1[q] * sizeof(double), hipMemcpyHostToDevice));
   }
   d.x = g1[0]; d.dx = g1[1]; d.r = g1[2]; d.dr = g1[3]; d.rs = g1[4]; d.z = g1[5]; d.dz = g1[6];
+  c->hx.assign(h->x, h->x + k->mp); c->hdx.assign(h->dx, h->dx + k->mp);
+  c->hr.assign(h->r, h->r + k->np); c->hdr.assign(h->dr, h->dr + k->np);
+  c->hz.assign(h->z, h->z + k->lp); c->hdz.assign(h->dz, h->dz + k->lp);
   double *kt[4];
   const double *hk[4] = {h->rtim, h->delrt, h->delr, h->timed};
   for (int q = 0; q < 4; q++) {

@@ -1084,6 +1084,15 @@ __global__ void k_scalar_assemble(Box b, int l, int igs, double sgm, double *par
           bs += (g.rad_qa[cc] - g.rad_qe[cc]) / (vol0 * g.qr0);
         }
         if (g.isothermal_walls) bs -= WALL_SINK(i, j, k, qw, qlo, qhi);
+        else if (g.i1st_rad_done > 0 && g.wall_qcf) {
+          // the patches of the last wall exchange, in patch order -x, +x, -y, +y, -z, +z (0 where none)
+          bs = bs - g.wall_qcf[lidx(i - 1, j, k)] / (DX(i) * g.r0 * g.qr0);
+          bs = bs - g.wall_qcf[lidx(i + 1, j, k)] / (DX(i) * g.r0 * g.qr0);
+          bs = bs - g.wall_qcf[lidx(i, j - 1, k)] / (DR(j) * g.r0 * g.qr0);
+          bs = bs - g.wall_qcf[lidx(i, j + 1, k)] / (DR(j) * g.r0 * g.qr0);
+          bs = bs - g.wall_qcf[lidx(i, j, k - 1)] / (DZ(k) * g.r0 * g.qr0);
+          bs = bs - g.wall_qcf[lidx(i, j, k + 1)] / (DZ(k) * g.r0 * g.qr0);
+        }
         if (itr_gas > 1) bs = g.rf_hsource * bs + (1.0 - g.rf_hsource) * g.bsh[q];
         g.bsh[q] = bs;
       } else if (nelg == 9) {  // k, C/Source.f90:529-572

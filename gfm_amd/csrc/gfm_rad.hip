@@ -972,6 +972,7 @@ __global__ __launch_bounds__(RAD_RED_NT) void k_rad_reduce(RadDev r, int id_rad)
 // ---------------- host ----------------
 struct RadHost {
   RadDev dv;
+  double qeg_s_last = 0.0;  // QRN1's melt-surface absorption of the last full pass (QRN_PRN)
   int nwl_cap = 0;
   int nslot = 0;
   size_t lds = 0;
@@ -1086,6 +1087,7 @@ static int rad_pass(gfm_ctx *c, gfm_rad_ctrl *rc, int id_rad) {
     rc->qeg_g = res[RAD_RES_QAG];
     rc->qeg_io = res[RAD_RES_QAIO];
     rc->qeg_s = res[RAD_RES_QAS];
+    h->qeg_s_last = res[RAD_RES_QAS];
     rc->qeg_w = res[RAD_RES_QAW];
     rc->l_ab = (int32_t)res[RAD_RES_LAB];
     rc->n_emit = (int32_t)res[RAD_RES_NEMIT];

@@ -90,19 +90,35 @@ class Radiation:
         return {k: getattr(self.ctrl, k) for k in RAD_SUMS}
 
 
-def reference_state(gd, sbc, rg, rs):
+def reference_state(gd, sbc, rg, rs, in_run=1):
     """A reference run's geometry and saved state for a radiation pass:
     gd, sbc -- paths of its grid and boundary-condition files; rg, rs --
     the bytes of its restart and minor-species records.  Returns (case,
     MinorSpecies, Radiation) with the records applied as INITSV and
-    INITSVM read them and the &RADIN inputs of the sbc file."""
+    INITSVM read them and the &RADIN inputs of the sbc file.  in_run = 1
+    (a later global iteration of the run, as the one that saved the
+    records) takes the melt-surface cells' temperature from the record;
+    0 (the first of a process) leaves it to the melt model's file."""
     from . import case as cs
     from . import gridfile, minor, restart, sbcfile
     g = gridfile.read_gd(gd)
     s = sbcfile.read_sbc(sbc)
     c = cs.from_grid(g, s.number("GMFR"))
     c.ctrl.itr_gas = 0
-    restart.apply_records(c, restart.read_records(rg, *c.shape))
+    restart.apply_records(c, restart.read_records(rg, *c.shape), in_run=in_run)
+    # the inlet cells' temperature as SBC sets it before the records are read (C/Sbc.f90:90-105; the first
+    # block holding a cell sets it); a restart reads only k and epsilon there
+    ib, t = c.arrays["ibcell"], c.arrays["t"]
+    taken = np.zeros(ib.shape, dtype=bool)
+    for inl in s.inlets:
+        i1, j1, k1, i2, j2, k2 = inl.box
+        for I in range(i1, i2 + 1, 2):
+            for J in range(j1, j2 + 1, 2):
+                for K in range(k1, k2 + 1, 2):
+                    q = (I - 1, J - 1, K - 1)
+                    if ib[q] == 2 and not taken[q]:
+                        taken[q] = True
+                        t[q] = inl.T / c.ctrl.t0
     ms = minor.MinorSpecies(c)
     ms.load_rs(restart.read_rs(rs, *c.shape))
     return c, ms, Radiation(c, **radin_from_sbc(s))

@@ -14,7 +14,7 @@ import os
 
 import numpy as np
 
-from .abi import GfmCtrl, GfmMsCtrl, GfmRadCtrl, host_arrays, ms_arrays, rad_arrays
+from .abi import GfmCtrl, GfmMsCtrl, GfmRadCtrl, GfmWallCtrl, host_arrays, ms_arrays, rad_arrays, wall_arrays
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgfm_hip.so")
 _lib = None
@@ -59,6 +59,11 @@ def lib():
         L.gfm_hip_rad_download.argtypes = [P, ct.POINTER(GfmRadCtrl), P]
         L.gfm_hip_rad_upload.argtypes = [P, P]
         L.gfm_hip_rad_ctrl_size.restype = ct.c_size_t
+        L.gfm_hip_wall_setup.argtypes = [P, ct.POINTER(GfmWallCtrl), P]
+        L.gfm_hip_wallq.argtypes = [P, ct.POINTER(GfmWallCtrl)]
+        L.gfm_hip_wall_download.argtypes = [P, ct.POINTER(GfmWallCtrl), P]
+        L.gfm_hip_wall_upload.argtypes = [P, ct.POINTER(GfmWallCtrl), P]
+        L.gfm_hip_wall_ctrl_size.restype = ct.c_size_t
         L.gfm_hip_kernel_times.argtypes = [P, ct.c_int, ct.c_char_p, ct.c_int, D, D, ct.POINTER(ct.c_int), ct.c_int,
                                            ct.POINTER(ct.c_int)]
         _lib = L
@@ -175,6 +180,38 @@ class CombustionSpace:
         radiation pass) for the enthalpy source GFM_RADSRC_FIELD."""
         a = rad_arrays(rad.arrays)
         _check(lib().gfm_hip_rad_upload(self._ctx, ct.cast(ct.byref(a), ct.c_void_p)), "gfm_hip_rad_upload")
+
+    def wall_setup(self, wall, iwall=None):
+        """prepare_for_wall_radiation_calculation on the device's cell codes
+        (wall: gfm_amd.wall.Wall with its inputs in ctrl; iwall: the wall
+        group of each cell or None); allocates and downloads wall's patch
+        arrays."""
+        iw = None if iwall is None else np.ascontiguousarray(iwall.ravel(order="F"), dtype=np.int16)
+        _check(lib().gfm_hip_wall_setup(self._ctx, ct.byref(wall.ctrl),
+                                        None if iw is None else iw.ctypes.data_as(ct.c_void_p)), "gfm_hip_wall_setup")
+        wall.allocate(wall.ctrl.b_cnt)
+        return self.wall_download(wall)
+
+    def wallq(self, wall, download=True):
+        """One radiation pass's wall exchange after rad_emis (wall.ctrl
+        carries i1st_rad_done, iterw, the mean wall temperature and q_melt
+        between passes and receives the sums)."""
+        _check(lib().gfm_hip_wallq(self._ctx, ct.byref(wall.ctrl)), "gfm_hip_wallq")
+        return self.wall_download(wall) if download else wall
+
+    def wall_download(self, wall):
+        a = wall_arrays(wall.arrays)
+        _check(lib().gfm_hip_wall_download(self._ctx, ct.byref(wall.ctrl), ct.cast(ct.byref(a), ct.c_void_p)),
+               "gfm_hip_wall_download")
+        return wall
+
+    def wall_upload(self, wall):
+        """The patch state of a radiation record (radiosity, q_conv, b_t, ...)
+        onto the device; the enthalpy sink takes the uploaded q_conv."""
+        a = wall_arrays(wall.arrays)
+        _check(lib().gfm_hip_wall_upload(self._ctx, ct.byref(wall.ctrl), ct.cast(ct.byref(a), ct.c_void_p)),
+               "gfm_hip_wall_upload")
+        return wall
 
     def set_pguess(self, net):
         """Attach a gfm_amd.pnet.PNet as the learned p' initial guess (None

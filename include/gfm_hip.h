@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define GFM_ABI_VERSION 5
+#define GFM_ABI_VERSION 6
 
 enum {
   GFM_OK = 0,
@@ -429,6 +429,95 @@ int gfm_hip_rad_qe_only(gfm_ctx *ctx, gfm_rad_ctrl *rc);
 int gfm_hip_rad_download(gfm_ctx *ctx, const gfm_rad_ctrl *rc, gfm_rad_arrays *a);
 int gfm_hip_rad_upload(gfm_ctx *ctx, const gfm_rad_arrays *a);
 size_t gfm_hip_rad_ctrl_size(void);
+
+/* ---------------------------------------------------------------------
+ * Wall radiosity exchange (C/Qwall.f90), the enclosure part of a radiation
+ * pass that QRNF runs after its volume loop (C/Qrn.f90:218-221).
+ *
+ * gfm_hip_wall_setup is prepare_for_wall_radiation_calculation
+ * (C/Qwall.f90:54-566, called by INITSV on a new start): the cell codes
+ * with each exit tunnel turned into wall up to an exit at its interior end
+ * (MAPB), one boundary patch per face between a wall, inlet, exit or melt
+ * cell and an open cell -- x-normal faces first, then y, then z, each in
+ * (K, J, I) loop order -- with its type, position, normal, the centre of
+ * its boundary cell, area and emissivity, its exhaust, the normalised view
+ * factors VF(n,m) (preset_vf = 1; kept on the device), the area of all
+ * patches on its boundary cell and the totals.  iwall (MP/2,NP/2,LP/2, the
+ * wall group of each cell, SBC_B C/Sbc.f90:402-456) may be null.
+ *
+ * gfm_hip_wallq is one radiation pass's wall part: the patch temperatures
+ * and radiosities at the first pass (C/Rad_emis.f90:190-212; later passes
+ * refresh the melt patches, :253-261), wall_q_amb (C/Qwall.f90:1585-1659),
+ * wallq (C/Qwall.f90:990-1369: qc from QA of the last gfm_hip_rad_emis,
+ * the outer loop of calc_q_conv_wall, Gauss-Seidel sweeps over the patches
+ * in patch order to the radiosity tolerance, wall temperatures and the melt
+ * heat flux qrs, to the melt-flux tolerance; the closing sums, the mean
+ * wall temperature, exhausts at a fraction of it, the soot calibration),
+ * and QRN_PRN's QA_S0/QA_S1 (C/Qrn.f90:399-402).  Then the state the next
+ * gas iterations see (C/Rad_emis.f90:337-355, rad_emis_rr :101-113): the
+ * wall temperature at every wall patch face and at its wall cell (with its
+ * enthalpy), and each wall and melt patch's q_conv as the enthalpy sink of
+ * its open cell (C/Source.f90:473-509), which every later SIMPLER
+ * iteration applies while gfm_ctrl.i1st_rad_done is 1 (set it after the
+ * first call).  The Gauss-Seidel sweep runs in one workgroup; each row's
+ * sum over the other patches is split into the part outside a 64-patch
+ * block (all waves) and the block's own part (in order), so sweeps follow
+ * the reference's order and differ from it only in the rounding of those
+ * sums.
+ * ------------------------------------------------------------------- */
+#define GFM_NEX_MAX 32     /* exhausts */
+#define GFM_NWALG_MAX 64   /* wall property groups */
+typedef struct gfm_wall_ctrl {
+  /* inputs: &RADIN (C/Sbc.f90:316-330), exhausts (:272-284), wall groups (:288-311) */
+  int32_t maxri1, maxri2, minri2, preset_vf;
+  int32_t have_crown, nex, nwal0, isoot_cal;
+  double eps_w, eps_m, eps_c, height_to_ceiling;
+  double t_a, h_a, w_d, w_k, h_g;          /* default wall: ambient T (K), outside coefficient, thickness,
+                                              conductivity; inside coefficient h_g (W/m2K) */
+  double wall_radiosity_tolerance;         /* 1e-10, C/Qwall.f90:1036 */
+  double radiation_boundary_tolerance;     /* 1e-10, C/Qwall.f90:1037 */
+  int32_t exh_box[GFM_NEX_MAX][6];         /* exh_i1, j1, k1, i2, j2, k2 */
+  int32_t exh_orient[GFM_NEX_MAX], exh_direct[GFM_NEX_MAX], exh_type[GFM_NEX_MAX];
+  double exh_frac[GFM_NEX_MAX], exh_fixed[GFM_NEX_MAX];
+  double w1_d[GFM_NWALG_MAX], w1_k[GFM_NWALG_MAX], w1_ha[GFM_NWALG_MAX], w1_ta[GFM_NWALG_MAX];
+  double q_melt_req, rf_aform, rf_afover;  /* soot calibration (isoot_cal = 1) */
+  /* carried between passes (in and out) */
+  int32_t i1st_rad_done;                   /* 0: initialise b_T and radiosity from the gas state */
+  int32_t iterw;                           /* Gauss-Seidel sweeps so far */
+  double avg_wall_T;                       /* mean wall temperature (K); 1600 before the first pass */
+  double aform;                            /* soot formation constant (adjusted by the soot calibration) */
+  double qa_s0, qa_s1;                     /* QRN_PRN: net radiation to the melt (MW) and its change */
+  /* outputs of gfm_hip_wall_setup */
+  int32_t b_cnt, iwall_count, imelt_count, inlet_count, iexit_count;
+  int32_t pad0;
+  double wall_area_total, melt_area_total, inlet_area_total, exit_area_total, boundary_area_total;
+  /* outputs of gfm_hip_wallq */
+  int32_t sweeps, outer;                   /* Gauss-Seidel sweeps and outer iterations of this call */
+  double q_incident_total, q_incident_wall, q_incident_inlet, q_incident_exit, q_incident_melt, qc_avg;
+  double qconv_total, qconv_avg, q_wall_loss, qamb_avg, q_melt, qrs_avg;
+  double q_rad_inlet, q_rad_outlet, qm_flame, qm_bnds, qm_conv;
+  double rel_rad_chg_last;                 /* mean relative radiosity change of the last sweep */
+} gfm_wall_ctrl;
+
+/* Host arrays of the patches (patch order) and the lattice fields of the
+ * wall exchange.  Any pointer may be null on download; upload (a radiation
+ * record, rr####c.d) needs radiosity, q_conv and b_t. */
+typedef struct gfm_wall_arrays {
+  int16_t *b_typ, *b_i, *b_j, *b_k, *b_orient, *b_direct, *bc_i, *bc_j, *bc_k, *b_exh;  /* (b_cnt) */
+  double *patch_area, *b_face_area, *emis, *radiosity, *q_amb, *q_conv, *qc, *qeb, *b_t;  /* (b_cnt) */
+  double *vf;          /* (b_cnt, b_cnt) column-major: VF(n,m) = vf[(n-1) + b_cnt*(m-1)] */
+  double *vf_sum;      /* (b_cnt): the sum of the row before normalising */
+  double *qrs;         /* (MP, NP): melt-surface heat flux (W/m2) */
+  double *wall_rad_t;  /* (MP, NP, LP): wall temperature (K) at wall cells, 0 elsewhere */
+  double *conv_log;    /* (log_cap): mean relative radiosity change of each sweep of the last call */
+  int32_t log_cap, pad;
+} gfm_wall_arrays;
+
+int gfm_hip_wall_setup(gfm_ctx *ctx, gfm_wall_ctrl *wc, const int16_t *iwall);
+int gfm_hip_wallq(gfm_ctx *ctx, gfm_wall_ctrl *wc);
+int gfm_hip_wall_download(gfm_ctx *ctx, const gfm_wall_ctrl *wc, gfm_wall_arrays *a);
+int gfm_hip_wall_upload(gfm_ctx *ctx, const gfm_wall_ctrl *wc, const gfm_wall_arrays *a);
+size_t gfm_hip_wall_ctrl_size(void);
 
 /* Per-kernel timing with HIP events recorded on the library's stream around
  * each launch.  enable=1 clears the records and starts recording, enable=0

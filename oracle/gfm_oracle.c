@@ -477,6 +477,9 @@ double oracle_emf(double xkl, double xl0) {
  * (TG_MN, C/Rad0.f90:42). */
 /* QA and QE (cell centres, W) of the enthalpy source GFM_RADSRC_FIELD; set by oracle_rad_field */
 static const double *RADF_QA = NULL, *RADF_QE = NULL;
+/* q_conv of each wall and melt patch at its face (lattice, 0 elsewhere); set by oracle_wall_field */
+static const double *WQCF_ = NULL;
+void oracle_wall_field(const double *qcf) { WQCF_ = qcf; }
 void oracle_rad_field(const double *qa, const double *qe) { RADF_QA = qa; RADF_QE = qe; }
 static double gray_qnet(int i, int j, int k) {
   double tk = T(i, j, k) * C->t0;
@@ -667,6 +670,16 @@ static void SOURCE(int mi1, int mi3, int nj1, int nj3, int lk1, int lk3) {
               BS(i, j, k) += (RADF_QA[cc] - RADF_QE[cc]) / (vol * C->qr0);
             }
             if (C->isothermal_walls) BS(i, j, k) -= wall_sink(i, j, k, &C->q_wall);
+            else if (C->i1st_rad_done > 0 && WQCF_) {
+              /* the patch q_conv of the last wall exchange, C/Source.f90:473-509: the patches of one
+               * open cell in patch order are its -x, +x, -y, +y, -z, +z faces; 0 where none */
+              BS(i, j, k) = BS(i, j, k) - WQCF_[IX3(i - 1, j, k)] / (DX(i) * C->r0 * C->qr0);
+              BS(i, j, k) = BS(i, j, k) - WQCF_[IX3(i + 1, j, k)] / (DX(i) * C->r0 * C->qr0);
+              BS(i, j, k) = BS(i, j, k) - WQCF_[IX3(i, j - 1, k)] / (DR(j) * C->r0 * C->qr0);
+              BS(i, j, k) = BS(i, j, k) - WQCF_[IX3(i, j + 1, k)] / (DR(j) * C->r0 * C->qr0);
+              BS(i, j, k) = BS(i, j, k) - WQCF_[IX3(i, j, k - 1)] / (DZ(k) * C->r0 * C->qr0);
+              BS(i, j, k) = BS(i, j, k) - WQCF_[IX3(i, j, k + 1)] / (DZ(k) * C->r0 * C->qr0);
+            }
           }
       if (C->itr_gas > 1) {
         for (int i = mi1; i <= mi3; i += 2)
@@ -3287,3 +3300,538 @@ int oracle_rad_probe(gfm_rad_ctrl *rc, int what, const double *x, double *out) {
 size_t oracle_rad_ctrl_size(void) { return sizeof(gfm_rad_ctrl); }
 /* uses of the carried wall-split column (total, whose value came from an earlier call), out-of-range angle reads */
 void oracle_rad_stats(long long *out) { out[0] = RAD_STALE_USES; out[1] = RAD_STALE_PREV; out[2] = RAD_OOB; }
+
+/* ======================================================================
+ * Wall radiosity exchange, C/Qwall.f90: prepare_for_wall_radiation_
+ * calculation (:54-566, without the debug prints and the wall-grid file of
+ * iwalvis), wall_q_amb (:1585-1659), wallq (:990-1369) with
+ * calc_q_conv_wall (:1370-1478) and get_radiation_from_boundaries
+ * (:1489-1583, preset view factors); the first-pass patch temperatures of
+ * RAD_EMIS (C/Rad_emis.f90:190-212, later passes :253-261), QRN_PRN's
+ * QA_S0/QA_S1 (C/Qrn.f90:399-402) and the state the next gas iterations
+ * see (C/Rad_emis.f90:337-355 and rad_emis_rr :101-113): T at every wall
+ * patch face, and at each wall cell the last wall patch's b_T with its
+ * enthalpy.  QRN2 only writes the GUI file (its sums of the removed wall
+ * absorption are zero) and is not restated; nor are the GUI poll and the
+ * log writes.
+ *
+ * Reference state kept: the corner variables of the exit search
+ * (kend, jend, iend, inew, ...) are not reset between exits, so an exit
+ * whose extent or interior end is not found reuses the last one found.
+ * ====================================================================== */
+static const double sig_w = 5.668e-8, t_mnk_w = 2.0e2, pi_w = 3.141592653589793;
+static int16_t *MAPB_ = NULL;
+#define MAPB(i, j, k) MAPB_[IX3(i, j, k)]
+/* mapb out of the lattice reads as "not open" (the reference never reads there for exits away from edges) */
+static int mapb_at(int i, int j, int k) {
+  if (i < 1 || i > MP || j < 1 || j > NP || k < 1 || k > LP) return 1;
+  return MAPB(i, j, k);
+}
+
+/* a write inside the lattice (an exit whose interior end was never found leaves a stale or zero index) */
+static void mapb_set(int i, int j, int k, int v) {
+  if (i >= 1 && i <= MP && j >= 1 && j <= NP && k >= 1 && k <= LP) MAPB(i, j, k) = (int16_t)v;
+}
+
+/* Steps 1-2: MAPB = IBCELL with each exit tunnel walled up to an exit at its interior end, C/Qwall.f90:74-296 */
+static void wall_mapb(void) {
+  for (size_t q = 0; q < NN; q++) MAPB_[q] = IBC[q];
+  int kstart = 0, kend = 0, jstart = 0, jend = 0, istart = 0, iend = 0, inew = 0, jnew = 0, knew = 0;
+  /* back wall, i = 2 */
+  for (int j = 2; j <= NP; j += 2)
+    for (int k = 2; k <= LP; k += 2) {
+      if (MAPB(2, j, k) != 3) continue;
+      kstart = k;
+      for (int kk = k + 2; kk <= LP; kk += 2) if (MAPB(2, j, kk) != 3) { kend = kk - 2; break; }
+      jstart = j;
+      for (int jj = j + 2; jj <= NP; jj += 2) if (MAPB(2, jj, k) != 3) { jend = jj - 2; break; }
+      for (int ii = 4; ii <= MP; ii += 2)
+        if (mapb_at(ii, jstart - 2, kstart - 2) == 0 || mapb_at(ii, jend + 2, kend + 2) == 0 ||
+            mapb_at(ii, jstart - 2, kend + 2) == 0 || mapb_at(ii, jend + 2, kstart - 2) == 0) { inew = ii - 2; break; }
+      for (int ii = inew; ii <= inew + 1; ii++)
+        for (int jj = jstart; jj <= jend; jj++)
+          for (int kk = kstart; kk <= kend; kk++) mapb_set(ii, jj, kk, 3);
+      for (int ii = 2; ii <= inew - 1; ii++)
+        for (int jj = jstart; jj <= jend; jj++)
+          for (int kk = kstart; kk <= kend; kk++) mapb_set(ii, jj, kk, 1);
+    }
+  /* front wall, i = MP */
+  for (int j = 2; j <= NP; j += 2)
+    for (int k = 2; k <= LP; k += 2) {
+      if (MAPB(MP, j, k) != 3) continue;
+      kstart = k;
+      for (int kk = k + 2; kk <= LP; kk += 2) if (MAPB(MP, j, kk) != 3) { kend = kk - 2; break; }
+      jstart = j;
+      for (int jj = j + 2; jj <= NP; jj += 2) if (MAPB(MP, jj, k) != 3) { jend = jj - 2; break; }
+      for (int ii = MP - 2; ii >= 2; ii -= 2)
+        if (mapb_at(ii, jstart - 2, kstart - 2) == 0 || mapb_at(ii, jend + 2, kend + 2) == 0 ||
+            mapb_at(ii, jstart - 2, kend + 2) == 0 || mapb_at(ii, jend + 2, kstart - 2) == 0) { inew = ii + 2; break; }
+      for (int ii = inew - 1; ii <= inew; ii++)
+        for (int jj = jstart; jj <= jend; jj++)
+          for (int kk = kstart; kk <= kend; kk++) mapb_set(ii, jj, kk, 3);
+      for (int ii = inew + 1; ii <= MP; ii++)
+        for (int jj = jstart; jj <= jend; jj++)
+          for (int kk = kstart; kk <= kend; kk++) mapb_set(ii, jj, kk, 1);
+    }
+  /* left wall, j = 2 */
+  for (int i = 2; i <= MP; i += 2)
+    for (int k = 2; k <= LP; k += 2) {
+      if (MAPB(i, 2, k) != 3) continue;
+      kstart = k;
+      for (int kk = k + 2; kk <= LP; kk += 2) if (MAPB(i, 2, kk) != 3) { kend = kk - 2; break; }
+      istart = i;
+      for (int ii = i + 2; ii <= MP; ii += 2) if (MAPB(ii, 2, k) != 3) { iend = ii - 2; break; }
+      for (int jj = 4; jj <= NP; jj += 2)
+        if (mapb_at(istart - 2, jj, kstart - 2) == 0 || mapb_at(iend + 2, jj, kend + 2) == 0 ||
+            mapb_at(istart - 2, jj, kend + 2) == 0 || mapb_at(iend + 2, jj, kstart - 2) == 0) { jnew = jj - 2; break; }
+      for (int ii = istart; ii <= iend; ii++)
+        for (int jj = jnew; jj <= jnew + 1; jj++)
+          for (int kk = kstart; kk <= kend; kk++) mapb_set(ii, jj, kk, 3);
+      for (int ii = istart; ii <= iend; ii++)
+        for (int jj = 2; jj <= jnew - 1; jj++)
+          for (int kk = kstart; kk <= kend; kk++) mapb_set(ii, jj, kk, 1);
+    }
+  /* right wall, j = NP */
+  for (int i = 2; i <= MP; i += 2)
+    for (int k = 2; k <= LP; k += 2) {
+      if (MAPB(i, NP, k) != 3) continue;
+      kstart = k;
+      for (int kk = k + 2; kk <= LP; kk += 2) if (MAPB(i, NP, kk) != 3) { kend = kk - 2; break; }
+      istart = i;
+      for (int ii = i + 2; ii <= MP; ii += 2) if (MAPB(ii, NP, k) != 3) { iend = ii - 2; break; }
+      for (int jj = NP - 2; jj >= 2; jj -= 2)
+        if (mapb_at(istart - 2, jj, kstart - 2) == 0 || mapb_at(iend + 2, jj, kend + 2) == 0 ||
+            mapb_at(istart - 2, jj, kend + 2) == 0 || mapb_at(iend + 2, jj, kstart - 2) == 0) { jnew = jj + 2; break; }
+      for (int ii = istart; ii <= iend; ii++)
+        for (int jj = jnew - 1; jj <= jnew; jj++)
+          for (int kk = kstart; kk <= kend; kk++) mapb_set(ii, jj, kk, 3);
+      for (int ii = istart; ii <= iend; ii++)
+        for (int jj = jnew + 1; jj <= NP; jj++)
+          for (int kk = kstart; kk <= kend; kk++) mapb_set(ii, jj, kk, 1);
+    }
+  /* top wall, k = LP */
+  for (int i = 2; i <= MP; i += 2)
+    for (int j = 2; j <= NP; j += 2) {
+      if (MAPB(i, j, LP) != 3) continue;
+      istart = i;
+      for (int ii = i + 2; ii <= MP; ii += 2) if (MAPB(ii, j, LP) != 3) { iend = ii - 2; break; }
+      jstart = j;
+      for (int jj = j + 2; jj <= NP; jj += 2) if (MAPB(i, jj, LP) != 3) { jend = jj - 2; break; }
+      for (int kk = LP - 2; kk >= 2; kk -= 2)
+        if (mapb_at(istart - 2, jstart - 2, kk) == 0 || mapb_at(iend + 2, jend + 2, kk) == 0 ||
+            mapb_at(istart - 2, jend + 2, kk) == 0 || mapb_at(iend + 2, jstart - 2, kk) == 0) { knew = kk + 2; break; }
+      for (int ii = istart; ii <= iend; ii++)
+        for (int jj = jstart; jj <= jend; jj++)
+          for (int kk = knew - 1; kk <= knew; kk++) mapb_set(ii, jj, kk, 3);
+      for (int ii = istart; ii <= iend; ii++)
+        for (int jj = jstart; jj <= jend; jj++)
+          for (int kk = knew + 1; kk <= LP; kk++) mapb_set(ii, jj, kk, 1);
+    }
+}
+
+/* emissivity of a patch, C/Qwall.f90:406-416, 443-453, 480-492 (z-normal: the crown test at LP-1) */
+static double wall_emis(const gfm_wall_ctrl *wc, int code, int k, int ktop) {
+  if (code == 1) {
+    if (Z(k) >= wc->height_to_ceiling || (k == ktop && wc->have_crown == 0)) return wc->eps_c;
+    return wc->eps_w;
+  }
+  if (code == 4 && ktop == LP - 1) return wc->eps_m;
+  return 1.0;
+}
+
+/* Steps 3-8 of prepare_for_wall_radiation_calculation.  With wa->b_typ null
+ * only counts (wc->b_cnt). */
+int oracle_wall_setup(gfm_ctrl *c, const gfm_host_arrays *h, gfm_wall_ctrl *wc, gfm_wall_arrays *wa,
+                      const int16_t *iwall) {
+  (void)iwall;
+  int r = bind(c, h);
+  if (r) return r;
+  if (!wc || !wa) return GFM_E_ARG;
+  free(MAPB_);
+  MAPB_ = malloc(sizeof(int16_t) * NN);
+  if (!MAPB_) return GFM_E_DEVICE;
+  wall_mapb();
+  int n = 0;
+  const int fill = wa->b_typ != NULL;
+  if (fill && wc->b_cnt <= 0) return GFM_E_ARG;
+  /* x-normal faces */
+  for (int k = 2; k <= LP; k += 2)
+    for (int j = 2; j <= NP; j += 2)
+      for (int i = 3; i <= MP - 1; i += 2) {
+        if (MAPB(i, j, k) < 1) continue;
+        if (MAPB(i - 1, j, k) > 0 && MAPB(i + 1, j, k) > 0) continue;
+        if (fill) {
+          wa->b_typ[n] = MAPB(i, j, k);
+          wa->emis[n] = wall_emis(wc, MAPB(i, j, k), k, LP);
+          wa->b_orient[n] = 1;
+          if (MAPB(i + 1, j, k) == 0) { wa->b_direct[n] = 0; wa->bc_i[n] = i - 1; }
+          else { wa->b_direct[n] = 1; wa->bc_i[n] = i + 1; }
+          wa->bc_j[n] = j; wa->bc_k[n] = k;
+          wa->b_i[n] = i; wa->b_j[n] = j; wa->b_k[n] = k;
+          wa->patch_area[n] = DR(j) * DZ(k);
+        }
+        n++;
+      }
+  /* y-normal faces */
+  for (int k = 2; k <= LP; k += 2)
+    for (int j = 3; j <= NP - 1; j += 2)
+      for (int i = 2; i <= MP; i += 2) {
+        if (MAPB(i, j, k) < 1) continue;
+        if (MAPB(i, j - 1, k) > 0 && MAPB(i, j + 1, k) > 0) continue;
+        if (fill) {
+          wa->b_typ[n] = MAPB(i, j, k);
+          wa->emis[n] = wall_emis(wc, MAPB(i, j, k), k, LP);
+          wa->b_orient[n] = 2;
+          if (MAPB(i, j + 1, k) == 0) { wa->b_direct[n] = 0; wa->bc_j[n] = j - 1; }
+          else { wa->b_direct[n] = 1; wa->bc_j[n] = j + 1; }
+          wa->bc_i[n] = i; wa->bc_k[n] = k;
+          wa->b_i[n] = i; wa->b_j[n] = j; wa->b_k[n] = k;
+          wa->patch_area[n] = DX(i) * DZ(k);
+        }
+        n++;
+      }
+  /* z-normal faces */
+  for (int k = 3; k <= LP - 1; k += 2)
+    for (int j = 2; j <= NP; j += 2)
+      for (int i = 2; i <= MP; i += 2) {
+        if (MAPB(i, j, k) < 1) continue;
+        if (MAPB(i, j, k - 1) > 0 && MAPB(i, j, k + 1) > 0) continue;
+        if (fill) {
+          wa->b_typ[n] = MAPB(i, j, k);
+          wa->emis[n] = wall_emis(wc, MAPB(i, j, k), k, LP - 1);
+          wa->b_orient[n] = 3;
+          if (MAPB(i, j, k + 1) == 0) { wa->b_direct[n] = 0; wa->bc_k[n] = k - 1; }
+          else { wa->b_direct[n] = 1; wa->bc_k[n] = k + 1; }
+          wa->bc_i[n] = i; wa->bc_j[n] = j;
+          wa->b_i[n] = i; wa->b_j[n] = j; wa->b_k[n] = k;
+          wa->patch_area[n] = DX(i) * DR(j);
+        }
+        n++;
+      }
+  if (!fill) { wc->b_cnt = n; return GFM_OK; }
+  if (n != wc->b_cnt) return GFM_E_ARG;
+  const int nb = n;
+  /* Step 6: exhausts */
+  for (int p = 0; p < nb; p++) {
+    wa->b_exh[p] = 0;
+    if (wa->b_typ[p] != 3) continue;
+    for (int m = 0; m < wc->nex; m++) {
+      if (wa->b_orient[p] != wc->exh_orient[m] || wa->b_direct[p] != wc->exh_direct[m]) continue;
+      const int *e = wc->exh_box[m];
+      int hit = 0;
+      if (wa->b_orient[p] == 1) hit = wa->bc_j[p] >= e[1] && wa->bc_j[p] <= e[4] && wa->bc_k[p] >= e[2] && wa->bc_k[p] <= e[5];
+      else if (wa->b_orient[p] == 2) hit = wa->bc_i[p] >= e[0] && wa->bc_i[p] <= e[3] && wa->bc_k[p] >= e[2] && wa->bc_k[p] <= e[5];
+      else hit = wa->bc_i[p] >= e[0] && wa->bc_i[p] <= e[3] && wa->bc_j[p] >= e[1] && wa->bc_j[p] <= e[4];
+      if (hit) { wa->b_exh[p] = (int16_t)(m + 1); break; }
+    }
+  }
+  /* Step 7: view factors */
+  if (wc->preset_vf == 1) {
+    if (!wa->vf || !wa->vf_sum) return GFM_E_ARG;
+    for (int p = 0; p < nb; p++) {
+      double s = 0.0;
+      for (int m = 0; m < nb; m++) {
+        double v = 0.0;
+        if (m != p) {
+          const double ddx = X(wa->b_i[p]) - X(wa->b_i[m]), ddy = R(wa->b_j[p]) - R(wa->b_j[m]),
+                       ddz = Z(wa->b_k[p]) - Z(wa->b_k[m]);
+          const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+          const double den = pi_w * (d2 * d2);
+          const double ex = X(wa->b_i[m]) - X(wa->b_i[p]), ey = R(wa->b_j[m]) - R(wa->b_j[p]), ez = Z(wa->b_k[m]) - Z(wa->b_k[p]);
+          const int om = wa->b_orient[m], dp = wa->b_direct[p];
+          int vis = 1;
+          double num = 0.0, e0 = 0.0;
+          if (wa->b_orient[p] == 1) {
+            if ((dp == 0 && wa->b_i[m] <= wa->b_i[p]) || (dp == 1 && wa->b_i[m] >= wa->b_i[p])) vis = 0;
+            e0 = ex; num = (om == 1) ? ex * ex : fabs(ex * (om == 2 ? ey : ez));
+          } else if (wa->b_orient[p] == 2) {
+            if ((dp == 0 && wa->b_j[m] <= wa->b_j[p]) || (dp == 1 && wa->b_j[m] >= wa->b_j[p])) vis = 0;
+            e0 = ey; num = (om == 2) ? ey * ey : fabs(ey * (om == 1 ? ex : ez));
+          } else {
+            if ((dp == 0 && wa->b_k[m] <= wa->b_k[p]) || (dp == 1 && wa->b_k[m] >= wa->b_k[p])) vis = 0;
+            e0 = ez; num = (om == 3) ? ez * ez : fabs(ez * (om == 1 ? ex : ey));
+          }
+          (void)e0;
+          if (vis) v = num * wa->patch_area[m] / den;
+        }
+        wa->vf[(size_t)p + (size_t)nb * m] = v;
+        s += v;
+      }
+      wa->vf_sum[p] = s;
+    }
+    for (int p = 0; p < nb; p++) {
+      if (wa->vf_sum[p] == 0.0) continue;
+      for (int m = 0; m < nb; m++) wa->vf[(size_t)p + (size_t)nb * m] /= wa->vf_sum[p];
+    }
+  }
+  /* Step 8: patches on the same boundary cell (sameCell1..4) and the totals */
+  int *sc = calloc((size_t)4 * nb, sizeof(int));  /* 1-based patch numbers, 0 = none */
+  if (!sc) return GFM_E_DEVICE;
+#define SC(q, s) sc[(size_t)(s) * nb + (q)]
+  for (int p = 0; p < nb; p++)
+    for (int m = p + 1; m < nb; m++) {
+      if (!(wa->bc_i[p] == wa->bc_i[m] && wa->bc_j[p] == wa->bc_j[m] && wa->bc_k[p] == wa->bc_k[m])) continue;
+      const int P1 = p + 1, M1 = m + 1;
+      if (SC(p, 0) == 0) { SC(p, 0) = M1; SC(m, 0) = P1; }
+      else if (SC(p, 1) == 0) {
+        if (SC(p, 0) == M1) continue;
+        SC(p, 1) = M1; SC(m, 0) = P1;
+        SC(m, 1) = SC(p, 0); SC(SC(p, 0) - 1, 1) = M1;
+      } else if (SC(p, 2) == 0) {
+        if (SC(p, 0) == M1 || SC(p, 1) == M1) continue;
+        SC(p, 2) = M1; SC(m, 0) = P1;
+        SC(m, 1) = SC(p, 0); SC(SC(p, 1) - 1, 2) = M1;
+        SC(m, 2) = SC(p, 1);
+      } else if (SC(p, 3) == 0) {
+        if (SC(p, 0) == M1 || SC(p, 1) == M1 || SC(p, 2) == M1) continue;
+        SC(p, 3) = M1; SC(m, 0) = P1;
+        SC(m, 1) = SC(p, 0); SC(SC(p, 1) - 1, 3) = M1;
+        SC(m, 2) = SC(p, 1); SC(SC(p, 2) - 1, 3) = M1;
+        SC(m, 3) = SC(p, 2);
+      }
+    }
+  wc->iwall_count = wc->imelt_count = wc->inlet_count = wc->iexit_count = 0;
+  wc->wall_area_total = wc->melt_area_total = wc->inlet_area_total = wc->exit_area_total = 0.0;
+  for (int p = 0; p < nb; p++) {
+    double a = wa->patch_area[p];
+    for (int s = 0; s < 4 && SC(p, s); s++) a = a + wa->patch_area[SC(p, s) - 1];
+    wa->b_face_area[p] = a;
+    const double pa = wa->patch_area[p];
+    if (wa->b_typ[p] == 1) { wc->iwall_count++; wc->wall_area_total += pa; }
+    else if (wa->b_typ[p] == 4) { wc->imelt_count++; wc->melt_area_total += pa; }
+    else if (wa->b_typ[p] == 2) { wc->inlet_count++; wc->inlet_area_total += pa; }
+    else { wc->iexit_count++; wc->exit_area_total += pa; }
+  }
+#undef SC
+  free(sc);
+  wc->boundary_area_total = wc->wall_area_total + wc->melt_area_total + wc->inlet_area_total + wc->exit_area_total;
+  return GFM_OK;
+}
+
+/* get_radiation_from_boundaries (preset view factors), C/Qwall.f90:1501-1507 */
+static double wall_rad_from(const gfm_wall_arrays *wa, int nb, int p) {
+  double s = 0.0;
+  for (int m = 0; m < nb; m++) s = s + wa->radiosity[m] * wa->vf[(size_t)p + (size_t)nb * m];
+  return s;
+}
+
+/* the open cell next to a patch and the face-to-centre distance, C/Qwall.f90:1401-1433 */
+static void wall_open_cell(const gfm_wall_arrays *wa, int p, int *ic, int *jc, int *kc, double *ds) {
+  int i = wa->b_i[p], j = wa->b_j[p], k = wa->b_k[p];
+  const int dn = wa->b_direct[p] == 0;
+  if (wa->b_orient[p] == 1) { if (dn) { *ds = X(i + 1) - X(i); i++; } else { *ds = X(i) - X(i - 1); i--; } }
+  else if (wa->b_orient[p] == 2) { if (dn) { *ds = R(j + 1) - R(j); j++; } else { *ds = R(j) - R(j - 1); j--; } }
+  else { if (dn) { *ds = Z(k + 1) - Z(k); k++; } else { *ds = Z(k) - Z(k - 1); k--; } }
+  *ic = i; *jc = j; *kc = k;
+}
+
+/* calc_q_conv_wall, C/Qwall.f90:1370-1478 */
+static void wall_q_conv(const gfm_wall_ctrl *wc, gfm_wall_arrays *wa, int nb, double *qconv_total) {
+  const double kappa_cmu = 0.4 * pow(9.0e-2, 0.25), e_cmu = 9.0 * pow(9.0e-2, 0.25);
+  const double c_htw = 0.4 * ((pi_w / 4) / sin(pi_w / 4)) * sqrt(26.0) * (0.7 - 1.0) / pow(0.7, 0.25);
+  double tot = 0.0;
+  for (int p = 0; p < nb; p++) {
+    if (wa->b_typ[p] == 2 || wa->b_typ[p] == 3) { wa->q_conv[p] = 0.0; continue; }
+    int i, j, k;
+    double ds;
+    wall_open_cell(wa, p, &i, &j, &k, &ds);
+    const double cpmix = 1.0057e3 * (cpf1 * GF(i, j, k, iyf) + (2.0e3 / 1.0057e3) * GF(i, j, k, iyco2) +
+                                     (2.0e3 / 1.0057e3) * GF(i, j, k, iyh2o) + (0.9136e3 / 1.0057e3) * GF(i, j, k, iyo2) +
+                                     (0.526e3 / 1.0057e3) * GF(i, j, k, iyn2));
+    const double tgas = T(i, j, k) * C->t0;
+    const double gmu_t = ((2.0e-15 * tgas + -1.1e-11) * tgas + 4.5e-8) * tgas + 7.0e-6;
+    double h_t;
+    if (C->tk0 * GF(i, j, k, ik) < 0.1) h_t = 0.0;
+    else {
+      const double sq = sqrt(C->tk0 * GF(i, j, k, ik));
+      const double dtmp = log(e_cmu * ds * C->dnst0 * DNST(i, j, k) * sq / gmu_t) + (c_htw / sq);
+      h_t = (dtmp < 0.01) ? 0.0 : dmin(50.0, cpmix * kappa_cmu * C->dnst0 * DNST(i, j, k) * sq / dtmp);
+    }
+    wa->q_conv[p] = dmax(wc->h_g, h_t) * (tgas - wa->b_t[p]);
+    tot = tot + wa->q_conv[p] * wa->patch_area[p];
+  }
+  *qconv_total = tot;
+}
+
+/* wall_q_amb, C/Qwall.f90:1585-1659 */
+static void wall_q_amb(gfm_wall_ctrl *wc, gfm_wall_arrays *wa, int nb, const int16_t *iwall) {
+  double loss = 0.0;
+  const int MZ = MP / 2, NZ = NP / 2;
+  for (int p = 0; p < nb; p++) {
+    if (wa->b_typ[p] != 1) { wa->q_amb[p] = 0.0; continue; }
+    int nwall = 0;
+    if (iwall) nwall = iwall[((size_t)(wa->bc_k[p] / 2 - 1) * NZ + (size_t)(wa->bc_j[p] / 2 - 1)) * MZ + (size_t)(wa->bc_i[p] / 2 - 1)];
+    double w_k0, h_a0, w_d0, t_a0;
+    if (nwall > 0 && nwall <= wc->nwal0) {
+      w_k0 = wc->w1_k[nwall - 1]; h_a0 = wc->w1_ha[nwall - 1]; w_d0 = wc->w1_d[nwall - 1]; t_a0 = wc->w1_ta[nwall - 1];
+    } else { w_k0 = wc->w_k; h_a0 = wc->h_a; w_d0 = wc->w_d; t_a0 = wc->t_a; }
+    const double u_a0 = (w_k0 < 1.0e-20 || h_a0 < 1.0e-20) ? 0.0 : 1.0 / (w_d0 / w_k0 + 1.0 / h_a0);
+    wa->q_amb[p] = u_a0 * (wa->b_t[p] - t_a0);
+    loss = loss + wa->q_amb[p] * wa->patch_area[p];
+  }
+  wc->q_wall_loss = loss;
+  wc->qamb_avg = loss / wc->wall_area_total;
+}
+
+/* One pass's wall part: patch temperatures, wall_q_amb, wallq, QA_S0/QA_S1, the state carried to the gas. */
+int oracle_wallq(gfm_ctrl *c, const gfm_host_arrays *h, gfm_wall_ctrl *wc, gfm_wall_arrays *wa, const double *qa,
+                 double qeg_s, const int16_t *iwall) {
+  int r = bind(c, h);
+  if (r) return r;
+  if (!wc || !wa || !qa || wc->b_cnt <= 0 || !wa->vf || wc->preset_vf != 1) return GFM_E_ARG;
+  const int nb = wc->b_cnt;
+  const int MZ = MP / 2, NZ = NP / 2;
+#define QAC(i, j, k) qa[((size_t)((k) / 2 - 1) * NZ + (size_t)((j) / 2 - 1)) * MZ + (size_t)((i) / 2 - 1)]
+  const double t0 = C->t0;
+  /* patch temperatures at the start of the pass, C/Rad_emis.f90:190-212 and 253-261 (T in K) */
+  for (int p = 0; p < nb; p++) {
+    const int t = wa->b_typ[p];
+    if (wc->i1st_rad_done == 0) {
+      if (t == 1 || t == 2) wa->b_t[p] = T(wa->bc_i[p], wa->bc_j[p], wa->bc_k[p]) * t0;
+      else if (t == 3) {
+        const int e = wa->b_exh[p] - 1;
+        if (e < 0) return GFM_E_ARG;
+        wa->b_t[p] = (wc->exh_type[e] == 0) ? wc->exh_frac[e] * wc->avg_wall_T : wc->exh_fixed[e];
+      } else wa->b_t[p] = T(wa->b_i[p], wa->b_j[p], wa->b_k[p]) * t0;
+      wa->radiosity[p] = sig_w * pow(wa->b_t[p], 4);
+    } else if (t == 4) {
+      wa->b_t[p] = T(wa->b_i[p], wa->b_j[p], wa->b_k[p]) * t0;
+      wa->radiosity[p] = sig_w * pow(wa->b_t[p], 4);
+    }
+  }
+  wall_q_amb(wc, wa, nb, iwall);
+  /* wallq: qc, qeb and the incident sums, C/Qwall.f90:997-1026 */
+  wc->q_incident_total = wc->q_incident_wall = wc->q_incident_inlet = wc->q_incident_exit = wc->q_incident_melt = 0.0;
+  for (int p = 0; p < nb; p++) {
+    wa->qc[p] = QAC(wa->bc_i[p], wa->bc_j[p], wa->bc_k[p]) / wa->b_face_area[p];
+    const double inc = wa->qc[p] * wa->patch_area[p];
+    wc->q_incident_total += inc;
+    switch (wa->b_typ[p]) {
+      case 1: wc->q_incident_wall += inc; break;
+      case 2: wc->q_incident_inlet += inc; wa->qeb[p] = sig_w * pow(wa->b_t[p], 4); break;
+      case 3: wc->q_incident_exit += inc; wa->qeb[p] = sig_w * pow(wa->b_t[p], 4); break;
+      case 4: wc->q_incident_melt += inc; wa->qeb[p] = wa->emis[p] * sig_w * pow(wa->b_t[p], 4); break;
+    }
+  }
+  wc->qc_avg = wc->q_incident_total / wc->boundary_area_total;
+  const double radi_min = sig_w * pow(t_mnk_w, 4);
+  const int nlog = (wa->conv_log && wa->log_cap > 0) ? wa->log_cap : 0;
+  double prev_q_melt_chg = 1.0e6, qconv_total = 0.0;
+  int nn = 1, nnextra = wc->maxri1, sweeps = 0, outer = 0;
+  (void)prev_q_melt_chg;
+  while (nn <= wc->maxri1 && nn <= nnextra) {
+    int minradi = wc->minri2;
+    wall_q_conv(wc, wa, nb, &qconv_total);
+    int n = 1, nextra = wc->maxri2;
+    while (n <= wc->maxri2 && n <= nextra) {
+      double mean = 0.0;
+      wc->iterw++;
+      for (int p = 0; p < nb; p++) {
+        double rfb = 0.0;
+        if (wa->b_typ[p] == 1 || wa->emis[p] < 1.0) rfb = wall_rad_from(wa, nb, p);
+        const double old = wa->radiosity[p];
+        if (wa->b_typ[p] == 1) wa->radiosity[p] = dmax(wa->qc[p] + wa->q_conv[p] - wa->q_amb[p] + rfb, wa->emis[p] * radi_min);
+        else wa->radiosity[p] = wa->qeb[p] + (1.0 - wa->emis[p]) * (rfb + wa->qc[p]);
+        if (!(old < wc->wall_radiosity_tolerance)) mean = mean + fabs((old - wa->radiosity[p]) / old);
+      }
+      mean = mean / nb;
+      if (sweeps < nlog) wa->conv_log[sweeps] = mean;
+      sweeps++;
+      wc->rel_rad_chg_last = mean;
+      if (mean < wc->wall_radiosity_tolerance && n > minradi)
+        if (nextra == wc->maxri2) nextra = n + 3;
+      n++;
+    }
+    /* wall temperatures and the melt-surface flux, C/Qwall.f90:1155-1191 */
+    const double prev_q_melt = wc->q_melt;
+    double q_melt = 0.0;
+    for (int p = 0; p < nb; p++) {
+      if (wa->b_typ[p] == 1) {
+        const double e = wa->emis[p];
+        wa->b_t[p] = pow(dmax(0.0, (wa->radiosity[p] - (wa->q_amb[p] - wa->q_conv[p]) * (1.0 - e) / e) / sig_w), 0.25);
+        wa->b_t[p] = dmax(t_mnk_w, wa->b_t[p]);
+      } else if (wa->b_typ[p] == 4) {
+        const double rfb = wall_rad_from(wa, nb, p);
+        const double q = wa->qc[p] + wa->q_conv[p] + rfb - wa->radiosity[p];
+        if (wa->qrs) wa->qrs[(size_t)(wa->b_j[p] - 1) * MP + (size_t)(wa->b_i[p] - 1)] = q;
+        q_melt = q_melt + q * wa->patch_area[p];
+      }
+    }
+    wc->q_melt = q_melt;
+    wall_q_amb(wc, wa, nb, iwall);
+    wc->qrs_avg = q_melt / wc->melt_area_total;
+    const double q_melt_chg = fabs((prev_q_melt - q_melt) / (q_melt + small16));
+    if (q_melt_chg < wc->radiation_boundary_tolerance)
+      if (nnextra == wc->maxri1) nnextra = nn + 1;
+    prev_q_melt_chg = q_melt_chg;
+    outer++;
+    nn++;
+  }
+  wc->qconv_total = qconv_total;
+  /* closing sums, C/Qwall.f90:1209-1247 */
+  double txa = 0.0;
+  wc->q_rad_inlet = wc->q_rad_outlet = wc->qm_flame = wc->qm_bnds = wc->qm_conv = 0.0;
+  for (int p = 0; p < nb; p++) {
+    const int t = wa->b_typ[p];
+    if (t == 1) {
+      T(wa->b_i[p], wa->b_j[p], wa->b_k[p]) = wa->b_t[p] / t0;
+      if (wa->wall_rad_t) wa->wall_rad_t[IX3(wa->bc_i[p], wa->bc_j[p], wa->bc_k[p])] = wa->b_t[p];
+      txa = txa + wa->b_t[p] * wa->patch_area[p];
+    } else if (t == 2 || t == 3) {
+      const double rfb = wall_rad_from(wa, nb, p);
+      const double q = (wa->qc[p] + rfb - wa->qeb[p]) * wa->patch_area[p];
+      if (t == 2) wc->q_rad_inlet += q; else wc->q_rad_outlet += q;
+    } else if (t == 4) {
+      const double rfb = wall_rad_from(wa, nb, p);
+      const double e = wa->emis[p];
+      const double f_melt = 1.0 - wa->qeb[p] / (e * (wa->qc[p] + rfb) + wa->q_conv[p]);
+      wc->qm_flame += f_melt * e * wa->qc[p] * wa->patch_area[p];
+      wc->qm_bnds += f_melt * e * rfb * wa->patch_area[p];
+      wc->qm_conv += f_melt * wa->q_conv[p] * wa->patch_area[p];
+    }
+  }
+  wc->avg_wall_T = txa / wc->wall_area_total;
+  int frac = 0;
+  for (int m = 0; m < wc->nex; m++) if (wc->exh_type[m] == 0) frac = 1;
+  if (frac)
+    for (int p = 0; p < nb; p++) {
+      if (wa->b_typ[p] != 3) continue;
+      const int e = wa->b_exh[p] - 1;
+      wa->b_t[p] = (wc->exh_type[e] == 0) ? wc->exh_frac[e] * wc->avg_wall_T : wc->exh_fixed[e];
+    }
+  wc->qconv_avg = qconv_total / wc->boundary_area_total;
+  /* soot calibration, C/Qwall.f90:1297-1320 */
+  if (wc->isoot_cal == 1) {
+    if (wc->q_melt < 0.0) wc->q_melt = 0.0;
+    const double q_ratio = wc->q_melt / wc->q_melt_req;
+    const double a_adj = dmax(dmin(q_ratio, 2.0), 0.5);
+    if (a_adj > 1.0) wc->aform = wc->rf_afover * wc->aform / a_adj + (1.0 - wc->rf_afover) * wc->aform;
+    else wc->aform = wc->rf_aform * wc->aform / a_adj + (1.0 - wc->rf_aform) * wc->aform;
+  }
+  /* QRN_PRN, C/Qrn.f90:399-402 (QEW_S and QES0 are zero) */
+  const double g0 = qeg_s * 1.0e-6;
+  wc->qa_s1 = g0 - wc->qa_s0;
+  wc->qa_s0 = g0;
+  wc->sweeps = sweeps;
+  wc->outer = outer;
+  /* the state of the next gas iterations: each wall cell takes its last wall patch's temperature, with its
+   * enthalpy (rad_emis_rr, C/Rad_emis.f90:101-113) */
+  for (int p = 0; p < nb; p++) {
+    if (wa->b_typ[p] != 1) continue;
+    T(wa->bc_i[p], wa->bc_j[p], wa->bc_k[p]) = wa->b_t[p] / t0;
+  }
+  for (int p = 0; p < nb; p++) {
+    if (wa->b_typ[p] != 1) continue;
+    T_TO_ENTH(wa->bc_i[p], wa->bc_j[p], wa->bc_k[p]);
+  }
+#undef QAC
+  return GFM_OK;
+}
+size_t oracle_wall_ctrl_size(void) { return sizeof(gfm_wall_ctrl); }
+/* INTP on the bound state (test support: face values of a state read from restart records) */
+int oracle_intp(gfm_ctrl *c, const gfm_host_arrays *h) {
+  int r = bind(c, h);
+  if (r) return r;
+  INTP();
+  return GFM_OK;
+}

@@ -277,3 +277,55 @@ def rad_stats():
     out = (ct.c_longlong * 3)()
     lib().oracle_rad_stats(out)
     return tuple(out)
+
+
+# ---- wall radiosity exchange (C/Qwall.f90) ----
+def wall_setup(case, wall, iwall=None):
+    """prepare_for_wall_radiation_calculation on the case's cell codes:
+    counts the patches, allocates wall's arrays and fills them."""
+    from gfm_amd.abi import host_arrays, wall_arrays
+    h = host_arrays(case.arrays)
+    iw = None if iwall is None else iwall.ctypes.data_as(ct.POINTER(ct.c_int16))
+    empty = wall_arrays({})
+    rc = lib().oracle_wall_setup(ct.byref(case.ctrl), ct.byref(h), ct.byref(wall.ctrl), ct.byref(empty), iw)
+    if rc:
+        raise RuntimeError(f"oracle_wall_setup (count) failed: {rc}")
+    wall.allocate(wall.ctrl.b_cnt)
+    wa = wall_arrays(wall.arrays)
+    rc = lib().oracle_wall_setup(ct.byref(case.ctrl), ct.byref(h), ct.byref(wall.ctrl), ct.byref(wa), iw)
+    if rc:
+        raise RuntimeError(f"oracle_wall_setup failed: {rc}")
+
+
+def wallq(case, wall, rad, iwall=None):
+    """One radiation pass's wall part on the host state (rad: the
+    gfm_amd.radiation.Radiation of the pass's volume part)."""
+    from gfm_amd.abi import host_arrays, wall_arrays
+    h = host_arrays(case.arrays)
+    wa = wall_arrays(wall.arrays)
+    iw = None if iwall is None else iwall.ctypes.data_as(ct.POINTER(ct.c_int16))
+    qa = np.asfortranarray(rad.arrays["qa"])
+    rc = lib().oracle_wallq(ct.byref(case.ctrl), ct.byref(h), ct.byref(wall.ctrl), ct.byref(wa), _dp(qa),
+                            ct.c_double(rad.ctrl.qeg_s), iw)
+    if rc:
+        raise RuntimeError(f"oracle_wallq failed: {rc}")
+
+
+_WALL_FIELD = None
+
+
+def wall_field(qcf):
+    """The per-face q_conv (gfm_amd.wall.Wall.qconv_face) the enthalpy
+    source subtracts while i1st_rad_done is 1; None to clear."""
+    global _WALL_FIELD
+    _WALL_FIELD = None if qcf is None else np.asfortranarray(qcf, dtype=np.float64)
+    lib().oracle_wall_field(None if _WALL_FIELD is None else _dp(_WALL_FIELD))
+
+
+def intp(case):
+    """INTP on the host state (face values of a state read from records)."""
+    from gfm_amd.abi import host_arrays
+    h = host_arrays(case.arrays)
+    rc = lib().oracle_intp(ct.byref(case.ctrl), ct.byref(h))
+    if rc:
+        raise RuntimeError(f"oracle_intp failed: {rc}")

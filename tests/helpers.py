@@ -67,3 +67,32 @@ def manufactured(case, box, seed):
     xt = xs.copy(order="F")
     xt[sl] = 0.0
     return ap, np.asfortranarray(as_), bs, xt, xs, sl
+
+
+def wall_reference_state(cn, gold):
+    """A reference run's saved state (case0041 or case1206) prepared for a
+    radiation pass with its wall exchange: the records applied, the face
+    values of INTP formed (the melt patches take the temperature at their
+    face), the wall exchange's sbc inputs, the previous pass's mean wall
+    temperature (exits of exh_type 0 use it) and the soot calibration's
+    constants.  Returns (case, MinorSpecies, Radiation, GfmWallCtrl, iwall,
+    fixture)."""
+    import gzip
+    import json
+    import os
+
+    import oracle
+    from gfm_amd import radiation, sbcfile
+    from gfm_amd import wall as wl
+    fx = json.load(open(os.path.join(gold, "wall_fixtures.json")))["case" + cn]
+    rg = gzip.open(os.path.join(gold, f"rg{cn}c.d.gz")).read()
+    rs = gzip.open(os.path.join(gold, f"rs{cn}c.d.gz")).read()
+    sb = os.path.join(gold, "grid", f"sbc{cn}c.dat")
+    c, ms, rad = radiation.reference_state(os.path.join(gold, "grid", f"gd{cn}c.dat"), sb, rg, rs)
+    oracle.intp(c)
+    mp, np_, lp = c.shape
+    wc, iwall = wl.inputs_from_sbc(sbcfile.read_sbc(sb), mp, np_, lp, c.arrays["ibcell"])
+    wc.avg_wall_T = fx["avg_wall_T_prev"]
+    if "aform_prev" in fx:
+        wc.aform, wc.q_melt_req = fx["aform_prev"], fx["q_melt_req"]
+    return c, ms, rad, wc, iwall, fx

@@ -9,7 +9,8 @@ import subprocess
 import pytest
 
 from gfm_amd import solver
-from gfm_amd.abi import GfmCtrl, GfmHostArrays, GfmMsArrays, GfmMsCtrl, GfmRadArrays, GfmRadCtrl
+from gfm_amd.abi import (GfmCtrl, GfmHostArrays, GfmMsArrays, GfmMsCtrl, GfmRadArrays, GfmRadCtrl, GfmWallArrays,
+                         GfmWallCtrl)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -36,13 +37,15 @@ def test_library_exports_header_symbols():
 def test_ctrl_layout():
     lib = solver.lib()
     assert lib.gfm_hip_ctrl_size() == ct.sizeof(GfmCtrl)
-    assert lib.gfm_hip_abi_version() == 5
+    assert lib.gfm_hip_abi_version() == 6
     assert ct.sizeof(GfmHostArrays) == 28 * 8
     lib.gfm_hip_ms_ctrl_size.restype = ct.c_size_t
     assert lib.gfm_hip_ms_ctrl_size() == ct.sizeof(GfmMsCtrl)
     assert ct.sizeof(GfmMsArrays) == 5 * 8
     assert lib.gfm_hip_rad_ctrl_size() == ct.sizeof(GfmRadCtrl)
     assert ct.sizeof(GfmRadArrays) == 4 * 8
+    assert lib.gfm_hip_wall_ctrl_size() == ct.sizeof(GfmWallCtrl)
+    assert ct.sizeof(GfmWallArrays) == 24 * 8 + 8
 
 
 def test_oracle_layout():
@@ -51,6 +54,8 @@ def test_oracle_layout():
     assert oracle.lib().oracle_ms_ctrl_size() == ct.sizeof(GfmMsCtrl)
     oracle.lib().oracle_rad_ctrl_size.restype = ct.c_size_t
     assert oracle.lib().oracle_rad_ctrl_size() == ct.sizeof(GfmRadCtrl)
+    oracle.lib().oracle_wall_ctrl_size.restype = ct.c_size_t
+    assert oracle.lib().oracle_wall_ctrl_size() == ct.sizeof(GfmWallCtrl)
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
