@@ -207,18 +207,29 @@ __global__ void __launch_bounds__(256) k_wall_part(WallDev w, int r0) {
 // the block's rows in patch order (one wavefront, row r0 + lane)
 __global__ void __launch_bounds__(64) k_wall_solve(WallDev w, int r0, double radi_min) {
   __shared__ double vb[WB][WB + 1];
+  __shared__ double jb[WB];
   const int l = threadIdx.x, row = r0 + l, nr = min(WB, w.nb - r0);
   for (int q = l; q < nr * nr; q += 64) {
     const int a = q / nr, b = q % nr;
     vb[a][b] = w.vf[(size_t)(r0 + a) * w.nb + r0 + b];
   }
-  __syncthreads();
-  double acc = 0.0, old = 0.0;
+  // the block's radiosities of the last sweep, and the chunk sums loaded eight at a time
+  const double old = (l < nr) ? w.jrad[row] : 0.0;
+  jb[l] = old;
+  double acc = 0.0;
   if (l < nr) {
-    for (int c = 0; c < w.nchunk; c++) acc += w.part[(size_t)c * WB + l];
-    old = w.jrad[row];
-    for (int b = l + 1; b < nr; b++) acc += vb[l][b] * w.jrad[r0 + b];
+    for (int c0 = 0; c0 < w.nchunk; c0 += 8) {
+      double pv[8];
+#pragma unroll
+      for (int c = 0; c < 8; c++) pv[c] = (c0 + c < w.nchunk) ? w.part[(size_t)(c0 + c) * WB + l] : 0.0;
+#pragma unroll
+      for (int c = 0; c < 8; c++)
+        if (c0 + c < w.nchunk) acc += pv[c];
+    }
   }
+  __syncthreads();
+  if (l < nr)
+    for (int b = l + 1; b < nr; b++) acc += vb[l][b] * jb[b];
   double jn = old, rel = 0.0;
   for (int i = 0; i < nr; i++) {
     double ji = 0.0;

@@ -113,6 +113,51 @@ module gfm_hip_mod
       type(c_ptr), value :: ctx
       type(gfm_ms_ctrl), intent(inout) :: ms
     end function
+    ! radiation transport (RAD_EMIS's volume part, rad_qe_only) and the wall exchange (QRNF's wall part)
+    integer(c_int) function gfm_hip_rad_emis(ctx, rc) bind(C, name="gfm_hip_rad_emis")
+      import :: c_int, c_ptr, gfm_rad_ctrl
+      type(c_ptr), value :: ctx
+      type(gfm_rad_ctrl), intent(inout) :: rc
+    end function
+    integer(c_int) function gfm_hip_rad_qe_only(ctx, rc) bind(C, name="gfm_hip_rad_qe_only")
+      import :: c_int, c_ptr, gfm_rad_ctrl
+      type(c_ptr), value :: ctx
+      type(gfm_rad_ctrl), intent(inout) :: rc
+    end function
+    integer(c_int) function gfm_hip_rad_download(ctx, rc, a) bind(C, name="gfm_hip_rad_download")
+      import :: c_int, c_ptr, gfm_rad_ctrl, gfm_rad_arrays
+      type(c_ptr), value :: ctx
+      type(gfm_rad_ctrl), intent(in) :: rc
+      type(gfm_rad_arrays), intent(in) :: a
+    end function
+    integer(c_int) function gfm_hip_rad_upload(ctx, a) bind(C, name="gfm_hip_rad_upload")
+      import :: c_int, c_ptr, gfm_rad_arrays
+      type(c_ptr), value :: ctx
+      type(gfm_rad_arrays), intent(in) :: a
+    end function
+    integer(c_int) function gfm_hip_wall_setup(ctx, wc, iwall) bind(C, name="gfm_hip_wall_setup")
+      import :: c_int, c_ptr, gfm_wall_ctrl
+      type(c_ptr), value :: ctx
+      type(gfm_wall_ctrl), intent(inout) :: wc
+      type(c_ptr), value :: iwall
+    end function
+    integer(c_int) function gfm_hip_wallq(ctx, wc) bind(C, name="gfm_hip_wallq")
+      import :: c_int, c_ptr, gfm_wall_ctrl
+      type(c_ptr), value :: ctx
+      type(gfm_wall_ctrl), intent(inout) :: wc
+    end function
+    integer(c_int) function gfm_hip_wall_download(ctx, wc, a) bind(C, name="gfm_hip_wall_download")
+      import :: c_int, c_ptr, gfm_wall_ctrl, gfm_wall_arrays
+      type(c_ptr), value :: ctx
+      type(gfm_wall_ctrl), intent(in) :: wc
+      type(gfm_wall_arrays), intent(in) :: a
+    end function
+    integer(c_int) function gfm_hip_wall_upload(ctx, wc, a) bind(C, name="gfm_hip_wall_upload")
+      import :: c_int, c_ptr, gfm_wall_ctrl, gfm_wall_arrays
+      type(c_ptr), value :: ctx
+      type(gfm_wall_ctrl), intent(in) :: wc
+      type(gfm_wall_arrays), intent(in) :: a
+    end function
     ! include/gfm_pnet.h: the network of the learned p' guess
     integer(c_int) function gfm_pnet_create(device, ch, max_batch, max_cells, seed, net) bind(C, name="gfm_pnet_create")
       import :: c_int, c_ptr, c_long_long, c_int64_t
