@@ -8,22 +8,23 @@
 // Gauss-Seidel row read one contiguous row.
 //
 // One Gauss-Seidel sweep updates the patches in patch order.  The patches
-// are taken in blocks of WB = 64: k_wall_part sums, for the block's rows,
-// VF(i,j)*J(j) over every column outside the block (the columns before it
-// hold this sweep's values, those after it the last sweep's) in column
-// chunks spread over the chip; k_wall_solve then, in one wavefront (one
-// row per lane), adds the chunk sums in a fixed order and the block's own
-// columns in patch order -- a row's block-internal sum starts from the
-// last sweep's values of the rows after it and takes each earlier row's new
-// value as that row is solved -- and appends each row's relative change to
-// the sweep's running sum in patch order.  The sums over the other patches
-// are therefore added in another order than the reference's, which changes
+// are taken in blocks of WB = 64, one launch of k_wall_block each: its
+// workgroups sum, for the block's rows, VF(i,j)*J(j) over every column
+// outside the block (the columns before it hold this sweep's values, those
+// after it the last sweep's) in column chunks spread over the chip; the
+// last workgroup to finish then, in one wavefront (one row per lane), adds
+// the chunk sums in a fixed order and the block's own columns in patch
+// order -- a row's block-internal sum starts from the last sweep's values
+// of the rows after it and takes each earlier row's new value as that row
+// is solved -- and appends each row's relative change to the sweep's
+// running sum in patch order.  The sums over the other patches are
+// therefore added in another order than the reference's, which changes
 // the rounding only; the update order is the reference's.
 #pragma once
 
 #define WB 64          // rows per Gauss-Seidel block (one wavefront)
-#define WCH 256        // columns per chunk of k_wall_part
-#define WROWS 16       // rows per workgroup of k_wall_part
+#define WCH 256        // columns per chunk of k_wall_block
+#define WROWS 16       // rows per workgroup of k_wall_block
 #define WALL_NT 1024   // threads of the one-workgroup kernels
 static constexpr double W_SIG = 5.668e-8, W_TMNK = 2.0e2, W_PI = 3.141592653589793;
 
@@ -38,7 +39,9 @@ struct WallDev {
   long long *cbc;      // cell-centre index of the boundary cell (radiation fields)
   double *area, *face_area, *emis, *jrad, *q_amb, *q_conv, *qc, *qeb, *bt, *ua, *ta, *ds, *rfb;
   double *vf, *vf_sum;
+  double *vd;          // the diagonal blocks of VF, block b at vd[b*WB*WB]: row a, column c at [a*WB + c]
   double *part;        // (nchunk, WB) partial sums of the current block
+  unsigned *cnt;       // workgroups of k_wall_block done (the last one solves the block and resets it)
   double *qrs;         // (MP, NP), reference layout
   double *wrt;         // wall_rad_T, reference layout (MP, NP, LP)
   double *qcf;         // q_conv at patch faces, device lattice
@@ -97,30 +100,38 @@ __global__ void k_wall_vf(WallDev w) {
 
 // ---- helpers of the one-workgroup kernels ----
 // sequential sum in patch order of f(p) over the patches with sel(p), by thread 0 (the reference's order)
+// (the terms are formed by all threads into the kernel's LDS buffer sh, 4096 at a time, and added there by thread 0)
+#define WSEQ 4096
+struct WallShared { double terms[WSEQ]; double sum; };
 template <typename F>
-__device__ double wall_seq_sum(const WallDev &w, F f) {
-  __shared__ double s_sum;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int p = 0; p < w.nb; p++) s = s + f(p);
-    s_sum = s;
+__device__ double wall_seq_sum(const WallDev &w, WallShared &sh, F f) {
+  double s = 0.0;
+  for (int p0 = 0; p0 < w.nb; p0 += WSEQ) {
+    const int n = min(WSEQ, w.nb - p0);
+    __syncthreads();
+    for (int q = threadIdx.x; q < n; q += blockDim.x) sh.terms[q] = f(p0 + q);
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int q = 0; q < n; q++) s = s + sh.terms[q];
   }
   __syncthreads();
-  return s_sum;
+  if (threadIdx.x == 0) sh.sum = s;
+  __syncthreads();
+  return sh.sum;
 }
 
 // wall_q_amb, C/Qwall.f90:1585-1659: q_amb and the wall loss
-__device__ void wall_q_amb_dev(const WallDev &w) {
+__device__ void wall_q_amb_dev(const WallDev &w, WallShared &sh) {
   for (int p = threadIdx.x; p < w.nb; p += blockDim.x)
     w.q_amb[p] = (w.typ[p] == 1) ? w.ua[p] * (w.bt[p] - w.ta[p]) : 0.0;
-  const double loss = wall_seq_sum(w, [&](int p) { return (w.typ[p] == 1) ? w.q_amb[p] * w.area[p] : 0.0; });
+  const double loss = wall_seq_sum(w, sh, [&](int p) { return (w.typ[p] == 1) ? w.q_amb[p] * w.area[p] : 0.0; });
   if (threadIdx.x == 0) w.res[WR_QLOSS] = loss;
 }
 
 // the pass's start: patch temperatures (C/Rad_emis.f90:190-212, 253-261), wall_q_amb, qc, qeb and the
 // incident sums (C/Qwall.f90:997-1026)
 __global__ void __launch_bounds__(WALL_NT) k_wall_begin(WallDev w, const double *qa) {
+  __shared__ WallShared sh;
   for (int p = threadIdx.x; p < w.nb; p += blockDim.x) {
     const int t = w.typ[p];
     if (w.first) {
@@ -136,7 +147,7 @@ __global__ void __launch_bounds__(WALL_NT) k_wall_begin(WallDev w, const double 
     }
   }
   __syncthreads();
-  wall_q_amb_dev(w);
+  wall_q_amb_dev(w, sh);
   for (int p = threadIdx.x; p < w.nb; p += blockDim.x) {
     w.qc[p] = qa[w.cbc[p]] / w.face_area[p];
     const int t = w.typ[p];
@@ -144,10 +155,10 @@ __global__ void __launch_bounds__(WALL_NT) k_wall_begin(WallDev w, const double 
     else if (t == 4) w.qeb[p] = w.emis[p] * W_SIG * pow(w.bt[p], 4.0);
   }
   __syncthreads();
-  const double tot = wall_seq_sum(w, [&](int p) { return w.qc[p] * w.area[p]; });
+  const double tot = wall_seq_sum(w, sh, [&](int p) { return w.qc[p] * w.area[p]; });
   double s[4];
   for (int t = 1; t <= 4; t++)
-    s[t - 1] = wall_seq_sum(w, [&](int p) { return (w.typ[p] == t) ? w.qc[p] * w.area[p] : 0.0; });
+    s[t - 1] = wall_seq_sum(w, sh, [&](int p) { return (w.typ[p] == t) ? w.qc[p] * w.area[p] : 0.0; });
   if (threadIdx.x == 0) {
     w.res[WR_QINC] = tot;
     w.res[WR_QINC_W] = s[0]; w.res[WR_QINC_I] = s[1]; w.res[WR_QINC_E] = s[2]; w.res[WR_QINC_M] = s[3];
@@ -156,6 +167,7 @@ __global__ void __launch_bounds__(WALL_NT) k_wall_begin(WallDev w, const double 
 
 // calc_q_conv_wall, C/Qwall.f90:1370-1478
 __global__ void __launch_bounds__(WALL_NT) k_wall_qconv(WallDev w) {
+  __shared__ WallShared sh;
   const double kappa_cmu = 0.4 * pow(9.0e-2, 0.25), e_cmu = 9.0 * pow(9.0e-2, 0.25);
   const double c_htw = 0.4 * ((W_PI / 4) / sin(W_PI / 4)) * sqrt(26.0) * (0.7 - 1.0) / pow(0.7, 0.25);
   for (int p = threadIdx.x; p < w.nb; p += blockDim.x) {
@@ -174,48 +186,30 @@ __global__ void __launch_bounds__(WALL_NT) k_wall_qconv(WallDev w) {
     }
     w.q_conv[p] = fmax(w.h_g, h_t) * (tgas - w.bt[p]);
   }
-  const double tot = wall_seq_sum(w, [&](int p) { return w.q_conv[p] * w.area[p]; });
+  const double tot = wall_seq_sum(w, sh, [&](int p) { return w.q_conv[p] * w.area[p]; });
   if (threadIdx.x == 0) { w.res[WR_QCONV] = tot; w.res[WR_MEAN] = 0.0; }
 }
 
 // does row p take the radiation of the other patches (C/Qwall.f90:1067-1069)
 __device__ __forceinline__ bool wall_sees(const WallDev &w, int p) { return w.typ[p] == 1 || w.emis[p] < 1.0; }
 
-// partial sums of one block's rows over one column chunk outside the block
-__global__ void __launch_bounds__(256) k_wall_part(WallDev w, int r0) {
-  __shared__ double red[WROWS][256 / 64];
-  const int c0 = blockIdx.x * WCH, rg = blockIdx.y * WROWS;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int r1 = min(r0 + WB, w.nb);
-  for (int rr = 0; rr < WROWS; rr++) {
-    const int row = r0 + rg + rr;
-    double s = 0.0;
-    if (row < r1) {
-      const double *vr = w.vf + (size_t)row * w.nb;
-      for (int m = c0 + threadIdx.x; m < min(c0 + WCH, w.nb); m += 256)
-        if (m < r0 || m >= r1) s += vr[m] * w.jrad[m];
-    }
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (lane == 0) red[rr][wv] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x < WROWS && rg + threadIdx.x < WB)
-    w.part[(size_t)blockIdx.x * WB + rg + threadIdx.x] =
-        ((red[threadIdx.x][0] + red[threadIdx.x][1]) + red[threadIdx.x][2]) + red[threadIdx.x][3];
+// lane i's value of a double, for a uniform i (two v_readlane)
+__device__ __forceinline__ double lane_value(double v, int i) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)b, i), hi = __builtin_amdgcn_readlane((int)(b >> 32), i);
+  return __builtin_bit_cast(double, (long long)(unsigned)lo | ((long long)hi << 32));
 }
-
-// the block's rows in patch order (one wavefront, row r0 + lane)
-__global__ void __launch_bounds__(64) k_wall_solve(WallDev w, int r0, double radi_min) {
-  __shared__ double vb[WB][WB + 1];
-  __shared__ double jb[WB];
+// The block's rows in patch order by the first wavefront of the workgroup (row r0 + lane): the chunk sums in
+// order, the block's own columns from vd in registers, forward substitution with lane i's value broadcast by
+// v_readlane.
+__device__ void wall_block_solve(const WallDev &w, int r0, double radi_min, double (*vb)[WB + 1]) {
   const int l = threadIdx.x, row = r0 + l, nr = min(WB, w.nb - r0);
-  for (int q = l; q < nr * nr; q += 64) {
-    const int a = q / nr, b = q % nr;
-    vb[a][b] = w.vf[(size_t)(r0 + a) * w.nb + r0 + b];
+  int ty = 0;
+  double old = 0.0, qcv = 0.0, qcn = 0.0, qam = 0.0, qeb = 0.0, em = 1.0;
+  if (l < nr) {
+    ty = w.typ[row]; old = w.jrad[row]; qcv = w.qc[row]; qcn = w.q_conv[row]; qam = w.q_amb[row];
+    qeb = w.qeb[row]; em = w.emis[row];
   }
-  // the block's radiosities of the last sweep, and the chunk sums loaded eight at a time
-  const double old = (l < nr) ? w.jrad[row] : 0.0;
-  jb[l] = old;
   double acc = 0.0;
   if (l < nr) {
     for (int c0 = 0; c0 < w.nchunk; c0 += 8) {
@@ -227,27 +221,86 @@ __global__ void __launch_bounds__(64) k_wall_solve(WallDev w, int r0, double rad
         if (c0 + c < w.nchunk) acc += pv[c];
     }
   }
-  __syncthreads();
-  if (l < nr)
-    for (int b = l + 1; b < nr; b++) acc += vb[l][b] * jb[b];
-  double jn = old, rel = 0.0;
-  for (int i = 0; i < nr; i++) {
-    double ji = 0.0;
-    if (l == i) {
-      const double rfb = wall_sees(w, row) ? acc : 0.0;
-      if (w.typ[row] == 1) ji = fmax(w.qc[row] + w.q_conv[row] - w.q_amb[row] + rfb, w.emis[row] * radi_min);
-      else ji = w.qeb[row] + (1.0 - w.emis[row]) * (rfb + w.qc[row]);
-      jn = ji;
-      rel = (old < w.tol_rad) ? 0.0 : fabs((old - ji) / old);
+  double rv[WB];
+#pragma unroll
+  for (int b = 0; b < WB; b++) rv[b] = vb[l][b];
+#pragma unroll
+  for (int b = 0; b < WB; b++) {
+    const double jo = lane_value(old, b);
+    if (b > l && b < nr) acc += rv[b] * jo;
+  }
+  const bool sees = ty == 1 || em < 1.0;
+  double jn = old;
+#pragma unroll
+  for (int i = 0; i < WB; i++) {
+    if (i < nr) {
+      const double rfb = sees ? acc : 0.0;
+      const double cand = (ty == 1) ? fmax(qcv + qcn - qam + rfb, em * radi_min) : qeb + (1.0 - em) * (rfb + qcv);
+      const double ji = lane_value(cand, i);
+      if (l == i) jn = ji;
+      if (l > i) acc += rv[i] * ji;
     }
-    ji = __shfl(ji, i);
-    if (l > i && l < nr) acc += vb[l][i] * ji;
   }
   if (l < nr) w.jrad[row] = jn;
   // the sweep's running sum of relative changes, in patch order
+  const double rel = (l < nr && !(old < w.tol_rad)) ? fabs((old - jn) / old) : 0.0;
   double m = w.res[WR_MEAN];
-  for (int i = 0; i < nr; i++) m = m + __shfl(rel, i);
+  for (int i = 0; i < nr; i++) m = m + lane_value(rel, i);
   if (l == 0) w.res[WR_MEAN] = m;
+}
+
+// One block of a sweep: every workgroup sums its rows over its column chunk; the last
+// workgroup to finish (counted on w.cnt) solves the block.  No workgroup waits for another.
+__global__ void __launch_bounds__(256) k_wall_block(WallDev w, int r0, double radi_min) {
+  __shared__ double red[WROWS][256 / 64];
+  __shared__ double vb[WB][WB + 1];
+  __shared__ int last;
+  const int c0 = blockIdx.x * WCH, rg = blockIdx.y * WROWS;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int r1 = min(r0 + WB, w.nb);
+  const int m = c0 + threadIdx.x;
+  const bool col = m < w.nb && (m < r0 || m >= r1);
+  const double jm = col ? w.jrad[m] : 0.0;
+  double v[WROWS];
+#pragma unroll
+  for (int rr = 0; rr < WROWS; rr++) {
+    const int row = r0 + rg + rr;
+    v[rr] = (col && row < r1) ? w.vf[(size_t)row * w.nb + m] : 0.0;
+  }
+#pragma unroll
+  for (int rr = 0; rr < WROWS; rr++) {
+    double s = v[rr] * jm;
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) red[rr][wv] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < WROWS && rg + threadIdx.x < WB)
+    w.part[(size_t)blockIdx.x * WB + rg + threadIdx.x] =
+        ((red[threadIdx.x][0] + red[threadIdx.x][1]) + red[threadIdx.x][2]) + red[threadIdx.x][3];
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned total = gridDim.x * gridDim.y;
+    last = (atomicAdd(w.cnt, 1u) == total - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  // the block's own columns, contiguous in vd
+  const double *blk = w.vd + (size_t)(r0 / WB) * WB * WB;
+  for (int q = threadIdx.x; q < WB * WB; q += blockDim.x) vb[q / WB][q % WB] = blk[q];
+  __syncthreads();
+  if (threadIdx.x < 64) wall_block_solve(w, r0, radi_min, vb);
+  if (threadIdx.x == 0) atomicExch(w.cnt, 0u);
+}
+
+// the diagonal blocks of VF, contiguous (zero outside the patch range)
+__global__ void k_wall_vd(WallDev w) {
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= (long long)w.nblk * WB * WB) return;
+  const int b = (int)(t / (WB * WB)), a = (int)((t / WB) % WB), c = (int)(t % WB);
+  const int r = b * WB + a, q = b * WB + c;
+  w.vd[t] = (r < w.nb && q < w.nb) ? w.vf[(size_t)r * w.nb + q] : 0.0;
 }
 
 // rad_from_bnds of the rows of the given types (one wavefront per row)
@@ -265,6 +318,7 @@ __global__ void __launch_bounds__(256) k_wall_rows(WallDev w, int types) {
 // after the sweeps of one outer iteration: wall temperatures, the melt flux, q_melt, wall_q_amb
 // (C/Qwall.f90:1155-1191); rfb holds the melt rows' sums
 __global__ void __launch_bounds__(WALL_NT) k_wall_outer(WallDev w) {
+  __shared__ WallShared sh;
   for (int p = threadIdx.x; p < w.nb; p += blockDim.x) {
     if (w.typ[p] == 1) {
       const double e = w.emis[p];
@@ -276,27 +330,28 @@ __global__ void __launch_bounds__(WALL_NT) k_wall_outer(WallDev w) {
     }
   }
   __syncthreads();
-  const double qm = wall_seq_sum(w, [&](int p) {
+  const double qm = wall_seq_sum(w, sh, [&](int p) {
     return (w.typ[p] == 4) ? w.qrs[(size_t)(w.bj[p] - 1) * w.mp + (w.bi[p] - 1)] * w.area[p] : 0.0;
   });
   if (threadIdx.x == 0) { w.res[WR_QMELT_PREV] = w.res[WR_QMELT]; w.res[WR_QMELT] = qm; }
   __syncthreads();
-  wall_q_amb_dev(w);
+  wall_q_amb_dev(w, sh);
 }
 
 // the closing loop (C/Qwall.f90:1209-1247), the exit reset, and the state of the next gas iterations: T at wall
 // patch faces and wall cells with their enthalpy (C/Rad_emis.f90:337-355, 101-113), q_conv at the faces
 __global__ void __launch_bounds__(WALL_NT) k_wall_close(WallDev w) {
-  const double txa = wall_seq_sum(w, [&](int p) { return (w.typ[p] == 1) ? w.bt[p] * w.area[p] : 0.0; });
-  const double qin = wall_seq_sum(w, [&](int p) {
+  __shared__ WallShared sh;
+  const double txa = wall_seq_sum(w, sh, [&](int p) { return (w.typ[p] == 1) ? w.bt[p] * w.area[p] : 0.0; });
+  const double qin = wall_seq_sum(w, sh, [&](int p) {
     return (w.typ[p] == 2) ? (w.qc[p] + w.rfb[p] - w.qeb[p]) * w.area[p] : 0.0; });
-  const double qout = wall_seq_sum(w, [&](int p) {
+  const double qout = wall_seq_sum(w, sh, [&](int p) {
     return (w.typ[p] == 3) ? (w.qc[p] + w.rfb[p] - w.qeb[p]) * w.area[p] : 0.0; });
   auto fm = [&](int p) { const double e = w.emis[p];
                          return 1.0 - w.qeb[p] / (e * (w.qc[p] + w.rfb[p]) + w.q_conv[p]); };
-  const double qmf = wall_seq_sum(w, [&](int p) { return (w.typ[p] == 4) ? fm(p) * w.emis[p] * w.qc[p] * w.area[p] : 0.0; });
-  const double qmb = wall_seq_sum(w, [&](int p) { return (w.typ[p] == 4) ? fm(p) * w.emis[p] * w.rfb[p] * w.area[p] : 0.0; });
-  const double qmc = wall_seq_sum(w, [&](int p) { return (w.typ[p] == 4) ? fm(p) * w.q_conv[p] * w.area[p] : 0.0; });
+  const double qmf = wall_seq_sum(w, sh, [&](int p) { return (w.typ[p] == 4) ? fm(p) * w.emis[p] * w.qc[p] * w.area[p] : 0.0; });
+  const double qmb = wall_seq_sum(w, sh, [&](int p) { return (w.typ[p] == 4) ? fm(p) * w.emis[p] * w.rfb[p] * w.area[p] : 0.0; });
+  const double qmc = wall_seq_sum(w, sh, [&](int p) { return (w.typ[p] == 4) ? fm(p) * w.q_conv[p] * w.area[p] : 0.0; });
   if (threadIdx.x == 0) {
     w.res[WR_QRIN] = qin; w.res[WR_QROUT] = qout; w.res[WR_QMF] = qmf; w.res[WR_QMB] = qmb; w.res[WR_QMC] = qmc;
   }
@@ -567,6 +622,7 @@ int gfm_hip_wall_setup(gfm_ctx *c, gfm_wall_ctrl *wc, const int16_t *iwall) {
         (rc = dalloc(c, &d.qeb, nb)) || (rc = dalloc(c, &d.bt, nb)) || (rc = dalloc(c, &d.ua, nb)) ||
         (rc = dalloc(c, &d.ta, nb)) || (rc = dalloc(c, &d.ds, nb)) || (rc = dalloc(c, &d.rfb, nb)) ||
         (rc = dalloc(c, &d.vf_sum, nb)) || (rc = dalloc(c, &d.vf, (size_t)nb * nb)) ||
+        (rc = dalloc(c, &d.vd, (size_t)((nb + WB - 1) / WB) * WB * WB)) ||
         (rc = dalloc(c, &d.part, (size_t)((nb + WCH - 1) / WCH) * WB)))
       return rc;
     h->cap_nb = nb;
@@ -574,7 +630,7 @@ int gfm_hip_wall_setup(gfm_ctx *c, gfm_wall_ctrl *wc, const int16_t *iwall) {
   if (!d.res) {
     if ((rc = dalloc(c, &d.res, WR_N)) || (rc = dalloc(c, &d.qrs, (size_t)k.mp * k.np)) ||
         (rc = dalloc(c, &d.wrt, (size_t)k.mp * k.np * k.lp)) || (rc = dalloc(c, &d.qcf, (size_t)c->ncls)) ||
-        (rc = dalloc(c, &d.log, 1)))
+        (rc = dalloc(c, &d.log, 1)) || (rc = dalloc(c, &d.cnt, 1)))
       return rc;
     d.log_cap = 0;
   }
@@ -615,6 +671,8 @@ int gfm_hip_wall_setup(gfm_ctx *c, gfm_wall_ctrl *wc, const int16_t *iwall) {
   HCHECK(hipMemset(d.qcf, 0, (size_t)c->ncls * sizeof(double)));
   hipLaunchKernelGGL(k_wall_vf_sum, dim3((nb + 255) / 256), dim3(256), 0, c->stream, d);
   hipLaunchKernelGGL(k_wall_vf, dim3((nb + 255) / 256, nb), dim3(256), 0, c->stream, d);
+  hipLaunchKernelGGL(k_wall_vd, dim3((unsigned)(((long long)d.nblk * WB * WB + 255) / 256)), dim3(256), 0, c->stream, d);
+  HCHECK(hipMemsetAsync(d.cnt, 0, sizeof(unsigned), c->stream));
   HCHECK(hipGetLastError());
   HCHECK(hipStreamSynchronize(c->stream));
   h->typ = L.typ;
@@ -654,10 +712,8 @@ int gfm_hip_wallq(gfm_ctx *c, gfm_wall_ctrl *wc) {
     hipLaunchKernelGGL(k_wall_qconv, dim3(1), dim3(WALL_NT), 0, c->stream, d);
     int n = 1, nextra = wc->maxri2;
     while (n <= wc->maxri2 && n <= nextra) {
-      for (int b = 0; b < d.nblk; b++) {
-        hipLaunchKernelGGL(k_wall_part, dim3(d.nchunk, WB / WROWS), dim3(256), 0, c->stream, d, b * WB);
-        hipLaunchKernelGGL(k_wall_solve, dim3(1), dim3(64), 0, c->stream, d, b * WB, radi_min);
-      }
+      for (int b = 0; b < d.nblk; b++)
+        hipLaunchKernelGGL(k_wall_block, dim3(d.nchunk, WB / WROWS), dim3(256), 0, c->stream, d, b * WB, radi_min);
       HCHECK(hipGetLastError());
       double m;
       HCHECK(hipMemcpyAsync(&m, d.res + WR_MEAN, sizeof(double), hipMemcpyDeviceToHost, c->stream));
