@@ -1340,6 +1340,13 @@ __device__ __forceinline__ void conc_r(double y[9]) {  // CONC on gf values in r
   }
 }
 // The direction DIR of the two source points is a template parameter.
+// Stored are only the fields a later kernel reads at these positions: the
+// density, theta and Gamma everywhere (assembly fluxes and diffusion), and
+// P, T and the mass fractions on the faces and the yz-edges (sources of the
+// later INTP stages; faces are read elsewhere too) but not on the xy- and
+// xz-edges and corners of the x-stage, where no kernel reads them.  mu_t,
+// c_p and lambda are read at cell centres only, so INTP forms none of them
+// (the reference interpolates all of them everywhere).
 template <int DIR>
 __global__ void k_intp(Box b) {
   constexpr int dir = DIR;
@@ -1359,8 +1366,7 @@ __global__ void k_intp(Box b) {
   double y1[7], y2[7];
 #pragma unroll
   for (int l = 0; l < 7; l++) { y1[l] = g.gf[l][q1]; y2[l] = g.gf[l][q2]; }
-  const double mu1 = g.tmu[q1], mu2 = g.tmu[q2], cp1 = g.gcp[q1], cp2 = g.gcp[q2];
-  const double la1 = g.glam[q1], la2 = g.glam[q2], ga1 = g.gama[q1], ga2 = g.gama[q2];
+  const double ga1 = g.gama[q1], ga2 = g.gama[q2];
   const double th1 = g.theta[q1], th2 = g.theta[q2];
   double y[9];
   y[7] = 0.0;
@@ -1370,20 +1376,18 @@ __global__ void k_intp(Box b) {
 #pragma unroll
   for (int l = 0; l < 7; l++) y[l] = a * y1[l] + bb * y2[l];
   conc_r(y);
-  const double mu = a * mu1 + bb * mu2, cp = a * cp1 + bb * cp2, la = a * la1 + bb * la2;
   const double ga = a * ga1 + bb * ga2, th = a * th1 + bb * th2;
   // DNST_CALC, C/Prop.f90:187-198
   conc_r(y);
   const double wt = 1.0 / (y[0] / WF + y[5] / WCO2 + y[8] / WH2O + y[1] / WO2 + y[6] / WN2);
   const double temp = (1.0 + p) / t * wt / g.wt0;
-  g.p[q] = p;
-  g.t[q] = t;
+  if (DIR != 1 || ((j & 1) == 0 && (k & 1) == 0)) {
+    g.p[q] = p;
+    g.t[q] = t;
 #pragma unroll
-  for (int l = 0; l < 7; l++) g.gf[l][q] = y[l];
-  g.gf[8][q] = y[8];
-  g.tmu[q] = mu;
-  g.gcp[q] = cp;
-  g.glam[q] = la;
+    for (int l = 0; l < 7; l++) g.gf[l][q] = y[l];
+    g.gf[8][q] = y[8];
+  }
   g.gama[q] = ga;
   g.theta[q] = th;
   g.dnst[q] = fmax(temp, SMALL20);
