@@ -495,7 +495,7 @@ __global__ void k_copy_ug_fz(Box b, int iu1) {
 // arrays below (vis, dxn, dx0, dx1, ffl) are indexed by constants and stay
 // in registers.
 template <int IU1>
-__global__ void k_momentum_assemble(Box b) {
+__global__ __launch_bounds__(256) void k_momentum_assemble(Box b) {
   constexpr int iu1 = IU1;
   int i, j, k;
   if (!box_at(b, i, j, k)) return;
@@ -1244,7 +1244,7 @@ __device__ __forceinline__ void EXTR1F(double &f1, double &f2, int i, int j, int
 // inlets and exits (partials: GFIN0, q_exhaust), over the list of the
 // inlet and exit cells of the cell box [2, MP]x[2, NP]x[2, LP] (step 2);
 // every other cell of that box leaves the fields unchanged.
-__global__ void k_extr_inout(const long long *lst, int n, double *part) {
+__global__ __launch_bounds__(64) void k_extr_inout(const long long *lst, int n, double *part) {
   int i, j, k;
   double s0 = 0.0, s1 = 0.0;
   if (list_pt(lst, n, i, j, k)) {

@@ -364,7 +364,7 @@ __global__ void k_rad_mark(RadDev r) {
 // carries over from cell to cell (k_rad_carry): AKLG, DEB, FVS, OPL, the
 // flags; the AKL of melt-surface and marked wall cells.  DEB is formed for
 // every open cell above 300 K (QRNF's DEBX of each emitter is the same).
-__global__ void k_rad0_cell(RadDev r, MsDev m) {
+__global__ __launch_bounds__(256) void k_rad0_cell(RadDev r, MsDev m) {
   const long long cc = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (cc >= r.ncell) return;
   int i, j, k;
