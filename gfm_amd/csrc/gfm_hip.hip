@@ -203,13 +203,16 @@ static long long box_blocks_f(const Box &b) { return (((long long)b.ni * b.nj + 
 // autotuner may choose (gfm_hip_autotune): MAP0 is the default mapping
 // (0: box_pt rows, 1: box_ptf flattened) at 64 threads.  Every shape hands
 // each position to one thread, so results do not depend on the shape.
+// A kernel that reads its position with box_pt itself is launched with
+// map0 = ROWS_ONLY: the tuner then chooses only its workgroup size.
 static const Shape TUNE_CANDS[6] = {{0, 64}, {0, 128}, {0, 256}, {1, 64}, {1, 128}, {1, 256}};
+static const int ROWS_ONLY = 2;
 static dim3 box_grid_bs(const Box &b, int bs) {
   return b.map ? dim3((unsigned)(((long long)b.ni * b.nj + bs - 1) / bs), b.nk) : dim3((b.ni + bs - 1) / bs, b.nj, b.nk);
 }
 template <typename K>
 static Shape tune_shape(gfm_ctx *c, const char *name, K kern, int map0) {
-  Shape def = {map0, 64};
+  const Shape def = {map0 == 1 ? 1 : 0, 64};
   if (c->tune_cand >= 0) {
     c->tune_map0[name] = map0;
     auto it = c->tune_maxbs.find(name);
@@ -222,7 +225,7 @@ static Shape tune_shape(gfm_ctx *c, const char *name, K kern, int map0) {
       mx = it->second;
     }
     const Shape s = TUNE_CANDS[c->tune_cand];
-    return s.bs <= mx ? s : def;
+    return (s.bs <= mx && !(map0 == ROWS_ONLY && s.map)) ? s : def;
   }
   auto it = c->tuned.find(name);
   return it == c->tuned.end() ? def : it->second;
@@ -754,10 +757,10 @@ static int gflow(gfm_ctx *c, int stage) {
     Box fb = make_box(2 + nx0, MPh - nx0, 2 + nx1, NPh - nx1, 2 + nx2, LPh - nx2, 2);
     LAUNCHT(k_copy_ug_fz, 1, fb, iu1);
   }
-  LAUNCHT(k_momentum_assemble<1>, 0, mom[1]);
-  LAUNCHT(k_momentum_assemble<2>, 0, mom[2]);
-  LAUNCHT(k_momentum_assemble<3>, 0, mom[3]);
-  LAUNCHT(k_pressure_assemble, 0, cen);
+  LAUNCHT(k_momentum_assemble<1>, ROWS_ONLY, mom[1]);
+  LAUNCHT(k_momentum_assemble<2>, ROWS_ONLY, mom[2]);
+  LAUNCHT(k_momentum_assemble<3>, ROWS_ONLY, mom[3]);
+  LAUNCHT(k_pressure_assemble, ROWS_ONLY, cen);
   if (k.want_pre_residuals) residual(c, d.p, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_P_PRE, 0);
   adlbl3(c, d.p, 4, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2);
   residual(c, d.p, 4, MPh - 2, 4, NPh - 2, 4, LPh - 2, RS_P, 0);
@@ -849,13 +852,13 @@ static int gflow(gfm_ctx *c, int stage) {
     int igs = 1;
     if (e == 0 || c->sgm != sigma[e]) { c->sgm = sigma[e]; igs = 0; }
     switch (nelg) {
-      case 6: LAUNCHT(k_scalar_assemble<6>, 0, cen, l, igs, c->sgm, c->part); break;
-      case 7: LAUNCHT(k_scalar_assemble<7>, 0, cen, l, igs, c->sgm, c->part); break;
+      case 6: LAUNCHT(k_scalar_assemble<6>, ROWS_ONLY, cen, l, igs, c->sgm, c->part); break;
+      case 7: LAUNCHT(k_scalar_assemble<7>, ROWS_ONLY, cen, l, igs, c->sgm, c->part); break;
       case 8: LAUNCH(k_scalar_assemble<8>, cen, l, igs, c->sgm, c->part); break;
-      case 9: LAUNCHT(k_scalar_assemble<9>, 0, cen, l, igs, c->sgm, c->part); break;
-      case 10: LAUNCHT(k_scalar_assemble<10>, 0, cen, l, igs, c->sgm, c->part); break;
-      case 11: LAUNCHT(k_scalar_assemble<11>, 0, cen, l, igs, c->sgm, c->part); break;
-      default: LAUNCHT(k_scalar_assemble<12>, 0, cen, l, igs, c->sgm, c->part); break;
+      case 9: LAUNCHT(k_scalar_assemble<9>, ROWS_ONLY, cen, l, igs, c->sgm, c->part); break;
+      case 10: LAUNCHT(k_scalar_assemble<10>, ROWS_ONLY, cen, l, igs, c->sgm, c->part); break;
+      case 11: LAUNCHT(k_scalar_assemble<11>, ROWS_ONLY, cen, l, igs, c->sgm, c->part); break;
+      default: LAUNCHT(k_scalar_assemble<12>, ROWS_ONLY, cen, l, igs, c->sgm, c->part); break;
     }
     if (nelg == 8) {
       hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(FIN_NT), 0, c->stream, c->part, (int)box_blocks(cen), 1, SC_QREACT, 0);
@@ -1357,10 +1360,12 @@ int gfm_hip_autotune(gfm_ctx *c, gfm_ctrl *k, int iters, char *report, int repor
   std::string rep = "kernel                          default(us)  best(us)  map  workgroup\n";
   for (auto &kv : us) {
     const int mx = c->tune_maxbs[kv.first], map0 = c->tune_map0[kv.first];
-    const int def = map0 ? 3 : 0;
+    const int def = map0 == 1 ? 3 : 0;
     int best = def;
     for (int q = 0; q < NC; q++)
-      if (TUNE_CANDS[q].bs <= mx && kv.second[q] > 0.0 && kv.second[q] < kv.second[best]) best = q;
+      if (TUNE_CANDS[q].bs <= mx && !(map0 == ROWS_ONLY && TUNE_CANDS[q].map) && kv.second[q] > 0.0 &&
+          kv.second[q] < kv.second[best])
+        best = q;
     c->tuned[kv.first] = TUNE_CANDS[best];
     char line[160];
     snprintf(line, sizeof line, "%-30s %12.2f %9.2f %4d %10d\n", kv.first.c_str(), kv.second[def], kv.second[best],

@@ -493,12 +493,17 @@ __global__ void k_copy_ug_fz(Box b, int iu1) {
 // SOURCE(IU1) -> SRC1 fused with GSLVX1 for one face (C/Gslv3x.f90:26-111).
 // The velocity component is a template parameter, so the per-direction
 // arrays below (vis, dxn, dx0, dx1, ffl) are indexed by constants and stay
-// in registers.
+// in registers.  It takes its position with box_pt only (one x row per row
+// of workgroups, launched with the row mapping): j and k stay uniform over
+// the wavefront, so the stencil's index and grid terms sit in scalar
+// registers and the kernel keeps 4 waves per SIMD (3 for w) without
+// scratch; under the mapping chosen at run time they took 136-186 vector
+// registers (2-3 waves per SIMD).
 template <int IU1>
 __global__ __launch_bounds__(256) void k_momentum_assemble(Box b) {
   constexpr int iu1 = IU1;
   int i, j, k;
-  if (!box_at(b, i, j, k)) return;
+  if (!box_pt(b, i, j, k)) return;
   long long q = lidx(i, j, k);
   if (IBC(i, j, k) >= 1) { SBS(q, 0.0); return; }
   // ---- SRC1 ----
@@ -615,7 +620,7 @@ __global__ __launch_bounds__(256) void k_momentum_assemble(Box b) {
 // C/Source.f90:35-74
 __global__ void k_pressure_assemble(Box b) {
   int i, j, k;
-  if (!box_at(b, i, j, k)) return;
+  if (!box_pt(b, i, j, k)) return;
   long long q = lidx(i, j, k);
   if (IBC(i, j, k) >= 1) { SBS(q, g.p[q]); return; }
   double ax = AREA_C(i, j, k, 1), ay = AREA_C(i, j, k, 2), az = AREA_C(i, j, k, 3);
@@ -1013,7 +1018,7 @@ __global__ void k_scalar_assemble(Box b, int l, int igs, double sgm, double *par
   const int itr_gas = (int)g.scal[SC_ITR];
   int i, j, k;
   double qr = 0.0, qw = 0.0, qlo = 0.0, qhi = 0.0;
-  if (box_at(b, i, j, k)) {
+  if (box_pt(b, i, j, k)) {
     long long q = lidx(i, j, k);
     const double fz = g.fz[q];
     if (IBC(i, j, k) >= 1) {
