@@ -201,8 +201,9 @@ int gfm_hip_sphase(gfm_ctx *ctx, gfm_ctrl *ctrl, int max_iters, double *res_hist
  * 64, 128 or 256 threads) of the per-position box kernels (assembly,
  * interpolation, properties, copies): `iters` SIMPLER iterations per
  * candidate shape, each kernel timed with events, then each keeps its
- * fastest shape for the rest of the context's life.  No counterpart in the
- * reference.  The iterations are ordinary iterations of the run (itr_gas
+ * fastest shape for the rest of the context's life (the momentum, pressure
+ * and scalar assemblies always take the row mapping and are tuned over the
+ * workgroup size only).  No counterpart in the reference.  The iterations are ordinary iterations of the run (itr_gas
  * advances by 6*iters; ctrl carries the last one's outputs): every shape
  * computes the same values bit for bit.  A table of the choices (kernel,
  * default and best microseconds per launch, mapping, workgroup size) is
