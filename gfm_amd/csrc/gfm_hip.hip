@@ -1584,6 +1584,7 @@ int gfm_hip_finalize(gfm_ctx *c) {
   if (g_bound == c) g_bound = nullptr;
   delete c->ms;
   delete c->rad;
+  delete c->wall;
   delete c;
   return GFM_OK;
 }
