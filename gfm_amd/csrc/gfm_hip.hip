@@ -724,6 +724,64 @@ static void extr(gfm_ctx *c);
 // ---------------------------------------------------------------------
 // stage 0: the whole iteration; 1: up to the initial guess of a p' solve
 // relaxed to a tolerance; 2: from the end of that solve on.
+// INTPV (C/Intp.f90:9-30 calls INTP1 over the velocity classes) in three
+// launches.  Per component: (1) the two edge classes interpolated from
+// the face values, (2) the corner class from the first edge class, (3)
+// the four classes between faces along the component's own axis from all
+// of these; the three components' arrays are separate, so each stage
+// takes all three.
+static void intpv_stage(gfm_ctx *c, const Box *bx, const int *iu, const int *ix, int n) {
+  IntpvSet s;
+  memset(&s, 0, sizeof s);
+  int wg = 0;
+  for (int q = 0; q < n; q++) {
+    const Box &b = bx[q];
+    if (b.ni <= 0 || b.nj <= 0 || b.nk <= 0) continue;
+    s.b[s.n] = b; s.iu[s.n] = iu[q]; s.ix[s.n] = ix[q]; s.wg0[s.n] = wg;
+    wg += (int)(((long long)b.ni * b.nj * b.nk + INTPV_NT - 1) / INTPV_NT);
+    s.n++;
+  }
+  s.wg0[s.n] = wg;
+  if (wg == 0) return;
+  if (c->prof.on) prof_begin(c);
+  hipLaunchKernelGGL(k_intpv, dim3((unsigned)wg), dim3(INTPV_NT), 0, c->stream, s);
+  if (c->prof.on) prof_end(c, "k_intpv", 0.0);
+}
+static void intpv(gfm_ctx *c) {
+  const gfm_ctrl &k = c->ctrl;
+  const int MPh = k.mp, NPh = k.np, LPh = k.lp;
+  {  // stage 1: edges from faces
+    const Box bx[6] = {make_box(3, MPh - 1, 3, NPh - 1, 2, LPh, 2), make_box(3, MPh - 1, 2, NPh, 3, LPh - 1, 2),
+                       make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2), make_box(3, MPh - 1, 3, NPh - 1, 2, LPh, 2),
+                       make_box(3, MPh - 1, 2, NPh, 3, LPh - 1, 2), make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2)};
+    const int iu[6] = {1, 1, 2, 2, 3, 3}, ix[6] = {2, 3, 3, 1, 1, 2};
+    intpv_stage(c, bx, iu, ix, 6);
+  }
+  {  // stage 2: corners
+    const Box cb = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2);
+    const Box bx[3] = {cb, cb, cb};
+    const int iu[3] = {1, 2, 3}, ix[3] = {3, 1, 2};
+    intpv_stage(c, bx, iu, ix, 3);
+  }
+  {  // stage 3: along the component's own axis
+    Box bx[12];
+    int iu[12], ix[12], n = 0;
+    for (int pj = 0; pj < 2; pj++)
+      for (int pk = 0; pk < 2; pk++) {
+        bx[n] = make_box(4, MPh, 2 + pj, pj ? NPh - 1 : NPh, 2 + pk, pk ? LPh - 1 : LPh, 2); iu[n] = 1; ix[n] = 1; n++;
+      }
+    for (int pi = 0; pi < 2; pi++)
+      for (int pk = 0; pk < 2; pk++) {
+        bx[n] = make_box(2 + pi, pi ? MPh - 1 : MPh, 4, NPh, 2 + pk, pk ? LPh - 1 : LPh, 2); iu[n] = 2; ix[n] = 2; n++;
+      }
+    for (int pi = 0; pi < 2; pi++)
+      for (int pj = 0; pj < 2; pj++) {
+        bx[n] = make_box(2 + pi, pi ? MPh - 1 : MPh, 2 + pj, pj ? NPh - 1 : NPh, 4, LPh, 2); iu[n] = 3; ix[n] = 3; n++;
+      }
+    intpv_stage(c, bx, iu, ix, n);
+  }
+}
+
 static int gflow(gfm_ctx *c, int stage) {
   const gfm_ctrl &k = c->ctrl;
   DevCase &d = c->dc;
@@ -808,37 +866,7 @@ static int gflow(gfm_ctx *c, int stage) {
   if (k.want_plane_flows) plane_flows(c);
   hipLaunchKernelGGL(k_factor, dim3(1), dim3(1), 0, c->stream);
   for (int q = 0; q < 3; q++) LAUNCHF(k_extrv, ex[q], q + 1, 1, c->part);
-  // INTPV
-  {
-    // IU=1
-    Box b;
-    b = make_box(3, MPh - 1, 3, NPh - 1, 2, LPh, 2); LAUNCHT(k_intpv, 1, b, 1, 2, 1, 1, 0);
-    b = make_box(3, MPh - 1, 2, NPh, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 1, 3, 1, 0, 1);
-    b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 1, 3, 1, 1, 1);
-    for (int pj = 0; pj < 2; pj++)
-      for (int pk = 0; pk < 2; pk++) {
-        b = make_box(4, MPh, 2 + pj, pj ? NPh - 1 : NPh, 2 + pk, pk ? LPh - 1 : LPh, 2);
-        LAUNCHT(k_intpv, 1, b, 1, 1, 0, pj, pk);
-      }
-    // IU=2
-    b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 2, 3, 0, 1, 1);
-    b = make_box(3, MPh - 1, 3, NPh - 1, 2, LPh, 2); LAUNCHT(k_intpv, 1, b, 2, 1, 1, 1, 0);
-    b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 2, 1, 1, 1, 1);
-    for (int pi = 0; pi < 2; pi++)
-      for (int pk = 0; pk < 2; pk++) {
-        b = make_box(2 + pi, pi ? MPh - 1 : MPh, 4, NPh, 2 + pk, pk ? LPh - 1 : LPh, 2);
-        LAUNCHT(k_intpv, 1, b, 2, 2, pi, 0, pk);
-      }
-    // IU=3
-    b = make_box(3, MPh - 1, 2, NPh, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 3, 1, 1, 0, 1);
-    b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 3, 2, 0, 1, 1);
-    b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 3, 2, 1, 1, 1);
-    for (int pi = 0; pi < 2; pi++)
-      for (int pj = 0; pj < 2; pj++) {
-        b = make_box(2 + pi, pi ? MPh - 1 : MPh, 2 + pj, pj ? NPh - 1 : NPh, 4, LPh, 2);
-        LAUNCHT(k_intpv, 1, b, 3, 3, pi, pj, 0);
-      }
-  }
+  intpv(c);
   // pressure reference
   hipLaunchKernelGGL(k_pref, dim3(1), dim3(1), 0, c->stream);
 
@@ -909,23 +937,8 @@ static void extr(gfm_ctx *c) {
 static int initsv_tail(gfm_ctx *c) {
   const gfm_ctrl &k = c->ctrl;
   const int MPh = k.mp, NPh = k.np, LPh = k.lp;
-  // INTPV (same launches as in gflow)
+  intpv(c);
   Box b;
-  b = make_box(3, MPh - 1, 3, NPh - 1, 2, LPh, 2); LAUNCHT(k_intpv, 1, b, 1, 2, 1, 1, 0);
-  b = make_box(3, MPh - 1, 2, NPh, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 1, 3, 1, 0, 1);
-  b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 1, 3, 1, 1, 1);
-  for (int pj = 0; pj < 2; pj++)
-    for (int pk = 0; pk < 2; pk++) { b = make_box(4, MPh, 2 + pj, pj ? NPh - 1 : NPh, 2 + pk, pk ? LPh - 1 : LPh, 2); LAUNCHT(k_intpv, 1, b, 1, 1, 0, pj, pk); }
-  b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 2, 3, 0, 1, 1);
-  b = make_box(3, MPh - 1, 3, NPh - 1, 2, LPh, 2); LAUNCHT(k_intpv, 1, b, 2, 1, 1, 1, 0);
-  b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 2, 1, 1, 1, 1);
-  for (int pi = 0; pi < 2; pi++)
-    for (int pk = 0; pk < 2; pk++) { b = make_box(2 + pi, pi ? MPh - 1 : MPh, 4, NPh, 2 + pk, pk ? LPh - 1 : LPh, 2); LAUNCHT(k_intpv, 1, b, 2, 2, pi, 0, pk); }
-  b = make_box(3, MPh - 1, 2, NPh, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 3, 1, 1, 0, 1);
-  b = make_box(2, MPh, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 3, 2, 0, 1, 1);
-  b = make_box(3, MPh - 1, 3, NPh - 1, 3, LPh - 1, 2); LAUNCHT(k_intpv, 1, b, 3, 2, 1, 1, 1);
-  for (int pi = 0; pi < 2; pi++)
-    for (int pj = 0; pj < 2; pj++) { b = make_box(2 + pi, pi ? MPh - 1 : MPh, 2 + pj, pj ? NPh - 1 : NPh, 4, LPh, 2); LAUNCHT(k_intpv, 1, b, 3, 3, pi, pj, 0); }
   extr(c);
   b = make_box(2, MPh, 2, NPh, 3, LPh - 1, 2); LAUNCHT(k_intp<3>, 0, b);
   b = make_box(2, MPh, 3, NPh - 1, 2, LPh, 2); LAUNCHT(k_intp<2>, 0, b);

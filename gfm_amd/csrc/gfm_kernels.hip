@@ -887,11 +887,8 @@ __device__ __forceinline__ double INTP1(int i, int j, int k, int ix) {
   if (ix == 2) return (DR(j) <= 0.0 || j > NP) ? 0.0 : (R(j) - R(j - 1)) / DR(j);
   return (DX(i) <= 0.0 || i > MP) ? 0.0 : (X(i) - X(i - 1)) / DX(i);
 }
-__global__ void k_intpv(Box b, int iu, int ix, int pi, int pj, int pk) {
-  int i, j, k;
-  if (!box_at(b, i, j, k)) return;
-  // box covers a class; parities fixed per launch
-  (void)pi; (void)pj; (void)pk;
+// INTPV at one position: velocity component iu interpolated along ix.
+__device__ __forceinline__ void intpv_at(int i, int j, int k, int iu, int ix) {
   int bc = IBC(i, j, k);
   if (bc >= 1 && bc != 4) return;
   double f1 = INTP1(i, j, k, ix), f2 = 1.0 - f1;
@@ -908,6 +905,29 @@ __global__ void k_intpv(Box b, int iu, int ix, int pi, int pj, int pk) {
     v = UG(i, j, k + 1, iu) * f1 + UG(i, j, k - 1, iu) * f2;
   }
   UG(i, j, k, iu) = BOUND_U(v);
+}
+// One stage of INTPV in one launch: up to INTPV_MAXE boxes, each with its
+// component and direction; workgroups wg0[e] .. wg0[e+1]-1 take box e, its
+// positions flattened (i fastest).  The boxes of a stage write disjoint
+// positions and read none that the stage writes.
+#define INTPV_MAXE 12
+#define INTPV_NT 256
+struct IntpvSet {
+  int n;
+  int iu[INTPV_MAXE], ix[INTPV_MAXE];
+  int wg0[INTPV_MAXE + 1];
+  Box b[INTPV_MAXE];
+};
+__global__ __launch_bounds__(INTPV_NT) void k_intpv(const IntpvSet s) {
+  const int bid = blockIdx.x;
+  int e = 0;
+  while (e + 1 < s.n && bid >= s.wg0[e + 1]) e++;
+  const Box &b = s.b[e];
+  const long long t = (long long)(bid - s.wg0[e]) * INTPV_NT + threadIdx.x;
+  if (t >= (long long)b.ni * b.nj * b.nk) return;
+  const long long r = t / b.ni;
+  const int ii = (int)(t - r * b.ni), jj = (int)(r % b.nj), kk = (int)(r / b.nj);
+  intpv_at(b.i0 + b.step * ii, b.j0 + b.step * jj, b.k0 + b.step * kk, s.iu[e], s.ix[e]);
 }
 
 // pressure referencing at the reference inlet cell, C/Gflow.f90:182-242
