@@ -301,6 +301,10 @@ def main():
                                              args.ttr_cap, args.pcorr_tol_tight)
                 ttr[name]["block_correction"] = mode.endswith("_bc")
         line["time_to_residual"] = ttr
+        # a configuration whose run from the initial state never passes 1e-3 has stalled: its throughput
+        # counts iterations that do not converge
+        for name, _, _ in others:
+            line[name]["reaches_1e-3"] = ttr[name]["first_below"]["0.001"] is not None
     if net is not None:
         net.close()
     kt_sum = {k: {"ms": round(v[0], 3), "launches": v[2]} for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0])}
