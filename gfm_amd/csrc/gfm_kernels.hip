@@ -502,6 +502,13 @@ __global__ void k_copy_ug_fz(Box b, int iu1) {
 template <int IU1>
 __global__ __launch_bounds__(256) void k_momentum_assemble(Box b) {
   constexpr int iu1 = IU1;
+  // HOIST: the neighbour values a branch on a cell code chooses between
+  // are all loaded before the code is known, so no load waits on it; the
+  // same values and operations, more registers (y, z faces: 125 and 157
+  // VGPRs, 4 and 3 waves per SIMD; the z-face kernel 198 -> 176 us per
+  // launch on 192x96x64).  The x-face instance keeps the branches: there
+  // the extra registers cost it a wave per SIMD (181 -> 193 us).
+  constexpr bool HOIST = IU1 != 1;
   int i, j, k;
   if (!box_pt(b, i, j, k)) return;
   long long q = lidx(i, j, k);
@@ -516,14 +523,18 @@ __global__ __launch_bounds__(256) void k_momentum_assemble(Box b) {
     if (iu1 == 1) { dx0[0] = DX(i); dx1[0] = DX(i1); }
     else if (iu1 == 2) { dx0[1] = DR(j) * RS(j); dx1[1] = DR(j1) * RS(j1); }
     else { dx0[2] = DZ(k); dx1[2] = DZ(k1); }
+    const bool in2 = IBC(i2, j2, k2) == 2;
     double thgm;
-    if (IBC(i2, j2, k2) == 2) {
-      thgm = TH(i2, j2, k2) * GAMA(i2, j2, k2);
+    if (HOIST) {
+      const double tg2 = TH(i2, j2, k2) * GAMA(i2, j2, k2), tg1 = TH(i1, j1, k1) * GAMA(i1, j1, k1);
+      thgm = in2 ? tg2 : tg1;
+    } else {
+      thgm = in2 ? TH(i2, j2, k2) * GAMA(i2, j2, k2) : TH(i1, j1, k1) * GAMA(i1, j1, k1);
+    }
+    if (in2) {
       if (iu1 == 1) dxn[0] += fabs(X(i2) - X(i1));
       if (iu1 == 2) dxn[1] += fabs(R(j2) - R(j1)) * RS(j);
       if (iu1 == 3) dxn[2] += fabs(Z(k2) - Z(k1));
-    } else {
-      thgm = TH(i1, j1, k1) * GAMA(i1, j1, k1);
     }
     if (m == -1) vis[iu1 - 1][0] = thgm * (fzc - FZ(i2, j2, k2)) / dx1[iu1 - 1];
     else vis[iu1 - 1][1] = thgm * (FZ(i2, j2, k2) - fzc) / dx1[iu1 - 1];
@@ -553,11 +564,14 @@ __global__ __launch_bounds__(256) void k_momentum_assemble(Box b) {
   for (int m = -1; m <= 1; m += 2) {  // GSLV3X1
     int m1 = (m + 3) / 2, i1, j1, k1, i2, j2, k2;
     VINDX(iu1, m, i, j, k, i1, j1, k1, i2, j2, k2);
-    int ibc1 = IBC(i2, j2, k2);
+    const int ibc1 = IBC(i2, j2, k2);
     double a;
-    if (ibc1 == 3) a = 0.0;
-    else if (ibc1 == 2) a = FLW(i2, j2, k2, iu1, m1);
-    else a = FLWD(i1, j1, k1, iu1, m1, 1.0);
+    if (HOIST) {
+      const double aw = FLW(i2, j2, k2, iu1, m1), ad = FLWD(i1, j1, k1, iu1, m1, 1.0);
+      a = ibc1 == 3 ? 0.0 : ibc1 == 2 ? aw : ad;
+    } else {
+      a = ibc1 == 3 ? 0.0 : ibc1 == 2 ? FLW(i2, j2, k2, iu1, m1) : FLWD(i1, j1, k1, iu1, m1, 1.0);
+    }
     SAS((iu1 - 1) + 3 * (m1 - 1), q, a);
     ap += a;
     g0 += a * FZ(i2, j2, k2);
@@ -577,13 +591,23 @@ __global__ __launch_bounds__(256) void k_momentum_assemble(Box b) {
       for (int n = -1; n <= 1; n += 2) {
         int n1 = (n + 3) / 2;
         ii1 = i1 + nx0 * n; jj1 = j1 + nx1 * n; kk1 = k1 + nx2 * n;
-        int ibc1 = IBC(ii1, jj1, kk1);
-        if (ibc1 == 3) continue;
-        double area = ffl[iu - 1][n1 - 1];
-        f += TH(ii1, jj1, kk1) * DN(ii1, jj1, kk1) * FZ(ii1, jj1, kk1) * area;
-        if (ibc1 == 2) continue;
-        double ddl = FDDL(ii1, jj1, kk1, iu);
-        diff += TH(ii1, jj1, kk1) * GAMA(ii1, jj1, kk1) / 1.0 * area / ddl;
+        const int ibc1 = IBC(ii1, jj1, kk1);
+        const double area = ffl[iu - 1][n1 - 1];
+        if (HOIST) {
+          const long long qq = lidx(ii1, jj1, kk1);
+          const double th = g.theta[qq], dn = g.dnst[qq], fzv = g.fz[qq], ga = g.gama[qq];
+          const double fnew = f + th * dn * fzv * area;
+          f = ibc1 == 3 ? f : fnew;
+          if (ibc1 >= 2 && ibc1 <= 3) continue;
+          double ddl = FDDL(ii1, jj1, kk1, iu);
+          diff += th * ga / 1.0 * area / ddl;
+        } else {
+          if (ibc1 == 3) continue;
+          f += TH(ii1, jj1, kk1) * DN(ii1, jj1, kk1) * FZ(ii1, jj1, kk1) * area;
+          if (ibc1 == 2) continue;
+          double ddl = FDDL(ii1, jj1, kk1, iu);
+          diff += TH(ii1, jj1, kk1) * GAMA(ii1, jj1, kk1) / 1.0 * area / ddl;
+        }
       }
       double fld = (m1 == 1) ? DA(diff, f) + fmax(0.0, f) : DA(diff, f) + fmax(0.0, -f);
       if (IBC(ii1, jj1, kk1) == 2 && IBC(i2, j2, k2) == 1) {
