@@ -663,11 +663,12 @@ __global__ void k_pressure_assemble(Box b) {
       int o = (m == 1) ? -1 : 1;
       if (d == 1) i1 += o; else if (d == 2) j1 += o; else k1 += o;
       double a = 0.0;
-      double apf = AP(i1, j1, k1);
+      // theta and rho loaded with a_p, before the cell code and a_p decide (no load waits on them)
+      const long long q1 = lidx(i1, j1, k1);
+      const double apf = AP(i1, j1, k1), th = g.theta[q1], dn = g.dnst[q1];
       if (IBC(i1, j1, k1) <= 0 && fabs(apf) > SMALL20) {
-        double th = TH(i1, j1, k1);
         double av = (d == 1) ? DXAV(i1, j1, k1) / DX(i1) : (d == 2) ? DRAV(i1, j1, k1) / DR(j1) : DZAV(i1, j1, k1) / DZ(k1);
-        a = th * th * DN(i1, j1, k1) * area * area / apf * av;
+        a = th * th * dn * area * area / apf * av;
       }
       SAS((d - 1) + 3 * (m - 1), q, a);
       ap += a;
